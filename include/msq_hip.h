@@ -1,0 +1,260 @@
+/*
+ * msq_hip.h -- C ABI of libmsq_hip.so, the MI355X (gfx950) backend of the
+ * mixed-precision SPD square root X = A^(1/2) = V diag(sqrt(lambda)) V^T with
+ * frozen-factor Sylvester correction.
+ *
+ * The reference exposes this path as a C++ class, MetalRuntime
+ * (src/metal_runtime.hpp:69-176), driven by solveGpuResidentSpd and
+ * runMetalCase (src/metal_solver.cpp:92-604).  A HIP backend is selected like
+ * the Metal and Accelerate ones (SolverBackend, src/precision.hpp:130).  This
+ * header is what a binding of that backend calls: plain pointers and sizes,
+ * int status codes, no C++ or torch types.  The C++ mirror of MetalRuntime
+ * (mixed-mlx_amd/csrc/hip_runtime.hpp) is written on top of it.
+ *
+ * Conventions
+ *  - Matrices are column-major.  Batched calls use the batch layout of
+ *    msq_batch_layout: matrix i occupies an ld_i x ld_i tile at element offset
+ *    offsets[i], ld_i = order_i rounded up to a multiple of 32, zero outside
+ *    the order; per-matrix vectors (roots) occupy ld_i entries at
+ *    vector_offsets[i].
+ *  - Device pointers are plain addresses in the runtime's device memory; all
+ *    work is enqueued on the runtime's stream and every call returns after
+ *    its work completed unless its comment says otherwise.
+ *  - Status: 0 on success; msq_last_error() describes the last failure of the
+ *    calling thread.
+ *  - Calls on one runtime are not reentrant (the reference's MetalRuntime is
+ *    single-threaded too, src/metal_runtime.cpp:1050-1058).
+ */
+#ifndef MSQ_HIP_H
+#define MSQ_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSQ_ABI_VERSION 1
+
+enum msq_status {
+  MSQ_OK = 0,
+  MSQ_ERROR_INVALID_ARGUMENT = 1,
+  MSQ_ERROR_HIP = 2,
+  MSQ_ERROR_OUT_OF_MEMORY = 3,
+  MSQ_ERROR_UNSUPPORTED = 4,
+};
+
+/* Precision, in the order and meaning of the reference enum
+ * (src/precision.hpp:9-20, spec table :64-90). */
+enum msq_precision {
+  MSQ_FP32 = 0,
+  MSQ_BF16 = 1,
+  MSQ_FP8_E4M3FN = 2,
+  MSQ_FP8_E5M2 = 3,
+  MSQ_MXFP8_E4M3FN = 4,
+  MSQ_MXFP8_E5M2 = 5,
+  MSQ_FP4_E2M1 = 6,
+  MSQ_MXFP4_E2M1 = 7,
+  MSQ_FP8_E4M3FN_LOG_S = 8,
+  MSQ_FP4_E2M1_LOG_S = 9,
+  /* No reference counterpart: FP8 E4M3FN with one per-tensor amax scale
+   * sigma = 448 / amax (the reference's FP8 is unscaled). */
+  MSQ_FP8_E4M3FN_SCALED = 16,
+};
+
+typedef struct msq_runtime msq_runtime;
+
+/* Solver entry state of one matrix (solve status). */
+enum msq_solve_status {
+  MSQ_SOLVE_RUNNING = 0,
+  MSQ_SOLVE_CONVERGED = 1,
+  MSQ_SOLVE_BUDGET_EXHAUSTED = 2,
+  MSQ_SOLVE_DIVERGED = 3,
+  MSQ_SOLVE_NONFINITE = 4,
+  MSQ_SOLVE_INVALID_SPECTRUM = 5,
+};
+
+#define MSQ_JACOBI_HISTORY 32
+#define MSQ_SOLVE_HISTORY 20
+
+/* Jacobi options of one matrix.  The reference runs a fixed 8 sweeps
+ * (src/metal_solver.cpp:210-213); fixed_sweeps = 8 reproduces that.  With
+ * fixed_sweeps = 0 the sweeps stop at the first sweep whose FP64 off-diagonal
+ * norm off(A) satisfies off <= off_tol ||A||_F, or off <= off_tol_lambda *
+ * min diag(A), or (stagnation) off > stagnation * previous off while
+ * off <= floor_tol ||A||_F, or after max_sweeps. */
+typedef struct msq_jacobi_options {
+  uint32_t fixed_sweeps;
+  uint32_t max_sweeps;
+  float off_tol;
+  float off_tol_lambda;
+  float stagnation;
+  float floor_tol;
+} msq_jacobi_options;
+
+typedef struct msq_jacobi_report {
+  uint32_t sweeps;
+  uint32_t invalid_spectrum; /* a diagonal entry <= 0 or non-finite */
+  double off_norm;           /* off(A) after the last sweep */
+  double matrix_norm;        /* ||A||_F (binary64) */
+  float min_diagonal;
+  float off_history[MSQ_JACOBI_HISTORY]; /* off(A) / ||A||_F per sweep */
+} msq_jacobi_report;
+
+/* Options of one matrix of msq_solve_batched.  The precisions select the
+ * products: reconstruction X0 = G G^T with G = V diag(sqrt(s)) rounded to
+ * basis_precision, and every correction product W = V^T R, Y = (W V) o H,
+ * W2 = V Y, E = W2 V^T in correction_precision; MSQ_FP32 for both is the
+ * reference's FP32 path.  Supported: MSQ_FP32, MSQ_BF16,
+ * MSQ_FP8_E4M3FN_SCALED, MSQ_MXFP4_E2M1.  Corrections stop when the relative
+ * FP64 residual ||A - X^2||_F / ||A||_F <= residual_tolerance or the forward
+ * error estimate ||R||_F / (2 s_min ||X||_F) <= forward_tolerance, after at
+ * least min_corrections and at most max_corrections (< MSQ_SOLVE_HISTORY). */
+typedef struct msq_solve_options {
+  int32_t basis_precision;
+  int32_t correction_precision;
+  msq_jacobi_options jacobi;
+  uint32_t min_corrections;
+  uint32_t max_corrections;
+  double residual_tolerance;
+  double forward_tolerance;
+} msq_solve_options;
+
+typedef struct msq_solve_report {
+  uint32_t order;
+  uint32_t sweeps;
+  uint32_t corrections;
+  uint32_t status; /* enum msq_solve_status */
+  double matrix_norm;
+  double min_root;
+  double off_norm;
+  double final_residual;
+  double final_forward_estimate;
+  double residuals[MSQ_SOLVE_HISTORY];         /* X0, then each correction */
+  double forward_estimates[MSQ_SOLVE_HISTORY];
+  float off_history[MSQ_JACOBI_HISTORY];
+} msq_solve_report;
+
+/* ---------------------------------------------------------------- runtime */
+
+/* Thread-local description of the last failure. */
+const char *msq_last_error(void);
+int msq_abi_version(void);
+
+/* MetalRuntime::MetalRuntime (src/metal_runtime.cpp:41-126): one device, one
+ * stream (created here, default priority). */
+int msq_runtime_create(int device, msq_runtime **out);
+/* ~MetalRuntime. */
+int msq_runtime_destroy(msq_runtime *runtime);
+/* Enqueue on a caller-owned stream instead (NULL restores the runtime's). */
+int msq_runtime_set_stream(msq_runtime *runtime, void *hip_stream);
+int msq_synchronize(msq_runtime *runtime);
+
+/* makeSharedBuffer / releaseBuffers (src/metal_runtime.cpp:1426-1432).  No
+ * unified memory here: device allocations plus explicit copies. */
+int msq_alloc(msq_runtime *runtime, size_t bytes, void **device_ptr);
+int msq_free(msq_runtime *runtime, void *device_ptr);
+int msq_copy_to_device(msq_runtime *runtime, void *dst, const void *src, size_t bytes);
+int msq_copy_to_host(msq_runtime *runtime, void *dst, const void *src, size_t bytes);
+int msq_memset(msq_runtime *runtime, void *dst, int value, size_t bytes);
+
+/* storageBytes / scaleBytes (src/configuration.cpp:188-213). */
+size_t msq_storage_bytes(int precision, size_t elements);
+size_t msq_scale_bytes(int precision, size_t elements);
+
+/* Batch layout: ld_i = roundup(order_i, 32); offsets[i] = sum_{j<i} ld_j^2;
+ * vector_offsets[i] = sum_{j<i} ld_j.  Totals in elements. */
+int msq_batch_layout(uint32_t count, const uint32_t *orders, uint32_t *lds,
+                     uint64_t *offsets, uint64_t *vector_offsets,
+                     uint64_t *total_elements, uint64_t *total_vector_elements);
+
+/* ----------------------------------------------------------------- codecs */
+
+/* The quantize half of quantizeAndDecode (src/metal_runtime.cpp:672-740),
+ * reference linear layout (MX blocks of 32 consecutive elements).  Supports
+ * MSQ_BF16, MSQ_FP8_E4M3FN, MSQ_FP8_E5M2, MSQ_MXFP8_E4M3FN, MSQ_MXFP8_E5M2,
+ * MSQ_FP4_E2M1, MSQ_MXFP4_E2M1; bit-exact with the reference host codecs
+ * (src/storage_codecs.cpp:16-198).  scales may be NULL for scalar formats. */
+int msq_quantize(msq_runtime *runtime, int precision, const float *src, void *values,
+                 void *scales, size_t elements);
+/* The decode half (decode_*_to_f32, metal_mixed_sqrt.metal:1661-1685). */
+int msq_decode(msq_runtime *runtime, int precision, const void *values,
+               const void *scales, float *dst, size_t elements);
+/* quantizeAndDecode as one call. */
+int msq_quantize_and_decode(msq_runtime *runtime, int precision, const float *src,
+                            void *values, void *scales, float *decoded, size_t elements);
+/* MSQ_FP8_E4M3FN_SCALED: sigma (one device float) = 448 / amax, codes =
+ * E4M3FN(src * sigma); decode = E4M3FN^-1(code) / sigma. */
+int msq_quantize_fp8_scaled(msq_runtime *runtime, const float *src, void *values,
+                            float *sigma, size_t elements);
+int msq_decode_fp8_scaled(msq_runtime *runtime, const void *values, const float *sigma,
+                          float *dst, size_t elements);
+
+/* ----------------------------------------------------------------- Jacobi */
+
+/* symmetricJacobi (src/metal_runtime.cpp:584-646) for a batch: the FP64
+ * matrices (batch layout) are rounded to float and diagonalised; vectors
+ * (float, batch layout) receive V, diagonal and roots (float, vector layout)
+ * receive diag(V^T A V) and its square roots.  options has 1 or count
+ * entries; reports (host, count entries) may be NULL.  Orders up to 256 run
+ * one workgroup per matrix in one launch; larger orders run the round-by-round
+ * device-memory solver, one matrix at a time. */
+int msq_jacobi_batched(msq_runtime *runtime, uint32_t count, const uint32_t *orders,
+                       const double *matrices, float *vectors, float *diagonal,
+                       float *roots, const msq_jacobi_options *options,
+                       uint32_t options_count, msq_jacobi_report *reports);
+
+/* --------------------------------------------------------------- products */
+
+/* reconstructTiled / reconstructLowFactorTiled (src/metal_runtime.cpp:429-534)
+ * for a batch: X0 = G G^T, G = V diag(sqrt(roots)) rounded to precision,
+ * FP32 accumulate; root (float, batch layout). */
+int msq_reconstruct(msq_runtime *runtime, int precision, uint32_t count,
+                    const uint32_t *orders, const float *vectors, const float *roots,
+                    float *root);
+
+/* computeWorkingResidual (src/metal_runtime.cpp:789-834) in binary64:
+ * R = A - X X for symmetric X; residual (double) and residual_f32 (float)
+ * in batch layout; relative_norms (host, count) = ||R||_F / ||A||_F. */
+int msq_residual_fp64(msq_runtime *runtime, uint32_t count, const uint32_t *orders,
+                      const double *matrices, const double *root, double *residual,
+                      float *residual_f32, double *relative_norms);
+
+/* refineTiled / refineLowFactorTiled (src/metal_runtime.cpp:461-582): one
+ * frozen-factor correction E = V [ (V^T R V) o H ] V^T with H_ij =
+ * 1 / (s_i + s_j), products in precision, FP32 accumulate; correction (float,
+ * batch layout). */
+int msq_correction(msq_runtime *runtime, int precision, uint32_t count,
+                   const uint32_t *orders, const float *vectors, const float *roots,
+                   const float *residual_f32, float *correction);
+
+/* measureF32Orthogonality (src/metal_runtime.cpp:742-776):
+ * ||V^T V - I||_F / sqrt(n) per matrix into results (host, count). */
+int msq_orthogonality(msq_runtime *runtime, uint32_t count, const uint32_t *orders,
+                      const float *vectors, double *results);
+
+/* ------------------------------------------------------------------ solve */
+
+/* solveGpuResidentSpd (src/metal_solver.cpp:92-336) for a batch of uneven
+ * SPD matrices: Jacobi, reconstruction, binary64 residual and corrections,
+ * one fixed launch sequence on the device; each matrix stops on its own
+ * (converged matrices skip the remaining launches).  matrices and roots are
+ * device pointers in batch layout (binary64); options has 1 or count entries;
+ * reports (host, count) may be NULL.  Returns MSQ_OK when the launches ran;
+ * per-matrix outcomes are in reports[i].status. */
+int msq_solve_batched(msq_runtime *runtime, uint32_t count, const uint32_t *orders,
+                      const double *matrices, double *roots,
+                      const msq_solve_options *options, uint32_t options_count,
+                      msq_solve_report *reports);
+
+/* Optional eigenvector / root-eigenvalue outputs of the last msq_solve_batched
+ * call on this runtime (device pointers owned by the runtime, valid until the
+ * next solve; float, batch / vector layout). */
+int msq_last_factors(msq_runtime *runtime, const float **vectors, const float **roots);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSQ_HIP_H */

@@ -1,0 +1,1208 @@
+// C ABI of libmsq_hip.so (include/msq_hip.h): runtime, codecs, Jacobi,
+// products and the batched GPU-resident solve.
+//
+// The solve follows solveGpuResidentSpd (src/metal_solver.cpp:92-336):
+// Jacobi -> frozen factor -> reconstruction -> residual -> corrections, with
+// three differences of design: the whole batch shares each launch, the
+// residual is binary64 (the reference's float-float, :1268-1317 of the .metal),
+// and the per-matrix stopping decision is taken on the device (the reference
+// reads the norm back after every correction, src/metal_runtime.cpp:950-965),
+// so the launch sequence is fixed and no host round trip sits between
+// corrections.
+#include "../../include/msq_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return set_error(MSQ_ERROR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                       __FILE__, __LINE__);                                               \
+  } while (0)
+
+#define MSQ_TRY(expr)          \
+  do {                         \
+    int s_ = (expr);           \
+    if (s_ != MSQ_OK) return s_; \
+  } while (0)
+
+uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
+
+// Device scratch owned by the runtime; grows, never shrinks, reused by calls.
+struct Arena {
+  char* base = nullptr;
+  size_t capacity = 0;
+};
+
+}  // namespace
+
+struct msq_runtime {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  Arena arena;
+  float* last_vectors = nullptr;
+  float* last_roots = nullptr;
+};
+
+namespace {
+
+int activate(msq_runtime* rt) {
+  if (!rt) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "runtime is null");
+  HIP_TRY(hipSetDevice(rt->device));
+  return MSQ_OK;
+}
+
+// Reserves `bytes` of runtime scratch for one call (256-byte aligned).  All
+// reservations of a call are made before any kernel is enqueued, and the
+// arena is only regrown between calls, after a synchronize.
+struct Scratch {
+  msq_runtime* rt;
+  std::vector<std::pair<void**, size_t>> wants;
+  template <typename T>
+  void want(T*& ptr, size_t count) {
+    wants.push_back({reinterpret_cast<void**>(&ptr), count * sizeof(T)});
+  }
+  int commit() {
+    size_t total = 0;
+    for (auto& w : wants) total += (w.second + 255) / 256 * 256;
+    Arena& a = rt->arena;
+    if (total > a.capacity) {
+      const size_t grow = std::max(total, a.capacity * 3 / 2);
+      HIP_TRY(hipStreamSynchronize(rt->stream));
+      if (a.base) HIP_TRY(hipFree(a.base));
+      a.base = nullptr;
+      a.capacity = 0;
+      hipError_t e = hipMalloc(&a.base, grow);
+      if (e != hipSuccess)
+        return set_error(MSQ_ERROR_OUT_OF_MEMORY, "device scratch of %zu bytes: %s", grow,
+                         hipGetErrorString(e));
+      a.capacity = grow;
+    }
+    size_t off = 0;
+    for (auto& w : wants) {
+      *w.first = a.base + off;
+      off += (w.second + 255) / 256 * 256;
+    }
+    return MSQ_OK;
+  }
+};
+
+struct Layout {
+  std::vector<uint32_t> order, ld;
+  std::vector<uint64_t> off, voff;
+  uint64_t total = 0, vtotal = 0;
+  uint32_t max_ld = 0;
+};
+
+int make_layout(uint32_t count, const uint32_t* orders, Layout& L) {
+  if (count == 0) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "empty batch");
+  if (!orders) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "orders is null");
+  L.order.assign(orders, orders + count);
+  L.ld.resize(count);
+  L.off.resize(count);
+  L.voff.resize(count);
+  for (uint32_t i = 0; i < count; ++i) {
+    if (orders[i] < 2 || orders[i] > 65504)
+      return set_error(MSQ_ERROR_INVALID_ARGUMENT, "order %u of matrix %u is outside [2, 65504]",
+                       orders[i], i);
+    L.ld[i] = round_up(orders[i], 32);
+    L.off[i] = L.total;
+    L.voff[i] = L.vtotal;
+    L.total += static_cast<uint64_t>(L.ld[i]) * L.ld[i];
+    L.vtotal += L.ld[i];
+    L.max_ld = std::max(L.max_ld, L.ld[i]);
+  }
+  return MSQ_OK;
+}
+
+msq::Format format_of(int precision, bool* ok) {
+  *ok = true;
+  switch (precision) {
+    case MSQ_BF16: return msq::Format::Bf16;
+    case MSQ_FP8_E4M3FN: return msq::Format::Fp8E4M3Fn;
+    case MSQ_FP8_E5M2: return msq::Format::Fp8E5M2;
+    case MSQ_MXFP8_E4M3FN: return msq::Format::Mxfp8E4M3Fn;
+    case MSQ_MXFP8_E5M2: return msq::Format::Mxfp8E5M2;
+    case MSQ_FP4_E2M1: return msq::Format::Fp4E2M1;
+    case MSQ_MXFP4_E2M1: return msq::Format::Mxfp4E2M1;
+    default: *ok = false; return msq::Format::Fp32;
+  }
+}
+
+bool is_mx(int precision) {
+  return precision == MSQ_MXFP8_E4M3FN || precision == MSQ_MXFP8_E5M2 ||
+         precision == MSQ_MXFP4_E2M1;
+}
+
+// Products supported by the MFMA GEMMs.
+bool product_supported(int p) {
+  return p == MSQ_FP32 || p == MSQ_BF16 || p == MSQ_FP8_E4M3FN_SCALED || p == MSQ_MXFP4_E2M1;
+}
+
+msq::GemmType gemm_type(int p) {
+  switch (p) {
+    case MSQ_BF16: return msq::GemmType::Bf16;
+    case MSQ_FP8_E4M3FN_SCALED: return msq::GemmType::Fp8;
+    case MSQ_MXFP4_E2M1: return msq::GemmType::Mxfp4;
+    default: return msq::GemmType::F32;
+  }
+}
+
+// Bytes of the encoded operand for `elements` values in precision p.
+size_t operand_bytes(int p, uint64_t elements) {
+  switch (p) {
+    case MSQ_BF16: return elements * 2;
+    case MSQ_FP8_E4M3FN_SCALED: return elements;
+    case MSQ_MXFP4_E2M1: return (elements + 1) / 2;
+    default: return elements * 4;
+  }
+}
+
+template <typename T>
+int upload(msq_runtime* rt, T* dst, const std::vector<T>& src) {
+  if (src.empty()) return MSQ_OK;
+  HIP_TRY(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice,
+                         rt->stream));
+  return MSQ_OK;
+}
+
+// -------------------------------------------------------------- Jacobi plan
+struct JacobiClass {
+  uint32_t max_order;
+  bool v_in_lds;
+  int threads;
+};
+// Size classes of the one-workgroup-per-matrix kernel: V in LDS while
+// A (packed) + V fit comfortably, more threads for larger orders.
+const JacobiClass kJacobiClasses[] = {
+    {32, true, 64},
+    {64, true, 128},
+    {128, true, 256},
+    {256, false, 512},
+};
+constexpr size_t kJacobiClassCount = sizeof(kJacobiClasses) / sizeof(kJacobiClasses[0]);
+
+msq::JacobiJob make_job(const Layout& L, uint32_t i, const msq_jacobi_options& o) {
+  msq::JacobiJob j{};
+  j.a_offset = L.off[i];
+  j.v_offset = L.off[i];
+  j.s_offset = L.voff[i];
+  j.order = L.order[i];
+  j.ld = L.ld[i];
+  j.max_sweeps = o.max_sweeps ? o.max_sweeps : 30;
+  j.fixed_sweeps = o.fixed_sweeps;
+  j.off_tol = o.off_tol;
+  j.off_tol_lambda = o.off_tol_lambda;
+  j.stagnation = o.stagnation > 0.0f ? o.stagnation : 0.5f;
+  j.floor_tol = o.floor_tol > 0.0f ? o.floor_tol : 1e-5f;
+  j.status_index = i;
+  return j;
+}
+
+// Device-memory Jacobi of one matrix above the batched kernel's limit: one
+// launch per round, the off-norm read back after each sweep.
+int jacobi_large(msq_runtime* rt, const Layout& L, uint32_t i, const double* a64, float* V,
+                 float* diag, float* roots, const msq::JacobiJob& job, msq::JacobiStatus& st) {
+  const uint32_t n = L.order[i], ld = L.ld[i];
+  const uint32_t np = n + (n & 1u);
+  const uint32_t m = np / 2;
+  const int partials = 256;
+  char* buf = nullptr;
+  const size_t abytes = static_cast<size_t>(ld) * ld * sizeof(float);
+  const size_t bytes = abytes + 2 * m * sizeof(int) + m * sizeof(float2) +
+                       partials * (sizeof(double) + sizeof(float)) + 1024;
+  HIP_TRY(hipMalloc(&buf, bytes));
+  float* A = reinterpret_cast<float*>(buf);
+  int* sp = reinterpret_cast<int*>(buf + abytes);
+  int* sq = sp + m;
+  float2* rot = reinterpret_cast<float2*>(sq + m);  // 8 m bytes past sp: aligned
+  double* off2 = reinterpret_cast<double*>(rot + m);
+  float* dmin = reinterpret_cast<float*>(off2 + partials);
+  float* Vi = V + L.off[i];
+  int status = MSQ_OK;
+  do {
+    hipError_t e = msq::launch_global_jacobi_init(a64 + L.off[i], A, Vi, n, ld, rt->stream);
+    if (e != hipSuccess) { status = set_error(MSQ_ERROR_HIP, "jacobi init: %s", hipGetErrorString(e)); break; }
+    // ||A||_F in binary64 from the input.
+    std::vector<double> host_a(static_cast<size_t>(ld) * ld);
+    e = hipMemcpyAsync(host_a.data(), a64 + L.off[i], host_a.size() * sizeof(double),
+                       hipMemcpyDeviceToHost, rt->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(rt->stream);
+    if (e != hipSuccess) { status = set_error(MSQ_ERROR_HIP, "jacobi norm: %s", hipGetErrorString(e)); break; }
+    double norm2 = 0.0;
+    for (double x : host_a) norm2 += x * x;
+    st = msq::JacobiStatus{};
+    st.matrix_norm = std::sqrt(norm2);
+    const uint32_t cap = job.fixed_sweeps ? job.fixed_sweeps : job.max_sweeps;
+    double previous = INFINITY;
+    std::vector<double> h_off(partials);
+    std::vector<float> h_min(partials);
+    for (uint32_t sweep = 0; sweep < cap && status == MSQ_OK; ++sweep) {
+      for (uint32_t t = 0; t + 1 < np && e == hipSuccess; ++t)
+        e = msq::launch_global_jacobi_round(A, Vi, n, ld, t, sp, sq, rot, rt->stream);
+      if (e == hipSuccess) e = msq::launch_global_offnorm(A, n, ld, off2, dmin, partials, rt->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(h_off.data(), off2, partials * sizeof(double), hipMemcpyDeviceToHost,
+                           rt->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(h_min.data(), dmin, partials * sizeof(float), hipMemcpyDeviceToHost,
+                           rt->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(rt->stream);
+      if (e != hipSuccess) { status = set_error(MSQ_ERROR_HIP, "jacobi sweep: %s", hipGetErrorString(e)); break; }
+      double s = 0.0;
+      float d = INFINITY;
+      for (int k = 0; k < partials; ++k) {
+        s += h_off[k];
+        d = std::min(d, h_min[k]);
+      }
+      const double off = std::sqrt(s);
+      st.sweeps = sweep + 1;
+      st.off_norm = off;
+      st.min_diagonal = d;
+      if (sweep < msq::kJacobiHistory) st.off_history[sweep] = static_cast<float>(off / st.matrix_norm);
+      bool stop;
+      if (job.fixed_sweeps) {
+        stop = st.sweeps >= job.fixed_sweeps;
+      } else {
+        double tol = job.off_tol * st.matrix_norm;
+        if (job.off_tol_lambda > 0.0f && d > 0.0f) tol = std::max(tol, double(job.off_tol_lambda) * d);
+        stop = off <= tol || st.sweeps >= cap ||
+               (off > job.stagnation * previous && off <= job.floor_tol * st.matrix_norm);
+      }
+      previous = off;
+      if (stop) break;
+    }
+    if (status != MSQ_OK) break;
+    e = msq::launch_global_jacobi_finish(A, n, ld, diag + L.voff[i], roots + L.voff[i], rt->stream);
+    std::vector<float> hd(n);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hd.data(), diag + L.voff[i], n * sizeof(float), hipMemcpyDeviceToHost,
+                         rt->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(rt->stream);
+    if (e != hipSuccess) { status = set_error(MSQ_ERROR_HIP, "jacobi finish: %s", hipGetErrorString(e)); break; }
+    for (float x : hd)
+      if (!(x > 0.0f && std::isfinite(x))) st.flags |= msq::kJacobiInvalidSpectrum;
+  } while (false);
+  hipError_t fe = hipFree(buf);
+  if (status == MSQ_OK && fe != hipSuccess)
+    status = set_error(MSQ_ERROR_HIP, "hipFree: %s", hipGetErrorString(fe));
+  return status;
+}
+
+// Jacobi on a whole batch.  djobs: device room for `count` jobs; jstatus:
+// device status array in matrix order.
+int run_jacobi(msq_runtime* rt, const Layout& L, const double* a64, float* V, float* diag,
+               float* roots, const msq_jacobi_options* opts, uint32_t opts_count,
+               msq::JacobiStatus* jstatus, msq::JacobiJob* djobs,
+               std::vector<msq::JacobiJob>& jobs) {
+  const uint32_t count = static_cast<uint32_t>(L.order.size());
+  jobs.clear();
+  size_t begin[kJacobiClassCount + 1];
+  for (size_t c = 0; c < kJacobiClassCount; ++c) {
+    const uint32_t lo = c == 0 ? 0 : kJacobiClasses[c - 1].max_order;
+    begin[c] = jobs.size();
+    for (uint32_t i = 0; i < count; ++i)
+      if (L.order[i] > lo && L.order[i] <= kJacobiClasses[c].max_order)
+        jobs.push_back(make_job(L, i, opts[opts_count == 1 ? 0 : i]));
+  }
+  begin[kJacobiClassCount] = jobs.size();
+  if (!jobs.empty()) {
+    MSQ_TRY(upload(rt, djobs, jobs));
+    for (size_t c = 0; c < kJacobiClassCount; ++c) {
+      const size_t b = begin[c], e = begin[c + 1];
+      if (b == e) continue;
+      HIP_TRY(msq::launch_batched_jacobi(djobs + b, static_cast<uint32_t>(e - b),
+                                         kJacobiClasses[c].max_order, kJacobiClasses[c].v_in_lds,
+                                         kJacobiClasses[c].threads, a64, V, diag, roots, jstatus,
+                                         rt->stream));
+    }
+  }
+  for (uint32_t i = 0; i < count; ++i) {
+    if (L.order[i] <= msq::kBatchedJacobiMaxOrder) continue;
+    msq::JacobiStatus st{};
+    MSQ_TRY(jacobi_large(rt, L, i, a64, V, diag, roots, make_job(L, i, opts[opts_count == 1 ? 0 : i]),
+                         st));
+    HIP_TRY(hipMemcpyAsync(jstatus + i, &st, sizeof(st), hipMemcpyHostToDevice, rt->stream));
+    HIP_TRY(hipStreamSynchronize(rt->stream));
+  }
+  return MSQ_OK;
+}
+
+void fill_jacobi_report(const msq::JacobiStatus& s, msq_jacobi_report& r) {
+  r.sweeps = s.sweeps;
+  r.invalid_spectrum = (s.flags & msq::kJacobiInvalidSpectrum) ? 1u : 0u;
+  r.off_norm = s.off_norm;
+  r.matrix_norm = s.matrix_norm;
+  r.min_diagonal = s.min_diagonal;
+  for (uint32_t k = 0; k < MSQ_JACOBI_HISTORY; ++k)
+    r.off_history[k] = k < s.sweeps && k < msq::kJacobiHistory ? s.off_history[k] : 0.0f;
+}
+
+// ---------------------------------------------------------------- GEMM plan
+// Tile lists of one group of matrices: 32 x 32 workgroup tiles for ld = 32,
+// 64 x 64 for larger pitches; only tiles that meet the order are listed.
+struct GemmPlan {
+  std::vector<msq::GemmJob> jobs;        // one per matrix of the group
+  std::vector<uint32_t> tile_job[2];     // [0]: 32-tiles, [1]: 64-tiles
+  std::vector<uint32_t> tile_pos[2];
+  msq::GemmJob* d_jobs = nullptr;
+  uint32_t* d_tile_job[2] = {nullptr, nullptr};
+  uint32_t* d_tile_pos[2] = {nullptr, nullptr};
+};
+
+void build_plan(const Layout& L, const std::vector<uint32_t>& members, bool aux_is_matrix,
+                GemmPlan& P) {
+  P.jobs.clear();
+  for (int t = 0; t < 2; ++t) {
+    P.tile_job[t].clear();
+    P.tile_pos[t].clear();
+  }
+  for (uint32_t i : members) {
+    msq::GemmJob j{};
+    j.a_offset = j.b_offset = j.c_offset = L.off[i];
+    j.aux_offset = aux_is_matrix ? L.off[i] : L.voff[i];
+    j.order = L.order[i];
+    j.ld = L.ld[i];
+    j.status_index = i;
+    const uint32_t jid = static_cast<uint32_t>(P.jobs.size());
+    P.jobs.push_back(j);
+    const int t = L.ld[i] == 32 ? 0 : 1;
+    const uint32_t tile = t == 0 ? 32 : 64;
+    const uint32_t nt = (L.order[i] + tile - 1) / tile;
+    for (uint32_t a = 0; a < nt; ++a)
+      for (uint32_t b = 0; b < nt; ++b) {
+        P.tile_job[t].push_back(jid);
+        P.tile_pos[t].push_back((a << 16) | b);
+      }
+  }
+}
+
+void plan_want(Scratch& S, GemmPlan& P) {
+  S.want(P.d_jobs, std::max<size_t>(1, P.jobs.size()));
+  for (int t = 0; t < 2; ++t) {
+    S.want(P.d_tile_job[t], std::max<size_t>(1, P.tile_job[t].size()));
+    S.want(P.d_tile_pos[t], std::max<size_t>(1, P.tile_pos[t].size()));
+  }
+}
+
+int plan_upload(msq_runtime* rt, GemmPlan& P) {
+  MSQ_TRY(upload(rt, P.d_jobs, P.jobs));
+  for (int t = 0; t < 2; ++t) {
+    MSQ_TRY(upload(rt, P.d_tile_job[t], P.tile_job[t]));
+    MSQ_TRY(upload(rt, P.d_tile_pos[t], P.tile_pos[t]));
+  }
+  return MSQ_OK;
+}
+
+int run_gemm(msq_runtime* rt, const GemmPlan& P, msq::GemmType type, msq::Epilogue epi,
+             msq::GemmArgs args) {
+  args.jobs = P.d_jobs;
+  for (int t = 0; t < 2; ++t) {
+    if (P.tile_job[t].empty()) continue;
+    args.tile_job = P.d_tile_job[t];
+    args.tile_pos = P.d_tile_pos[t];
+    HIP_TRY(msq::launch_gemm(type, epi, t == 0 ? 32 : 64, args,
+                             static_cast<uint32_t>(P.tile_job[t].size()), rt->stream));
+  }
+  return MSQ_OK;
+}
+
+// ------------------------------------------------------- operand encodings
+// One operand of a product in its tier: float data, or codes (+ E8M0 scales
+// for MXFP4, + per-matrix sigma for scaled FP8).
+struct Operand {
+  const void* data = nullptr;
+  const uint8_t* scales = nullptr;
+  const float* sigma = nullptr;
+};
+
+struct TierBuffers {
+  void* codes = nullptr;      // operand_bytes(tier, total)
+  uint8_t* scales = nullptr;  // total / 32 (MXFP4)
+  float* sigma = nullptr;     // count (FP8)
+  uint32_t* amax = nullptr;   // count (FP8)
+};
+
+void tier_want(Scratch& S, TierBuffers& T, int tier, const Layout& L) {
+  const size_t count = L.order.size();
+  if (tier == MSQ_FP32) return;
+  S.want(reinterpret_cast<uint8_t*&>(T.codes), operand_bytes(tier, L.total));
+  if (tier == MSQ_MXFP4_E2M1) S.want(T.scales, L.total / 32);
+  if (tier == MSQ_FP8_E4M3FN_SCALED) {
+    S.want(T.sigma, count);
+    S.want(T.amax, count);
+  }
+}
+
+// Encodes src (float, batch layout) for the segments of one group.  BF16 and
+// MXFP4 are elementwise / 32-block codecs over the whole segment range: the
+// segments of a group are encoded one contiguous run at a time.
+int encode_operand(msq_runtime* rt, int tier, const Layout& L, const std::vector<uint32_t>& members,
+                   const msq::Segment* dsegs, const float* src, TierBuffers& T, Operand& out) {
+  if (tier == MSQ_FP32) {
+    out.data = src;
+    return MSQ_OK;
+  }
+  if (tier == MSQ_FP8_E4M3FN_SCALED) {
+    HIP_TRY(msq::launch_fill_u32(T.amax, L.order.size(), 0u, rt->stream));
+    HIP_TRY(msq::launch_segment_fp8(dsegs, static_cast<uint32_t>(members.size()), L.max_ld, src,
+                                    T.amax, T.sigma, static_cast<uint8_t*>(T.codes), rt->stream));
+    out.data = T.codes;
+    out.sigma = T.sigma;
+    return MSQ_OK;
+  }
+  const msq::Format f = tier == MSQ_BF16 ? msq::Format::Bf16 : msq::Format::Mxfp4E2M1;
+  size_t k = 0;
+  while (k < members.size()) {
+    size_t e = k + 1;
+    while (e < members.size() && members[e] == members[e - 1] + 1) ++e;
+    const uint64_t begin = L.off[members[k]];
+    const uint64_t end = L.off[members[e - 1]] + static_cast<uint64_t>(L.ld[members[e - 1]]) * L.ld[members[e - 1]];
+    void* dst = f == msq::Format::Bf16
+                    ? static_cast<void*>(static_cast<uint16_t*>(T.codes) + begin)
+                    : static_cast<void*>(static_cast<uint8_t*>(T.codes) + begin / 2);
+    HIP_TRY(msq::launch_encode(f, src + begin, dst, T.scales ? T.scales + begin / 32 : nullptr,
+                               end - begin, rt->stream));
+    k = e;
+  }
+  out.data = T.codes;
+  out.scales = T.scales;
+  return MSQ_OK;
+}
+
+msq::GemmArgs product_args(const Operand& a, const Operand& b, void* c, const float* alpha,
+                           const int* active) {
+  msq::GemmArgs g{};
+  g.a = a.data;
+  g.b = b.data;
+  g.a_scales = a.scales;
+  g.b_scales = b.scales;
+  g.alpha = alpha;
+  g.active = active;
+  g.c = c;
+  return g;
+}
+
+}  // namespace
+
+namespace {
+
+std::vector<msq::Segment> segments_of(const Layout& L, const std::vector<uint32_t>& members) {
+  std::vector<msq::Segment> segs;
+  segs.reserve(members.size());
+  for (uint32_t i : members) {
+    msq::Segment g{};
+    g.offset = L.off[i];
+    g.vec_offset = L.voff[i];
+    g.order = L.order[i];
+    g.ld = L.ld[i];
+    g.index = i;
+    segs.push_back(g);
+  }
+  return segs;
+}
+
+std::vector<uint32_t> iota_members(uint32_t count) {
+  std::vector<uint32_t> m(count);
+  for (uint32_t i = 0; i < count; ++i) m[i] = i;
+  return m;
+}
+
+// Products of one group share these device buffers.
+struct CorrectionBuffers {
+  float* t1 = nullptr;  // float, batch layout
+  float* t2 = nullptr;
+  TierBuffers enc_a, enc_b;
+  float* alpha = nullptr;  // count
+};
+
+int fp8_alpha(msq_runtime* rt, int tier, const Operand& a, const Operand& b, float* alpha,
+              uint32_t count, const float** out) {
+  if (tier != MSQ_FP8_E4M3FN_SCALED) {
+    *out = nullptr;
+    return MSQ_OK;
+  }
+  HIP_TRY(msq::launch_alpha(a.sigma, b.sigma, alpha, count, rt->stream));
+  *out = alpha;
+  return MSQ_OK;
+}
+
+// One frozen-factor correction for a group (refineUpdateAndMeasure's product
+// chain, src/metal_runtime.cpp:867-928):
+//   W = V^T R ; Y = (W V) o H ; W2 = V Y ; E = W2 V^T   (E into cb.t2)
+// ov / ovt: V and V^T encoded in the group's tier; r32: R as float.
+int correction_products(msq_runtime* rt, int tier, const Layout& L,
+                        const std::vector<uint32_t>& members, const msq::Segment* dsegs,
+                        const GemmPlan& plan, const Operand& ov, const Operand& ovt,
+                        const float* r32, const float* roots, const int* active,
+                        CorrectionBuffers& cb) {
+  const uint32_t count = static_cast<uint32_t>(L.order.size());
+  const msq::GemmType type = gemm_type(tier);
+  const float* alpha = nullptr;
+  Operand opr, opa;
+  // W^T <- (V^T R)^T
+  MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, r32, cb.enc_b, opr));
+  MSQ_TRY(fp8_alpha(rt, tier, ov, opr, cb.alpha, count, &alpha));
+  msq::GemmArgs g = product_args(ov, opr, cb.t1, alpha, active);
+  g.transpose_c = 1;
+  MSQ_TRY(run_gemm(rt, plan, type, msq::Epilogue::Store, g));
+  // Y <- (W V) o H
+  MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, cb.t1, cb.enc_a, opa));
+  MSQ_TRY(fp8_alpha(rt, tier, opa, ov, cb.alpha, count, &alpha));
+  g = product_args(opa, ov, cb.t2, alpha, active);
+  g.aux = roots;
+  MSQ_TRY(run_gemm(rt, plan, type, msq::Epilogue::Hadamard, g));
+  // W2^T storage <- V Y, row-major
+  MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, cb.t2, cb.enc_b, opr));
+  MSQ_TRY(fp8_alpha(rt, tier, ovt, opr, cb.alpha, count, &alpha));
+  g = product_args(ovt, opr, cb.t1, alpha, active);
+  g.transpose_c = 1;
+  MSQ_TRY(run_gemm(rt, plan, type, msq::Epilogue::Store, g));
+  // E <- W2 V^T
+  MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, cb.t1, cb.enc_a, opa));
+  MSQ_TRY(fp8_alpha(rt, tier, opa, ovt, cb.alpha, count, &alpha));
+  g = product_args(opa, ovt, cb.t2, alpha, active);
+  MSQ_TRY(run_gemm(rt, plan, type, msq::Epilogue::Store, g));
+  return MSQ_OK;
+}
+
+// X0 = G G^T, G = V diag(sqrt(s)) in the tier (into cb.t2).
+int reconstruction_product(msq_runtime* rt, int tier, const Layout& L,
+                           const std::vector<uint32_t>& members, const msq::Segment* dsegs,
+                           const GemmPlan& plan, const float* V, const float* roots,
+                           const int* active, CorrectionBuffers& cb) {
+  const uint32_t count = static_cast<uint32_t>(L.order.size());
+  HIP_TRY(msq::launch_scale_rows_transpose(dsegs, static_cast<uint32_t>(members.size()), L.max_ld,
+                                           V, roots, 1, cb.t1, rt->stream));
+  Operand og;
+  MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, cb.t1, cb.enc_a, og));
+  const float* alpha = nullptr;
+  MSQ_TRY(fp8_alpha(rt, tier, og, og, cb.alpha, count, &alpha));
+  msq::GemmArgs g = product_args(og, og, cb.t2, alpha, active);
+  MSQ_TRY(run_gemm(rt, plan, gemm_type(tier), msq::Epilogue::Store, g));
+  return MSQ_OK;
+}
+
+int residual_product(msq_runtime* rt, const GemmPlan& plan, const double* A, const double* X,
+                     double* R, float* R32, double* norms, const int* active, uint32_t count) {
+  HIP_TRY(msq::launch_zero_f64(norms, count, rt->stream));
+  msq::GemmArgs g{};
+  g.a = X;
+  g.b = X;
+  g.c = R;
+  g.aux = A;
+  g.c_float = R32;
+  g.norms = norms;
+  g.active = active;
+  MSQ_TRY(run_gemm(rt, plan, msq::GemmType::F64, msq::Epilogue::Residual, g));
+  return MSQ_OK;
+}
+
+void want_correction(Scratch& S, CorrectionBuffers& cb, const Layout& L, int max_tier) {
+  S.want(cb.t1, L.total);
+  S.want(cb.t2, L.total);
+  tier_want(S, cb.enc_a, max_tier, L);
+  tier_want(S, cb.enc_b, max_tier, L);
+  S.want(cb.alpha, L.order.size());
+}
+
+// The widest tier needs the most code bytes; FP8 needs sigma arrays, MXFP4
+// scale planes: reserve all three shapes when several tiers share buffers.
+void tier_want_all(Scratch& S, TierBuffers& T, const Layout& L) {
+  const size_t count = L.order.size();
+  S.want(reinterpret_cast<uint8_t*&>(T.codes), operand_bytes(MSQ_BF16, L.total));
+  S.want(T.scales, L.total / 32);
+  S.want(T.sigma, count);
+  S.want(T.amax, count);
+}
+
+int validate_tier(int p, const char* what) {
+  if (!product_supported(p))
+    return set_error(MSQ_ERROR_UNSUPPORTED,
+                     "%s precision %d has no MFMA product (use FP32, BF16, FP8_E4M3FN_SCALED "
+                     "or MXFP4_E2M1)",
+                     what, p);
+  return MSQ_OK;
+}
+
+}  // namespace
+
+// ============================================================== C entry points
+extern "C" {
+
+const char* msq_last_error(void) { return g_last_error.c_str(); }
+int msq_abi_version(void) { return MSQ_ABI_VERSION; }
+
+int msq_runtime_create(int device, msq_runtime** out) {
+  if (!out) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "out is null");
+  *out = nullptr;
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "device %d not in [0, %d)", device, n);
+  HIP_TRY(hipSetDevice(device));
+  msq_runtime* rt = new msq_runtime();
+  rt->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&rt->own_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete rt;
+    return set_error(MSQ_ERROR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  rt->stream = rt->own_stream;
+  *out = rt;
+  return MSQ_OK;
+}
+
+int msq_runtime_destroy(msq_runtime* rt) {
+  if (!rt) return MSQ_OK;
+  // Teardown is best effort: report the first failure but release everything.
+  int status = MSQ_OK;
+  hipError_t e = hipSetDevice(rt->device);
+  if (e == hipSuccess) e = hipStreamSynchronize(rt->stream);
+  if (e != hipSuccess) status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
+  if (rt->arena.base && (e = hipFree(rt->arena.base)) != hipSuccess && status == MSQ_OK)
+    status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
+  if (rt->own_stream && (e = hipStreamDestroy(rt->own_stream)) != hipSuccess && status == MSQ_OK)
+    status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
+  delete rt;
+  return status;
+}
+
+int msq_runtime_set_stream(msq_runtime* rt, void* stream) {
+  MSQ_TRY(activate(rt));
+  rt->stream = stream ? static_cast<hipStream_t>(stream) : rt->own_stream;
+  return MSQ_OK;
+}
+
+int msq_synchronize(msq_runtime* rt) {
+  MSQ_TRY(activate(rt));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  return MSQ_OK;
+}
+
+int msq_alloc(msq_runtime* rt, size_t bytes, void** ptr) {
+  MSQ_TRY(activate(rt));
+  if (!ptr) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "ptr is null");
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+  if (e != hipSuccess)
+    return set_error(MSQ_ERROR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  return MSQ_OK;
+}
+
+int msq_free(msq_runtime* rt, void* ptr) {
+  MSQ_TRY(activate(rt));
+  if (ptr) HIP_TRY(hipFree(ptr));
+  return MSQ_OK;
+}
+
+int msq_copy_to_device(msq_runtime* rt, void* dst, const void* src, size_t bytes) {
+  MSQ_TRY(activate(rt));
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  return MSQ_OK;
+}
+
+int msq_copy_to_host(msq_runtime* rt, void* dst, const void* src, size_t bytes) {
+  MSQ_TRY(activate(rt));
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  return MSQ_OK;
+}
+
+int msq_memset(msq_runtime* rt, void* dst, int value, size_t bytes) {
+  MSQ_TRY(activate(rt));
+  HIP_TRY(hipMemsetAsync(dst, value, bytes, rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  return MSQ_OK;
+}
+
+size_t msq_storage_bytes(int precision, size_t elements) {
+  size_t bits = 32;
+  switch (precision) {
+    case MSQ_FP32: bits = 32; break;
+    case MSQ_BF16: bits = 16; break;
+    case MSQ_FP4_E2M1:
+    case MSQ_MXFP4_E2M1:
+    case MSQ_FP4_E2M1_LOG_S: bits = 4; break;
+    default: bits = 8; break;
+  }
+  return (elements * bits + 7) / 8;
+}
+
+size_t msq_scale_bytes(int precision, size_t elements) {
+  return is_mx(precision) ? (elements + 31) / 32 : 0;
+}
+
+int msq_batch_layout(uint32_t count, const uint32_t* orders, uint32_t* lds, uint64_t* offsets,
+                     uint64_t* vector_offsets, uint64_t* total, uint64_t* vtotal) {
+  Layout L;
+  MSQ_TRY(make_layout(count, orders, L));
+  for (uint32_t i = 0; i < count; ++i) {
+    if (lds) lds[i] = L.ld[i];
+    if (offsets) offsets[i] = L.off[i];
+    if (vector_offsets) vector_offsets[i] = L.voff[i];
+  }
+  if (total) *total = L.total;
+  if (vtotal) *vtotal = L.vtotal;
+  return MSQ_OK;
+}
+
+int msq_quantize(msq_runtime* rt, int precision, const float* src, void* values, void* scales,
+                 size_t elements) {
+  MSQ_TRY(activate(rt));
+  bool ok = false;
+  const msq::Format f = format_of(precision, &ok);
+  if (!ok) return set_error(MSQ_ERROR_UNSUPPORTED, "precision %d has no linear codec", precision);
+  if (is_mx(precision) && !scales)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "MX storage requires a scale plane");
+  if (elements && (!src || !values)) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  if (reinterpret_cast<uintptr_t>(src) % 16 || reinterpret_cast<uintptr_t>(values) % 16)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "buffers must be 16-byte aligned");
+  HIP_TRY(msq::launch_encode(f, src, values, static_cast<uint8_t*>(scales), elements, rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  return MSQ_OK;
+}
+
+int msq_decode(msq_runtime* rt, int precision, const void* values, const void* scales,
+               float* dst, size_t elements) {
+  MSQ_TRY(activate(rt));
+  bool ok = false;
+  const msq::Format f = format_of(precision, &ok);
+  if (!ok) return set_error(MSQ_ERROR_UNSUPPORTED, "precision %d has no linear codec", precision);
+  if (is_mx(precision) && !scales)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "MX storage requires a scale plane");
+  if (elements && (!dst || !values)) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  if (reinterpret_cast<uintptr_t>(dst) % 16 || reinterpret_cast<uintptr_t>(values) % 16)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "buffers must be 16-byte aligned");
+  HIP_TRY(msq::launch_decode(f, values, static_cast<const uint8_t*>(scales), dst, elements,
+                             rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  return MSQ_OK;
+}
+
+int msq_quantize_and_decode(msq_runtime* rt, int precision, const float* src, void* values,
+                            void* scales, float* decoded, size_t elements) {
+  MSQ_TRY(msq_quantize(rt, precision, src, values, scales, elements));
+  return msq_decode(rt, precision, values, scales, decoded, elements);
+}
+
+int msq_quantize_fp8_scaled(msq_runtime* rt, const float* src, void* values, float* sigma,
+                            size_t elements) {
+  MSQ_TRY(activate(rt));
+  if (!sigma || (elements && (!src || !values)))
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  uint32_t* amax = nullptr;
+  Scratch S{rt};
+  S.want(amax, 1);
+  MSQ_TRY(S.commit());
+  HIP_TRY(msq::launch_fp8_amax_sigma(src, elements, amax, sigma, rt->stream));
+  HIP_TRY(msq::launch_encode_fp8_scaled(src, static_cast<uint8_t*>(values), sigma, elements,
+                                        rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  return MSQ_OK;
+}
+
+int msq_decode_fp8_scaled(msq_runtime* rt, const void* values, const float* sigma, float* dst,
+                          size_t elements) {
+  MSQ_TRY(activate(rt));
+  if (!sigma || (elements && (!dst || !values)))
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  HIP_TRY(msq::launch_decode_fp8_scaled(static_cast<const uint8_t*>(values), sigma, dst, elements,
+                                        rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  return MSQ_OK;
+}
+
+int msq_jacobi_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
+                       const double* matrices, float* vectors, float* diagonal, float* roots,
+                       const msq_jacobi_options* options, uint32_t options_count,
+                       msq_jacobi_report* reports) {
+  MSQ_TRY(activate(rt));
+  Layout L;
+  MSQ_TRY(make_layout(count, orders, L));
+  if (!matrices || !vectors || !diagonal || !roots || !options)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  if (options_count != 1 && options_count != count)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "options_count must be 1 or count");
+  Scratch S{rt};
+  msq::JacobiStatus* jst = nullptr;
+  msq::JacobiJob* djobs = nullptr;
+  S.want(jst, count);
+  S.want(djobs, count);
+  MSQ_TRY(S.commit());
+  std::vector<msq::JacobiJob> jobs;
+  MSQ_TRY(run_jacobi(rt, L, matrices, vectors, diagonal, roots, options, options_count, jst, djobs,
+                     jobs));
+  std::vector<msq::JacobiStatus> hs(count);
+  HIP_TRY(hipMemcpyAsync(hs.data(), jst, count * sizeof(msq::JacobiStatus), hipMemcpyDeviceToHost,
+                         rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  if (reports)
+    for (uint32_t i = 0; i < count; ++i) fill_jacobi_report(hs[i], reports[i]);
+  return MSQ_OK;
+}
+
+int msq_reconstruct(msq_runtime* rt, int precision, uint32_t count, const uint32_t* orders,
+                    const float* vectors, const float* roots, float* root) {
+  MSQ_TRY(activate(rt));
+  MSQ_TRY(validate_tier(precision, "reconstruction"));
+  Layout L;
+  MSQ_TRY(make_layout(count, orders, L));
+  if (!vectors || !roots || !root) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  const std::vector<uint32_t> members = iota_members(count);
+  const std::vector<msq::Segment> segs = segments_of(L, members);
+  GemmPlan plan;
+  build_plan(L, members, false, plan);
+  Scratch S{rt};
+  msq::Segment* dsegs = nullptr;
+  CorrectionBuffers cb;
+  S.want(dsegs, count);
+  want_correction(S, cb, L, precision);
+  plan_want(S, plan);
+  MSQ_TRY(S.commit());
+  MSQ_TRY(upload(rt, dsegs, segs));
+  MSQ_TRY(plan_upload(rt, plan));
+  MSQ_TRY(reconstruction_product(rt, precision, L, members, dsegs, plan, vectors, roots, nullptr,
+                                 cb));
+  HIP_TRY(hipMemcpyAsync(root, cb.t2, L.total * sizeof(float), hipMemcpyDeviceToDevice, rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  return MSQ_OK;
+}
+
+int msq_residual_fp64(msq_runtime* rt, uint32_t count, const uint32_t* orders,
+                      const double* matrices, const double* root, double* residual,
+                      float* residual_f32, double* relative_norms) {
+  MSQ_TRY(activate(rt));
+  Layout L;
+  MSQ_TRY(make_layout(count, orders, L));
+  if (!matrices || !root || !residual || !residual_f32)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  const std::vector<uint32_t> members = iota_members(count);
+  const std::vector<msq::Segment> segs = segments_of(L, members);
+  GemmPlan plan;
+  build_plan(L, members, true, plan);
+  Scratch S{rt};
+  msq::Segment* dsegs = nullptr;
+  double* norms = nullptr;
+  double* anorms = nullptr;
+  S.want(dsegs, count);
+  S.want(norms, count);
+  S.want(anorms, count);
+  plan_want(S, plan);
+  MSQ_TRY(S.commit());
+  MSQ_TRY(upload(rt, dsegs, segs));
+  MSQ_TRY(plan_upload(rt, plan));
+  MSQ_TRY(residual_product(rt, plan, matrices, root, residual, residual_f32, norms, nullptr, count));
+  HIP_TRY(msq::launch_zero_f64(anorms, count, rt->stream));
+  HIP_TRY(msq::launch_sumsq_f64(dsegs, count, L.max_ld, matrices, anorms, rt->stream));
+  std::vector<double> hn(count), ha(count);
+  HIP_TRY(hipMemcpyAsync(hn.data(), norms, count * sizeof(double), hipMemcpyDeviceToHost, rt->stream));
+  HIP_TRY(hipMemcpyAsync(ha.data(), anorms, count * sizeof(double), hipMemcpyDeviceToHost, rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  if (relative_norms)
+    for (uint32_t i = 0; i < count; ++i) relative_norms[i] = std::sqrt(hn[i] / ha[i]);
+  return MSQ_OK;
+}
+
+int msq_correction(msq_runtime* rt, int precision, uint32_t count, const uint32_t* orders,
+                   const float* vectors, const float* roots, const float* residual_f32,
+                   float* correction) {
+  MSQ_TRY(activate(rt));
+  MSQ_TRY(validate_tier(precision, "correction"));
+  Layout L;
+  MSQ_TRY(make_layout(count, orders, L));
+  if (!vectors || !roots || !residual_f32 || !correction)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  const std::vector<uint32_t> members = iota_members(count);
+  const std::vector<msq::Segment> segs = segments_of(L, members);
+  GemmPlan plan;
+  build_plan(L, members, false, plan);
+  Scratch S{rt};
+  msq::Segment* dsegs = nullptr;
+  float* vt = nullptr;
+  CorrectionBuffers cb;
+  TierBuffers ev, evt;
+  S.want(dsegs, count);
+  S.want(vt, L.total);
+  want_correction(S, cb, L, precision);
+  tier_want(S, ev, precision, L);
+  tier_want(S, evt, precision, L);
+  plan_want(S, plan);
+  MSQ_TRY(S.commit());
+  MSQ_TRY(upload(rt, dsegs, segs));
+  MSQ_TRY(plan_upload(rt, plan));
+  HIP_TRY(msq::launch_transpose(dsegs, count, L.max_ld, vectors, vt, rt->stream));
+  Operand ov, ovt;
+  MSQ_TRY(encode_operand(rt, precision, L, members, dsegs, vectors, ev, ov));
+  MSQ_TRY(encode_operand(rt, precision, L, members, dsegs, vt, evt, ovt));
+  MSQ_TRY(correction_products(rt, precision, L, members, dsegs, plan, ov, ovt, residual_f32, roots,
+                              nullptr, cb));
+  HIP_TRY(hipMemcpyAsync(correction, cb.t2, L.total * sizeof(float), hipMemcpyDeviceToDevice,
+                         rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  return MSQ_OK;
+}
+
+int msq_orthogonality(msq_runtime* rt, uint32_t count, const uint32_t* orders,
+                      const float* vectors, double* results) {
+  MSQ_TRY(activate(rt));
+  Layout L;
+  MSQ_TRY(make_layout(count, orders, L));
+  if (!vectors || !results) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  const std::vector<uint32_t> members = iota_members(count);
+  const std::vector<msq::Segment> segs = segments_of(L, members);
+  GemmPlan plan;
+  build_plan(L, members, false, plan);
+  Scratch S{rt};
+  msq::Segment* dsegs = nullptr;
+  float* gram = nullptr;
+  double* out = nullptr;
+  S.want(dsegs, count);
+  S.want(gram, L.total);
+  S.want(out, count);
+  plan_want(S, plan);
+  MSQ_TRY(S.commit());
+  MSQ_TRY(upload(rt, dsegs, segs));
+  MSQ_TRY(plan_upload(rt, plan));
+  Operand ov;
+  ov.data = vectors;
+  msq::GemmArgs g = product_args(ov, ov, gram, nullptr, nullptr);
+  MSQ_TRY(run_gemm(rt, plan, msq::GemmType::F32, msq::Epilogue::Store, g));
+  HIP_TRY(msq::launch_zero_f64(out, count, rt->stream));
+  HIP_TRY(msq::launch_orthogonality(dsegs, count, L.max_ld, gram, out, rt->stream));
+  std::vector<double> h(count);
+  HIP_TRY(hipMemcpyAsync(h.data(), out, count * sizeof(double), hipMemcpyDeviceToHost, rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  for (uint32_t i = 0; i < count; ++i) results[i] = std::sqrt(h[i] / L.order[i]);
+  return MSQ_OK;
+}
+
+int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
+                      const double* matrices, double* roots_out, const msq_solve_options* options,
+                      uint32_t options_count, msq_solve_report* reports) {
+  MSQ_TRY(activate(rt));
+  Layout L;
+  MSQ_TRY(make_layout(count, orders, L));
+  if (!matrices || !roots_out || !options)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  if (options_count != 1 && options_count != count)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "options_count must be 1 or count");
+  auto opt = [&](uint32_t i) -> const msq_solve_options& {
+    return options[options_count == 1 ? 0 : i];
+  };
+  // Groups of matrices sharing (basis, correction) precisions.
+  std::map<std::pair<int, int>, std::vector<uint32_t>> groups;
+  std::map<int, std::vector<uint32_t>> by_corr;
+  uint32_t max_corrections = 0;
+  std::vector<msq_jacobi_options> jopts(options_count);
+  std::vector<msq::SolveLimits> limits(count);
+  for (uint32_t i = 0; i < count; ++i) {
+    const msq_solve_options& o = opt(i);
+    MSQ_TRY(validate_tier(o.basis_precision, "basis"));
+    MSQ_TRY(validate_tier(o.correction_precision, "correction"));
+    if (o.max_corrections >= msq::kSolveHistory)
+      return set_error(MSQ_ERROR_INVALID_ARGUMENT, "max_corrections must be < %u",
+                       msq::kSolveHistory);
+    groups[{o.basis_precision, o.correction_precision}].push_back(i);
+    by_corr[o.correction_precision].push_back(i);
+    max_corrections = std::max(max_corrections, o.max_corrections);
+    limits[i].residual_tolerance = o.residual_tolerance;
+    limits[i].forward_tolerance = o.forward_tolerance;
+    limits[i].min_corrections = std::min(o.min_corrections, o.max_corrections);
+    limits[i].max_corrections = o.max_corrections;
+  }
+  for (uint32_t k = 0; k < options_count; ++k) jopts[k] = options[k].jacobi;
+
+  // Host plans.
+  const std::vector<uint32_t> all = iota_members(count);
+  const std::vector<msq::Segment> segs_all = segments_of(L, all);
+  GemmPlan plan_res;
+  build_plan(L, all, true, plan_res);
+  struct Group {
+    int basis, corr;
+    std::vector<uint32_t> members;
+    std::vector<msq::Segment> segs;
+    GemmPlan plan;
+    msq::Segment* dsegs = nullptr;
+  };
+  std::vector<Group> gl;
+  for (auto& kv : groups) {
+    Group g;
+    g.basis = kv.first.first;
+    g.corr = kv.first.second;
+    g.members = kv.second;
+    g.segs = segments_of(L, g.members);
+    build_plan(L, g.members, false, g.plan);
+    gl.push_back(std::move(g));
+  }
+  struct CorrTier {
+    int tier;
+    std::vector<uint32_t> members;
+    std::vector<msq::Segment> segs;
+    msq::Segment* dsegs = nullptr;
+    TierBuffers ev, evt;
+    Operand ov, ovt;
+  };
+  std::vector<CorrTier> tiers;
+  for (auto& kv : by_corr) {
+    CorrTier t;
+    t.tier = kv.first;
+    t.members = kv.second;
+    t.segs = segments_of(L, t.members);
+    tiers.push_back(std::move(t));
+  }
+
+  // Device buffers.
+  Scratch S{rt};
+  float *V = nullptr, *Vt = nullptr, *diag = nullptr, *sroot = nullptr, *R32 = nullptr;
+  double *R = nullptr, *norms = nullptr;
+  int* active = nullptr;
+  msq::JacobiStatus* jst = nullptr;
+  msq::JacobiJob* djobs = nullptr;
+  msq::SolveState* st = nullptr;
+  msq::SolveLimits* dlim = nullptr;
+  msq::Segment* dsegs_all = nullptr;
+  CorrectionBuffers cb;
+  S.want(V, L.total);
+  S.want(Vt, L.total);
+  S.want(diag, L.vtotal);
+  S.want(sroot, L.vtotal);
+  S.want(R32, L.total);
+  S.want(R, L.total);
+  S.want(norms, count);
+  S.want(active, count);
+  S.want(jst, count);
+  S.want(djobs, count);
+  S.want(st, count);
+  S.want(dlim, count);
+  S.want(dsegs_all, count);
+  S.want(cb.t1, L.total);
+  S.want(cb.t2, L.total);
+  S.want(cb.alpha, count);
+  tier_want_all(S, cb.enc_a, L);
+  tier_want_all(S, cb.enc_b, L);
+  plan_want(S, plan_res);
+  for (Group& g : gl) {
+    S.want(g.dsegs, g.members.size());
+    plan_want(S, g.plan);
+  }
+  for (CorrTier& t : tiers) {
+    S.want(t.dsegs, t.members.size());
+    tier_want(S, t.ev, t.tier, L);
+    tier_want(S, t.evt, t.tier, L);
+  }
+  MSQ_TRY(S.commit());
+
+  MSQ_TRY(upload(rt, dsegs_all, segs_all));
+  MSQ_TRY(upload(rt, dlim, limits));
+  MSQ_TRY(plan_upload(rt, plan_res));
+  for (Group& g : gl) {
+    MSQ_TRY(upload(rt, g.dsegs, g.segs));
+    MSQ_TRY(plan_upload(rt, g.plan));
+  }
+  for (CorrTier& t : tiers) MSQ_TRY(upload(rt, t.dsegs, t.segs));
+
+  // 1. Jacobi (symmetricJacobi).
+  std::vector<msq::JacobiJob> jobs;
+  MSQ_TRY(run_jacobi(rt, L, matrices, V, diag, sroot, jopts.data(), options_count, jst, djobs, jobs));
+  HIP_TRY(msq::launch_solve_state(dsegs_all, jst, diag, dlim, st, active, count, rt->stream));
+  // 2. Reconstruction X0 = G G^T in the basis tier, X = sym(X0) in binary64.
+  for (Group& g : gl) {
+    MSQ_TRY(reconstruction_product(rt, g.basis, L, g.members, g.dsegs, g.plan, V, sroot, active,
+                                   cb));
+    HIP_TRY(msq::launch_symmetrize_to_f64(g.dsegs, static_cast<uint32_t>(g.members.size()),
+                                          L.max_ld, cb.t2, roots_out, active, rt->stream));
+  }
+  // 3. Frozen factor in each correction tier: V and V^T.
+  HIP_TRY(msq::launch_transpose(dsegs_all, count, L.max_ld, V, Vt, rt->stream));
+  for (CorrTier& t : tiers) {
+    MSQ_TRY(encode_operand(rt, t.tier, L, t.members, t.dsegs, V, t.ev, t.ov));
+    MSQ_TRY(encode_operand(rt, t.tier, L, t.members, t.dsegs, Vt, t.evt, t.ovt));
+  }
+  // 4. Residual of X0, then corrections until each matrix stops.
+  MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, R, R32, norms, active, count));
+  HIP_TRY(msq::launch_convergence(st, norms, active, count, 0, rt->stream));
+  for (uint32_t step = 1; step <= max_corrections; ++step) {
+    for (Group& g : gl) {
+      const CorrTier* t = nullptr;
+      for (const CorrTier& c : tiers)
+        if (c.tier == g.corr) t = &c;
+      MSQ_TRY(correction_products(rt, g.corr, L, g.members, g.dsegs, g.plan, t->ov, t->ovt, R32,
+                                  sroot, active, cb));
+      HIP_TRY(msq::launch_add_symmetric(g.dsegs, static_cast<uint32_t>(g.members.size()), L.max_ld,
+                                        cb.t2, roots_out, active, rt->stream));
+    }
+    MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, R, R32, norms, active, count));
+    HIP_TRY(msq::launch_convergence(st, norms, active, count, step, rt->stream));
+  }
+  // Reports.
+  std::vector<msq::SolveState> hs(count);
+  std::vector<msq::JacobiStatus> hj(count);
+  HIP_TRY(hipMemcpyAsync(hs.data(), st, count * sizeof(msq::SolveState), hipMemcpyDeviceToHost,
+                         rt->stream));
+  HIP_TRY(hipMemcpyAsync(hj.data(), jst, count * sizeof(msq::JacobiStatus), hipMemcpyDeviceToHost,
+                         rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  if (reports) {
+    for (uint32_t i = 0; i < count; ++i) {
+      msq_solve_report& r = reports[i];
+      std::memset(&r, 0, sizeof(r));
+      const msq::SolveState& s = hs[i];
+      r.order = L.order[i];
+      r.sweeps = hj[i].sweeps;
+      r.corrections = s.corrections;
+      r.status = s.status == msq::kSolveRunning ? msq::kSolveBudgetExhausted : s.status;
+      r.matrix_norm = s.matrix_norm;
+      r.min_root = s.min_root;
+      r.off_norm = hj[i].off_norm;
+      r.final_residual = s.final_residual;
+      r.final_forward_estimate = s.final_forward_estimate;
+      for (uint32_t k = 0; k < MSQ_SOLVE_HISTORY && k < msq::kSolveHistory; ++k) {
+        r.residuals[k] = k < s.steps ? s.residuals[k] : 0.0;
+        r.forward_estimates[k] = k < s.steps ? s.forward_estimates[k] : 0.0;
+      }
+      for (uint32_t k = 0; k < MSQ_JACOBI_HISTORY; ++k)
+        r.off_history[k] = k < hj[i].sweeps ? hj[i].off_history[k] : 0.0f;
+    }
+  }
+  rt->last_vectors = V;
+  rt->last_roots = sroot;
+  return MSQ_OK;
+}
+
+int msq_last_factors(msq_runtime* rt, const float** vectors, const float** roots) {
+  MSQ_TRY(activate(rt));
+  if (!rt->last_vectors) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "no solve has run");
+  if (vectors) *vectors = rt->last_vectors;
+  if (roots) *roots = rt->last_roots;
+  return MSQ_OK;
+}
+
+}  // extern "C"

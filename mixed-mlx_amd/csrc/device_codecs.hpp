@@ -1,0 +1,236 @@
+// Storage codecs for BF16, FP8 E4M3FN / E5M2, FP4 E2M1 and the MX block
+// formats, usable from host and device code.
+//
+// Semantics are those of the reference host codecs (src/storage_codecs.cpp):
+//   * BF16: round to nearest even on the bit pattern, NaN keeps a quiet bit
+//     (:16-28);
+//   * E4M3FN: NaN -> 0x7f, zero keeps its sign, |v| > 448 or inf -> 0x7e,
+//     otherwise nearest finite code with ties to the even code (:96-108);
+//   * E5M2: NaN -> 0x7e, inf -> 0x7c, finite |v| > 57344 -> 0x7b (:110-125);
+//   * FP4 E2M1: NaN and zero -> signed zero, |v| > 6 or inf -> 0x7 (:127-136);
+//   * MX: scale exponent clamp(ilogb(max finite |v|) - emax, -127, 128), stored
+//     biased by 127, elements encoded as v / 2^exponent (:148-198).
+// The reference finds the nearest code by a scan over all codes; here the same
+// result comes from integer round-half-even on the float's significand, which
+// needs no floating-point mode and is exact for subnormal inputs too.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define MSQ_HD __host__ __device__ __forceinline__
+#else
+#define MSQ_HD inline
+#endif
+
+namespace msq {
+
+MSQ_HD uint32_t f32_bits(float v) {
+  union {
+    float f;
+    uint32_t u;
+  } x;
+  x.f = v;
+  return x.u;
+}
+MSQ_HD float bits_f32(uint32_t u) {
+  union {
+    float f;
+    uint32_t u;
+  } x;
+  x.u = u;
+  return x.f;
+}
+
+// ---------------------------------------------------------------- BF16
+MSQ_HD uint16_t encode_bf16(float value) {
+  const uint32_t bits = f32_bits(value);
+  uint32_t rounded = bits >> 16;
+  const uint32_t remainder = bits & 0xffffu;
+  if (remainder > 0x8000u || (remainder == 0x8000u && (rounded & 1u) != 0)) {
+    ++rounded;
+  }
+  if ((bits & 0x7f800000u) == 0x7f800000u && (bits & 0x007fffffu) != 0) {
+    rounded |= 0x0040u;
+  }
+  return static_cast<uint16_t>(rounded);
+}
+MSQ_HD float decode_bf16(uint16_t code) {
+  return bits_f32(static_cast<uint32_t>(code) << 16);
+}
+
+// ------------------------------------------------- minifloat round to nearest
+// Rounds the finite positive value sig * 2^e2 (sig < 2^24) to the code of a
+// minifloat with `mant` mantissa bits and exponent bias `bias`, where codes
+// count up monotonically with value: code = ((E + bias - 1) << mant) + k with
+// E = max(floor(log2 value), 1 - bias) and k = round_half_even(value /
+// 2^(E - mant)).  A carry out of k walks into the next binade correctly.
+MSQ_HD uint32_t minifloat_round(uint32_t sig, int e2, int mant, int bias) {
+  int msb = 31 - __builtin_clz(sig);       // sig != 0
+  int log2v = e2 + msb;
+  int E = log2v > 1 - bias ? log2v : 1 - bias;
+  int shift = (E - mant) - e2;  // value / 2^(E - mant) = sig / 2^shift
+  uint32_t k;
+  if (shift <= 0) {
+    k = sig << (-shift);
+  } else if (shift > 25) {
+    k = 0;  // value < 2^(E - mant - 1): below half the quantum
+  } else {
+    k = sig >> shift;
+    const uint32_t rem = sig & ((1u << shift) - 1u);
+    const uint32_t half = 1u << (shift - 1);
+    if (rem > half || (rem == half && (k & 1u))) ++k;
+  }
+  return (static_cast<uint32_t>(E + bias - 1) << mant) + k;
+}
+
+// Splits |v| (finite, non-zero) into sig * 2^e2, applying an extra power of
+// two `scale_exp` exactly (value / 2^scale_exp).
+MSQ_HD void split_abs(uint32_t bits, int scale_exp, uint32_t &sig, int &e2) {
+  const uint32_t ef = (bits >> 23) & 0xffu;
+  const uint32_t mf = bits & 0x7fffffu;
+  if (ef == 0) {
+    sig = mf;
+    e2 = -149;
+  } else {
+    sig = mf | 0x800000u;
+    e2 = static_cast<int>(ef) - 150;
+  }
+  e2 -= scale_exp;
+}
+
+// E4M3FN of v / 2^scale_exp (scale_exp = 0 for the scalar codec).
+MSQ_HD uint8_t encode_e4m3(float v, int scale_exp = 0) {
+  const uint32_t bits = f32_bits(v);
+  const uint8_t sign = (bits >> 31) ? 0x80u : 0u;
+  const uint32_t a = bits & 0x7fffffffu;
+  if (a > 0x7f800000u) return sign | 0x7fu;   // NaN
+  if (a == 0) return sign;                     // +-0
+  if (a == 0x7f800000u) return sign | 0x7eu;  // inf saturates
+  uint32_t sig;
+  int e2;
+  split_abs(bits, scale_exp, sig, e2);
+  // |v| / 2^s > 448  <=>  sig * 2^e2 > 7 * 2^6
+  const int msb = 31 - __builtin_clz(sig);
+  if (e2 + msb > 8) return sign | 0x7eu;
+  if (e2 + msb == 8) {
+    // value in [256, 512): compare with 448 = 0b111 << 6 exactly
+    const int sh = msb - 2;  // keep the top three bits
+    const uint32_t top = sig >> sh;
+    const bool above = top > 7u || (top == 7u && (sig & ((1u << sh) - 1u)));
+    if (above) return sign | 0x7eu;
+  }
+  return sign | static_cast<uint8_t>(minifloat_round(sig, e2, 3, 7));
+}
+
+// E5M2 of v / 2^scale_exp.
+MSQ_HD uint8_t encode_e5m2(float v, int scale_exp = 0) {
+  const uint32_t bits = f32_bits(v);
+  const uint8_t sign = (bits >> 31) ? 0x80u : 0u;
+  const uint32_t a = bits & 0x7fffffffu;
+  if (a > 0x7f800000u) return sign | 0x7eu;   // NaN
+  if (a == 0) return sign;
+  if (a == 0x7f800000u) return sign | 0x7cu;  // inf stays inf
+  uint32_t sig;
+  int e2;
+  split_abs(bits, scale_exp, sig, e2);
+  // finite |v| / 2^s > 57344 = 0b111 << 13 saturates to 0x7b
+  const int msb = 31 - __builtin_clz(sig);
+  if (e2 + msb > 15) return sign | 0x7bu;
+  if (e2 + msb == 15) {
+    const int sh = msb - 2;
+    const uint32_t top = sig >> sh;
+    const bool above = top > 7u || (top == 7u && (sig & ((1u << sh) - 1u)));
+    if (above) return sign | 0x7bu;
+  }
+  return sign | static_cast<uint8_t>(minifloat_round(sig, e2, 2, 15));
+}
+
+// FP4 E2M1 of v / 2^scale_exp (a nibble in the low four bits).
+MSQ_HD uint8_t encode_e2m1(float v, int scale_exp = 0) {
+  const uint32_t bits = f32_bits(v);
+  const uint8_t sign = (bits >> 31) ? 0x8u : 0u;
+  const uint32_t a = bits & 0x7fffffffu;
+  if (a > 0x7f800000u || a == 0) return sign;  // NaN and zero
+  if (a == 0x7f800000u) return sign | 0x7u;
+  uint32_t sig;
+  int e2;
+  split_abs(bits, scale_exp, sig, e2);
+  // > 6 = 0b11 << 1 saturates
+  const int msb = 31 - __builtin_clz(sig);
+  if (e2 + msb > 2) return sign | 0x7u;
+  if (e2 + msb == 2) {
+    const int sh = msb - 1;  // keep the top two bits
+    const uint32_t top = sig >> sh;
+    const bool above = top > 3u || (top == 3u && (sig & ((1u << sh) - 1u)));
+    if (above) return sign | 0x7u;
+  }
+  return sign | static_cast<uint8_t>(minifloat_round(sig, e2, 1, 1));
+}
+
+// ------------------------------------------------------------------ decoders
+MSQ_HD float decode_e4m3(uint8_t code) {
+  const uint32_t sign = static_cast<uint32_t>(code & 0x80u) << 24;
+  const uint32_t e = (code >> 3) & 0xfu;
+  const uint32_t f = code & 0x7u;
+  if (e == 0xfu && f == 0x7u) return bits_f32(sign | 0x7fc00000u);
+  if (e == 0) {
+    // f * 2^-9, exact
+    return bits_f32(sign | f32_bits(static_cast<float>(f) * 0.001953125f));
+  }
+  return bits_f32(sign | ((e + 120u) << 23) | (f << 20));
+}
+MSQ_HD float decode_e5m2(uint8_t code) {
+  const uint32_t sign = static_cast<uint32_t>(code & 0x80u) << 24;
+  const uint32_t e = (code >> 2) & 0x1fu;
+  const uint32_t f = code & 0x3u;
+  if (e == 0x1fu) {
+    return bits_f32(sign | (f == 0 ? 0x7f800000u : 0x7fc00000u));
+  }
+  if (e == 0) {
+    return bits_f32(sign | f32_bits(static_cast<float>(f) * 1.52587890625e-05f));
+  }
+  return bits_f32(sign | ((e + 112u) << 23) | (f << 21));
+}
+MSQ_HD float decode_e2m1(uint8_t nibble) {
+  const uint32_t sign = static_cast<uint32_t>(nibble & 0x8u) << 28;
+  const uint32_t e = (nibble >> 1) & 0x3u;
+  const uint32_t f = nibble & 0x1u;
+  if (e == 0) return bits_f32(sign | (f ? 0x3f000000u : 0u));  // 0 or 0.5
+  return bits_f32(sign | ((e + 126u) << 23) | (f << 22));
+}
+
+// x * 2^k exactly as std::ldexp does for these few-bit values: the product is
+// representable unless it overflows (then inf); two steps keep 2^k finite.
+MSQ_HD float ldexp_exact(float x, int k) {
+  if (k > 127) {
+    x *= bits_f32(0x7f000000u);  // 2^127
+    k -= 127;
+  } else if (k < -126) {
+    x *= bits_f32(0x00800000u);  // 2^-126
+    k += 126;
+  }
+  return x * bits_f32(static_cast<uint32_t>(k + 127) << 23);
+}
+
+// MX scale exponent from the largest finite magnitude's bit pattern (bit
+// patterns of non-negative floats order like their values).
+MSQ_HD int mx_scale_exponent(uint32_t max_abs_bits, int emax) {
+  if (max_abs_bits == 0) return 0;
+  int ilogb;
+  const uint32_t ef = max_abs_bits >> 23;
+  if (ef == 0) {
+    ilogb = (31 - __builtin_clz(max_abs_bits & 0x7fffffu)) - 149;
+  } else {
+    ilogb = static_cast<int>(ef) - 127;
+  }
+  int e = ilogb - emax;
+  return e < -127 ? -127 : (e > 128 ? 128 : e);
+}
+// |v| bits if finite, else 0 (non-finite values do not enter the block max).
+MSQ_HD uint32_t finite_abs_bits(float v) {
+  const uint32_t a = f32_bits(v) & 0x7fffffffu;
+  return a < 0x7f800000u ? a : 0u;
+}
+
+}  // namespace msq

@@ -1,0 +1,207 @@
+// Kernel launch interface shared by the runtime and the kernel translation
+// units.  Nothing here is exported from the library; the C ABI is
+// include/msq_hip.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace msq {
+
+// Element formats of the codec kernels, in the order of the reference's
+// HostStorageCodec (src/precision.hpp:24-33).
+enum class Format : int {
+  Fp32 = 0,
+  Bf16 = 1,
+  Fp8E4M3Fn = 2,
+  Fp8E5M2 = 3,
+  Mxfp8E4M3Fn = 4,
+  Mxfp8E5M2 = 5,
+  Fp4E2M1 = 6,
+  Mxfp4E2M1 = 7,
+};
+
+hipError_t launch_encode(Format fmt, const float* src, void* values, uint8_t* scales,
+                         size_t n, hipStream_t stream);
+hipError_t launch_decode(Format fmt, const void* values, const uint8_t* scales,
+                         float* dst, size_t n, hipStream_t stream);
+hipError_t launch_fp8_amax_sigma(const float* src, size_t n, uint32_t* amax_bits,
+                                 float* sigma, hipStream_t stream);
+hipError_t launch_encode_fp8_scaled(const float* src, uint8_t* values,
+                                    const float* sigma, size_t n, hipStream_t stream);
+hipError_t launch_decode_fp8_scaled(const uint8_t* values, const float* sigma,
+                                    float* dst, size_t n, hipStream_t stream);
+
+// ------------------------------------------------------------------ Jacobi
+constexpr uint32_t kBatchedJacobiMaxOrder = 256;
+constexpr uint32_t kJacobiHistory = 32;
+constexpr uint32_t kJacobiInvalidSpectrum = 1u;
+
+// One matrix of a batched Jacobi launch.  Offsets are in elements.
+struct JacobiJob {
+  uint64_t a_offset;       // binary64 input, column-major, pitch ld
+  uint64_t v_offset;       // float eigenvectors, column-major, pitch ld
+  uint64_t s_offset;       // float diagonal / roots, ld entries
+  uint32_t order;
+  uint32_t ld;
+  uint32_t max_sweeps;     // cap when converging by tolerance
+  uint32_t fixed_sweeps;   // >0: run exactly this many sweeps, no test
+  float off_tol;           // stop when off(A) <= off_tol * ||A||_F ...
+  float off_tol_lambda;    // ... or <= off_tol_lambda * min diag(A) (0: off)
+  float stagnation;        // ... or off > stagnation * previous off and
+  float floor_tol;         //     off <= floor_tol * ||A||_F (FP32 floor)
+  uint32_t status_index;   // slot of this matrix in the status array
+  uint32_t reserved;
+};
+
+struct JacobiStatus {
+  uint32_t sweeps;
+  uint32_t flags;
+  double off_norm;
+  double matrix_norm;
+  float min_diagonal;
+  float off_history[kJacobiHistory];  // off(A)/||A||_F after each sweep
+};
+
+size_t batched_jacobi_lds_bytes(uint32_t order, bool v_in_lds, int threads);
+hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t max_order,
+                                 bool v_in_lds, int threads, const double* a64,
+                                 float* vectors, float* diagonal, float* roots,
+                                 JacobiStatus* status, hipStream_t stream);
+
+hipError_t launch_global_jacobi_init(const double* a64, float* A, float* V, uint32_t n,
+                                     uint32_t ld, hipStream_t stream);
+hipError_t launch_global_jacobi_round(float* A, float* V, uint32_t n, uint32_t ld,
+                                      uint32_t round, int* slot_p, int* slot_q,
+                                      float2* rot, hipStream_t stream);
+hipError_t launch_global_offnorm(const float* A, uint32_t n, uint32_t ld, double* off2,
+                                 float* dmin, int partials, hipStream_t stream);
+hipError_t launch_global_jacobi_finish(const float* A, uint32_t n, uint32_t ld,
+                                       float* diagonal, float* roots, hipStream_t stream);
+
+// -------------------------------------------------------------------- GEMM
+// Operand element types of the batched square GEMMs.
+enum class GemmType : int { F64 = 0, F32 = 1, Bf16 = 2, Fp8 = 3, Mxfp4 = 4 };
+
+// Epilogues.
+enum class Epilogue : int {
+  Store = 0,     // C (or C^T) as float (double for F64), times alpha
+  Hadamard = 1,  // C o H, H_ij = 1 / (s_i + s_j) inside the order, 0 outside
+  Residual = 2,  // F64 only: R = A - C (double) and R as float, ||R||_F^2
+};
+
+// One matrix of a batched GEMM launch: C = A B with both operands stored
+// K-contiguous: element (m, k) of A at a[k + m ld] (that is, A^T column-major)
+// and element (k, n) of B at b[k + n ld].  Offsets are in elements (bytes for
+// the packed FP4 codes are offset / 2; MX scale bytes are offset / 32).
+struct GemmJob {
+  uint64_t a_offset;
+  uint64_t b_offset;
+  uint64_t c_offset;
+  uint64_t aux_offset;   // residual: the binary64 matrix; Hadamard: roots (ld)
+  uint32_t order;
+  uint32_t ld;           // multiple of 32 (of 128 for MXFP4)
+  uint32_t reserved;
+  uint32_t status_index; // index into active[] / norms[]
+};
+
+struct GemmArgs {
+  const GemmJob* jobs;
+  const uint32_t* tile_job;  // job of each tile
+  const uint32_t* tile_pos;  // (tile_row << 16) | tile_col
+  const void* a;
+  const void* b;
+  const uint8_t* a_scales;   // MXFP4 block scales (E8M0) of A and B
+  const uint8_t* b_scales;
+  const float* alpha;        // per status_index multiplier (FP8 dequant), or null
+  const int* active;         // per status_index flag, or null (all active)
+  void* c;                   // float (double for F64)
+  const void* aux;           // see GemmJob::aux_offset
+  float* c_float;            // Residual: R as float
+  double* norms;             // Residual: ||R||^2 accumulators per status_index
+  int transpose_c;           // store C^T (row-major C) instead of C
+};
+
+hipError_t launch_gemm(GemmType type, Epilogue epi, int tile, const GemmArgs& args,
+                       uint32_t tiles, hipStream_t stream);
+
+// ------------------------------------------------------ elementwise helpers
+// All per-matrix arrays of a batch share one segment descriptor: an ld x ld
+// column-major tile at `offset` (zero outside the order) and ld-vectors at
+// `vec_offset`.
+struct Segment {
+  uint64_t offset;
+  uint64_t vec_offset;
+  uint32_t order;
+  uint32_t ld;
+  uint32_t index;     // matrix index: slot in per-matrix arrays (active, sigma, ...)
+  uint32_t reserved;
+};
+
+hipError_t launch_symmetrize_to_f64(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                    const float* c, double* x, const int* active,
+                                    hipStream_t stream);
+hipError_t launch_add_symmetric(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                const float* e, double* x, const int* active,
+                                hipStream_t stream);
+hipError_t launch_scale_rows_transpose(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                       const float* v, const float* roots, int use_sqrt,
+                                       float* out, hipStream_t stream);
+hipError_t launch_transpose(const Segment* segs, uint32_t count, uint32_t max_ld,
+                            const float* v, float* out, hipStream_t stream);
+hipError_t launch_zero_f64(double* p, size_t n, hipStream_t stream);
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t value, hipStream_t stream);
+hipError_t launch_sumsq_f64(const Segment* segs, uint32_t count, uint32_t max_ld, const double* a,
+                            double* out, hipStream_t stream);
+hipError_t launch_orthogonality(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                const float* gram, double* out, hipStream_t stream);
+hipError_t launch_segment_fp8(const Segment* segs, uint32_t count, uint32_t max_ld,
+                              const float* src, uint32_t* amax, float* sigma, uint8_t* dst,
+                              hipStream_t stream);
+hipError_t launch_alpha(const float* sa, const float* sb, float* alpha, uint32_t count,
+                        hipStream_t stream);
+
+// ------------------------------------------------------------ solve state
+constexpr uint32_t kSolveHistory = 20;
+enum : uint32_t {
+  kSolveRunning = 0,
+  kSolveConverged = 1,
+  kSolveBudgetExhausted = 2,
+  kSolveDiverged = 3,
+  kSolveNonfinite = 4,
+  kSolveInvalidSpectrum = 5,
+};
+
+struct SolveLimits {
+  double residual_tolerance;  // stop when ||A - X^2||_F / ||A||_F <= this ...
+  double forward_tolerance;   // ... or the forward-error estimate is <= this
+  uint32_t min_corrections;
+  uint32_t max_corrections;
+};
+
+struct SolveState {
+  double matrix_norm;            // ||A||_F
+  double root_norm;              // sqrt(trace A) = ||A^(1/2)||_F
+  double min_root;               // sqrt of the smallest Jacobi diagonal entry
+  double residual_tolerance;
+  double forward_tolerance;
+  uint32_t min_corrections;
+  uint32_t max_corrections;
+  uint32_t steps;                // residuals recorded (initial root + corrections)
+  uint32_t corrections;          // corrections applied
+  uint32_t status;
+  uint32_t reserved;
+  double final_residual;
+  double final_forward_estimate;
+  double residuals[kSolveHistory];          // after X0, then after each correction
+  double forward_estimates[kSolveHistory];  // ||R||_F / (2 s_min ||X||_F)
+};
+
+hipError_t launch_solve_state(const Segment* segs, const JacobiStatus* js,
+                              const float* diagonal, const SolveLimits* limits, SolveState* st,
+                              int* active, uint32_t count, hipStream_t stream);
+hipError_t launch_convergence(SolveState* st, const double* norms, int* active, uint32_t count,
+                              uint32_t step, hipStream_t stream);
+
+}  // namespace msq

@@ -1,0 +1,295 @@
+// Batched square matrix products on the matrix cores, for every matrix of an
+// uneven batch in one launch.
+//
+// Reference: matmul_tiled_{f32,bf16} / matmul_tiled_scaled_* and the low-factor
+// products (metal_mixed_sqrt.metal:162-366, :1493-1600), which are 16x16
+// threadgroup tiles of scalar FMAs, and the float-float residual product
+// working_residual (:1927-1942), an O(n) loop per entry.  Here every product
+// is an MFMA tile loop:
+//
+//   type    instruction                              k per MFMA   operand
+//   F64     v_mfma_f64_16x16x4_f64                    4           double
+//   F32     v_mfma_f32_16x16x4_f32                    4           float
+//   Bf16    v_mfma_f32_16x16x32_bf16                  32          8 x bf16
+//   Fp8     v_mfma_f32_16x16x32_fp8_fp8               32          8 x E4M3FN
+//   Mxfp4   v_mfma_scale_f32_16x16x128_f8f6f4 (fp4)   128         32 x E2M1 + E8M0
+//
+// Both operands are stored K-contiguous (A as A^T column-major, B column-
+// major), so a 16-byte global load feeds LDS directly and every lane's
+// fragment is one contiguous LDS read.  An MXFP4 lane fragment is exactly one
+// 32-element block along K, so its E8M0 scale is one byte per lane.
+// A workgroup computes a (32 WM) x (32 WN) tile, each wave a 32 x 32 block as
+// 2 x 2 MFMA tiles; tiles of all matrices form one flat grid (the host's tile
+// list), so small and large orders share a launch.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace msq {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+template <GemmType T>
+struct Traits;
+template <>
+struct Traits<GemmType::F64> {
+  static constexpr int KS = 16, BITS = 64, MPS = 4;
+  typedef f64x4 Acc;
+  typedef double Out;
+};
+template <>
+struct Traits<GemmType::F32> {
+  static constexpr int KS = 16, BITS = 32, MPS = 4;
+  typedef f32x4 Acc;
+  typedef float Out;
+};
+template <>
+struct Traits<GemmType::Bf16> {
+  static constexpr int KS = 32, BITS = 16, MPS = 1;
+  typedef f32x4 Acc;
+  typedef float Out;
+};
+template <>
+struct Traits<GemmType::Fp8> {
+  static constexpr int KS = 32, BITS = 8, MPS = 1;
+  typedef f32x4 Acc;
+  typedef float Out;
+};
+template <>
+struct Traits<GemmType::Mxfp4> {
+  static constexpr int KS = 128, BITS = 4, MPS = 1;
+  typedef f32x4 Acc;
+  typedef float Out;
+};
+
+// One MFMA sub-step on fragments read from LDS row pointers.
+template <GemmType T>
+__device__ __forceinline__ void mma_step(typename Traits<T>::Acc& acc, const unsigned char* arow,
+                                         const unsigned char* brow, int sub, int quad,
+                                         int sa, int sb) {
+  if constexpr (T == GemmType::F64) {
+    const double a = reinterpret_cast<const double*>(arow)[4 * sub + quad];
+    const double b = reinterpret_cast<const double*>(brow)[4 * sub + quad];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  } else if constexpr (T == GemmType::F32) {
+    const float a = reinterpret_cast<const float*>(arow)[4 * sub + quad];
+    const float b = reinterpret_cast<const float*>(brow)[4 * sub + quad];
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+  } else if constexpr (T == GemmType::Bf16) {
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow + 16 * quad);
+    const bf16x8 b = *reinterpret_cast<const bf16x8*>(brow + 16 * quad);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+  } else if constexpr (T == GemmType::Fp8) {
+    const long a = *reinterpret_cast<const long*>(arow + 8 * quad);
+    const long b = *reinterpret_cast<const long*>(brow + 8 * quad);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, acc, 0, 0, 0);
+  } else {
+    const uint4 a4 = *reinterpret_cast<const uint4*>(arow + 16 * quad);
+    const uint4 b4 = *reinterpret_cast<const uint4*>(brow + 16 * quad);
+    i32x8 a = {static_cast<int>(a4.x), static_cast<int>(a4.y), static_cast<int>(a4.z),
+               static_cast<int>(a4.w), 0, 0, 0, 0};
+    i32x8 b = {static_cast<int>(b4.x), static_cast<int>(b4.y), static_cast<int>(b4.z),
+               static_cast<int>(b4.w), 0, 0, 0, 0};
+    acc = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, acc, 4, 4, 0, sa, 0, sb);
+  }
+}
+
+template <GemmType T, int WM, int WN, Epilogue EPI>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
+  typedef Traits<T> Tr;
+  constexpr int NT = 64 * WM * WN;
+  constexpr int BM = 32 * WM, BN = 32 * WN;
+  constexpr int RB = Tr::KS * Tr::BITS / 8;  // bytes of one row per stage
+  constexpr int RP = RB + 16;                // padded LDS row pitch
+  constexpr int CHUNKS = RB / 16;
+  constexpr bool MX = T == GemmType::Mxfp4;
+
+  const uint32_t tile = blockIdx.x;
+  const GemmJob job = args.jobs[args.tile_job[tile]];
+  if (args.active && !args.active[job.status_index]) return;
+  const int tm = static_cast<int>(args.tile_pos[tile] >> 16) * BM;
+  const int tn = static_cast<int>(args.tile_pos[tile] & 0xffffu) * BN;
+  const int ld = static_cast<int>(job.ld);
+  const int order = static_cast<int>(job.order);
+
+  __shared__ __align__(16) unsigned char As[BM * RP];
+  __shared__ __align__(16) unsigned char Bs[BN * RP];
+  __shared__ uint8_t Ss[MX ? (BM + BN) * 4 : 1];
+
+  const unsigned char* abase;
+  const unsigned char* bbase;
+  if constexpr (MX) {
+    abase = static_cast<const unsigned char*>(args.a) + job.a_offset / 2;
+    bbase = static_cast<const unsigned char*>(args.b) + job.b_offset / 2;
+  } else {
+    abase = static_cast<const unsigned char*>(args.a) + job.a_offset * (Tr::BITS / 8);
+    bbase = static_cast<const unsigned char*>(args.b) + job.b_offset * (Tr::BITS / 8);
+  }
+  const uint8_t* asc = MX ? args.a_scales + job.a_offset / 32 : nullptr;
+  const uint8_t* bsc = MX ? args.b_scales + job.b_offset / 32 : nullptr;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave % WM) * 32;
+  const int wn = (wave / WM) * 32;
+  const int quad = lane >> 4;
+  const int row16 = lane & 15;
+
+  typename Tr::Acc acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[i][j][r] = 0;
+
+  const int kend = order < ld ? ((order + 31) & ~31) : ld;
+  const size_t row_bytes = MX ? static_cast<size_t>(ld) / 2
+                              : static_cast<size_t>(ld) * (Tr::BITS / 8);
+  for (int kk = 0; kk < kend; kk += Tr::KS) {
+    // Stage A (BM rows) and B (BN rows) of this K step.
+    for (int c = tid; c < (BM + BN) * CHUNKS; c += NT) {
+      const int r = c / CHUNKS;
+      const int ch = c % CHUNKS;
+      const bool is_a = r < BM;
+      const int rr = is_a ? r : r - BM;
+      const int g = (is_a ? tm : tn) + rr;
+      const int k0 = kk + ch * (128 / Tr::BITS);  // first element of the chunk
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (g < ld && k0 < kend) {
+        const unsigned char* src = (is_a ? abase : bbase) + static_cast<size_t>(g) * row_bytes +
+                                   (MX ? static_cast<size_t>(k0) / 2
+                                       : static_cast<size_t>(k0) * (Tr::BITS / 8));
+        v = *reinterpret_cast<const uint4*>(src);
+      }
+      *reinterpret_cast<uint4*>((is_a ? As : Bs) + rr * RP + ch * 16) = v;
+    }
+    if constexpr (MX) {
+      for (int c = tid; c < (BM + BN) * 4; c += NT) {
+        const int r = c / 4;
+        const int blk = c % 4;
+        const bool is_a = r < BM;
+        const int rr = is_a ? r : r - BM;
+        const int g = (is_a ? tm : tn) + rr;
+        const int k0 = kk + 32 * blk;
+        uint8_t s = 127;
+        if (g < ld && k0 < kend)
+          s = (is_a ? asc : bsc)[(static_cast<size_t>(g) * ld + k0) / 32];
+        Ss[r * 4 + blk] = s;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int ar = wm + 16 * i + row16;
+      const unsigned char* arow = As + ar * RP;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int br = wn + 16 * j + row16;
+        const unsigned char* brow = Bs + br * RP;
+        int sa = 0, sb = 0;
+        if constexpr (MX) {
+          sa = Ss[ar * 4 + quad];
+          sb = Ss[(BM + br) * 4 + quad];
+        }
+#pragma unroll
+        for (int s = 0; s < Tr::MPS; ++s) mma_step<T>(acc[i][j], arow, brow, s, quad, sa, sb);
+      }
+    }
+    __syncthreads();
+  }
+
+  // Epilogue.  Lane holds C[row = 4 quad + r][col = row16] of each 16x16 tile.
+  const float alpha = args.alpha ? args.alpha[job.status_index] : 1.0f;
+  typedef typename Tr::Out Out;
+  Out* C = static_cast<Out*>(args.c) + job.c_offset;
+  double norm2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = tn + wn + 16 * j + row16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = tm + wm + 16 * i + 4 * quad + r;
+        if (row >= ld || col >= ld) continue;
+        if constexpr (EPI == Epilogue::Residual) {
+          const double* A = static_cast<const double*>(args.aux) + job.aux_offset;
+          const size_t idx = static_cast<size_t>(row) + static_cast<size_t>(col) * ld;
+          const double res = A[idx] - static_cast<double>(acc[i][j][r]);
+          C[idx] = static_cast<Out>(res);
+          args.c_float[job.c_offset + idx] = static_cast<float>(res);
+          if (row < order && col < order) norm2 += res * res;
+        } else {
+          Out v = static_cast<Out>(acc[i][j][r]);
+          if constexpr (T != GemmType::F64) v *= alpha;
+          if constexpr (EPI == Epilogue::Hadamard) {
+            const float* s = static_cast<const float*>(args.aux) + job.aux_offset;
+            const bool in = row < order && col < order;
+            v = in ? v * (1.0f / (s[row] + s[col])) : Out(0);
+          }
+          const size_t idx = args.transpose_c
+                                 ? static_cast<size_t>(col) + static_cast<size_t>(row) * ld
+                                 : static_cast<size_t>(row) + static_cast<size_t>(col) * ld;
+          C[idx] = v;
+        }
+      }
+    }
+  }
+  if constexpr (EPI == Epilogue::Residual) {
+    for (int off = 32; off > 0; off >>= 1) norm2 += __shfl_xor(norm2, off);
+    if (lane == 0) atomicAdd(args.norms + job.status_index, norm2);
+  }
+}
+
+template <GemmType T, Epilogue E>
+hipError_t launch_typed(int tile, const GemmArgs& args, uint32_t tiles, hipStream_t stream) {
+  if (tile == 32) {
+    gemm_kernel<T, 1, 1, E><<<tiles, 64, 0, stream>>>(args);
+  } else if (tile == 64) {
+    gemm_kernel<T, 2, 2, E><<<tiles, 256, 0, stream>>>(args);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_gemm(GemmType type, Epilogue epi, int tile, const GemmArgs& args,
+                       uint32_t tiles, hipStream_t stream) {
+  if (tiles == 0) return hipSuccess;
+  if (epi == Epilogue::Residual) {
+    if (type != GemmType::F64) return hipErrorInvalidValue;
+    return launch_typed<GemmType::F64, Epilogue::Residual>(tile, args, tiles, stream);
+  }
+  switch (type) {
+    case GemmType::F64:
+      if (epi != Epilogue::Store) return hipErrorInvalidValue;
+      return launch_typed<GemmType::F64, Epilogue::Store>(tile, args, tiles, stream);
+    case GemmType::F32:
+      return epi == Epilogue::Store
+                 ? launch_typed<GemmType::F32, Epilogue::Store>(tile, args, tiles, stream)
+                 : launch_typed<GemmType::F32, Epilogue::Hadamard>(tile, args, tiles, stream);
+    case GemmType::Bf16:
+      return epi == Epilogue::Store
+                 ? launch_typed<GemmType::Bf16, Epilogue::Store>(tile, args, tiles, stream)
+                 : launch_typed<GemmType::Bf16, Epilogue::Hadamard>(tile, args, tiles, stream);
+    case GemmType::Fp8:
+      return epi == Epilogue::Store
+                 ? launch_typed<GemmType::Fp8, Epilogue::Store>(tile, args, tiles, stream)
+                 : launch_typed<GemmType::Fp8, Epilogue::Hadamard>(tile, args, tiles, stream);
+    case GemmType::Mxfp4:
+      return epi == Epilogue::Store
+                 ? launch_typed<GemmType::Mxfp4, Epilogue::Store>(tile, args, tiles, stream)
+                 : launch_typed<GemmType::Mxfp4, Epilogue::Hadamard>(tile, args, tiles, stream);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace msq

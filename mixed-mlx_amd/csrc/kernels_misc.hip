@@ -1,0 +1,333 @@
+// Elementwise and reduction kernels of the solve, batched over uneven
+// segments (grid.y = segment).
+//
+// Reference counterparts: working_from_symmetric_f32 and
+// working_add_symmetric_correction (metal_mixed_sqrt.metal:1911-1925,
+// :1955-1970; float-float there, binary64 here), scale_columns_f32 (:21-37),
+// transpose_square_f32 (:1413-1421), and the host reduction of
+// measureF32Orthogonality (src/metal_runtime.cpp:765-775), done on the device.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "device_codecs.hpp"
+
+namespace msq {
+namespace {
+
+constexpr int kT = 256;
+
+// Grid-stride over the ld x ld tile of segment blockIdx.y.
+#define SEGMENT_LOOP(seg, e)                                                        \
+  const Segment seg = segs[blockIdx.y];                                             \
+  const size_t seg##_n = static_cast<size_t>(seg.ld) * seg.ld;                      \
+  for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;       \
+       e < seg##_n; e += static_cast<size_t>(gridDim.x) * blockDim.x)
+
+__global__ void symmetrize_to_f64_kernel(const Segment* __restrict__ segs,
+                                         const float* __restrict__ c,
+                                         double* __restrict__ x,
+                                         const int* __restrict__ active) {
+  if (active && !active[segs[blockIdx.y].index]) return;
+  SEGMENT_LOOP(sg, e) {
+    const int i = static_cast<int>(e % sg.ld);
+    const int j = static_cast<int>(e / sg.ld);
+    const double a = c[sg.offset + e];
+    const double b = c[sg.offset + j + static_cast<size_t>(i) * sg.ld];
+    x[sg.offset + e] = 0.5 * (a + b);
+  }
+}
+
+__global__ void add_symmetric_kernel(const Segment* __restrict__ segs,
+                                     const float* __restrict__ ecorr,
+                                     double* __restrict__ x,
+                                     const int* __restrict__ active) {
+  if (active && !active[segs[blockIdx.y].index]) return;
+  SEGMENT_LOOP(sg, e) {
+    const int i = static_cast<int>(e % sg.ld);
+    const int j = static_cast<int>(e / sg.ld);
+    const double a = ecorr[sg.offset + e];
+    const double b = ecorr[sg.offset + j + static_cast<size_t>(i) * sg.ld];
+    x[sg.offset + e] += 0.5 * (a + b);
+  }
+}
+
+// out[k + m ld] = V[m][k] * f(s_k): the K-contiguous storage of G = V diag(f(s)),
+// f = sqrt (use_sqrt) or identity.  Entries outside the order are zero.
+__global__ void scale_rows_transpose_kernel(const Segment* __restrict__ segs,
+                                            const float* __restrict__ v,
+                                            const float* __restrict__ roots, int use_sqrt,
+                                            float* __restrict__ out) {
+  SEGMENT_LOOP(sg, e) {
+    const int k = static_cast<int>(e % sg.ld);
+    const int m = static_cast<int>(e / sg.ld);
+    float val = 0.0f;
+    if (k < static_cast<int>(sg.order) && m < static_cast<int>(sg.order)) {
+      const float s = roots[sg.vec_offset + k];
+      val = v[sg.offset + m + static_cast<size_t>(k) * sg.ld] * (use_sqrt ? sqrtf(s) : s);
+    }
+    out[sg.offset + e] = val;
+  }
+}
+
+__global__ void transpose_kernel(const Segment* __restrict__ segs, const float* __restrict__ v,
+                                 float* __restrict__ out) {
+  SEGMENT_LOOP(sg, e) {
+    const int i = static_cast<int>(e % sg.ld);
+    const int j = static_cast<int>(e / sg.ld);
+    out[sg.offset + e] = v[sg.offset + j + static_cast<size_t>(i) * sg.ld];
+  }
+}
+
+__global__ void zero_f64_kernel(double* p, size_t n) {
+  for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+       e += static_cast<size_t>(gridDim.x) * blockDim.x)
+    p[e] = 0.0;
+}
+
+// sum (G_ij - delta_ij)^2 over the order x order block, per segment.
+__global__ void orthogonality_kernel(const Segment* __restrict__ segs,
+                                     const float* __restrict__ gram,
+                                     double* __restrict__ out) {
+  double s = 0.0;
+  SEGMENT_LOOP(sg, e) {
+    const int i = static_cast<int>(e % sg.ld);
+    const int j = static_cast<int>(e / sg.ld);
+    if (i < static_cast<int>(sg.order) && j < static_cast<int>(sg.order)) {
+      const double d = static_cast<double>(gram[sg.offset + e]) - (i == j ? 1.0 : 0.0);
+      s += d * d;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out + segs[blockIdx.y].index, s);
+}
+
+// Segmented amax (finite |v| bit patterns) for per-matrix FP8 scales.
+__global__ void segment_amax_kernel(const Segment* __restrict__ segs,
+                                    const float* __restrict__ src,
+                                    uint32_t* __restrict__ amax) {
+  uint32_t m = 0;
+  SEGMENT_LOOP(sg, e) {
+    const uint32_t a = __float_as_uint(src[sg.offset + e]) & 0x7fffffffu;
+    if (a < 0x7f800000u) m = max(m, a);
+  }
+  for (int off = 32; off > 0; off >>= 1)
+    m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), off)));
+  if ((threadIdx.x & 63) == 0) atomicMax(amax + segs[blockIdx.y].index, m);
+}
+
+__global__ void segment_sigma_kernel(const uint32_t* __restrict__ amax, float* __restrict__ sigma,
+                                     uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const float a = __uint_as_float(amax[i]);
+  sigma[i] = a == 0.0f ? 1.0f : static_cast<float>(448.0 / static_cast<double>(a));
+}
+
+// Encodes src * sigma[segment] to E4M3FN over each segment.
+__global__ void segment_encode_fp8_kernel(const Segment* __restrict__ segs,
+                                          const float* __restrict__ src,
+                                          const float* __restrict__ sigma,
+                                          uint8_t* __restrict__ dst) {
+  const float s = sigma[segs[blockIdx.y].index];
+  SEGMENT_LOOP(sg, e) { dst[sg.offset + e] = encode_e4m3(src[sg.offset + e] * s); }
+}
+
+__global__ void alpha_kernel(const float* __restrict__ sa, const float* __restrict__ sb,
+                             float* __restrict__ alpha, uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  alpha[i] = static_cast<float>(1.0 / (static_cast<double>(sa[i]) * static_cast<double>(sb[i])));
+}
+
+// Residual bookkeeping after R = A - X^2: relative residual, forward-error
+// estimate ||R||_F / (2 s_min ||X||_F) with ||X||_F^2 = trace(A), the next
+// iteration's active flag.
+__global__ void convergence_kernel(SolveState* __restrict__ st, const double* __restrict__ norms,
+                                   int* __restrict__ active, uint32_t count, uint32_t step) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  SolveState& s = st[i];
+  if (!active[i]) return;
+  const double rel = sqrt(norms[i]) / s.matrix_norm;
+  const double fwd = sqrt(norms[i]) / (2.0 * s.min_root * s.root_norm);
+  if (step < kSolveHistory) {
+    s.residuals[step] = rel;
+    s.forward_estimates[step] = fwd;
+  }
+  s.steps = step + 1;
+  s.final_residual = rel;
+  s.final_forward_estimate = fwd;
+  const bool finite = isfinite(rel);
+  const bool met = rel <= s.residual_tolerance || fwd <= s.forward_tolerance;
+  const uint32_t done_corrections = step;  // step 0 is the initial root
+  int next = 1;
+  if (!finite) {
+    s.status = kSolveNonfinite;
+    next = 0;
+  } else if (met && done_corrections >= s.min_corrections) {
+    s.status = kSolveConverged;
+    next = 0;
+  } else if (done_corrections >= s.max_corrections) {
+    s.status = met ? kSolveConverged : kSolveBudgetExhausted;
+    next = 0;
+  }
+  // Diverged: the residual grew well past the initial root's.
+  if (next && step >= 1 && rel > 4.0 * s.residuals[0]) {
+    s.status = kSolveDiverged;
+    next = 0;
+  }
+  s.corrections = done_corrections;
+  active[i] = next;
+}
+
+// Per-matrix solve state from the Jacobi results: ||A||_F, sqrt(trace A) =
+// ||A^(1/2)||_F, the smallest root; tolerances and budgets from the options.
+__global__ void solve_state_kernel(const Segment* __restrict__ segs,
+                                   const JacobiStatus* __restrict__ js,
+                                   const float* __restrict__ diagonal,
+                                   const SolveLimits* __restrict__ limits,
+                                   SolveState* __restrict__ st, int* __restrict__ active,
+                                   uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const Segment sg = segs[i];
+  double trace = 0.0;
+  float dmin = INFINITY;
+  for (uint32_t k = 0; k < sg.order; ++k) {
+    const float d = diagonal[sg.vec_offset + k];
+    trace += d;
+    dmin = fminf(dmin, d);
+  }
+  SolveState s = {};
+  s.matrix_norm = js[i].matrix_norm;
+  s.root_norm = sqrt(trace > 0.0 ? trace : 0.0);
+  s.min_root = dmin > 0.0f ? sqrt(static_cast<double>(dmin)) : 0.0;
+  s.residual_tolerance = limits[i].residual_tolerance;
+  s.forward_tolerance = limits[i].forward_tolerance;
+  s.min_corrections = limits[i].min_corrections;
+  s.max_corrections = limits[i].max_corrections;
+  const bool valid = (js[i].flags & kJacobiInvalidSpectrum) == 0 && dmin > 0.0f &&
+                     isfinite(s.matrix_norm) && s.matrix_norm > 0.0;
+  s.status = valid ? kSolveRunning : kSolveInvalidSpectrum;
+  st[i] = s;
+  active[i] = valid ? 1 : 0;
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t value) {
+  for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+       e += static_cast<size_t>(gridDim.x) * blockDim.x)
+    p[e] = value;
+}
+
+// sum of squares of the order x order block of a binary64 segment.
+__global__ void sumsq_f64_kernel(const Segment* __restrict__ segs, const double* __restrict__ a,
+                                 double* __restrict__ out) {
+  double s = 0.0;
+  SEGMENT_LOOP(sg, e) {
+    const int i = static_cast<int>(e % sg.ld);
+    const int j = static_cast<int>(e / sg.ld);
+    if (i < static_cast<int>(sg.order) && j < static_cast<int>(sg.order)) {
+      const double v = a[sg.offset + e];
+      s += v * v;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out + segs[blockIdx.y].index, s);
+}
+
+int seg_grid_x(uint32_t max_ld) {
+  const size_t n = static_cast<size_t>(max_ld) * max_ld;
+  size_t g = (n + kT * 4 - 1) / (kT * 4);
+  if (g < 1) g = 1;
+  if (g > 1024) g = 1024;
+  return static_cast<int>(g);
+}
+
+}  // namespace
+
+hipError_t launch_symmetrize_to_f64(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                    const float* c, double* x, const int* active,
+                                    hipStream_t stream) {
+  if (!count) return hipSuccess;
+  symmetrize_to_f64_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(segs, c, x, active);
+  return hipGetLastError();
+}
+hipError_t launch_add_symmetric(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                const float* e, double* x, const int* active,
+                                hipStream_t stream) {
+  if (!count) return hipSuccess;
+  add_symmetric_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(segs, e, x, active);
+  return hipGetLastError();
+}
+hipError_t launch_scale_rows_transpose(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                       const float* v, const float* roots, int use_sqrt,
+                                       float* out, hipStream_t stream) {
+  if (!count) return hipSuccess;
+  scale_rows_transpose_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(
+      segs, v, roots, use_sqrt, out);
+  return hipGetLastError();
+}
+hipError_t launch_transpose(const Segment* segs, uint32_t count, uint32_t max_ld,
+                            const float* v, float* out, hipStream_t stream) {
+  if (!count) return hipSuccess;
+  transpose_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(segs, v, out);
+  return hipGetLastError();
+}
+hipError_t launch_zero_f64(double* p, size_t n, hipStream_t stream) {
+  if (!n) return hipSuccess;
+  size_t g = (n + kT - 1) / kT;
+  if (g > 4096) g = 4096;
+  zero_f64_kernel<<<static_cast<int>(g), kT, 0, stream>>>(p, n);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t value, hipStream_t stream) {
+  if (!n) return hipSuccess;
+  size_t g = (n + kT - 1) / kT;
+  if (g > 4096) g = 4096;
+  fill_u32_kernel<<<static_cast<int>(g), kT, 0, stream>>>(p, n, value);
+  return hipGetLastError();
+}
+hipError_t launch_sumsq_f64(const Segment* segs, uint32_t count, uint32_t max_ld, const double* a,
+                            double* out, hipStream_t stream) {
+  if (!count) return hipSuccess;
+  sumsq_f64_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(segs, a, out);
+  return hipGetLastError();
+}
+hipError_t launch_orthogonality(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                const float* gram, double* out, hipStream_t stream) {
+  if (!count) return hipSuccess;
+  orthogonality_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(segs, gram, out);
+  return hipGetLastError();
+}
+hipError_t launch_segment_fp8(const Segment* segs, uint32_t count, uint32_t max_ld,
+                              const float* src, uint32_t* amax, float* sigma, uint8_t* dst,
+                              hipStream_t stream) {
+  if (!count) return hipSuccess;
+  segment_amax_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(segs, src, amax);
+  segment_sigma_kernel<<<(count + kT - 1) / kT, kT, 0, stream>>>(amax, sigma, count);
+  segment_encode_fp8_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(segs, src, sigma,
+                                                                                dst);
+  return hipGetLastError();
+}
+hipError_t launch_alpha(const float* sa, const float* sb, float* alpha, uint32_t count,
+                        hipStream_t stream) {
+  if (!count) return hipSuccess;
+  alpha_kernel<<<(count + kT - 1) / kT, kT, 0, stream>>>(sa, sb, alpha, count);
+  return hipGetLastError();
+}
+hipError_t launch_solve_state(const Segment* segs, const JacobiStatus* js,
+                              const float* diagonal, const SolveLimits* limits, SolveState* st,
+                              int* active, uint32_t count, hipStream_t stream) {
+  if (!count) return hipSuccess;
+  solve_state_kernel<<<(count + kT - 1) / kT, kT, 0, stream>>>(segs, js, diagonal, limits, st,
+                                                              active, count);
+  return hipGetLastError();
+}
+hipError_t launch_convergence(SolveState* st, const double* norms, int* active, uint32_t count,
+                              uint32_t step, hipStream_t stream) {
+  if (!count) return hipSuccess;
+  convergence_kernel<<<(count + kT - 1) / kT, kT, 0, stream>>>(st, norms, active, count, step);
+  return hipGetLastError();
+}
+
+}  // namespace msq

@@ -1,0 +1,331 @@
+"""ctypes binding of libmsq_hip.so (include/msq_hip.h).
+
+Mirrors the reference's device interface, MetalRuntime
+(src/metal_runtime.hpp:69-176), as a Python class: one runtime per device,
+methods named after the reference's (quantizeAndDecode -> quantize_and_decode,
+symmetricJacobi -> jacobi_batched, ...).  Device memory is passed as torch
+tensors (or raw addresses); torch is plumbing here, every operation runs in
+the HIP library.  Importing this module fails loudly when the library is
+missing: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+
+import numpy as np
+
+LIB_PATH = pathlib.Path(os.environ.get(
+    "MSQ_HIP_LIBRARY", pathlib.Path(__file__).resolve().parent / "libmsq_hip.so"))
+
+if not LIB_PATH.exists():
+    raise ImportError(
+        f"libmsq_hip.so not found at {LIB_PATH}; build it with "
+        "`python mixed-mlx_amd/build.py` (or __graft_entry__.build())")
+
+lib = ctypes.CDLL(str(LIB_PATH))
+
+# Precision values (src/precision.hpp:9-20 order) and the scaled FP8 tier.
+FP32, BF16, FP8_E4M3FN, FP8_E5M2, MXFP8_E4M3FN, MXFP8_E5M2, FP4_E2M1, \
+    MXFP4_E2M1, FP8_E4M3FN_LOG_S, FP4_E2M1_LOG_S = range(10)
+FP8_E4M3FN_SCALED = 16
+PRECISION_NAMES = {
+    FP32: "fp32", BF16: "bf16", FP8_E4M3FN: "fp8_e4m3fn", FP8_E5M2: "fp8_e5m2",
+    MXFP8_E4M3FN: "mxfp8_e4m3fn", MXFP8_E5M2: "mxfp8_e5m2", FP4_E2M1: "fp4_e2m1",
+    MXFP4_E2M1: "mxfp4_e2m1", FP8_E4M3FN_LOG_S: "fp8_e4m3fn_log_s",
+    FP4_E2M1_LOG_S: "fp4_e2m1_log_s", FP8_E4M3FN_SCALED: "fp8_e4m3fn_scaled",
+}
+PRECISION_BY_NAME = {v: k for k, v in PRECISION_NAMES.items()}
+
+SOLVE_STATUS = {0: "running", 1: "converged", 2: "budget_exhausted",
+                3: "diverged", 4: "nonfinite", 5: "invalid_spectrum"}
+JACOBI_HISTORY = 32
+SOLVE_HISTORY = 20
+
+# Every symbol include/msq_hip.h declares.
+EXPORTED_SYMBOLS = (
+    "msq_last_error", "msq_abi_version", "msq_runtime_create", "msq_runtime_destroy",
+    "msq_runtime_set_stream", "msq_synchronize", "msq_alloc", "msq_free",
+    "msq_copy_to_device", "msq_copy_to_host", "msq_memset", "msq_storage_bytes",
+    "msq_scale_bytes", "msq_batch_layout", "msq_quantize", "msq_decode",
+    "msq_quantize_and_decode", "msq_quantize_fp8_scaled", "msq_decode_fp8_scaled",
+    "msq_jacobi_batched", "msq_reconstruct", "msq_residual_fp64", "msq_correction",
+    "msq_orthogonality", "msq_solve_batched", "msq_last_factors",
+)
+
+
+class JacobiOptions(ctypes.Structure):
+    _fields_ = [("fixed_sweeps", ctypes.c_uint32), ("max_sweeps", ctypes.c_uint32),
+                ("off_tol", ctypes.c_float), ("off_tol_lambda", ctypes.c_float),
+                ("stagnation", ctypes.c_float), ("floor_tol", ctypes.c_float)]
+
+
+class JacobiReport(ctypes.Structure):
+    _fields_ = [("sweeps", ctypes.c_uint32), ("invalid_spectrum", ctypes.c_uint32),
+                ("off_norm", ctypes.c_double), ("matrix_norm", ctypes.c_double),
+                ("min_diagonal", ctypes.c_float),
+                ("off_history", ctypes.c_float * JACOBI_HISTORY)]
+
+
+class SolveOptions(ctypes.Structure):
+    _fields_ = [("basis_precision", ctypes.c_int32), ("correction_precision", ctypes.c_int32),
+                ("jacobi", JacobiOptions), ("min_corrections", ctypes.c_uint32),
+                ("max_corrections", ctypes.c_uint32), ("residual_tolerance", ctypes.c_double),
+                ("forward_tolerance", ctypes.c_double)]
+
+
+class SolveReport(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_uint32), ("sweeps", ctypes.c_uint32),
+                ("corrections", ctypes.c_uint32), ("status", ctypes.c_uint32),
+                ("matrix_norm", ctypes.c_double), ("min_root", ctypes.c_double),
+                ("off_norm", ctypes.c_double), ("final_residual", ctypes.c_double),
+                ("final_forward_estimate", ctypes.c_double),
+                ("residuals", ctypes.c_double * SOLVE_HISTORY),
+                ("forward_estimates", ctypes.c_double * SOLVE_HISTORY),
+                ("off_history", ctypes.c_float * JACOBI_HISTORY)]
+
+
+_p = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_sz = ctypes.c_size_t
+_i = ctypes.c_int
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+def _sig(name, restype, *argtypes):
+    f = getattr(lib, name)
+    f.restype = restype
+    f.argtypes = list(argtypes)
+    return f
+
+
+_sig("msq_last_error", ctypes.c_char_p)
+_sig("msq_abi_version", _i)
+_sig("msq_runtime_create", _i, _i, ctypes.POINTER(_p))
+_sig("msq_runtime_destroy", _i, _p)
+_sig("msq_runtime_set_stream", _i, _p, _p)
+_sig("msq_synchronize", _i, _p)
+_sig("msq_alloc", _i, _p, _sz, ctypes.POINTER(_p))
+_sig("msq_free", _i, _p, _p)
+_sig("msq_copy_to_device", _i, _p, _p, _p, _sz)
+_sig("msq_copy_to_host", _i, _p, _p, _p, _sz)
+_sig("msq_memset", _i, _p, _p, _i, _sz)
+_sig("msq_storage_bytes", _sz, _i, _sz)
+_sig("msq_scale_bytes", _sz, _i, _sz)
+_sig("msq_batch_layout", _i, _u32, _u32p, _u32p, _u64p, _u64p, _u64p, _u64p)
+_sig("msq_quantize", _i, _p, _i, _p, _p, _p, _sz)
+_sig("msq_decode", _i, _p, _i, _p, _p, _p, _sz)
+_sig("msq_quantize_and_decode", _i, _p, _i, _p, _p, _p, _p, _sz)
+_sig("msq_quantize_fp8_scaled", _i, _p, _p, _p, _p, _sz)
+_sig("msq_decode_fp8_scaled", _i, _p, _p, _p, _p, _sz)
+_sig("msq_jacobi_batched", _i, _p, _u32, _u32p, _p, _p, _p, _p,
+     ctypes.POINTER(JacobiOptions), _u32, ctypes.POINTER(JacobiReport))
+_sig("msq_reconstruct", _i, _p, _i, _u32, _u32p, _p, _p, _p)
+_sig("msq_residual_fp64", _i, _p, _u32, _u32p, _p, _p, _p, _p, _f64p)
+_sig("msq_correction", _i, _p, _i, _u32, _u32p, _p, _p, _p, _p)
+_sig("msq_orthogonality", _i, _p, _u32, _u32p, _p, _f64p)
+_sig("msq_solve_batched", _i, _p, _u32, _u32p, _p, _p, ctypes.POINTER(SolveOptions), _u32,
+     ctypes.POINTER(SolveReport))
+_sig("msq_last_factors", _i, _p, ctypes.POINTER(_p), ctypes.POINTER(_p))
+
+
+class MsqError(RuntimeError):
+    pass
+
+
+def check(status):
+    if status != 0:
+        raise MsqError(f"msq status {status}: {lib.msq_last_error().decode()}")
+
+
+def storage_bytes(precision, elements):
+    return lib.msq_storage_bytes(precision, elements)
+
+
+def scale_bytes(precision, elements):
+    return lib.msq_scale_bytes(precision, elements)
+
+
+def _addr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _u32_array(values):
+    arr = (ctypes.c_uint32 * len(values))(*[int(v) for v in values])
+    return arr
+
+
+# ---------------------------------------------------------------- layout
+def batch_layout(orders):
+    """ld = roundup(n, 32), offsets of ld x ld tiles and of ld-vectors."""
+    orders = [int(n) for n in orders]
+    count = len(orders)
+    lds = (ctypes.c_uint32 * count)()
+    offs = (ctypes.c_uint64 * count)()
+    voffs = (ctypes.c_uint64 * count)()
+    total = ctypes.c_uint64()
+    vtotal = ctypes.c_uint64()
+    check(lib.msq_batch_layout(count, _u32_array(orders), lds, offs, voffs,
+                               ctypes.byref(total), ctypes.byref(vtotal)))
+    return (np.array(lds[:], dtype=np.int64), np.array(offs[:], dtype=np.int64),
+            np.array(voffs[:], dtype=np.int64), int(total.value), int(vtotal.value))
+
+
+def pack_batch(matrices, dtype=np.float64):
+    """Host arrays (n_i x n_i) -> (orders, flat column-major batch layout)."""
+    orders = [m.shape[0] for m in matrices]
+    lds, offs, _, total, _ = batch_layout(orders)
+    flat = np.zeros(total, dtype=dtype)
+    for m, ld, off in zip(matrices, lds, offs):
+        n = m.shape[0]
+        tile = np.zeros((ld, ld), dtype=dtype)
+        tile[:n, :n] = m
+        flat[off:off + ld * ld] = tile.reshape(-1, order="F")
+    return orders, flat
+
+
+def unpack_batch(orders, flat):
+    lds, offs, _, _, _ = batch_layout(orders)
+    out = []
+    for n, ld, off in zip(orders, lds, offs):
+        out.append(np.asarray(flat[off:off + ld * ld]).reshape(ld, ld, order="F")[:n, :n].copy())
+    return out
+
+
+def unpack_vectors(orders, flat):
+    lds, _, voffs, _, _ = batch_layout(orders)
+    return [np.asarray(flat[v:v + n]).copy() for n, v in zip(orders, voffs)]
+
+
+# ---------------------------------------------------------------- options
+def jacobi_options(fixed_sweeps=0, max_sweeps=30, off_tol=1e-7, off_tol_lambda=0.0,
+                   stagnation=0.5, floor_tol=1e-5):
+    return JacobiOptions(fixed_sweeps, max_sweeps, off_tol, off_tol_lambda, stagnation,
+                         floor_tol)
+
+
+def solve_options(basis=FP32, correction=FP32, jacobi=None, min_corrections=0,
+                  max_corrections=12, residual_tolerance=1e-12, forward_tolerance=0.0):
+    if jacobi is None:
+        jacobi = jacobi_options()
+    return SolveOptions(basis, correction, jacobi, min_corrections, max_corrections,
+                        residual_tolerance, forward_tolerance)
+
+
+def report_dict(r: SolveReport):
+    steps = r.corrections + 1
+    return {
+        "order": r.order, "sweeps": r.sweeps, "corrections": r.corrections,
+        "status": SOLVE_STATUS.get(r.status, str(r.status)),
+        "matrix_norm": r.matrix_norm, "min_root": r.min_root, "off_norm": r.off_norm,
+        "final_residual": r.final_residual,
+        "final_forward_estimate": r.final_forward_estimate,
+        "residuals": list(r.residuals[:steps]),
+        "forward_estimates": list(r.forward_estimates[:steps]),
+        "off_history": list(r.off_history[:r.sweeps]),
+    }
+
+
+# ---------------------------------------------------------------- runtime
+class Runtime:
+    """HIP backend runtime (MetalRuntime's role, src/metal_runtime.hpp:69-176)."""
+
+    def __init__(self, device: int = 0):
+        h = _p()
+        check(lib.msq_runtime_create(device, ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            check(lib.msq_runtime_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                lib.msq_runtime_destroy(self.handle)
+        except Exception:
+            pass
+
+    def set_stream(self, stream=None):
+        """Enqueue on a torch stream (or HIP stream address); None: own stream."""
+        addr = None if stream is None else (stream if isinstance(stream, int)
+                                             else stream.cuda_stream)
+        check(lib.msq_runtime_set_stream(self.handle, addr))
+
+    def synchronize(self):
+        check(lib.msq_synchronize(self.handle))
+
+    # codecs -----------------------------------------------------------
+    def quantize(self, precision, src, values, scales=None):
+        check(lib.msq_quantize(self.handle, precision, _addr(src), _addr(values),
+                               _addr(scales), src.numel()))
+
+    def decode(self, precision, values, scales, dst):
+        check(lib.msq_decode(self.handle, precision, _addr(values), _addr(scales),
+                             _addr(dst), dst.numel()))
+
+    def quantize_and_decode(self, precision, src, values, scales, decoded):
+        check(lib.msq_quantize_and_decode(self.handle, precision, _addr(src), _addr(values),
+                                          _addr(scales), _addr(decoded), src.numel()))
+
+    def quantize_fp8_scaled(self, src, values, sigma):
+        check(lib.msq_quantize_fp8_scaled(self.handle, _addr(src), _addr(values),
+                                          _addr(sigma), src.numel()))
+
+    def decode_fp8_scaled(self, values, sigma, dst):
+        check(lib.msq_decode_fp8_scaled(self.handle, _addr(values), _addr(sigma),
+                                        _addr(dst), dst.numel()))
+
+    # solver pieces ------------------------------------------------------
+    def jacobi_batched(self, orders, matrices, vectors, diagonal, roots, options):
+        opts = options if isinstance(options, (list, tuple)) else [options]
+        arr = (JacobiOptions * len(opts))(*opts)
+        reports = (JacobiReport * len(orders))()
+        check(lib.msq_jacobi_batched(self.handle, len(orders), _u32_array(orders),
+                                     _addr(matrices), _addr(vectors), _addr(diagonal),
+                                     _addr(roots), arr, len(opts), reports))
+        return list(reports)
+
+    def reconstruct(self, precision, orders, vectors, roots, out):
+        check(lib.msq_reconstruct(self.handle, precision, len(orders), _u32_array(orders),
+                                  _addr(vectors), _addr(roots), _addr(out)))
+
+    def residual_fp64(self, orders, matrices, root, residual, residual_f32):
+        rel = (ctypes.c_double * len(orders))()
+        check(lib.msq_residual_fp64(self.handle, len(orders), _u32_array(orders),
+                                    _addr(matrices), _addr(root), _addr(residual),
+                                    _addr(residual_f32), rel))
+        return np.array(rel[:])
+
+    def correction(self, precision, orders, vectors, roots, residual_f32, out):
+        check(lib.msq_correction(self.handle, precision, len(orders), _u32_array(orders),
+                                 _addr(vectors), _addr(roots), _addr(residual_f32), _addr(out)))
+
+    def orthogonality(self, orders, vectors):
+        res = (ctypes.c_double * len(orders))()
+        check(lib.msq_orthogonality(self.handle, len(orders), _u32_array(orders),
+                                    _addr(vectors), res))
+        return np.array(res[:])
+
+    def solve_batched(self, orders, matrices, roots, options, reports=True):
+        opts = options if isinstance(options, (list, tuple)) else [options]
+        arr = (SolveOptions * len(opts))(*opts)
+        reps = (SolveReport * len(orders))() if reports else None
+        check(lib.msq_solve_batched(self.handle, len(orders), _u32_array(orders),
+                                    _addr(matrices), _addr(roots), arr, len(opts), reps))
+        return [report_dict(r) for r in reps] if reports else None
+
+    def last_factors(self):
+        v = _p()
+        s = _p()
+        check(lib.msq_last_factors(self.handle, ctypes.byref(v), ctypes.byref(s)))
+        return v.value, s.value

@@ -1,0 +1,195 @@
+"""MFMA products, the binary64 residual and the Jacobi kernels against the
+oracle and closed forms."""
+import numpy as np
+import pytest
+import torch
+
+from gpu_util import dev, host, say
+from msq import capi
+from msq import generators as G
+from oracle import codecs as C
+from oracle import solver as S
+
+pytestmark = pytest.mark.gpu
+
+TIERS = [capi.FP32, capi.BF16, capi.FP8_E4M3FN_SCALED, capi.MXFP4_E2M1]
+
+
+def random_basis(n, seed):
+    return G.haar_orthogonal([G.matrix_seed(11, seed)], n)[0].astype(np.float32)
+
+
+def quantized(tier, M):
+    """Tier rounding of a float matrix stored K-contiguous (rows of M)."""
+    M = np.ascontiguousarray(M, np.float32)
+    if tier == capi.FP32:
+        return M.astype(np.float64)
+    if tier == capi.BF16:
+        c, _ = C.encode("bf16", M.reshape(-1))
+        return C.decode("bf16", c, None, M.size).reshape(M.shape).astype(np.float64)
+    if tier == capi.FP8_E4M3FN_SCALED:
+        c, s = C.fp8_scaled(M.reshape(-1))
+        return (C.decode("fp8_e4m3fn", c, None, M.size).reshape(M.shape) / np.float64(s))
+    # MXFP4 blocks of 32 along the contiguous (K) direction of each row
+    ld = M.shape[1]
+    c, s = C.encode("mxfp4_e2m1", M.reshape(-1))
+    return C.decode("mxfp4_e2m1", c, s, M.size).reshape(M.shape).astype(np.float64)
+
+
+@pytest.mark.parametrize("tier", TIERS)
+def test_reconstruction_products(runtime, tier):
+    orders = [16, 32, 48, 64, 96, 200]
+    Vs = [random_basis(n, i) for i, n in enumerate(orders)]
+    roots = [np.geomspace(0.01, 1.0, n).astype(np.float32) for n in orders]
+    o, vflat = capi.pack_batch(Vs, np.float32)
+    _, _, voffs, total, vtotal = capi.batch_layout(orders)
+    rflat = np.zeros(vtotal, np.float32)
+    for r, v0 in zip(roots, voffs):
+        rflat[v0:v0 + r.size] = r
+    out = torch.zeros(total, dtype=torch.float32, device="cuda")
+    say(f"reconstruct tier {tier}")
+    runtime.reconstruct(tier, o, dev(vflat), dev(rflat), out)
+    got = capi.unpack_batch(o, host(out))
+    for V, r, X, n in zip(Vs, roots, got, orders):
+        ld = (n + 31) // 32 * 32
+        Gt = np.zeros((ld, ld), np.float32)       # rows of G = V diag(sqrt s)
+        Gt[:n, :n] = V * np.sqrt(r)[None, :]
+        Gq = quantized(tier, Gt)[:n, :n]
+        want = Gq @ Gq.T
+        # FP32 accumulation bound (3_spectral/product_benchmark.cpp:422-444 form)
+        bound = (2 * n + 8) * 2.0**-24 * np.linalg.norm(Gq) ** 2
+        assert np.linalg.norm(X - want) <= bound, (tier, n)
+
+
+@pytest.mark.parametrize("tier", TIERS)
+def test_correction_known_answer(runtime, tier):
+    """src/metal_checks.cpp:537-601: identity basis, roots 1, rhs 2 -> 1."""
+    orders = [16, 16, 64]
+    o, vflat = capi.pack_batch([np.eye(n, dtype=np.float32) for n in orders], np.float32)
+    _, rflat = capi.pack_batch([np.full((n, n), 2.0, np.float32) for n in orders], np.float32)
+    _, _, voffs, total, vtotal = capi.batch_layout(orders)
+    roots = np.zeros(vtotal, np.float32)
+    for n, v0 in zip(orders, voffs):
+        roots[v0:v0 + n] = 1.0
+    out = torch.zeros(total, dtype=torch.float32, device="cuda")
+    runtime.correction(tier, o, dev(vflat), dev(roots), dev(rflat), out)
+    for E, n in zip(capi.unpack_batch(o, host(out)), orders):
+        tol = 0 if tier != capi.FP8_E4M3FN_SCALED else 1e-6
+        np.testing.assert_allclose(E, np.ones((n, n)), rtol=0, atol=tol)
+    # reconstruction of the identity basis with unit roots is the identity
+    X = torch.zeros(total, dtype=torch.float32, device="cuda")
+    runtime.reconstruct(tier, o, dev(vflat), dev(roots), X)
+    for Xi, n in zip(capi.unpack_batch(o, host(X)), orders):
+        np.testing.assert_allclose(Xi, np.eye(n), rtol=0, atol=tol)
+
+
+def test_correction_fp32_matches_oracle(runtime):
+    orders = [32, 80, 128]
+    As = []
+    for i, n in enumerate(orders):
+        A, X, _ = G.single_matrix(5, n, 1e-2, index=i)
+        As.append(A)
+    Vs, roots, Rs = [], [], []
+    for A in As:
+        V, r, _ = S.jacobi_f32(A, sweeps=6)
+        X = S.symmetrize(S.reconstruct_f32(V, r))
+        R, _ = S.residual_fp64(A, X)
+        Vs.append(V)
+        roots.append(r)
+        Rs.append(R.astype(np.float32))
+    o, vflat = capi.pack_batch(Vs, np.float32)
+    _, rflat = capi.pack_batch(Rs, np.float32)
+    _, _, voffs, total, vtotal = capi.batch_layout(orders)
+    sflat = np.zeros(vtotal, np.float32)
+    for r, v0 in zip(roots, voffs):
+        sflat[v0:v0 + r.size] = r
+    out = torch.zeros(total, dtype=torch.float32, device="cuda")
+    runtime.correction(capi.FP32, o, dev(vflat), dev(sflat), dev(rflat), out)
+    for E, V, r, R in zip(capi.unpack_batch(o, host(out)), Vs, roots, Rs):
+        _, H = S.frozen_factor(V, r)
+        want = S.correction_f32(V, H, R)
+        assert np.linalg.norm(E - want) <= 1e-5 * np.linalg.norm(want)
+
+
+def test_residual_fp64(runtime):
+    orders = [16, 50, 130]
+    As, Xs = [], []
+    for i, n in enumerate(orders):
+        A, X, _ = G.single_matrix(6, n, 1e-3, index=i)
+        As.append(A)
+        Xs.append(X * (1 + 1e-4 * np.random.default_rng(i).standard_normal((n, n))))
+        Xs[-1] = 0.5 * (Xs[-1] + Xs[-1].T)
+    o, aflat = capi.pack_batch(As)
+    _, xflat = capi.pack_batch(Xs)
+    _, _, _, total, _ = capi.batch_layout(orders)
+    R = torch.zeros(total, dtype=torch.float64, device="cuda")
+    R32 = torch.zeros(total, dtype=torch.float32, device="cuda")
+    rel = runtime.residual_fp64(o, dev(aflat), dev(xflat), R, R32)
+    for Ri, A, X, ri in zip(capi.unpack_batch(o, host(R)), As, Xs, rel):
+        want, wrel = S.residual_fp64(A, X)
+        assert np.max(np.abs(Ri - want)) <= 1e-14 * np.abs(X).max() ** 2 * A.shape[0]
+        assert abs(ri - wrel) <= 1e-10 * wrel
+
+
+def _jacobi_batch(runtime, mats, options):
+    orders = [m.shape[0] for m in mats]
+    o, aflat = capi.pack_batch(mats)
+    _, _, _, total, vtotal = capi.batch_layout(orders)
+    V = torch.zeros(total, dtype=torch.float32, device="cuda")
+    D = torch.zeros(vtotal, dtype=torch.float32, device="cuda")
+    Rt = torch.zeros(vtotal, dtype=torch.float32, device="cuda")
+    reps = runtime.jacobi_batched(o, dev(aflat), V, D, Rt, options)
+    return (capi.unpack_batch(o, host(V)), capi.unpack_vectors(o, host(D)),
+            capi.unpack_vectors(o, host(Rt)), reps)
+
+
+def test_jacobi_fixed_sweeps_parity(runtime):
+    """The reference's fixed 8 sweeps (src/metal_solver.cpp:210-213) against
+    the oracle restatement on the same inputs."""
+    orders = [5, 16, 31, 64, 100, 128, 150, 256]
+    mats = [G.single_matrix(7, n, 1e-2, index=i)[0] for i, n in enumerate(orders)]
+    say("jacobi fixed 8 sweeps", orders)
+    Vs, Ds, Rts, reps = _jacobi_batch(runtime, mats, capi.jacobi_options(fixed_sweeps=8))
+    for A, V, d, rt, rep in zip(mats, Vs, Ds, Rts, reps):
+        n = A.shape[0]
+        assert rep.sweeps == 8 and not rep.invalid_spectrum
+        Vo, ro, info = S.jacobi_f32(A, sweeps=8)
+        np.testing.assert_allclose(np.sort(d), np.sort(info["diagonal"]), rtol=2e-5, atol=1e-6)
+        np.testing.assert_allclose(np.sort(d), np.linalg.eigvalsh(A), rtol=1e-4, atol=2e-6)
+        np.testing.assert_allclose(rt, np.sqrt(d), rtol=1e-7)
+        assert S.orthogonality_error(V) < 2e-6 * np.sqrt(n)
+        # V diag(d) V^T reproduces A
+        assert np.linalg.norm((V * d) @ V.T - A) <= 2e-6 * np.sqrt(n) * np.linalg.norm(A)
+
+
+def test_jacobi_tolerance_mode(runtime):
+    orders = [16, 64, 128, 200, 256]
+    mats = [G.single_matrix(8, n, 1e-4, index=i)[0] for i, n in enumerate(orders)]
+    say("jacobi tolerance", orders)
+    Vs, Ds, _, reps = _jacobi_batch(runtime, mats, capi.jacobi_options(off_tol=1e-7))
+    for A, V, d, rep in zip(mats, Vs, Ds, reps):
+        assert 3 <= rep.sweeps < 30
+        hist = list(rep.off_history[: rep.sweeps])
+        assert hist[-1] <= 1e-5
+        VAV = V.astype(np.float64).T @ A @ V.astype(np.float64)
+        off = np.sqrt(max(np.sum(VAV**2) - np.sum(np.diag(VAV) ** 2), 0))
+        assert off <= 2e-5 * np.linalg.norm(A)
+
+
+def test_jacobi_large_order_global_path(runtime):
+    n = 300
+    A = G.single_matrix(9, n, 1e-2)[0]
+    say("jacobi global n=300")
+    Vs, Ds, _, reps = _jacobi_batch(runtime, [A], capi.jacobi_options(off_tol=1e-7))
+    np.testing.assert_allclose(np.sort(Ds[0]), np.linalg.eigvalsh(A), rtol=1e-4, atol=2e-6)
+    assert S.orthogonality_error(Vs[0]) < 1e-4
+
+
+def test_orthogonality_measure(runtime):
+    orders = [16, 64]
+    Vs = [random_basis(n, 3 + i) for i, n in enumerate(orders)]
+    Vs[1] = Vs[1] * np.float32(1.001)
+    o, vflat = capi.pack_batch(Vs, np.float32)
+    got = runtime.orthogonality(o, dev(vflat))
+    for g, V in zip(got, Vs):
+        assert abs(g - S.orthogonality_error(V)) <= 1e-6 + 1e-3 * S.orthogonality_error(V)

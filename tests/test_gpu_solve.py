@@ -1,0 +1,71 @@
+"""End-to-end GPU-resident solves against the FP64 LAPACK oracle."""
+import numpy as np
+import pytest
+import torch
+
+from gpu_util import dev, host, say
+from msq import capi
+from msq import generators as G
+from msq import solver as MS
+from oracle import solver as S
+
+pytestmark = pytest.mark.gpu
+
+
+def solve(runtime, mats, options):
+    o, aflat = capi.pack_batch(mats)
+    _, _, _, total, _ = capi.batch_layout(o)
+    X = torch.zeros(total, dtype=torch.float64, device="cuda")
+    reps = runtime.solve_batched(o, dev(aflat), X, options)
+    return capi.unpack_batch(o, host(X)), reps
+
+
+def test_config0_n64_fp32_one_correction(runtime):
+    """n = 64, cond 1e2, FP32: Jacobi to off < 1e-7 ||A||_F (or the FP32
+    floor), X0, one correction, compared with the FP64 LAPACK root."""
+    A, Xc, lam = G.single_matrix(0, 64, 1e-2)
+    Xref = S.reference_root(A)
+    opts = capi.solve_options(capi.FP32, capi.FP32, capi.jacobi_options(off_tol=1e-7),
+                              min_corrections=1, max_corrections=1, residual_tolerance=0.0)
+    (X,), (rep,) = solve(runtime, [A], opts)
+    say("config0", rep)
+    assert rep["corrections"] == 1
+    err = S.relative_error(X, Xref)
+    assert err <= 1e-6
+    assert rep["residuals"][1] < rep["residuals"][0]
+    assert S.relative_symmetry_error(X) <= 1e-12
+
+
+@pytest.mark.parametrize("tier", ["plain_fp32", "bf16"])
+def test_batched_uneven_every_matrix(runtime, tier):
+    mats = G.batched_uneven(count=48)
+    As = [m["A"] for m in mats]
+    opts = MS.plain_fp32_options() if tier == "plain_fp32" else MS.tier_options("bf16")
+    Xs, reps = solve(runtime, As, opts)
+    worst = 0.0
+    for m, X, rep in zip(mats, Xs, reps):
+        Xref = S.reference_root(m["A"])
+        err = S.relative_error(X, Xref)
+        worst = max(worst, err)
+        assert rep["status"] == "converged", (m["order"], m["condition"], rep)
+        assert err <= 1e-6, (m["order"], m["condition"], err, rep)
+    say(tier, "worst", worst)
+
+
+def test_config1_n1024_bf16(runtime):
+    """n = 1024, cond 1e4: BF16 reconstruction, FP64 residual, 2 corrections."""
+    A, Xc, lam = G.single_matrix(1, 1024, 1e-4)
+    say("config1 n=1024")
+    opts = capi.solve_options(capi.BF16, capi.FP32, capi.jacobi_options(off_tol=1e-7),
+                              min_corrections=2, max_corrections=2, residual_tolerance=0.0)
+    (X,), (rep,) = solve(runtime, [A], opts)
+    err = S.relative_error(X, Xc)
+    say("config1", rep["sweeps"], rep["residuals"], err)
+    assert err <= 1e-6
+
+
+def test_invalid_spectrum_reported(runtime):
+    A = G.single_matrix(2, 32, 1e-2)[0]
+    A = A - 2.0 * np.eye(32)   # indefinite
+    (X,), (rep,) = solve(runtime, [A], MS.plain_fp32_options())
+    assert rep["status"] == "invalid_spectrum"
