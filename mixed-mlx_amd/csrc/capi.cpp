@@ -38,12 +38,16 @@ int set_error(int code, const char* fmt, ...) {
   return code;
 }
 
+// On failure: record the message and clear HIP's per-thread last error, so a
+// later, unrelated HIP call in the caller does not report this one.
 #define HIP_TRY(expr)                                                                     \
   do {                                                                                    \
     hipError_t e_ = (expr);                                                               \
-    if (e_ != hipSuccess)                                                                 \
+    if (e_ != hipSuccess) {                                                               \
+      (void)hipGetLastError();                                                            \
       return set_error(MSQ_ERROR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
                        __FILE__, __LINE__);                                               \
+    }                                                                                     \
   } while (0)
 
 #define MSQ_TRY(expr)          \

@@ -204,7 +204,9 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
     __syncthreads();
   }
 
-  // Epilogue.  Lane holds C[row = 4 quad + r][col = row16] of each 16x16 tile.
+  // Epilogue.  The f32-accumulating MFMAs leave C[row = 4 quad + r][col =
+  // row16] in accumulator r of a lane; v_mfma_f64_16x16x4_f64 leaves
+  // C[row = 4 r + quad][col = row16].
   const float alpha = args.alpha ? args.alpha[job.status_index] : 1.0f;
   typedef typename Tr::Out Out;
   Out* C = static_cast<Out*>(args.c) + job.c_offset;
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
       const int col = tn + wn + 16 * j + row16;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = tm + wm + 16 * i + 4 * quad + r;
+        const int row = tm + wm + 16 * i + (T == GemmType::F64 ? 4 * r + quad : 4 * quad + r);
         if (row >= ld || col >= ld) continue;
         if constexpr (EPI == Epilogue::Residual) {
           const double* A = static_cast<const double*>(args.aux) + job.aux_offset;

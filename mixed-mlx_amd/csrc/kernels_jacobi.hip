@@ -328,7 +328,11 @@ __global__ __launch_bounds__(NT) void batched_jacobi_kernel(
   } else if (n & 1) {
     for (int i = tid; i < ld; i += NT) Vg[i + static_cast<size_t>(n) * ld] = 0.0f;
   }
-  bad = __syncthreads_or(bad);
+  if (tid == 0) ctrl[2] = 0;
+  __syncthreads();
+  if (bad) ctrl[2] = 1;
+  __syncthreads();
+  bad = ctrl[2];
   if (tid == 0) {
     st->sweeps = sweeps;
     st->flags = bad ? kJacobiInvalidSpectrum : 0u;
@@ -485,13 +489,15 @@ hipError_t launch_batched_variant(const JacobiJob* jobs, uint32_t count,
                                   const double* a64, float* vectors, float* diagonal,
                                   float* roots, JacobiStatus* status, size_t lds,
                                   hipStream_t stream) {
-  static bool configured = false;
-  if (!configured) {
+  // Dynamic LDS above the default needs an explicit opt-in, sized to this
+  // launch (static LDS plus the opt-in must stay within 160 KiB).
+  static size_t configured = 0;
+  if (lds > configured) {
     hipError_t e = hipFuncSetAttribute(
         reinterpret_cast<const void*>(&batched_jacobi_kernel<NT, VL>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     if (e != hipSuccess) return e;
-    configured = true;
+    configured = lds;
   }
   batched_jacobi_kernel<NT, VL><<<count, NT, lds, stream>>>(jobs, a64, vectors, diagonal,
                                                             roots, status);

@@ -56,8 +56,11 @@ def test_reconstruction_products(runtime, tier):
         Gt[:n, :n] = V * np.sqrt(r)[None, :]
         Gq = quantized(tier, Gt)[:n, :n]
         want = Gq @ Gq.T
-        # FP32 accumulation bound (3_spectral/product_benchmark.cpp:422-444 form)
-        bound = (2 * n + 8) * 2.0**-24 * np.linalg.norm(Gq) ** 2
+        # FP32 accumulation bound (3_spectral/product_benchmark.cpp:422-444 form);
+        # the FP8 MFMA accumulates its 32-term chunks more coarsely (measured
+        # about 2^-17 relative on MI355X), so FP8 gets u = 2^-16 per term.
+        u = 2.0**-16 if tier == capi.FP8_E4M3FN_SCALED else 2.0**-24
+        bound = (2 * n + 8) * u * np.linalg.norm(Gq) ** 2
         assert np.linalg.norm(X - want) <= bound, (tier, n)
 
 
@@ -154,12 +157,24 @@ def test_jacobi_fixed_sweeps_parity(runtime):
         n = A.shape[0]
         assert rep.sweeps == 8 and not rep.invalid_spectrum
         Vo, ro, info = S.jacobi_f32(A, sweeps=8)
-        np.testing.assert_allclose(np.sort(d), np.sort(info["diagonal"]), rtol=2e-5, atol=1e-6)
-        np.testing.assert_allclose(np.sort(d), np.linalg.eigvalsh(A), rtol=1e-4, atol=2e-6)
+        # Hoffman-Wielandt: the sorted diagonal of an orthogonal similarity of A
+        # is within its off-diagonal norm of the eigenvalues; the FP32 iterates
+        # also carry the rounding of S (n - 1) rounds, ~ eps ||A||_F sqrt(S n).
+        lam = np.linalg.eigvalsh(A)
+        drift = 2.0 * 2.0**-23 * np.linalg.norm(A) * np.sqrt(8.0 * (n - 1))
+        gpu_gap = np.linalg.norm(np.sort(d) - lam)
+        ora_gap = np.linalg.norm(np.sort(info["diagonal"]) - lam)
+        assert gpu_gap <= rep.off_norm + drift, (n, gpu_gap, rep.off_norm, drift)
+        assert ora_gap <= info["off_norm"] + drift, (n, ora_gap)
+        if n <= 128:   # converged in 8 sweeps: the diagonals agree to FP32 rounding
+            np.testing.assert_allclose(np.sort(d), np.sort(info["diagonal"]), rtol=2e-5,
+                                       atol=1e-6)
+            np.testing.assert_allclose(np.sort(d), np.linalg.eigvalsh(A), rtol=1e-4, atol=2e-6)
         np.testing.assert_allclose(rt, np.sqrt(d), rtol=1e-7)
         assert S.orthogonality_error(V) < 2e-6 * np.sqrt(n)
-        # V diag(d) V^T reproduces A
-        assert np.linalg.norm((V * d) @ V.T - A) <= 2e-6 * np.sqrt(n) * np.linalg.norm(A)
+        # V diag(d) V^T reproduces A up to the off-diagonal part left
+        err = np.linalg.norm((V * d) @ V.T - A)
+        assert err <= rep.off_norm + 2e-6 * np.sqrt(n) * np.linalg.norm(A)
 
 
 def test_jacobi_tolerance_mode(runtime):
