@@ -198,21 +198,6 @@ int upload(msq_runtime* rt, T* dst, const std::vector<T>& src) {
 }
 
 // -------------------------------------------------------------- Jacobi plan
-struct JacobiClass {
-  uint32_t max_order;
-  bool v_in_lds;
-  int threads;
-};
-// Size classes of the one-workgroup-per-matrix kernel: V in LDS while
-// A (packed) + V fit comfortably, more threads for larger orders.
-const JacobiClass kJacobiClasses[] = {
-    {32, true, 64},
-    {64, true, 128},
-    {128, true, 256},
-    {256, false, 512},
-};
-constexpr size_t kJacobiClassCount = sizeof(kJacobiClasses) / sizeof(kJacobiClasses[0]);
-
 msq::JacobiJob make_job(const Layout& L, uint32_t i, const msq_jacobi_options& o) {
   msq::JacobiJob j{};
   j.a_offset = L.off[i];
@@ -321,30 +306,31 @@ int jacobi_large(msq_runtime* rt, const Layout& L, uint32_t i, const double* a64
 }
 
 // Jacobi on a whole batch.  djobs: device room for `count` jobs; jstatus:
-// device status array in matrix order.
+// device status array in matrix order.  Orders up to 256 run one workgroup
+// per matrix, one launch per padded order; larger orders run one at a time.
 int run_jacobi(msq_runtime* rt, const Layout& L, const double* a64, float* V, float* diag,
                float* roots, const msq_jacobi_options* opts, uint32_t opts_count,
                msq::JacobiStatus* jstatus, msq::JacobiJob* djobs,
                std::vector<msq::JacobiJob>& jobs) {
   const uint32_t count = static_cast<uint32_t>(L.order.size());
   jobs.clear();
-  size_t begin[kJacobiClassCount + 1];
-  for (size_t c = 0; c < kJacobiClassCount; ++c) {
-    const uint32_t lo = c == 0 ? 0 : kJacobiClasses[c - 1].max_order;
-    begin[c] = jobs.size();
-    for (uint32_t i = 0; i < count; ++i)
-      if (L.order[i] > lo && L.order[i] <= kJacobiClasses[c].max_order)
-        jobs.push_back(make_job(L, i, opts[opts_count == 1 ? 0 : i]));
+  std::map<uint32_t, std::vector<uint32_t>> by_padded;
+  for (uint32_t i = 0; i < count; ++i)
+    if (L.order[i] <= msq::kBatchedJacobiMaxOrder)
+      by_padded[msq::batched_jacobi_padded_order(L.order[i])].push_back(i);
+  std::vector<std::pair<size_t, uint32_t>> launches;  // (first job, padded order)
+  // Largest orders first: their workgroups are the longest-running.
+  for (auto it = by_padded.rbegin(); it != by_padded.rend(); ++it) {
+    launches.push_back({jobs.size(), it->first});
+    for (uint32_t i : it->second) jobs.push_back(make_job(L, i, opts[opts_count == 1 ? 0 : i]));
   }
-  begin[kJacobiClassCount] = jobs.size();
+  launches.push_back({jobs.size(), 0});
   if (!jobs.empty()) {
     MSQ_TRY(upload(rt, djobs, jobs));
-    for (size_t c = 0; c < kJacobiClassCount; ++c) {
-      const size_t b = begin[c], e = begin[c + 1];
-      if (b == e) continue;
+    for (size_t c = 0; c + 1 < launches.size(); ++c) {
+      const size_t b = launches[c].first, e = launches[c + 1].first;
       HIP_TRY(msq::launch_batched_jacobi(djobs + b, static_cast<uint32_t>(e - b),
-                                         kJacobiClasses[c].max_order, kJacobiClasses[c].v_in_lds,
-                                         kJacobiClasses[c].threads, a64, V, diag, roots, jstatus,
+                                         launches[c].second, a64, V, diag, roots, jstatus,
                                          rt->stream));
     }
   }

@@ -64,11 +64,14 @@ struct JacobiStatus {
   float off_history[kJacobiHistory];  // off(A)/||A||_F after each sweep
 };
 
-size_t batched_jacobi_lds_bytes(uint32_t order, bool v_in_lds, int threads);
-hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t max_order,
-                                 bool v_in_lds, int threads, const double* a64,
-                                 float* vectors, float* diagonal, float* roots,
-                                 JacobiStatus* status, hipStream_t stream);
+// One launch covers matrices sharing the padded order (order rounded up to
+// 16: the launch's block size and LDS size follow it).
+uint32_t batched_jacobi_padded_order(uint32_t order);
+int batched_jacobi_threads(uint32_t order);
+size_t batched_jacobi_lds_bytes(uint32_t order);
+hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t order,
+                                 const double* a64, float* vectors, float* diagonal,
+                                 float* roots, JacobiStatus* status, hipStream_t stream);
 
 hipError_t launch_global_jacobi_init(const double* a64, float* A, float* V, uint32_t n,
                                      uint32_t ld, hipStream_t stream);

@@ -37,6 +37,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace msq {
@@ -118,38 +120,287 @@ __device__ __forceinline__ float wave_min(float v) {
   return v;
 }
 
-template <int NT, bool V_IN_LDS>
-__global__ __launch_bounds__(NT) void batched_jacobi_kernel(
+// Launch shape for a padded order N (a multiple of 16 up to 256).
+//
+// V rows are split into H parts of S slots on adjacent lanes; a lane holds
+// the 2 S positions of its slots ("lo" = k, "hi" = N - 1 - k).
+//  * N <= 128: S = 8, H = N / 16.  V stays in registers for the whole solve
+//    (one row per H lanes, all rows at once).
+//  * N > 128: S = N / 16 (9..16), H = 8.  V lives in device memory between
+//    groups of rounds; a group is applied in two passes of N / 2 rows, so a
+//    lane never holds more than 32 V values.
+__host__ __device__ constexpr int slots_for_order(int N) { return N <= 128 ? 8 : N / 16; }
+__host__ __device__ constexpr bool v_in_memory(int S) { return S != 8; }
+__host__ __device__ constexpr int parts_for_order(int N) { return N <= 128 ? N / 16 : 8; }
+// Rounds buffered before V is updated: a whole phase cycle (S) for the
+// register-resident V, 16 for V in device memory.
+__host__ __device__ constexpr int group_for_slots(int S) { return S == 8 ? 8 : 16; }
+// History row stride per part: odd, so the H parts' broadcast reads of one
+// slot fall in distinct LDS banks.
+__host__ __device__ constexpr int padded_slots(int S) { return S | 1; }
+// Upper bound of off-diagonal 2x2 blocks per thread (m (m - 1) / 2 blocks
+// over the launch's threads; checked on the host).
+__host__ __device__ constexpr int blocks_per_thread(int S) {
+  return S == 8 ? 2 : (8 * S - 1 + 15) / 16;
+}
+
+__host__ __device__ constexpr int threads_for_order(int N) {
+  return N <= 128 ? ((N + (64 / (N / 16)) - 1) / (64 / (N / 16))) * 64 : 4 * N;
+}
+
+// Lane shifts inside a V row (8 adjacent lanes, inside one 16-lane DPP row):
+// value of lane - 1 (row_shr:1) and of lane + 1 (row_shl:1).  Lanes whose
+// source lies outside their DPP row get 0 and never use it.
+__device__ __forceinline__ float from_lane_below(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_lane_above(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xf, 0xf, false));
+}
+
+// One round of V updates at phase PHI = round mod S (register-resident V).
+// Register r holds the lo value of slot (r + PHI) mod S of this part and hi[r]
+// the hi value of slot (r - PHI) mod S: values stay in their registers while
+// the round-robin moves positions, and only the mapping advances with the
+// phase.  After the rotations one lo value leaves for the next part (one hi
+// value for the previous part) and the registers they free take the arriving
+// values.
+template <int S, int PHI>
+__device__ __forceinline__ void v_round(float (&lo)[S], float (&hi)[S],
+                                        const float2* __restrict__ rotv, int h, int H,
+                                        int lane) {
+#pragma unroll
+  for (int r0 = 0; r0 < S; r0 += 4) {
+    asm volatile("" ::: "memory");
+    float2 r4[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (r0 + u < S) r4[u] = rotv[h * padded_slots(S) + (r0 + u + PHI) % S];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int rl = r0 + u;
+      if (rl < S) {
+        const int rh = (rl + 2 * PHI) % S;
+        const float x = lo[rl], y = hi[rh];
+        lo[rl] = r4[u].x * x + r4[u].y * y;
+        hi[rh] = r4[u].x * y - r4[u].y * x;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  constexpr int r_last_lo = ((S - 1 - PHI) % S + S) % S;  // slot S-1's lo
+  constexpr int r_first_lo = ((0 - PHI) % S + S) % S;     // slot 0's lo
+  constexpr int r_first_hi = PHI % S;                     // slot 0's hi
+  const float out_up = lo[r_last_lo];
+  const float out_dn = hi[r_first_hi];
+  const float pos0 = lo[r_first_lo];
+  const float in_up = __shfl(out_up, lane > 0 ? lane - 1 : 0);
+  const float in_dn = __shfl(out_dn, lane < 63 ? lane + 1 : 63);
+  // Selects, not branches (h differs between lanes): part 0 keeps position 0
+  // (slot 0's lo next round) and moves position N - 1 to 1; the last part
+  // turns its last lo into its last hi.
+  const bool first = h == 0;
+  lo[r_last_lo] = first ? pos0 : in_up;
+  lo[r_first_lo] = first ? out_dn : lo[r_first_lo];
+  hi[r_first_hi] = h == H - 1 ? out_up : in_dn;
+  // Round boundary: pin every V value to its register (empty asm, no
+  // instruction), so the next round's arithmetic cannot overlap this one's
+  // and the live set stays 2 S values plus one round's temporaries.
+#pragma unroll
+  for (int r = 0; r < S; ++r) asm volatile("" : "+v"(lo[r]), "+v"(hi[r]));
+}
+
+// Applies G buffered rounds to V, starting at phase P0; each round's
+// (c, s') per slot is one row of `hist` (`row` entries apart).  Straight-line
+// code: rounds a group did not reach are identity rotations in `hist`.
+template <int S, int G, int P0, int I = 0>
+__device__ __forceinline__ void v_group(float (&lo)[S], float (&hi)[S],
+                                        const float2* __restrict__ hist, int row, int h, int H,
+                                        int lane) {
+  if constexpr (I < G) {
+    v_round<S, (P0 + I) % S>(lo, hi, hist + I * row, h, H, lane);
+    v_group<S, G, P0, I + 1>(lo, hi, hist, row, h, H, lane);
+  }
+}
+
+// One round on V rows held slot-aligned (8 parts per row): lo[j] / hi[j] are
+// slot j's two positions, rotated together, then shifted one position along
+// the round-robin circle (straight-line code turns the shifts into register
+// renames; the caller stores every register back to its own position).
+template <int S>
+__device__ __forceinline__ void v_round_shift(float (&lo)[S], float (&hi)[S],
+                                              const float2* __restrict__ rotv, int h) {
+#pragma unroll
+  for (int j0 = 0; j0 < S; j0 += 4) {
+    float2 r4[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (j0 + u < S) r4[u] = rotv[h * padded_slots(S) + j0 + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = j0 + u;
+      if (j < S) {
+        const float x = lo[j], y = hi[j];
+        lo[j] = r4[u].x * x + r4[u].y * y;
+        hi[j] = r4[u].x * y - r4[u].y * x;
+      }
+    }
+  }
+  const float out_up = lo[S - 1];  // to the next part's lo[0] (last part: own hi[S-1])
+  const float out_dn = hi[0];      // to the previous part's hi[S-1] (part 0: own lo[1])
+  const float keep0 = lo[0];
+  const float in_up = from_lane_below(out_up);
+  const float in_dn = from_lane_above(out_dn);
+#pragma unroll
+  for (int j = S - 1; j >= 2; --j) lo[j] = lo[j - 1];
+  const bool first = h == 0;
+  lo[1] = first ? out_dn : lo[0];
+  lo[0] = first ? keep0 : in_up;
+#pragma unroll
+  for (int j = 0; j < S - 1; ++j) hi[j] = hi[j + 1];
+  hi[S - 1] = h == 7 ? out_up : in_dn;
+  // Round boundary (empty asm, no instruction): V values pinned to registers
+  // and memory accesses kept on their side, so rounds do not overlap.
+#pragma unroll
+  for (int r = 0; r < S; ++r) asm volatile("" : "+v"(lo[r]), "+v"(hi[r]));
+  asm volatile("" ::: "memory");
+}
+
+template <int S, int G, int I = 0>
+__device__ __forceinline__ void v_group_shift(float (&lo)[S], float (&hi)[S],
+                                              const float2* __restrict__ hist, int row, int h) {
+  if constexpr (I < G) {
+    v_round_shift<S>(lo, hi, hist + I * row, h);
+    v_group_shift<S, G, I + 1>(lo, hi, hist, row, h);
+  }
+}
+
+// Packed-triangle element index from the two participants and their column
+// bases (cb_x = x (x + 1) / 2).
+__device__ __forceinline__ int packed_index(int x, int cbx, int y, int cby) {
+  return x < y ? cby + x : cbx + y;
+}
+
+// Rotates U off-diagonal blocks (k < l) in place.  All loads are issued before
+// any store (the blocks are disjoint, which the compiler cannot know); entries
+// of `code` equal to ~0 are padding: they load a real block and store nothing.
+template <int U>
+__device__ __forceinline__ void rotate_blocks(float* __restrict__ Ap,
+                                              const int4* __restrict__ sinfo,
+                                              const float2* __restrict__ rot,
+                                              const uint32_t* code) {
+  int i00[U], i01[U], i10[U], i11[U];
+  Block blk[U];
+  float2 rk[U], rl[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint32_t c = code[u] == ~0u ? (1u << 16) : code[u];
+    const int k = static_cast<int>(c & 0xffffu), l = static_cast<int>(c >> 16);
+    const int4 sk = sinfo[k], sl = sinfo[l];  // (p, q, cb_p, cb_q)
+    rk[u] = rot[k];
+    rl[u] = rot[l];
+    i00[u] = packed_index(sk.x, sk.z, sl.x, sl.z);
+    i01[u] = packed_index(sk.x, sk.z, sl.y, sl.w);
+    i10[u] = packed_index(sk.y, sk.w, sl.x, sl.z);
+    i11[u] = packed_index(sk.y, sk.w, sl.y, sl.w);
+    blk[u].a00 = Ap[i00[u]];
+    blk[u].a01 = Ap[i01[u]];
+    blk[u].a10 = Ap[i10[u]];
+    blk[u].a11 = Ap[i11[u]];
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const float2 rpk = make_float2(rk[u].x, -rk[u].y);  // seen by p
+    const float2 rpl = make_float2(rl[u].x, -rl[u].y);
+    const Block o = rotate_block(blk[u], rpk, rk[u], rpl, rl[u]);
+    if (code[u] != ~0u) {
+      Ap[i00[u]] = o.a00;
+      Ap[i01[u]] = o.a01;
+      Ap[i10[u]] = o.a10;
+      Ap[i11[u]] = o.a11;
+    }
+  }
+}
+
+// One workgroup per matrix, orders up to 256.
+//
+// A: packed upper triangle in LDS, indexed by matrix index, rotated in place.
+// A round is two phases.  Phase 1: one thread per slot k computes the
+// rotation of its pair from a_pp, a_qq, a_pq and rotates the diagonal block
+// {p, q} x {p, q} (no other block touches it).  Phase 2: every off-diagonal
+// block {p_k, q_k} x {p_l, q_l} (k < l) is read, rotated on both sides and
+// written back by one thread; the blocks of a round are disjoint, so there is
+// no second buffer.  Each thread's blocks are fixed for the whole solve.
+//
+// V: slot-aligned rows.  Thread (row i, part h) holds V[i][x] for the
+// positions x of slots k in [h S, (h + 1) S): "lo" = position k, "hi" =
+// position N - 1 - k.  Slot k pairs positions k and N - 1 - k in every round,
+// so each rotation is local to a thread.  Between rounds the round-robin
+// moves every position x >= 1 to x + 1 (N - 1 to 1): lo shifts up, hi shifts
+// down, and one value crosses to each neighbouring part of the row (adjacent
+// lanes).  The rotations of a group of rounds are buffered in LDS and applied
+// to V after the group.
+template <int S>
+__global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     const JacobiJob* __restrict__ jobs, const double* __restrict__ a64,
     float* __restrict__ vectors, float* __restrict__ diagonal,
     float* __restrict__ roots, JacobiStatus* __restrict__ status) {
+  constexpr bool VMEM = v_in_memory(S);
+  constexpr int G = group_for_slots(S);
+  constexpr int SP = padded_slots(S);
+  constexpr int BPT = blocks_per_thread(S);
   const JacobiJob job = jobs[blockIdx.x];
   const int tid = threadIdx.x;
+  const int nt = blockDim.x;
   const int n = static_cast<int>(job.order);
   const int ld = static_cast<int>(job.ld);
-  const int np = n + (n & 1);
-  const int m = np / 2;
-  const int packed = np * (np + 1) / 2;
-  const int blocks = m * (m + 1) / 2;
+  const int N = (n + 15) & ~15;  // padded order: zero rows/columns beyond n
+  const int m = N / 2;
+  const int H = parts_for_order(N);
+  // History entries per round: room for 8 parts whatever H is, so every
+  // history offset is a compile-time constant from one base address.
+  constexpr int hrow = 8 * SP;
+  const int rows_per_wave = 64 / H;
+  const int packed = N * (N + 1) / 2;
+  const int offblocks = m * (m - 1) / 2;
 
   extern __shared__ __align__(16) unsigned char smem[];
   float* Ap = reinterpret_cast<float*>(smem);
-  float* Vl = Ap + ((packed + 3) & ~3);
-  int* slot_p = reinterpret_cast<int*>(Vl + (V_IN_LDS ? n * np : 0));
-  int* slot_q = slot_p + 128;
-  float2* rot = reinterpret_cast<float2*>(slot_q + 128);
-  double* red = reinterpret_cast<double*>(rot + 128);  // 2 * NT / 64
-  float* redf = reinterpret_cast<float*>(red + 2 * (NT / 64));
-  int* ctrl = reinterpret_cast<int*>(redf + NT / 64);
+  int4* sinfo = reinterpret_cast<int4*>(Ap + ((packed + 3) & ~3));  // (p, q, cb_p, cb_q)
+  float2* rot = reinterpret_cast<float2*>(sinfo + 128);  // (c, s), p < q
+  float2* hist = rot + 128;  // G rounds x H parts x SP of (c, s') for V
+  double* red = reinterpret_cast<double*>(hist + G * hrow);  // 32
+  float* redf = reinterpret_cast<float*>(red + 32);         // 16
+  int* ctrl = reinterpret_cast<int*>(redf + 16);            // 4
+
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int h = lane % H;
+  // Register-resident V: one row per H lanes; lanes beyond the last whole
+  // row of a wave idle in the V work but take part in everything else.
+  const int row = wave * rows_per_wave + lane / H;
+  const bool vlane = lane / H < rows_per_wave && row < N;
+
+  // This thread's off-diagonal blocks, (k | l << 16), row-major over k < l.
+  uint32_t code[BPT];
+#pragma unroll
+  for (int i = 0; i < BPT; ++i) {
+    const int b = tid + i * nt;
+    code[i] = ~0u;
+    if (b < offblocks) {
+      int k = 0, rem = b;
+      while (rem >= m - 1 - k) {
+        rem -= m - 1 - k;
+        ++k;
+      }
+      code[i] = static_cast<uint32_t>(k) | (static_cast<uint32_t>(k + 1 + rem) << 16);
+    }
+  }
 
   const double* A = a64 + job.a_offset;
-  float* Vg = vectors + job.v_offset;
-  float* V = V_IN_LDS ? Vl : Vg;
-  const int vld = V_IN_LDS ? n : ld;
-
-  // Load A (binary64 -> float, as the reference does) and its FP64 norm.
+  if (tid == 0) red[0] = 0.0;
   double norm2 = 0.0;
-  for (int e = tid; e < packed; e += NT) {
+  for (int e = tid; e < packed; e += nt) {
     const int j = tri_col(e);
     const int i = e - (j * (j + 1)) / 2;
     float v = 0.0f;
@@ -160,105 +411,144 @@ __global__ __launch_bounds__(NT) void batched_jacobi_kernel(
     }
     Ap[e] = v;
   }
-  // V = I over n x np (the dummy column of an odd order stays zero); in device
-  // memory the whole ld x ld tile is written so the padding is zero.
-  if (V_IN_LDS) {
-    for (int e = tid; e < n * np; e += NT) {
-      const int i = e % n;
-      const int x = e / n;
-      Vl[e] = i == x ? 1.0f : 0.0f;
+  float* Vg = vectors + job.v_offset;
+  // Register-resident V (VMEM false) starts as the identity; the device-memory
+  // copy (VMEM true) holds V in position order: column x = position x.
+  float lo[VMEM ? 1 : S], hi[VMEM ? 1 : S];
+  if constexpr (!VMEM) {
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const int k = h * S + j;
+      lo[j] = (vlane && row == k) ? 1.0f : 0.0f;
+      hi[j] = (vlane && row == N - 1 - k) ? 1.0f : 0.0f;
     }
   } else {
-    for (int e = tid; e < ld * ld; e += NT) {
-      const int i = e % ld;
-      const int x = e / ld;
-      Vg[e] = (i == x && i < n) ? 1.0f : 0.0f;
+    for (int e = tid; e < N * N; e += nt) {
+      const int i = e % N;
+      const int x = e / N;
+      Vg[i + static_cast<size_t>(x) * ld] = i == x ? 1.0f : 0.0f;
     }
   }
-  norm2 = wave_sum(norm2);
-  if ((tid & 63) == 0) red[tid / 64] = norm2;
+  // Block reductions go through LDS atomics: wave shuffles would keep their
+  // lane-address registers live through the whole kernel.
   __syncthreads();
-  if (tid == 0) {
-    double s = 0.0;
-    for (int w = 0; w < NT / 64; ++w) s += red[w];
-    red[2 * (NT / 64) - 1] = s;  // stash
-  }
+  atomicAdd(&red[0], norm2);
   __syncthreads();
-  const double matrix_norm = sqrt(red[2 * (NT / 64) - 1]);
-  __syncthreads();
+  const double matrix_norm = sqrt(red[0]);
 
   const uint32_t fixed = job.fixed_sweeps;
   const uint32_t cap = fixed ? fixed : job.max_sweeps;
+  const int rounds_per_sweep = N - 1;
+  const int limit = static_cast<int>(cap) * rounds_per_sweep;
   double previous = INFINITY;
   double off = 0.0;
   float min_diag = 0.0f;
   uint32_t sweeps = 0;
+  int t = 0;   // rounds applied to A
+  int tv = 0;  // rounds applied to V (whole groups; t rounded up to G)
   JacobiStatus* st = status + job.status_index;
 
-  for (uint32_t sweep = 0; sweep < cap; ++sweep) {
-    for (int t = 0; t < np - 1; ++t) {
-      for (int k = tid; k < m; k += NT) {
-        const int a = participant(t, k, np);
-        const int b = participant(t, np - 1 - k, np);
+  // One group: up to G rounds on A (rotation parameters buffered), the G
+  // buffered rounds on V, then the convergence test if a sweep finished.
+  // Returns true when the solve stops.
+  auto group = [&]() -> bool {
+    const int group_start = t;
+    const int group_end = min(t + G, limit);
+#pragma unroll 1
+    for (; t < group_end; ++t) {
+      const int round = t % rounds_per_sweep;
+      float2* rv = hist + (t - group_start) * hrow;
+      for (int k = tid; k < m; k += nt) {
+        const int a = participant(round, k, N);
+        const int b = participant(round, N - 1 - k, N);
         const int p = a < b ? a : b;
         const int q = a < b ? b : a;
-        slot_p[k] = p;
-        slot_q[k] = q;
-        rot[k] = rotation_angles(Ap[tri(p, p)], Ap[tri(q, q)], Ap[tri(p, q)]);
+        const int cbp = (p * (p + 1)) >> 1;
+        const int cbq = (q * (q + 1)) >> 1;
+        const int ipp = cbp + p, ipq = cbq + p, iqq = cbq + q;
+        Block blk;
+        blk.a00 = Ap[ipp];
+        blk.a01 = Ap[ipq];
+        blk.a10 = blk.a01;
+        blk.a11 = Ap[iqq];
+        const float2 r = rotation_angles(blk.a00, blk.a11, blk.a01);
+        const float2 rp = make_float2(r.x, -r.y);
+        const Block o = rotate_block(blk, rp, r, rp, r);
+        Ap[ipp] = o.a00;
+        Ap[ipq] = o.a01;
+        Ap[iqq] = o.a11;
+        sinfo[k] = make_int4(p, q, cbp, cbq);
+        rot[k] = r;
+        // position k holds a: it is p (sees (c, -s)) when a < b
+        rv[(k / S) * SP + k % S] = make_float2(r.x, a < b ? -r.y : r.y);
       }
       __syncthreads();
-      for (int b = tid; b < blocks; b += NT) {
-        int k, l;
-        block_slots(b, m, k, l);
-        const int pk = slot_p[k], qk = slot_q[k];
-        const float2 rk = rot[k];
-        const float2 rpk = make_float2(rk.x, -rk.y);  // seen by p
-        if (k == l) {
-          Block blk;
-          blk.a00 = Ap[tri(pk, pk)];
-          blk.a01 = Ap[tri(pk, qk)];
-          blk.a10 = blk.a01;
-          blk.a11 = Ap[tri(qk, qk)];
-          const Block o = rotate_block(blk, rpk, rk, rpk, rk);
-          Ap[tri(pk, pk)] = o.a00;
-          Ap[tri(pk, qk)] = o.a01;
-          Ap[tri(qk, qk)] = o.a11;
+#pragma unroll
+      for (int i = 0; i < BPT; i += 2) {
+        if (i + 1 < BPT) {
+          rotate_blocks<2>(Ap, sinfo, rot, code + i);
         } else {
-          const int pl = slot_p[l], ql = slot_q[l];
-          const float2 rl = rot[l];
-          const float2 rpl = make_float2(rl.x, -rl.y);
-          const int i00 = tri_sym(pk, pl), i01 = tri_sym(pk, ql);
-          const int i10 = tri_sym(qk, pl), i11 = tri_sym(qk, ql);
-          Block blk;
-          blk.a00 = Ap[i00];
-          blk.a01 = Ap[i01];
-          blk.a10 = Ap[i10];
-          blk.a11 = Ap[i11];
-          const Block o = rotate_block(blk, rpk, rk, rpl, rl);
-          Ap[i00] = o.a00;
-          Ap[i01] = o.a01;
-          Ap[i10] = o.a10;
-          Ap[i11] = o.a11;
+          rotate_blocks<1>(Ap, sinfo, rot, code + i);
         }
-      }
-      for (int item = tid; item < n * m; item += NT) {
-        const int i = item % n;
-        const int k = item / n;
-        const int p = slot_p[k], q = slot_q[k];
-        const float2 r = rot[k];
-        float* vp = V + i + static_cast<size_t>(p) * vld;
-        float* vq = V + i + static_cast<size_t>(q) * vld;
-        const float x = *vp, y = *vq;
-        *vp = r.x * x + (-r.y) * y;
-        *vq = r.x * y + r.y * x;
       }
       __syncthreads();
     }
-    ++sweeps;
-    // FP64 off-diagonal norm and the smallest diagonal entry.
+    // Rounds the group did not reach: identity rotations (V's positions still
+    // advance; the output maps them through V's own round count).
+    for (int e = (group_end - group_start) * hrow + tid; e < G * hrow; e += nt)
+      hist[e] = make_float2(1.0f, 0.0f);
+    __syncthreads();
+    if constexpr (!VMEM) {
+      // Every lane runs the V code (lanes past the last whole row of a wave
+      // compute on zeros that no valid lane reads): no divergent branch around
+      // the 2 S live V registers.  G = S: every group starts at phase 0.
+      v_group<S, G, 0>(lo, hi, hist, hrow, h, H, lane);
+    } else {
+      // Two passes of N / 2 rows (8 lanes per row).  Each lane loads its
+      // positions, applies the group's rounds with the values shifting along
+      // the circle in registers, and stores every register back to its own
+      // position: no other lane touches them, so no barrier in between.
+      // Element offsets from a value the compiler must treat as new in every
+      // group (empty asm): otherwise it hoists all 2 S addresses out of the
+      // loop and keeps them live (spilled) through the A update.
+#pragma unroll 1
+      for (int pass = 0; pass < 2; ++pass) {
+        const int vrow = pass * m + (tid >> 3);
+        const int vh = tid & 7;
+        uint32_t lo_off = static_cast<uint32_t>(vrow + vh * S * ld);
+        uint32_t hi_off = static_cast<uint32_t>(vrow + (N - 1 - vh * S) * ld);
+        uint32_t step = static_cast<uint32_t>(ld);
+        asm volatile("" : "+v"(lo_off), "+v"(hi_off), "+s"(step));
+        float vl[S], vhv[S];
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+          vl[j] = Vg[lo_off + j * step];
+          vhv[j] = Vg[hi_off - j * step];
+        }
+        v_group_shift<S, G>(vl, vhv, hist, hrow, vh);
+        asm volatile("" : "+v"(lo_off), "+v"(hi_off), "+s"(step));
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+          Vg[lo_off + j * step] = vl[j];
+          Vg[hi_off - j * step] = vhv[j];
+        }
+      }
+    }
+    tv += G;
+    __syncthreads();  // the next group overwrites the history (and V)
+
+    // Convergence, at the first group boundary after each finished sweep.
+    const bool at_limit = t >= limit;
+    if (!(t / rounds_per_sweep > static_cast<int>(sweeps) || at_limit)) return false;
+    sweeps = static_cast<uint32_t>(t / rounds_per_sweep);
     double off2 = 0.0;
     float dmin = INFINITY;
-    for (int e = tid; e < packed; e += NT) {
+    if (tid == 0) {
+      red[1] = 0.0;
+      redf[0] = INFINITY;
+    }
+    __syncthreads();
+    for (int e = tid; e < packed; e += nt) {
       const int j = tri_col(e);
       const int i = e - (j * (j + 1)) / 2;
       if (j >= n) continue;
@@ -269,64 +559,92 @@ __global__ __launch_bounds__(NT) void batched_jacobi_kernel(
         off2 += 2.0 * v * v;
       }
     }
-    off2 = wave_sum(off2);
-    dmin = wave_min(dmin);
-    if ((tid & 63) == 0) {
-      red[tid / 64] = off2;
-      redf[tid / 64] = dmin;
-    }
+    if (off2 != 0.0) atomicAdd(&red[1], off2);
+    if (dmin < INFINITY) atomicMin(&redf[0], dmin);
     __syncthreads();
     if (tid == 0) {
-      double s = 0.0;
-      float d = INFINITY;
-      for (int w = 0; w < NT / 64; ++w) {
-        s += red[w];
-        d = fminf(d, redf[w]);
-      }
-      off = sqrt(s);
+      const double s2 = red[1];
+      const float d = redf[0];
+      off = sqrt(s2);
       min_diag = d;
-      if (sweeps <= kJacobiHistory) st->off_history[sweeps - 1] = static_cast<float>(off / matrix_norm);
-      int stop = 0;
-      if (fixed) {
-        stop = sweeps >= fixed;
-      } else {
+      if (sweeps >= 1 && sweeps <= kJacobiHistory)
+        st->off_history[sweeps - 1] = static_cast<float>(off / matrix_norm);
+      int stop = at_limit ? 1 : 0;
+      if (!fixed) {
         double tol = static_cast<double>(job.off_tol) * matrix_norm;
         if (job.off_tol_lambda > 0.0f && d > 0.0f) {
           const double tl = static_cast<double>(job.off_tol_lambda) * d;
           tol = tl > tol ? tl : tol;
         }
-        stop = off <= tol || sweeps >= cap ||
+        stop = stop || off <= tol ||
                (off > static_cast<double>(job.stagnation) * previous &&
                 off <= static_cast<double>(job.floor_tol) * matrix_norm);
       }
       previous = off;
       ctrl[0] = stop;
-      ctrl[1] = *reinterpret_cast<const int*>(&d);
     }
     __syncthreads();
     const int stop = ctrl[0];
     __syncthreads();
-    if (stop) break;
-  }
+    return stop != 0;
+  };
 
-  // Outputs: diagonal, square roots, V (padding zero).
+#pragma unroll 1
+  while (!group()) {
+  }
+  static_assert(VMEM || G == S, "register-resident V needs whole phase cycles");
+
+  // Outputs: diagonal and square roots (ld entries), V (ld x ld, zero outside
+  // the order).
   float* dg = diagonal + job.s_offset;
   float* rt = roots + job.s_offset;
   int bad = 0;
-  for (int i = tid; i < ld; i += NT) {
+  for (int i = tid; i < ld; i += nt) {
     const float d = i < n ? Ap[tri(i, i)] : 0.0f;
     dg[i] = d;
     rt[i] = i < n ? sqrtf(d) : 0.0f;
     if (i < n && !(d > 0.0f && isfinite(d))) bad = 1;
   }
-  if (V_IN_LDS) {
-    for (int e = tid; e < ld * ld; e += NT) {
-      const int i = e % ld;
-      const int x = e / ld;
-      Vg[e] = (i < n && x < n) ? Vl[i + x * n] : 0.0f;
+  const int round = tv % rounds_per_sweep;
+  if constexpr (VMEM) {
+    // Positions -> matrix indices: position x of round tv holds P_tv(x).
+#pragma unroll 1
+    for (int pass = 0; pass < 2; ++pass) {
+      const int vrow = pass * m + (tid >> 3);
+      const int vh = tid & 7;
+      float vl[S], vhv[S];
+      const uint32_t base = static_cast<uint32_t>(vrow);
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        vl[j] = Vg[base + static_cast<uint32_t>((vh * S + j) * ld)];
+        vhv[j] = Vg[base + static_cast<uint32_t>((N - 1 - (vh * S + j)) * ld)];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        const int kl = participant(round, vh * S + j, N);
+        const int kh = participant(round, N - 1 - (vh * S + j), N);
+        Vg[base + static_cast<uint32_t>(kl * ld)] = vl[j];
+        Vg[base + static_cast<uint32_t>(kh * ld)] = vhv[j];
+      }
+      __syncthreads();
     }
-  } else if (n & 1) {
-    for (int i = tid; i < ld; i += NT) Vg[i + static_cast<size_t>(n) * ld] = 0.0f;
+  }
+  for (int e = tid; e < ld * ld; e += nt) {
+    const int i = e % ld;
+    const int x = e / ld;
+    if (i >= n || x >= n) Vg[e] = 0.0f;
+  }
+  if (!VMEM && vlane && row < n) {
+    // G = S, so V's phase is 0: register r holds slot h S + r; the position
+    // x of a slot holds matrix index P_t(x) of the round reached.
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+      const int kl = participant(round, h * S + r, N);
+      const int kh = participant(round, N - 1 - (h * S + r), N);
+      if (kl < n) Vg[row + static_cast<size_t>(kl) * ld] = lo[r];
+      if (kh < n) Vg[row + static_cast<size_t>(kh) * ld] = hi[r];
+    }
   }
   if (tid == 0) ctrl[2] = 0;
   __syncthreads();
@@ -484,58 +802,67 @@ __global__ void global_finish_kernel(const float* __restrict__ A, int n, int ld,
   roots[i] = i < n ? sqrtf(d) : 0.0f;
 }
 
-template <int NT, bool VL>
-hipError_t launch_batched_variant(const JacobiJob* jobs, uint32_t count,
-                                  const double* a64, float* vectors, float* diagonal,
-                                  float* roots, JacobiStatus* status, size_t lds,
-                                  hipStream_t stream) {
+template <int S>
+hipError_t launch_slots(const JacobiJob* jobs, uint32_t count, int threads, const double* a64,
+                        float* vectors, float* diagonal, float* roots, JacobiStatus* status,
+                        size_t lds, hipStream_t stream) {
   // Dynamic LDS above the default needs an explicit opt-in, sized to this
   // launch (static LDS plus the opt-in must stay within 160 KiB).
   static size_t configured = 0;
   if (lds > configured) {
-    hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&batched_jacobi_kernel<NT, VL>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&batched_jacobi_kernel<S>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(lds));
     if (e != hipSuccess) return e;
     configured = lds;
   }
-  batched_jacobi_kernel<NT, VL><<<count, NT, lds, stream>>>(jobs, a64, vectors, diagonal,
-                                                            roots, status);
+  batched_jacobi_kernel<S><<<count, threads, lds, stream>>>(jobs, a64, vectors, diagonal, roots,
+                                                            status);
   return hipGetLastError();
 }
 
 }  // namespace
 
-size_t batched_jacobi_lds_bytes(uint32_t order, bool v_in_lds, int threads) {
-  const size_t np = order + (order & 1u);
-  const size_t packed = np * (np + 1) / 2;
-  size_t bytes = ((packed + 3) & ~size_t{3}) * sizeof(float);
-  if (v_in_lds) bytes += static_cast<size_t>(order) * np * sizeof(float);
-  bytes += 2 * 128 * sizeof(int) + 128 * sizeof(float2);
-  bytes += 2 * (threads / 64) * sizeof(double) + (threads / 64) * sizeof(float);
-  bytes += 4 * sizeof(int);
-  return bytes;
+uint32_t batched_jacobi_padded_order(uint32_t order) { return (order + 15u) & ~15u; }
+
+int batched_jacobi_threads(uint32_t order) {
+  return threads_for_order(static_cast<int>(batched_jacobi_padded_order(order)));
 }
 
-hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count,
-                                 uint32_t max_order, bool v_in_lds, int threads,
+size_t batched_jacobi_lds_bytes(uint32_t order) {
+  const int N = static_cast<int>(batched_jacobi_padded_order(order));
+  const size_t packed = static_cast<size_t>(N) * (N + 1) / 2;
+  const int S = slots_for_order(N);
+  const size_t hist = static_cast<size_t>(group_for_slots(S)) * 8 * padded_slots(S);
+  return ((packed + 3) & ~size_t{3}) * sizeof(float) + 128 * sizeof(int4) +
+         (128 + hist) * sizeof(float2) + 32 * sizeof(double) + 16 * sizeof(float) +
+         4 * sizeof(int);
+}
+
+hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t order,
                                  const double* a64, float* vectors, float* diagonal,
-                                 float* roots, JacobiStatus* status,
-                                 hipStream_t stream) {
+                                 float* roots, JacobiStatus* status, hipStream_t stream) {
   if (count == 0) return hipSuccess;
-  if (max_order > kBatchedJacobiMaxOrder || (v_in_lds && max_order > 128))
+  if (order < 2 || order > kBatchedJacobiMaxOrder) return hipErrorInvalidValue;
+  const int N = static_cast<int>(batched_jacobi_padded_order(order));
+  const int S = slots_for_order(N);
+  const int threads = batched_jacobi_threads(order);
+  const size_t lds = batched_jacobi_lds_bytes(order);
+  const int m = N / 2;
+  const int offblocks = m * (m - 1) / 2;
+  // The kernel's shape assumptions: whole parts per row, at most 1024 lanes,
+  // every off-diagonal block in some thread's list, LDS within 160 KiB, and
+  // for V in device memory two passes of N / 2 rows of 8 lanes.
+  if (parts_for_order(N) * S != m || threads > 1024 || threads % 64 != 0 ||
+      (offblocks + threads - 1) / threads > blocks_per_thread(S) || lds > 160 * 1024 ||
+      (v_in_memory(S) && threads != 8 * m))
     return hipErrorInvalidValue;
-  const size_t lds = batched_jacobi_lds_bytes(max_order, v_in_lds, threads);
-  if (lds > 160 * 1024) return hipErrorInvalidValue;
-  if (v_in_lds) {
-    if (threads == 64) return launch_batched_variant<64, true>(jobs, count, a64, vectors, diagonal, roots, status, lds, stream);
-    if (threads == 128) return launch_batched_variant<128, true>(jobs, count, a64, vectors, diagonal, roots, status, lds, stream);
-    if (threads == 256) return launch_batched_variant<256, true>(jobs, count, a64, vectors, diagonal, roots, status, lds, stream);
-    if (threads == 512) return launch_batched_variant<512, true>(jobs, count, a64, vectors, diagonal, roots, status, lds, stream);
-  } else {
-    if (threads == 256) return launch_batched_variant<256, false>(jobs, count, a64, vectors, diagonal, roots, status, lds, stream);
-    if (threads == 512) return launch_batched_variant<512, false>(jobs, count, a64, vectors, diagonal, roots, status, lds, stream);
-    if (threads == 1024) return launch_batched_variant<1024, false>(jobs, count, a64, vectors, diagonal, roots, status, lds, stream);
+  switch (S) {
+#define MSQ_SLOTS(SV) \
+    case SV: return launch_slots<SV>(jobs, count, threads, a64, vectors, diagonal, roots, status, lds, stream);
+    MSQ_SLOTS(8) MSQ_SLOTS(9) MSQ_SLOTS(10) MSQ_SLOTS(11) MSQ_SLOTS(12) MSQ_SLOTS(13)
+    MSQ_SLOTS(14) MSQ_SLOTS(15) MSQ_SLOTS(16)
+#undef MSQ_SLOTS
   }
   return hipErrorInvalidValue;
 }

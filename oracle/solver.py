@@ -108,7 +108,7 @@ def _rotations(A, p, q, n, angle_dtype):
 
 
 def jacobi_f32(A, sweeps=8, off_tolerance=None, max_sweeps=None,
-               angle_dtype=F64, return_history=False):
+               angle_dtype=F64, return_history=False, pad_to=None):
     """Parallel cyclic Jacobi on a float32 copy of A.
 
     sweeps          -- fixed sweep count (the reference runs 8,
@@ -117,12 +117,26 @@ def jacobi_f32(A, sweeps=8, off_tolerance=None, max_sweeps=None,
                        is <= off_tolerance * ||A||_F (new: the reference has no
                        convergence test); at most max_sweeps sweeps.
 
+    pad_to          -- run the schedule of this (even) order, with zero rows
+                       and columns beyond n (the HIP library pads orders to a
+                       multiple of 16; for orders that are already multiples
+                       of 16 the schedule is the reference's).
+
     Returns (V, root_eigenvalues, info) with V float32 column-major-equivalent
     (V[:, j] is the j-th eigenvector) and root_eigenvalues = sqrt(diag) as in
     jacobi_root_eigenvalues_f32 (.metal:1436-1444), unsorted.
     """
     A = np.array(A, dtype=F32, copy=True)
-    n = A.shape[0]
+    n_real = A.shape[0]
+    if pad_to is not None and pad_to > n_real:
+        Ap = np.zeros((pad_to, pad_to), dtype=F32)
+        Ap[:n_real, :n_real] = A
+        V, roots, info = jacobi_f32(Ap, sweeps=sweeps, off_tolerance=off_tolerance,
+                                    max_sweeps=max_sweeps, angle_dtype=angle_dtype,
+                                    return_history=return_history)
+        info["diagonal"] = info["diagonal"][:n_real]
+        return V[:n_real, :n_real], roots[:n_real], info
+    n = n_real
     V = np.eye(n, dtype=F32)
     padded, partners, pair_indices, pair_members = jacobi_schedule(n)
     rows = np.arange(n)

@@ -156,7 +156,10 @@ def test_jacobi_fixed_sweeps_parity(runtime):
     for A, V, d, rt, rep in zip(mats, Vs, Ds, Rts, reps):
         n = A.shape[0]
         assert rep.sweeps == 8 and not rep.invalid_spectrum
-        Vo, ro, info = S.jacobi_f32(A, sweeps=8)
+        # The library pads orders to a multiple of 16 (zero rows beyond n);
+        # the oracle runs the same schedule (for n % 16 == 0 it is the
+        # reference's own).
+        Vo, ro, info = S.jacobi_f32(A, sweeps=8, pad_to=(n + 15) // 16 * 16)
         # Hoffman-Wielandt: the sorted diagonal of an orthogonal similarity of A
         # is within its off-diagonal norm of the eigenvalues; the FP32 iterates
         # also carry the rounding of S (n - 1) rounds, ~ eps ||A||_F sqrt(S n).
