@@ -275,10 +275,14 @@ __device__ __forceinline__ void v_group_shift(float (&lo)[S], float (&hi)[S],
   }
 }
 
-// Packed-triangle element index from the two participants and their column
-// bases (cb_x = x (x + 1) / 2).
-__device__ __forceinline__ int packed_index(int x, int cbx, int y, int cby) {
-  return x < y ? cby + x : cbx + y;
+// Byte offset of the packed-triangle entry (x, y), x != y, from the
+// byte-scaled participants and column bases (cb(x) = x (x + 1) / 2): cb is
+// increasing, so the larger participant has the larger base.
+__device__ __forceinline__ int packed_offset(int x4, int cb4x, int y4, int cb4y) {
+  return max(cb4x, cb4y) + min(x4, y4);
+}
+__device__ __forceinline__ float& lds_at(float* base, int byte_offset) {
+  return *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_offset);
 }
 
 // Rotates U off-diagonal blocks (k < l) in place.  All loads are issued before
@@ -296,17 +300,17 @@ __device__ __forceinline__ void rotate_blocks(float* __restrict__ Ap,
   for (int u = 0; u < U; ++u) {
     const uint32_t c = code[u] == ~0u ? (1u << 16) : code[u];
     const int k = static_cast<int>(c & 0xffffu), l = static_cast<int>(c >> 16);
-    const int4 sk = sinfo[k], sl = sinfo[l];  // (p, q, cb_p, cb_q)
+    const int4 sk = sinfo[k], sl = sinfo[l];  // 4 (p, q, cb_p, cb_q)
     rk[u] = rot[k];
     rl[u] = rot[l];
-    i00[u] = packed_index(sk.x, sk.z, sl.x, sl.z);
-    i01[u] = packed_index(sk.x, sk.z, sl.y, sl.w);
-    i10[u] = packed_index(sk.y, sk.w, sl.x, sl.z);
-    i11[u] = packed_index(sk.y, sk.w, sl.y, sl.w);
-    blk[u].a00 = Ap[i00[u]];
-    blk[u].a01 = Ap[i01[u]];
-    blk[u].a10 = Ap[i10[u]];
-    blk[u].a11 = Ap[i11[u]];
+    i00[u] = packed_offset(sk.x, sk.z, sl.x, sl.z);
+    i01[u] = packed_offset(sk.x, sk.z, sl.y, sl.w);
+    i10[u] = packed_offset(sk.y, sk.w, sl.x, sl.z);
+    i11[u] = packed_offset(sk.y, sk.w, sl.y, sl.w);
+    blk[u].a00 = lds_at(Ap, i00[u]);
+    blk[u].a01 = lds_at(Ap, i01[u]);
+    blk[u].a10 = lds_at(Ap, i10[u]);
+    blk[u].a11 = lds_at(Ap, i11[u]);
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
@@ -314,23 +318,46 @@ __device__ __forceinline__ void rotate_blocks(float* __restrict__ Ap,
     const float2 rpl = make_float2(rl[u].x, -rl[u].y);
     const Block o = rotate_block(blk[u], rpk, rk[u], rpl, rl[u]);
     if (code[u] != ~0u) {
-      Ap[i00[u]] = o.a00;
-      Ap[i01[u]] = o.a01;
-      Ap[i10[u]] = o.a10;
-      Ap[i11[u]] = o.a11;
+      lds_at(Ap, i00[u]) = o.a00;
+      lds_at(Ap, i01[u]) = o.a01;
+      lds_at(Ap, i10[u]) = o.a10;
+      lds_at(Ap, i11[u]) = o.a11;
     }
   }
+}
+
+// Off-diagonal blocks (k < l) whose rotated entries complete the next round's
+// pairs.  Round t + 1 pairs, at slot k', the participants that round t holds
+// at positions k' - 1 and N - 2 - k' (the round-robin moves every position
+// x >= 1 to x + 1, N - 1 to 1); as slots of round t these are
+//   k' = 0:           lo(0), hi(1)        -> block (0, 1)
+//   1 <= k' <= m - 2: lo/hi(k' - 1), hi(k' + 1) -> block (k' - 1, k' + 1)
+//   k' = m - 1:       lo(m - 2), lo(m - 1) -> block (m - 2, m - 1)
+// (for k' = 1 position 0 is fixed and position N - 1 arrives: hi(0)).
+__device__ __forceinline__ uint32_t next_pair_block(int kp, int m) {
+  const int k = kp == 0 ? 0 : (kp == m - 1 ? m - 2 : kp - 1);
+  const int l = kp == 0 ? 1 : (kp == m - 1 ? m - 1 : kp + 1);
+  return static_cast<uint32_t>(k) | (static_cast<uint32_t>(l) << 16);
+}
+__device__ __forceinline__ bool is_next_pair_block(int k, int l, int m) {
+  return l == k + 2 || (k == 0 && l == 1) || (k == m - 2 && l == m - 1);
 }
 
 // One workgroup per matrix, orders up to 256.
 //
 // A: packed upper triangle in LDS, indexed by matrix index, rotated in place.
-// A round is two phases.  Phase 1: one thread per slot k computes the
+// A round is two phases.  Pair phase: one thread per slot k computes the
 // rotation of its pair from a_pp, a_qq, a_pq and rotates the diagonal block
-// {p, q} x {p, q} (no other block touches it).  Phase 2: every off-diagonal
-// block {p_k, q_k} x {p_l, q_l} (k < l) is read, rotated on both sides and
-// written back by one thread; the blocks of a round are disjoint, so there is
-// no second buffer.  Each thread's blocks are fixed for the whole solve.
+// {p, q} x {p, q} (no other block touches it).  Block phase: every
+// off-diagonal block {p_k, q_k} x {p_l, q_l} (k < l) is read, rotated on both
+// sides and written back by one thread; the blocks of a round are disjoint,
+// so there is no second buffer.  Each thread's blocks are fixed for the whole
+// solve.  Inside a group of rounds the pair phase of round t + 1 runs in the
+// block phase of round t: thread k' owns the block that completes round
+// t + 1's pair at slot k' (next_pair_block), rotates it first, then computes
+// the next rotation and rotates the next diagonal block, which no other thread
+// of round t touches.  Slot data is double-buffered by round parity, so a
+// round costs one barrier.
 //
 // V: slot-aligned rows.  Thread (row i, part h) holds V[i][x] for the
 // positions x of slots k in [h S, (h + 1) S): "lo" = position k, "hi" =
@@ -366,9 +393,10 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
 
   extern __shared__ __align__(16) unsigned char smem[];
   float* Ap = reinterpret_cast<float*>(smem);
-  int4* sinfo = reinterpret_cast<int4*>(Ap + ((packed + 3) & ~3));  // (p, q, cb_p, cb_q)
-  float2* rot = reinterpret_cast<float2*>(sinfo + 128);  // (c, s), p < q
-  float2* hist = rot + 128;  // G rounds x H parts x SP of (c, s') for V
+  // Per slot, two buffers (round parity): 4 (p, q, cb_p, cb_q) and (c, s), p < q.
+  int4* sinfo = reinterpret_cast<int4*>(Ap + ((packed + 3) & ~3));
+  float2* rot = reinterpret_cast<float2*>(sinfo + 256);
+  float2* hist = rot + 256;  // G rounds x 8 parts x SP of (c, s') for V
   double* red = reinterpret_cast<double*>(hist + G * hrow);  // 32
   float* redf = reinterpret_cast<float*>(red + 32);         // 16
   int* ctrl = reinterpret_cast<int*>(redf + 16);            // 4
@@ -381,19 +409,24 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   const int row = wave * rows_per_wave + lane / H;
   const bool vlane = lane / H < rows_per_wave && row < N;
 
-  // This thread's off-diagonal blocks, (k | l << 16), row-major over k < l.
-  uint32_t code[BPT];
+  // This thread's off-diagonal blocks, (k | l << 16): first the block that
+  // completes the next round's pair at slot tid (threads below m), then the
+  // other blocks, row-major over k < l, dealt round-robin.
+  uint32_t code[BPT + 1];
+  code[0] = tid < m ? next_pair_block(tid, m) : ~0u;
 #pragma unroll
   for (int i = 0; i < BPT; ++i) {
     const int b = tid + i * nt;
-    code[i] = ~0u;
+    code[i + 1] = ~0u;
     if (b < offblocks) {
       int k = 0, rem = b;
       while (rem >= m - 1 - k) {
         rem -= m - 1 - k;
         ++k;
       }
-      code[i] = static_cast<uint32_t>(k) | (static_cast<uint32_t>(k + 1 + rem) << 16);
+      const int l = k + 1 + rem;
+      if (!is_next_pair_block(k, l, m))
+        code[i + 1] = static_cast<uint32_t>(k) | (static_cast<uint32_t>(l) << 16);
     }
   }
 
@@ -451,46 +484,54 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   // One group: up to G rounds on A (rotation parameters buffered), the G
   // buffered rounds on V, then the convergence test if a sweep finished.
   // Returns true when the solve stops.
+  // Pair phase of absolute round tt at slot k: rotation of the pair, the
+  // rotated diagonal block, slot data into buffer tt & 1, V's (c, s') into
+  // history row rv.
+  auto pair_phase = [&](int k, int tt, float2* rv) {
+    const int round = tt % rounds_per_sweep;
+    const int a = participant(round, k, N);
+    const int b = participant(round, N - 1 - k, N);
+    const int p = a < b ? a : b;
+    const int q = a < b ? b : a;
+    const int cbp = (p * (p + 1)) >> 1;
+    const int cbq = (q * (q + 1)) >> 1;
+    const int ipp = cbp + p, ipq = cbq + p, iqq = cbq + q;
+    Block blk;
+    blk.a00 = Ap[ipp];
+    blk.a01 = Ap[ipq];
+    blk.a10 = blk.a01;
+    blk.a11 = Ap[iqq];
+    const float2 r = rotation_angles(blk.a00, blk.a11, blk.a01);
+    const float2 rp = make_float2(r.x, -r.y);
+    const Block o = rotate_block(blk, rp, r, rp, r);
+    Ap[ipp] = o.a00;
+    Ap[ipq] = o.a01;
+    Ap[iqq] = o.a11;
+    const int buf = (tt & 1) * 128;
+    sinfo[buf + k] = make_int4(4 * p, 4 * q, 4 * cbp, 4 * cbq);
+    rot[buf + k] = r;
+    // position k holds a: it is p (sees (c, -s)) when a < b
+    rv[(k / S) * SP + k % S] = make_float2(r.x, a < b ? -r.y : r.y);
+  };
+
   auto group = [&]() -> bool {
     const int group_start = t;
     const int group_end = min(t + G, limit);
+    for (int k = tid; k < m; k += nt) pair_phase(k, t, hist);
+    __syncthreads();
 #pragma unroll 1
     for (; t < group_end; ++t) {
-      const int round = t % rounds_per_sweep;
-      float2* rv = hist + (t - group_start) * hrow;
-      for (int k = tid; k < m; k += nt) {
-        const int a = participant(round, k, N);
-        const int b = participant(round, N - 1 - k, N);
-        const int p = a < b ? a : b;
-        const int q = a < b ? b : a;
-        const int cbp = (p * (p + 1)) >> 1;
-        const int cbq = (q * (q + 1)) >> 1;
-        const int ipp = cbp + p, ipq = cbq + p, iqq = cbq + q;
-        Block blk;
-        blk.a00 = Ap[ipp];
-        blk.a01 = Ap[ipq];
-        blk.a10 = blk.a01;
-        blk.a11 = Ap[iqq];
-        const float2 r = rotation_angles(blk.a00, blk.a11, blk.a01);
-        const float2 rp = make_float2(r.x, -r.y);
-        const Block o = rotate_block(blk, rp, r, rp, r);
-        Ap[ipp] = o.a00;
-        Ap[ipq] = o.a01;
-        Ap[iqq] = o.a11;
-        sinfo[k] = make_int4(p, q, cbp, cbq);
-        rot[k] = r;
-        // position k holds a: it is p (sees (c, -s)) when a < b
-        rv[(k / S) * SP + k % S] = make_float2(r.x, a < b ? -r.y : r.y);
-      }
-      __syncthreads();
+      const int buf = (t & 1) * 128;
 #pragma unroll
-      for (int i = 0; i < BPT; i += 2) {
-        if (i + 1 < BPT) {
-          rotate_blocks<2>(Ap, sinfo, rot, code + i);
+      for (int i = 0; i < BPT + 1; i += 2) {
+        if (i + 1 < BPT + 1) {
+          rotate_blocks<2>(Ap, sinfo + buf, rot + buf, code + i);
         } else {
-          rotate_blocks<1>(Ap, sinfo, rot, code + i);
+          rotate_blocks<1>(Ap, sinfo + buf, rot + buf, code + i);
         }
       }
+      if (tid < m && t + 1 < group_end)
+        pair_phase(tid, t + 1, hist + (t + 1 - group_start) * hrow);
       __syncthreads();
     }
     // Rounds the group did not reach: identity rotations (V's positions still
@@ -834,8 +875,8 @@ size_t batched_jacobi_lds_bytes(uint32_t order) {
   const size_t packed = static_cast<size_t>(N) * (N + 1) / 2;
   const int S = slots_for_order(N);
   const size_t hist = static_cast<size_t>(group_for_slots(S)) * 8 * padded_slots(S);
-  return ((packed + 3) & ~size_t{3}) * sizeof(float) + 128 * sizeof(int4) +
-         (128 + hist) * sizeof(float2) + 32 * sizeof(double) + 16 * sizeof(float) +
+  return ((packed + 3) & ~size_t{3}) * sizeof(float) + 256 * sizeof(int4) +
+         (256 + hist) * sizeof(float2) + 32 * sizeof(double) + 16 * sizeof(float) +
          4 * sizeof(int);
 }
 
@@ -853,7 +894,7 @@ hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t
   // The kernel's shape assumptions: whole parts per row, at most 1024 lanes,
   // every off-diagonal block in some thread's list, LDS within 160 KiB, and
   // for V in device memory two passes of N / 2 rows of 8 lanes.
-  if (parts_for_order(N) * S != m || threads > 1024 || threads % 64 != 0 ||
+  if (parts_for_order(N) * S != m || threads > 1024 || threads % 64 != 0 || threads < m ||
       (offblocks + threads - 1) / threads > blocks_per_thread(S) || lds > 160 * 1024 ||
       (v_in_memory(S) && threads != 8 * m))
     return hipErrorInvalidValue;

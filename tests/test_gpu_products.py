@@ -169,10 +169,15 @@ def test_jacobi_fixed_sweeps_parity(runtime):
         ora_gap = np.linalg.norm(np.sort(info["diagonal"]) - lam)
         assert gpu_gap <= rep.off_norm + drift, (n, gpu_gap, rep.off_norm, drift)
         assert ora_gap <= info["off_norm"] + drift, (n, ora_gap)
-        if n <= 128:   # converged in 8 sweeps: the diagonals agree to FP32 rounding
-            np.testing.assert_allclose(np.sort(d), np.sort(info["diagonal"]), rtol=2e-5,
-                                       atol=1e-6)
-            np.testing.assert_allclose(np.sort(d), np.linalg.eigvalsh(A), rtol=1e-4, atol=2e-6)
+        if n <= 128:
+            # Converged in 8 sweeps: the diagonals agree to the FP32 rounding
+            # that each diagonal entry collects over its 8 (n - 1) updates, a
+            # random walk of ~2^-24 max|lambda| per update (4 sigma here); the
+            # two sides round differently (FMA contraction), so no tighter.
+            walk = 4.0 * 2.0**-24 * np.abs(lam).max() * np.sqrt(8.0 * (n - 1))
+            np.testing.assert_allclose(np.sort(d), np.sort(info["diagonal"]), rtol=0,
+                                       atol=walk)
+            np.testing.assert_allclose(np.sort(d), lam, rtol=0, atol=walk + rep.off_norm)
         np.testing.assert_allclose(rt, np.sqrt(d), rtol=1e-7)
         assert S.orthogonality_error(V) < 2e-6 * np.sqrt(n)
         # V diag(d) V^T reproduces A up to the off-diagonal part left
