@@ -9,15 +9,17 @@ Two paths are timed on the same device-resident batch:
   * plain FP32: FP32 Jacobi to off(A) <= 1e-7 ||A||_F (or its FP32 floor),
     FP32 reconstruction and corrections, binary64 residual;
   * low-bit + refine: the BF16 tier (BF16 MFMA reconstruction and
-    corrections, Jacobi stopped against the smallest eigenvalue), with every
-    matrix that misses the binary64 residual check re-solved on the plain path
-    inside the timed region.
+    corrections, Jacobi stopped once the frozen-factor corrections are
+    predicted to contract by 0.5 per step), with every matrix that misses the
+    binary64 residual check re-solved on the plain path inside the timed
+    region.
 ``value`` is the low-bit path's matrices/s; every matrix of both paths is
 checked against the FP64 root Q diag(sqrt(lambda)) Q^T.
 
-Also reported: the roofline of the dominant kernel (the batched Jacobi launch
-of the 129..256 order class, timed with HIP events on the library's stream)
-and a CPU baseline: the reference's CPU plain path (1_baseline/spd.cpp:348-430,
+Also reported: the roofline of the dominant kernel (batched_jacobi_kernel<16>,
+the launch of the orders 241..256 of the batch, run on its own and timed with
+HIP events on the library's stream; ``--roofline-only`` runs just these
+launches, for the profiler) and a CPU baseline: the reference's CPU plain path (1_baseline/spd.cpp:348-430,
 LAPACK ssyevd + BLAS, restated in oracle/solver.py) on a bounded sample of
 the same matrices, one core.
 """
@@ -44,6 +46,41 @@ def log(*a):
 def jacobi_flops(orders, sweeps):
     """9 S n^2 (n - 1): 18 n flops per rotation, n (n - 1) / 2 rotations a sweep."""
     return sum(9.0 * s * n * n * (n - 1) for n, s in zip(orders, sweeps))
+
+
+def roofline_leg(args, mats, orders, plain, rt, stream, torch, np, capi, dev):
+    """Dominant kernel: the Jacobi launch of the orders 241..256 (one launch of
+    batched_jacobi_kernel<16>, one workgroup per matrix), timed on its own."""
+    big = [i for i, n in enumerate(orders) if n > 240]
+    sub_orders = [orders[i] for i in big]
+    _, sub_flat = capi.pack_batch([mats[i]["A"] for i in big])
+    _, _, _, stotal, svtotal = capi.batch_layout(sub_orders)
+    SA = torch.from_numpy(sub_flat).to(dev)
+    SV = torch.zeros(stotal, dtype=torch.float32, device=dev)
+    SD = torch.zeros(svtotal, dtype=torch.float32, device=dev)
+    SR = torch.zeros(svtotal, dtype=torch.float32, device=dev)
+    jopt = plain.jacobi
+    reps = rt.jacobi_batched(sub_orders, SA, SV, SD, SR, jopt)  # warm
+    durations = []
+    for _ in range(args.roofline_reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        reps = rt.jacobi_batched(sub_orders, SA, SV, SD, SR, jopt)
+        e1.record(stream)
+        stream.synchronize()
+        durations.append(e0.elapsed_time(e1) * 1e-3)
+    jt = float(np.mean(durations))
+    sweeps = [r.sweeps for r in reps]
+    flops = jacobi_flops(sub_orders, sweeps)
+    achieved = flops / jt / 1e12
+    return {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
+            "traffic": None,
+            "kernel": (f"batched_jacobi_kernel<16> (orders 241..256: {len(big)} matrices, "
+                       f"sweeps mean {np.mean(sweeps):.2f} max {max(sweeps)}, "
+                       f"{flops / 1e12:.3f} TFLOP per launch at 9 S n^2 (n - 1), "
+                       f"{jt * 1e3:.2f} ms mean of {len(durations)} launches)")}
 
 
 def cpu_baseline(mats, budget_s):
@@ -79,6 +116,9 @@ def main():
     ap.add_argument("--tier", default="bf16")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rate", type=float, default=0.5)
+    ap.add_argument("--roofline-only", action="store_true")
+    ap.add_argument("--roofline-reps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the batched SPD solve shards by matrix with no "
@@ -131,6 +171,10 @@ def main():
         return (time.perf_counter() - t0) / args.steps, out
 
     plain = MS.plain_fp32_options()
+    if args.roofline_only:
+        log("roofline", roofline_leg(args, mats, orders, plain, rt, stream, torch, np, capi, dev))
+        rt.close()
+        return
     dt_plain, rep_plain = timed(lambda: rt.solve_batched(orders, A, X, plain))
     err_plain = check_all(X, "plain fp32")
     ok_plain = sum(r["status"] == "converged" for r in rep_plain)
@@ -139,7 +183,7 @@ def main():
         f"{np.mean([r['sweeps'] for r in rep_plain]):.2f}, mean corrections "
         f"{np.mean([r['corrections'] for r in rep_plain]):.2f}")
 
-    tier = MS.tier_options(args.tier)
+    tier = MS.tier_options(args.tier, rate=args.rate, max_corrections=12)
     dt_low, (rep_low, fallback) = timed(
         lambda: MS.solve_with_fallback(rt, orders, A, X, tier))
     err_low = check_all(X, f"low-bit {args.tier}")
@@ -147,35 +191,7 @@ def main():
         f"matrices/s, fallback {len(fallback)}/{len(mats)}, mean sweeps "
         f"{np.mean([r['sweeps'] for r in rep_low]):.2f}")
 
-    # Roofline of the dominant kernel: the 129..256 batched Jacobi launch.
-    big = [i for i, n in enumerate(orders) if n > 128]
-    sub_orders = [orders[i] for i in big]
-    _, sub_flat = capi.pack_batch([mats[i]["A"] for i in big])
-    _, _, _, stotal, svtotal = capi.batch_layout(sub_orders)
-    SA = torch.from_numpy(sub_flat).to(dev)
-    SV = torch.zeros(stotal, dtype=torch.float32, device=dev)
-    SD = torch.zeros(svtotal, dtype=torch.float32, device=dev)
-    SR = torch.zeros(svtotal, dtype=torch.float32, device=dev)
-    jopt = plain.jacobi
-    rt.jacobi_batched(sub_orders, SA, SV, SD, SR, jopt)  # warm
-    durations = []
-    for _ in range(3):
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        reps = rt.jacobi_batched(sub_orders, SA, SV, SD, SR, jopt)
-        e1.record(stream)
-        stream.synchronize()
-        durations.append(e0.elapsed_time(e1) * 1e-3)
-    jt = float(np.mean(durations))
-    flops = jacobi_flops(sub_orders, [r.sweeps for r in reps])
-    achieved = flops / jt / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-                "traffic": None,
-                "kernel": "batched_jacobi_kernel<512,false> (orders 144..256, "
-                          f"{len(big)} matrices, {flops / 1e12:.3f} TFLOP per launch, "
-                          f"{jt * 1e3:.2f} ms)"}
+    roofline = roofline_leg(args, mats, orders, plain, rt, stream, torch, np, capi, dev)
     log("roofline", roofline)
 
     cpu = None if args.no_cpu else cpu_baseline(mats, args.cpu_budget)
@@ -197,12 +213,14 @@ def main():
         "data": "synthetic (seeded Haar-random SPD matrices)",
         "config": {"workload": "batched uneven: 4096 SPD matrices, n in {16,...,256}, "
                                "cond log-uniform in [1e1, 1e5], half clustered spectra",
-                   "count": len(mats), "tier": args.tier},
+                   "count": len(mats), "tier": args.tier, "correction_rate": args.rate},
         "plain_fp32": {"matrices_per_s": round(len(mats) / dt_plain, 2),
                        "ms_per_step": round(dt_plain * 1e3, 3),
                        "max_rel_error": err_plain, "converged": ok_plain},
-        "low_bit": {"max_rel_error": err_low, "fallbacks": len(fallback),
-                    "mean_sweeps": float(np.mean([r["sweeps"] for r in rep_low]))},
+        "low_bit": {"matrices_per_s": round(len(mats) / dt_low, 2),
+                    "max_rel_error": err_low, "fallbacks": len(fallback),
+                    "mean_sweeps": float(np.mean([r["sweeps"] for r in rep_low])),
+                    "mean_corrections": float(np.mean([r["corrections"] for r in rep_low]))},
         "speedup_vs_plain_fp32": round(dt_plain / dt_low, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define MSQ_ABI_VERSION 1
+#define MSQ_ABI_VERSION 2
 
 enum msq_status {
   MSQ_OK = 0,
@@ -83,7 +83,14 @@ enum msq_solve_status {
  * fixed_sweeps = 0 the sweeps stop at the first sweep whose FP64 off-diagonal
  * norm off(A) satisfies off <= off_tol ||A||_F, or off <= off_tol_lambda *
  * min diag(A), or (stagnation) off > stagnation * previous off while
- * off <= floor_tol ||A||_F, or after max_sweeps. */
+ * off <= floor_tol ||A||_F, or once the frozen-factor correction is predicted
+ * to contract by correction_rate per step, or after max_sweeps.
+ *
+ * correction_rate (0: off; no counterpart in the reference): with F the
+ * off-diagonal part of the rotated A and s_i = sqrt(a_ii), one Sylvester
+ * correction in the frozen basis contracts the root's error, to first order,
+ * by at most ||G||_F / min_i s_i with G_ij = F_ij / (s_i + s_j).  Stopping at
+ * that bound leaves the rest of the accuracy to the corrections. */
 typedef struct msq_jacobi_options {
   uint32_t fixed_sweeps;
   uint32_t max_sweeps;
@@ -91,6 +98,7 @@ typedef struct msq_jacobi_options {
   float off_tol_lambda;
   float stagnation;
   float floor_tol;
+  float correction_rate;
 } msq_jacobi_options;
 
 typedef struct msq_jacobi_report {
@@ -100,6 +108,7 @@ typedef struct msq_jacobi_report {
   double matrix_norm;        /* ||A||_F (binary64) */
   float min_diagonal;
   float off_history[MSQ_JACOBI_HISTORY]; /* off(A) / ||A||_F per sweep */
+  float correction_rate;                 /* ||G||_F / min s after the last sweep */
 } msq_jacobi_report;
 
 /* Options of one matrix of msq_solve_batched.  The precisions select the
@@ -134,6 +143,7 @@ typedef struct msq_solve_report {
   double residuals[MSQ_SOLVE_HISTORY];         /* X0, then each correction */
   double forward_estimates[MSQ_SOLVE_HISTORY];
   float off_history[MSQ_JACOBI_HISTORY];
+  float correction_rate; /* Jacobi's ||G||_F / min s (see msq_jacobi_options) */
 } msq_solve_report;
 
 /* ---------------------------------------------------------------- runtime */

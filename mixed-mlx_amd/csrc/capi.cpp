@@ -66,10 +66,17 @@ struct Arena {
 
 }  // namespace
 
+// Streams on which independent launches of one call overlap (the order
+// classes of a batched Jacobi); forked from and joined back into `stream`.
+constexpr int kLanes = 4;
+
 struct msq_runtime {
   int device = 0;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
+  hipStream_t lanes[kLanes] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t joins[kLanes] = {};
   Arena arena;
   float* last_vectors = nullptr;
   float* last_roots = nullptr;
@@ -211,6 +218,7 @@ msq::JacobiJob make_job(const Layout& L, uint32_t i, const msq_jacobi_options& o
   j.off_tol_lambda = o.off_tol_lambda;
   j.stagnation = o.stagnation > 0.0f ? o.stagnation : 0.5f;
   j.floor_tol = o.floor_tol > 0.0f ? o.floor_tol : 1e-5f;
+  j.correction_rate = o.correction_rate;
   j.status_index = i;
   return j;
 }
@@ -226,14 +234,15 @@ int jacobi_large(msq_runtime* rt, const Layout& L, uint32_t i, const double* a64
   char* buf = nullptr;
   const size_t abytes = static_cast<size_t>(ld) * ld * sizeof(float);
   const size_t bytes = abytes + 2 * m * sizeof(int) + m * sizeof(float2) +
-                       partials * (sizeof(double) + sizeof(float)) + 1024;
+                       partials * (2 * sizeof(double) + sizeof(float)) + 1024;
   HIP_TRY(hipMalloc(&buf, bytes));
   float* A = reinterpret_cast<float*>(buf);
   int* sp = reinterpret_cast<int*>(buf + abytes);
   int* sq = sp + m;
   float2* rot = reinterpret_cast<float2*>(sq + m);  // 8 m bytes past sp: aligned
   double* off2 = reinterpret_cast<double*>(rot + m);
-  float* dmin = reinterpret_cast<float*>(off2 + partials);
+  double* g2 = off2 + partials;
+  float* dmin = reinterpret_cast<float*>(g2 + partials);
   float* Vi = V + L.off[i];
   int status = MSQ_OK;
   do {
@@ -251,27 +260,33 @@ int jacobi_large(msq_runtime* rt, const Layout& L, uint32_t i, const double* a64
     st.matrix_norm = std::sqrt(norm2);
     const uint32_t cap = job.fixed_sweeps ? job.fixed_sweeps : job.max_sweeps;
     double previous = INFINITY;
-    std::vector<double> h_off(partials);
+    std::vector<double> h_off(partials), h_g(partials);
     std::vector<float> h_min(partials);
     for (uint32_t sweep = 0; sweep < cap && status == MSQ_OK; ++sweep) {
       for (uint32_t t = 0; t + 1 < np && e == hipSuccess; ++t)
         e = msq::launch_global_jacobi_round(A, Vi, n, ld, t, sp, sq, rot, rt->stream);
-      if (e == hipSuccess) e = msq::launch_global_offnorm(A, n, ld, off2, dmin, partials, rt->stream);
+      if (e == hipSuccess) e = msq::launch_global_offnorm(A, n, ld, off2, g2, dmin, partials, rt->stream);
       if (e == hipSuccess)
         e = hipMemcpyAsync(h_off.data(), off2, partials * sizeof(double), hipMemcpyDeviceToHost,
+                           rt->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(h_g.data(), g2, partials * sizeof(double), hipMemcpyDeviceToHost,
                            rt->stream);
       if (e == hipSuccess)
         e = hipMemcpyAsync(h_min.data(), dmin, partials * sizeof(float), hipMemcpyDeviceToHost,
                            rt->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(rt->stream);
       if (e != hipSuccess) { status = set_error(MSQ_ERROR_HIP, "jacobi sweep: %s", hipGetErrorString(e)); break; }
-      double s = 0.0;
+      double s = 0.0, g = 0.0;
       float d = INFINITY;
       for (int k = 0; k < partials; ++k) {
         s += h_off[k];
+        g += h_g[k];
         d = std::min(d, h_min[k]);
       }
       const double off = std::sqrt(s);
+      const double rate = d > 0.0f ? std::sqrt(g) / std::sqrt(static_cast<double>(d)) : INFINITY;
+      st.correction_rate = static_cast<float>(rate);
       st.sweeps = sweep + 1;
       st.off_norm = off;
       st.min_diagonal = d;
@@ -283,6 +298,7 @@ int jacobi_large(msq_runtime* rt, const Layout& L, uint32_t i, const double* a64
         double tol = job.off_tol * st.matrix_norm;
         if (job.off_tol_lambda > 0.0f && d > 0.0f) tol = std::max(tol, double(job.off_tol_lambda) * d);
         stop = off <= tol || st.sweeps >= cap ||
+               (job.correction_rate > 0.0f && rate <= job.correction_rate) ||
                (off > job.stagnation * previous && off <= job.floor_tol * st.matrix_norm);
       }
       previous = off;
@@ -327,11 +343,24 @@ int run_jacobi(msq_runtime* rt, const Layout& L, const double* a64, float* V, fl
   launches.push_back({jobs.size(), 0});
   if (!jobs.empty()) {
     MSQ_TRY(upload(rt, djobs, jobs));
-    for (size_t c = 0; c + 1 < launches.size(); ++c) {
+    // Order classes run concurrently on the runtime's lanes (largest first,
+    // dealt round-robin): a launch lasts as long as its slowest matrix, and
+    // the CUs its faster matrices free take work from the other classes.
+    const size_t classes = launches.size() - 1;
+    const int used = classes > 1 ? static_cast<int>(std::min<size_t>(classes, kLanes)) : 0;
+    if (used) {
+      HIP_TRY(hipEventRecord(rt->fork, rt->stream));
+      for (int l = 0; l < used; ++l) HIP_TRY(hipStreamWaitEvent(rt->lanes[l], rt->fork, 0));
+    }
+    for (size_t c = 0; c < classes; ++c) {
       const size_t b = launches[c].first, e = launches[c + 1].first;
+      hipStream_t s = used ? rt->lanes[c % used] : rt->stream;
       HIP_TRY(msq::launch_batched_jacobi(djobs + b, static_cast<uint32_t>(e - b),
-                                         launches[c].second, a64, V, diag, roots, jstatus,
-                                         rt->stream));
+                                         launches[c].second, a64, V, diag, roots, jstatus, s));
+    }
+    for (int l = 0; l < used; ++l) {
+      HIP_TRY(hipEventRecord(rt->joins[l], rt->lanes[l]));
+      HIP_TRY(hipStreamWaitEvent(rt->stream, rt->joins[l], 0));
     }
   }
   for (uint32_t i = 0; i < count; ++i) {
@@ -351,6 +380,7 @@ void fill_jacobi_report(const msq::JacobiStatus& s, msq_jacobi_report& r) {
   r.off_norm = s.off_norm;
   r.matrix_norm = s.matrix_norm;
   r.min_diagonal = s.min_diagonal;
+  r.correction_rate = s.correction_rate;
   for (uint32_t k = 0; k < MSQ_JACOBI_HISTORY; ++k)
     r.off_history[k] = k < s.sweeps && k < msq::kJacobiHistory ? s.off_history[k] : 0.0f;
 }
@@ -666,6 +696,15 @@ int msq_runtime_create(int device, msq_runtime** out) {
     return set_error(MSQ_ERROR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
   }
   rt->stream = rt->own_stream;
+  for (int l = 0; l < kLanes && e == hipSuccess; ++l) {
+    e = hipStreamCreateWithFlags(&rt->lanes[l], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&rt->joins[l], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&rt->fork, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    msq_runtime_destroy(rt);
+    return set_error(MSQ_ERROR_HIP, "runtime streams: %s", hipGetErrorString(e));
+  }
   *out = rt;
   return MSQ_OK;
 }
@@ -678,6 +717,16 @@ int msq_runtime_destroy(msq_runtime* rt) {
   if (e == hipSuccess) e = hipStreamSynchronize(rt->stream);
   if (e != hipSuccess) status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
   if (rt->arena.base && (e = hipFree(rt->arena.base)) != hipSuccess && status == MSQ_OK)
+    status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
+  for (int l = 0; l < kLanes; ++l) {
+    if (rt->lanes[l] && (e = hipStreamSynchronize(rt->lanes[l])) == hipSuccess)
+      e = hipStreamDestroy(rt->lanes[l]);
+    if (e != hipSuccess && status == MSQ_OK)
+      status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
+    if (rt->joins[l] && (e = hipEventDestroy(rt->joins[l])) != hipSuccess && status == MSQ_OK)
+      status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
+  }
+  if (rt->fork && (e = hipEventDestroy(rt->fork)) != hipSuccess && status == MSQ_OK)
     status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
   if (rt->own_stream && (e = hipStreamDestroy(rt->own_stream)) != hipSuccess && status == MSQ_OK)
     status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
@@ -1172,6 +1221,7 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
       r.matrix_norm = s.matrix_norm;
       r.min_root = s.min_root;
       r.off_norm = hj[i].off_norm;
+      r.correction_rate = hj[i].correction_rate;
       r.final_residual = s.final_residual;
       r.final_forward_estimate = s.final_forward_estimate;
       for (uint32_t k = 0; k < MSQ_SOLVE_HISTORY && k < msq::kSolveHistory; ++k) {

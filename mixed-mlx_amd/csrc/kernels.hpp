@@ -52,7 +52,7 @@ struct JacobiJob {
   float stagnation;        // ... or off > stagnation * previous off and
   float floor_tol;         //     off <= floor_tol * ||A||_F (FP32 floor)
   uint32_t status_index;   // slot of this matrix in the status array
-  uint32_t reserved;
+  float correction_rate;   // ... or ||G||_F / min s <= correction_rate (0: off)
 };
 
 struct JacobiStatus {
@@ -62,6 +62,7 @@ struct JacobiStatus {
   double matrix_norm;
   float min_diagonal;
   float off_history[kJacobiHistory];  // off(A)/||A||_F after each sweep
+  float correction_rate;              // ||G||_F / min s after the last sweep
 };
 
 // One launch covers matrices sharing the padded order (order rounded up to
@@ -78,7 +79,7 @@ hipError_t launch_global_jacobi_init(const double* a64, float* A, float* V, uint
 hipError_t launch_global_jacobi_round(float* A, float* V, uint32_t n, uint32_t ld,
                                       uint32_t round, int* slot_p, int* slot_q,
                                       float2* rot, hipStream_t stream);
-hipError_t launch_global_offnorm(const float* A, uint32_t n, uint32_t ld, double* off2,
+hipError_t launch_global_offnorm(const float* A, uint32_t n, uint32_t ld, double* off2, double* g2,
                                  float* dmin, int partials, hipStream_t stream);
 hipError_t launch_global_jacobi_finish(const float* A, uint32_t n, uint32_t ld,
                                        float* diagonal, float* roots, hipStream_t stream);

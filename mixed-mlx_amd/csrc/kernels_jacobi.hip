@@ -474,7 +474,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   const int rounds_per_sweep = N - 1;
   const int limit = static_cast<int>(cap) * rounds_per_sweep;
   double previous = INFINITY;
-  double off = 0.0;
+  double off = 0.0, rate = INFINITY;
   float min_diag = 0.0f;
   uint32_t sweeps = 0;
   int t = 0;   // rounds applied to A
@@ -582,10 +582,14 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     const bool at_limit = t >= limit;
     if (!(t / rounds_per_sweep > static_cast<int>(sweeps) || at_limit)) return false;
     sweeps = static_cast<uint32_t>(t / rounds_per_sweep);
-    double off2 = 0.0;
+    double off2 = 0.0, g2 = 0.0;
     float dmin = INFINITY;
+    // s_i = sqrt(a_ii) in the history buffer (idle until the next group).
+    float* sroot = reinterpret_cast<float*>(hist);
+    for (int i = tid; i < N; i += nt) sroot[i] = i < n ? sqrtf(fmaxf(Ap[tri(i, i)], 0.0f)) : 0.0f;
     if (tid == 0) {
       red[1] = 0.0;
+      red[2] = 0.0;
       redf[0] = INFINITY;
     }
     __syncthreads();
@@ -598,9 +602,12 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
         dmin = fminf(dmin, Ap[e]);
       } else {
         off2 += 2.0 * v * v;
+        const double den = static_cast<double>(sroot[i]) + static_cast<double>(sroot[j]);
+        g2 += den > 0.0 ? 2.0 * (v / den) * (v / den) : (v != 0.0 ? INFINITY : 0.0);
       }
     }
     if (off2 != 0.0) atomicAdd(&red[1], off2);
+    if (g2 != 0.0) atomicAdd(&red[2], g2);
     if (dmin < INFINITY) atomicMin(&redf[0], dmin);
     __syncthreads();
     if (tid == 0) {
@@ -608,6 +615,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
       const float d = redf[0];
       off = sqrt(s2);
       min_diag = d;
+      rate = d > 0.0f ? sqrt(red[2]) / sqrt(static_cast<double>(d)) : INFINITY;
       if (sweeps >= 1 && sweeps <= kJacobiHistory)
         st->off_history[sweeps - 1] = static_cast<float>(off / matrix_norm);
       int stop = at_limit ? 1 : 0;
@@ -618,6 +626,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
           tol = tl > tol ? tl : tol;
         }
         stop = stop || off <= tol ||
+               (job.correction_rate > 0.0f && rate <= static_cast<double>(job.correction_rate)) ||
                (off > static_cast<double>(job.stagnation) * previous &&
                 off <= static_cast<double>(job.floor_tol) * matrix_norm);
       }
@@ -698,6 +707,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     st->off_norm = off;
     st->matrix_norm = matrix_norm;
     st->min_diagonal = min_diag;
+    st->correction_rate = static_cast<float>(rate);
   }
 }
 
@@ -797,8 +807,9 @@ __global__ void global_init_kernel(const double* __restrict__ a64, float* __rest
 // Partial sums of the FP64 off-diagonal norm and the diagonal minimum.
 __global__ void global_offnorm_kernel(const float* __restrict__ A, int n, int ld,
                                       double* __restrict__ out_off2,
+                                      double* __restrict__ out_g2,
                                       float* __restrict__ out_min) {
-  double s = 0.0;
+  double s = 0.0, g = 0.0;
   float d = INFINITY;
   const size_t total = static_cast<size_t>(n) * n;
   for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
@@ -810,25 +821,32 @@ __global__ void global_offnorm_kernel(const float* __restrict__ A, int n, int ld
       d = fminf(d, v);
     } else {
       s += static_cast<double>(v) * v;
+      const double den = sqrt(static_cast<double>(fmaxf(A[i + static_cast<size_t>(i) * ld], 0.0f))) +
+                         sqrt(static_cast<double>(fmaxf(A[j + static_cast<size_t>(j) * ld], 0.0f)));
+      g += den > 0.0 ? (v / den) * (v / den) : (v != 0.0f ? INFINITY : 0.0);
     }
   }
-  __shared__ double sd[16];
+  __shared__ double sd[16], sg[16];
   __shared__ float sf[16];
   s = wave_sum(s);
+  g = wave_sum(g);
   d = wave_min(d);
   if ((threadIdx.x & 63) == 0) {
     sd[threadIdx.x / 64] = s;
+    sg[threadIdx.x / 64] = g;
     sf[threadIdx.x / 64] = d;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double t = 0.0;
+    double t = 0.0, tg = 0.0;
     float m = INFINITY;
     for (int w = 0; w < static_cast<int>(blockDim.x / 64); ++w) {
       t += sd[w];
+      tg += sg[w];
       m = fminf(m, sf[w]);
     }
     out_off2[blockIdx.x] = t;
+    out_g2[blockIdx.x] = tg;
     out_min[blockIdx.x] = m;
   }
 }
@@ -931,8 +949,8 @@ hipError_t launch_global_jacobi_round(float* A, float* V, uint32_t n, uint32_t l
 }
 
 hipError_t launch_global_offnorm(const float* A, uint32_t n, uint32_t ld, double* off2,
-                                 float* dmin, int partials, hipStream_t stream) {
-  global_offnorm_kernel<<<partials, 256, 0, stream>>>(A, n, ld, off2, dmin);
+                                 double* g2, float* dmin, int partials, hipStream_t stream) {
+  global_offnorm_kernel<<<partials, 256, 0, stream>>>(A, n, ld, off2, g2, dmin);
   return hipGetLastError();
 }
 

@@ -58,14 +58,16 @@ EXPORTED_SYMBOLS = (
 class JacobiOptions(ctypes.Structure):
     _fields_ = [("fixed_sweeps", ctypes.c_uint32), ("max_sweeps", ctypes.c_uint32),
                 ("off_tol", ctypes.c_float), ("off_tol_lambda", ctypes.c_float),
-                ("stagnation", ctypes.c_float), ("floor_tol", ctypes.c_float)]
+                ("stagnation", ctypes.c_float), ("floor_tol", ctypes.c_float),
+                ("correction_rate", ctypes.c_float)]
 
 
 class JacobiReport(ctypes.Structure):
     _fields_ = [("sweeps", ctypes.c_uint32), ("invalid_spectrum", ctypes.c_uint32),
                 ("off_norm", ctypes.c_double), ("matrix_norm", ctypes.c_double),
                 ("min_diagonal", ctypes.c_float),
-                ("off_history", ctypes.c_float * JACOBI_HISTORY)]
+                ("off_history", ctypes.c_float * JACOBI_HISTORY),
+                ("correction_rate", ctypes.c_float)]
 
 
 class SolveOptions(ctypes.Structure):
@@ -83,7 +85,8 @@ class SolveReport(ctypes.Structure):
                 ("final_forward_estimate", ctypes.c_double),
                 ("residuals", ctypes.c_double * SOLVE_HISTORY),
                 ("forward_estimates", ctypes.c_double * SOLVE_HISTORY),
-                ("off_history", ctypes.c_float * JACOBI_HISTORY)]
+                ("off_history", ctypes.c_float * JACOBI_HISTORY),
+                ("correction_rate", ctypes.c_float)]
 
 
 _p = ctypes.c_void_p
@@ -206,9 +209,9 @@ def unpack_vectors(orders, flat):
 
 # ---------------------------------------------------------------- options
 def jacobi_options(fixed_sweeps=0, max_sweeps=30, off_tol=1e-7, off_tol_lambda=0.0,
-                   stagnation=0.5, floor_tol=1e-5):
+                   stagnation=0.5, floor_tol=1e-5, correction_rate=0.0):
     return JacobiOptions(fixed_sweeps, max_sweeps, off_tol, off_tol_lambda, stagnation,
-                         floor_tol)
+                         floor_tol, correction_rate)
 
 
 def solve_options(basis=FP32, correction=FP32, jacobi=None, min_corrections=0,
@@ -230,6 +233,7 @@ def report_dict(r: SolveReport):
         "residuals": list(r.residuals[:steps]),
         "forward_estimates": list(r.forward_estimates[:steps]),
         "off_history": list(r.off_history[:r.sweeps]),
+        "correction_rate": r.correction_rate,
     }
 
 

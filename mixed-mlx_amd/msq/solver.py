@@ -11,10 +11,10 @@ Here a configuration is an ``msq_solve_options`` record per matrix:
   reconstruction and corrections, binary64 residual; corrections stop when the
   forward-error estimate ||R||_F / (2 s_min ||X||_F) reaches the target.
 * ``tier_options(name)`` -- a low-bit tier: reconstruction and corrections in
-  BF16 / scaled FP8 / MXFP4 products, Jacobi stopped earlier, once its
-  off-diagonal norm is small against the smallest eigenvalue (the frozen
-  factor's corrections contract at a rate ~ off(A) / lambda_min), and more
-  corrections allowed.
+  BF16 / scaled FP8 / MXFP4 products, Jacobi stopped as soon as the frozen
+  factor's corrections are predicted to contract by ``rate`` per step
+  (||G||_F / min s, G_ij = F_ij / (s_i + s_j) with F the off-diagonal part of
+  the rotated matrix), and more corrections allowed.
 * ``solve_with_fallback`` -- runs tiered options and re-solves every matrix
   whose binary64 residual check missed the target on the plain FP32 path.
 """
@@ -46,14 +46,14 @@ def plain_fp32_options(max_corrections=8, min_corrections=0):
         residual_tolerance=0.0, forward_tolerance=ESTIMATE_TARGET)
 
 
-def tier_options(name, theta=0.05, max_corrections=8, min_corrections=0):
-    """A low-bit tier.  theta: Jacobi stops once off(A) <= theta * min diag(A)
-    (the corrections' contraction rate is about theta / 2)."""
+def tier_options(name, rate=0.5, max_corrections=12, min_corrections=0):
+    """A low-bit tier.  rate: Jacobi stops once one correction is predicted to
+    shrink the root's error by this factor (first-order bound)."""
     basis, corr = TIER_PRECISIONS[name]
     return capi.solve_options(
         basis, corr,
-        capi.jacobi_options(off_tol=1e-7, off_tol_lambda=theta, max_sweeps=30,
-                            stagnation=0.5, floor_tol=1e-5),
+        capi.jacobi_options(off_tol=1e-7, max_sweeps=30, stagnation=0.5, floor_tol=1e-5,
+                            correction_rate=rate),
         min_corrections=min_corrections, max_corrections=max_corrections,
         residual_tolerance=0.0, forward_tolerance=ESTIMATE_TARGET)
 
