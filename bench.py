@@ -8,11 +8,13 @@ indices log-uniform spectra and odd indices four tight clusters -- solved to
 Two paths are timed on the same device-resident batch:
   * plain FP32: FP32 Jacobi to off(A) <= 1e-7 ||A||_F (or its FP32 floor),
     FP32 reconstruction and corrections, binary64 residual;
-  * low-bit + refine: the BF16 tier (BF16 MFMA reconstruction and
-    corrections, Jacobi stopped once the frozen-factor corrections are
-    predicted to contract by 0.5 per step), with every matrix that misses the
-    binary64 residual check re-solved on the plain path inside the timed
-    region.
+  * low-bit + refine + learned tier: per matrix, the learned selector
+    (features and MLP inference on the GPU, msq/selector.py) picks one of the
+    low-bit tiers (BF16 / FP8 / MXFP4 MFMA reconstruction, BF16 corrections,
+    Jacobi stopped once the frozen-factor corrections are predicted to
+    contract by a tier-specific rate) or the plain path; every matrix that
+    misses the binary64 residual check is re-solved on the plain path inside
+    the timed region.  --no-selector: the fixed BF16 tier (rate --rate).
 ``value`` is the low-bit path's matrices/s; every matrix of both paths is
 checked against the FP64 root Q diag(sqrt(lambda)) Q^T.
 
@@ -119,6 +121,8 @@ def main():
     ap.add_argument("--rate", type=float, default=0.5)
     ap.add_argument("--roofline-only", action="store_true")
     ap.add_argument("--roofline-reps", type=int, default=5)
+    ap.add_argument("--no-selector", action="store_true",
+                    help="fixed tier instead of the learned tier selector")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the batched SPD solve shards by matrix with no "
@@ -129,7 +133,7 @@ def main():
     import numpy as np
     import torch
 
-    from msq import capi, generators as G, solver as MS
+    from msq import capi, generators as G, selector as SEL, solver as MS
 
     t = time.perf_counter()
     mats = G.batched_uneven(count=args.count)
@@ -140,9 +144,8 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    stream = torch.cuda.Stream()
-    rt = capi.Runtime(0)
-    rt.set_stream(stream)
+    stream = torch.cuda.current_stream()
+    rt = capi.Runtime(0, stream=stream)
     A = torch.from_numpy(aflat).to(dev)
     X = torch.zeros(total, dtype=torch.float64, device=dev)
 
@@ -183,9 +186,21 @@ def main():
         f"{np.mean([r['sweeps'] for r in rep_plain]):.2f}, mean corrections "
         f"{np.mean([r['corrections'] for r in rep_plain]):.2f}")
 
-    tier = MS.tier_options(args.tier, rate=args.rate, max_corrections=12)
-    dt_low, (rep_low, fallback) = timed(
-        lambda: MS.solve_with_fallback(rt, orders, A, X, tier))
+    use_selector = not args.no_selector and SEL.WEIGHTS.exists()
+    if use_selector:
+        sel = SEL.Selector.load()
+        chosen = []
+
+        def low_bit():
+            reports, fb, tiers = SEL.solve_selected(rt, sel, orders, A, X)
+            chosen[:] = [tiers]
+            return reports, fb
+    else:
+        tier = MS.tier_options(args.tier, rate=args.rate, max_corrections=12)
+
+        def low_bit():
+            return MS.solve_with_fallback(rt, orders, A, X, tier)
+    dt_low, (rep_low, fallback) = timed(low_bit)
     err_low = check_all(X, f"low-bit {args.tier}")
     log(f"low-bit {args.tier}: {dt_low * 1e3:.1f} ms/step, {len(mats) / dt_low:.0f} "
         f"matrices/s, fallback {len(fallback)}/{len(mats)}, mean sweeps "
@@ -218,6 +233,10 @@ def main():
                        "ms_per_step": round(dt_plain * 1e3, 3),
                        "max_rel_error": err_plain, "converged": ok_plain},
         "low_bit": {"matrices_per_s": round(len(mats) / dt_low, 2),
+                    "selector": ({"tier_counts": dict(zip(
+                        [t[0] for t in SEL.TIERS],
+                        np.bincount(chosen[0], minlength=len(SEL.TIERS)).tolist()))}
+                                 if use_selector else None),
                     "max_rel_error": err_low, "fallbacks": len(fallback),
                     "mean_sweeps": float(np.mean([r["sweeps"] for r in rep_low])),
                     "mean_corrections": float(np.mean([r["corrections"] for r in rep_low]))},

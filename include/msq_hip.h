@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define MSQ_ABI_VERSION 2
+#define MSQ_ABI_VERSION 3
 
 enum msq_status {
   MSQ_OK = 0,
@@ -157,7 +157,11 @@ int msq_abi_version(void);
 int msq_runtime_create(int device, msq_runtime **out);
 /* ~MetalRuntime. */
 int msq_runtime_destroy(msq_runtime *runtime);
-/* Enqueue on a caller-owned stream instead (NULL restores the runtime's). */
+/* Enqueue on a caller's stream instead: any hipStream_t, NULL being HIP's
+ * null stream; MSQ_OWN_STREAM restores the runtime's own stream.  Work that a
+ * call spreads over the runtime's internal streams is forked from and joined
+ * back into this stream with events. */
+#define MSQ_OWN_STREAM ((void *)(~(uintptr_t)0))
 int msq_runtime_set_stream(msq_runtime *runtime, void *hip_stream);
 int msq_synchronize(msq_runtime *runtime);
 
@@ -262,6 +266,51 @@ int msq_solve_batched(msq_runtime *runtime, uint32_t count, const uint32_t *orde
  * call on this runtime (device pointers owned by the runtime, valid until the
  * next solve; float, batch / vector layout). */
 int msq_last_factors(msq_runtime *runtime, const float **vectors, const float **roots);
+
+/* ------------------------------------------------------- learned selector */
+/* No counterpart in the reference: the tier selector of the batched solver.
+ * A small MLP (MSQ_MLP_INPUTS -> MSQ_MLP_HIDDEN -> MSQ_MLP_HIDDEN ->
+ * MSQ_MLP_OUTPUTS, ReLU, float32) reads per-matrix features and predicts,
+ * per solver configuration, whether it meets the accuracy target.  Parameter
+ * vector (MSQ_MLP_PARAMS floats): W1[64][16], b1[64], W2[64][64], b2[64],
+ * W3[8][64], b3[8], each W row-major (W[j][i]: input i into output j).
+ * The four calls below only enqueue their work on the runtime's stream and
+ * return before it completes (training issues thousands of them). */
+#define MSQ_MLP_INPUTS 16
+#define MSQ_MLP_HIDDEN 64
+#define MSQ_MLP_OUTPUTS 8
+#define MSQ_MLP_PARAMS 5768
+#define MSQ_FEATURE_MAX_ORDER 512
+
+/* Features of each SPD matrix (binary64, batch layout, orders <= 512) into
+ * features (device, count x MSQ_MLP_INPUTS floats): order, power-iteration
+ * lambda_max / lambda_min estimates (16 / 24 steps), Gershgorin bounds,
+ * diagonal spread and dominance, norms. */
+int msq_matrix_features(msq_runtime *runtime, uint32_t count, const uint32_t *orders,
+                        const double *matrices, float *features);
+
+/* logits (device, batch x MSQ_MLP_OUTPUTS) of the network at params for the
+ * inputs x (device, batch x MSQ_MLP_INPUTS).  h1 / h2 (device, batch x
+ * MSQ_MLP_HIDDEN, post-ReLU) are written when non-null, for the backward. */
+int msq_mlp_forward(msq_runtime *runtime, const float *params, const float *x, uint32_t batch,
+                    float *logits, float *h1, float *h2);
+
+/* Gradient (device, MSQ_MLP_PARAMS) of the masked binary cross-entropy
+ *   L = sum_{b,k} mask_bk (softplus(z_bk) - y_bk z_bk) / mask_total
+ * at the forward pass (x, h1, h2, logits) with targets y and mask (device,
+ * batch x MSQ_MLP_OUTPUTS); per-sample losses (device, batch, unnormalised)
+ * into losses when non-null.  Deterministic (fixed summation order). */
+int msq_mlp_backward(msq_runtime *runtime, const float *params, const float *x, const float *h1,
+                     const float *h2, const float *logits, const float *targets,
+                     const float *mask, uint32_t batch, float mask_total, float *grads,
+                     float *losses);
+
+/* One Adam step (step >= 1, bias-corrected moments) with an L2 term added
+ * to the gradient, g = grads + decay params; m1, m2 the first and second
+ * moments (device, count floats). */
+int msq_mlp_adam(msq_runtime *runtime, float *params, const float *grads, float *m1, float *m2,
+                 uint32_t count, float lr, float beta1, float beta2, float eps, float decay,
+                 uint32_t step);
 
 #ifdef __cplusplus
 }

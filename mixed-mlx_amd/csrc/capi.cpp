@@ -482,8 +482,9 @@ void tier_want(Scratch& S, TierBuffers& T, int tier, const Layout& L) {
 }
 
 // Encodes src (float, batch layout) for the segments of one group.  BF16 and
-// MXFP4 are elementwise / 32-block codecs over the whole segment range: the
-// segments of a group are encoded one contiguous run at a time.
+// MXFP4 are elementwise / 32-block codecs (tiles are whole 32-blocks): one
+// launch over the group's span of the batch.  FP8 carries a per-matrix scale:
+// one segmented launch.
 int encode_operand(msq_runtime* rt, int tier, const Layout& L, const std::vector<uint32_t>& members,
                    const msq::Segment* dsegs, const float* src, TierBuffers& T, Operand& out) {
   if (tier == MSQ_FP32) {
@@ -499,18 +500,21 @@ int encode_operand(msq_runtime* rt, int tier, const Layout& L, const std::vector
     return MSQ_OK;
   }
   const msq::Format f = tier == MSQ_BF16 ? msq::Format::Bf16 : msq::Format::Mxfp4E2M1;
-  size_t k = 0;
-  while (k < members.size()) {
-    size_t e = k + 1;
-    while (e < members.size() && members[e] == members[e - 1] + 1) ++e;
-    const uint64_t begin = L.off[members[k]];
-    const uint64_t end = L.off[members[e - 1]] + static_cast<uint64_t>(L.ld[members[e - 1]]) * L.ld[members[e - 1]];
+  // One launch over the span from the first member's tile to the last one's.
+  // Tiles of other matrices inside the span are encoded too (their codes are
+  // never read): with members interleaved across tiers, one launch over the
+  // span is far cheaper than one launch per contiguous run.
+  if (!members.empty()) {
+    uint64_t begin = UINT64_MAX, end = 0;
+    for (uint32_t i : members) {
+      begin = std::min<uint64_t>(begin, L.off[i]);
+      end = std::max<uint64_t>(end, L.off[i] + static_cast<uint64_t>(L.ld[i]) * L.ld[i]);
+    }
     void* dst = f == msq::Format::Bf16
                     ? static_cast<void*>(static_cast<uint16_t*>(T.codes) + begin)
                     : static_cast<void*>(static_cast<uint8_t*>(T.codes) + begin / 2);
     HIP_TRY(msq::launch_encode(f, src + begin, dst, T.scales ? T.scales + begin / 32 : nullptr,
                                end - begin, rt->stream));
-    k = e;
   }
   out.data = T.codes;
   out.scales = T.scales;
@@ -736,7 +740,7 @@ int msq_runtime_destroy(msq_runtime* rt) {
 
 int msq_runtime_set_stream(msq_runtime* rt, void* stream) {
   MSQ_TRY(activate(rt));
-  rt->stream = stream ? static_cast<hipStream_t>(stream) : rt->own_stream;
+  rt->stream = stream == MSQ_OWN_STREAM ? rt->own_stream : static_cast<hipStream_t>(stream);
   return MSQ_OK;
 }
 
@@ -1242,6 +1246,76 @@ int msq_last_factors(msq_runtime* rt, const float** vectors, const float** roots
   if (!rt->last_vectors) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "no solve has run");
   if (vectors) *vectors = rt->last_vectors;
   if (roots) *roots = rt->last_roots;
+  return MSQ_OK;
+}
+
+
+// ------------------------------------------------------------ tier selector
+static_assert(msq::kMlpParams == MSQ_MLP_PARAMS && msq::kMlpInputs == MSQ_MLP_INPUTS &&
+                  msq::kMlpHidden == MSQ_MLP_HIDDEN && msq::kMlpOutputs == MSQ_MLP_OUTPUTS &&
+                  msq::kMaxFeatureOrder == MSQ_FEATURE_MAX_ORDER,
+              "selector constants of the C ABI");
+
+int msq_matrix_features(msq_runtime* rt, uint32_t count, const uint32_t* orders,
+                        const double* matrices, float* features) {
+  MSQ_TRY(activate(rt));
+  Layout L;
+  MSQ_TRY(make_layout(count, orders, L));
+  if (!matrices || !features) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  for (uint32_t i = 0; i < count; ++i)
+    if (L.order[i] > static_cast<uint32_t>(msq::kMaxFeatureOrder))
+      return set_error(MSQ_ERROR_INVALID_ARGUMENT, "order %u above %d", L.order[i],
+                       msq::kMaxFeatureOrder);
+  const std::vector<msq::Segment> segs = segments_of(L, iota_members(count));
+  Scratch S{rt};
+  msq::Segment* dsegs = nullptr;
+  S.want(dsegs, count);
+  MSQ_TRY(S.commit());
+  MSQ_TRY(upload(rt, dsegs, segs));
+  HIP_TRY(msq::launch_features(dsegs, count, matrices, features, rt->stream));
+  return MSQ_OK;
+}
+
+int msq_mlp_forward(msq_runtime* rt, const float* params, const float* x, uint32_t batch,
+                    float* logits, float* h1, float* h2) {
+  MSQ_TRY(activate(rt));
+  if (!params || !x || !logits) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  if (batch > (1u << 24)) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "batch too large");
+  HIP_TRY(msq::launch_mlp_forward(params, x, static_cast<int>(batch), logits, h1, h2,
+                                  rt->stream));
+  return MSQ_OK;
+}
+
+int msq_mlp_backward(msq_runtime* rt, const float* params, const float* x, const float* h1,
+                     const float* h2, const float* logits, const float* targets,
+                     const float* mask, uint32_t batch, float mask_total, float* grads,
+                     float* losses) {
+  MSQ_TRY(activate(rt));
+  if (!params || !x || !h1 || !h2 || !logits || !targets || !mask || !grads)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  if (batch == 0 || batch > (1u << 24))
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "batch must be in [1, 2^24]");
+  Scratch S{rt};
+  float *dz = nullptr, *dh1 = nullptr, *dh2 = nullptr, *loss = nullptr;
+  S.want(dz, static_cast<size_t>(batch) * msq::kMlpOutputs);
+  S.want(dh1, static_cast<size_t>(batch) * msq::kMlpHidden);
+  S.want(dh2, static_cast<size_t>(batch) * msq::kMlpHidden);
+  if (!losses) S.want(loss, batch);
+  MSQ_TRY(S.commit());
+  HIP_TRY(msq::launch_mlp_backward(params, x, h1, h2, logits, targets, mask,
+                                   static_cast<int>(batch), mask_total, dz, dh1, dh2,
+                                   losses ? losses : loss, grads, rt->stream));
+  return MSQ_OK;
+}
+
+int msq_mlp_adam(msq_runtime* rt, float* params, const float* grads, float* m1, float* m2,
+                 uint32_t count, float lr, float beta1, float beta2, float eps, float decay,
+                 uint32_t step) {
+  MSQ_TRY(activate(rt));
+  if (!params || !grads || !m1 || !m2) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  if (step == 0) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "step counts from 1");
+  HIP_TRY(msq::launch_mlp_adam(params, grads, m1, m2, static_cast<int>(count), lr, beta1, beta2,
+                               eps, decay, static_cast<int>(step), rt->stream));
   return MSQ_OK;
 }
 

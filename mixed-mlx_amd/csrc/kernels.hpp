@@ -84,6 +84,30 @@ hipError_t launch_global_offnorm(const float* A, uint32_t n, uint32_t ld, double
 hipError_t launch_global_jacobi_finish(const float* A, uint32_t n, uint32_t ld,
                                        float* diagonal, float* roots, hipStream_t stream);
 
+// ------------------------------------------------------ learned selector
+constexpr int kMlpInputs = 16;
+constexpr int kMlpHidden = 64;
+constexpr int kMlpOutputs = 8;
+constexpr int kMlpParams = kMlpHidden * kMlpInputs + kMlpHidden + kMlpHidden * kMlpHidden +
+                           kMlpHidden + kMlpOutputs * kMlpHidden + kMlpOutputs;  // 5768
+constexpr int kMaxFeatureOrder = 512;
+constexpr int kPowerIterationsMax = 16;  // lambda_max: power iteration on A
+constexpr int kPowerIterationsMin = 24;  // lambda_min: on (c I - A), c > lambda_max
+
+struct Segment;
+hipError_t launch_features(const Segment* segs, uint32_t count, const double* a64,
+                           float* features, hipStream_t stream);
+hipError_t launch_mlp_forward(const float* params, const float* x, int batch, float* logits,
+                              float* h1, float* h2, hipStream_t stream);
+hipError_t launch_mlp_backward(const float* params, const float* x, const float* h1,
+                               const float* h2, const float* logits, const float* targets,
+                               const float* mask, int batch, float mask_total, float* dz,
+                               float* dh1, float* dh2, float* losses, float* grads,
+                               hipStream_t stream);
+hipError_t launch_mlp_adam(float* params, const float* grads, float* m1, float* m2, int count,
+                           float lr, float beta1, float beta2, float eps, float decay, int step,
+                           hipStream_t stream);
+
 // -------------------------------------------------------------------- GEMM
 // Operand element types of the batched square GEMMs.
 enum class GemmType : int { F64 = 0, F32 = 1, Bf16 = 2, Fp8 = 3, Mxfp4 = 4 };

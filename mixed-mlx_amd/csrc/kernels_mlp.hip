@@ -1,0 +1,408 @@
+// Learned tier selector: per-matrix features and a small MLP (forward,
+// backward, Adam) trained on the GPU.
+//
+// No counterpart in the reference.  The selector reads cheap features of each
+// SPD matrix -- order, Gershgorin bounds, diagonal dominance, power-iteration
+// estimates of lambda_max and lambda_min -- and predicts, for each of the
+// solver configurations it chooses between, whether that configuration meets
+// the accuracy target.
+//
+// Network: 16 inputs -> 64 (ReLU) -> 64 (ReLU) -> 8 logits, float32.
+// Parameter vector (5768 floats): W1[64][16], b1[64], W2[64][64], b2[64],
+// W3[8][64], b3[8], each weight matrix row-major (W[j][i] multiplies input i
+// into output j).
+// Loss: masked binary cross-entropy with logits, averaged over the mask.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include "kernels.hpp"
+
+namespace msq {
+namespace {
+
+constexpr int kIn = kMlpInputs, kHid = kMlpHidden, kOut = kMlpOutputs;
+constexpr int kW1 = 0, kB1 = kW1 + kHid * kIn, kW2 = kB1 + kHid, kB2 = kW2 + kHid * kHid,
+              kW3 = kB2 + kHid, kB3 = kW3 + kOut * kHid, kParams = kB3 + kOut;
+static_assert(kParams == kMlpParams, "parameter layout");
+
+// ------------------------------------------------------------------ features
+constexpr int kFeatThreads = 256;
+
+// Block-wide sum in a fixed order (wave shuffles, then wave partials in
+// wave order): the result does not depend on timing.
+__device__ double block_sum(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < kFeatThreads / 64; ++w) s += red[w];
+  return s;
+}
+__device__ double block_min(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = red[0];
+  for (int w = 1; w < kFeatThreads / 64; ++w) s = fmin(s, red[w]);
+  return s;
+}
+__device__ double block_max(double v, double* red) { return -block_min(-v, red); }
+
+// Start vector of the lambda_min iteration: a fixed hash of the index in
+// [-0.5, 0.5) (any vector with a component along the bottom eigenvector).
+__device__ __forceinline__ double start_vector(int j) {
+  const uint32_t h = static_cast<uint32_t>(j + 1) * 2654435761u;
+  return static_cast<double>((h >> 16) & 0xffffu) / 65536.0 - 0.5;
+}
+
+// One workgroup per matrix; A binary64 column-major (pitch ld), order <= 512.
+__global__ __launch_bounds__(kFeatThreads) void features_kernel(const Segment* __restrict__ segs,
+                                                                const double* __restrict__ a64,
+                                                                float* __restrict__ features) {
+  __shared__ double x[kMaxFeatureOrder];
+  __shared__ double red[kFeatThreads / 64];
+  const Segment sg = segs[blockIdx.x];
+  const int n = static_cast<int>(sg.order);
+  const int ld = static_cast<int>(sg.ld);
+  const double* A = a64 + sg.offset;
+  const int tid = threadIdx.x;
+  constexpr int kRows = kMaxFeatureOrder / kFeatThreads;  // rows per thread
+
+  // Pass 1: diagonal, off-diagonal row sums, Frobenius norms.
+  double diag[kRows], rsum[kRows];
+  double fro = 0.0, off = 0.0;
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    diag[r] = 0.0;
+    rsum[r] = 0.0;
+  }
+  for (int j = 0; j < n; ++j) {
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int i = tid + r * kFeatThreads;
+      if (i < n) {
+        const double v = A[i + static_cast<size_t>(j) * ld];
+        fro += v * v;
+        if (i == j) {
+          diag[r] = v;
+        } else {
+          rsum[r] += fabs(v);
+          off += v * v;
+        }
+      }
+    }
+  }
+  double tr = 0.0, dmin = INFINITY, dmax = -INFINITY, gl = INFINITY, gh = -INFINITY;
+  double logd = 0.0, logd2 = 0.0, dommin = INFINITY, domsum = 0.0, weak = 0.0;
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const int i = tid + r * kFeatThreads;
+    if (i < n) {
+      const double d = diag[r];
+      tr += d;
+      dmin = fmin(dmin, d);
+      dmax = fmax(dmax, d);
+      gl = fmin(gl, d - rsum[r]);
+      gh = fmax(gh, d + rsum[r]);
+      const double ld10 = log10(fmax(d, 1e-300));
+      logd += ld10;
+      logd2 += ld10 * ld10;
+      const double dom = fmin(fmax(d / fmax(rsum[r], 1e-300), 1e-6), 1e6);
+      dommin = fmin(dommin, dom);
+      domsum += dom;
+      weak += d < rsum[r] ? 1.0 : 0.0;
+    }
+  }
+  tr = block_sum(tr, red);
+  fro = block_sum(fro, red);
+  off = block_sum(off, red);
+  logd = block_sum(logd, red);
+  logd2 = block_sum(logd2, red);
+  domsum = block_sum(domsum, red);
+  weak = block_sum(weak, red);
+  dmin = block_min(dmin, red);
+  dmax = block_max(dmax, red);
+  gl = block_min(gl, red);
+  gh = block_max(gh, red);
+  dommin = block_min(dommin, red);
+
+  // y = A x (c != 0: y = c x - A x) for this thread's rows; x in LDS.
+  auto matvec = [&](double c, double (&y)[kRows]) {
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) y[r] = 0.0;
+    for (int j = 0; j < n; ++j) {
+      const double xj = x[j];
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        const int i = tid + r * kFeatThreads;
+        if (i < n) y[r] += A[i + static_cast<size_t>(j) * ld] * xj;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int i = tid + r * kFeatThreads;
+      if (i < n) y[r] = c != 0.0 ? c * x[i] - y[r] : y[r];
+    }
+  };
+  // Power iteration: `iters` steps of x <- y / ||y|| with y = (c I - A) x
+  // (c = 0: y = A x); returns the last two Rayleigh quotients x^T y.
+  auto power = [&](double c, int iters, double& last, double& prev) {
+    double y[kRows];
+    last = prev = 0.0;
+    for (int it = 0; it < iters; ++it) {
+      matvec(c, y);
+      double q = 0.0, yy = 0.0;
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        const int i = tid + r * kFeatThreads;
+        if (i < n) {
+          q += x[i] * y[r];
+          yy += y[r] * y[r];
+        }
+      }
+      q = block_sum(q, red);
+      yy = block_sum(yy, red);
+      prev = last;
+      last = q;
+      const double inv = yy > 0.0 ? 1.0 / sqrt(yy) : 0.0;
+      __syncthreads();  // every thread has read x
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        const int i = tid + r * kFeatThreads;
+        if (i < n) x[i] = y[r] * inv;
+      }
+      __syncthreads();
+    }
+  };
+  const double inv_sqrt_n = 1.0 / sqrt(static_cast<double>(n));
+  for (int i = tid; i < n; i += kFeatThreads) x[i] = inv_sqrt_n;
+  __syncthreads();
+  double lmax, lmax_prev;
+  power(0.0, kPowerIterationsMax, lmax, lmax_prev);
+  // Bottom of the spectrum: top of (c I - A) with c just above lambda_max.
+  const double c = 1.0001 * fmax(lmax, 1e-300);
+  double snorm = 0.0;
+  for (int i = tid; i < n; i += kFeatThreads) snorm += start_vector(i) * start_vector(i);
+  snorm = block_sum(snorm, red);
+  for (int i = tid; i < n; i += kFeatThreads) x[i] = start_vector(i) / sqrt(snorm);
+  __syncthreads();
+  double mu, mu_prev;
+  power(c, kPowerIterationsMin, mu, mu_prev);
+  const double lmin = c - mu;
+
+  if (tid == 0) {
+    const double t = tr / n;
+    const double lmax_c = fmax(lmax, 1e-300);
+    const double lmin_c = fmax(lmin, lmax_c * 1e-16);
+    const double mean_logd = logd / n;
+    float* f = features + static_cast<size_t>(blockIdx.x) * kIn;
+    f[0] = static_cast<float>(n / 256.0);
+    f[1] = static_cast<float>(log2(static_cast<double>(n)) / 8.0);
+    f[2] = static_cast<float>(log10(lmax_c / lmin_c));
+    f[3] = static_cast<float>(log10(lmax_c / t));
+    f[4] = static_cast<float>(log10(lmin_c / t));
+    f[5] = static_cast<float>(fmin(fmax(gl / lmax_c, -16.0), 1.0));
+    f[6] = static_cast<float>(log10(fmax(gh, 1e-300) / lmax_c));
+    f[7] = static_cast<float>(log10(fmax(dmin, 1e-300) / fmax(dmax, 1e-300)));
+    f[8] = static_cast<float>(sqrt(fmax(logd2 / n - mean_logd * mean_logd, 0.0)));
+    f[9] = static_cast<float>(log10(dommin));
+    f[10] = static_cast<float>(log10(domsum / n));
+    f[11] = static_cast<float>(fro > 0.0 ? sqrt(off / fro) : 0.0);
+    f[12] = static_cast<float>(log10(fmax(sqrt(fro) * sqrt(static_cast<double>(n)) / tr, 1e-300)));
+    f[13] = static_cast<float>(log10(fabs(lmax - lmax_prev) / lmax_c + 1e-30));
+    f[14] = static_cast<float>(log10(fabs(mu - mu_prev) / fmax(fabs(mu), 1e-300) + 1e-30));
+    f[15] = static_cast<float>(weak / n);
+  }
+}
+
+// ----------------------------------------------------------------------- MLP
+constexpr int kMlpThreads = 256;
+
+__device__ __forceinline__ void stage_params(const float* __restrict__ params, float* p) {
+  for (int e = threadIdx.x; e < kParams; e += blockDim.x) p[e] = params[e];
+  __syncthreads();
+}
+
+// One thread per sample.  h1 / h2 (post-ReLU) are stored when non-null.
+__global__ __launch_bounds__(kMlpThreads) void mlp_forward_kernel(
+    const float* __restrict__ params, const float* __restrict__ x, int batch,
+    float* __restrict__ logits, float* __restrict__ h1_out, float* __restrict__ h2_out) {
+  __shared__ float p[kParams];
+  stage_params(params, p);
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  float xi[kIn];
+#pragma unroll
+  for (int i = 0; i < kIn; ++i) xi[i] = x[static_cast<size_t>(b) * kIn + i];
+  float h1[kHid];
+#pragma unroll
+  for (int j = 0; j < kHid; ++j) {
+    float acc = p[kB1 + j];
+#pragma unroll
+    for (int i = 0; i < kIn; ++i) acc = fmaf(p[kW1 + j * kIn + i], xi[i], acc);
+    h1[j] = fmaxf(acc, 0.0f);
+    if (h1_out) h1_out[static_cast<size_t>(b) * kHid + j] = h1[j];
+  }
+  float z[kOut];
+#pragma unroll
+  for (int k = 0; k < kOut; ++k) z[k] = p[kB3 + k];
+  for (int j = 0; j < kHid; ++j) {
+    float acc = p[kB2 + j];
+#pragma unroll
+    for (int i = 0; i < kHid; ++i) acc = fmaf(p[kW2 + j * kHid + i], h1[i], acc);
+    const float h2 = fmaxf(acc, 0.0f);
+    if (h2_out) h2_out[static_cast<size_t>(b) * kHid + j] = h2;
+#pragma unroll
+    for (int k = 0; k < kOut; ++k) z[k] = fmaf(p[kW3 + k * kHid + j], h2, z[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < kOut; ++k) logits[static_cast<size_t>(b) * kOut + k] = z[k];
+}
+
+// Per-sample deltas: dz = mask (sigmoid(z) - y) / mask_total,
+// dh2 = (W3^T dz) o [h2 > 0], dh1 = (W2^T dh2) o [h1 > 0]; and the sample's
+// loss sum_k mask (softplus(z) - y z).
+__global__ __launch_bounds__(kMlpThreads) void mlp_delta_kernel(
+    const float* __restrict__ params, const float* __restrict__ logits,
+    const float* __restrict__ h1, const float* __restrict__ h2,
+    const float* __restrict__ targets, const float* __restrict__ mask, int batch,
+    float inv_total, float* __restrict__ dz_out, float* __restrict__ dh1_out,
+    float* __restrict__ dh2_out, float* __restrict__ loss_out) {
+  __shared__ float p[kParams];
+  stage_params(params, p);
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  float dz[kOut];
+  float loss = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kOut; ++k) {
+    const size_t e = static_cast<size_t>(b) * kOut + k;
+    const float z = logits[e], y = targets[e], m = mask[e];
+    const float sig = 1.0f / (1.0f + expf(-z));
+    dz[k] = m * (sig - y) * inv_total;
+    // softplus(z) - y z, evaluated without overflow
+    loss += m * (fmaxf(z, 0.0f) + log1pf(expf(-fabsf(z))) - y * z);
+    dz_out[e] = dz[k];
+  }
+  loss_out[b] = loss;
+  float dh2[kHid];
+#pragma unroll
+  for (int j = 0; j < kHid; ++j) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kOut; ++k) acc = fmaf(p[kW3 + k * kHid + j], dz[k], acc);
+    dh2[j] = h2[static_cast<size_t>(b) * kHid + j] > 0.0f ? acc : 0.0f;
+    dh2_out[static_cast<size_t>(b) * kHid + j] = dh2[j];
+  }
+  for (int i = 0; i < kHid; ++i) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int j = 0; j < kHid; ++j) acc = fmaf(p[kW2 + j * kHid + i], dh2[j], acc);
+    dh1_out[static_cast<size_t>(b) * kHid + i] = h1[static_cast<size_t>(b) * kHid + i] > 0.0f ? acc : 0.0f;
+  }
+}
+
+// Parameter gradients, one thread per parameter, summing over the batch in
+// sample order (deterministic).
+__global__ __launch_bounds__(kMlpThreads) void mlp_grad_kernel(
+    const float* __restrict__ x, const float* __restrict__ h1, const float* __restrict__ h2,
+    const float* __restrict__ dz, const float* __restrict__ dh1, const float* __restrict__ dh2,
+    int batch, float* __restrict__ grads) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= kParams) return;
+  const float* delta;  // per-sample output delta, row stride dstride
+  const float* act;    // per-sample input activation (null for a bias)
+  int dstride, astride, j, i;
+  if (q < kB1) {
+    j = (q - kW1) / kIn; i = (q - kW1) % kIn;
+    delta = dh1; dstride = kHid; act = x; astride = kIn;
+  } else if (q < kW2) {
+    j = q - kB1; i = 0;
+    delta = dh1; dstride = kHid; act = nullptr; astride = 0;
+  } else if (q < kB2) {
+    j = (q - kW2) / kHid; i = (q - kW2) % kHid;
+    delta = dh2; dstride = kHid; act = h1; astride = kHid;
+  } else if (q < kW3) {
+    j = q - kB2; i = 0;
+    delta = dh2; dstride = kHid; act = nullptr; astride = 0;
+  } else if (q < kB3) {
+    j = (q - kW3) / kHid; i = (q - kW3) % kHid;
+    delta = dz; dstride = kOut; act = h2; astride = kHid;
+  } else {
+    j = q - kB3; i = 0;
+    delta = dz; dstride = kOut; act = nullptr; astride = 0;
+  }
+  float g = 0.0f;
+  if (act) {
+    for (int b = 0; b < batch; ++b)
+      g = fmaf(delta[static_cast<size_t>(b) * dstride + j], act[static_cast<size_t>(b) * astride + i], g);
+  } else {
+    for (int b = 0; b < batch; ++b) g += delta[static_cast<size_t>(b) * dstride + j];
+  }
+  grads[q] = g;
+}
+
+__global__ void mlp_adam_kernel(float* __restrict__ params, const float* __restrict__ grads,
+                                float* __restrict__ m1, float* __restrict__ m2, int count,
+                                float lr, float beta1, float beta2, float eps, float decay,
+                                float corr1, float corr2) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= count) return;
+  const float g = grads[q] + decay * params[q];
+  const float a = beta1 * m1[q] + (1.0f - beta1) * g;
+  const float v = beta2 * m2[q] + (1.0f - beta2) * g * g;
+  m1[q] = a;
+  m2[q] = v;
+  params[q] -= lr * (a / corr1) / (sqrtf(v / corr2) + eps);
+}
+
+}  // namespace
+
+hipError_t launch_features(const Segment* segs, uint32_t count, const double* a64,
+                           float* features, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  features_kernel<<<count, kFeatThreads, 0, stream>>>(segs, a64, features);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_forward(const float* params, const float* x, int batch, float* logits,
+                              float* h1, float* h2, hipStream_t stream) {
+  if (batch <= 0) return hipSuccess;
+  mlp_forward_kernel<<<(batch + kMlpThreads - 1) / kMlpThreads, kMlpThreads, 0, stream>>>(
+      params, x, batch, logits, h1, h2);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_backward(const float* params, const float* x, const float* h1,
+                               const float* h2, const float* logits, const float* targets,
+                               const float* mask, int batch, float mask_total, float* dz,
+                               float* dh1, float* dh2, float* losses, float* grads,
+                               hipStream_t stream) {
+  if (batch <= 0) return hipSuccess;
+  const float inv = mask_total > 0.0f ? 1.0f / mask_total : 0.0f;
+  mlp_delta_kernel<<<(batch + kMlpThreads - 1) / kMlpThreads, kMlpThreads, 0, stream>>>(
+      params, logits, h1, h2, targets, mask, batch, inv, dz, dh1, dh2, losses);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  mlp_grad_kernel<<<(kParams + kMlpThreads - 1) / kMlpThreads, kMlpThreads, 0, stream>>>(
+      x, h1, h2, dz, dh1, dh2, batch, grads);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_adam(float* params, const float* grads, float* m1, float* m2, int count,
+                           float lr, float beta1, float beta2, float eps, float decay, int step,
+                           hipStream_t stream) {
+  if (count <= 0) return hipSuccess;
+  const float corr1 = 1.0f - powf(beta1, static_cast<float>(step));
+  const float corr2 = 1.0f - powf(beta2, static_cast<float>(step));
+  mlp_adam_kernel<<<(count + 255) / 256, 256, 0, stream>>>(params, grads, m1, m2, count, lr,
+                                                          beta1, beta2, eps, decay, corr1, corr2);
+  return hipGetLastError();
+}
+
+}  // namespace msq

@@ -24,6 +24,14 @@ if not LIB_PATH.exists():
         f"libmsq_hip.so not found at {LIB_PATH}; build it with "
         "`python mixed-mlx_amd/build.py` (or __graft_entry__.build())")
 
+# torch ships its own HIP runtime under the same soname as the system one the
+# library links against.  The process must load torch's first: if the
+# library's runtime is loaded first, torch's device initialisation fails.
+try:
+    import torch  # noqa: F401
+except ImportError:  # the library itself does not need torch
+    pass
+
 lib = ctypes.CDLL(str(LIB_PATH))
 
 # Precision values (src/precision.hpp:9-20 order) and the scaled FP8 tier.
@@ -42,6 +50,12 @@ SOLVE_STATUS = {0: "running", 1: "converged", 2: "budget_exhausted",
                 3: "diverged", 4: "nonfinite", 5: "invalid_spectrum"}
 JACOBI_HISTORY = 32
 SOLVE_HISTORY = 20
+OWN_STREAM = 2**64 - 1  # MSQ_OWN_STREAM
+MLP_INPUTS = 16
+MLP_HIDDEN = 64
+MLP_OUTPUTS = 8
+MLP_PARAMS = 5768
+FEATURE_MAX_ORDER = 512
 
 # Every symbol include/msq_hip.h declares.
 EXPORTED_SYMBOLS = (
@@ -52,6 +66,7 @@ EXPORTED_SYMBOLS = (
     "msq_quantize_and_decode", "msq_quantize_fp8_scaled", "msq_decode_fp8_scaled",
     "msq_jacobi_batched", "msq_reconstruct", "msq_residual_fp64", "msq_correction",
     "msq_orthogonality", "msq_solve_batched", "msq_last_factors",
+    "msq_matrix_features", "msq_mlp_forward", "msq_mlp_backward", "msq_mlp_adam",
 )
 
 
@@ -133,6 +148,11 @@ _sig("msq_orthogonality", _i, _p, _u32, _u32p, _p, _f64p)
 _sig("msq_solve_batched", _i, _p, _u32, _u32p, _p, _p, ctypes.POINTER(SolveOptions), _u32,
      ctypes.POINTER(SolveReport))
 _sig("msq_last_factors", _i, _p, ctypes.POINTER(_p), ctypes.POINTER(_p))
+_sig("msq_matrix_features", _i, _p, _u32, _u32p, _p, _p)
+_sig("msq_mlp_forward", _i, _p, _p, _p, _u32, _p, _p, _p)
+_sig("msq_mlp_backward", _i, _p, _p, _p, _p, _p, _p, _p, _p, _u32, ctypes.c_float, _p, _p)
+_sig("msq_mlp_adam", _i, _p, _p, _p, _p, _p, _u32, ctypes.c_float, ctypes.c_float,
+     ctypes.c_float, ctypes.c_float, ctypes.c_float, _u32)
 
 
 class MsqError(RuntimeError):
@@ -241,11 +261,18 @@ def report_dict(r: SolveReport):
 class Runtime:
     """HIP backend runtime (MetalRuntime's role, src/metal_runtime.hpp:69-176)."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, stream="current"):
+        """stream: "current" enqueues on torch's current stream (ordered with
+        the torch operations that fill and read the buffers), None on the
+        runtime's own stream, else a torch stream or HIP stream address."""
         h = _p()
         check(lib.msq_runtime_create(device, ctypes.byref(h)))
         self.handle = h
         self.device = device
+        if stream == "current":
+            import torch
+            stream = torch.cuda.current_stream(device)
+        self.set_stream(stream)
 
     def close(self):
         if self.handle:
@@ -260,9 +287,12 @@ class Runtime:
             pass
 
     def set_stream(self, stream=None):
-        """Enqueue on a torch stream (or HIP stream address); None: own stream."""
-        addr = None if stream is None else (stream if isinstance(stream, int)
-                                             else stream.cuda_stream)
+        """Enqueue on a torch stream (or HIP stream address, 0 = the null
+        stream); None: the runtime's own stream."""
+        if stream is None:
+            addr = OWN_STREAM
+        else:
+            addr = stream if isinstance(stream, int) else stream.cuda_stream
         check(lib.msq_runtime_set_stream(self.handle, addr))
 
     def synchronize(self):
@@ -333,3 +363,25 @@ class Runtime:
         s = _p()
         check(lib.msq_last_factors(self.handle, ctypes.byref(v), ctypes.byref(s)))
         return v.value, s.value
+
+    # tier selector ------------------------------------------------------
+    def matrix_features(self, orders, matrices, features):
+        check(lib.msq_matrix_features(self.handle, len(orders), _u32_array(orders),
+                                      _addr(matrices), _addr(features)))
+
+    def mlp_forward(self, params, x, logits, h1=None, h2=None):
+        batch = x.numel() // MLP_INPUTS
+        check(lib.msq_mlp_forward(self.handle, _addr(params), _addr(x), batch, _addr(logits),
+                                  _addr(h1), _addr(h2)))
+
+    def mlp_backward(self, params, x, h1, h2, logits, targets, mask, mask_total, grads,
+                     losses=None):
+        batch = x.numel() // MLP_INPUTS
+        check(lib.msq_mlp_backward(self.handle, _addr(params), _addr(x), _addr(h1), _addr(h2),
+                                   _addr(logits), _addr(targets), _addr(mask), batch,
+                                   float(mask_total), _addr(grads), _addr(losses)))
+
+    def mlp_adam(self, params, grads, m1, m2, lr, step, beta1=0.9, beta2=0.999, eps=1e-8,
+                 decay=0.0):
+        check(lib.msq_mlp_adam(self.handle, _addr(params), _addr(grads), _addr(m1), _addr(m2),
+                               params.numel(), lr, beta1, beta2, eps, decay, step))

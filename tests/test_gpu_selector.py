@@ -1,0 +1,149 @@
+"""GPU checks of the learned tier selector's kernels against the CPU oracle
+(oracle/mlp.py), its training, and what the trained network gains."""
+import numpy as np
+import pytest
+import torch
+
+from gpu_util import dev, host, say
+from msq import capi
+from msq import generators as G
+from msq import selector as SEL
+from oracle import mlp as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _spd(rng, n, cond):
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = cond ** (rng.random(n) - 1.0)
+    A = (Q * lam) @ Q.T
+    return 0.5 * (A + A.T)
+
+
+def test_features_match_oracle(runtime):
+    rng = np.random.default_rng(11)
+    orders = [5, 16, 17, 64, 200, 300, 512]
+    mats = [_spd(rng, n, 10.0 ** (1 + i % 4)) for i, n in enumerate(orders)]
+    mats.append(np.diag(np.array([10.0] + [5.0] * 30 + [0.1])))
+    orders.append(32)
+    say("features", orders)
+    o, aflat = capi.pack_batch(mats)
+    F = torch.zeros(len(o) * capi.MLP_INPUTS, dtype=torch.float32, device="cuda")
+    runtime.matrix_features(o, dev(aflat), F)
+    got = host(F).reshape(len(o), capi.MLP_INPUTS)
+    for A, g in zip(mats, got):
+        want = M.features(A)
+        # binary64 in both, different summation orders; float32 output.
+        plain = [k for k in range(M.INPUTS) if k not in (13, 14)]
+        np.testing.assert_allclose(g[plain], want[plain], rtol=2e-6, atol=2e-6)
+        # 13, 14: relative change of the last power step, a difference of two
+        # nearly equal Rayleigh quotients: compared on the linear scale, to
+        # the summation-order rounding of a quotient (~ n eps).
+        n = A.shape[0]
+        np.testing.assert_allclose(10.0 ** g[[13, 14]].astype(np.float64), 10.0 ** want[[13, 14]],
+                                   rtol=1e-5, atol=8 * n * 2.0**-52)
+
+
+def test_features_reject_orders_above_512(runtime):
+    A = np.eye(520)
+    o, aflat = capi.pack_batch([A])
+    F = torch.zeros(capi.MLP_INPUTS, dtype=torch.float32, device="cuda")
+    with pytest.raises(capi.MsqError):
+        runtime.matrix_features(o, dev(aflat), F)
+
+
+def _batch(seed, b):
+    rng = np.random.default_rng(seed)
+    p = (rng.standard_normal(M.PARAMS) * 0.3).astype(np.float32)
+    x = rng.standard_normal((b, M.INPUTS)).astype(np.float32)
+    y = (rng.random((b, M.OUTPUTS)) < 0.4).astype(np.float32)
+    m = (rng.random((b, M.OUTPUTS)) < 0.9).astype(np.float32)
+    return p, x, y, m
+
+
+@pytest.mark.parametrize("b", [1, 37, 700])
+def test_forward_and_backward_match_oracle(runtime, b):
+    p, x, y, m = _batch(20 + b, b)
+    P, X, Y, Mk = dev(p), dev(x), dev(y), dev(m)
+    z = torch.zeros(b * M.OUTPUTS, device="cuda")
+    h1 = torch.zeros(b * M.HIDDEN, device="cuda")
+    h2 = torch.zeros(b * M.HIDDEN, device="cuda")
+    runtime.mlp_forward(P, X, z, h1, h2)
+    zo, h1o, h2o = M.forward(p.astype(np.float64), x.astype(np.float64))
+    np.testing.assert_allclose(host(z).reshape(b, -1), zo, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(host(h2).reshape(b, -1), h2o, rtol=1e-4, atol=1e-4)
+    g = torch.zeros(M.PARAMS, device="cuda")
+    losses = torch.zeros(b, device="cuda")
+    runtime.mlp_backward(P, X, h1, h2, z, Y, Mk, float(m.sum()), g, losses)
+    go = M.backward(p.astype(np.float64), x.astype(np.float64), y, m)
+    gg = host(g)
+    scale = np.abs(go).max()
+    say("backward", b, "max |g - oracle| / max|g|", np.abs(gg - go).max() / scale)
+    assert np.abs(gg - go).max() <= 1e-4 * scale
+    lo = M.loss(p.astype(np.float64), x.astype(np.float64), y, m)
+    np.testing.assert_allclose(host(losses).sum() / m.sum(), lo, rtol=1e-4)
+
+
+def test_backward_is_deterministic(runtime):
+    p, x, y, m = _batch(31, 300)
+    P, X, Y, Mk = dev(p), dev(x), dev(y), dev(m)
+    z = torch.zeros(300 * M.OUTPUTS, device="cuda")
+    h1 = torch.zeros(300 * M.HIDDEN, device="cuda")
+    h2 = torch.zeros(300 * M.HIDDEN, device="cuda")
+    runtime.mlp_forward(P, X, z, h1, h2)
+    g1 = torch.zeros(M.PARAMS, device="cuda")
+    g2 = torch.zeros(M.PARAMS, device="cuda")
+    runtime.mlp_backward(P, X, h1, h2, z, Y, Mk, float(m.sum()), g1)
+    runtime.mlp_backward(P, X, h1, h2, z, Y, Mk, float(m.sum()), g2)
+    assert torch.equal(g1, g2)
+
+
+def test_adam_matches_oracle(runtime):
+    rng = np.random.default_rng(40)
+    p = rng.standard_normal(M.PARAMS).astype(np.float32)
+    P = dev(p.copy())
+    m1 = torch.zeros(M.PARAMS, device="cuda")
+    m2 = torch.zeros(M.PARAMS, device="cuda")
+    po, a, v = p.astype(np.float64), np.zeros(M.PARAMS), np.zeros(M.PARAMS)
+    for step in range(1, 4):
+        g = rng.standard_normal(M.PARAMS).astype(np.float32)
+        runtime.mlp_adam(P, dev(g), m1, m2, 1e-2, step, decay=1e-3)
+        po, a, v = M.adam(po, g.astype(np.float64), a, v, 1e-2, step, decay=1e-3)
+    np.testing.assert_allclose(host(P), po, rtol=1e-5, atol=1e-6)
+
+
+def test_training_fits_a_learnable_rule(runtime):
+    """Targets that a feature threshold decides: the GPU-trained network
+    learns them (loss falls, accuracy > 95 %)."""
+    rng = np.random.default_rng(50)
+    x = rng.standard_normal((4000, M.INPUTS)).astype(np.float32)
+    y = np.stack([(x[:, k % M.INPUTS] + 0.5 * x[:, (k + 3) % M.INPUTS] > 0.2 * k - 0.7)
+                  for k in range(M.OUTPUTS)], axis=1).astype(np.float32)
+    params, hist = SEL.train(runtime, x, y, epochs=40, batch=256, lr=3e-3)
+    z, _, _ = M.forward(params.astype(np.float64), x.astype(np.float64))
+    acc = np.mean((z > 0) == (y > 0.5))
+    say("training", hist[0], hist[-1], acc)
+    assert hist[-1] < 0.5 * hist[0] and acc > 0.95
+
+
+def test_trained_selector_beats_the_fixed_tier(runtime):
+    """Held-out batch (seed 13, never used in training): the committed
+    network's tier choices cost less than the fixed BF16 tier with the same
+    plain-path fallback, measured on the solver's own runs of every tier."""
+    if not SEL.WEIGHTS.exists():
+        pytest.skip("no trained selector weights")
+    sel = SEL.Selector.load()
+    mats = G.batched_uneven(count=768, config=13)
+    say("selector held-out batch", len(mats))
+    x, success, sweeps, corr, orders = SEL.label_batch(runtime, mats, log=say)
+    logits = torch.zeros(len(mats) * M.OUTPUTS, device="cuda")
+    runtime.mlp_forward(dev(sel.params), dev(x), logits)
+    prob = 1.0 / (1.0 + np.exp(-host(logits).reshape(len(mats), -1).astype(np.float64)))
+    choice = sel.choose(prob)
+    c_sel, f_sel = SEL.policy_cost(choice, success, sweeps, corr, orders)
+    c_fix, f_fix = SEL.policy_cost(np.ones(len(mats), int), success, sweeps, corr, orders)
+    c_plain, _ = SEL.policy_cost(np.zeros(len(mats), int), success, sweeps, corr, orders)
+    say("cost selector", c_sel, f_sel, "fixed", c_fix, f_fix, "plain", c_plain,
+        "choices", np.bincount(choice, minlength=8).tolist())
+    assert success[:, 0].all()          # the fallback always meets the target
+    assert c_sel < c_fix < c_plain
