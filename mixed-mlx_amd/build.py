@@ -60,7 +60,28 @@ def build(jobs: int = 4) -> pathlib.Path:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, OUT)
+    build_cli()
     return OUT
+
+
+CLI = HERE / "msq" / "msq_cli"
+CLI_SOURCES = ["hip_runtime.cpp", "msq_cli.cpp"]
+
+
+def build_cli() -> pathlib.Path:
+    """The C++ mirror of the solver entry points (hip_runtime.cpp) and its
+    command-line driver: host code over the C ABI, built with g++."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Wextra",
+           *[str(CSRC / s) for s in CLI_SOURCES], "-o", str(CLI) + ".tmp",
+           "-L", str(OUT.parent), "-lmsq_hip", "-Wl,-rpath,$ORIGIN",
+           f"-Wl,-rpath-link,{rocm}/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("msq_cli build failed")
+    os.replace(str(CLI) + ".tmp", CLI)
+    return CLI
 
 
 if __name__ == "__main__":
