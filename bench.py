@@ -39,6 +39,10 @@ sys.path[:0] = [str(ROOT), str(ROOT / "mixed-mlx_amd")]
 
 METRIC = "SPD sqrts/sec at ≤1e-6 rel err vs FP64: low-bit+refine vs plain FP32, 1 GPU"
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per launch of the roofline kernel on the default workload: two
+# rocprofv3 --pmc runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a
+# 256 MiB copy gives), profiles/jacobi_n256_hbm_traffic_pmc.md.
+ROOFLINE_TRAFFIC_BYTES = 2.627e10
 
 
 def log(*a):
@@ -78,7 +82,7 @@ def roofline_leg(args, mats, orders, plain, rt, stream, torch, np, capi, dev):
     achieved = flops / jt / 1e12
     return {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-            "traffic": None,
+            "traffic": ROOFLINE_TRAFFIC_BYTES if len(mats) == 4096 else None,
             "kernel": (f"batched_jacobi_kernel<16> (orders 241..256: {len(big)} matrices, "
                        f"sweeps mean {np.mean(sweeps):.2f} max {max(sweeps)}, "
                        f"{flops / 1e12:.3f} TFLOP per launch at 9 S n^2 (n - 1), "
