@@ -223,30 +223,64 @@ msq::JacobiJob make_job(const Layout& L, uint32_t i, const msq_jacobi_options& o
   return j;
 }
 
-// Device-memory Jacobi of one matrix above the batched kernel's limit: one
-// launch per round, the off-norm read back after each sweep.
+// Blocked Jacobi of one matrix above the batched kernel's limit (orders >
+// 256): block rounds of 64 x 64 subproblems (diagonalised by the batched
+// kernel) whose rotations are applied to A and V as MFMA products; the FP64
+// off-norm is read back after each sweep for the stop rules.
+constexpr uint32_t kBlockedInnerSweeps = 2;  // sweeps of each 64 x 64 subproblem
+
 int jacobi_large(msq_runtime* rt, const Layout& L, uint32_t i, const double* a64, float* V,
                  float* diag, float* roots, const msq::JacobiJob& job, msq::JacobiStatus& st) {
   const uint32_t n = L.order[i], ld = L.ld[i];
-  const uint32_t np = n + (n & 1u);
-  const uint32_t m = np / 2;
+  const uint32_t N = msq::blocked_padded_order(n);
+  const uint32_t nb = N / msq::kBlockedWidth, np = nb / 2;
   const int partials = 256;
+  const size_t sq = static_cast<size_t>(N) * N;
+  const size_t sub = static_cast<size_t>(np) * 64 * 64;
   char* buf = nullptr;
-  const size_t abytes = static_cast<size_t>(ld) * ld * sizeof(float);
-  const size_t bytes = abytes + 2 * m * sizeof(int) + m * sizeof(float2) +
-                       partials * (2 * sizeof(double) + sizeof(float)) + 1024;
-  HIP_TRY(hipMalloc(&buf, bytes));
-  float* A = reinterpret_cast<float*>(buf);
-  int* sp = reinterpret_cast<int*>(buf + abytes);
-  int* sq = sp + m;
-  float2* rot = reinterpret_cast<float2*>(sq + m);  // 8 m bytes past sp: aligned
-  double* off2 = reinterpret_cast<double*>(rot + m);
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t oA = carve(sq * sizeof(float)), oV = carve(sq * sizeof(float));
+  const size_t oS = carve(sub * sizeof(double)), oU = carve(sub * sizeof(float));
+  const size_t oD = carve(np * 64 * sizeof(float)), oR = carve(np * 64 * sizeof(float));
+  const size_t oSt = carve(np * sizeof(msq::JacobiStatus)), oJ = carve(np * sizeof(msq::JacobiJob));
+  const size_t oO = carve(partials * 2 * sizeof(double)), oM = carve(partials * sizeof(float));
+  HIP_TRY(hipMalloc(&buf, off));
+  float* A = reinterpret_cast<float*>(buf + oA);
+  float* Vw = reinterpret_cast<float*>(buf + oV);
+  double* staging = reinterpret_cast<double*>(buf + oS);
+  float* U = reinterpret_cast<float*>(buf + oU);
+  float* pdiag = reinterpret_cast<float*>(buf + oD);
+  float* proots = reinterpret_cast<float*>(buf + oR);
+  msq::JacobiStatus* pst = reinterpret_cast<msq::JacobiStatus*>(buf + oSt);
+  msq::JacobiJob* pjobs = reinterpret_cast<msq::JacobiJob*>(buf + oJ);
+  double* off2 = reinterpret_cast<double*>(buf + oO);
   double* g2 = off2 + partials;
-  float* dmin = reinterpret_cast<float*>(g2 + partials);
-  float* Vi = V + L.off[i];
+  float* dmin = reinterpret_cast<float*>(buf + oM);
   int status = MSQ_OK;
   do {
-    hipError_t e = msq::launch_global_jacobi_init(a64 + L.off[i], A, Vi, n, ld, rt->stream);
+    std::vector<msq::JacobiJob> jobs(np);
+    for (uint32_t p = 0; p < np; ++p) {
+      msq::JacobiJob& jb = jobs[p];
+      jb = msq::JacobiJob{};
+      jb.a_offset = jb.v_offset = static_cast<uint64_t>(p) * 64 * 64;
+      jb.s_offset = static_cast<uint64_t>(p) * 64;
+      jb.order = 64;
+      jb.ld = 64;
+      jb.max_sweeps = kBlockedInnerSweeps;
+      jb.fixed_sweeps = kBlockedInnerSweeps;
+      jb.off_tol = 1e-7f;
+      jb.stagnation = 0.5f;
+      jb.floor_tol = 1e-5f;
+      jb.status_index = p;
+    }
+    hipError_t e = hipMemcpyAsync(pjobs, jobs.data(), np * sizeof(msq::JacobiJob),
+                                  hipMemcpyHostToDevice, rt->stream);
+    if (e == hipSuccess) e = msq::launch_blocked_init(a64 + L.off[i], n, ld, A, Vw, N, rt->stream);
     if (e != hipSuccess) { status = set_error(MSQ_ERROR_HIP, "jacobi init: %s", hipGetErrorString(e)); break; }
     // ||A||_F in binary64 from the input.
     std::vector<double> host_a(static_cast<size_t>(ld) * ld);
@@ -263,9 +297,10 @@ int jacobi_large(msq_runtime* rt, const Layout& L, uint32_t i, const double* a64
     std::vector<double> h_off(partials), h_g(partials);
     std::vector<float> h_min(partials);
     for (uint32_t sweep = 0; sweep < cap && status == MSQ_OK; ++sweep) {
-      for (uint32_t t = 0; t + 1 < np && e == hipSuccess; ++t)
-        e = msq::launch_global_jacobi_round(A, Vi, n, ld, t, sp, sq, rot, rt->stream);
-      if (e == hipSuccess) e = msq::launch_global_offnorm(A, n, ld, off2, g2, dmin, partials, rt->stream);
+      for (uint32_t t = 0; t + 1 < nb && e == hipSuccess; ++t)
+        e = msq::launch_blocked_round(A, Vw, N, t, staging, U, pjobs, pdiag, proots, pst,
+                                      rt->stream);
+      if (e == hipSuccess) e = msq::launch_global_offnorm(A, n, N, off2, g2, dmin, partials, rt->stream);
       if (e == hipSuccess)
         e = hipMemcpyAsync(h_off.data(), off2, partials * sizeof(double), hipMemcpyDeviceToHost,
                            rt->stream);
@@ -277,35 +312,36 @@ int jacobi_large(msq_runtime* rt, const Layout& L, uint32_t i, const double* a64
                            rt->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(rt->stream);
       if (e != hipSuccess) { status = set_error(MSQ_ERROR_HIP, "jacobi sweep: %s", hipGetErrorString(e)); break; }
-      double s = 0.0, g = 0.0;
+      double s2 = 0.0, g = 0.0;
       float d = INFINITY;
       for (int k = 0; k < partials; ++k) {
-        s += h_off[k];
+        s2 += h_off[k];
         g += h_g[k];
         d = std::min(d, h_min[k]);
       }
-      const double off = std::sqrt(s);
+      const double offn = std::sqrt(s2);
       const double rate = d > 0.0f ? std::sqrt(g) / std::sqrt(static_cast<double>(d)) : INFINITY;
       st.correction_rate = static_cast<float>(rate);
       st.sweeps = sweep + 1;
-      st.off_norm = off;
+      st.off_norm = offn;
       st.min_diagonal = d;
-      if (sweep < msq::kJacobiHistory) st.off_history[sweep] = static_cast<float>(off / st.matrix_norm);
+      if (sweep < msq::kJacobiHistory) st.off_history[sweep] = static_cast<float>(offn / st.matrix_norm);
       bool stop;
       if (job.fixed_sweeps) {
         stop = st.sweeps >= job.fixed_sweeps;
       } else {
         double tol = job.off_tol * st.matrix_norm;
         if (job.off_tol_lambda > 0.0f && d > 0.0f) tol = std::max(tol, double(job.off_tol_lambda) * d);
-        stop = off <= tol || st.sweeps >= cap ||
+        stop = offn <= tol || st.sweeps >= cap ||
                (job.correction_rate > 0.0f && rate <= job.correction_rate) ||
-               (off > job.stagnation * previous && off <= job.floor_tol * st.matrix_norm);
+               (offn > job.stagnation * previous && offn <= job.floor_tol * st.matrix_norm);
       }
-      previous = off;
+      previous = offn;
       if (stop) break;
     }
     if (status != MSQ_OK) break;
-    e = msq::launch_global_jacobi_finish(A, n, ld, diag + L.voff[i], roots + L.voff[i], rt->stream);
+    e = msq::launch_blocked_finish(A, Vw, n, N, V + L.off[i], diag + L.voff[i], roots + L.voff[i],
+                                   ld, rt->stream);
     std::vector<float> hd(n);
     if (e == hipSuccess)
       e = hipMemcpyAsync(hd.data(), diag + L.voff[i], n * sizeof(float), hipMemcpyDeviceToHost,

@@ -108,6 +108,22 @@ hipError_t launch_mlp_adam(float* params, const float* grads, float* m1, float* 
                            float lr, float beta1, float beta2, float eps, float decay, int step,
                            hipStream_t stream);
 
+// Blocked Jacobi (orders above kBatchedJacobiMaxOrder): 32-wide blocks paired
+// on the circle, 64 x 64 subproblems, rotations applied as MFMA products.
+constexpr int kBlockedWidth = 32;
+uint32_t blocked_padded_order(uint32_t order);  // multiple of 64
+hipError_t launch_blocked_init(const double* a64, uint32_t n, uint32_t ld_in, float* A, float* V,
+                               uint32_t N, hipStream_t stream);
+// One block round: gather the pairs' subproblems into staging (binary64,
+// N / 64 matrices of 64 x 64), diagonalise them with the batched kernel (jobs
+// prepared by the caller: order 64, ld 64, one per pair) into U, then apply
+// the rotations to A and V (pitch N).
+hipError_t launch_blocked_round(float* A, float* V, uint32_t N, uint32_t round, double* staging,
+                                float* U, const JacobiJob* jobs, float* pair_diag,
+                                float* pair_roots, JacobiStatus* pair_status, hipStream_t stream);
+hipError_t launch_blocked_finish(const float* A, const float* Vw, uint32_t n, uint32_t N, float* V,
+                                 float* diag, float* roots, uint32_t ld_out, hipStream_t stream);
+
 // -------------------------------------------------------------------- GEMM
 // Operand element types of the batched square GEMMs.
 enum class GemmType : int { F64 = 0, F32 = 1, Bf16 = 2, Fp8 = 3, Mxfp4 = 4 };

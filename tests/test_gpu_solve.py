@@ -69,3 +69,36 @@ def test_invalid_spectrum_reported(runtime):
     A = A - 2.0 * np.eye(32)   # indefinite
     (X,), (rep,) = solve(runtime, [A], MS.plain_fp32_options())
     assert rep["status"] == "invalid_spectrum"
+
+
+@pytest.mark.parametrize("tier", ["fp8", "mxfp4"])
+def test_config2_n4096_low_bit_products(runtime, tier):
+    """n = 4096, lambda log-uniform in [1e-3, 1]: blocked MFMA Jacobi, then the
+    reconstruction and 3 corrections with every product in the tier
+    (per-tensor-scaled FP8 E4M3 / MXFP4 with E8M0 block scales).  Native
+    low-bit corrections meet the reference's verification tolerance of the
+    storage width (src/configuration.cpp:98-107: 1e-2 for 8 bits, 0.25 for 4
+    bits); FP8's shrink the FP64 residual at every step, 4-bit ones reach
+    their floor.  With BF16 corrections on the same low-bit basis the root
+    meets 1e-6 against the construction root."""
+    A, Xc, lam = G.single_matrix(2, 4096, 1e-3)
+    p = capi.FP8_E4M3FN_SCALED if tier == "fp8" else capi.MXFP4_E2M1
+    say("config2", tier)
+    opts = capi.solve_options(p, p, capi.jacobi_options(off_tol=1e-7), min_corrections=3,
+                              max_corrections=3, residual_tolerance=0.0)
+    (X,), (rep,) = solve(runtime, [A], opts)
+    res = rep["residuals"][:rep["corrections"] + 1]
+    err = S.relative_error(X, Xc)
+    say("config2", tier, "residuals", res, "error", err)
+    assert rep["corrections"] == 3
+    assert err <= (1e-2 if tier == "fp8" else 0.25)
+    assert min(res[1:]) < res[0]
+    if tier == "fp8":
+        assert all(b < a for a, b in zip(res, res[1:]))
+    opts = capi.solve_options(p, capi.BF16, capi.jacobi_options(off_tol=1e-7),
+                              max_corrections=8, residual_tolerance=0.0,
+                              forward_tolerance=MS.ESTIMATE_TARGET)
+    (X,), (rep,) = solve(runtime, [A], opts)
+    err = S.relative_error(X, Xc)
+    say("config2", tier, "+ bf16 corrections", rep["corrections"], "error", err)
+    assert rep["status"] == "converged" and err <= 1e-6
