@@ -164,6 +164,13 @@ int msq_runtime_destroy(msq_runtime *runtime);
 #define MSQ_OWN_STREAM ((void *)(~(uintptr_t)0))
 int msq_runtime_set_stream(msq_runtime *runtime, void *hip_stream);
 int msq_synchronize(msq_runtime *runtime);
+/* No counterpart in the reference.  enable != 0: msq_solve_batched records
+ * the device work of each batch signature (orders, options, matrices and
+ * roots pointers) in a HIP graph -- the first call runs eagerly, the second
+ * is captured, later ones replay it (the plans and uploads stay in an arena
+ * of the signature's own).  Batches with orders above 256 always run
+ * eagerly.  At most 4 signatures are kept; enable = 0 releases them. */
+int msq_runtime_set_graphs(msq_runtime *runtime, int enable);
 
 /* makeSharedBuffer / releaseBuffers (src/metal_runtime.cpp:1426-1432).  No
  * unified memory here: device allocations plus explicit copies. */

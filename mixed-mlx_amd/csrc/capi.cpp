@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -69,6 +70,9 @@ struct Arena {
 // Streams on which independent launches of one call overlap (the order
 // classes of a batched Jacobi); forked from and joined back into `stream`.
 constexpr int kLanes = 4;
+// Captured solve graphs kept per runtime (one per batch signature).
+constexpr size_t kMaxGraphs = 4;
+struct GraphEntry;
 
 struct msq_runtime {
   int device = 0;
@@ -77,7 +81,10 @@ struct msq_runtime {
   hipStream_t lanes[kLanes] = {};
   hipEvent_t fork = nullptr;
   hipEvent_t joins[kLanes] = {};
+  hipEvent_t graph_in = nullptr, graph_out = nullptr;
   Arena arena;
+  bool graphs_enabled = false;
+  std::vector<std::unique_ptr<GraphEntry>> graphs;
   float* last_vectors = nullptr;
   float* last_roots = nullptr;
 };
@@ -357,49 +364,66 @@ int jacobi_large(msq_runtime* rt, const Layout& L, uint32_t i, const double* a64
   return status;
 }
 
-// Jacobi on a whole batch.  djobs: device room for `count` jobs; jstatus:
-// device status array in matrix order.  Orders up to 256 run one workgroup
-// per matrix, one launch per padded order; larger orders run one at a time.
-int run_jacobi(msq_runtime* rt, const Layout& L, const double* a64, float* V, float* diag,
-               float* roots, const msq_jacobi_options* opts, uint32_t opts_count,
-               msq::JacobiStatus* jstatus, msq::JacobiJob* djobs,
-               std::vector<msq::JacobiJob>& jobs) {
+// Jacobi launch plan of a batch: jobs of the orders up to 256 grouped by
+// padded order (largest first), as (first job, padded order) launches.
+struct JacobiPlan {
+  std::vector<msq::JacobiJob> jobs;
+  std::vector<std::pair<size_t, uint32_t>> launches;  // ends with (jobs.size(), 0)
+  bool has_large = false;                             // orders above 256
+};
+
+void plan_jacobi(const Layout& L, const msq_jacobi_options* opts, uint32_t opts_count,
+                 JacobiPlan& P) {
   const uint32_t count = static_cast<uint32_t>(L.order.size());
-  jobs.clear();
+  P = JacobiPlan{};
   std::map<uint32_t, std::vector<uint32_t>> by_padded;
-  for (uint32_t i = 0; i < count; ++i)
+  for (uint32_t i = 0; i < count; ++i) {
     if (L.order[i] <= msq::kBatchedJacobiMaxOrder)
       by_padded[msq::batched_jacobi_padded_order(L.order[i])].push_back(i);
-  std::vector<std::pair<size_t, uint32_t>> launches;  // (first job, padded order)
+    else
+      P.has_large = true;
+  }
   // Largest orders first: their workgroups are the longest-running.
   for (auto it = by_padded.rbegin(); it != by_padded.rend(); ++it) {
-    launches.push_back({jobs.size(), it->first});
-    for (uint32_t i : it->second) jobs.push_back(make_job(L, i, opts[opts_count == 1 ? 0 : i]));
+    P.launches.push_back({P.jobs.size(), it->first});
+    for (uint32_t i : it->second) P.jobs.push_back(make_job(L, i, opts[opts_count == 1 ? 0 : i]));
   }
-  launches.push_back({jobs.size(), 0});
-  if (!jobs.empty()) {
-    MSQ_TRY(upload(rt, djobs, jobs));
-    // Order classes run concurrently on the runtime's lanes (largest first,
-    // dealt round-robin): a launch lasts as long as its slowest matrix, and
-    // the CUs its faster matrices free take work from the other classes.
-    const size_t classes = launches.size() - 1;
-    const int used = classes > 1 ? static_cast<int>(std::min<size_t>(classes, kLanes)) : 0;
-    if (used) {
-      HIP_TRY(hipEventRecord(rt->fork, rt->stream));
-      for (int l = 0; l < used; ++l) HIP_TRY(hipStreamWaitEvent(rt->lanes[l], rt->fork, 0));
-    }
-    for (size_t c = 0; c < classes; ++c) {
-      const size_t b = launches[c].first, e = launches[c + 1].first;
-      hipStream_t s = used ? rt->lanes[c % used] : rt->stream;
-      HIP_TRY(msq::launch_batched_jacobi(djobs + b, static_cast<uint32_t>(e - b),
-                                         launches[c].second, a64, V, diag, roots, jstatus, s));
-    }
-    for (int l = 0; l < used; ++l) {
-      HIP_TRY(hipEventRecord(rt->joins[l], rt->lanes[l]));
-      HIP_TRY(hipStreamWaitEvent(rt->stream, rt->joins[l], 0));
-    }
+  P.launches.push_back({P.jobs.size(), 0});
+}
+
+// The batched launches of a plan whose jobs are already on the device
+// (device work only: no host synchronisation, capturable in a graph).
+int enqueue_jacobi(msq_runtime* rt, const JacobiPlan& P, const msq::JacobiJob* djobs,
+                   const double* a64, float* V, float* diag, float* roots,
+                   msq::JacobiStatus* jstatus) {
+  if (P.jobs.empty()) return MSQ_OK;
+  // Order classes run concurrently on the runtime's lanes (largest first,
+  // dealt round-robin): a launch lasts as long as its slowest matrix, and
+  // the CUs its faster matrices free take work from the other classes.
+  const size_t classes = P.launches.size() - 1;
+  const int used = classes > 1 ? static_cast<int>(std::min<size_t>(classes, kLanes)) : 0;
+  if (used) {
+    HIP_TRY(hipEventRecord(rt->fork, rt->stream));
+    for (int l = 0; l < used; ++l) HIP_TRY(hipStreamWaitEvent(rt->lanes[l], rt->fork, 0));
   }
-  for (uint32_t i = 0; i < count; ++i) {
+  for (size_t c = 0; c < classes; ++c) {
+    const size_t b = P.launches[c].first, e = P.launches[c + 1].first;
+    hipStream_t s = used ? rt->lanes[c % used] : rt->stream;
+    HIP_TRY(msq::launch_batched_jacobi(djobs + b, static_cast<uint32_t>(e - b),
+                                       P.launches[c].second, a64, V, diag, roots, jstatus, s));
+  }
+  for (int l = 0; l < used; ++l) {
+    HIP_TRY(hipEventRecord(rt->joins[l], rt->lanes[l]));
+    HIP_TRY(hipStreamWaitEvent(rt->stream, rt->joins[l], 0));
+  }
+  return MSQ_OK;
+}
+
+// Orders above 256, one matrix at a time (host-synchronised per sweep).
+int run_large_jacobi(msq_runtime* rt, const Layout& L, const double* a64, float* V, float* diag,
+                     float* roots, const msq_jacobi_options* opts, uint32_t opts_count,
+                     msq::JacobiStatus* jstatus) {
+  for (uint32_t i = 0; i < L.order.size(); ++i) {
     if (L.order[i] <= msq::kBatchedJacobiMaxOrder) continue;
     msq::JacobiStatus st{};
     MSQ_TRY(jacobi_large(rt, L, i, a64, V, diag, roots, make_job(L, i, opts[opts_count == 1 ? 0 : i]),
@@ -408,6 +432,18 @@ int run_jacobi(msq_runtime* rt, const Layout& L, const double* a64, float* V, fl
     HIP_TRY(hipStreamSynchronize(rt->stream));
   }
   return MSQ_OK;
+}
+
+// Jacobi on a whole batch.  djobs: device room for `count` jobs; jstatus:
+// device status array in matrix order.
+int run_jacobi(msq_runtime* rt, const Layout& L, const double* a64, float* V, float* diag,
+               float* roots, const msq_jacobi_options* opts, uint32_t opts_count,
+               msq::JacobiStatus* jstatus, msq::JacobiJob* djobs) {
+  JacobiPlan P;
+  plan_jacobi(L, opts, opts_count, P);
+  MSQ_TRY(upload(rt, djobs, P.jobs));
+  MSQ_TRY(enqueue_jacobi(rt, P, djobs, a64, V, diag, roots, jstatus));
+  return run_large_jacobi(rt, L, a64, V, diag, roots, opts, opts_count, jstatus);
 }
 
 void fill_jacobi_report(const msq::JacobiStatus& s, msq_jacobi_report& r) {
@@ -715,6 +751,142 @@ int validate_tier(int p, const char* what) {
 }  // namespace
 
 // ============================================================== C entry points
+// ----------------------------------------------------------- solve graphs
+// A captured solve: the device work of one batch signature (orders, options,
+// input and output pointers) as a HIP graph, with the buffers it reads in an
+// arena of its own.
+struct GraphEntry {
+  std::vector<uint32_t> orders;
+  std::vector<unsigned char> options;
+  const void* matrices = nullptr;
+  void* roots = nullptr;
+  Arena arena;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int calls = 0;  // eager calls so far (the second one is captured)
+  Layout layout;
+  msq::SolveState* st = nullptr;
+  msq::JacobiStatus* jst = nullptr;
+  float* V = nullptr;
+  float* sroot = nullptr;
+};
+
+namespace {
+
+void release_graph(GraphEntry& g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (g.graph) (void)hipGraphDestroy(g.graph);
+  if (g.arena.base) (void)hipFree(g.arena.base);
+  g = GraphEntry{};
+}
+
+// The entry of this signature (created on first use; at most kMaxGraphs,
+// the oldest evicted).
+GraphEntry* find_graph(msq_runtime* rt, uint32_t count, const uint32_t* orders,
+                       const double* matrices, double* roots, const msq_solve_options* options,
+                       uint32_t options_count) {
+  const unsigned char* ob = reinterpret_cast<const unsigned char*>(options);
+  const size_t obytes = static_cast<size_t>(options_count) * sizeof(msq_solve_options);
+  for (auto& g : rt->graphs)
+    if (g->matrices == matrices && g->roots == roots && g->orders.size() == count &&
+        std::equal(g->orders.begin(), g->orders.end(), orders) && g->options.size() == obytes &&
+        std::equal(g->options.begin(), g->options.end(), ob))
+      return g.get();
+  if (rt->graphs.size() >= kMaxGraphs) {
+    (void)hipStreamSynchronize(rt->stream);
+    release_graph(*rt->graphs.front());
+    rt->graphs.erase(rt->graphs.begin());
+  }
+  auto g = std::make_unique<GraphEntry>();
+  g->orders.assign(orders, orders + count);
+  g->options.assign(ob, ob + obytes);
+  g->matrices = matrices;
+  g->roots = roots;
+  rt->graphs.push_back(std::move(g));
+  return rt->graphs.back().get();
+}
+
+// Graphs are captured and launched on a stream of the runtime's own (the
+// null stream cannot be captured), ordered after and before the caller's
+// stream with events.
+template <typename F>
+int capture_graph(msq_runtime* rt, GraphEntry& g, F&& enqueue) {
+  hipStream_t caller = rt->stream;
+  HIP_TRY(hipEventRecord(rt->graph_in, caller));
+  HIP_TRY(hipStreamWaitEvent(rt->own_stream, rt->graph_in, 0));
+  rt->stream = rt->own_stream;
+  hipError_t e = hipStreamBeginCapture(rt->stream, hipStreamCaptureModeRelaxed);
+  int status = e == hipSuccess ? enqueue() : MSQ_OK;
+  hipGraph_t graph = nullptr;
+  const hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(rt->stream, &graph) : e;
+  rt->stream = caller;
+  if (e != hipSuccess || e2 != hipSuccess || status != MSQ_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (status != MSQ_OK) return status;
+    return set_error(MSQ_ERROR_HIP, "graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return set_error(MSQ_ERROR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  }
+  g.graph = graph;
+  g.exec = exec;
+  return MSQ_OK;
+}
+
+int launch_graph(msq_runtime* rt, GraphEntry& g) {
+  HIP_TRY(hipEventRecord(rt->graph_in, rt->stream));
+  HIP_TRY(hipStreamWaitEvent(rt->own_stream, rt->graph_in, 0));
+  HIP_TRY(hipGraphLaunch(g.exec, rt->own_stream));
+  HIP_TRY(hipEventRecord(rt->graph_out, rt->own_stream));
+  HIP_TRY(hipStreamWaitEvent(rt->stream, rt->graph_out, 0));
+  return MSQ_OK;
+}
+
+// Reports of a solve whose device work is enqueued: copies the per-matrix
+// states back and waits.
+int finish_solve(msq_runtime* rt, const Layout& L, const msq::SolveState* st,
+                 const msq::JacobiStatus* jst, float* V, float* sroot, msq_solve_report* reports) {
+  const uint32_t count = static_cast<uint32_t>(L.order.size());
+  std::vector<msq::SolveState> hs(count);
+  std::vector<msq::JacobiStatus> hj(count);
+  HIP_TRY(hipMemcpyAsync(hs.data(), st, count * sizeof(msq::SolveState), hipMemcpyDeviceToHost,
+                         rt->stream));
+  HIP_TRY(hipMemcpyAsync(hj.data(), jst, count * sizeof(msq::JacobiStatus), hipMemcpyDeviceToHost,
+                         rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  if (reports) {
+    for (uint32_t i = 0; i < count; ++i) {
+      msq_solve_report& r = reports[i];
+      std::memset(&r, 0, sizeof(r));
+      const msq::SolveState& s = hs[i];
+      r.order = L.order[i];
+      r.sweeps = hj[i].sweeps;
+      r.corrections = s.corrections;
+      r.status = s.status == msq::kSolveRunning ? msq::kSolveBudgetExhausted : s.status;
+      r.matrix_norm = s.matrix_norm;
+      r.min_root = s.min_root;
+      r.off_norm = hj[i].off_norm;
+      r.correction_rate = hj[i].correction_rate;
+      r.final_residual = s.final_residual;
+      r.final_forward_estimate = s.final_forward_estimate;
+      for (uint32_t k = 0; k < MSQ_SOLVE_HISTORY && k < msq::kSolveHistory; ++k) {
+        r.residuals[k] = k < s.steps ? s.residuals[k] : 0.0;
+        r.forward_estimates[k] = k < s.steps ? s.forward_estimates[k] : 0.0;
+      }
+      for (uint32_t k = 0; k < MSQ_JACOBI_HISTORY; ++k)
+        r.off_history[k] = k < hj[i].sweeps ? hj[i].off_history[k] : 0.0f;
+    }
+  }
+  rt->last_vectors = V;
+  rt->last_roots = sroot;
+  return MSQ_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* msq_last_error(void) { return g_last_error.c_str(); }
@@ -741,6 +913,8 @@ int msq_runtime_create(int device, msq_runtime** out) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&rt->joins[l], hipEventDisableTiming);
   }
   if (e == hipSuccess) e = hipEventCreateWithFlags(&rt->fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&rt->graph_in, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&rt->graph_out, hipEventDisableTiming);
   if (e != hipSuccess) {
     msq_runtime_destroy(rt);
     return set_error(MSQ_ERROR_HIP, "runtime streams: %s", hipGetErrorString(e));
@@ -756,6 +930,9 @@ int msq_runtime_destroy(msq_runtime* rt) {
   hipError_t e = hipSetDevice(rt->device);
   if (e == hipSuccess) e = hipStreamSynchronize(rt->stream);
   if (e != hipSuccess) status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
+  if (rt->own_stream) (void)hipStreamSynchronize(rt->own_stream);
+  for (auto& g : rt->graphs) release_graph(*g);
+  rt->graphs.clear();
   if (rt->arena.base && (e = hipFree(rt->arena.base)) != hipSuccess && status == MSQ_OK)
     status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
   for (int l = 0; l < kLanes; ++l) {
@@ -768,6 +945,9 @@ int msq_runtime_destroy(msq_runtime* rt) {
   }
   if (rt->fork && (e = hipEventDestroy(rt->fork)) != hipSuccess && status == MSQ_OK)
     status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
+  for (hipEvent_t ev : {rt->graph_in, rt->graph_out})
+    if (ev && (e = hipEventDestroy(ev)) != hipSuccess && status == MSQ_OK)
+      status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
   if (rt->own_stream && (e = hipStreamDestroy(rt->own_stream)) != hipSuccess && status == MSQ_OK)
     status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
   delete rt;
@@ -777,6 +957,18 @@ int msq_runtime_destroy(msq_runtime* rt) {
 int msq_runtime_set_stream(msq_runtime* rt, void* stream) {
   MSQ_TRY(activate(rt));
   rt->stream = stream == MSQ_OWN_STREAM ? rt->own_stream : static_cast<hipStream_t>(stream);
+  return MSQ_OK;
+}
+
+int msq_runtime_set_graphs(msq_runtime* rt, int enable) {
+  MSQ_TRY(activate(rt));
+  if (!enable && !rt->graphs.empty()) {
+    HIP_TRY(hipStreamSynchronize(rt->stream));
+    HIP_TRY(hipStreamSynchronize(rt->own_stream));
+    for (auto& g : rt->graphs) release_graph(*g);
+    rt->graphs.clear();
+  }
+  rt->graphs_enabled = enable != 0;
   return MSQ_OK;
 }
 
@@ -936,9 +1128,7 @@ int msq_jacobi_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   S.want(jst, count);
   S.want(djobs, count);
   MSQ_TRY(S.commit());
-  std::vector<msq::JacobiJob> jobs;
-  MSQ_TRY(run_jacobi(rt, L, matrices, vectors, diagonal, roots, options, options_count, jst, djobs,
-                     jobs));
+  MSQ_TRY(run_jacobi(rt, L, matrices, vectors, diagonal, roots, options, options_count, jst, djobs));
   std::vector<msq::JacobiStatus> hs(count);
   HIP_TRY(hipMemcpyAsync(hs.data(), jst, count * sizeof(msq::JacobiStatus), hipMemcpyDeviceToHost,
                          rt->stream));
@@ -1087,6 +1277,17 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
                       const double* matrices, double* roots_out, const msq_solve_options* options,
                       uint32_t options_count, msq_solve_report* reports) {
   MSQ_TRY(activate(rt));
+  GraphEntry* graph = nullptr;
+  if (rt->graphs_enabled) {
+    graph = find_graph(rt, count, orders, matrices, roots_out, options, options_count);
+    if (graph && graph->exec) {
+      // Replay: the plans and uploads of this signature live in the entry's
+      // arena; only the device work runs again.
+      MSQ_TRY(launch_graph(rt, *graph));
+      return finish_solve(rt, graph->layout, graph->st, graph->jst, graph->V, graph->sroot,
+                          reports);
+    }
+  }
   Layout L;
   MSQ_TRY(make_layout(count, orders, L));
   if (!matrices || !roots_out || !options)
@@ -1197,7 +1398,21 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
     tier_want(S, t.ev, t.tier, L);
     tier_want(S, t.evt, t.tier, L);
   }
+  // Graph mode: the call's buffers come from the signature's own arena, so
+  // that a captured launch sequence keeps valid pointers.
+  if (graph) std::swap(rt->arena, graph->arena);
+  struct ArenaRestore {
+    msq_runtime* rt;
+    GraphEntry* g;
+    ~ArenaRestore() {
+      if (g) std::swap(rt->arena, g->arena);
+    }
+  } restore{rt, graph};
   MSQ_TRY(S.commit());
+  JacobiPlan jplan;
+  plan_jacobi(L, jopts.data(), options_count, jplan);
+  MSQ_TRY(upload(rt, djobs, jplan.jobs));
+
 
   MSQ_TRY(upload(rt, dsegs_all, segs_all));
   MSQ_TRY(upload(rt, dlim, limits));
@@ -1208,73 +1423,58 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   }
   for (CorrTier& t : tiers) MSQ_TRY(upload(rt, t.dsegs, t.segs));
 
-  // 1. Jacobi (symmetricJacobi).
-  std::vector<msq::JacobiJob> jobs;
-  MSQ_TRY(run_jacobi(rt, L, matrices, V, diag, sroot, jopts.data(), options_count, jst, djobs, jobs));
-  HIP_TRY(msq::launch_solve_state(dsegs_all, jst, diag, dlim, st, active, count, rt->stream));
-  // 2. Reconstruction X0 = G G^T in the basis tier, X = sym(X0) in binary64.
-  for (Group& g : gl) {
-    MSQ_TRY(reconstruction_product(rt, g.basis, L, g.members, g.dsegs, g.plan, V, sroot, active,
-                                   cb));
-    HIP_TRY(msq::launch_symmetrize_to_f64(g.dsegs, static_cast<uint32_t>(g.members.size()),
-                                          L.max_ld, cb.t2, roots_out, active, rt->stream));
-  }
-  // 3. Frozen factor in each correction tier: V and V^T.
-  HIP_TRY(msq::launch_transpose(dsegs_all, count, L.max_ld, V, Vt, rt->stream));
-  for (CorrTier& t : tiers) {
-    MSQ_TRY(encode_operand(rt, t.tier, L, t.members, t.dsegs, V, t.ev, t.ov));
-    MSQ_TRY(encode_operand(rt, t.tier, L, t.members, t.dsegs, Vt, t.evt, t.ovt));
-  }
-  // 4. Residual of X0, then corrections until each matrix stops.
-  MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, R, R32, norms, active, count));
-  HIP_TRY(msq::launch_convergence(st, norms, active, count, 0, rt->stream));
-  for (uint32_t step = 1; step <= max_corrections; ++step) {
+  // Device work: Jacobi, reconstruction, frozen factor, residual and
+  // corrections (no host synchronisation below, except for orders above 256).
+  auto enqueue = [&]() -> int {
+    // 1. Jacobi (symmetricJacobi).
+    MSQ_TRY(enqueue_jacobi(rt, jplan, djobs, matrices, V, diag, sroot, jst));
+    if (jplan.has_large)
+      MSQ_TRY(run_large_jacobi(rt, L, matrices, V, diag, sroot, jopts.data(), options_count, jst));
+    HIP_TRY(msq::launch_solve_state(dsegs_all, jst, diag, dlim, st, active, count, rt->stream));
+    // 2. Reconstruction X0 = G G^T in the basis tier, X = sym(X0) in binary64.
     for (Group& g : gl) {
-      const CorrTier* t = nullptr;
-      for (const CorrTier& c : tiers)
-        if (c.tier == g.corr) t = &c;
-      MSQ_TRY(correction_products(rt, g.corr, L, g.members, g.dsegs, g.plan, t->ov, t->ovt, R32,
-                                  sroot, active, cb));
-      HIP_TRY(msq::launch_add_symmetric(g.dsegs, static_cast<uint32_t>(g.members.size()), L.max_ld,
-                                        cb.t2, roots_out, active, rt->stream));
+      MSQ_TRY(reconstruction_product(rt, g.basis, L, g.members, g.dsegs, g.plan, V, sroot, active,
+                                     cb));
+      HIP_TRY(msq::launch_symmetrize_to_f64(g.dsegs, static_cast<uint32_t>(g.members.size()),
+                                            L.max_ld, cb.t2, roots_out, active, rt->stream));
     }
+    // 3. Frozen factor in each correction tier: V and V^T.
+    HIP_TRY(msq::launch_transpose(dsegs_all, count, L.max_ld, V, Vt, rt->stream));
+    for (CorrTier& t : tiers) {
+      MSQ_TRY(encode_operand(rt, t.tier, L, t.members, t.dsegs, V, t.ev, t.ov));
+      MSQ_TRY(encode_operand(rt, t.tier, L, t.members, t.dsegs, Vt, t.evt, t.ovt));
+    }
+    // 4. Residual of X0, then corrections until each matrix stops.
     MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, R, R32, norms, active, count));
-    HIP_TRY(msq::launch_convergence(st, norms, active, count, step, rt->stream));
-  }
-  // Reports.
-  std::vector<msq::SolveState> hs(count);
-  std::vector<msq::JacobiStatus> hj(count);
-  HIP_TRY(hipMemcpyAsync(hs.data(), st, count * sizeof(msq::SolveState), hipMemcpyDeviceToHost,
-                         rt->stream));
-  HIP_TRY(hipMemcpyAsync(hj.data(), jst, count * sizeof(msq::JacobiStatus), hipMemcpyDeviceToHost,
-                         rt->stream));
-  HIP_TRY(hipStreamSynchronize(rt->stream));
-  if (reports) {
-    for (uint32_t i = 0; i < count; ++i) {
-      msq_solve_report& r = reports[i];
-      std::memset(&r, 0, sizeof(r));
-      const msq::SolveState& s = hs[i];
-      r.order = L.order[i];
-      r.sweeps = hj[i].sweeps;
-      r.corrections = s.corrections;
-      r.status = s.status == msq::kSolveRunning ? msq::kSolveBudgetExhausted : s.status;
-      r.matrix_norm = s.matrix_norm;
-      r.min_root = s.min_root;
-      r.off_norm = hj[i].off_norm;
-      r.correction_rate = hj[i].correction_rate;
-      r.final_residual = s.final_residual;
-      r.final_forward_estimate = s.final_forward_estimate;
-      for (uint32_t k = 0; k < MSQ_SOLVE_HISTORY && k < msq::kSolveHistory; ++k) {
-        r.residuals[k] = k < s.steps ? s.residuals[k] : 0.0;
-        r.forward_estimates[k] = k < s.steps ? s.forward_estimates[k] : 0.0;
+    HIP_TRY(msq::launch_convergence(st, norms, active, count, 0, rt->stream));
+    for (uint32_t step = 1; step <= max_corrections; ++step) {
+      for (Group& g : gl) {
+        const CorrTier* t = nullptr;
+        for (const CorrTier& c : tiers)
+          if (c.tier == g.corr) t = &c;
+        MSQ_TRY(correction_products(rt, g.corr, L, g.members, g.dsegs, g.plan, t->ov, t->ovt, R32,
+                                    sroot, active, cb));
+        HIP_TRY(msq::launch_add_symmetric(g.dsegs, static_cast<uint32_t>(g.members.size()), L.max_ld,
+                                          cb.t2, roots_out, active, rt->stream));
       }
-      for (uint32_t k = 0; k < MSQ_JACOBI_HISTORY; ++k)
-        r.off_history[k] = k < hj[i].sweeps ? hj[i].off_history[k] : 0.0f;
+      MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, R, R32, norms, active, count));
+      HIP_TRY(msq::launch_convergence(st, norms, active, count, step, rt->stream));
     }
+    return MSQ_OK;
+  };
+  if (graph && !jplan.has_large && graph->calls >= 1) {
+    MSQ_TRY(capture_graph(rt, *graph, enqueue));
+    graph->layout = L;
+    graph->st = st;
+    graph->jst = jst;
+    graph->V = V;
+    graph->sroot = sroot;
+    MSQ_TRY(launch_graph(rt, *graph));
+  } else {
+    MSQ_TRY(enqueue());
+    if (graph) ++graph->calls;
   }
-  rt->last_vectors = V;
-  rt->last_roots = sroot;
-  return MSQ_OK;
+  return finish_solve(rt, L, st, jst, V, sroot, reports);
 }
 
 int msq_last_factors(msq_runtime* rt, const float** vectors, const float** roots) {

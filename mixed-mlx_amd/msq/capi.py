@@ -60,7 +60,8 @@ FEATURE_MAX_ORDER = 512
 # Every symbol include/msq_hip.h declares.
 EXPORTED_SYMBOLS = (
     "msq_last_error", "msq_abi_version", "msq_runtime_create", "msq_runtime_destroy",
-    "msq_runtime_set_stream", "msq_synchronize", "msq_alloc", "msq_free",
+    "msq_runtime_set_stream", "msq_synchronize", "msq_runtime_set_graphs", "msq_alloc",
+    "msq_free",
     "msq_copy_to_device", "msq_copy_to_host", "msq_memset", "msq_storage_bytes",
     "msq_scale_bytes", "msq_batch_layout", "msq_quantize", "msq_decode",
     "msq_quantize_and_decode", "msq_quantize_fp8_scaled", "msq_decode_fp8_scaled",
@@ -126,6 +127,7 @@ _sig("msq_runtime_create", _i, _i, ctypes.POINTER(_p))
 _sig("msq_runtime_destroy", _i, _p)
 _sig("msq_runtime_set_stream", _i, _p, _p)
 _sig("msq_synchronize", _i, _p)
+_sig("msq_runtime_set_graphs", _i, _p, _i)
 _sig("msq_alloc", _i, _p, _sz, ctypes.POINTER(_p))
 _sig("msq_free", _i, _p, _p)
 _sig("msq_copy_to_device", _i, _p, _p, _p, _sz)
@@ -297,6 +299,10 @@ class Runtime:
 
     def synchronize(self):
         check(lib.msq_synchronize(self.handle))
+
+    def set_graphs(self, enable=True):
+        """Record each batch signature's solve in a HIP graph and replay it."""
+        check(lib.msq_runtime_set_graphs(self.handle, 1 if enable else 0))
 
     # codecs -----------------------------------------------------------
     def quantize(self, precision, src, values, scales=None):

@@ -95,10 +95,56 @@ def test_config2_n4096_low_bit_products(runtime, tier):
     assert min(res[1:]) < res[0]
     if tier == "fp8":
         assert all(b < a for a, b in zip(res, res[1:]))
+    # BF16 corrections on the low-bit basis, with the plain-path fallback:
+    # FP8's X0 is inside the corrections' basin and converges on its own; an
+    # MXFP4 X0 (residual ~0.2 here) may not, and the FP64 check must catch
+    # that rather than return a wrong root.
     opts = capi.solve_options(p, capi.BF16, capi.jacobi_options(off_tol=1e-7),
                               max_corrections=8, residual_tolerance=0.0,
                               forward_tolerance=MS.ESTIMATE_TARGET)
-    (X,), (rep,) = solve(runtime, [A], opts)
-    err = S.relative_error(X, Xc)
-    say("config2", tier, "+ bf16 corrections", rep["corrections"], "error", err)
+    o, aflat = capi.pack_batch([A])
+    _, _, _, total, _ = capi.batch_layout(o)
+    Xd = torch.zeros(total, dtype=torch.float64, device="cuda")
+    (rep,), fallback = MS.solve_with_fallback(runtime, o, dev(aflat), Xd, opts)
+    err = S.relative_error(capi.unpack_batch(o, host(Xd))[0], Xc)
+    say("config2", tier, "+ bf16 corrections", rep["corrections"], "fallback", fallback,
+        "error", err)
     assert rep["status"] == "converged" and err <= 1e-6
+    if tier == "fp8":
+        assert not fallback
+
+
+def test_graph_replays_match_eager(runtime):
+    """Graph mode: the first call of a signature runs eagerly, the second is
+    captured, later ones replay.  Every replay -- with eager work, host
+    synchronisation and new input contents (same buffers) in between -- must
+    give the eager result bit for bit."""
+    mats = G.batched_uneven(count=40, config=21)
+    o, a1 = capi.pack_batch([m["A"] for m in mats])
+    a2 = a1 * 1.5  # the same layout with other values
+    _, _, _, total, _ = capi.batch_layout(o)
+    opts = MS.tier_options("bf16")
+    eager = []
+    for a in (a1, a2):
+        A = dev(a)
+        X = torch.zeros(total, dtype=torch.float64, device="cuda")
+        reps = runtime.solve_batched(o, A, X, opts)
+        eager.append((host(X).copy(), [r["corrections"] for r in reps]))
+    A = dev(a1)
+    X = torch.zeros(total, dtype=torch.float64, device="cuda")
+    runtime.set_graphs(True)
+    try:
+        for call, a in enumerate([a1, a1, a1, a2, a1, a2]):
+            A.copy_(torch.from_numpy(a))
+            X.zero_()
+            reps = runtime.solve_batched(o, A, X, opts)
+            k = 0 if a is a1 else 1
+            got = host(X)
+            say("graph call", call, "matches eager", np.array_equal(got, eager[k][0]))
+            assert np.array_equal(got, eager[k][0]), call
+            assert [r["corrections"] for r in reps] == eager[k][1]
+            torch.cuda.synchronize()
+            # unrelated eager work on the same runtime between replays
+            runtime.orthogonality([16], torch.eye(32, device="cuda").reshape(-1))
+    finally:
+        runtime.set_graphs(False)
