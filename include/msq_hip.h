@@ -274,6 +274,21 @@ int msq_solve_batched(msq_runtime *runtime, uint32_t count, const uint32_t *orde
  * next solve; float, batch / vector layout). */
 int msq_last_factors(msq_runtime *runtime, const float **vectors, const float **roots);
 
+/* -------------------------------------------------------------- autotuner */
+/* No counterpart in the reference.  Times the batched GEMM kernel with tile
+ * edges 32, 64 and 128 (HIP events, synthetic batches of about 2^25
+ * elements) for each product type (FP64 residual, FP32, BF16, scaled FP8,
+ * MXFP4) and each order class (ld 32, 64, ..., 256 up to max_order, and one
+ * class above 256), and keeps the fastest edge per (type, class) in the
+ * runtime: later plans use it.  table (host, MSQ_TILE_TABLE_BYTES, may be
+ * NULL) receives the choices, row-major [type][class], 0 = default.  A tile
+ * edge only partitions the output: results are bit-identical for every
+ * choice. */
+#define MSQ_TILE_TABLE_BYTES 45
+int msq_autotune(msq_runtime *runtime, uint32_t max_order, uint8_t *table);
+/* Loads a table from msq_autotune (NULL: back to the defaults). */
+int msq_set_tile_table(msq_runtime *runtime, const uint8_t *table);
+
 /* ------------------------------------------------------- learned selector */
 /* No counterpart in the reference: the tier selector of the batched solver.
  * A small MLP (MSQ_MLP_INPUTS -> MSQ_MLP_HIDDEN -> MSQ_MLP_HIDDEN ->

@@ -72,6 +72,11 @@ struct Arena {
 constexpr int kLanes = 4;
 // Captured solve graphs kept per runtime (one per batch signature).
 constexpr size_t kMaxGraphs = 4;
+// GEMM tile table: 5 operand types x ld classes 32, 64, ..., 256, and > 256.
+constexpr int kGemmTypes = 5;
+constexpr int kTileClasses = 9;
+constexpr int kTileEdges[3] = {32, 64, 128};
+static_assert(kGemmTypes * kTileClasses == MSQ_TILE_TABLE_BYTES, "tile table size");
 struct GraphEntry;
 
 struct msq_runtime {
@@ -84,6 +89,8 @@ struct msq_runtime {
   hipEvent_t graph_in = nullptr, graph_out = nullptr;
   Arena arena;
   bool graphs_enabled = false;
+  // Autotuned GEMM tile edge per (GemmType, ld class); 0: the default.
+  uint8_t tiles[kGemmTypes][kTileClasses] = {};
   std::vector<std::unique_ptr<GraphEntry>> graphs;
   float* last_vectors = nullptr;
   float* last_roots = nullptr;
@@ -462,17 +469,29 @@ void fill_jacobi_report(const msq::JacobiStatus& s, msq_jacobi_report& r) {
 // 64 x 64 for larger pitches; only tiles that meet the order are listed.
 struct GemmPlan {
   std::vector<msq::GemmJob> jobs;        // one per matrix of the group
-  std::vector<uint32_t> tile_job[2];     // [0]: 32-tiles, [1]: 64-tiles
-  std::vector<uint32_t> tile_pos[2];
+  std::vector<uint32_t> tile_job[3];     // per tile edge kTileEdges[t]
+  std::vector<uint32_t> tile_pos[3];
   msq::GemmJob* d_jobs = nullptr;
-  uint32_t* d_tile_job[2] = {nullptr, nullptr};
-  uint32_t* d_tile_pos[2] = {nullptr, nullptr};
+  uint32_t* d_tile_job[3] = {nullptr, nullptr, nullptr};
+  uint32_t* d_tile_pos[3] = {nullptr, nullptr, nullptr};
 };
 
+int tile_class(uint32_t ld) { return ld > 256 ? kTileClasses - 1 : static_cast<int>(ld / 32) - 1; }
+
+// Tile edge index for matrices of pitch ld in products of this type: the
+// autotuned choice if any, else 32-tiles for ld = 32 and 64-tiles above.
+int tile_kind(const msq_runtime* rt, msq::GemmType type, uint32_t ld) {
+  const uint8_t t = rt ? rt->tiles[static_cast<int>(type)][tile_class(ld)] : 0;
+  for (int k = 0; k < 3; ++k)
+    if (t == kTileEdges[k]) return k;
+  return ld == 32 ? 0 : 1;
+}
+
 void build_plan(const Layout& L, const std::vector<uint32_t>& members, bool aux_is_matrix,
-                GemmPlan& P) {
+                GemmPlan& P, const msq_runtime* rt = nullptr,
+                msq::GemmType type = msq::GemmType::F32) {
   P.jobs.clear();
-  for (int t = 0; t < 2; ++t) {
+  for (int t = 0; t < 3; ++t) {
     P.tile_job[t].clear();
     P.tile_pos[t].clear();
   }
@@ -485,8 +504,8 @@ void build_plan(const Layout& L, const std::vector<uint32_t>& members, bool aux_
     j.status_index = i;
     const uint32_t jid = static_cast<uint32_t>(P.jobs.size());
     P.jobs.push_back(j);
-    const int t = L.ld[i] == 32 ? 0 : 1;
-    const uint32_t tile = t == 0 ? 32 : 64;
+    const int t = tile_kind(rt, type, L.ld[i]);
+    const uint32_t tile = static_cast<uint32_t>(kTileEdges[t]);
     const uint32_t nt = (L.order[i] + tile - 1) / tile;
     for (uint32_t a = 0; a < nt; ++a)
       for (uint32_t b = 0; b < nt; ++b) {
@@ -498,7 +517,7 @@ void build_plan(const Layout& L, const std::vector<uint32_t>& members, bool aux_
 
 void plan_want(Scratch& S, GemmPlan& P) {
   S.want(P.d_jobs, std::max<size_t>(1, P.jobs.size()));
-  for (int t = 0; t < 2; ++t) {
+  for (int t = 0; t < 3; ++t) {
     S.want(P.d_tile_job[t], std::max<size_t>(1, P.tile_job[t].size()));
     S.want(P.d_tile_pos[t], std::max<size_t>(1, P.tile_pos[t].size()));
   }
@@ -506,7 +525,7 @@ void plan_want(Scratch& S, GemmPlan& P) {
 
 int plan_upload(msq_runtime* rt, GemmPlan& P) {
   MSQ_TRY(upload(rt, P.d_jobs, P.jobs));
-  for (int t = 0; t < 2; ++t) {
+  for (int t = 0; t < 3; ++t) {
     MSQ_TRY(upload(rt, P.d_tile_job[t], P.tile_job[t]));
     MSQ_TRY(upload(rt, P.d_tile_pos[t], P.tile_pos[t]));
   }
@@ -516,11 +535,11 @@ int plan_upload(msq_runtime* rt, GemmPlan& P) {
 int run_gemm(msq_runtime* rt, const GemmPlan& P, msq::GemmType type, msq::Epilogue epi,
              msq::GemmArgs args) {
   args.jobs = P.d_jobs;
-  for (int t = 0; t < 2; ++t) {
+  for (int t = 0; t < 3; ++t) {
     if (P.tile_job[t].empty()) continue;
     args.tile_job = P.d_tile_job[t];
     args.tile_pos = P.d_tile_pos[t];
-    HIP_TRY(msq::launch_gemm(type, epi, t == 0 ? 32 : 64, args,
+    HIP_TRY(msq::launch_gemm(type, epi, kTileEdges[t], args,
                              static_cast<uint32_t>(P.tile_job[t].size()), rt->stream));
   }
   return MSQ_OK;
@@ -1148,7 +1167,7 @@ int msq_reconstruct(msq_runtime* rt, int precision, uint32_t count, const uint32
   const std::vector<uint32_t> members = iota_members(count);
   const std::vector<msq::Segment> segs = segments_of(L, members);
   GemmPlan plan;
-  build_plan(L, members, false, plan);
+  build_plan(L, members, false, plan, rt, gemm_type(precision));
   Scratch S{rt};
   msq::Segment* dsegs = nullptr;
   CorrectionBuffers cb;
@@ -1176,7 +1195,7 @@ int msq_residual_fp64(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   const std::vector<uint32_t> members = iota_members(count);
   const std::vector<msq::Segment> segs = segments_of(L, members);
   GemmPlan plan;
-  build_plan(L, members, true, plan);
+  build_plan(L, members, true, plan, rt, msq::GemmType::F64);
   Scratch S{rt};
   msq::Segment* dsegs = nullptr;
   double* norms = nullptr;
@@ -1212,7 +1231,7 @@ int msq_correction(msq_runtime* rt, int precision, uint32_t count, const uint32_
   const std::vector<uint32_t> members = iota_members(count);
   const std::vector<msq::Segment> segs = segments_of(L, members);
   GemmPlan plan;
-  build_plan(L, members, false, plan);
+  build_plan(L, members, false, plan, rt, gemm_type(precision));
   Scratch S{rt};
   msq::Segment* dsegs = nullptr;
   float* vt = nullptr;
@@ -1248,7 +1267,7 @@ int msq_orthogonality(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   const std::vector<uint32_t> members = iota_members(count);
   const std::vector<msq::Segment> segs = segments_of(L, members);
   GemmPlan plan;
-  build_plan(L, members, false, plan);
+  build_plan(L, members, false, plan, rt, msq::GemmType::F32);
   Scratch S{rt};
   msq::Segment* dsegs = nullptr;
   float* gram = nullptr;
@@ -1324,7 +1343,7 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   const std::vector<uint32_t> all = iota_members(count);
   const std::vector<msq::Segment> segs_all = segments_of(L, all);
   GemmPlan plan_res;
-  build_plan(L, all, true, plan_res);
+  build_plan(L, all, true, plan_res, rt, msq::GemmType::F64);
   struct Group {
     int basis, corr;
     std::vector<uint32_t> members;
@@ -1339,7 +1358,7 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
     g.corr = kv.first.second;
     g.members = kv.second;
     g.segs = segments_of(L, g.members);
-    build_plan(L, g.members, false, g.plan);
+    build_plan(L, g.members, false, g.plan, rt, gemm_type(g.corr));
     gl.push_back(std::move(g));
   }
   struct CorrTier {
@@ -1552,6 +1571,120 @@ int msq_mlp_adam(msq_runtime* rt, float* params, const float* grads, float* m1, 
   if (step == 0) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "step counts from 1");
   HIP_TRY(msq::launch_mlp_adam(params, grads, m1, m2, static_cast<int>(count), lr, beta1, beta2,
                                eps, decay, static_cast<int>(step), rt->stream));
+  return MSQ_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+int msq_autotune(msq_runtime* rt, uint32_t max_order, uint8_t* table) {
+  MSQ_TRY(activate(rt));
+  if (max_order == 0 || max_order > 65535)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "max_order must be in [1, 65535]");
+  const msq::GemmType types[kGemmTypes] = {msq::GemmType::F64, msq::GemmType::F32,
+                                           msq::GemmType::Bf16, msq::GemmType::Fp8,
+                                           msq::GemmType::Mxfp4};
+  // Synthetic batches of about 2^25 elements per class (operand contents do
+  // not change MFMA timing; zero codes are valid in every format).
+  const size_t target = size_t{1} << 25;
+  std::vector<uint32_t> lds;
+  for (uint32_t ld = 32; ld <= 256 && ld <= ((max_order + 31) / 32) * 32; ld += 32) lds.push_back(ld);
+  if (max_order > 256) lds.push_back(512);
+  const size_t max_elems = target + 512 * 512;
+  char* buf = nullptr;
+  const size_t bytes = max_elems * (8 * 4 + 4) + max_elems / 32 * 2 + 4096 * 16 + 1024;
+  HIP_TRY(hipMalloc(&buf, bytes));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int status = MSQ_OK;
+  do {
+    hipError_t e = hipMemsetAsync(buf, 0, bytes, rt->stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) { status = set_error(MSQ_ERROR_HIP, "autotune: %s", hipGetErrorString(e)); break; }
+    char* p = buf;
+    auto take = [&](size_t b) { char* q = p; p += (b + 255) / 256 * 256; return q; };
+    void* a = take(max_elems * 8);
+    void* bo = take(max_elems * 8);
+    void* c = take(max_elems * 8);
+    void* aux = take(max_elems * 8);
+    float* cf = reinterpret_cast<float*>(take(max_elems * 4));
+    uint8_t* sa = reinterpret_cast<uint8_t*>(take(max_elems / 32));
+    uint8_t* sb = reinterpret_cast<uint8_t*>(take(max_elems / 32));
+    double* norms = reinterpret_cast<double*>(take(4096 * sizeof(double)));
+    float* alpha = reinterpret_cast<float*>(take(4096 * sizeof(float)));
+    for (uint32_t ld : lds) {
+      const uint32_t count = static_cast<uint32_t>(std::min<size_t>(4096, std::max<size_t>(1, target / (size_t{ld} * ld))));
+      std::vector<uint32_t> orders(count, ld);
+      Layout L;
+      MSQ_TRY(make_layout(count, orders.data(), L));
+      const std::vector<uint32_t> members = iota_members(count);
+      for (int ti = 0; ti < kGemmTypes && status == MSQ_OK; ++ti) {
+        const msq::GemmType type = types[ti];
+        const msq::Epilogue epi = type == msq::GemmType::F64 ? msq::Epilogue::Residual
+                                                             : msq::Epilogue::Store;
+        float best = INFINITY;
+        int best_edge = 0;
+        for (int k = 0; k < 3 && status == MSQ_OK; ++k) {
+          const uint8_t saved = rt->tiles[ti][tile_class(ld)];
+          rt->tiles[ti][tile_class(ld)] = static_cast<uint8_t>(kTileEdges[k]);
+          GemmPlan plan;
+          build_plan(L, members, true, plan, rt, type);
+          rt->tiles[ti][tile_class(ld)] = saved;
+          Scratch S{rt};
+          plan_want(S, plan);
+          status = S.commit();
+          if (status == MSQ_OK) status = plan_upload(rt, plan);
+          if (status != MSQ_OK) break;
+          msq::GemmArgs g{};
+          g.a = a;
+          g.b = bo;
+          g.c = c;
+          g.aux = aux;
+          g.c_float = cf;
+          g.norms = norms;
+          g.a_scales = sa;
+          g.b_scales = sb;
+          g.alpha = type == msq::GemmType::Fp8 ? alpha : nullptr;
+          status = run_gemm(rt, plan, type, epi, g);  // warm
+          float ms = INFINITY;
+          for (int r = 0; r < 3 && status == MSQ_OK; ++r) {
+            e = hipEventRecord(e0, rt->stream);
+            if (e == hipSuccess) status = run_gemm(rt, plan, type, epi, g);
+            if (e == hipSuccess) e = hipEventRecord(e1, rt->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float t = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+            if (e != hipSuccess) { status = set_error(MSQ_ERROR_HIP, "autotune: %s", hipGetErrorString(e)); break; }
+            ms = std::min(ms, t);
+          }
+          if (ms < best) {
+            best = ms;
+            best_edge = kTileEdges[k];
+          }
+        }
+        if (status == MSQ_OK) rt->tiles[ti][tile_class(ld)] = static_cast<uint8_t>(best_edge);
+      }
+      if (status != MSQ_OK) break;
+    }
+  } while (false);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(rt->stream);
+  hipError_t fe = hipFree(buf);
+  if (status == MSQ_OK && fe != hipSuccess)
+    status = set_error(MSQ_ERROR_HIP, "hipFree: %s", hipGetErrorString(fe));
+  if (status == MSQ_OK && table) std::memcpy(table, rt->tiles, sizeof(rt->tiles));
+  return status;
+}
+
+int msq_set_tile_table(msq_runtime* rt, const uint8_t* table) {
+  MSQ_TRY(activate(rt));
+  if (!table) {
+    std::memset(rt->tiles, 0, sizeof(rt->tiles));
+    return MSQ_OK;
+  }
+  for (size_t k = 0; k < sizeof(rt->tiles); ++k)
+    if (table[k] != 0 && table[k] != 32 && table[k] != 64 && table[k] != 128)
+      return set_error(MSQ_ERROR_INVALID_ARGUMENT, "tile edges are 0, 32, 64 or 128");
+  std::memcpy(rt->tiles, table, sizeof(rt->tiles));
   return MSQ_OK;
 }
 

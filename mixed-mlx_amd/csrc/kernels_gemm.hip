@@ -255,6 +255,8 @@ hipError_t launch_typed(int tile, const GemmArgs& args, uint32_t tiles, hipStrea
     gemm_kernel<T, 1, 1, E><<<tiles, 64, 0, stream>>>(args);
   } else if (tile == 64) {
     gemm_kernel<T, 2, 2, E><<<tiles, 256, 0, stream>>>(args);
+  } else if (tile == 128) {
+    gemm_kernel<T, 4, 4, E><<<tiles, 1024, 0, stream>>>(args);
   } else {
     return hipErrorInvalidValue;
   }

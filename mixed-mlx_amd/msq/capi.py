@@ -68,7 +68,10 @@ EXPORTED_SYMBOLS = (
     "msq_jacobi_batched", "msq_reconstruct", "msq_residual_fp64", "msq_correction",
     "msq_orthogonality", "msq_solve_batched", "msq_last_factors",
     "msq_matrix_features", "msq_mlp_forward", "msq_mlp_backward", "msq_mlp_adam",
+    "msq_autotune", "msq_set_tile_table",
 )
+TILE_TYPES = ("fp64_residual", "fp32", "bf16", "fp8_scaled", "mxfp4")
+TILE_CLASSES = (32, 64, 96, 128, 160, 192, 224, 256, 512)
 
 
 class JacobiOptions(ctypes.Structure):
@@ -128,6 +131,8 @@ _sig("msq_runtime_destroy", _i, _p)
 _sig("msq_runtime_set_stream", _i, _p, _p)
 _sig("msq_synchronize", _i, _p)
 _sig("msq_runtime_set_graphs", _i, _p, _i)
+_sig("msq_autotune", _i, _p, _u32, _p)
+_sig("msq_set_tile_table", _i, _p, _p)
 _sig("msq_alloc", _i, _p, _sz, ctypes.POINTER(_p))
 _sig("msq_free", _i, _p, _p)
 _sig("msq_copy_to_device", _i, _p, _p, _p, _sz)
@@ -299,6 +304,20 @@ class Runtime:
 
     def synchronize(self):
         check(lib.msq_synchronize(self.handle))
+
+    def autotune(self, max_order=256):
+        """Times tile edges 32 / 64 / 128 per product type and order class and
+        keeps the fastest; returns the (5, 9) table of edges (0: default)."""
+        table = (ctypes.c_uint8 * (len(TILE_TYPES) * len(TILE_CLASSES)))()
+        check(lib.msq_autotune(self.handle, max_order, table))
+        return np.array(table[:], dtype=np.uint8).reshape(len(TILE_TYPES), len(TILE_CLASSES))
+
+    def set_tile_table(self, table=None):
+        if table is None:
+            check(lib.msq_set_tile_table(self.handle, None))
+            return
+        t = np.ascontiguousarray(table, dtype=np.uint8).reshape(-1)
+        check(lib.msq_set_tile_table(self.handle, t.ctypes.data))
 
     def set_graphs(self, enable=True):
         """Record each batch signature's solve in a HIP graph and replay it."""

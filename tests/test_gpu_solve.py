@@ -148,3 +148,31 @@ def test_graph_replays_match_eager(runtime):
             runtime.orthogonality([16], torch.eye(32, device="cuda").reshape(-1))
     finally:
         runtime.set_graphs(False)
+
+
+def test_autotuned_tiles_give_identical_roots(runtime):
+    """The autotuner's table holds one of the kernel's tile edges per product
+    type and order class; a tile edge only partitions the output, so solves
+    with the tuned table, all-32 and all-128 tiles match the defaults bit for
+    bit."""
+    table = runtime.autotune(256)
+    say("tile table\n", table)
+    assert set(np.unique(table[:, :8])) <= {32, 64, 128}
+    assert (table[:, 8] == 0).all()   # no class above 256 requested
+    mats = G.batched_uneven(count=40, config=23)
+    o, aflat = capi.pack_batch([m["A"] for m in mats])
+    _, _, _, total, _ = capi.batch_layout(o)
+    A = dev(aflat)
+    results = []
+    try:
+        for t in (None, table, np.full_like(table, 32), np.full_like(table, 128)):
+            runtime.set_tile_table(t)
+            X = torch.zeros(total, dtype=torch.float64, device="cuda")
+            runtime.solve_batched(o, A, X, MS.tier_options("bf16"))
+            results.append(host(X).copy())
+    finally:
+        runtime.set_tile_table(None)
+    for r in results[1:]:
+        assert np.array_equal(r, results[0])
+    with pytest.raises(capi.MsqError):
+        runtime.set_tile_table(np.full_like(table, 48))
