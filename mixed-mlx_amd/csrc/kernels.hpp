@@ -74,15 +74,8 @@ hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t
                                  const double* a64, float* vectors, float* diagonal,
                                  float* roots, JacobiStatus* status, hipStream_t stream);
 
-hipError_t launch_global_jacobi_init(const double* a64, float* A, float* V, uint32_t n,
-                                     uint32_t ld, hipStream_t stream);
-hipError_t launch_global_jacobi_round(float* A, float* V, uint32_t n, uint32_t ld,
-                                      uint32_t round, int* slot_p, int* slot_q,
-                                      float2* rot, hipStream_t stream);
 hipError_t launch_global_offnorm(const float* A, uint32_t n, uint32_t ld, double* off2, double* g2,
                                  float* dmin, int partials, hipStream_t stream);
-hipError_t launch_global_jacobi_finish(const float* A, uint32_t n, uint32_t ld,
-                                       float* diagonal, float* roots, hipStream_t stream);
 
 // ------------------------------------------------------ learned selector
 constexpr int kMlpInputs = 16;

@@ -5,26 +5,25 @@
 // jacobi_similarity_vectors_f32 (metal_mixed_sqrt.metal:1331-1411).
 //
 // The reference dispatches two kernels per round, ping-ponging A and V through
-// device memory (16 (n_pad - 1) dispatches per sweep pair).  Here:
-//
-//  * batched_jacobi_kernel: one workgroup per matrix for n <= 256.  A lives in
-//    LDS as its packed upper triangle (n(n+1)/2 floats, 132 KiB at n = 256) and
-//    is rotated in place: the blocks {p_k, q_k} x {p_l, q_l} of one round are
-//    disjoint, so each block is read, rotated on both sides and written back
-//    by one thread with no second buffer.  V lives in LDS up to n = 128 and in
-//    device memory above.  The whole solve -- all sweeps, the FP64 off-norm
-//    test after each sweep and the square roots -- is one launch for the whole
-//    batch of uneven matrices.
-//  * single-matrix global kernels for large n: one launch per round (rotation
-//    angles fused into the block update kernel's prologue is not possible since
-//    all blocks need all angles, so angles are one tiny launch), A kept exactly
-//    symmetric in full storage.
+// device memory (16 (n_pad - 1) dispatches per sweep pair).  Here, for orders
+// up to 256, batched_jacobi_kernel runs one workgroup per matrix: A lives in
+// LDS as its packed upper triangle (n(n+1)/2 floats, 129 KiB at n = 256) and
+// is rotated in place -- the blocks {p_k, q_k} x {p_l, q_l} of one round are
+// disjoint, so each block is read, rotated on both sides and written back by
+// one thread with no second buffer.  V lives in registers up to n = 128 and in
+// device memory above.  The whole solve -- all sweeps, the FP64 off-norm test
+// after each sweep and the square roots -- is one launch per order class of
+// the batch.  Larger orders use the blocked solver (kernels_blocked.hip),
+// whose 64 x 64 subproblems run on this kernel; global_offnorm_kernel below
+// measures its off-norm.
 //
 // The schedule is the reference's circle method in closed form: in round t the
-// participant at slot x is P_t(0) = 0, P_t(x) = 1 + ((x - 1 - t) mod (N - 1))
-// for N = n + n % 2, and slot k pairs with slot N - 1 - k.  An odd order gets
-// the reference's dummy index n, represented by a zero row and column (its
-// rotation is the identity because its off-diagonal entry is zero).
+// participant at slot x is P_t(0) = 0, P_t(x) = 1 + ((x - 1 - t) mod (N - 1)),
+// and slot k pairs with slot N - 1 - k.  N is the order rounded up to 16 (the
+// reference rounds to even, N = n + n % 2): the extra indices are zero rows
+// and columns, whose pairs have a_pq = 0 and rotate by the identity, so the
+// rotations of the matrix's own pairs are the reference's in another order
+// (for n a multiple of 16, the same schedule).
 //
 // Rotation angles: tau = (a_qq - a_pp) / (2 a_pq), t = sign(tau) / (|tau| +
 // sqrt(1 + tau^2)), c = 1 / sqrt(1 + t^2), s = t c, with p < q, evaluated in
@@ -87,28 +86,6 @@ __device__ __forceinline__ Block rotate_block(Block b, float2 r0, float2 r1,
   o.a10 = r1.x * c0.x * b.a10 + r1.y * c0.x * b.a00 + r1.x * c0.y * b.a11 + r1.y * c0.y * b.a01;
   o.a11 = r1.x * c1.x * b.a11 + r1.y * c1.x * b.a01 + r1.x * c1.y * b.a10 + r1.y * c1.y * b.a00;
   return o;
-}
-
-// Block id -> slots (k, l), k <= l, over an m x m upper triangle.  Rows k and
-// m - 1 - k are paired so every unit has m + 1 blocks; an odd m leaves the
-// middle row on its own.
-__device__ __forceinline__ void block_slots(int b, int m, int& k, int& l) {
-  const int half = m / 2;
-  const int paired = half * (m + 1);
-  if (b < paired) {
-    const int rp = b / (m + 1);
-    const int j = b - rp * (m + 1);
-    if (j < m - rp) {
-      k = rp;
-      l = rp + j;
-    } else {
-      k = m - 1 - rp;
-      l = k + (j - (m - rp));
-    }
-  } else {
-    k = half;  // only when m is odd
-    l = half + (b - paired);
-  }
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -285,43 +262,60 @@ __device__ __forceinline__ float& lds_at(float* base, int byte_offset) {
   return *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_offset);
 }
 
+// Per slot and round: byte-scaled participants p < q and their column bases
+// (one 16-byte read), and the rotation (c, s) in an array of its own
+// (consecutive slots stay 16 / 8 bytes apart: no LDS bank conflicts).
+struct __align__(16) SlotIdx {
+  int p4, q4, cbp4, cbq4;
+};
+
 // Rotates U off-diagonal blocks (k < l) in place.  All loads are issued before
 // any store (the blocks are disjoint, which the compiler cannot know); entries
 // of `code` equal to ~0 are padding: they load a real block and store nothing.
 template <int U>
 __device__ __forceinline__ void rotate_blocks(float* __restrict__ Ap,
-                                              const int4* __restrict__ sinfo,
+                                              const SlotIdx* __restrict__ sidx,
                                               const float2* __restrict__ rot,
                                               const uint32_t* code) {
   int i00[U], i01[U], i10[U], i11[U];
   Block blk[U];
-  float2 rk[U], rl[U];
+  float cc[U], sc[U], cs[U], ss[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const uint32_t c = code[u] == ~0u ? (1u << 16) : code[u];
     const int k = static_cast<int>(c & 0xffffu), l = static_cast<int>(c >> 16);
-    const int4 sk = sinfo[k], sl = sinfo[l];  // 4 (p, q, cb_p, cb_q)
-    rk[u] = rot[k];
-    rl[u] = rot[l];
-    i00[u] = packed_offset(sk.x, sk.z, sl.x, sl.z);
-    i01[u] = packed_offset(sk.x, sk.z, sl.y, sl.w);
-    i10[u] = packed_offset(sk.y, sk.w, sl.x, sl.z);
-    i11[u] = packed_offset(sk.y, sk.w, sl.y, sl.w);
+    const SlotIdx rk = sidx[k], rl = sidx[l];
+    const float2 ak = rot[k], al = rot[l];
+    i00[u] = packed_offset(rk.p4, rk.cbp4, rl.p4, rl.cbp4);
+    i01[u] = packed_offset(rk.p4, rk.cbp4, rl.q4, rl.cbq4);
+    i10[u] = packed_offset(rk.q4, rk.cbq4, rl.p4, rl.cbp4);
+    i11[u] = packed_offset(rk.q4, rk.cbq4, rl.q4, rl.cbq4);
     blk[u].a00 = lds_at(Ap, i00[u]);
     blk[u].a01 = lds_at(Ap, i01[u]);
     blk[u].a10 = lds_at(Ap, i10[u]);
     blk[u].a11 = lds_at(Ap, i11[u]);
+    // The element formula's coefficient products: p sees (c, -s) and q
+    // (c, s), so every coefficient is one of c_k c_l, s_k c_l, c_k s_l,
+    // s_k s_l up to an exact sign.
+    cc[u] = ak.x * al.x;
+    sc[u] = ak.y * al.x;
+    cs[u] = ak.x * al.y;
+    ss[u] = ak.y * al.y;
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const float2 rpk = make_float2(rk[u].x, -rk[u].y);  // seen by p
-    const float2 rpl = make_float2(rl[u].x, -rl[u].y);
-    const Block o = rotate_block(blk[u], rpk, rk[u], rpl, rl[u]);
+    const Block b = blk[u];
+    // Rows (p_k, q_k), columns (p_l, q_l); A'[r][c] = rr.x cc.x A[r][c] +
+    // rr.y cc.x A[r'][c] + rr.x cc.y A[r][c'] + rr.y cc.y A[r'][c'].
+    const float o00 = cc[u] * b.a00 - sc[u] * b.a10 - cs[u] * b.a01 + ss[u] * b.a11;
+    const float o01 = cc[u] * b.a01 - sc[u] * b.a11 + cs[u] * b.a00 - ss[u] * b.a10;
+    const float o10 = cc[u] * b.a10 + sc[u] * b.a00 - cs[u] * b.a11 - ss[u] * b.a01;
+    const float o11 = cc[u] * b.a11 + sc[u] * b.a01 + cs[u] * b.a10 + ss[u] * b.a00;
     if (code[u] != ~0u) {
-      lds_at(Ap, i00[u]) = o.a00;
-      lds_at(Ap, i01[u]) = o.a01;
-      lds_at(Ap, i10[u]) = o.a10;
-      lds_at(Ap, i11[u]) = o.a11;
+      lds_at(Ap, i00[u]) = o00;
+      lds_at(Ap, i01[u]) = o01;
+      lds_at(Ap, i10[u]) = o10;
+      lds_at(Ap, i11[u]) = o11;
     }
   }
 }
@@ -367,11 +361,12 @@ __device__ __forceinline__ bool is_next_pair_block(int k, int l, int m) {
 // down, and one value crosses to each neighbouring part of the row (adjacent
 // lanes).  The rotations of a group of rounds are buffered in LDS and applied
 // to V after the group.
-template <int S>
+template <int NP>
 __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     const JacobiJob* __restrict__ jobs, const double* __restrict__ a64,
     float* __restrict__ vectors, float* __restrict__ diagonal,
     float* __restrict__ roots, JacobiStatus* __restrict__ status) {
+  constexpr int S = slots_for_order(NP);
   constexpr bool VMEM = v_in_memory(S);
   constexpr int G = group_for_slots(S);
   constexpr int SP = padded_slots(S);
@@ -381,25 +376,25 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   const int nt = blockDim.x;
   const int n = static_cast<int>(job.order);
   const int ld = static_cast<int>(job.ld);
-  const int N = (n + 15) & ~15;  // padded order: zero rows/columns beyond n
-  const int m = N / 2;
-  const int H = parts_for_order(N);
+  constexpr int N = NP;  // padded order: zero rows/columns beyond n
+  constexpr int m = N / 2;
+  constexpr int H = parts_for_order(N);
   // History entries per round: room for 8 parts whatever H is, so every
   // history offset is a compile-time constant from one base address.
   constexpr int hrow = 8 * SP;
-  const int rows_per_wave = 64 / H;
-  const int packed = N * (N + 1) / 2;
-  const int offblocks = m * (m - 1) / 2;
+  constexpr int rows_per_wave = 64 / H;
+  constexpr int packed = N * (N + 1) / 2;
+  constexpr int offblocks = m * (m - 1) / 2;
 
-  extern __shared__ __align__(16) unsigned char smem[];
-  float* Ap = reinterpret_cast<float*>(smem);
-  // Per slot, two buffers (round parity): 4 (p, q, cb_p, cb_q) and (c, s), p < q.
-  int4* sinfo = reinterpret_cast<int4*>(Ap + ((packed + 3) & ~3));
-  float2* rot = reinterpret_cast<float2*>(sinfo + 256);
-  float2* hist = rot + 256;  // G rounds x 8 parts x SP of (c, s') for V
-  double* red = reinterpret_cast<double*>(hist + G * hrow);  // 32
-  float* redf = reinterpret_cast<float*>(red + 32);         // 16
-  int* ctrl = reinterpret_cast<int*>(redf + 16);            // 4
+  // Static LDS: every array at a compile-time offset (addresses fold into
+  // the ds instructions' offset fields).
+  __shared__ __align__(16) float Ap[(packed + 3) & ~3];
+  __shared__ SlotIdx sidx[2][m];  // round parity x slot
+  __shared__ float2 rot[2][m];     // (c, s), p < q
+  __shared__ float2 hist[G * hrow];  // G rounds x 8 parts x SP of (c, s') for V
+  __shared__ double red[32];
+  __shared__ float redf[16];
+  __shared__ int ctrl[4];
 
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -507,9 +502,13 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     Ap[ipp] = o.a00;
     Ap[ipq] = o.a01;
     Ap[iqq] = o.a11;
-    const int buf = (tt & 1) * 128;
-    sinfo[buf + k] = make_int4(4 * p, 4 * q, 4 * cbp, 4 * cbq);
-    rot[buf + k] = r;
+    SlotIdx rec;
+    rec.p4 = 4 * p;
+    rec.q4 = 4 * q;
+    rec.cbp4 = 4 * cbp;
+    rec.cbq4 = 4 * cbq;
+    sidx[tt & 1][k] = rec;
+    rot[tt & 1][k] = r;
     // position k holds a: it is p (sees (c, -s)) when a < b
     rv[(k / S) * SP + k % S] = make_float2(r.x, a < b ? -r.y : r.y);
   };
@@ -521,13 +520,14 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     __syncthreads();
 #pragma unroll 1
     for (; t < group_end; ++t) {
-      const int buf = (t & 1) * 128;
+      const SlotIdx* cur = sidx[t & 1];
+      const float2* cur_rot = rot[t & 1];
 #pragma unroll
       for (int i = 0; i < BPT + 1; i += 2) {
         if (i + 1 < BPT + 1) {
-          rotate_blocks<2>(Ap, sinfo + buf, rot + buf, code + i);
+          rotate_blocks<2>(Ap, cur, cur_rot, code + i);
         } else {
-          rotate_blocks<1>(Ap, sinfo + buf, rot + buf, code + i);
+          rotate_blocks<1>(Ap, cur, cur_rot, code + i);
         }
       }
       if (tid < m && t + 1 < group_end)
@@ -602,8 +602,10 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
         dmin = fminf(dmin, Ap[e]);
       } else {
         off2 += 2.0 * v * v;
-        const double den = static_cast<double>(sroot[i]) + static_cast<double>(sroot[j]);
-        g2 += den > 0.0 ? 2.0 * (v / den) * (v / den) : (v != 0.0 ? INFINITY : 0.0);
+        // The rate estimate needs no binary64 quotient: float ratio, double sum.
+        const float den = sroot[i] + sroot[j];
+        const float g = den > 0.0f ? Ap[e] * __frcp_rn(den) : (Ap[e] != 0.0f ? INFINITY : 0.0f);
+        g2 += 2.0 * static_cast<double>(g * g);
       }
     }
     if (off2 != 0.0) atomicAdd(&red[1], off2);
@@ -711,99 +713,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   }
 }
 
-// ----------------------------------------------------------- large n, global
-__global__ void global_rotations_kernel(const float* __restrict__ A, int ld,
-                                        int np, int t, int* __restrict__ slot_p,
-                                        int* __restrict__ slot_q,
-                                        float2* __restrict__ rot) {
-  const int m = np / 2;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
-  const int a = participant(t, k, np);
-  const int b = participant(t, np - 1 - k, np);
-  const int p = a < b ? a : b;
-  const int q = a < b ? b : a;
-  slot_p[k] = p;
-  slot_q[k] = q;
-  rot[k] = rotation_angles(A[p + static_cast<size_t>(p) * ld],
-                           A[q + static_cast<size_t>(q) * ld],
-                           A[p + static_cast<size_t>(q) * ld]);
-}
-
-// A blocks (first `blocks` ids) and V items (the rest) of one round.
-__global__ void global_round_kernel(float* __restrict__ A, float* __restrict__ V,
-                                    int n, int ld, int np,
-                                    const int* __restrict__ slot_p,
-                                    const int* __restrict__ slot_q,
-                                    const float2* __restrict__ rot) {
-  const int m = np / 2;
-  const long long blocks = static_cast<long long>(m) * (m + 1) / 2;
-  const long long id = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (id < blocks) {
-    int k, l;
-    block_slots(static_cast<int>(id), m, k, l);
-    const int pk = slot_p[k], qk = slot_q[k];
-    const float2 rk = rot[k];
-    const float2 rpk = make_float2(rk.x, -rk.y);
-    auto at = [&](int r, int c) -> float& { return A[r + static_cast<size_t>(c) * ld]; };
-    if (k == l) {
-      Block blk;
-      blk.a00 = at(pk, pk);
-      blk.a01 = at(pk, qk);
-      blk.a10 = blk.a01;
-      blk.a11 = at(qk, qk);
-      const Block o = rotate_block(blk, rpk, rk, rpk, rk);
-      at(pk, pk) = o.a00;
-      at(pk, qk) = o.a01;
-      at(qk, pk) = o.a01;
-      at(qk, qk) = o.a11;
-    } else {
-      const int pl = slot_p[l], ql = slot_q[l];
-      const float2 rl = rot[l];
-      const float2 rpl = make_float2(rl.x, -rl.y);
-      Block blk;
-      blk.a00 = at(pk, pl);
-      blk.a01 = at(pk, ql);
-      blk.a10 = at(qk, pl);
-      blk.a11 = at(qk, ql);
-      const Block o = rotate_block(blk, rpk, rk, rpl, rl);
-      at(pk, pl) = o.a00;
-      at(pl, pk) = o.a00;
-      at(pk, ql) = o.a01;
-      at(ql, pk) = o.a01;
-      at(qk, pl) = o.a10;
-      at(pl, qk) = o.a10;
-      at(qk, ql) = o.a11;
-      at(ql, qk) = o.a11;
-    }
-    return;
-  }
-  const long long item = id - blocks;
-  if (item >= static_cast<long long>(n) * m) return;
-  const int i = static_cast<int>(item % n);
-  const int k = static_cast<int>(item / n);
-  const int p = slot_p[k], q = slot_q[k];
-  const float2 r = rot[k];
-  float* vp = V + i + static_cast<size_t>(p) * ld;
-  float* vq = V + i + static_cast<size_t>(q) * ld;
-  const float x = *vp, y = *vq;
-  *vp = r.x * x + (-r.y) * y;
-  *vq = r.x * y + r.y * x;
-}
-
-__global__ void global_init_kernel(const double* __restrict__ a64, float* __restrict__ A,
-                                   float* __restrict__ V, int n, int ld) {
-  const size_t total = static_cast<size_t>(ld) * ld;
-  for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
-       e += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    const int i = static_cast<int>(e % ld);
-    const int j = static_cast<int>(e / ld);
-    const bool in = i < n && j < n;
-    A[e] = in ? static_cast<float>(a64[e]) : 0.0f;
-    V[e] = (in && i == j) ? 1.0f : 0.0f;
-  }
-}
-
+// ------------------------------------------------- large n: off-norm only
 // Partial sums of the FP64 off-diagonal norm and the diagonal minimum.
 __global__ void global_offnorm_kernel(const float* __restrict__ A, int n, int ld,
                                       double* __restrict__ out_off2,
@@ -851,32 +761,13 @@ __global__ void global_offnorm_kernel(const float* __restrict__ A, int n, int ld
   }
 }
 
-__global__ void global_finish_kernel(const float* __restrict__ A, int n, int ld,
-                                     float* __restrict__ diagonal,
-                                     float* __restrict__ roots) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ld) return;
-  const float d = i < n ? A[i + static_cast<size_t>(i) * ld] : 0.0f;
-  diagonal[i] = d;
-  roots[i] = i < n ? sqrtf(d) : 0.0f;
-}
 
-template <int S>
-hipError_t launch_slots(const JacobiJob* jobs, uint32_t count, int threads, const double* a64,
+template <int NP>
+hipError_t launch_order(const JacobiJob* jobs, uint32_t count, int threads, const double* a64,
                         float* vectors, float* diagonal, float* roots, JacobiStatus* status,
-                        size_t lds, hipStream_t stream) {
-  // Dynamic LDS above the default needs an explicit opt-in, sized to this
-  // launch (static LDS plus the opt-in must stay within 160 KiB).
-  static size_t configured = 0;
-  if (lds > configured) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&batched_jacobi_kernel<S>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       static_cast<int>(lds));
-    if (e != hipSuccess) return e;
-    configured = lds;
-  }
-  batched_jacobi_kernel<S><<<count, threads, lds, stream>>>(jobs, a64, vectors, diagonal, roots,
-                                                            status);
+                        hipStream_t stream) {
+  batched_jacobi_kernel<NP><<<count, threads, 0, stream>>>(jobs, a64, vectors, diagonal, roots,
+                                                           status);
   return hipGetLastError();
 }
 
@@ -893,9 +784,8 @@ size_t batched_jacobi_lds_bytes(uint32_t order) {
   const size_t packed = static_cast<size_t>(N) * (N + 1) / 2;
   const int S = slots_for_order(N);
   const size_t hist = static_cast<size_t>(group_for_slots(S)) * 8 * padded_slots(S);
-  return ((packed + 3) & ~size_t{3}) * sizeof(float) + 256 * sizeof(int4) +
-         (256 + hist) * sizeof(float2) + 32 * sizeof(double) + 16 * sizeof(float) +
-         4 * sizeof(int);
+  return ((packed + 3) & ~size_t{3}) * sizeof(float) + 2 * (N / 2) * (sizeof(SlotIdx) + sizeof(float2)) +
+         hist * sizeof(float2) + 32 * sizeof(double) + 16 * sizeof(float) + 4 * sizeof(int);
 }
 
 hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t order,
@@ -916,37 +806,18 @@ hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t
       (offblocks + threads - 1) / threads > blocks_per_thread(S) || lds > 160 * 1024 ||
       (v_in_memory(S) && threads != 8 * m))
     return hipErrorInvalidValue;
-  switch (S) {
-#define MSQ_SLOTS(SV) \
-    case SV: return launch_slots<SV>(jobs, count, threads, a64, vectors, diagonal, roots, status, lds, stream);
-    MSQ_SLOTS(8) MSQ_SLOTS(9) MSQ_SLOTS(10) MSQ_SLOTS(11) MSQ_SLOTS(12) MSQ_SLOTS(13)
-    MSQ_SLOTS(14) MSQ_SLOTS(15) MSQ_SLOTS(16)
-#undef MSQ_SLOTS
+  switch (N) {
+#define MSQ_ORDER(NV) \
+    case NV: return launch_order<NV>(jobs, count, threads, a64, vectors, diagonal, roots, status, stream);
+    MSQ_ORDER(16) MSQ_ORDER(32) MSQ_ORDER(48) MSQ_ORDER(64) MSQ_ORDER(80) MSQ_ORDER(96)
+    MSQ_ORDER(112) MSQ_ORDER(128) MSQ_ORDER(144) MSQ_ORDER(160) MSQ_ORDER(176) MSQ_ORDER(192)
+    MSQ_ORDER(208) MSQ_ORDER(224) MSQ_ORDER(240) MSQ_ORDER(256)
+#undef MSQ_ORDER
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_global_jacobi_init(const double* a64, float* A, float* V,
-                                     uint32_t n, uint32_t ld, hipStream_t stream) {
-  const size_t total = static_cast<size_t>(ld) * ld;
-  size_t grid = (total + 255) / 256;
-  if (grid > 65536) grid = 65536;
-  global_init_kernel<<<static_cast<int>(grid), 256, 0, stream>>>(a64, A, V, n, ld);
-  return hipGetLastError();
-}
 
-hipError_t launch_global_jacobi_round(float* A, float* V, uint32_t n, uint32_t ld,
-                                      uint32_t round, int* slot_p, int* slot_q,
-                                      float2* rot, hipStream_t stream) {
-  const int np = static_cast<int>(n + (n & 1u));
-  const int m = np / 2;
-  global_rotations_kernel<<<(m + 255) / 256, 256, 0, stream>>>(A, ld, np, round, slot_p,
-                                                              slot_q, rot);
-  const long long items = static_cast<long long>(m) * (m + 1) / 2 + static_cast<long long>(n) * m;
-  global_round_kernel<<<static_cast<int>((items + 255) / 256), 256, 0, stream>>>(
-      A, V, n, ld, np, slot_p, slot_q, rot);
-  return hipGetLastError();
-}
 
 hipError_t launch_global_offnorm(const float* A, uint32_t n, uint32_t ld, double* off2,
                                  double* g2, float* dmin, int partials, hipStream_t stream) {
@@ -954,11 +825,5 @@ hipError_t launch_global_offnorm(const float* A, uint32_t n, uint32_t ld, double
   return hipGetLastError();
 }
 
-hipError_t launch_global_jacobi_finish(const float* A, uint32_t n, uint32_t ld,
-                                       float* diagonal, float* roots,
-                                       hipStream_t stream) {
-  global_finish_kernel<<<(ld + 255) / 256, 256, 0, stream>>>(A, n, ld, diagonal, roots);
-  return hipGetLastError();
-}
 
 }  // namespace msq
