@@ -127,6 +127,9 @@ def main():
     ap.add_argument("--roofline-reps", type=int, default=5)
     ap.add_argument("--no-selector", action="store_true",
                     help="fixed tier instead of the learned tier selector")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="eager launches instead of per-batch HIP graph replay")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the batched SPD solve shards by matrix with no "
@@ -150,6 +153,13 @@ def main():
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
     rt = capi.Runtime(0, stream=stream)
+    tile_table = None
+    if not args.no_autotune:
+        t = time.perf_counter()
+        tile_table = rt.autotune(256)
+        log(f"autotuned GEMM tiles in {time.perf_counter() - t:.1f} s")
+    if not args.no_graphs:
+        rt.set_graphs(True)
     A = torch.from_numpy(aflat).to(dev)
     X = torch.zeros(total, dtype=torch.float64, device=dev)
 
@@ -232,7 +242,9 @@ def main():
         "data": "synthetic (seeded Haar-random SPD matrices)",
         "config": {"workload": "batched uneven: 4096 SPD matrices, n in {16,...,256}, "
                                "cond log-uniform in [1e1, 1e5], half clustered spectra",
-                   "count": len(mats), "tier": args.tier, "correction_rate": args.rate},
+                   "count": len(mats), "tier": args.tier, "correction_rate": args.rate,
+                   "hip_graphs": not args.no_graphs,
+                   "tile_table": tile_table.tolist() if tile_table is not None else None},
         "plain_fp32": {"matrices_per_s": round(len(mats) / dt_plain, 2),
                        "ms_per_step": round(dt_plain * 1e3, 3),
                        "max_rel_error": err_plain, "converged": ok_plain},

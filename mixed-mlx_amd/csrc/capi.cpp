@@ -672,36 +672,66 @@ int fp8_alpha(msq_runtime* rt, int tier, const Operand& a, const Operand& b, flo
 // One frozen-factor correction for a group (refineUpdateAndMeasure's product
 // chain, src/metal_runtime.cpp:867-928):
 //   W = V^T R ; Y = (W V) o H ; W2 = V Y ; E = W2 V^T   (E into cb.t2)
-// ov / ovt: V and V^T encoded in the group's tier; r32: R as float.
+// ov / ovt: V and V^T encoded in the group's tier; r32: R as float; rb: R
+// as BF16 codes (batch layout) or null.  In the BF16 tier the intermediate
+// products store their codes from the GEMM epilogue (the same RNE codec as
+// the encode pass, so the operands are bit-identical) and rb replaces the
+// encode of r32.
 int correction_products(msq_runtime* rt, int tier, const Layout& L,
                         const std::vector<uint32_t>& members, const msq::Segment* dsegs,
                         const GemmPlan& plan, const Operand& ov, const Operand& ovt,
-                        const float* r32, const float* roots, const int* active,
-                        CorrectionBuffers& cb) {
+                        const float* r32, const uint16_t* rb, const float* roots,
+                        const int* active, CorrectionBuffers& cb) {
   const uint32_t count = static_cast<uint32_t>(L.order.size());
   const msq::GemmType type = gemm_type(tier);
+  const bool direct = tier == MSQ_BF16;
   const float* alpha = nullptr;
   Operand opr, opa;
+  // Output of an intermediate product: codes of `enc` (BF16) or float `t`.
+  auto out_to = [&](msq::GemmArgs& g, TierBuffers& enc, float* t) {
+    if (direct) {
+      g.c = nullptr;
+      g.c_bf16 = static_cast<uint16_t*>(enc.codes);
+    } else {
+      g.c = t;
+    }
+  };
+  auto operand_of = [&](TierBuffers& enc, const float* t, Operand& op) -> int {
+    if (direct) {
+      op = Operand{};
+      op.data = enc.codes;
+      return MSQ_OK;
+    }
+    return encode_operand(rt, tier, L, members, dsegs, t, enc, op);
+  };
   // W^T <- (V^T R)^T
-  MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, r32, cb.enc_b, opr));
+  if (direct && rb) {
+    opr = Operand{};
+    opr.data = rb;
+  } else {
+    MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, r32, cb.enc_b, opr));
+  }
   MSQ_TRY(fp8_alpha(rt, tier, ov, opr, cb.alpha, count, &alpha));
   msq::GemmArgs g = product_args(ov, opr, cb.t1, alpha, active);
   g.transpose_c = 1;
+  out_to(g, cb.enc_a, cb.t1);
   MSQ_TRY(run_gemm(rt, plan, type, msq::Epilogue::Store, g));
   // Y <- (W V) o H
-  MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, cb.t1, cb.enc_a, opa));
+  MSQ_TRY(operand_of(cb.enc_a, cb.t1, opa));
   MSQ_TRY(fp8_alpha(rt, tier, opa, ov, cb.alpha, count, &alpha));
   g = product_args(opa, ov, cb.t2, alpha, active);
   g.aux = roots;
+  out_to(g, cb.enc_b, cb.t2);
   MSQ_TRY(run_gemm(rt, plan, type, msq::Epilogue::Hadamard, g));
   // W2^T storage <- V Y, row-major
-  MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, cb.t2, cb.enc_b, opr));
+  MSQ_TRY(operand_of(cb.enc_b, cb.t2, opr));
   MSQ_TRY(fp8_alpha(rt, tier, ovt, opr, cb.alpha, count, &alpha));
   g = product_args(ovt, opr, cb.t1, alpha, active);
   g.transpose_c = 1;
+  out_to(g, cb.enc_a, cb.t1);
   MSQ_TRY(run_gemm(rt, plan, type, msq::Epilogue::Store, g));
   // E <- W2 V^T
-  MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, cb.t1, cb.enc_a, opa));
+  MSQ_TRY(operand_of(cb.enc_a, cb.t1, opa));
   MSQ_TRY(fp8_alpha(rt, tier, opa, ovt, cb.alpha, count, &alpha));
   g = product_args(opa, ovt, cb.t2, alpha, active);
   MSQ_TRY(run_gemm(rt, plan, type, msq::Epilogue::Store, g));
@@ -725,8 +755,11 @@ int reconstruction_product(msq_runtime* rt, int tier, const Layout& L,
   return MSQ_OK;
 }
 
+// R = A - X X in binary64 with ||R||_F^2 per matrix; R (binary64), R32
+// (float) and Rb (BF16 codes) are each stored when non-null.
 int residual_product(msq_runtime* rt, const GemmPlan& plan, const double* A, const double* X,
-                     double* R, float* R32, double* norms, const int* active, uint32_t count) {
+                     double* R, float* R32, uint16_t* Rb, double* norms, const int* active,
+                     uint32_t count) {
   HIP_TRY(msq::launch_zero_f64(norms, count, rt->stream));
   msq::GemmArgs g{};
   g.a = X;
@@ -734,6 +767,7 @@ int residual_product(msq_runtime* rt, const GemmPlan& plan, const double* A, con
   g.c = R;
   g.aux = A;
   g.c_float = R32;
+  g.c_bf16 = Rb;
   g.norms = norms;
   g.active = active;
   MSQ_TRY(run_gemm(rt, plan, msq::GemmType::F64, msq::Epilogue::Residual, g));
@@ -1207,7 +1241,8 @@ int msq_residual_fp64(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   MSQ_TRY(S.commit());
   MSQ_TRY(upload(rt, dsegs, segs));
   MSQ_TRY(plan_upload(rt, plan));
-  MSQ_TRY(residual_product(rt, plan, matrices, root, residual, residual_f32, norms, nullptr, count));
+  MSQ_TRY(residual_product(rt, plan, matrices, root, residual, residual_f32, nullptr, norms, nullptr,
+                           count));
   HIP_TRY(msq::launch_zero_f64(anorms, count, rt->stream));
   HIP_TRY(msq::launch_sumsq_f64(dsegs, count, L.max_ld, matrices, anorms, rt->stream));
   std::vector<double> hn(count), ha(count);
@@ -1250,8 +1285,8 @@ int msq_correction(msq_runtime* rt, int precision, uint32_t count, const uint32_
   Operand ov, ovt;
   MSQ_TRY(encode_operand(rt, precision, L, members, dsegs, vectors, ev, ov));
   MSQ_TRY(encode_operand(rt, precision, L, members, dsegs, vt, evt, ovt));
-  MSQ_TRY(correction_products(rt, precision, L, members, dsegs, plan, ov, ovt, residual_f32, roots,
-                              nullptr, cb));
+  MSQ_TRY(correction_products(rt, precision, L, members, dsegs, plan, ov, ovt, residual_f32, nullptr,
+                              roots, nullptr, cb));
   HIP_TRY(hipMemcpyAsync(correction, cb.t2, L.total * sizeof(float), hipMemcpyDeviceToDevice,
                          rt->stream));
   HIP_TRY(hipStreamSynchronize(rt->stream));
@@ -1381,7 +1416,8 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   // Device buffers.
   Scratch S{rt};
   float *V = nullptr, *Vt = nullptr, *diag = nullptr, *sroot = nullptr, *R32 = nullptr;
-  double *R = nullptr, *norms = nullptr;
+  uint16_t* Rb = nullptr;
+  double* norms = nullptr;
   int* active = nullptr;
   msq::JacobiStatus* jst = nullptr;
   msq::JacobiJob* djobs = nullptr;
@@ -1393,8 +1429,12 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   S.want(Vt, L.total);
   S.want(diag, L.vtotal);
   S.want(sroot, L.vtotal);
-  S.want(R32, L.total);
-  S.want(R, L.total);
+  // R feeds the corrections as float (FP32 / FP8 / MXFP4 tiers) and as BF16
+  // codes (BF16 tier); the binary64 R is not kept.
+  bool any_bf16 = false, any_float = false;
+  for (const Group& g : gl) (g.corr == MSQ_BF16 ? any_bf16 : any_float) = true;
+  if (any_float) S.want(R32, L.total);
+  if (any_bf16) S.want(Rb, L.total);
   S.want(norms, count);
   S.want(active, count);
   S.want(jst, count);
@@ -1464,7 +1504,8 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
       MSQ_TRY(encode_operand(rt, t.tier, L, t.members, t.dsegs, Vt, t.evt, t.ovt));
     }
     // 4. Residual of X0, then corrections until each matrix stops.
-    MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, R, R32, norms, active, count));
+    MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, nullptr, R32, Rb, norms, active,
+                             count));
     HIP_TRY(msq::launch_convergence(st, norms, active, count, 0, rt->stream));
     for (uint32_t step = 1; step <= max_corrections; ++step) {
       for (Group& g : gl) {
@@ -1472,11 +1513,12 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
         for (const CorrTier& c : tiers)
           if (c.tier == g.corr) t = &c;
         MSQ_TRY(correction_products(rt, g.corr, L, g.members, g.dsegs, g.plan, t->ov, t->ovt, R32,
-                                    sroot, active, cb));
+                                    Rb, sroot, active, cb));
         HIP_TRY(msq::launch_add_symmetric(g.dsegs, static_cast<uint32_t>(g.members.size()), L.max_ld,
                                           cb.t2, roots_out, active, rt->stream));
       }
-      MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, R, R32, norms, active, count));
+      MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, nullptr, R32, Rb, norms, active,
+                             count));
       HIP_TRY(msq::launch_convergence(st, norms, active, count, step, rt->stream));
     }
     return MSQ_OK;

@@ -158,6 +158,9 @@ struct GemmArgs {
   float* c_float;            // Residual: R as float
   double* norms;             // Residual: ||R||^2 accumulators per status_index
   int transpose_c;           // store C^T (row-major C) instead of C
+  uint16_t* c_bf16;          // non-null: store C (Residual: R) as BF16 codes
+                             // here instead of float / double (RNE, the
+                             // codec of msq_quantize)
 };
 
 hipError_t launch_gemm(GemmType type, Epilogue epi, int tile, const GemmArgs& args,

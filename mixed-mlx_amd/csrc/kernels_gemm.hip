@@ -23,6 +23,7 @@
 // list), so small and large orders share a launch.
 #include <hip/hip_runtime.h>
 
+#include "device_codecs.hpp"
 #include "kernels.hpp"
 
 namespace msq {
@@ -224,8 +225,10 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
           const double* A = static_cast<const double*>(args.aux) + job.aux_offset;
           const size_t idx = static_cast<size_t>(row) + static_cast<size_t>(col) * ld;
           const double res = A[idx] - static_cast<double>(acc[i][j][r]);
-          C[idx] = static_cast<Out>(res);
-          args.c_float[job.c_offset + idx] = static_cast<float>(res);
+          if (args.c) C[idx] = static_cast<Out>(res);
+          if (args.c_float) args.c_float[job.c_offset + idx] = static_cast<float>(res);
+          if (args.c_bf16)
+            args.c_bf16[job.c_offset + idx] = encode_bf16(static_cast<float>(res));
           if (row < order && col < order) norm2 += res * res;
         } else {
           Out v = static_cast<Out>(acc[i][j][r]);
@@ -238,7 +241,14 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
           const size_t idx = args.transpose_c
                                  ? static_cast<size_t>(col) + static_cast<size_t>(row) * ld
                                  : static_cast<size_t>(row) + static_cast<size_t>(col) * ld;
-          C[idx] = v;
+          if constexpr (T == GemmType::F64) {
+            C[idx] = v;
+          } else {
+            if (args.c_bf16)
+              args.c_bf16[job.c_offset + idx] = encode_bf16(static_cast<float>(v));
+            else
+              C[idx] = v;
+          }
         }
       }
     }
