@@ -74,9 +74,6 @@ hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t
                                  const double* a64, float* vectors, float* diagonal,
                                  float* roots, JacobiStatus* status, hipStream_t stream);
 
-hipError_t launch_global_offnorm(const float* A, uint32_t n, uint32_t ld, double* off2, double* g2,
-                                 float* dmin, int partials, hipStream_t stream);
-
 // ------------------------------------------------------ learned selector
 constexpr int kMlpInputs = 16;
 constexpr int kMlpHidden = 64;
@@ -102,20 +99,44 @@ hipError_t launch_mlp_adam(float* params, const float* grads, float* m1, float* 
                            hipStream_t stream);
 
 // Blocked Jacobi (orders above kBatchedJacobiMaxOrder): 32-wide blocks paired
-// on the circle, 64 x 64 subproblems, rotations applied as MFMA products.
+// on the circle, 64 x 64 subproblems, rotations applied as MFMA products.  The
+// matrices of one padded order N form a class; member k's working copies are
+// A + k N^2 and V + k N^2 (float, pitch N).
 constexpr int kBlockedWidth = 32;
 uint32_t blocked_padded_order(uint32_t order);  // multiple of 64
-hipError_t launch_blocked_init(const double* a64, uint32_t n, uint32_t ld_in, float* A, float* V,
-                               uint32_t N, hipStream_t stream);
-// One block round: gather the pairs' subproblems into staging (binary64,
-// N / 64 matrices of 64 x 64), diagonalise them with the batched kernel (jobs
-// prepared by the caller: order 64, ld 64, one per pair) into U, then apply
-// the rotations to A and V (pitch N).
-hipError_t launch_blocked_round(float* A, float* V, uint32_t N, uint32_t round, double* staging,
-                                float* U, const JacobiJob* jobs, float* pair_diag,
-                                float* pair_roots, JacobiStatus* pair_status, hipStream_t stream);
-hipError_t launch_blocked_finish(const float* A, const float* Vw, uint32_t n, uint32_t N, float* V,
-                                 float* diag, float* roots, uint32_t ld_out, hipStream_t stream);
+struct BlockedMember {
+  uint64_t offset;    // binary64 input and float V output (pitch ld)
+  uint64_t s_offset;  // diagonal / roots (ld entries)
+  uint32_t order;
+  uint32_t ld;
+};
+// Working copies of every member and norm2[k] = ||A_k||_F^2 (binary64).
+hipError_t launch_blocked_init(const BlockedMember* members, uint32_t count, const double* a64,
+                               float* A, float* V, uint32_t N, double* norm2, hipStream_t stream);
+// One block round of the `count` members listed in `active`: gather each
+// member's pair subproblems into staging (binary64, count x N / 64 matrices of
+// 64 x 64, in active-list order), diagonalise them with the batched kernel
+// (jobs prepared by the caller: order 64, ld 64, one per subproblem) into U,
+// then apply the rotations to A and V.
+hipError_t launch_blocked_round(const uint32_t* active, uint32_t count, float* A, float* V,
+                                uint32_t N, uint32_t round, double* staging, float* U,
+                                const JacobiJob* jobs, float* pair_diag, float* pair_roots,
+                                JacobiStatus* pair_status, hipStream_t stream);
+// off(A)^2, ||G||_F^2 and min diag of each active member in
+// blocked_offnorm_parts(count) partial results at [member][part] (arrays of
+// kOffnormParts entries per member).
+constexpr int kOffnormParts = 64;
+uint32_t blocked_offnorm_parts(uint32_t count);
+hipError_t launch_blocked_offnorm(const uint32_t* active, uint32_t count,
+                                  const BlockedMember* members, const float* A, uint32_t N,
+                                  double* off2, double* g2, float* dmin, hipStream_t stream);
+// V (pitch ld), diagonal and roots of every member; bad[k] |= 1 for a
+// diagonal entry that is not positive and finite (bad zeroed by the caller).
+hipError_t launch_blocked_finish(const BlockedMember* members, uint32_t count, const float* A,
+                                 const float* Vw, uint32_t N, float* V, float* diag, float* roots,
+                                 uint32_t* bad, hipStream_t stream);
+hipError_t launch_scatter_status(const JacobiStatus* src, const uint32_t* index, uint32_t count,
+                                 JacobiStatus* dst, hipStream_t stream);
 
 // -------------------------------------------------------------------- GEMM
 // Operand element types of the batched square GEMMs.

@@ -12,7 +12,11 @@
 //  3. apply: A[P_p, P_q] <- U_p^T A[P_p, P_q] U_q for every p <= q (both
 //     triangles written), and V[:, P_p] <- V[:, P_p] U_p, as 64 x 64 x 64
 //     products on v_mfma_f32_16x16x4_f32 with FP32 accumulation.
-// A and V are float in device memory with pitch N.
+// A and V are float working copies in device memory with pitch N.  All
+// matrices of one padded order advance together: every launch covers the
+// active members of the class (a device list of member indices, rebuilt by
+// the host after each sweep), so a batch of thousands of large matrices costs
+// four launches per block round.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -83,40 +87,43 @@ __device__ __forceinline__ void lds_gemm64(const float* X, const float* Y, float
         C[(r0 + 16 * a + 4 * kq + r) + kPitch * (c0 + 16 * b + m)] = acc[a][b][r];
 }
 
-__global__ __launch_bounds__(256) void blocked_gather_kernel(const float* __restrict__ A, int lda,
-                                                             int nb, int t,
-                                                             double* __restrict__ S) {
+__global__ __launch_bounds__(256) void blocked_gather_kernel(const uint32_t* __restrict__ active,
+                                                             const float* __restrict__ A, int N,
+                                                             int t, double* __restrict__ S) {
+  const int nb = N / kB, np = nb / 2;
   const int p = blockIdx.x;
-  double* Sp = S + static_cast<size_t>(p) * kP * kP;
+  const float* Am = A + static_cast<size_t>(active[blockIdx.y]) * N * N;
+  double* Sp = S + (static_cast<size_t>(blockIdx.y) * np + p) * kP * kP;
   for (int e = threadIdx.x; e < kP * kP; e += blockDim.x) {
     const int i = e % kP, j = e / kP;
     Sp[e] = static_cast<double>(
-        A[pair_index(t, p, nb, i) + static_cast<size_t>(lda) * pair_index(t, p, nb, j)]);
+        Am[pair_index(t, p, nb, i) + static_cast<size_t>(N) * pair_index(t, p, nb, j)]);
   }
 }
 
 // Block (p, q), p <= q, of the pair grid: A[P_p, P_q] <- U_p^T A[P_p, P_q] U_q,
 // and its transpose into A[P_q, P_p].
-__global__ __launch_bounds__(256) void blocked_apply_a_kernel(float* __restrict__ A, int lda,
-                                                              int nb, int t,
+__global__ __launch_bounds__(256) void blocked_apply_a_kernel(const uint32_t* __restrict__ active,
+                                                              float* __restrict__ A, int N, int t,
                                                               const float* __restrict__ U) {
   __shared__ float Up[kP * kPitch], Uq[kP * kPitch], Bm[kP * kPitch];
-  // blockIdx.x -> (p, q), p <= q, over npairs (npairs + 1) / 2 entries.
-  const int np = nb / 2;
+  const int nb = N / kB, np = nb / 2;
+  // blockIdx.x -> (p, q), p <= q, over np (np + 1) / 2 entries.
   int p = 0, rem = blockIdx.x;
   while (rem >= np - p) {
     rem -= np - p;
     ++p;
   }
   const int q = p + rem;
-  const float* up = U + static_cast<size_t>(p) * kP * kP;
-  const float* uq = U + static_cast<size_t>(q) * kP * kP;
+  float* Am = A + static_cast<size_t>(active[blockIdx.y]) * N * N;
+  const float* up = U + (static_cast<size_t>(blockIdx.y) * np + p) * kP * kP;
+  const float* uq = U + (static_cast<size_t>(blockIdx.y) * np + q) * kP * kP;
   for (int e = threadIdx.x; e < kP * kP; e += blockDim.x) {
     const int i = e % kP, j = e / kP;
     Up[i + kPitch * j] = up[e];
     Uq[i + kPitch * j] = uq[e];
     Bm[i + kPitch * j] =
-        A[pair_index(t, p, nb, i) + static_cast<size_t>(lda) * pair_index(t, q, nb, j)];
+        Am[pair_index(t, p, nb, i) + static_cast<size_t>(N) * pair_index(t, q, nb, j)];
   }
   __syncthreads();
   lds_gemm64<true>(Up, Bm, Up);  // T = U_p^T B, over U_p
@@ -125,105 +132,246 @@ __global__ __launch_bounds__(256) void blocked_apply_a_kernel(float* __restrict_
   __syncthreads();
   for (int e = threadIdx.x; e < kP * kP; e += blockDim.x) {
     const int i = e % kP, j = e / kP;
-    A[pair_index(t, p, nb, i) + static_cast<size_t>(lda) * pair_index(t, q, nb, j)] =
+    Am[pair_index(t, p, nb, i) + static_cast<size_t>(N) * pair_index(t, q, nb, j)] =
         Bm[i + kPitch * j];
     if (p != q)
-      A[pair_index(t, q, nb, i) + static_cast<size_t>(lda) * pair_index(t, p, nb, j)] =
+      Am[pair_index(t, q, nb, i) + static_cast<size_t>(N) * pair_index(t, p, nb, j)] =
           Bm[j + kPitch * i];
   }
 }
 
 // V[rows of chunk c, P_p] <- V[rows, P_p] U_p (64 rows per workgroup).
-__global__ __launch_bounds__(256) void blocked_apply_v_kernel(float* __restrict__ V, int ldv,
-                                                              int nb, int t,
+__global__ __launch_bounds__(256) void blocked_apply_v_kernel(const uint32_t* __restrict__ active,
+                                                              float* __restrict__ V, int N, int t,
                                                               const float* __restrict__ U) {
   __shared__ float Up[kP * kPitch], Vc[kP * kPitch];
+  const int nb = N / kB, np = nb / 2;
   const int c = blockIdx.x, p = blockIdx.y;
-  const float* up = U + static_cast<size_t>(p) * kP * kP;
+  float* Vm = V + static_cast<size_t>(active[blockIdx.z]) * N * N;
+  const float* up = U + (static_cast<size_t>(blockIdx.z) * np + p) * kP * kP;
   for (int e = threadIdx.x; e < kP * kP; e += blockDim.x) {
     const int i = e % kP, j = e / kP;
     Up[i + kPitch * j] = up[e];
-    Vc[i + kPitch * j] = V[(kP * c + i) + static_cast<size_t>(ldv) * pair_index(t, p, nb, j)];
+    Vc[i + kPitch * j] = Vm[(kP * c + i) + static_cast<size_t>(N) * pair_index(t, p, nb, j)];
   }
   __syncthreads();
   lds_gemm64<false>(Vc, Up, Vc);
   __syncthreads();
   for (int e = threadIdx.x; e < kP * kP; e += blockDim.x) {
     const int i = e % kP, j = e / kP;
-    V[(kP * c + i) + static_cast<size_t>(ldv) * pair_index(t, p, nb, j)] = Vc[i + kPitch * j];
+    Vm[(kP * c + i) + static_cast<size_t>(N) * pair_index(t, p, nb, j)] = Vc[i + kPitch * j];
   }
 }
 
-// Working copies (pitch N, zero padding): A from the binary64 input (pitch
-// ld_in), V = I.
-__global__ void blocked_init_kernel(const double* __restrict__ a64, int n, int ld_in,
-                                    float* __restrict__ A, float* __restrict__ V, int N) {
+// Working copies of member blockIdx.y (pitch N, zero padding): A from the
+// binary64 input, V = I.
+__global__ void blocked_init_kernel(const BlockedMember* __restrict__ members,
+                                    const double* __restrict__ a64, float* __restrict__ A,
+                                    float* __restrict__ V, int N) {
+  const BlockedMember mb = members[blockIdx.y];
+  const int n = static_cast<int>(mb.order), ld = static_cast<int>(mb.ld);
+  const double* a = a64 + mb.offset;
+  float* Am = A + static_cast<size_t>(blockIdx.y) * N * N;
+  float* Vm = V + static_cast<size_t>(blockIdx.y) * N * N;
   const size_t total = static_cast<size_t>(N) * N;
   for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
        e += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const int i = static_cast<int>(e % N), j = static_cast<int>(e / N);
-    A[e] = (i < n && j < n) ? static_cast<float>(a64[i + static_cast<size_t>(j) * ld_in]) : 0.0f;
-    V[e] = i == j ? 1.0f : 0.0f;
+    Am[e] = (i < n && j < n) ? static_cast<float>(a[i + static_cast<size_t>(j) * ld]) : 0.0f;
+    Vm[e] = i == j ? 1.0f : 0.0f;
   }
 }
 
-// Outputs: V (pitch ld_out, zero outside the order), diagonal and roots
-// (ld_out entries).
-__global__ void blocked_finish_kernel(const float* __restrict__ A, const float* __restrict__ Vw,
-                                      int n, int N, float* __restrict__ V, float* __restrict__ diag,
-                                      float* __restrict__ roots, int ld_out) {
-  const size_t total = static_cast<size_t>(ld_out) * ld_out;
+// Block-wide sums / minimum through LDS (blockDim.x a multiple of 64, at most
+// 1024); every thread gets the result.
+__device__ double block_sum(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int k = 0; k < nw; ++k) t += red[k];
+  return t;
+}
+__device__ float block_min(float v, float* red) {
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  float t = INFINITY;
+  for (int k = 0; k < nw; ++k) t = fminf(t, red[k]);
+  return t;
+}
+
+// ||A||_F^2 of member blockIdx.x from its binary64 input.
+__global__ __launch_bounds__(512) void blocked_norm_kernel(const BlockedMember* __restrict__ members,
+                                                           const double* __restrict__ a64,
+                                                           double* __restrict__ norm2) {
+  __shared__ double red[16];
+  const BlockedMember mb = members[blockIdx.x];
+  const int n = static_cast<int>(mb.order), ld = static_cast<int>(mb.ld);
+  const double* a = a64 + mb.offset;
+  double s = 0.0;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const double x = a[(e % n) + static_cast<size_t>(e / n) * ld];
+    s += x * x;
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) norm2[blockIdx.x] = s;
+}
+
+// Stop-rule measures of active member active[blockIdx.y] (entries 0..n-1 of
+// its working copy), part blockIdx.x of gridDim.x: off(A)^2, ||G||_F^2 with
+// G_ij = a_ij / (s_i + s_j), s_i = sqrt(a_ii) (float quotient, binary64 sum,
+// as in the batched kernel), and min a_ii; written at [member][part].
+__global__ __launch_bounds__(512) void blocked_offnorm_kernel(
+    const uint32_t* __restrict__ active, const BlockedMember* __restrict__ members,
+    const float* __restrict__ A, int N, double* __restrict__ off2, double* __restrict__ g2,
+    float* __restrict__ dmin) {
+  __shared__ double red[16];
+  __shared__ float redf[16];
+  const uint32_t mem = active[blockIdx.y];
+  const int n = static_cast<int>(members[mem].order);
+  const float* Am = A + static_cast<size_t>(mem) * N * N;
+  const size_t total = static_cast<size_t>(n) * n;
+  double s = 0.0, g = 0.0;
+  float d = INFINITY;
   for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
        e += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    const int i = static_cast<int>(e % ld_out), j = static_cast<int>(e / ld_out);
-    V[e] = (i < n && j < n) ? Vw[i + static_cast<size_t>(j) * N] : 0.0f;
-    if (j == 0) {
-      const float d = i < n ? A[i + static_cast<size_t>(i) * N] : 0.0f;
-      diag[i] = d;
-      roots[i] = i < n ? sqrtf(d) : 0.0f;
+    const int i = static_cast<int>(e % n), j = static_cast<int>(e / n);
+    const float v = Am[i + static_cast<size_t>(j) * N];
+    if (i == j) {
+      d = fminf(d, v);
+    } else {
+      s += static_cast<double>(v) * v;
+      const float den = sqrtf(fmaxf(Am[i + static_cast<size_t>(i) * N], 0.0f)) +
+                        sqrtf(fmaxf(Am[j + static_cast<size_t>(j) * N], 0.0f));
+      const float q = den > 0.0f ? v * __frcp_rn(den) : (v != 0.0f ? INFINITY : 0.0f);
+      g += static_cast<double>(q * q);
     }
   }
+  s = block_sum(s, red);
+  g = block_sum(g, red);
+  d = block_min(d, redf);
+  if (threadIdx.x == 0) {
+    const size_t slot = static_cast<size_t>(mem) * kOffnormParts + blockIdx.x;
+    off2[slot] = s;
+    g2[slot] = g;
+    dmin[slot] = d;
+  }
+}
+
+// Outputs of member blockIdx.y: V (pitch ld, zero outside the order),
+// diagonal and roots (ld entries); bad[member] = 1 if a diagonal entry of
+// the order is not positive and finite.
+__global__ void blocked_finish_kernel(const BlockedMember* __restrict__ members,
+                                      const float* __restrict__ A, const float* __restrict__ Vw,
+                                      int N, float* __restrict__ V, float* __restrict__ diag,
+                                      float* __restrict__ roots, uint32_t* __restrict__ bad) {
+  const BlockedMember mb = members[blockIdx.y];
+  const int n = static_cast<int>(mb.order), ld = static_cast<int>(mb.ld);
+  const float* Am = A + static_cast<size_t>(blockIdx.y) * N * N;
+  const float* Vm = Vw + static_cast<size_t>(blockIdx.y) * N * N;
+  float* Vo = V + mb.offset;
+  const size_t total = static_cast<size_t>(ld) * ld;
+  for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+       e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int i = static_cast<int>(e % ld), j = static_cast<int>(e / ld);
+    Vo[e] = (i < n && j < n) ? Vm[i + static_cast<size_t>(j) * N] : 0.0f;
+    if (j == 0) {
+      const float d = i < n ? Am[i + static_cast<size_t>(i) * N] : 0.0f;
+      diag[mb.s_offset + i] = d;
+      roots[mb.s_offset + i] = i < n ? sqrtf(d) : 0.0f;
+      if (i < n && !(d > 0.0f && isfinite(d))) atomicOr(bad + blockIdx.y, 1u);
+    }
+  }
+}
+
+__global__ void scatter_status_kernel(const JacobiStatus* __restrict__ src,
+                                      const uint32_t* __restrict__ index, uint32_t count,
+                                      JacobiStatus* __restrict__ dst) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < count) dst[index[k]] = src[k];
+}
+
+// Workgroups per member of an elementwise pass: enough for the whole batch
+// to fill the chip, at least 4 per member.
+int grid_chunks(size_t elements, uint32_t count) {
+  const size_t g = (elements + 255) / 256;
+  size_t cap = 16384 / count;
+  if (cap < 4) cap = 4;
+  return static_cast<int>(g < cap ? g : cap);
 }
 
 }  // namespace
 
 uint32_t blocked_padded_order(uint32_t order) { return (order + 63u) & ~63u; }
 
-hipError_t launch_blocked_init(const double* a64, uint32_t n, uint32_t ld_in, float* A, float* V,
-                               uint32_t N, hipStream_t stream) {
-  const size_t total = static_cast<size_t>(N) * N;
-  size_t grid = (total + 255) / 256;
-  if (grid > 65536) grid = 65536;
-  blocked_init_kernel<<<static_cast<int>(grid), 256, 0, stream>>>(a64, n, ld_in, A, V, N);
-  return hipGetLastError();
-}
-
-hipError_t launch_blocked_round(float* A, float* V, uint32_t N, uint32_t round, double* staging,
-                                float* U, const JacobiJob* jobs, float* pair_diag,
-                                float* pair_roots, JacobiStatus* pair_status,
-                                hipStream_t stream) {
-  const int nb = static_cast<int>(N) / kB;
-  const int np = nb / 2;
-  if (N % kP != 0 || nb < 2) return hipErrorInvalidValue;
-  blocked_gather_kernel<<<np, 256, 0, stream>>>(A, N, nb, round, staging);
+hipError_t launch_blocked_init(const BlockedMember* members, uint32_t count, const double* a64,
+                               float* A, float* V, uint32_t N, double* norm2, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  if (N % kP != 0 || count > 65535) return hipErrorInvalidValue;
+  blocked_init_kernel<<<dim3(grid_chunks(static_cast<size_t>(N) * N, count), count), 256, 0, stream>>>(
+      members, a64, A, V, static_cast<int>(N));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  e = launch_batched_jacobi(jobs, np, kP, staging, U, pair_diag, pair_roots, pair_status, stream);
-  if (e != hipSuccess) return e;
-  blocked_apply_a_kernel<<<np * (np + 1) / 2, 256, 0, stream>>>(A, N, nb, round, U);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  blocked_apply_v_kernel<<<dim3(N / kP, np), 256, 0, stream>>>(V, N, nb, round, U);
+  blocked_norm_kernel<<<count, 512, 0, stream>>>(members, a64, norm2);
   return hipGetLastError();
 }
 
-hipError_t launch_blocked_finish(const float* A, const float* Vw, uint32_t n, uint32_t N, float* V,
-                                 float* diag, float* roots, uint32_t ld_out, hipStream_t stream) {
-  const size_t total = static_cast<size_t>(ld_out) * ld_out;
-  size_t grid = (total + 255) / 256;
-  if (grid > 65536) grid = 65536;
-  blocked_finish_kernel<<<static_cast<int>(grid), 256, 0, stream>>>(A, Vw, n, N, V, diag, roots,
-                                                                    ld_out);
+hipError_t launch_blocked_round(const uint32_t* active, uint32_t count, float* A, float* V,
+                                uint32_t N, uint32_t round, double* staging, float* U,
+                                const JacobiJob* jobs, float* pair_diag, float* pair_roots,
+                                JacobiStatus* pair_status, hipStream_t stream) {
+  const int nb = static_cast<int>(N) / kB;
+  const int np = nb / 2;
+  if (N % kP != 0 || nb < 2 || count > 65535) return hipErrorInvalidValue;
+  if (count == 0) return hipSuccess;
+  const int Ni = static_cast<int>(N), t = static_cast<int>(round);
+  blocked_gather_kernel<<<dim3(np, count), 256, 0, stream>>>(active, A, Ni, t, staging);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = launch_batched_jacobi(jobs, count * np, kP, staging, U, pair_diag, pair_roots, pair_status,
+                            stream);
+  if (e != hipSuccess) return e;
+  blocked_apply_a_kernel<<<dim3(np * (np + 1) / 2, count), 256, 0, stream>>>(active, A, Ni, t, U);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  blocked_apply_v_kernel<<<dim3(N / kP, np, count), 256, 0, stream>>>(active, V, Ni, t, U);
+  return hipGetLastError();
+}
+
+uint32_t blocked_offnorm_parts(uint32_t count) {
+  const uint32_t p = count ? (16384u + count - 1) / count : 1u;
+  return p < 1u ? 1u : (p > static_cast<uint32_t>(kOffnormParts) ? kOffnormParts : p);
+}
+
+hipError_t launch_blocked_offnorm(const uint32_t* active, uint32_t count,
+                                  const BlockedMember* members, const float* A, uint32_t N,
+                                  double* off2, double* g2, float* dmin, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  if (count > 65535) return hipErrorInvalidValue;
+  blocked_offnorm_kernel<<<dim3(blocked_offnorm_parts(count), count), 512, 0, stream>>>(
+      active, members, A, static_cast<int>(N), off2, g2, dmin);
+  return hipGetLastError();
+}
+
+hipError_t launch_blocked_finish(const BlockedMember* members, uint32_t count, const float* A,
+                                 const float* Vw, uint32_t N, float* V, float* diag, float* roots,
+                                 uint32_t* bad, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  if (count > 65535) return hipErrorInvalidValue;
+  blocked_finish_kernel<<<dim3(grid_chunks(static_cast<size_t>(N) * N, count), count), 256, 0, stream>>>(
+      members, A, Vw, static_cast<int>(N), V, diag, roots, bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_status(const JacobiStatus* src, const uint32_t* index, uint32_t count,
+                                 JacobiStatus* dst, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  scatter_status_kernel<<<(count + 255) / 256, 256, 0, stream>>>(src, index, count, dst);
   return hipGetLastError();
 }
 

@@ -14,8 +14,7 @@
 // device memory above.  The whole solve -- all sweeps, the FP64 off-norm test
 // after each sweep and the square roots -- is one launch per order class of
 // the batch.  Larger orders use the blocked solver (kernels_blocked.hip),
-// whose 64 x 64 subproblems run on this kernel; global_offnorm_kernel below
-// measures its off-norm.
+// whose 64 x 64 subproblems run on this kernel.
 //
 // The schedule is the reference's circle method in closed form: in round t the
 // participant at slot x is P_t(0) = 0, P_t(x) = 1 + ((x - 1 - t) mod (N - 1)),
@@ -88,14 +87,6 @@ __device__ __forceinline__ Block rotate_block(Block b, float2 r0, float2 r1,
   return o;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-__device__ __forceinline__ float wave_min(float v) {
-  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
-  return v;
-}
 
 // Launch shape for a padded order N (a multiple of 16 up to 256).
 //
@@ -713,55 +704,6 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   }
 }
 
-// ------------------------------------------------- large n: off-norm only
-// Partial sums of the FP64 off-diagonal norm and the diagonal minimum.
-__global__ void global_offnorm_kernel(const float* __restrict__ A, int n, int ld,
-                                      double* __restrict__ out_off2,
-                                      double* __restrict__ out_g2,
-                                      float* __restrict__ out_min) {
-  double s = 0.0, g = 0.0;
-  float d = INFINITY;
-  const size_t total = static_cast<size_t>(n) * n;
-  for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
-       e += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    const int i = static_cast<int>(e % n);
-    const int j = static_cast<int>(e / n);
-    const float v = A[i + static_cast<size_t>(j) * ld];
-    if (i == j) {
-      d = fminf(d, v);
-    } else {
-      s += static_cast<double>(v) * v;
-      const double den = sqrt(static_cast<double>(fmaxf(A[i + static_cast<size_t>(i) * ld], 0.0f))) +
-                         sqrt(static_cast<double>(fmaxf(A[j + static_cast<size_t>(j) * ld], 0.0f)));
-      g += den > 0.0 ? (v / den) * (v / den) : (v != 0.0f ? INFINITY : 0.0);
-    }
-  }
-  __shared__ double sd[16], sg[16];
-  __shared__ float sf[16];
-  s = wave_sum(s);
-  g = wave_sum(g);
-  d = wave_min(d);
-  if ((threadIdx.x & 63) == 0) {
-    sd[threadIdx.x / 64] = s;
-    sg[threadIdx.x / 64] = g;
-    sf[threadIdx.x / 64] = d;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0, tg = 0.0;
-    float m = INFINITY;
-    for (int w = 0; w < static_cast<int>(blockDim.x / 64); ++w) {
-      t += sd[w];
-      tg += sg[w];
-      m = fminf(m, sf[w]);
-    }
-    out_off2[blockIdx.x] = t;
-    out_g2[blockIdx.x] = tg;
-    out_min[blockIdx.x] = m;
-  }
-}
-
-
 template <int NP>
 hipError_t launch_order(const JacobiJob* jobs, uint32_t count, int threads, const double* a64,
                         float* vectors, float* diagonal, float* roots, JacobiStatus* status,
@@ -815,14 +757,6 @@ hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t
 #undef MSQ_ORDER
   }
   return hipErrorInvalidValue;
-}
-
-
-
-hipError_t launch_global_offnorm(const float* A, uint32_t n, uint32_t ld, double* off2,
-                                 double* g2, float* dmin, int partials, hipStream_t stream) {
-  global_offnorm_kernel<<<partials, 256, 0, stream>>>(A, n, ld, off2, g2, dmin);
-  return hipGetLastError();
 }
 
 

@@ -218,6 +218,33 @@ def test_jacobi_large_order_blocked_path(runtime, n):
     assert err <= rep.off_norm + 4e-6 * np.sqrt(n) * np.linalg.norm(A)
 
 
+def test_jacobi_large_orders_batched_classes(runtime):
+    """Many orders above 256 in one batch: the matrices of each padded order
+    advance together (one launch per kernel per block round for the class)
+    and leave the active list as they stop.  Every matrix's eigenvectors,
+    diagonal and report equal those of the same matrix solved alone, bit for
+    bit, and pass the accuracy checks of the single-matrix test."""
+    orders = [300, 260, 520, 400, 64, 300, 257, 384]
+    conds = [1e2, 1e4, 1e1, 1e3, 1e2, 1e5, 1e3, 1e2]
+    mats = [G.single_matrix(11, n, 1.0 / c, index=i)[0]
+            for i, (n, c) in enumerate(zip(orders, conds))]
+    say("jacobi blocked batch", orders)
+    opts = capi.jacobi_options(off_tol=1e-7)
+    Vs, Ds, _, reps = _jacobi_batch(runtime, mats, opts)
+    for k in (0, 2, 5):
+        Va, Da, _, ra = _jacobi_batch(runtime, [mats[k]], opts)
+        assert np.array_equal(Va[0], Vs[k]) and np.array_equal(Da[0], Ds[k])
+        assert ra[0].sweeps == reps[k].sweeps and ra[0].off_norm == reps[k].off_norm
+    for A, V, d, rep in zip(mats, Vs, Ds, reps):
+        n = A.shape[0]
+        lam = np.linalg.eigvalsh(A)
+        drift = 2.0 * 2.0**-23 * np.linalg.norm(A) * np.sqrt(rep.sweeps * n)
+        assert np.linalg.norm(np.sort(d) - lam) <= rep.off_norm + drift
+        assert S.orthogonality_error(V) < 2e-6 * np.sqrt(n)
+        err = np.linalg.norm((V * d) @ V.T - A)
+        assert err <= rep.off_norm + 4e-6 * np.sqrt(n) * np.linalg.norm(A)
+
+
 def test_orthogonality_measure(runtime):
     orders = [16, 64]
     Vs = [random_basis(n, 3 + i) for i, n in enumerate(orders)]
