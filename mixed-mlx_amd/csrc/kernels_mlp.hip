@@ -27,19 +27,46 @@ constexpr int kW1 = 0, kB1 = kW1 + kHid * kIn, kW2 = kB1 + kHid, kB2 = kW2 + kHi
 static_assert(kParams == kMlpParams, "parameter layout");
 
 // ------------------------------------------------------------------ features
-constexpr int kFeatThreads = 256;
+// One workgroup of 1024 threads per matrix.  Pass 1 reads A (binary64) once
+// for the diagonal, Gershgorin, dominance and norm features.  The power
+// iterations run on A rounded to float (an A-perturbation of 2^-24 relative,
+// far below what the features resolve) with binary64 accumulation:
+//  * order <= 256: A lives in registers, four lanes per row (lane p of row i
+//    holds columns 4 k + p), so the 40 iterations never touch memory again;
+//  * order 257..512: one row per thread, A re-read from memory each step.
+constexpr int kFeatThreads = 1024;
+constexpr int kFeatRegOrder = 256;               // register path up to this order
+constexpr int kFeatCols = kFeatRegOrder / 4;     // register values per lane
 
-// Block-wide sum in a fixed order (wave shuffles, then wave partials in
-// wave order): the result does not depend on timing.
-__device__ double block_sum(double v, double* red) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+// Block-wide reductions in a fixed order (a butterfly within each wave, then
+// a butterfly over the 16 wave partials, which every lane reads back): the
+// results do not depend on timing, and few values are live at a time.
+constexpr int kFeatWaves = kFeatThreads / 64;
+static_assert(kFeatWaves == 16, "the partial butterfly spans 16 lanes");
+__device__ void block_sum2(double& a, double& b, double* red) {
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
-  if (lane == 0) red[wave] = v;
+  if (lane == 0) {
+    red[2 * wave] = a;
+    red[2 * wave + 1] = b;
+  }
   __syncthreads();
-  double s = 0.0;
-  for (int w = 0; w < kFeatThreads / 64; ++w) s += red[w];
-  return s;
+  const int w = lane & (kFeatWaves - 1);
+  a = red[2 * w];
+  b = red[2 * w + 1];
+  for (int off = kFeatWaves / 2; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+  }
+}
+__device__ double block_sum(double v, double* red) {
+  double z = 0.0;
+  block_sum2(v, z, red);
+  return v;
 }
 __device__ double block_min(double v, double* red) {
   for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
@@ -47,11 +74,17 @@ __device__ double block_min(double v, double* red) {
   __syncthreads();
   if (lane == 0) red[wave] = v;
   __syncthreads();
-  double s = red[0];
-  for (int w = 1; w < kFeatThreads / 64; ++w) s = fmin(s, red[w]);
-  return s;
+  v = red[lane & (kFeatWaves - 1)];
+  for (int off = kFeatWaves / 2; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+  return v;
 }
 __device__ double block_max(double v, double* red) { return -block_min(-v, red); }
+// Sum over the four lanes of a row (lanes 4 r .. 4 r + 3).
+__device__ __forceinline__ double quad_sum(double v) {
+  v += __shfl_xor(v, 1);
+  v += __shfl_xor(v, 2);
+  return v;
+}
 
 // Start vector of the lambda_min iteration: a fixed hash of the index in
 // [-0.5, 0.5) (any vector with a component along the bottom eigenvector).
@@ -60,70 +93,93 @@ __device__ __forceinline__ double start_vector(int j) {
   return static_cast<double>((h >> 16) & 0xffffu) / 65536.0 - 0.5;
 }
 
-// One workgroup per matrix; A binary64 column-major (pitch ld), order <= 512.
+// A binary64 column-major (pitch ld), order <= 512.  REGS: the register
+// path; each instantiation runs over the whole batch and its workgroups of
+// the other path's orders return at once.
+template <bool REGS>
 __global__ __launch_bounds__(kFeatThreads) void features_kernel(const Segment* __restrict__ segs,
                                                                 const double* __restrict__ a64,
                                                                 float* __restrict__ features) {
   __shared__ double x[kMaxFeatureOrder];
-  __shared__ double red[kFeatThreads / 64];
+  __shared__ double red[2 * kFeatThreads / 64];
   const Segment sg = segs[blockIdx.x];
   const int n = static_cast<int>(sg.order);
+  if (REGS != (n <= kFeatRegOrder)) return;
   const int ld = static_cast<int>(sg.ld);
   const double* A = a64 + sg.offset;
   const int tid = threadIdx.x;
-  constexpr int kRows = kMaxFeatureOrder / kFeatThreads;  // rows per thread
+  constexpr bool regs = REGS;
+  // Row and column phase of this lane: register path (row tid / 4, columns
+  // 4 k + tid % 4), else row tid over all columns.
+  const int row = regs ? tid >> 2 : tid;
+  const int phase = regs ? tid & 3 : 0;
+  const int stride = regs ? 4 : 1;
+  const bool row_ok = row < n;
 
-  // Pass 1: diagonal, off-diagonal row sums, Frobenius norms.
-  double diag[kRows], rsum[kRows];
-  double fro = 0.0, off = 0.0;
+  // Pass 1: diagonal, off-diagonal absolute row sums, Frobenius norms.
+  float a[REGS ? kFeatCols : 1];
+  double diag = 0.0, rsum = 0.0, fro = 0.0, off = 0.0;
+  if constexpr (REGS) {
+    // Chunks of 8 columns, kept apart (sched_barrier) so that the loads in
+    // flight stay within the register budget of 1024-lane workgroups.
 #pragma unroll
-  for (int r = 0; r < kRows; ++r) {
-    diag[r] = 0.0;
-    rsum[r] = 0.0;
-  }
-  for (int j = 0; j < n; ++j) {
+    for (int k0 = 0; k0 < kFeatCols; k0 += 8) {
 #pragma unroll
-    for (int r = 0; r < kRows; ++r) {
-      const int i = tid + r * kFeatThreads;
-      if (i < n) {
-        const double v = A[i + static_cast<size_t>(j) * ld];
+      for (int k = k0; k < k0 + 8; ++k) {
+        // Unconditional in-bounds load (clamped indices), then a select; the
+        // empty asm keeps the load out of a branch (the compiler would
+        // otherwise make each load conditional, one basic block per column).
+        const int j = 4 * k + phase;
+        double raw = A[min(row, n - 1) + static_cast<size_t>(min(j, n - 1)) * ld];
+        asm volatile("" : "+v"(raw));
+        const double v = (row_ok && j < n) ? raw : 0.0;
+        a[k] = static_cast<float>(v);
         fro += v * v;
-        if (i == j) {
-          diag[r] = v;
+        if (j == row) {
+          diag = v;
         } else {
-          rsum[r] += fabs(v);
+          rsum += fabs(v);
           off += v * v;
         }
       }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    diag = quad_sum(diag);
+    rsum = quad_sum(rsum);
+  } else {
+    for (int j = 0; j < n && row_ok; ++j) {
+      const double v = A[row + static_cast<size_t>(j) * ld];
+      fro += v * v;
+      if (j == row) {
+        diag = v;
+      } else {
+        rsum += fabs(v);
+        off += v * v;
+      }
     }
   }
+  // Per-row features from one lane of each row.
+  const bool lead = row_ok && phase == 0;
   double tr = 0.0, dmin = INFINITY, dmax = -INFINITY, gl = INFINITY, gh = -INFINITY;
   double logd = 0.0, logd2 = 0.0, dommin = INFINITY, domsum = 0.0, weak = 0.0;
-#pragma unroll
-  for (int r = 0; r < kRows; ++r) {
-    const int i = tid + r * kFeatThreads;
-    if (i < n) {
-      const double d = diag[r];
-      tr += d;
-      dmin = fmin(dmin, d);
-      dmax = fmax(dmax, d);
-      gl = fmin(gl, d - rsum[r]);
-      gh = fmax(gh, d + rsum[r]);
-      const double ld10 = log10(fmax(d, 1e-300));
-      logd += ld10;
-      logd2 += ld10 * ld10;
-      const double dom = fmin(fmax(d / fmax(rsum[r], 1e-300), 1e-6), 1e6);
-      dommin = fmin(dommin, dom);
-      domsum += dom;
-      weak += d < rsum[r] ? 1.0 : 0.0;
-    }
+  if (lead) {
+    const double d = diag;
+    tr = d;
+    dmin = d;
+    dmax = d;
+    gl = d - rsum;
+    gh = d + rsum;
+    const double ld10 = log10(fmax(d, 1e-300));
+    logd = ld10;
+    logd2 = ld10 * ld10;
+    const double dom = fmin(fmax(d / fmax(rsum, 1e-300), 1e-6), 1e6);
+    dommin = dom;
+    domsum = dom;
+    weak = d < rsum ? 1.0 : 0.0;
   }
-  tr = block_sum(tr, red);
-  fro = block_sum(fro, red);
-  off = block_sum(off, red);
-  logd = block_sum(logd, red);
-  logd2 = block_sum(logd2, red);
-  domsum = block_sum(domsum, red);
+  block_sum2(tr, fro, red);
+  block_sum2(off, logd, red);
+  block_sum2(logd2, domsum, red);
   weak = block_sum(weak, red);
   dmin = block_min(dmin, red);
   dmax = block_max(dmax, red);
@@ -131,68 +187,63 @@ __global__ __launch_bounds__(kFeatThreads) void features_kernel(const Segment* _
   gh = block_max(gh, red);
   dommin = block_min(dommin, red);
 
-  // y = A x (c != 0: y = c x - A x) for this thread's rows; x in LDS.
-  auto matvec = [&](double c, double (&y)[kRows]) {
-#pragma unroll
-    for (int r = 0; r < kRows; ++r) y[r] = 0.0;
-    for (int j = 0; j < n; ++j) {
-      const double xj = x[j];
-#pragma unroll
-      for (int r = 0; r < kRows; ++r) {
-        const int i = tid + r * kFeatThreads;
-        if (i < n) y[r] += A[i + static_cast<size_t>(j) * ld] * xj;
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < kRows; ++r) {
-      const int i = tid + r * kFeatThreads;
-      if (i < n) y[r] = c != 0.0 ? c * x[i] - y[r] : y[r];
-    }
-  };
-  // Power iteration: `iters` steps of x <- y / ||y|| with y = (c I - A) x
-  // (c = 0: y = A x); returns the last two Rayleigh quotients x^T y.
-  auto power = [&](double c, int iters, double& last, double& prev) {
-    double y[kRows];
-    last = prev = 0.0;
-    for (int it = 0; it < iters; ++it) {
-      matvec(c, y);
-      double q = 0.0, yy = 0.0;
-#pragma unroll
-      for (int r = 0; r < kRows; ++r) {
-        const int i = tid + r * kFeatThreads;
-        if (i < n) {
-          q += x[i] * y[r];
-          yy += y[r] * y[r];
-        }
-      }
-      q = block_sum(q, red);
-      yy = block_sum(yy, red);
-      prev = last;
-      last = q;
-      const double inv = yy > 0.0 ? 1.0 / sqrt(yy) : 0.0;
-      __syncthreads();  // every thread has read x
-#pragma unroll
-      for (int r = 0; r < kRows; ++r) {
-        const int i = tid + r * kFeatThreads;
-        if (i < n) x[i] = y[r] * inv;
-      }
+  // Power iterations: kPowerIterationsMax steps of x <- y / ||y|| with
+  // y = A x from x = 1 / sqrt(n) (lambda_max), then kPowerIterationsMin steps
+  // with y = c x - A x, c just above lambda_max, from the start vector (top of
+  // c I - A: lambda_min = c - mu).  The last two Rayleigh quotients x^T y of
+  // each phase are kept.  x beyond the order stays zero (the register path
+  // reads columns up to the next multiple of 4).
+  const double inv_sqrt_n = 1.0 / sqrt(static_cast<double>(n));
+  for (int i = tid; i < kMaxFeatureOrder; i += kFeatThreads) x[i] = i < n ? inv_sqrt_n : 0.0;
+  __syncthreads();
+  double lmax = 0.0, lmax_prev = 0.0, mu = 0.0, mu_prev = 0.0, c = 0.0;
+  const int total_iters = kPowerIterationsMax + kPowerIterationsMin;
+#pragma unroll 1
+  for (int it = 0; it < total_iters; ++it) {
+    if (it == kPowerIterationsMax) {
+      c = 1.0001 * fmax(lmax, 1e-300);
+      double snorm = 0.0;
+      for (int i = tid; i < n; i += kFeatThreads) snorm += start_vector(i) * start_vector(i);
+      snorm = block_sum(snorm, red);
+      for (int i = tid; i < n; i += kFeatThreads) x[i] = start_vector(i) / sqrt(snorm);
       __syncthreads();
     }
-  };
-  const double inv_sqrt_n = 1.0 / sqrt(static_cast<double>(n));
-  for (int i = tid; i < n; i += kFeatThreads) x[i] = inv_sqrt_n;
-  __syncthreads();
-  double lmax, lmax_prev;
-  power(0.0, kPowerIterationsMax, lmax, lmax_prev);
-  // Bottom of the spectrum: top of (c I - A) with c just above lambda_max.
-  const double c = 1.0001 * fmax(lmax, 1e-300);
-  double snorm = 0.0;
-  for (int i = tid; i < n; i += kFeatThreads) snorm += start_vector(i) * start_vector(i);
-  snorm = block_sum(snorm, red);
-  for (int i = tid; i < n; i += kFeatThreads) x[i] = start_vector(i) / sqrt(snorm);
-  __syncthreads();
-  double mu, mu_prev;
-  power(c, kPowerIterationsMin, mu, mu_prev);
+    double y = 0.0;
+    if constexpr (REGS) {
+      // The register copy of A is pinned as float for each step (empty asm, no
+      // instruction): otherwise its widening to binary64 is hoisted out of the
+      // loop and the doubled copy no longer fits the register file.
+#pragma unroll
+      for (int k = 0; k < kFeatCols; ++k) asm volatile("" : "+v"(a[k]));
+#pragma unroll
+      for (int k0 = 0; k0 < kFeatCols; k0 += 8) {
+        if (4 * k0 < n) {
+#pragma unroll
+          for (int k = k0; k < k0 + 8; ++k) y += static_cast<double>(a[k]) * x[4 * k + phase];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      y = quad_sum(y);
+    } else if (row_ok) {
+      for (int j = 0; j < n; ++j)
+        y += static_cast<double>(static_cast<float>(A[row + static_cast<size_t>(j) * ld])) * x[j];
+    }
+    if (c != 0.0 && row_ok) y = c * x[row] - y;
+    double q = lead ? x[row] * y : 0.0;
+    double yy = lead ? y * y : 0.0;
+    block_sum2(q, yy, red);
+    if (it < kPowerIterationsMax) {
+      lmax_prev = lmax;
+      lmax = q;
+    } else {
+      mu_prev = mu;
+      mu = q;
+    }
+    const double inv = yy > 0.0 ? 1.0 / sqrt(yy) : 0.0;
+    __syncthreads();  // every thread has read x
+    if (lead) x[row] = y * inv;
+    __syncthreads();
+  }
   const double lmin = c - mu;
 
   if (tid == 0) {
@@ -366,7 +417,10 @@ __global__ void mlp_adam_kernel(float* __restrict__ params, const float* __restr
 hipError_t launch_features(const Segment* segs, uint32_t count, const double* a64,
                            float* features, hipStream_t stream) {
   if (count == 0) return hipSuccess;
-  features_kernel<<<count, kFeatThreads, 0, stream>>>(segs, a64, features);
+  features_kernel<true><<<count, kFeatThreads, 0, stream>>>(segs, a64, features);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  features_kernel<false><<<count, kFeatThreads, 0, stream>>>(segs, a64, features);
   return hipGetLastError();
 }
 

@@ -6,9 +6,10 @@ cross-entropy, Adam).  This module restates the HIP library's definitions in
 numpy so that the GPU kernels can be checked:
 
 * ``features``  -- the 16 per-matrix features of the features kernel
-                   (power iteration: 16 steps on A from the normalised ones
-                   vector; 24 steps on (c I - A), c = 1.0001 lambda_max, from
-                   a fixed hash start vector);
+                   (power iteration on A rounded to float32, in binary64:
+                   16 steps on A from the normalised ones vector; 24 steps on
+                   (c I - A), c = 1.0001 lambda_max, from a fixed hash start
+                   vector; every other feature from A in binary64);
 * ``forward``   -- logits and hidden activations;
 * ``backward``  -- the gradient of the masked BCE loss;
 * ``adam``      -- one bias-corrected Adam step.
@@ -71,10 +72,11 @@ def features(A):
     off = fro - float(np.sum(d * d))
     tr = float(d.sum())
     t = tr / n
-    lmax, lmax_prev = _power(A, np.full(n, 1.0 / np.sqrt(n)), 0.0, POWER_MAX)
+    A32 = A.astype(np.float32).astype(np.float64)
+    lmax, lmax_prev = _power(A32, np.full(n, 1.0 / np.sqrt(n)), 0.0, POWER_MAX)
     c = 1.0001 * max(lmax, 1e-300)
     s = start_vector(n)
-    mu, mu_prev = _power(A, s / np.sqrt(s @ s), c, POWER_MIN)
+    mu, mu_prev = _power(A32, s / np.sqrt(s @ s), c, POWER_MIN)
     lmin = c - mu
     lmax_c = max(lmax, 1e-300)
     lmin_c = max(lmin, lmax_c * 1e-16)

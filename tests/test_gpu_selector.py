@@ -33,7 +33,8 @@ def test_features_match_oracle(runtime):
     got = host(F).reshape(len(o), capi.MLP_INPUTS)
     for A, g in zip(mats, got):
         want = M.features(A)
-        # binary64 in both, different summation orders; float32 output.
+        # Same arithmetic (binary64; power steps on A rounded to float32),
+        # different summation orders; float32 output.
         plain = [k for k in range(M.INPUTS) if k not in (13, 14)]
         np.testing.assert_allclose(g[plain], want[plain], rtol=2e-6, atol=2e-6)
         # 13, 14: relative change of the last power step, a difference of two

@@ -93,9 +93,13 @@ def test_features_bound_the_spectrum_of_random_spd_matrices():
         t = np.trace(A) / n
         lmax_est = t * 10 ** f[3]
         lmin_est = t * 10 ** f[4]
-        # Rayleigh quotients: lambda_max estimate from below, lambda_min from above.
-        assert lmax_est <= lam.max() * (1 + 1e-12) and lmax_est >= 0.5 * lam.max()
-        assert lmin_est >= lam.min() * (1 - 1e-9)
+        # Rayleigh quotients of the float32-rounded matrix the power steps
+        # use: lambda_max estimate from below, lambda_min from above.
+        lam32 = np.linalg.eigvalsh(A.astype(np.float32).astype(np.float64))
+        assert lmax_est <= lam32.max() * (1 + 1e-12) and lmax_est >= 0.5 * lam32.max()
+        assert lmin_est >= lam32.min() * (1 - 1e-9)
+        # ... which are the spectrum of A to float32 rounding.
+        assert abs(lam32.max() - lam.max()) <= 2.0**-23 * np.linalg.norm(A)
         # Gershgorin discs contain the spectrum.
         assert f[5] * lmax_est <= lam.min() + 1e-12
         assert lmax_est * 10 ** f[6] >= lam.max() - 1e-12
