@@ -741,9 +741,18 @@ msq::GemmArgs product_args(const Operand& a, const Operand& b, void* c, const fl
 
 namespace {
 
+// 32 x 32 tile pairs {(ti, tj), (tj, ti)}, ti <= tj, of a pitch.
+uint32_t tile_pairs(uint32_t ld) {
+  const uint32_t t = ld / 32;
+  return t * (t + 1) / 2;
+}
+
+// Segments of the members, in member order; `reserved` counts the tile pairs
+// of the segments before each one (the tile-pair passes' workgroup map).
 std::vector<msq::Segment> segments_of(const Layout& L, const std::vector<uint32_t>& members) {
   std::vector<msq::Segment> segs;
   segs.reserve(members.size());
+  uint32_t pairs = 0;
   for (uint32_t i : members) {
     msq::Segment g{};
     g.offset = L.off[i];
@@ -751,9 +760,20 @@ std::vector<msq::Segment> segments_of(const Layout& L, const std::vector<uint32_
     g.order = L.order[i];
     g.ld = L.ld[i];
     g.index = i;
+    g.reserved = pairs;
+    pairs += tile_pairs(L.ld[i]);
     segs.push_back(g);
   }
   return segs;
+}
+
+uint32_t pair_count(const std::vector<msq::Segment>& segs) {
+  return segs.empty() ? 0u : segs.back().reserved + tile_pairs(segs.back().ld);
+}
+uint32_t pair_count(const Layout& L, const std::vector<uint32_t>& members) {
+  uint32_t pairs = 0;
+  for (uint32_t i : members) pairs += tile_pairs(L.ld[i]);
+  return pairs;
 }
 
 std::vector<uint32_t> iota_members(uint32_t count) {
@@ -856,7 +876,8 @@ int reconstruction_product(msq_runtime* rt, int tier, const Layout& L,
                            const GemmPlan& plan, const float* V, const float* roots,
                            const int* active, CorrectionBuffers& cb) {
   const uint32_t count = static_cast<uint32_t>(L.order.size());
-  HIP_TRY(msq::launch_scale_rows_transpose(dsegs, static_cast<uint32_t>(members.size()), L.max_ld,
+  HIP_TRY(msq::launch_scale_rows_transpose(dsegs, static_cast<uint32_t>(members.size()),
+                                           pair_count(L, members),
                                            V, roots, 1, cb.t1, rt->stream));
   Operand og;
   MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, cb.t1, cb.enc_a, og));
@@ -1395,7 +1416,7 @@ int msq_correction(msq_runtime* rt, int precision, uint32_t count, const uint32_
   MSQ_TRY(S.commit());
   MSQ_TRY(upload(rt, dsegs, segs));
   MSQ_TRY(plan_upload(rt, plan));
-  HIP_TRY(msq::launch_transpose(dsegs, count, L.max_ld, vectors, vt, rt->stream));
+  HIP_TRY(msq::launch_transpose(dsegs, count, pair_count(segs), vectors, vt, rt->stream));
   Operand ov, ovt;
   MSQ_TRY(encode_operand(rt, precision, L, members, dsegs, vectors, ev, ov));
   MSQ_TRY(encode_operand(rt, precision, L, members, dsegs, vt, evt, ovt));
@@ -1609,10 +1630,11 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
       MSQ_TRY(reconstruction_product(rt, g.basis, L, g.members, g.dsegs, g.plan, V, sroot, active,
                                      cb));
       HIP_TRY(msq::launch_symmetrize_to_f64(g.dsegs, static_cast<uint32_t>(g.members.size()),
-                                            L.max_ld, cb.t2, roots_out, active, rt->stream));
+                                            pair_count(g.segs), cb.t2, roots_out, active,
+                                            rt->stream));
     }
     // 3. Frozen factor in each correction tier: V and V^T.
-    HIP_TRY(msq::launch_transpose(dsegs_all, count, L.max_ld, V, Vt, rt->stream));
+    HIP_TRY(msq::launch_transpose(dsegs_all, count, pair_count(segs_all), V, Vt, rt->stream));
     for (CorrTier& t : tiers) {
       MSQ_TRY(encode_operand(rt, t.tier, L, t.members, t.dsegs, V, t.ev, t.ov));
       MSQ_TRY(encode_operand(rt, t.tier, L, t.members, t.dsegs, Vt, t.evt, t.ovt));
@@ -1628,8 +1650,9 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
           if (c.tier == g.corr) t = &c;
         MSQ_TRY(correction_products(rt, g.corr, L, g.members, g.dsegs, g.plan, t->ov, t->ovt, R32,
                                     Rb, sroot, active, cb));
-        HIP_TRY(msq::launch_add_symmetric(g.dsegs, static_cast<uint32_t>(g.members.size()), L.max_ld,
-                                          cb.t2, roots_out, active, rt->stream));
+        HIP_TRY(msq::launch_add_symmetric(g.dsegs, static_cast<uint32_t>(g.members.size()),
+                                          pair_count(g.segs), cb.t2, roots_out, active,
+                                          rt->stream));
       }
       MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, nullptr, R32, Rb, norms, active,
                              count));

@@ -197,19 +197,22 @@ struct Segment {
   uint32_t order;
   uint32_t ld;
   uint32_t index;     // matrix index: slot in per-matrix arrays (active, sigma, ...)
-  uint32_t reserved;
+  uint32_t reserved;  // tile-pair passes: pairs of the segments before this one
 };
 
-hipError_t launch_symmetrize_to_f64(const Segment* segs, uint32_t count, uint32_t max_ld,
+// Tile-pair passes: one workgroup per pair {(ti, tj), (tj, ti)}, ti <= tj, of
+// 32 x 32 tiles; `pairs` is the total over the list, and each segment's
+// `reserved` field the number of pairs before it (segments in list order).
+hipError_t launch_symmetrize_to_f64(const Segment* segs, uint32_t count, uint32_t pairs,
                                     const float* c, double* x, const int* active,
                                     hipStream_t stream);
-hipError_t launch_add_symmetric(const Segment* segs, uint32_t count, uint32_t max_ld,
+hipError_t launch_add_symmetric(const Segment* segs, uint32_t count, uint32_t pairs,
                                 const float* e, double* x, const int* active,
                                 hipStream_t stream);
-hipError_t launch_scale_rows_transpose(const Segment* segs, uint32_t count, uint32_t max_ld,
+hipError_t launch_scale_rows_transpose(const Segment* segs, uint32_t count, uint32_t pairs,
                                        const float* v, const float* roots, int use_sqrt,
                                        float* out, hipStream_t stream);
-hipError_t launch_transpose(const Segment* segs, uint32_t count, uint32_t max_ld,
+hipError_t launch_transpose(const Segment* segs, uint32_t count, uint32_t pairs,
                             const float* v, float* out, hipStream_t stream);
 hipError_t launch_zero_f64(double* p, size_t n, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t value, hipStream_t stream);

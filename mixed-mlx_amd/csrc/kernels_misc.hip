@@ -23,58 +23,147 @@ constexpr int kT = 256;
   for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;       \
        e < seg##_n; e += static_cast<size_t>(gridDim.x) * blockDim.x)
 
-__global__ void symmetrize_to_f64_kernel(const Segment* __restrict__ segs,
-                                         const float* __restrict__ c,
-                                         double* __restrict__ x,
-                                         const int* __restrict__ active) {
-  if (active && !active[segs[blockIdx.y].index]) return;
-  SEGMENT_LOOP(sg, e) {
-    const int i = static_cast<int>(e % sg.ld);
-    const int j = static_cast<int>(e / sg.ld);
-    const double a = c[sg.offset + e];
-    const double b = c[sg.offset + j + static_cast<size_t>(i) * sg.ld];
-    x[sg.offset + e] = 0.5 * (a + b);
+// ------------------------------------------------------- tile-pair kernels
+// Transposing and symmetrising passes work on pairs of 32 x 32 tiles
+// {(ti, tj), (tj, ti)}, ti <= tj, so that both tiles are read and written
+// along columns (coalesced) through LDS.  One workgroup per pair over the
+// whole batch: Segment::reserved holds the number of pairs of the segments
+// before it in the list (filled by the host), and a workgroup finds its
+// segment by binary search.
+constexpr int kTile = 32;
+constexpr int kTilePitch = kTile + 1;
+
+struct PairTile {
+  Segment sg;
+  int ti, tj;  // tile row / column of the first tile, ti <= tj
+};
+
+__device__ __forceinline__ PairTile pair_tile(const Segment* __restrict__ segs, uint32_t count) {
+  const uint32_t b = blockIdx.x;
+  uint32_t lo = 0, hi = count;  // last segment with reserved <= b
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (segs[mid].reserved <= b) lo = mid; else hi = mid;
+  }
+  PairTile pt;
+  pt.sg = segs[lo];
+  // Pairs of a segment enumerate ti <= tj column by column: column tj holds
+  // tj + 1 pairs.
+  const uint32_t r = b - pt.sg.reserved;
+  int tj = static_cast<int>((sqrtf(8.0f * static_cast<float>(r) + 1.0f) - 1.0f) * 0.5f);
+  while (tj > 0 && static_cast<uint32_t>(tj) * (tj + 1) / 2 > r) --tj;
+  while (static_cast<uint32_t>(tj + 1) * (tj + 2) / 2 <= r) ++tj;
+  pt.tj = tj;
+  pt.ti = static_cast<int>(r - static_cast<uint32_t>(tj) * (tj + 1) / 2);
+  return pt;
+}
+
+// P <- src[tile (ti, tj)], Q <- src[tile (tj, ti)] (row i of a tile at i,
+// column j at j * kTilePitch).
+__device__ __forceinline__ void load_pair(const float* __restrict__ src, const PairTile& pt,
+                                          float* P, float* Q) {
+  const size_t ld = pt.sg.ld;
+  const float* m = src + pt.sg.offset;
+  for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+    const int i = e % kTile, j = e / kTile;
+    P[i + kTilePitch * j] =
+        m[(kTile * pt.ti + i) + static_cast<size_t>(kTile * pt.tj + j) * ld];
+    Q[i + kTilePitch * j] =
+        m[(kTile * pt.tj + i) + static_cast<size_t>(kTile * pt.ti + j) * ld];
+  }
+  __syncthreads();
+}
+
+// x = (c + c^T) / 2 in binary64 (both tiles of the pair).
+__global__ __launch_bounds__(256) void symmetrize_to_f64_kernel(const Segment* __restrict__ segs,
+                                                                uint32_t count,
+                                                                const float* __restrict__ c,
+                                                                double* __restrict__ x,
+                                                                const int* __restrict__ active) {
+  __shared__ float P[kTile * kTilePitch], Q[kTile * kTilePitch];
+  const PairTile pt = pair_tile(segs, count);
+  if (active && !active[pt.sg.index]) return;
+  load_pair(c, pt, P, Q);
+  const size_t ld = pt.sg.ld;
+  double* xm = x + pt.sg.offset;
+  for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+    const int i = e % kTile, j = e / kTile;
+    const double s = 0.5 * (static_cast<double>(P[i + kTilePitch * j]) +
+                            static_cast<double>(Q[j + kTilePitch * i]));
+    xm[(kTile * pt.ti + i) + static_cast<size_t>(kTile * pt.tj + j) * ld] = s;
+    if (pt.ti != pt.tj) {
+      const double t = 0.5 * (static_cast<double>(Q[i + kTilePitch * j]) +
+                              static_cast<double>(P[j + kTilePitch * i]));
+      xm[(kTile * pt.tj + i) + static_cast<size_t>(kTile * pt.ti + j) * ld] = t;
+    }
   }
 }
 
-__global__ void add_symmetric_kernel(const Segment* __restrict__ segs,
-                                     const float* __restrict__ ecorr,
-                                     double* __restrict__ x,
-                                     const int* __restrict__ active) {
-  if (active && !active[segs[blockIdx.y].index]) return;
-  SEGMENT_LOOP(sg, e) {
-    const int i = static_cast<int>(e % sg.ld);
-    const int j = static_cast<int>(e / sg.ld);
-    const double a = ecorr[sg.offset + e];
-    const double b = ecorr[sg.offset + j + static_cast<size_t>(i) * sg.ld];
-    x[sg.offset + e] += 0.5 * (a + b);
+// x += (e + e^T) / 2.
+__global__ __launch_bounds__(256) void add_symmetric_kernel(const Segment* __restrict__ segs,
+                                                            uint32_t count,
+                                                            const float* __restrict__ ecorr,
+                                                            double* __restrict__ x,
+                                                            const int* __restrict__ active) {
+  __shared__ float P[kTile * kTilePitch], Q[kTile * kTilePitch];
+  const PairTile pt = pair_tile(segs, count);
+  if (active && !active[pt.sg.index]) return;
+  load_pair(ecorr, pt, P, Q);
+  const size_t ld = pt.sg.ld;
+  double* xm = x + pt.sg.offset;
+  for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+    const int i = e % kTile, j = e / kTile;
+    xm[(kTile * pt.ti + i) + static_cast<size_t>(kTile * pt.tj + j) * ld] +=
+        0.5 * (static_cast<double>(P[i + kTilePitch * j]) +
+               static_cast<double>(Q[j + kTilePitch * i]));
+    if (pt.ti != pt.tj)
+      xm[(kTile * pt.tj + i) + static_cast<size_t>(kTile * pt.ti + j) * ld] +=
+          0.5 * (static_cast<double>(Q[i + kTilePitch * j]) +
+                 static_cast<double>(P[j + kTilePitch * i]));
   }
 }
 
 // out[k + m ld] = V[m][k] * f(s_k): the K-contiguous storage of G = V diag(f(s)),
 // f = sqrt (use_sqrt) or identity.  Entries outside the order are zero.
-__global__ void scale_rows_transpose_kernel(const Segment* __restrict__ segs,
-                                            const float* __restrict__ v,
-                                            const float* __restrict__ roots, int use_sqrt,
-                                            float* __restrict__ out) {
-  SEGMENT_LOOP(sg, e) {
-    const int k = static_cast<int>(e % sg.ld);
-    const int m = static_cast<int>(e / sg.ld);
-    float val = 0.0f;
-    if (k < static_cast<int>(sg.order) && m < static_cast<int>(sg.order)) {
-      const float s = roots[sg.vec_offset + k];
-      val = v[sg.offset + m + static_cast<size_t>(k) * sg.ld] * (use_sqrt ? sqrtf(s) : s);
+__global__ __launch_bounds__(256) void scale_rows_transpose_kernel(
+    const Segment* __restrict__ segs, uint32_t count, const float* __restrict__ v,
+    const float* __restrict__ roots, int use_sqrt, float* __restrict__ out) {
+  __shared__ float P[kTile * kTilePitch], Q[kTile * kTilePitch];
+  const PairTile pt = pair_tile(segs, count);
+  load_pair(v, pt, P, Q);
+  const size_t ld = pt.sg.ld;
+  const int n = static_cast<int>(pt.sg.order);
+  float* om = out + pt.sg.offset;
+  auto put = [&](int tk, int tm, const float* T) {  // out tile (tk, tm) from V^T
+    for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+      const int kk = kTile * tk + e % kTile, mm = kTile * tm + e / kTile;
+      float val = 0.0f;
+      if (kk < n && mm < n) {
+        const float s = roots[pt.sg.vec_offset + kk];
+        val = T[(e / kTile) + kTilePitch * (e % kTile)] * (use_sqrt ? sqrtf(s) : s);
+      }
+      om[kk + static_cast<size_t>(mm) * ld] = val;
     }
-    out[sg.offset + e] = val;
-  }
+  };
+  put(pt.ti, pt.tj, Q);  // out tile (ti, tj) = V tile (tj, ti)^T
+  if (pt.ti != pt.tj) put(pt.tj, pt.ti, P);
 }
 
-__global__ void transpose_kernel(const Segment* __restrict__ segs, const float* __restrict__ v,
-                                 float* __restrict__ out) {
-  SEGMENT_LOOP(sg, e) {
-    const int i = static_cast<int>(e % sg.ld);
-    const int j = static_cast<int>(e / sg.ld);
-    out[sg.offset + e] = v[sg.offset + j + static_cast<size_t>(i) * sg.ld];
+__global__ __launch_bounds__(256) void transpose_kernel(const Segment* __restrict__ segs,
+                                                        uint32_t count,
+                                                        const float* __restrict__ v,
+                                                        float* __restrict__ out) {
+  __shared__ float P[kTile * kTilePitch], Q[kTile * kTilePitch];
+  const PairTile pt = pair_tile(segs, count);
+  load_pair(v, pt, P, Q);
+  const size_t ld = pt.sg.ld;
+  float* om = out + pt.sg.offset;
+  for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+    const int i = e % kTile, j = e / kTile;
+    om[(kTile * pt.ti + i) + static_cast<size_t>(kTile * pt.tj + j) * ld] = Q[j + kTilePitch * i];
+    if (pt.ti != pt.tj)
+      om[(kTile * pt.tj + i) + static_cast<size_t>(kTile * pt.ti + j) * ld] =
+          P[j + kTilePitch * i];
   }
 }
 
@@ -245,32 +334,31 @@ int seg_grid_x(uint32_t max_ld) {
 
 }  // namespace
 
-hipError_t launch_symmetrize_to_f64(const Segment* segs, uint32_t count, uint32_t max_ld,
+hipError_t launch_symmetrize_to_f64(const Segment* segs, uint32_t count, uint32_t pairs,
                                     const float* c, double* x, const int* active,
                                     hipStream_t stream) {
-  if (!count) return hipSuccess;
-  symmetrize_to_f64_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(segs, c, x, active);
+  if (!count || !pairs) return hipSuccess;
+  symmetrize_to_f64_kernel<<<pairs, kT, 0, stream>>>(segs, count, c, x, active);
   return hipGetLastError();
 }
-hipError_t launch_add_symmetric(const Segment* segs, uint32_t count, uint32_t max_ld,
+hipError_t launch_add_symmetric(const Segment* segs, uint32_t count, uint32_t pairs,
                                 const float* e, double* x, const int* active,
                                 hipStream_t stream) {
-  if (!count) return hipSuccess;
-  add_symmetric_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(segs, e, x, active);
+  if (!count || !pairs) return hipSuccess;
+  add_symmetric_kernel<<<pairs, kT, 0, stream>>>(segs, count, e, x, active);
   return hipGetLastError();
 }
-hipError_t launch_scale_rows_transpose(const Segment* segs, uint32_t count, uint32_t max_ld,
+hipError_t launch_scale_rows_transpose(const Segment* segs, uint32_t count, uint32_t pairs,
                                        const float* v, const float* roots, int use_sqrt,
                                        float* out, hipStream_t stream) {
-  if (!count) return hipSuccess;
-  scale_rows_transpose_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(
-      segs, v, roots, use_sqrt, out);
+  if (!count || !pairs) return hipSuccess;
+  scale_rows_transpose_kernel<<<pairs, kT, 0, stream>>>(segs, count, v, roots, use_sqrt, out);
   return hipGetLastError();
 }
-hipError_t launch_transpose(const Segment* segs, uint32_t count, uint32_t max_ld,
-                            const float* v, float* out, hipStream_t stream) {
-  if (!count) return hipSuccess;
-  transpose_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(segs, v, out);
+hipError_t launch_transpose(const Segment* segs, uint32_t count, uint32_t pairs, const float* v,
+                            float* out, hipStream_t stream) {
+  if (!count || !pairs) return hipSuccess;
+  transpose_kernel<<<pairs, kT, 0, stream>>>(segs, count, v, out);
   return hipGetLastError();
 }
 hipError_t launch_zero_f64(double* p, size_t n, hipStream_t stream) {

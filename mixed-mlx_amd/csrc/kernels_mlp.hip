@@ -27,23 +27,29 @@ constexpr int kW1 = 0, kB1 = kW1 + kHid * kIn, kW2 = kB1 + kHid, kB2 = kW2 + kHi
 static_assert(kParams == kMlpParams, "parameter layout");
 
 // ------------------------------------------------------------------ features
-// One workgroup of 1024 threads per matrix.  Pass 1 reads A (binary64) once
-// for the diagonal, Gershgorin, dominance and norm features.  The power
-// iterations run on A rounded to float (an A-perturbation of 2^-24 relative,
-// far below what the features resolve) with binary64 accumulation:
-//  * order <= 256: A lives in registers, four lanes per row (lane p of row i
-//    holds columns 4 k + p), so the 40 iterations never touch memory again;
-//  * order 257..512: one row per thread, A re-read from memory each step.
-constexpr int kFeatThreads = 1024;
-constexpr int kFeatRegOrder = 256;               // register path up to this order
-constexpr int kFeatCols = kFeatRegOrder / 4;     // register values per lane
+// One workgroup per matrix.  Pass 1 reads A (binary64) once for the
+// diagonal, Gershgorin, dominance and norm features.  The power iterations
+// run on A rounded to float (an A-perturbation of 2^-24 relative, far below
+// what the features resolve) with binary64 accumulation:
+//  * orders up to 256: A lives in registers, four lanes per row (lane p of
+//    row i holds columns 4 k + p), so the 40 steps never touch memory again;
+//    one instantiation per order range (1..64, 65..128, 129..256) with
+//    4 ROWS lanes, so small matrices take small workgroups;
+//  * orders 257..512: one row per lane, A re-read from memory each step.
+// Each instantiation runs over the whole batch; workgroups of matrices
+// outside its range return at once.
+constexpr int kFeatGlobalThreads = 512;
+
+__host__ __device__ constexpr int feat_threads(int rows) {
+  return rows ? 4 * rows : kFeatGlobalThreads;
+}
 
 // Block-wide reductions in a fixed order (a butterfly within each wave, then
-// a butterfly over the 16 wave partials, which every lane reads back): the
+// a butterfly over the NW wave partials, which every lane reads back): the
 // results do not depend on timing, and few values are live at a time.
-constexpr int kFeatWaves = kFeatThreads / 64;
-static_assert(kFeatWaves == 16, "the partial butterfly spans 16 lanes");
+template <int NW>
 __device__ void block_sum2(double& a, double& b, double* red) {
+  static_assert(NW >= 1 && NW <= 64 && (NW & (NW - 1)) == 0, "waves: a power of two");
   for (int off = 32; off > 0; off >>= 1) {
     a += __shfl_xor(a, off);
     b += __shfl_xor(b, off);
@@ -55,30 +61,33 @@ __device__ void block_sum2(double& a, double& b, double* red) {
     red[2 * wave + 1] = b;
   }
   __syncthreads();
-  const int w = lane & (kFeatWaves - 1);
+  const int w = lane & (NW - 1);
   a = red[2 * w];
   b = red[2 * w + 1];
-  for (int off = kFeatWaves / 2; off > 0; off >>= 1) {
+  for (int off = NW / 2; off > 0; off >>= 1) {
     a += __shfl_xor(a, off);
     b += __shfl_xor(b, off);
   }
 }
+template <int NW>
 __device__ double block_sum(double v, double* red) {
   double z = 0.0;
-  block_sum2(v, z, red);
+  block_sum2<NW>(v, z, red);
   return v;
 }
+template <int NW>
 __device__ double block_min(double v, double* red) {
   for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) red[wave] = v;
   __syncthreads();
-  v = red[lane & (kFeatWaves - 1)];
-  for (int off = kFeatWaves / 2; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+  v = red[lane & (NW - 1)];
+  for (int off = NW / 2; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
   return v;
 }
-__device__ double block_max(double v, double* red) { return -block_min(-v, red); }
+template <int NW>
+__device__ double block_max(double v, double* red) { return -block_min<NW>(-v, red); }
 // Sum over the four lanes of a row (lanes 4 r .. 4 r + 3).
 __device__ __forceinline__ double quad_sum(double v) {
   v += __shfl_xor(v, 1);
@@ -93,37 +102,38 @@ __device__ __forceinline__ double start_vector(int j) {
   return static_cast<double>((h >> 16) & 0xffffu) / 65536.0 - 0.5;
 }
 
-// A binary64 column-major (pitch ld), order <= 512.  REGS: the register
-// path; each instantiation runs over the whole batch and its workgroups of
-// the other path's orders return at once.
-template <bool REGS>
-__global__ __launch_bounds__(kFeatThreads) void features_kernel(const Segment* __restrict__ segs,
-                                                                const double* __restrict__ a64,
-                                                                float* __restrict__ features) {
-  __shared__ double x[kMaxFeatureOrder];
-  __shared__ double red[2 * kFeatThreads / 64];
+// A binary64 column-major (pitch ld), order <= 512.  ROWS: orders
+// (ROWS / 2, ROWS] on the register path, or 0 for orders 257..512.
+template <int ROWS>
+__global__ __launch_bounds__(feat_threads(ROWS)) void features_kernel(
+    const Segment* __restrict__ segs, const double* __restrict__ a64,
+    float* __restrict__ features) {
+  constexpr bool REGS = ROWS != 0;
+  constexpr int T = feat_threads(ROWS);
+  constexpr int NW = T / 64;
+  constexpr int COLS = REGS ? ROWS / 4 : 1;  // register values per lane
+  __shared__ double x[kMaxFeatureOrder];  // power-step vector
+  __shared__ double red[2 * NW];
   const Segment sg = segs[blockIdx.x];
   const int n = static_cast<int>(sg.order);
-  if (REGS != (n <= kFeatRegOrder)) return;
+  if (REGS ? (n > ROWS || (ROWS > 64 && n <= ROWS / 2)) : n <= 256) return;
   const int ld = static_cast<int>(sg.ld);
   const double* A = a64 + sg.offset;
   const int tid = threadIdx.x;
-  constexpr bool regs = REGS;
   // Row and column phase of this lane: register path (row tid / 4, columns
   // 4 k + tid % 4), else row tid over all columns.
-  const int row = regs ? tid >> 2 : tid;
-  const int phase = regs ? tid & 3 : 0;
-  const int stride = regs ? 4 : 1;
+  const int row = REGS ? tid >> 2 : tid;
+  const int phase = REGS ? tid & 3 : 0;
   const bool row_ok = row < n;
 
   // Pass 1: diagonal, off-diagonal absolute row sums, Frobenius norms.
-  float a[REGS ? kFeatCols : 1];
+  float a[COLS];
   double diag = 0.0, rsum = 0.0, fro = 0.0, off = 0.0;
   if constexpr (REGS) {
     // Chunks of 8 columns, kept apart (sched_barrier) so that the loads in
-    // flight stay within the register budget of 1024-lane workgroups.
+    // flight stay within the register budget.
 #pragma unroll
-    for (int k0 = 0; k0 < kFeatCols; k0 += 8) {
+    for (int k0 = 0; k0 < COLS; k0 += 8) {
 #pragma unroll
       for (int k = k0; k < k0 + 8; ++k) {
         // Unconditional in-bounds load (clamped indices), then a select; the
@@ -177,15 +187,15 @@ __global__ __launch_bounds__(kFeatThreads) void features_kernel(const Segment* _
     domsum = dom;
     weak = d < rsum ? 1.0 : 0.0;
   }
-  block_sum2(tr, fro, red);
-  block_sum2(off, logd, red);
-  block_sum2(logd2, domsum, red);
-  weak = block_sum(weak, red);
-  dmin = block_min(dmin, red);
-  dmax = block_max(dmax, red);
-  gl = block_min(gl, red);
-  gh = block_max(gh, red);
-  dommin = block_min(dommin, red);
+  block_sum2<NW>(tr, fro, red);
+  block_sum2<NW>(off, logd, red);
+  block_sum2<NW>(logd2, domsum, red);
+  weak = block_sum<NW>(weak, red);
+  dmin = block_min<NW>(dmin, red);
+  dmax = block_max<NW>(dmax, red);
+  gl = block_min<NW>(gl, red);
+  gh = block_max<NW>(gh, red);
+  dommin = block_min<NW>(dommin, red);
 
   // Power iterations: kPowerIterationsMax steps of x <- y / ||y|| with
   // y = A x from x = 1 / sqrt(n) (lambda_max), then kPowerIterationsMin steps
@@ -194,44 +204,51 @@ __global__ __launch_bounds__(kFeatThreads) void features_kernel(const Segment* _
   // each phase are kept.  x beyond the order stays zero (the register path
   // reads columns up to the next multiple of 4).
   const double inv_sqrt_n = 1.0 / sqrt(static_cast<double>(n));
-  for (int i = tid; i < kMaxFeatureOrder; i += kFeatThreads) x[i] = i < n ? inv_sqrt_n : 0.0;
+  for (int i = tid; i < kMaxFeatureOrder; i += T) x[i] = i < n ? inv_sqrt_n : 0.0;
   __syncthreads();
   double lmax = 0.0, lmax_prev = 0.0, mu = 0.0, mu_prev = 0.0, c = 0.0;
   const int total_iters = kPowerIterationsMax + kPowerIterationsMin;
 #pragma unroll 1
   for (int it = 0; it < total_iters; ++it) {
+    // Every read of x in a step precedes the barriers of its reduction, so
+    // the new x is written in place after it.
+    double* xc = x;
     if (it == kPowerIterationsMax) {
       c = 1.0001 * fmax(lmax, 1e-300);
       double snorm = 0.0;
-      for (int i = tid; i < n; i += kFeatThreads) snorm += start_vector(i) * start_vector(i);
-      snorm = block_sum(snorm, red);
-      for (int i = tid; i < n; i += kFeatThreads) x[i] = start_vector(i) / sqrt(snorm);
+      for (int i = tid; i < n; i += T) snorm += start_vector(i) * start_vector(i);
+      snorm = block_sum<NW>(snorm, red);
+      for (int i = tid; i < n; i += T) xc[i] = start_vector(i) / sqrt(snorm);
       __syncthreads();
     }
     double y = 0.0;
     if constexpr (REGS) {
       // The register copy of A is pinned as float for each step (empty asm, no
       // instruction): otherwise its widening to binary64 is hoisted out of the
-      // loop and the doubled copy no longer fits the register file.
+      // loop and the doubled copy no longer fits the register file.  Four
+      // partial sums (columns 16 k0 + 4 u + p, u = k mod 4) shorten the
+      // dependent chain of binary64 FMAs.
 #pragma unroll
-      for (int k = 0; k < kFeatCols; ++k) asm volatile("" : "+v"(a[k]));
+      for (int k = 0; k < COLS; ++k) asm volatile("" : "+v"(a[k]));
+      double ys[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int k0 = 0; k0 < kFeatCols; k0 += 8) {
+      for (int k0 = 0; k0 < COLS; k0 += 8) {
         if (4 * k0 < n) {
 #pragma unroll
-          for (int k = k0; k < k0 + 8; ++k) y += static_cast<double>(a[k]) * x[4 * k + phase];
+          for (int k = k0; k < k0 + 8; ++k)
+            ys[k & 3] += static_cast<double>(a[k]) * xc[4 * k + phase];
         }
         __builtin_amdgcn_sched_barrier(0);
       }
-      y = quad_sum(y);
+      y = quad_sum((ys[0] + ys[1]) + (ys[2] + ys[3]));
     } else if (row_ok) {
       for (int j = 0; j < n; ++j)
-        y += static_cast<double>(static_cast<float>(A[row + static_cast<size_t>(j) * ld])) * x[j];
+        y += static_cast<double>(static_cast<float>(A[row + static_cast<size_t>(j) * ld])) * xc[j];
     }
-    if (c != 0.0 && row_ok) y = c * x[row] - y;
-    double q = lead ? x[row] * y : 0.0;
+    if (c != 0.0 && row_ok) y = c * xc[row] - y;
+    double q = lead ? xc[row] * y : 0.0;
     double yy = lead ? y * y : 0.0;
-    block_sum2(q, yy, red);
+    block_sum2<NW>(q, yy, red);
     if (it < kPowerIterationsMax) {
       lmax_prev = lmax;
       lmax = q;
@@ -240,8 +257,7 @@ __global__ __launch_bounds__(kFeatThreads) void features_kernel(const Segment* _
       mu = q;
     }
     const double inv = yy > 0.0 ? 1.0 / sqrt(yy) : 0.0;
-    __syncthreads();  // every thread has read x
-    if (lead) x[row] = y * inv;
+    if (lead) xc[row] = y * inv;
     __syncthreads();
   }
   const double lmin = c - mu;
@@ -417,10 +433,16 @@ __global__ void mlp_adam_kernel(float* __restrict__ params, const float* __restr
 hipError_t launch_features(const Segment* segs, uint32_t count, const double* a64,
                            float* features, hipStream_t stream) {
   if (count == 0) return hipSuccess;
-  features_kernel<true><<<count, kFeatThreads, 0, stream>>>(segs, a64, features);
+  features_kernel<64><<<count, feat_threads(64), 0, stream>>>(segs, a64, features);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  features_kernel<false><<<count, kFeatThreads, 0, stream>>>(segs, a64, features);
+  features_kernel<128><<<count, feat_threads(128), 0, stream>>>(segs, a64, features);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  features_kernel<256><<<count, feat_threads(256), 0, stream>>>(segs, a64, features);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  features_kernel<0><<<count, feat_threads(0), 0, stream>>>(segs, a64, features);
   return hipGetLastError();
 }
 
