@@ -599,9 +599,11 @@ int tile_kind(const msq_runtime* rt, msq::GemmType type, uint32_t ld) {
   return ld == 32 ? 0 : 1;
 }
 
+// upper: only the tiles on and above the diagonal (tile row <= tile column),
+// for symmetric products whose epilogue mirrors the off-diagonal tiles.
 void build_plan(const Layout& L, const std::vector<uint32_t>& members, bool aux_is_matrix,
                 GemmPlan& P, const msq_runtime* rt = nullptr,
-                msq::GemmType type = msq::GemmType::F32) {
+                msq::GemmType type = msq::GemmType::F32, bool upper = false) {
   P.jobs.clear();
   for (int t = 0; t < 3; ++t) {
     P.tile_job[t].clear();
@@ -620,7 +622,7 @@ void build_plan(const Layout& L, const std::vector<uint32_t>& members, bool aux_
     const uint32_t tile = static_cast<uint32_t>(kTileEdges[t]);
     const uint32_t nt = (L.order[i] + tile - 1) / tile;
     for (uint32_t a = 0; a < nt; ++a)
-      for (uint32_t b = 0; b < nt; ++b) {
+      for (uint32_t b = upper ? a : 0; b < nt; ++b) {
         P.tile_job[t].push_back(jid);
         P.tile_pos[t].push_back((a << 16) | b);
       }
@@ -889,10 +891,14 @@ int reconstruction_product(msq_runtime* rt, int tier, const Layout& L,
 }
 
 // R = A - X X in binary64 with ||R||_F^2 per matrix; R (binary64), R32
-// (float) and Rb (BF16 codes) are each stored when non-null.
+// (float) and Rb (BF16 codes) are each stored when non-null.  symmetric_x:
+// X is exactly symmetric (the solve's iterates are), so (X X)^T = X X; the
+// plan then lists the tiles on and above the diagonal (build_plan upper) and
+// the epilogue stores each off-diagonal tile's mirror, A_ji - (X X)_ij (half
+// the products).
 int residual_product(msq_runtime* rt, const GemmPlan& plan, const double* A, const double* X,
                      double* R, float* R32, uint16_t* Rb, double* norms, const int* active,
-                     uint32_t count) {
+                     uint32_t count, bool symmetric_x = true) {
   HIP_TRY(msq::launch_zero_f64(norms, count, rt->stream));
   msq::GemmArgs g{};
   g.a = X;
@@ -901,6 +907,7 @@ int residual_product(msq_runtime* rt, const GemmPlan& plan, const double* A, con
   g.aux = A;
   g.c_float = R32;
   g.c_bf16 = Rb;
+  g.mirror = symmetric_x ? 1 : 0;
   g.norms = norms;
   g.active = active;
   MSQ_TRY(run_gemm(rt, plan, msq::GemmType::F64, msq::Epilogue::Residual, g));
@@ -1377,7 +1384,7 @@ int msq_residual_fp64(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   MSQ_TRY(upload(rt, dsegs, segs));
   MSQ_TRY(plan_upload(rt, plan));
   MSQ_TRY(residual_product(rt, plan, matrices, root, residual, residual_f32, nullptr, norms, nullptr,
-                           count));
+                           count, false));
   HIP_TRY(msq::launch_zero_f64(anorms, count, rt->stream));
   HIP_TRY(msq::launch_sumsq_f64(dsegs, count, L.max_ld, matrices, anorms, rt->stream));
   std::vector<double> hn(count), ha(count);
@@ -1513,7 +1520,7 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   const std::vector<uint32_t> all = iota_members(count);
   const std::vector<msq::Segment> segs_all = segments_of(L, all);
   GemmPlan plan_res;
-  build_plan(L, all, true, plan_res, rt, msq::GemmType::F64);
+  build_plan(L, all, true, plan_res, rt, msq::GemmType::F64, true);
   struct Group {
     int basis, corr;
     std::vector<uint32_t> members;
@@ -1805,7 +1812,7 @@ int msq_autotune(msq_runtime* rt, uint32_t max_order, uint8_t* table) {
           const uint8_t saved = rt->tiles[ti][tile_class(ld)];
           rt->tiles[ti][tile_class(ld)] = static_cast<uint8_t>(kTileEdges[k]);
           GemmPlan plan;
-          build_plan(L, members, true, plan, rt, type);
+          build_plan(L, members, true, plan, rt, type, type == msq::GemmType::F64);
           rt->tiles[ti][tile_class(ld)] = saved;
           Scratch S{rt};
           plan_want(S, plan);
@@ -1822,6 +1829,7 @@ int msq_autotune(msq_runtime* rt, uint32_t max_order, uint8_t* table) {
           g.a_scales = sa;
           g.b_scales = sb;
           g.alpha = type == msq::GemmType::Fp8 ? alpha : nullptr;
+          g.mirror = type == msq::GemmType::F64 ? 1 : 0;
           status = run_gemm(rt, plan, type, epi, g);  // warm
           float ms = INFINITY;
           for (int r = 0; r < 3 && status == MSQ_OK; ++r) {

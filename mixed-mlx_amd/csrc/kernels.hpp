@@ -182,6 +182,9 @@ struct GemmArgs {
   uint16_t* c_bf16;          // non-null: store C (Residual: R) as BF16 codes
                              // here instead of float / double (RNE, the
                              // codec of msq_quantize)
+  int mirror;                // Residual: the plan lists tiles with tile row <=
+                             // tile column; off-diagonal tiles also store their
+                             // transpose and count twice in the norm
 };
 
 hipError_t launch_gemm(GemmType type, Epilogue epi, int tile, const GemmArgs& args,

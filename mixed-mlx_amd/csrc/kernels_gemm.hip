@@ -224,12 +224,25 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
         if constexpr (EPI == Epilogue::Residual) {
           const double* A = static_cast<const double*>(args.aux) + job.aux_offset;
           const size_t idx = static_cast<size_t>(row) + static_cast<size_t>(col) * ld;
-          const double res = A[idx] - static_cast<double>(acc[i][j][r]);
-          if (args.c) C[idx] = static_cast<Out>(res);
-          if (args.c_float) args.c_float[job.c_offset + idx] = static_cast<float>(res);
-          if (args.c_bf16)
+          const size_t midx = static_cast<size_t>(col) + static_cast<size_t>(row) * ld;
+          const bool mirror = args.mirror && tm != tn;
+          const double xx = static_cast<double>(acc[i][j][r]);
+          const double res = A[idx] - xx;
+          // Mirror entry (col, row): A_ji - (X X)_ij, (X X)_ji = (X X)_ij.
+          const double mres = mirror ? A[midx] - xx : 0.0;
+          if (args.c) {
+            C[idx] = static_cast<Out>(res);
+            if (mirror) C[midx] = static_cast<Out>(mres);
+          }
+          if (args.c_float) {
+            args.c_float[job.c_offset + idx] = static_cast<float>(res);
+            if (mirror) args.c_float[job.c_offset + midx] = static_cast<float>(mres);
+          }
+          if (args.c_bf16) {
             args.c_bf16[job.c_offset + idx] = encode_bf16(static_cast<float>(res));
-          if (row < order && col < order) norm2 += res * res;
+            if (mirror) args.c_bf16[job.c_offset + midx] = encode_bf16(static_cast<float>(mres));
+          }
+          if (row < order && col < order) norm2 += res * res + mres * mres;
         } else {
           Out v = static_cast<Out>(acc[i][j][r]);
           if constexpr (T != GemmType::F64) v *= alpha;
