@@ -16,7 +16,14 @@ Two paths are timed on the same device-resident batch:
     misses the binary64 residual check is re-solved on the plain path inside
     the timed region.  --no-selector: the fixed BF16 tier (rate --rate).
 ``value`` is the low-bit path's matrices/s; every matrix of both paths is
-checked against the FP64 root Q diag(sqrt(lambda)) Q^T.
+checked against the FP64 root Q diag(sqrt(lambda)) Q^T.  With the selector,
+the fixed BF16 tier (rate --rate) is timed as well and reported beside it.
+
+``--workload mixture`` runs the learned-tier workload instead (default 2000
+matrices, n uniform in [32, 512], spectra one third each log-uniform with
+cond up to 1e6, power law k^-alpha with alpha in [0.5, 3], four clusters):
+the same paths, with the pass rate (<= 1e-6 against the known root) and the
+fallback rate of each; no roofline leg.
 
 Also reported: the roofline of the dominant kernel (batched_jacobi_kernel<16>,
 the launch of the orders 241..256 of the batch, run on its own and timed with
@@ -118,7 +125,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--count", type=int, default=4096)
+    ap.add_argument("--count", type=int, default=None,
+                    help="matrices (default 4096; mixture workload 2000)")
+    ap.add_argument("--workload", choices=["uneven", "mixture"], default="uneven")
     ap.add_argument("--tier", default="bf16")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -143,7 +152,15 @@ def main():
     from msq import capi, generators as G, selector as SEL, solver as MS
 
     t = time.perf_counter()
-    mats = G.batched_uneven(count=args.count)
+    if args.workload == "uneven":
+        mats = G.batched_uneven(count=args.count or 4096)
+        workload = ("batched uneven: 4096 SPD matrices, n in {16,...,256}, cond log-uniform "
+                    "in [1e1, 1e5], half clustered spectra")
+    else:
+        mats = list(G.mixture(count=args.count or 2000, config=4, order_range=(32, 512)))
+        workload = (f"learned tier selection: {len(mats)} SPD matrices, n uniform in [32, 512], "
+                    "spectra log-uniform (cond <= 1e6) / power law k^-alpha (alpha in "
+                    "[0.5, 3]) / 4 clusters")
     orders = [m["order"] for m in mats]
     _, aflat = capi.pack_batch([m["A"] for m in mats])
     lds, offs, voffs, total, vtotal = capi.batch_layout(orders)
@@ -164,15 +181,19 @@ def main():
     X = torch.zeros(total, dtype=torch.float64, device=dev)
 
     def check_all(Xdev, label):
+        """(worst relative error, matrices within 1e-6) against the known roots."""
         Xh = Xdev.cpu().numpy()
-        worst = 0.0
+        worst, passed = 0.0, 0
         for m, off, ld in zip(mats, offs, lds):
             n = m["order"]
             Xi = Xh[off:off + ld * ld].reshape(ld, ld, order="F")[:n, :n]
             Xr = m["X"]
-            worst = max(worst, float(np.linalg.norm(Xi - Xr) / np.linalg.norm(Xr)))
-        log(f"{label}: worst relative error vs FP64 root {worst:.3e}")
-        return worst
+            err = float(np.linalg.norm(Xi - Xr) / np.linalg.norm(Xr))
+            worst = max(worst, err)
+            passed += err <= 1e-6
+        log(f"{label}: worst relative error vs FP64 root {worst:.3e}, "
+            f"{passed}/{len(mats)} within 1e-6")
+        return worst, passed
 
     def timed(fn):
         for _ in range(args.warmup):
@@ -193,7 +214,7 @@ def main():
         rt.close()
         return
     dt_plain, rep_plain = timed(lambda: rt.solve_batched(orders, A, X, plain))
-    err_plain = check_all(X, "plain fp32")
+    err_plain, pass_plain = check_all(X, "plain fp32")
     ok_plain = sum(r["status"] == "converged" for r in rep_plain)
     log(f"plain fp32: {dt_plain * 1e3:.1f} ms/step, {len(mats) / dt_plain:.0f} matrices/s, "
         f"converged {ok_plain}/{len(mats)}, mean sweeps "
@@ -209,19 +230,35 @@ def main():
             reports, fb, tiers = SEL.solve_selected(rt, sel, orders, A, X)
             chosen[:] = [tiers]
             return reports, fb
-    else:
-        tier = MS.tier_options(args.tier, rate=args.rate, max_corrections=12)
+    tier = MS.tier_options(args.tier, rate=args.rate, max_corrections=12)
 
-        def low_bit():
-            return MS.solve_with_fallback(rt, orders, A, X, tier)
+    def fixed_tier():
+        return MS.solve_with_fallback(rt, orders, A, X, tier)
+    if not use_selector:
+        low_bit = fixed_tier
+    label = "learned tier" if use_selector else f"fixed {args.tier} tier, rate {args.rate}"
     dt_low, (rep_low, fallback) = timed(low_bit)
-    err_low = check_all(X, f"low-bit {args.tier}")
-    log(f"low-bit {args.tier}: {dt_low * 1e3:.1f} ms/step, {len(mats) / dt_low:.0f} "
+    err_low, pass_low = check_all(X, label)
+    log(f"{label}: {dt_low * 1e3:.1f} ms/step, {len(mats) / dt_low:.0f} "
         f"matrices/s, fallback {len(fallback)}/{len(mats)}, mean sweeps "
         f"{np.mean([r['sweeps'] for r in rep_low]):.2f}")
+    fixed = None
+    if use_selector:
+        # The method without the learned part, on the same batch.
+        dt_fix, (rep_fix, fb_fix) = timed(fixed_tier)
+        err_fix, pass_fix = check_all(X, f"fixed {args.tier} tier")
+        fixed = {"tier": args.tier, "correction_rate": args.rate,
+                 "matrices_per_s": round(len(mats) / dt_fix, 2),
+                 "ms_per_step": round(dt_fix * 1e3, 3), "max_rel_error": err_fix,
+                 "within_1e-6": pass_fix, "fallbacks": len(fb_fix),
+                 "mean_sweeps": float(np.mean([r["sweeps"] for r in rep_fix]))}
+        log(f"fixed {args.tier} tier: {dt_fix * 1e3:.1f} ms/step, "
+            f"{len(mats) / dt_fix:.0f} matrices/s, fallback {len(fb_fix)}/{len(mats)}")
 
-    roofline = roofline_leg(args, mats, orders, plain, rt, stream, torch, np, capi, dev)
-    log("roofline", roofline)
+    roofline = None
+    if args.workload == "uneven":
+        roofline = roofline_leg(args, mats, orders, plain, rt, stream, torch, np, capi, dev)
+        log("roofline", roofline)
 
     cpu = None if args.no_cpu else cpu_baseline(mats, args.cpu_budget)
     if cpu:
@@ -240,23 +277,25 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 jacobi / bf16 mfma / fp64 residual",
         "data": "synthetic (seeded Haar-random SPD matrices)",
-        "config": {"workload": "batched uneven: 4096 SPD matrices, n in {16,...,256}, "
-                               "cond log-uniform in [1e1, 1e5], half clustered spectra",
+        "config": {"workload": workload,
                    "count": len(mats), "tier": args.tier, "correction_rate": args.rate,
                    "hip_graphs": not args.no_graphs,
                    "tile_table": tile_table.tolist() if tile_table is not None else None},
         "plain_fp32": {"matrices_per_s": round(len(mats) / dt_plain, 2),
                        "ms_per_step": round(dt_plain * 1e3, 3),
-                       "max_rel_error": err_plain, "converged": ok_plain},
+                       "max_rel_error": err_plain, "converged": ok_plain,
+                       "within_1e-6": pass_plain},
         "low_bit": {"matrices_per_s": round(len(mats) / dt_low, 2),
                     "selector": ({"tier_counts": dict(zip(
                         [t[0] for t in SEL.TIERS],
                         np.bincount(chosen[0], minlength=len(SEL.TIERS)).tolist()))}
                                  if use_selector else None),
-                    "max_rel_error": err_low, "fallbacks": len(fallback),
+                    "max_rel_error": err_low, "within_1e-6": pass_low,
+                    "fallbacks": len(fallback),
                     "mean_sweeps": float(np.mean([r["sweeps"] for r in rep_low])),
                     "mean_corrections": float(np.mean([r["corrections"] for r in rep_low]))},
         "speedup_vs_plain_fp32": round(dt_plain / dt_low, 3),
+        "fixed_tier": fixed,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -15,10 +15,15 @@ costs time, never accuracy.
 * labels: the solver's own runs of every tier on seeded synthetic batches
   (``label_batch``), success = converged and relative Frobenius error against
   the construction-known root <= 1e-6;
-* decision: the tier of least expected cost c_t + (1 - p_t) c_plain, with c_t
-  the tier's mean cost per matrix in the training runs (Jacobi sweeps plus
-  half a sweep per correction: a correction step of the batched solver costs
-  about half a Jacobi sweep on the same matrix) and p_t the predicted success.
+* decision: the tier of least expected cost c_t + (1 - p_t) F c_plain, with
+  p_t the predicted success and c_t the tier's measured cost: the wall time of
+  the tier's batched solve of each labelled batch divided by the batch's work
+  sum (n / 256)^3, pooled over the batches (milliseconds per unit of work; a
+  matrix of order n costs c_t (n / 256)^3, so the factor cancels in the
+  choice).  A failed choice pays the plain path again, in a second solve call
+  whose latency is that of its slowest matrix rather than the matrix's share
+  of a batch: F = FALLBACK_WEIGHT weights it accordingly, so a low-bit tier is
+  chosen only when its predicted success is high.
 
 Feature standardisation is folded into the first layer when the weights are
 saved, so inference runs the raw features through the network unchanged.
@@ -33,7 +38,7 @@ from . import capi
 from . import solver as MS
 
 WEIGHTS = pathlib.Path(__file__).resolve().parent / "data" / "selector.npz"
-CORRECTION_COST = 0.5  # Jacobi-sweep equivalents per correction step
+FALLBACK_WEIGHT = 10.0
 
 # (name, basis/correction tier, correction_rate); tier 0 is the plain path.
 TIERS = [
@@ -56,21 +61,25 @@ def tier_options(k, max_corrections=12):
     return MS.tier_options(tier, rate=rate, max_corrections=max_corrections)
 
 
-def cost_units(sweeps, corrections):
-    return np.asarray(sweeps, dtype=np.float64) + CORRECTION_COST * np.asarray(corrections)
+def work_units(orders):
+    """(n / 256)^3 per matrix: the batched solve's time scales with n^3."""
+    return (np.asarray(orders, dtype=np.float64) / 256.0) ** 3
 
 
-def policy_cost(choice, success, sweeps, corrections, orders=None):
-    """Total cost of a per-matrix tier choice on labelled runs, with the plain
-    path paid again for every choice that failed; (cost, failures).  With
-    orders, each matrix's units are weighted by (n / 256)^3 (time scales
-    with n^3)."""
+def unit_costs(tier_ms, orders):
+    """Milliseconds per unit of work of each tier on one labelled batch."""
+    return np.asarray(tier_ms, dtype=np.float64) / float(np.sum(work_units(orders)))
+
+
+def policy_cost(choice, success, unit_cost, orders):
+    """Modelled time (ms) of a per-matrix tier choice on a labelled batch: a
+    matrix costs its tier's unit cost times (n / 256)^3, and every choice that
+    failed pays the plain path again; returns (cost, failures)."""
     rows = np.arange(len(choice))
-    cost = cost_units(sweeps, corrections)
-    if orders is not None:
-        cost = cost * (np.asarray(orders, dtype=np.float64)[:, None] / 256.0) ** 3
+    w = work_units(orders)
+    c = np.asarray(unit_cost, dtype=np.float64)
     ok = success[rows, choice] > 0
-    return float(np.sum(cost[rows, choice] + np.where(ok, 0.0, cost[:, 0]))), int(np.sum(~ok))
+    return float(np.sum(w * (c[choice] + np.where(ok, 0.0, c[0])))), int(np.sum(~ok))
 
 
 # -------------------------------------------------------------- labelling
@@ -87,7 +96,12 @@ def true_errors(roots, mats, offs, lds):
 
 def label_batch(rt, mats, target=MS.TARGET, log=None):
     """Runs every tier on the batch.  Returns (features, success, sweeps,
-    corrections, orders): arrays of shape (count, 16), (count, 8) x 3, (count,)."""
+    corrections, orders, tier_ms): arrays of shape (count, 16), (count, 8) x 3,
+    (count,), (8,); tier_ms is the wall time of each tier's batched solve (the
+    median of three runs after a first, so graph capture and workspace growth
+    are excluded)."""
+    import time
+
     import torch
 
     orders = [m["order"] for m in mats]
@@ -101,8 +115,18 @@ def label_batch(rt, mats, target=MS.TARGET, log=None):
     success = np.zeros((count, tiers), dtype=np.float32)
     sweeps = np.zeros((count, tiers))
     corrections = np.zeros((count, tiers))
+    tier_ms = np.zeros(tiers)
     for k in range(tiers):
-        reps = rt.solve_batched(orders, A, X, tier_options(k))
+        opts = tier_options(k)
+        rt.solve_batched(orders, A, X, opts)
+        runs = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reps = rt.solve_batched(orders, A, X, opts)
+            torch.cuda.synchronize()
+            runs.append((time.perf_counter() - t0) * 1e3)
+        tier_ms[k] = float(np.median(runs))
         err = true_errors(X, mats, offs, lds)
         ok = np.array([r["status"] == "converged" for r in reps]) & (err <= target)
         success[:, k] = ok
@@ -110,9 +134,10 @@ def label_batch(rt, mats, target=MS.TARGET, log=None):
         corrections[:, k] = [r["corrections"] for r in reps]
         if log:
             log(f"  tier {TIERS[k][0]}: success {ok.mean():.3f}, mean sweeps "
-                f"{sweeps[:, k].mean():.2f}, mean corrections {corrections[:, k].mean():.2f}")
+                f"{sweeps[:, k].mean():.2f}, mean corrections {corrections[:, k].mean():.2f}, "
+                f"{tier_ms[k]:.1f} ms")
     return (feats.cpu().numpy().reshape(count, capi.MLP_INPUTS), success, sweeps, corrections,
-            np.array(orders))
+            np.array(orders), tier_ms)
 
 
 # ------------------------------------------------------------------ network
@@ -188,7 +213,8 @@ def train(rt, x, y, mask=None, epochs=200, batch=512, lr=3e-3, decay=1e-4, seed=
 
 # ------------------------------------------------------------------ selector
 class Selector:
-    """Trained network (raw-feature weights) and per-tier mean costs."""
+    """Trained network (raw-feature weights) and per-tier unit costs (ms per
+    unit of work, see unit_costs)."""
 
     def __init__(self, params, tier_cost):
         self.params = np.asarray(params, dtype=np.float32)
@@ -218,11 +244,11 @@ class Selector:
         z = logits.cpu().numpy().reshape(count, capi.MLP_OUTPUTS).astype(np.float64)
         return 1.0 / (1.0 + np.exp(-z))
 
-    def choose(self, prob, orders=None):
-        """Tier per matrix minimising c_t + (1 - p_t) c_plain (tier 0: p = 1)."""
+    def choose(self, prob, orders=None, fallback_weight=FALLBACK_WEIGHT):
+        """Tier per matrix minimising c_t + (1 - p_t) F c_plain (tier 0: p = 1)."""
         p = np.array(prob, dtype=np.float64)
         p[:, 0] = 1.0
-        expected = self.tier_cost[None, :] + (1.0 - p) * self.tier_cost[0]
+        expected = self.tier_cost[None, :] + (1.0 - p) * fallback_weight * self.tier_cost[0]
         return np.argmin(expected, axis=1)
 
     def options(self, rt, orders, matrices):

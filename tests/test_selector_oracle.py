@@ -122,12 +122,18 @@ def test_choice_minimises_expected_cost_and_policy_cost():
     sel = SEL.Selector(np.zeros(M.PARAMS, np.float32), cost)
     prob = np.array([[0.0, 1.0, 1.0, 1.0, 1.0, 0.2, 1.0, 1.0],    # r4 sure, r8 unlikely
                      [0.0, 0.3, 0.1, 0.0, 0.0, 0.0, 0.0, 0.0]])   # nothing likely: plain
-    ch = sel.choose(prob)
+    ch = sel.choose(prob, fallback_weight=1.0)
     assert ch.tolist() == [4, 0]
+    # A heavier fallback weight turns away from the uncertain tiers only.
+    assert sel.choose(prob, fallback_weight=10.0).tolist() == [4, 0]
+    prob[0, 4] = 0.95
+    assert sel.choose(prob, fallback_weight=1.0).tolist() == [4, 0]
+    assert sel.choose(prob, fallback_weight=10.0).tolist() == [3, 0]
     success = np.array([[1, 1, 1, 1, 1, 0, 1, 1], [1, 0, 0, 0, 0, 0, 0, 0]], dtype=float)
-    sweeps = np.tile(cost, (2, 1))
-    corr = np.zeros_like(sweeps)
-    total, fails = SEL.policy_cost(np.array([5, 1]), success, sweeps, corr)
-    assert fails == 2 and total == (3 + 10) + (7 + 10)
-    total, fails = SEL.policy_cost(ch, success, sweeps, corr, orders=[256, 128])
+    # Orders 256 and 128: work units 1 and 1/8.
+    total, fails = SEL.policy_cost(np.array([5, 1]), success, cost, [256, 128])
+    assert fails == 2 and total == (3 + 10) + (7 + 10) / 8
+    total, fails = SEL.policy_cost(ch, success, cost, [256, 128])
     assert fails == 0 and total == 4 + 10 / 8
+    # Unit costs: batch time over total work.
+    np.testing.assert_allclose(SEL.unit_costs([9.0, 4.5], [256, 128]), [8.0, 4.0])
