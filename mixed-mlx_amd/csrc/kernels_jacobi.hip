@@ -106,14 +106,35 @@ __host__ __device__ constexpr int group_for_slots(int S) { return S == 8 ? 8 : 1
 // History row stride per part: odd, so the H parts' broadcast reads of one
 // slot fall in distinct LDS banks.
 __host__ __device__ constexpr int padded_slots(int S) { return S | 1; }
-// Upper bound of off-diagonal 2x2 blocks per thread (m (m - 1) / 2 blocks
-// over the launch's threads; checked on the host).
+__host__ __device__ constexpr int threads_for_order(int N) {
+  return N <= 128 ? ((N + (64 / (N / 16)) - 1) / (64 / (N / 16))) * 64 : 4 * N;
+}
+
+// Off-diagonal 2x2 blocks per thread when each thread takes single blocks
+// (m (m - 1) / 2 blocks over the launch's threads; checked on the host).
 __host__ __device__ constexpr int blocks_per_thread(int S) {
   return S == 8 ? 2 : (8 * S - 1 + 15) / 16;
 }
+// Orders whose block phase is bound by LDS throughput rather than by the
+// latency of a round take super-blocks (below); smaller orders take single
+// blocks, which keep the per-wave critical path shortest.  Measured on
+// MI355X (one launch per padded order of a 4096-matrix batch): N = 256 runs
+// 5.5% faster with super-blocks, N = 240 4% slower, N <= 128 about 2x slower.
+__host__ __device__ constexpr bool use_super_blocks(int N) { return N >= 256; }
 
-__host__ __device__ constexpr int threads_for_order(int N) {
-  return N <= 128 ? ((N + (64 / (N / 16)) - 1) / (64 / (N / 16))) * 64 : 4 * N;
+// Large orders handle off-diagonal blocks as 2 x 2 super-blocks of slot pairs:
+// super-block (K, L), K <= L < m / 2, covers slots {2K, 2K + 1} x {2L, 2L + 1}
+// (four blocks for K < L; the single block (2K, 2K + 1) for K = L), so one
+// thread reads four slot records for four blocks.  The super-diagonal ones
+// (K, K + 1) and the two corner diagonal ones hold the blocks that complete
+// the next round's pairs and belong to the pair-phase owners; the others
+// ("items": K + 2 <= L, and K = L for 0 < K < m / 2 - 1) are dealt to all
+// threads.
+__host__ __device__ constexpr int super_items(int N) {
+  return (N / 4 - 1) * (N / 4 - 2) / 2 + (N / 4 - 2);
+}
+__host__ __device__ constexpr int items_per_thread(int N) {
+  return super_items(N) / threads_for_order(N) + 1;
 }
 
 // Lane shifts inside a V row (8 adjacent lanes, inside one 16-lane DPP row):
@@ -311,14 +332,76 @@ __device__ __forceinline__ void rotate_blocks(float* __restrict__ Ap,
   }
 }
 
-// Off-diagonal blocks (k < l) whose rotated entries complete the next round's
-// pairs.  Round t + 1 pairs, at slot k', the participants that round t holds
-// at positions k' - 1 and N - 2 - k' (the round-robin moves every position
-// x >= 1 to x + 1, N - 1 to 1); as slots of round t these are
+// Rotates the blocks of super-block `code` = K | L << 16 in place: all four
+// for K < L, only (2K, 2K + 1) for K = L (the other three load and are
+// dropped).  All loads are issued before any store (the blocks of a round are
+// disjoint, which the compiler cannot know).
+__device__ __forceinline__ void rotate_super(float* __restrict__ Ap,
+                                             const SlotIdx* __restrict__ sidx,
+                                             const float2* __restrict__ rot, uint32_t code) {
+  const int K = static_cast<int>(code & 0xffffu), L = static_cast<int>(code >> 16);
+  const SlotIdx rk[2] = {sidx[2 * K], sidx[2 * K + 1]};
+  const SlotIdx rl[2] = {sidx[2 * L], sidx[2 * L + 1]};
+  const float4 ck = *reinterpret_cast<const float4*>(rot + 2 * K);  // (c, s) of 2K, 2K + 1
+  const float4 cl = *reinterpret_cast<const float4*>(rot + 2 * L);
+  const float2 ak[2] = {make_float2(ck.x, ck.y), make_float2(ck.z, ck.w)};
+  const float2 al[2] = {make_float2(cl.x, cl.y), make_float2(cl.z, cl.w)};
+  int i00[4], i01[4], i10[4], i11[4];
+  Block blk[4];
+  float cc[4], sc[4], cs[4], ss[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const SlotIdx& x = rk[u >> 1];
+    const SlotIdx& y = rl[u & 1];
+    i00[u] = packed_offset(x.p4, x.cbp4, y.p4, y.cbp4);
+    i01[u] = packed_offset(x.p4, x.cbp4, y.q4, y.cbq4);
+    i10[u] = packed_offset(x.q4, x.cbq4, y.p4, y.cbp4);
+    i11[u] = packed_offset(x.q4, x.cbq4, y.q4, y.cbq4);
+    blk[u].a00 = lds_at(Ap, i00[u]);
+    blk[u].a01 = lds_at(Ap, i01[u]);
+    blk[u].a10 = lds_at(Ap, i10[u]);
+    blk[u].a11 = lds_at(Ap, i11[u]);
+    // The element formula's coefficient products: p sees (c, -s) and q
+    // (c, s), so every coefficient is one of c_k c_l, s_k c_l, c_k s_l,
+    // s_k s_l up to an exact sign.
+    const float2 a = ak[u >> 1], b = al[u & 1];
+    cc[u] = a.x * b.x;
+    sc[u] = a.y * b.x;
+    cs[u] = a.x * b.y;
+    ss[u] = a.y * b.y;
+  }
+  const bool diag = K == L;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const Block b = blk[u];
+    // Rows (p_k, q_k), columns (p_l, q_l); A'[r][c] = rr.x cc.x A[r][c] +
+    // rr.y cc.x A[r'][c] + rr.x cc.y A[r][c'] + rr.y cc.y A[r'][c'].
+    const float o00 = cc[u] * b.a00 - sc[u] * b.a10 - cs[u] * b.a01 + ss[u] * b.a11;
+    const float o01 = cc[u] * b.a01 - sc[u] * b.a11 + cs[u] * b.a00 - ss[u] * b.a10;
+    const float o10 = cc[u] * b.a10 + sc[u] * b.a00 - cs[u] * b.a11 - ss[u] * b.a01;
+    const float o11 = cc[u] * b.a11 + sc[u] * b.a01 + cs[u] * b.a10 + ss[u] * b.a00;
+    if (!diag || u == 1) {
+      lds_at(Ap, i00[u]) = o00;
+      lds_at(Ap, i01[u]) = o01;
+      lds_at(Ap, i10[u]) = o10;
+      lds_at(Ap, i11[u]) = o11;
+    }
+  }
+}
+
+// Blocks whose rotated entries complete the next round's pairs.  Round t + 1
+// pairs, at slot k', the participants that round t holds at positions k' - 1
+// and N - 2 - k' (the round-robin moves every position x >= 1 to x + 1,
+// N - 1 to 1); as slots of round t these are
 //   k' = 0:           lo(0), hi(1)        -> block (0, 1)
 //   1 <= k' <= m - 2: lo/hi(k' - 1), hi(k' + 1) -> block (k' - 1, k' + 1)
 //   k' = m - 1:       lo(m - 2), lo(m - 1) -> block (m - 2, m - 1)
-// (for k' = 1 position 0 is fixed and position N - 1 arrives: hi(0)).
+// (for k' = 1 position 0 is fixed and position N - 1 arrives: hi(0)).  With
+// single blocks, thread k' owns next_pair_block(k').  With super-blocks,
+// super-block (j, j + 1) holds the blocks of slots 2j + 1 and 2j + 2, and
+// the corner super-blocks (0, 0) and (m/2 - 1, m/2 - 1) those of slots 0 and
+// m - 1.  Owner thread j < m/2 - 1 takes super-block (j, j + 1); thread
+// m/2 - 1 the corner (0, 0); thread m/2 the corner (m/2 - 1, m/2 - 1).
 __device__ __forceinline__ uint32_t next_pair_block(int kp, int m) {
   const int k = kp == 0 ? 0 : (kp == m - 1 ? m - 2 : kp - 1);
   const int l = kp == 0 ? 1 : (kp == m - 1 ? m - 1 : kp + 1);
@@ -326,6 +409,27 @@ __device__ __forceinline__ uint32_t next_pair_block(int kp, int m) {
 }
 __device__ __forceinline__ bool is_next_pair_block(int k, int l, int m) {
   return l == k + 2 || (k == 0 && l == 1) || (k == m - 2 && l == m - 1);
+}
+__device__ __forceinline__ uint32_t super_code(int K, int L) {
+  return static_cast<uint32_t>(K) | (static_cast<uint32_t>(L) << 16);
+}
+__device__ __forceinline__ uint32_t owner_super(int tid, int m2) {
+  if (tid < m2 - 1) return super_code(tid, tid + 1);
+  if (tid == m2 - 1) return super_code(0, 0);
+  if (tid == m2) return super_code(m2 - 1, m2 - 1);
+  return ~0u;
+}
+// Item r of the dealt super-blocks: (K, L) with K + 2 <= L row-major, then
+// the diagonal ones (K, K), 0 < K < m/2 - 1.
+__device__ __forceinline__ uint32_t item_super(int r, int m2) {
+  const int off = (m2 - 1) * (m2 - 2) / 2;
+  if (r >= off) return super_code(1 + r - off, 1 + r - off);
+  int K = 0;
+  while (r >= m2 - 2 - K) {
+    r -= m2 - 2 - K;
+    ++K;
+  }
+  return super_code(K, K + 2 + r);
 }
 
 // One workgroup per matrix, orders up to 256.
@@ -336,13 +440,14 @@ __device__ __forceinline__ bool is_next_pair_block(int k, int l, int m) {
 // {p, q} x {p, q} (no other block touches it).  Block phase: every
 // off-diagonal block {p_k, q_k} x {p_l, q_l} (k < l) is read, rotated on both
 // sides and written back by one thread; the blocks of a round are disjoint,
-// so there is no second buffer.  Each thread's blocks are fixed for the whole
-// solve.  Inside a group of rounds the pair phase of round t + 1 runs in the
-// block phase of round t: thread k' owns the block that completes round
-// t + 1's pair at slot k' (next_pair_block), rotates it first, then computes
-// the next rotation and rotates the next diagonal block, which no other thread
-// of round t touches.  Slot data is double-buffered by round parity, so a
-// round costs one barrier.
+// so there is no second buffer.  A thread works on 2 x 2 super-blocks of slot
+// pairs, fixed for the whole solve.  Inside a group of rounds the pair phase
+// of round t + 1 runs in the block phase of round t: the owner of the
+// super-block that completes round t + 1's pairs at slots 2j + 1 and 2j + 2
+// (owner_super) rotates it first, then computes those two rotations and
+// rotates their diagonal blocks, which no other thread of round t touches.
+// Slot data is double-buffered by round parity, so a round costs one
+// barrier.
 //
 // V: slot-aligned rows.  Thread (row i, part h) holds V[i][x] for the
 // positions x of slots k in [h S, (h + 1) S): "lo" = position k, "hi" =
@@ -361,6 +466,8 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   constexpr bool VMEM = v_in_memory(S);
   constexpr int G = group_for_slots(S);
   constexpr int SP = padded_slots(S);
+  constexpr bool SUPER = use_super_blocks(NP);
+  constexpr int IPT = items_per_thread(NP);
   constexpr int BPT = blocks_per_thread(S);
   const JacobiJob job = jobs[blockIdx.x];
   const int tid = threadIdx.x;
@@ -375,13 +482,16 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   constexpr int hrow = 8 * SP;
   constexpr int rows_per_wave = 64 / H;
   constexpr int packed = N * (N + 1) / 2;
+  constexpr int m2 = m / 2;  // slot pairs
+  constexpr int items = super_items(NP);
+  constexpr int nthreads = threads_for_order(NP);  // == blockDim.x (host check)
   constexpr int offblocks = m * (m - 1) / 2;
 
   // Static LDS: every array at a compile-time offset (addresses fold into
   // the ds instructions' offset fields).
   __shared__ __align__(16) float Ap[(packed + 3) & ~3];
   __shared__ SlotIdx sidx[2][m];  // round parity x slot
-  __shared__ float2 rot[2][m];     // (c, s), p < q
+  __shared__ __align__(16) float2 rot[2][m];  // (c, s), p < q
   __shared__ float2 hist[G * hrow];  // G rounds x 8 parts x SP of (c, s') for V
   __shared__ double red[32];
   __shared__ float redf[16];
@@ -395,24 +505,40 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   const int row = wave * rows_per_wave + lane / H;
   const bool vlane = lane / H < rows_per_wave && row < N;
 
-  // This thread's off-diagonal blocks, (k | l << 16): first the block that
+  // This thread's super-blocks, K | L << 16: first its owner super-block
+  // (pair-phase owners only), then its dealt items, item r to thread
+  // r mod nthreads: the empty entries (~0) gather in the last threads, so
+  // whole waves skip them (LDS bandwidth is the block phase's bound).
+  //
+  // Single blocks (smaller orders): (k | l << 16), first the block that
   // completes the next round's pair at slot tid (threads below m), then the
-  // other blocks, row-major over k < l, dealt round-robin.
-  uint32_t code[BPT + 1];
-  code[0] = tid < m ? next_pair_block(tid, m) : ~0u;
+  // other blocks, row-major over k < l, dealt round-robin; empty entries (~0)
+  // load a real block and store nothing, keeping the waves in step.
+  constexpr int CODES = SUPER ? IPT + 1 : BPT + 1;
+  uint32_t code[CODES];
+  if constexpr (SUPER) {
+    code[0] = owner_super(tid, m2);
 #pragma unroll
-  for (int i = 0; i < BPT; ++i) {
-    const int b = tid + i * nt;
-    code[i + 1] = ~0u;
-    if (b < offblocks) {
-      int k = 0, rem = b;
-      while (rem >= m - 1 - k) {
-        rem -= m - 1 - k;
-        ++k;
+    for (int i = 0; i < IPT; ++i) {
+      const int r = tid + i * nthreads;
+      code[i + 1] = r < items ? item_super(r, m2) : ~0u;
+    }
+  } else {
+    code[0] = tid < m ? next_pair_block(tid, m) : ~0u;
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+      const int b = tid + i * nthreads;
+      code[i + 1] = ~0u;
+      if (b < offblocks) {
+        int k = 0, rem = b;
+        while (rem >= m - 1 - k) {
+          rem -= m - 1 - k;
+          ++k;
+        }
+        const int l = k + 1 + rem;
+        if (!is_next_pair_block(k, l, m))
+          code[i + 1] = static_cast<uint32_t>(k) | (static_cast<uint32_t>(l) << 16);
       }
-      const int l = k + 1 + rem;
-      if (!is_next_pair_block(k, l, m))
-        code[i + 1] = static_cast<uint32_t>(k) | (static_cast<uint32_t>(l) << 16);
     }
   }
 
@@ -503,6 +629,46 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     // position k holds a: it is p (sees (c, -s)) when a < b
     rv[(k / S) * SP + k % S] = make_float2(r.x, a < b ? -r.y : r.y);
   };
+  // Pair phases of two slots with every load before any store, so that the
+  // two binary64 angle computations overlap (the pairs are disjoint).
+  auto pair_phase2 = [&](int k0, int k1, int tt, float2* rv) {
+    const int round = tt % rounds_per_sweep;
+    const int ks[2] = {k0, k1};
+    int a[2], b[2], p[2], q[2], cbp[2], cbq[2];
+    Block blk[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      a[u] = participant(round, ks[u], N);
+      b[u] = participant(round, N - 1 - ks[u], N);
+      p[u] = a[u] < b[u] ? a[u] : b[u];
+      q[u] = a[u] < b[u] ? b[u] : a[u];
+      cbp[u] = (p[u] * (p[u] + 1)) >> 1;
+      cbq[u] = (q[u] * (q[u] + 1)) >> 1;
+      blk[u].a00 = Ap[cbp[u] + p[u]];
+      blk[u].a01 = Ap[cbq[u] + p[u]];
+      blk[u].a10 = blk[u].a01;
+      blk[u].a11 = Ap[cbq[u] + q[u]];
+    }
+    float2 r[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) r[u] = rotation_angles(blk[u].a00, blk[u].a11, blk[u].a01);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const float2 rp = make_float2(r[u].x, -r[u].y);
+      const Block o = rotate_block(blk[u], rp, r[u], rp, r[u]);
+      Ap[cbp[u] + p[u]] = o.a00;
+      Ap[cbq[u] + p[u]] = o.a01;
+      Ap[cbq[u] + q[u]] = o.a11;
+      SlotIdx rec;
+      rec.p4 = 4 * p[u];
+      rec.q4 = 4 * q[u];
+      rec.cbp4 = 4 * cbp[u];
+      rec.cbq4 = 4 * cbq[u];
+      sidx[tt & 1][ks[u]] = rec;
+      rot[tt & 1][ks[u]] = r[u];
+      rv[(ks[u] / S) * SP + ks[u] % S] = make_float2(r[u].x, a[u] < b[u] ? -r[u].y : r[u].y);
+    }
+  };
 
   auto group = [&]() -> bool {
     const int group_start = t;
@@ -513,16 +679,35 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     for (; t < group_end; ++t) {
       const SlotIdx* cur = sidx[t & 1];
       const float2* cur_rot = rot[t & 1];
-#pragma unroll
-      for (int i = 0; i < BPT + 1; i += 2) {
-        if (i + 1 < BPT + 1) {
-          rotate_blocks<2>(Ap, cur, cur_rot, code + i);
-        } else {
-          rotate_blocks<1>(Ap, cur, cur_rot, code + i);
+      if constexpr (SUPER) {
+        // Owner super-blocks first (waves without owners skip them), then the
+        // pair phases of round t + 1, then the dealt items.
+        if (code[0] != ~0u) rotate_super(Ap, cur, cur_rot, code[0]);
+        if (t + 1 < group_end) {
+          float2* rv = hist + (t + 1 - group_start) * hrow;
+          if (tid < m2 - 1) {
+            pair_phase2(2 * tid + 1, 2 * tid + 2, t + 1, rv);
+          } else if (tid == m2 - 1) {
+            pair_phase(0, t + 1, rv);
+          } else if (tid == m2) {
+            pair_phase(m - 1, t + 1, rv);
+          }
         }
+#pragma unroll
+        for (int i = 1; i <= IPT; ++i)
+          if (code[i] != ~0u) rotate_super(Ap, cur, cur_rot, code[i]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < BPT + 1; i += 2) {
+          if (i + 1 < BPT + 1) {
+            rotate_blocks<2>(Ap, cur, cur_rot, code + i);
+          } else {
+            rotate_blocks<1>(Ap, cur, cur_rot, code + i);
+          }
+        }
+        if (tid < m && t + 1 < group_end)
+          pair_phase(tid, t + 1, hist + (t + 1 - group_start) * hrow);
       }
-      if (tid < m && t + 1 < group_end)
-        pair_phase(tid, t + 1, hist + (t + 1 - group_start) * hrow);
       __syncthreads();
     }
     // Rounds the group did not reach: identity rotations (V's positions still
@@ -740,11 +925,15 @@ hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t
   const int threads = batched_jacobi_threads(order);
   const size_t lds = batched_jacobi_lds_bytes(order);
   const int m = N / 2;
+  // The kernel's shape assumptions: whole parts per row, at most 1024 lanes
+  // (the count the kernel's item dealing is compiled for), a thread for each
+  // of the N / 4 + 1 pair-phase owners, every dealt super-block in some
+  // thread's list, LDS within 160 KiB, and for V in device memory two passes
+  // of N / 2 rows of 8 lanes.
   const int offblocks = m * (m - 1) / 2;
-  // The kernel's shape assumptions: whole parts per row, at most 1024 lanes,
-  // every off-diagonal block in some thread's list, LDS within 160 KiB, and
-  // for V in device memory two passes of N / 2 rows of 8 lanes.
   if (parts_for_order(N) * S != m || threads > 1024 || threads % 64 != 0 || threads < m ||
+      threads != threads_for_order(N) || threads < N / 4 + 1 ||
+      super_items(N) > items_per_thread(N) * threads ||
       (offblocks + threads - 1) / threads > blocks_per_thread(S) || lds > 160 * 1024 ||
       (v_in_memory(S) && threads != 8 * m))
     return hipErrorInvalidValue;
