@@ -25,7 +25,7 @@ cond up to 1e6, power law k^-alpha with alpha in [0.5, 3], four clusters):
 the same paths, with the pass rate (<= 1e-6 against the known root) and the
 fallback rate of each; no roofline leg.
 
-Also reported: the roofline of the dominant kernel (batched_jacobi_kernel<16>,
+Also reported: the roofline of the dominant kernel (batched_jacobi_kernel<256>,
 the launch of the orders 241..256 of the batch, run on its own and timed with
 HIP events on the library's stream; ``--roofline-only`` runs just these
 launches, for the profiler) and a CPU baseline: the reference's CPU plain path (1_baseline/spd.cpp:348-430,
@@ -49,7 +49,7 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch of the roofline kernel on the default workload: two
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a
 # 256 MiB copy gives), profiles/jacobi_n256_hbm_traffic_pmc.md.
-ROOFLINE_TRAFFIC_BYTES = 2.627e10
+ROOFLINE_TRAFFIC_BYTES = 2.635e10
 
 
 def log(*a):
@@ -63,7 +63,7 @@ def jacobi_flops(orders, sweeps):
 
 def roofline_leg(args, mats, orders, plain, rt, stream, torch, np, capi, dev):
     """Dominant kernel: the Jacobi launch of the orders 241..256 (one launch of
-    batched_jacobi_kernel<16>, one workgroup per matrix), timed on its own."""
+    batched_jacobi_kernel<256>, one workgroup per matrix), timed on its own."""
     big = [i for i, n in enumerate(orders) if n > 240]
     sub_orders = [orders[i] for i in big]
     _, sub_flat = capi.pack_batch([mats[i]["A"] for i in big])
@@ -90,7 +90,7 @@ def roofline_leg(args, mats, orders, plain, rt, stream, torch, np, capi, dev):
     return {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
             "traffic": ROOFLINE_TRAFFIC_BYTES if len(mats) == 4096 else None,
-            "kernel": (f"batched_jacobi_kernel<16> (orders 241..256: {len(big)} matrices, "
+            "kernel": (f"batched_jacobi_kernel<256> (orders 241..256: {len(big)} matrices, "
                        f"sweeps mean {np.mean(sweeps):.2f} max {max(sweeps)}, "
                        f"{flops / 1e12:.3f} TFLOP per launch at 9 S n^2 (n - 1), "
                        f"{jt * 1e3:.2f} ms mean of {len(durations)} launches)")}
