@@ -121,6 +121,14 @@ __host__ __device__ constexpr int blocks_per_thread(int S) {
 // MI355X (one launch per padded order of a 4096-matrix batch): N = 256 runs
 // 5.5% faster with super-blocks, N = 240 4% slower, N <= 128 about 2x slower.
 __host__ __device__ constexpr bool use_super_blocks(int N) { return N >= 256; }
+// V in device memory: two rows per lane (each history read serves both) or
+// one row per lane in two passes.  Two rows hold 4 S values per lane, which
+// spill at S >= 11; measured per order on MI355X (one launch per padded order
+// of a 4096-matrix batch), two rows run 2-5% faster at N = 144, 160, 176
+// and 256, and 2-4% slower at N = 192 .. 240.
+__host__ __device__ constexpr bool two_v_rows(int N) {
+  return N == 144 || N == 160 || N == 176 || N == 256;
+}
 
 // Large orders handle off-diagonal blocks as 2 x 2 super-blocks of slot pairs:
 // super-block (K, L), K <= L < m / 2, covers slots {2K, 2K + 1} x {2L, 2L + 1}
@@ -261,6 +269,76 @@ __device__ __forceinline__ void v_group_shift(float (&lo)[S], float (&hi)[S],
   if constexpr (I < G) {
     v_round_shift<S>(lo, hi, hist + I * row, h);
     v_group_shift<S, G, I + 1>(lo, hi, hist, row, h);
+  }
+}
+
+// v_round_shift on two rows of the same part: each rotation is read from LDS
+// once and applied to both (the history's LDS traffic is per lane, so two
+// rows per lane halve it).
+template <int S>
+__device__ __forceinline__ void v_round_shift2(float (&lo0)[S], float (&hi0)[S],
+                                               float (&lo1)[S], float (&hi1)[S],
+                                               const float2* __restrict__ rotv, int h) {
+#pragma unroll
+  for (int j0 = 0; j0 < S; j0 += 4) {
+    float2 r4[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (j0 + u < S) r4[u] = rotv[h * padded_slots(S) + j0 + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = j0 + u;
+      if (j < S) {
+        const float x0 = lo0[j], y0 = hi0[j];
+        lo0[j] = r4[u].x * x0 + r4[u].y * y0;
+        hi0[j] = r4[u].x * y0 - r4[u].y * x0;
+        const float x1 = lo1[j], y1 = hi1[j];
+        lo1[j] = r4[u].x * x1 + r4[u].y * y1;
+        hi1[j] = r4[u].x * y1 - r4[u].y * x1;
+      }
+    }
+  }
+  const bool first = h == 0, last = h == 7;
+  {
+    const float out_up = lo0[S - 1], out_dn = hi0[0], keep0 = lo0[0];
+    const float in_up = from_lane_below(out_up);
+    const float in_dn = from_lane_above(out_dn);
+#pragma unroll
+    for (int j = S - 1; j >= 2; --j) lo0[j] = lo0[j - 1];
+    lo0[1] = first ? out_dn : lo0[0];
+    lo0[0] = first ? keep0 : in_up;
+#pragma unroll
+    for (int j = 0; j < S - 1; ++j) hi0[j] = hi0[j + 1];
+    hi0[S - 1] = last ? out_up : in_dn;
+  }
+  {
+    const float out_up = lo1[S - 1], out_dn = hi1[0], keep0 = lo1[0];
+    const float in_up = from_lane_below(out_up);
+    const float in_dn = from_lane_above(out_dn);
+#pragma unroll
+    for (int j = S - 1; j >= 2; --j) lo1[j] = lo1[j - 1];
+    lo1[1] = first ? out_dn : lo1[0];
+    lo1[0] = first ? keep0 : in_up;
+#pragma unroll
+    for (int j = 0; j < S - 1; ++j) hi1[j] = hi1[j + 1];
+    hi1[S - 1] = last ? out_up : in_dn;
+  }
+  // Round boundary (empty asm, no instruction), as in v_round_shift.
+#pragma unroll
+  for (int r = 0; r < S; ++r) {
+    asm volatile("" : "+v"(lo0[r]), "+v"(hi0[r]));
+    asm volatile("" : "+v"(lo1[r]), "+v"(hi1[r]));
+  }
+  asm volatile("" ::: "memory");
+}
+
+template <int S, int G, int I = 0>
+__device__ __forceinline__ void v_group_shift2(float (&lo0)[S], float (&hi0)[S],
+                                               float (&lo1)[S], float (&hi1)[S],
+                                               const float2* __restrict__ hist, int row, int h) {
+  if constexpr (I < G) {
+    v_round_shift2<S>(lo0, hi0, lo1, hi1, hist + I * row, h);
+    v_group_shift2<S, G, I + 1>(lo0, hi0, lo1, hi1, hist, row, h);
   }
 }
 
@@ -721,33 +799,62 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
       // the 2 S live V registers.  G = S: every group starts at phase 0.
       v_group<S, G, 0>(lo, hi, hist, hrow, h, H, lane);
     } else {
-      // Two passes of N / 2 rows (8 lanes per row).  Each lane loads its
-      // positions, applies the group's rounds with the values shifting along
-      // the circle in registers, and stores every register back to its own
-      // position: no other lane touches them, so no barrier in between.
-      // Element offsets from a value the compiler must treat as new in every
-      // group (empty asm): otherwise it hoists all 2 S addresses out of the
-      // loop and keeps them live (spilled) through the A update.
-#pragma unroll 1
-      for (int pass = 0; pass < 2; ++pass) {
-        const int vrow = pass * m + (tid >> 3);
+      // Each lane loads its positions, applies the group's rounds with the
+      // values shifting along the circle in registers, and stores every
+      // register back to its own position: no other lane touches them, so no
+      // barrier in between.  Element offsets from a value the compiler must
+      // treat as new in every group (empty asm): otherwise it hoists all the
+      // addresses out of the loop and keeps them live (spilled) through the A
+      // update.
+      if constexpr (two_v_rows(NP)) {
+        // Rows r and r + N / 2 on the same lane (8 lanes per row), so each
+        // rotation read from the history serves two rows.
+        const int vrow = tid >> 3;
         const int vh = tid & 7;
         uint32_t lo_off = static_cast<uint32_t>(vrow + vh * S * ld);
         uint32_t hi_off = static_cast<uint32_t>(vrow + (N - 1 - vh * S) * ld);
         uint32_t step = static_cast<uint32_t>(ld);
         asm volatile("" : "+v"(lo_off), "+v"(hi_off), "+s"(step));
-        float vl[S], vhv[S];
+        float vl0[S], vh0[S], vl1[S], vh1[S];
 #pragma unroll
         for (int j = 0; j < S; ++j) {
-          vl[j] = Vg[lo_off + j * step];
-          vhv[j] = Vg[hi_off - j * step];
+          vl0[j] = Vg[lo_off + j * step];
+          vh0[j] = Vg[hi_off - j * step];
+          vl1[j] = Vg[lo_off + m + j * step];
+          vh1[j] = Vg[hi_off + m - j * step];
         }
-        v_group_shift<S, G>(vl, vhv, hist, hrow, vh);
+        v_group_shift2<S, G>(vl0, vh0, vl1, vh1, hist, hrow, vh);
         asm volatile("" : "+v"(lo_off), "+v"(hi_off), "+s"(step));
 #pragma unroll
         for (int j = 0; j < S; ++j) {
-          Vg[lo_off + j * step] = vl[j];
-          Vg[hi_off - j * step] = vhv[j];
+          Vg[lo_off + j * step] = vl0[j];
+          Vg[hi_off - j * step] = vh0[j];
+          Vg[lo_off + m + j * step] = vl1[j];
+          Vg[hi_off + m - j * step] = vh1[j];
+        }
+      } else {
+        // Two passes of N / 2 rows (8 lanes per row).
+#pragma unroll 1
+        for (int pass = 0; pass < 2; ++pass) {
+          const int vrow = pass * m + (tid >> 3);
+          const int vh = tid & 7;
+          uint32_t lo_off = static_cast<uint32_t>(vrow + vh * S * ld);
+          uint32_t hi_off = static_cast<uint32_t>(vrow + (N - 1 - vh * S) * ld);
+          uint32_t step = static_cast<uint32_t>(ld);
+          asm volatile("" : "+v"(lo_off), "+v"(hi_off), "+s"(step));
+          float vl[S], vhv[S];
+#pragma unroll
+          for (int j = 0; j < S; ++j) {
+            vl[j] = Vg[lo_off + j * step];
+            vhv[j] = Vg[hi_off - j * step];
+          }
+          v_group_shift<S, G>(vl, vhv, hist, hrow, vh);
+          asm volatile("" : "+v"(lo_off), "+v"(hi_off), "+s"(step));
+#pragma unroll
+          for (int j = 0; j < S; ++j) {
+            Vg[lo_off + j * step] = vl[j];
+            Vg[hi_off - j * step] = vhv[j];
+          }
         }
       }
     }
