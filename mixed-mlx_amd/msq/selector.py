@@ -244,22 +244,61 @@ class Selector:
         z = logits.cpu().numpy().reshape(count, capi.MLP_OUTPUTS).astype(np.float64)
         return 1.0 / (1.0 + np.exp(-z))
 
-    def choose(self, prob, orders=None, fallback_weight=FALLBACK_WEIGHT):
-        """Tier per matrix minimising c_t + (1 - p_t) F c_plain (tier 0: p = 1)."""
+    def choose(self, prob, orders=None, fallback_weight=FALLBACK_WEIGHT, candidates=None):
+        """Tier per matrix minimising c_t + (1 - p_t) F c_plain (tier 0: p = 1),
+        over `candidates` (tier indices; all tiers when None; the plain tier
+        is always a candidate)."""
         p = np.array(prob, dtype=np.float64)
         p[:, 0] = 1.0
         expected = self.tier_cost[None, :] + (1.0 - p) * fallback_weight * self.tier_cost[0]
+        if candidates is not None:
+            allowed = np.zeros(len(self.tier_cost), dtype=bool)
+            allowed[list(candidates)] = True
+            allowed[0] = True
+            expected = np.where(allowed[None, :], expected, np.inf)
         return np.argmin(expected, axis=1)
 
-    def options(self, rt, orders, matrices):
-        tiers = self.choose(self.probabilities(rt, orders, matrices), orders)
+    def options(self, rt, orders, matrices, candidates=None):
+        tiers = self.choose(self.probabilities(rt, orders, matrices), orders,
+                            candidates=candidates)
         opts = [tier_options(int(k)) for k in range(len(TIERS))]
         return [opts[int(k)] for k in tiers], tiers
 
 
-def solve_selected(rt, selector, orders, matrices, roots):
+# Tiers the batched solve runs as one precision group: BF16 basis and
+# corrections at stop rates 0.5 to 4.  A second low-bit precision group (FP8,
+# MXFP4) costs more in one batched solve than the per-matrix cost model
+# predicts, because each group adds its own launches to every correction step.
+# Measured on MI355X, in the same process on the same batches:
+#   * 1000-matrix mixture, orders 32..512: 364.7 ms against 378.4 ms for the
+#     fixed BF16 tier, with 17 fallbacks against 45; all eight tiers give
+#     376.0 ms;
+#   * 4096-matrix uneven batch: 89.1 ms against 75.8 ms.  The fixed tier
+#     never fails there, so the features and the matrices sent to the plain
+#     path are pure cost.
+BATCH_CANDIDATES = (1, 2, 3, 4)
+
+
+# Batch gate: when the network expects less than this fraction of the batch
+# to fail on the gentlest BF16 tier (mean of 1 - p), the whole batch runs
+# that tier.  The few fallback re-solves it may save cost less than routing
+# matrices to other tiers adds to the batched solve.  Measured: the uneven
+# 4096-matrix batch expects 0.3% (12.5 matrices; none fail) and gates to the
+# fixed tier; the 32..512 mixture expects 4.6% (45.9; 45 fail) and is routed
+# per matrix.
+GATE_FAILURE_FRACTION = 0.005
+
+
+def solve_selected(rt, selector, orders, matrices, roots, candidates=BATCH_CANDIDATES,
+                   gate=GATE_FAILURE_FRACTION):
     """Selector-chosen tiers with the plain-path fallback.  Returns
     (reports, fallback_indices, tiers)."""
-    opts, tiers = selector.options(rt, orders, matrices)
+    prob = selector.probabilities(rt, orders, matrices)
+    if gate is not None and float(np.mean(1.0 - prob[:, 1])) < gate:
+        tiers = np.ones(len(orders), dtype=np.int64)
+    else:
+        tiers = selector.choose(prob, orders, candidates=candidates)
+    opts_all = [tier_options(int(k)) for k in range(len(TIERS))]
+    opts = [opts_all[int(k)] for k in tiers]
     reports, fallback = MS.solve_with_fallback(rt, orders, matrices, roots, opts)
     return reports, fallback, tiers
