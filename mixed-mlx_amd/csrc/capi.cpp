@@ -503,6 +503,7 @@ int run_large_jacobi(msq_runtime* rt, const Layout& L, const double* a64, float*
 struct JacobiPlan {
   std::vector<msq::JacobiJob> jobs;
   std::vector<std::pair<size_t, uint32_t>> launches;  // ends with (jobs.size(), 0)
+  std::vector<double> cost;                           // per launch: sum of n^3
   bool has_large = false;                             // orders above 256
 };
 
@@ -520,7 +521,12 @@ void plan_jacobi(const Layout& L, const msq_jacobi_options* opts, uint32_t opts_
   // Largest orders first: their workgroups are the longest-running.
   for (auto it = by_padded.rbegin(); it != by_padded.rend(); ++it) {
     P.launches.push_back({P.jobs.size(), it->first});
-    for (uint32_t i : it->second) P.jobs.push_back(make_job(L, i, opts[opts_count == 1 ? 0 : i]));
+    double cost = 0.0;
+    for (uint32_t i : it->second) {
+      P.jobs.push_back(make_job(L, i, opts[opts_count == 1 ? 0 : i]));
+      cost += static_cast<double>(L.order[i]) * L.order[i] * L.order[i];
+    }
+    P.cost.push_back(cost);
   }
   P.launches.push_back({P.jobs.size(), 0});
 }
@@ -531,18 +537,29 @@ int enqueue_jacobi(msq_runtime* rt, const JacobiPlan& P, const msq::JacobiJob* d
                    const double* a64, float* V, float* diag, float* roots,
                    msq::JacobiStatus* jstatus) {
   if (P.jobs.empty()) return MSQ_OK;
-  // Order classes run concurrently on the runtime's lanes (largest first,
-  // dealt round-robin): a launch lasts as long as its slowest matrix, and
-  // the CUs its faster matrices free take work from the other classes.
+  // Order classes run concurrently on the runtime's lanes: a launch lasts as
+  // long as its slowest matrix, and the CUs its faster matrices free take
+  // work from the other classes.  Each lane is in order, so the classes are
+  // dealt longest first to the lane with the least work so far (work: sum of
+  // n^3 over the class), which keeps the lanes' totals level.
   const size_t classes = P.launches.size() - 1;
   const int used = classes > 1 ? static_cast<int>(std::min<size_t>(classes, kLanes)) : 0;
   if (used) {
     HIP_TRY(hipEventRecord(rt->fork, rt->stream));
     for (int l = 0; l < used; ++l) HIP_TRY(hipStreamWaitEvent(rt->lanes[l], rt->fork, 0));
   }
-  for (size_t c = 0; c < classes; ++c) {
+  std::vector<size_t> order(classes);
+  for (size_t c = 0; c < classes; ++c) order[c] = c;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t x, size_t y) { return P.cost[x] > P.cost[y]; });
+  double load[kLanes] = {};
+  for (size_t c : order) {
     const size_t b = P.launches[c].first, e = P.launches[c + 1].first;
-    hipStream_t s = used ? rt->lanes[c % used] : rt->stream;
+    int lane = 0;
+    for (int l = 1; l < used; ++l)
+      if (load[l] < load[lane]) lane = l;
+    if (used) load[lane] += P.cost[c];
+    hipStream_t s = used ? rt->lanes[lane] : rt->stream;
     HIP_TRY(msq::launch_batched_jacobi(djobs + b, static_cast<uint32_t>(e - b),
                                        P.launches[c].second, a64, V, diag, roots, jstatus, s));
   }
