@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define MSQ_ABI_VERSION 3
+#define MSQ_ABI_VERSION 4
 
 enum msq_status {
   MSQ_OK = 0,
@@ -144,6 +144,12 @@ typedef struct msq_solve_report {
   double forward_estimates[MSQ_SOLVE_HISTORY];
   float off_history[MSQ_JACOBI_HISTORY];
   float correction_rate; /* Jacobi's ||G||_F / min s (see msq_jacobi_options) */
+  /* Basis tier acceptance (src/metal_solver.cpp:217-236): ||V_t^T V_t - I||_F /
+   * sqrt(n) of V rounded to the basis tier (0 for an FP32 basis), and 1 when it
+   * exceeded the tier's tolerance (BF16 5e-3, 8-bit 0.15, 4-bit 0.5;
+   * src/configuration.cpp:109-118) and X0 was formed from the FP32 basis. */
+  float basis_orthogonality;
+  uint32_t precision_escalations;
 } msq_solve_report;
 
 /* ---------------------------------------------------------------- runtime */

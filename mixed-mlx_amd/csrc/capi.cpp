@@ -913,6 +913,20 @@ int reconstruction_product(msq_runtime* rt, int tier, const Layout& L,
 // plan then lists the tiles on and above the diagonal (build_plan upper) and
 // the epilogue stores each off-diagonal tile's mirror, A_ji - (X X)_ij (half
 // the products).
+// Basis acceptance tolerance of a tier (basisOrthogonalityTolerance,
+// src/configuration.cpp:109-118): BF16 5e-3, 4-bit formats 0.5, the other
+// 8-bit formats 0.15.
+double basis_tolerance(int tier) {
+  switch (tier) {
+    case MSQ_FP32: return 1.0e-3;
+    case MSQ_BF16: return 5.0e-3;
+    case MSQ_FP4_E2M1:
+    case MSQ_MXFP4_E2M1:
+    case MSQ_FP4_E2M1_LOG_S: return 5.0e-1;
+    default: return 1.5e-1;
+  }
+}
+
 int residual_product(msq_runtime* rt, const GemmPlan& plan, const double* A, const double* X,
                      double* R, float* R32, uint16_t* Rb, double* norms, const int* active,
                      uint32_t count, bool symmetric_x = true) {
@@ -1082,6 +1096,8 @@ int finish_solve(msq_runtime* rt, const Layout& L, const msq::SolveState* st,
       r.correction_rate = hj[i].correction_rate;
       r.final_residual = s.final_residual;
       r.final_forward_estimate = s.final_forward_estimate;
+      r.basis_orthogonality = static_cast<float>(s.basis_orthogonality);
+      r.precision_escalations = s.escalated;
       for (uint32_t k = 0; k < MSQ_SOLVE_HISTORY && k < msq::kSolveHistory; ++k) {
         r.residuals[k] = k < s.steps ? s.residuals[k] : 0.0;
         r.forward_estimates[k] = k < s.steps ? s.forward_estimates[k] : 0.0;
@@ -1577,6 +1593,8 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   float *V = nullptr, *Vt = nullptr, *diag = nullptr, *sroot = nullptr, *R32 = nullptr;
   uint16_t* Rb = nullptr;
   double* norms = nullptr;
+  double* orth2 = nullptr;  // per matrix: ||V_t^T V_t - I||_F^2 (basis acceptance)
+  int* esc = nullptr;       // per matrix: basis tier rejected
   int* active = nullptr;
   msq::JacobiStatus* jst = nullptr;
   msq::JacobiJob* djobs = nullptr;
@@ -1595,6 +1613,8 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   if (any_float) S.want(R32, L.total);
   if (any_bf16) S.want(Rb, L.total);
   S.want(norms, count);
+  S.want(orth2, count);
+  S.want(esc, count);
   S.want(active, count);
   S.want(jst, count);
   S.want(djobs, count);
@@ -1650,9 +1670,30 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
       MSQ_TRY(run_large_jacobi(rt, L, matrices, V, diag, sroot, jopts.data(), options_count, jst));
     HIP_TRY(msq::launch_solve_state(dsegs_all, jst, diag, dlim, st, active, count, rt->stream));
     // 2. Reconstruction X0 = G G^T in the basis tier, X = sym(X0) in binary64.
+    //    A low-bit basis is accepted per matrix as the reference's
+    //    quantizeAndDecode + measureF32Orthogonality do (src/metal_solver.cpp:
+    //    217-236): ||V_t^T V_t - I||_F / sqrt(n) of V rounded to the tier
+    //    within the tier's tolerance (src/configuration.cpp:109-118); the
+    //    matrices that miss it get X0 from the FP32 basis instead.
     for (Group& g : gl) {
+      const uint32_t gm = static_cast<uint32_t>(g.members.size());
+      if (g.basis != MSQ_FP32) {
+        Operand ob;
+        const float* alpha = nullptr;
+        MSQ_TRY(encode_operand(rt, g.basis, L, g.members, g.dsegs, V, cb.enc_a, ob));
+        MSQ_TRY(fp8_alpha(rt, g.basis, ob, ob, cb.alpha, count, &alpha));
+        msq::GemmArgs gg = product_args(ob, ob, cb.t2, alpha, active);
+        MSQ_TRY(run_gemm(rt, g.plan, gemm_type(g.basis), msq::Epilogue::Store, gg));
+        HIP_TRY(msq::launch_zero_f64(orth2, count, rt->stream));
+        HIP_TRY(msq::launch_orthogonality(g.dsegs, gm, L.max_ld, cb.t2, orth2, rt->stream));
+        HIP_TRY(msq::launch_basis_escalation(g.dsegs, gm, orth2, basis_tolerance(g.basis), active,
+                                             esc, st, rt->stream));
+      }
       MSQ_TRY(reconstruction_product(rt, g.basis, L, g.members, g.dsegs, g.plan, V, sroot, active,
                                      cb));
+      if (g.basis != MSQ_FP32)  // the rejected ones again, from the FP32 basis
+        MSQ_TRY(reconstruction_product(rt, MSQ_FP32, L, g.members, g.dsegs, g.plan, V, sroot, esc,
+                                       cb));
       HIP_TRY(msq::launch_symmetrize_to_f64(g.dsegs, static_cast<uint32_t>(g.members.size()),
                                             pair_count(g.segs), cb.t2, roots_out, active,
                                             rt->stream));

@@ -263,6 +263,9 @@ struct SolveState {
   double final_forward_estimate;
   double residuals[kSolveHistory];          // after X0, then after each correction
   double forward_estimates[kSolveHistory];  // ||R||_F / (2 s_min ||X||_F)
+  double basis_orthogonality;    // ||V_t^T V_t - I||_F / sqrt(n) of the basis-tier V
+  uint32_t escalated;            // 1: the basis tier was rejected, X0 rebuilt in FP32
+  uint32_t reserved2;
 };
 
 hipError_t launch_solve_state(const Segment* segs, const JacobiStatus* js,
@@ -270,5 +273,10 @@ hipError_t launch_solve_state(const Segment* segs, const JacobiStatus* js,
                               int* active, uint32_t count, hipStream_t stream);
 hipError_t launch_convergence(SolveState* st, const double* norms, int* active, uint32_t count,
                               uint32_t step, hipStream_t stream);
+// Basis escalation of the segments' matrices: orthogonality o = sqrt(orth2 /
+// n) into the state, and esc = active && !(o <= tolerance).
+hipError_t launch_basis_escalation(const Segment* segs, uint32_t count, const double* orth2,
+                                   double tolerance, const int* active, int* esc,
+                                   SolveState* st, hipStream_t stream);
 
 }  // namespace msq

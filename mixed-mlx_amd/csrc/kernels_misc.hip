@@ -302,6 +302,23 @@ __global__ void solve_state_kernel(const Segment* __restrict__ segs,
   active[i] = valid ? 1 : 0;
 }
 
+// measureF32Orthogonality's acceptance test (src/metal_solver.cpp:217-236):
+// the basis tier is kept when ||V_t^T V_t - I||_F / sqrt(n) <= tolerance.
+__global__ void basis_escalation_kernel(const Segment* __restrict__ segs, uint32_t count,
+                                        const double* __restrict__ orth2, double tolerance,
+                                        const int* __restrict__ active, int* __restrict__ esc,
+                                        SolveState* __restrict__ st) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const Segment sg = segs[k];
+  const uint32_t i = sg.index;
+  const double o = sqrt(orth2[i] / static_cast<double>(sg.order));
+  const int e = (active[i] && !(o <= tolerance)) ? 1 : 0;
+  esc[i] = e;
+  st[i].basis_orthogonality = o;
+  st[i].escalated = static_cast<uint32_t>(e);
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t value) {
   for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
        e += static_cast<size_t>(gridDim.x) * blockDim.x)
@@ -409,6 +426,14 @@ hipError_t launch_solve_state(const Segment* segs, const JacobiStatus* js,
   if (!count) return hipSuccess;
   solve_state_kernel<<<(count + kT - 1) / kT, kT, 0, stream>>>(segs, js, diagonal, limits, st,
                                                               active, count);
+  return hipGetLastError();
+}
+hipError_t launch_basis_escalation(const Segment* segs, uint32_t count, const double* orth2,
+                                   double tolerance, const int* active, int* esc,
+                                   SolveState* st, hipStream_t stream) {
+  if (!count) return hipSuccess;
+  basis_escalation_kernel<<<(count + kT - 1) / kT, kT, 0, stream>>>(segs, count, orth2, tolerance,
+                                                                   active, esc, st);
   return hipGetLastError();
 }
 hipError_t launch_convergence(SolveState* st, const double* norms, int* active, uint32_t count,

@@ -105,7 +105,9 @@ class SolveReport(ctypes.Structure):
                 ("residuals", ctypes.c_double * SOLVE_HISTORY),
                 ("forward_estimates", ctypes.c_double * SOLVE_HISTORY),
                 ("off_history", ctypes.c_float * JACOBI_HISTORY),
-                ("correction_rate", ctypes.c_float)]
+                ("correction_rate", ctypes.c_float),
+                ("basis_orthogonality", ctypes.c_float),
+                ("precision_escalations", ctypes.c_uint32)]
 
 
 _p = ctypes.c_void_p
@@ -261,6 +263,8 @@ def report_dict(r: SolveReport):
         "forward_estimates": list(r.forward_estimates[:steps]),
         "off_history": list(r.off_history[:r.sweeps]),
         "correction_rate": r.correction_rate,
+        "basis_orthogonality": r.basis_orthogonality,
+        "precision_escalations": r.precision_escalations,
     }
 
 

@@ -52,6 +52,62 @@ def test_batched_uneven_every_matrix(runtime, tier):
     say(tier, "worst", worst)
 
 
+@pytest.mark.parametrize("tier, tolerance", [("bf16", 5e-3), ("fp8", 0.15), ("mxfp4", 0.5)])
+def test_basis_acceptance_and_escalation(runtime, tier, tolerance):
+    """The reference's basis acceptance (src/metal_solver.cpp:217-236): V is
+    rounded to the basis tier and kept when ||V_t^T V_t - I||_F / sqrt(n) is
+    within the tier's tolerance (src/configuration.cpp:109-118), else X0 comes
+    from the FP32 basis.  The reported orthogonality equals msq_orthogonality
+    of the same Jacobi V quantised and decoded by the codec API, the
+    escalation flag follows the tolerance, and every root meets 1e-6."""
+    orders = [32, 48, 64, 100, 128, 160, 200, 256]
+    mats = [G.single_matrix(14, n, 1e-3, index=i)[0] for i, n in enumerate(orders)]
+    opts = MS.tier_options(tier)
+    o, aflat = capi.pack_batch(mats)
+    _, _, _, total, vtotal = capi.batch_layout(o)
+    A = dev(aflat)
+    X = torch.zeros(total, dtype=torch.float64, device="cuda")
+    say("basis acceptance", tier, orders)
+    reps = runtime.solve_batched(o, A, X, opts)
+    # The same Jacobi (same options, deterministic), quantised by the codec API.
+    V = torch.zeros(total, dtype=torch.float32, device="cuda")
+    D = torch.zeros(vtotal, dtype=torch.float32, device="cuda")
+    Rt = torch.zeros(vtotal, dtype=torch.float32, device="cuda")
+    runtime.jacobi_batched(o, A, V, D, Rt, opts.jacobi)
+    basis = MS.TIER_PRECISIONS[tier][0]
+    decoded = torch.zeros_like(V)
+    if basis == capi.FP8_E4M3FN_SCALED:
+        codes = torch.zeros(total, dtype=torch.uint8, device="cuda")
+        # Per-matrix scale sigma = 448 / amax, applied per segment.
+        lds, offs, _, _, _ = capi.batch_layout(o)
+        for off, ld in zip(offs, lds):
+            seg = V[off:off + ld * ld]
+            sig = torch.zeros(1, dtype=torch.float32, device="cuda")
+            runtime.quantize_fp8_scaled(seg, codes[off:off + ld * ld], sig)
+            runtime.decode_fp8_scaled(codes[off:off + ld * ld], sig, decoded[off:off + ld * ld])
+    else:
+        codes = torch.zeros(capi.storage_bytes(basis, total), dtype=torch.uint8, device="cuda")
+        scales = torch.zeros(max(1, capi.scale_bytes(basis, total)), dtype=torch.uint8,
+                             device="cuda")
+        runtime.quantize_and_decode(basis, V, codes, scales, decoded)
+    orth = runtime.orthogonality(o, decoded)
+    got = np.array([r["basis_orthogonality"] for r in reps])
+    esc = np.array([r["precision_escalations"] for r in reps])
+    say("orthogonality", np.round(orth, 5).tolist(), "escalated", esc.tolist())
+    np.testing.assert_allclose(got, orth, rtol=1e-3, atol=1e-6)
+    assert (esc == (orth > tolerance)).all()
+    # Roots to 1e-6: directly, or (an MXFP4 basis, whose X0 can leave the
+    # BF16 corrections diverging) after the plain-path re-solve.
+    reps, fallback = MS.solve_with_fallback(runtime, o, A, X, opts)
+    say("fallbacks", fallback)
+    if tier != "mxfp4":
+        assert not fallback
+    Xs = capi.unpack_batch(o, host(X))
+    for Ai, Xi, rep in zip(mats, Xs, reps):
+        assert rep["status"] == "converged", rep
+        assert S.relative_error(Xi, S.reference_root(Ai)) <= 1e-6
+
+
 def test_config1_n1024_bf16(runtime):
     """n = 1024, cond 1e4: BF16 reconstruction, FP64 residual, 2 corrections."""
     A, Xc, lam = G.single_matrix(1, 1024, 1e-4)
