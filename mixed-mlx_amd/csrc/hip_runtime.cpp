@@ -7,8 +7,10 @@
 #include <cmath>
 #include <cstdio>
 #include <limits>
+#include <sstream>
 #include <stdexcept>
 #include <utility>
+#include <vector>
 
 namespace msq_hip {
 
@@ -339,6 +341,7 @@ HipCaseResult runTimedHipCase(HipRuntime& runtime, const HipSolverConfiguration&
     for (size_t i = 0; i < n; ++i) X[i + j * n] = tile[i + j * ld];
 
   r.name = "hip_spd_sqrt";
+  r.warmups = warmups;
   r.basis_precision = precisionName(configuration.basis_precision);
   r.correction_precision = precisionName(configuration.correction_precision);
   r.order = n;
@@ -350,8 +353,13 @@ HipCaseResult runTimedHipCase(HipRuntime& runtime, const HipSolverConfiguration&
   r.off_norm = rep.off_norm;
   r.final_residual = rep.final_residual;
   r.final_forward_estimate = rep.final_forward_estimate;
-  for (uint32_t k = 0; k <= rep.corrections && k < MSQ_SOLVE_HISTORY; ++k)
+  for (uint32_t k = 0; k <= rep.corrections && k < MSQ_SOLVE_HISTORY; ++k) {
     r.residual_history.push_back(rep.residuals[k]);
+    r.forward_estimate_history.push_back(rep.forward_estimates[k]);
+  }
+  r.basis_orthogonality = rep.basis_orthogonality;
+  r.precision_escalations = rep.precision_escalations;
+  r.max_corrections = problem.max_iterations;
   r.finite = std::all_of(X.begin(), X.end(), [](double v) { return std::isfinite(v); });
   double asym = 0.0;
   for (size_t j = 0; j < n; ++j)
@@ -400,6 +408,234 @@ void printResult(const HipCaseResult& r) {
     std::printf("evaluations=%zu\nmedian_seconds=%.6e\nmin_seconds=%.6e\n", s.size(),
                 s[s.size() / 2], s.front());
   }
+}
+
+
+namespace {
+
+using Field = std::pair<const char*, std::string>;
+
+std::string num(double v) {
+  std::ostringstream o;
+  o.precision(17);
+  o << v;
+  return o.str();
+}
+std::string num(size_t v) { return std::to_string(v); }
+
+struct TimingSummary {
+  double min = 0.0, max = 0.0, median = 0.0, mean = 0.0, variance = 0.0;
+};
+TimingSummary summarize(std::vector<double> s) {
+  TimingSummary t;
+  if (s.empty()) return t;
+  std::sort(s.begin(), s.end());
+  t.min = s.front();
+  t.max = s.back();
+  t.median = s.size() % 2 ? s[s.size() / 2] : 0.5 * (s[s.size() / 2 - 1] + s[s.size() / 2]);
+  for (double x : s) t.mean += x;
+  t.mean /= static_cast<double>(s.size());
+  for (double x : s) t.variance += (x - t.mean) * (x - t.mean);
+  t.variance /= static_cast<double>(s.size());
+  return t;
+}
+
+// The columns shared by both files, up to kernel_path (src/reporting.cpp:
+// 124-131 and :307-311).
+std::vector<Field> case_fields(const HipCaseResult& r) {
+  return {
+      {"paper_test_set", "controlled"},
+      {"matrix_family", "conditioned_spd"},
+      {"matrix_type", "symmetric"},
+      {"solver_backend", r.solver_backend},
+      {"rng_seed", "0"},
+      {"rounding_mode", "nearest"},
+      {"rounding_seed", "0"},
+      {"diagonal_shift", "0"},
+      {"shift_rule", "not_applicable"},
+      {"case", r.name},
+      {"precision", r.basis_precision},
+      {"basis_precision", r.basis_precision},
+      {"spectrum_precision", "fp32"},
+      {"correction_mode", r.correction_precision + "_frozen_factor"},
+      {"precision_policy", "forward_error_target"},
+      {"stopping_policy", "forward_estimate_or_" + num(r.max_corrections) + "_corrections"},
+      {"dispatch_policy", "not_applicable"},
+      {"termination_reason", r.termination_reason},
+      {"selection_policy", "fixed"},
+      {"precision_attempts", "1"},
+      {"execution_model", "gpu_resident_batched"},
+      {"kernel_path", "hip_mfma"},
+  };
+}
+
+// basis stage .. basis_reorthogonalization_seconds (both files).
+std::vector<Field> basis_fields(const HipCaseResult& r) {
+  const bool low_basis = r.basis_precision != "fp32";
+  const size_t n2 = r.order * r.order;
+  const Precision basis = parsePrecision(r.basis_precision);
+  const size_t low_bytes = msq_storage_bytes(static_cast<int>(basis), n2) +
+                           msq_scale_bytes(static_cast<int>(basis), n2);
+  const size_t fp32_bytes = 4 * n2;
+  return {
+      {"selected_basis_stage",
+       !low_basis ? "fp32"
+                  : (r.precision_escalations ? r.basis_precision + "_rejected_to_fp32"
+                                             : r.basis_precision)},
+      {"selected_spectrum_stage", "fp32"},
+      {"correction_stage_history", r.correction_precision},
+      {"precision_escalations", num(r.precision_escalations)},
+      {"low_factor_corrections", num(r.correction_precision != "fp32" ? r.corrections : 0)},
+      {"low_factor_basis_bytes", num(low_bytes)},
+      {"fp32_basis_bytes", num(fp32_bytes)},
+      {"basis_storage_ratio",
+       num(fp32_bytes ? static_cast<double>(low_bytes) / static_cast<double>(fp32_bytes) : 0.0)},
+      {"quantized_basis_orthogonality",
+       low_basis ? num(r.basis_orthogonality) : std::string("nan")},
+      {"factor_quantization_seconds", "0"},
+      {"basis_reorthogonalization_seconds", "0"},
+  };
+}
+
+std::vector<Field> reference_timing_fields() {
+  return {{"reference_timing_min_seconds", "0"},
+          {"reference_timing_max_seconds", "0"},
+          {"reference_timing_median_seconds", "0"},
+          {"reference_timing_mean_seconds", "0"},
+          {"reference_timing_variance_seconds2", "0"}};
+}
+std::vector<Field> paired_fields() {
+  return {{"paired_ratio_mean", "0"},
+          {"paired_ratio_variance", "0"},
+          {"paired_ratio_min", "0"},
+          {"paired_ratio_max", "0"},
+          {"paired_ratio_median", "0"}};
+}
+// Phase times (factorization .. mixed_total): only the total is measured.
+std::vector<Field> phase_fields(const TimingSummary& t) {
+  return {{"factorization_seconds", "0"},
+          {"initial_root_seconds", "0"},
+          {"single_seconds", "0"},
+          {"residual_seconds", "0"},
+          {"correction_system_seconds", "0"},
+          {"updates_and_others_seconds", "0"},
+          {"mixed_total_seconds", num(t.median)}};
+}
+
+void append(std::vector<Field>& out, std::vector<Field> more) {
+  out.insert(out.end(), std::make_move_iterator(more.begin()),
+             std::make_move_iterator(more.end()));
+}
+
+std::vector<Field> summary_fields(const HipCaseResult& r) {
+  const TimingSummary t = summarize(r.seconds);
+  const size_t evaluations = r.seconds.size();
+  std::vector<Field> f = case_fields(r);
+  append(f, {{"command_buffers_per_correction", "0"}, {"kernel_dispatches_per_correction", "0"}});
+  append(f, basis_fields(r));
+  append(f, {{"warmups", num(r.warmups)},
+             {"evaluations", num(evaluations)},
+             {"timing_evaluations", num(evaluations)},
+             {"timing_min_seconds", num(t.min)},
+             {"timing_max_seconds", num(t.max)},
+             {"timing_median_seconds", num(t.median)},
+             {"timing_mean_seconds", num(t.mean)},
+             {"timing_variance_seconds2", num(t.variance)}});
+  append(f, reference_timing_fields());
+  size_t best = 0;
+  for (size_t k = 1; k < r.residual_history.size(); ++k)
+    if (r.residual_history[k] < r.residual_history[best]) best = k;
+  const double initial_residual = r.residual_history.empty() ? 0.0 : r.residual_history.front();
+  const double initial_forward =
+      r.forward_estimate_history.empty() ? 0.0 : r.forward_estimate_history.front();
+  append(f, {{"order", num(r.order)},
+             {"condition", num(r.condition)},
+             {"iterations", num(r.corrections)},
+             {"best_iteration", num(best)},
+             {"best_residual", num(r.residual_history.empty() ? 0.0 : r.residual_history[best])},
+             {"best_forward_error", num(r.final_forward_error)},
+             {"stagnation_steps", "0"},
+             {"converged", r.converged ? "1" : "0"},
+             {"finite", r.finite ? "1" : "0"},
+             {"status", r.status},
+             {"initial_residual", num(initial_residual)},
+             {"final_residual", num(r.final_residual)},
+             {"initial_forward_error", num(initial_forward)},
+             {"final_forward_error", num(r.final_forward_error)},
+             {"symmetry_error", num(r.symmetry_error)},
+             {"reference_residual", "0"},
+             {"frozen_factor_error", "0"},
+             {"frozen_factor_relative_error", "0"},
+             {"frozen_basis_orthogonality", "0"},
+             {"frozen_sylvester_inverse_norm", "0"},
+             {"frozen_contraction_bound", "0"},
+             {"certificate_usable", "0"},
+             {"reference_seconds", "0"},
+             {"low_eigen_seconds", "0"},
+             {"initial_gpu_seconds", "0"},
+             {"residual_fp64_seconds", "0"},
+             {"correction_gpu_seconds", "0"},
+             {"update_fp64_seconds", "0"}});
+  append(f, phase_fields(t));
+  append(f, {{"low_eigen_fraction", "0"},
+             {"initial_gpu_fraction", "0"},
+             {"residual_fraction", "0"},
+             {"correction_gpu_fraction", "0"},
+             {"update_fraction", "0"}});
+  append(f, paired_fields());
+  append(f, {{"mixed_to_reference", "0"}});
+  return f;
+}
+
+std::vector<Field> iteration_fields(const HipCaseResult& r, size_t k) {
+  const TimingSummary t = summarize(r.seconds);
+  const size_t evaluations = r.seconds.size();
+  std::vector<Field> f = case_fields(r);
+  append(f, basis_fields(r));
+  append(f, {{"order", num(r.order)},
+             {"condition", num(r.condition)},
+             {"warmups", num(r.warmups)},
+             {"evaluations", num(evaluations)},
+             {"timing_evaluations", num(evaluations)}});
+  append(f, reference_timing_fields());
+  append(f, paired_fields());
+  append(f, phase_fields(t));
+  append(f, {{"iteration", num(k)},
+             {"residual", num(k < r.residual_history.size() ? r.residual_history[k] : 0.0)},
+             {"forward_error",
+              num(k < r.forward_estimate_history.size() ? r.forward_estimate_history[k] : 0.0)}});
+  return f;
+}
+
+void write_names(std::ostream& out, const std::vector<Field>& f) {
+  for (size_t i = 0; i < f.size(); ++i) out << (i ? "," : "") << f[i].first;
+  out << '\n';
+}
+void write_values(std::ostream& out, const std::vector<Field>& f) {
+  for (size_t i = 0; i < f.size(); ++i) out << (i ? "," : "") << f[i].second;
+  out << '\n';
+}
+
+HipCaseResult header_case() {
+  HipCaseResult r;
+  r.basis_precision = r.correction_precision = "fp32";
+  r.residual_history = {0.0};
+  r.forward_estimate_history = {0.0};
+  return r;
+}
+
+}  // namespace
+
+void writeSummaryHeader(std::ostream& output) { write_names(output, summary_fields(header_case())); }
+
+void writeIterationsHeader(std::ostream& output) {
+  write_names(output, iteration_fields(header_case(), 0));
+}
+
+void writeCaseCsv(const HipCaseResult& result, std::ostream& summary, std::ostream& iterations) {
+  write_values(summary, summary_fields(result));
+  for (size_t k = 0; k < result.residual_history.size(); ++k)
+    write_values(iterations, iteration_fields(result, k));
 }
 
 }  // namespace msq_hip

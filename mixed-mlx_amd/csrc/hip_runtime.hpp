@@ -18,6 +18,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <ostream>
 #include <string>
 #include <vector>
 
@@ -150,7 +151,12 @@ struct HipCaseResult {
   double final_residual = 0.0;
   double final_forward_estimate = 0.0;
   double final_forward_error = -1.0;  // vs the reference root (-1: none)
-  std::vector<double> residual_history;
+  double basis_orthogonality = 0.0;   // ||V_t^T V_t - I||_F / sqrt(n), low-bit basis
+  size_t precision_escalations = 0;   // basis tier rejected for FP32
+  size_t max_corrections = 0;
+  size_t warmups = 0;
+  std::vector<double> residual_history;          // relative FP64 residual per step
+  std::vector<double> forward_estimate_history;  // first-order forward error bound per step
   std::vector<double> seconds;        // per evaluation (runTimedHipCase)
 };
 
@@ -167,5 +173,18 @@ HipCaseResult runTimedHipCase(HipRuntime& runtime, const HipSolverConfiguration&
 
 // key=value lines, as the reference's printResult (src/reporting.cpp:22).
 void printResult(const HipCaseResult& result);
+
+// CSV output with the reference's schema (writeSummaryHeader / writeCaseCsv /
+// writeIterationsHeader, src/reporting.cpp:124-166, :168-305, :305-328), so
+// HIP results drop into the reference's analysis pipeline: the same columns
+// in the same order.  Columns this backend does not measure (the reference
+// timing of the FP64 solve, the frozen-factor certificate, per-phase times)
+// are written as 0, the string columns with the values the reference writes
+// for a controlled SPD case.  iterations.csv has one row per residual step
+// (the initial root, then each correction), with the step's first-order
+// forward error bound in `forward_error`.
+void writeSummaryHeader(std::ostream& output);
+void writeIterationsHeader(std::ostream& output);
+void writeCaseCsv(const HipCaseResult& result, std::ostream& summary, std::ostream& iterations);
 
 }  // namespace msq_hip

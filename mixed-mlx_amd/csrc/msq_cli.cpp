@@ -2,12 +2,16 @@
 // application (src/application.cpp:79-215: --solver-backend, --precision,
 // --basis-precision, --orders, --conditions, --warmups, --evaluations) and
 // printing key=value results like its reporting (src/reporting.cpp:22).
+// With --output-dir it also writes summary.csv and iterations.csv in the
+// reference's column schema (src/benchmark.cpp:105-134).
 //
 //   msq_cli --solver-backend hip --orders 64,1024 --conditions 100,1e4 \
 //           --basis-precision bf16 --correction-precision bf16 --evaluations 3
 #include <cstdio>
 #include <cstdlib>
 #include <exception>
+#include <filesystem>
+#include <fstream>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -22,7 +26,7 @@ void usage() {
       "               [--precision P] [--basis-precision P] [--correction-precision P]\n"
       "               [--jacobi-sweeps S] [--off-tolerance T] [--correction-rate R]\n"
       "               [--max-iterations K] [--tolerance T] [--warmups W] [--evaluations E]\n"
-      "               [--enforce] [--device D]\n"
+      "               [--enforce] [--device D] [--output-dir DIR]\n"
       "precisions: fp32 bf16 fp8_e4m3fn_scaled mxfp4_e2m1 (products); S = 8 is the\n"
       "reference's fixed budget, 0 stops at off(A) <= T ||A||_F.\n");
 }
@@ -52,6 +56,7 @@ int main(int argc, char** argv) {
     msq_hip::HipProblem problem;
     size_t warmups = 0, evaluations = 1;
     int device = 0;
+    std::string output_dir;
     for (int i = 1; i < argc; ++i) {
       const std::string a = argv[i];
       auto take = [&](const char* msg) -> std::string {
@@ -94,9 +99,23 @@ int main(int argc, char** argv) {
         problem.enforce_policy = true;
       } else if (a == "--device") {
         device = std::stoi(take("--device requires D"));
+      } else if (a == "--output-dir") {
+        output_dir = take("--output-dir requires a directory");
       } else {
         msq_hip::fail("unknown argument " + a);
       }
+    }
+    std::ofstream summary, iterations;
+    std::filesystem::path summary_path, iterations_path;
+    if (!output_dir.empty()) {
+      std::filesystem::create_directories(output_dir);
+      summary_path = std::filesystem::path(output_dir) / "summary.csv";
+      iterations_path = std::filesystem::path(output_dir) / "iterations.csv";
+      summary.open(summary_path);
+      iterations.open(iterations_path);
+      if (!summary || !iterations) msq_hip::fail("cannot write CSV files in " + output_dir);
+      msq_hip::writeSummaryHeader(summary);
+      msq_hip::writeIterationsHeader(iterations);
     }
     msq_hip::HipRuntime runtime(device);
     for (size_t n : orders)
@@ -106,9 +125,17 @@ int main(int argc, char** argv) {
         const msq_hip::HipCaseResult r =
             msq_hip::runTimedHipCase(runtime, config, problem, warmups, evaluations);
         msq_hip::printResult(r);
+        if (summary.is_open()) msq_hip::writeCaseCsv(r, summary, iterations);
         std::printf("\n");
         std::fflush(stdout);
       }
+    if (summary.is_open()) {
+      summary.close();
+      iterations.close();
+      if (!summary || !iterations) msq_hip::fail("writing the CSV files failed");
+      std::printf("summary_csv=%s\niterations_csv=%s\n", summary_path.c_str(),
+                  iterations_path.c_str());
+    }
     return EXIT_SUCCESS;
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
