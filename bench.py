@@ -25,10 +25,11 @@ cond up to 1e6, power law k^-alpha with alpha in [0.5, 3], four clusters):
 the same paths, with the pass rate (<= 1e-6 against the known root) and the
 fallback rate of each; no roofline leg.
 
-Also reported: the roofline of the dominant kernel (batched_jacobi_kernel<256>,
-the launch of the orders 241..256 of the batch, run on its own and timed with
-HIP events on the library's stream; ``--roofline-only`` runs just these
-launches, for the profiler) and a CPU baseline: the reference's CPU plain path (1_baseline/spd.cpp:348-430,
+Also reported: the roofline of the dominant kernel (batched_jacobi_kernel<256>
+on 256 of the batch's matrices of orders 241..256, one per CU, with the
+reference's fixed 8 sweeps, run on its own and timed with HIP events on the
+library's stream; ``--roofline-only`` runs just these launches, for the
+profiler) and a CPU baseline: the reference's CPU plain path (1_baseline/spd.cpp:348-430,
 LAPACK ssyevd + BLAS, restated in oracle/solver.py) on a bounded sample of
 the same matrices, one core.
 """
@@ -49,7 +50,7 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch of the roofline kernel on the default workload: two
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a
 # 256 MiB copy gives), profiles/jacobi_n256_hbm_traffic_pmc.md.
-ROOFLINE_TRAFFIC_BYTES = 2.635e10
+ROOFLINE_TRAFFIC_BYTES = 1.753e10
 
 
 def log(*a):
@@ -61,10 +62,18 @@ def jacobi_flops(orders, sweeps):
     return sum(9.0 * s * n * n * (n - 1) for n, s in zip(orders, sweeps))
 
 
-def roofline_leg(args, mats, orders, plain, rt, stream, torch, np, capi, dev):
-    """Dominant kernel: the Jacobi launch of the orders 241..256 (one launch of
-    batched_jacobi_kernel<256>, one workgroup per matrix), timed on its own."""
-    big = [i for i, n in enumerate(orders) if n > 240]
+ROOFLINE_MATRICES = 256  # one matrix per CU
+ROOFLINE_SWEEPS = 8      # the reference's fixed sweep count (src/metal_solver.cpp:210-213)
+
+
+def roofline_leg(args, mats, orders, rt, stream, torch, np, capi, dev):
+    """Dominant kernel: batched_jacobi_kernel<256> (one workgroup per matrix,
+    the whole solve in LDS) on 256 of the batch's matrices of orders 241..256
+    -- one per CU, a single wave of workgroups -- with the reference's fixed 8
+    sweeps, so that every workgroup does the same work and the launch time is
+    the kernel's time per matrix.  In the batched solve the same kernel stops
+    per matrix and shares the machine with the other order classes."""
+    big = [i for i, n in enumerate(orders) if n > 240][:ROOFLINE_MATRICES]
     sub_orders = [orders[i] for i in big]
     _, sub_flat = capi.pack_batch([mats[i]["A"] for i in big])
     _, _, _, stotal, svtotal = capi.batch_layout(sub_orders)
@@ -72,7 +81,7 @@ def roofline_leg(args, mats, orders, plain, rt, stream, torch, np, capi, dev):
     SV = torch.zeros(stotal, dtype=torch.float32, device=dev)
     SD = torch.zeros(svtotal, dtype=torch.float32, device=dev)
     SR = torch.zeros(svtotal, dtype=torch.float32, device=dev)
-    jopt = plain.jacobi
+    jopt = capi.jacobi_options(fixed_sweeps=ROOFLINE_SWEEPS)
     reps = rt.jacobi_batched(sub_orders, SA, SV, SD, SR, jopt)  # warm
     durations = []
     for _ in range(args.roofline_reps):
@@ -90,8 +99,8 @@ def roofline_leg(args, mats, orders, plain, rt, stream, torch, np, capi, dev):
     return {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
             "traffic": ROOFLINE_TRAFFIC_BYTES if len(mats) == 4096 else None,
-            "kernel": (f"batched_jacobi_kernel<256> (orders 241..256: {len(big)} matrices, "
-                       f"sweeps mean {np.mean(sweeps):.2f} max {max(sweeps)}, "
+            "kernel": (f"batched_jacobi_kernel<256> ({len(big)} matrices of orders 241..256, "
+                       f"one per CU, {ROOFLINE_SWEEPS} fixed sweeps; "
                        f"{flops / 1e12:.3f} TFLOP per launch at 9 S n^2 (n - 1), "
                        f"{jt * 1e3:.2f} ms mean of {len(durations)} launches)")}
 
@@ -210,16 +219,16 @@ def main():
 
     plain = MS.plain_fp32_options()
     if args.roofline_only:
-        log("roofline", roofline_leg(args, mats, orders, plain, rt, stream, torch, np, capi, dev))
+        log("roofline", roofline_leg(args, mats, orders, rt, stream, torch, np, capi, dev))
         rt.close()
         return
     dt_plain, rep_plain = timed(lambda: rt.solve_batched(orders, A, X, plain))
     err_plain, pass_plain = check_all(X, "plain fp32")
-    ok_plain = sum(r["status"] == "converged" for r in rep_plain)
+    ok_plain = int(rep_plain.converged.sum())
     log(f"plain fp32: {dt_plain * 1e3:.1f} ms/step, {len(mats) / dt_plain:.0f} matrices/s, "
         f"converged {ok_plain}/{len(mats)}, mean sweeps "
-        f"{np.mean([r['sweeps'] for r in rep_plain]):.2f}, mean corrections "
-        f"{np.mean([r['corrections'] for r in rep_plain]):.2f}")
+        f"{np.mean(rep_plain.sweeps):.2f}, mean corrections "
+        f"{np.mean(rep_plain.corrections):.2f}")
 
     use_selector = not args.no_selector and SEL.WEIGHTS.exists()
     if use_selector:
@@ -241,7 +250,7 @@ def main():
     err_low, pass_low = check_all(X, label)
     log(f"{label}: {dt_low * 1e3:.1f} ms/step, {len(mats) / dt_low:.0f} "
         f"matrices/s, fallback {len(fallback)}/{len(mats)}, mean sweeps "
-        f"{np.mean([r['sweeps'] for r in rep_low]):.2f}")
+        f"{np.mean(rep_low.sweeps):.2f}")
     fixed = None
     if use_selector:
         # The method without the learned part, on the same batch.
@@ -251,13 +260,13 @@ def main():
                  "matrices_per_s": round(len(mats) / dt_fix, 2),
                  "ms_per_step": round(dt_fix * 1e3, 3), "max_rel_error": err_fix,
                  "within_1e-6": pass_fix, "fallbacks": len(fb_fix),
-                 "mean_sweeps": float(np.mean([r["sweeps"] for r in rep_fix]))}
+                 "mean_sweeps": float(np.mean(rep_fix.sweeps))}
         log(f"fixed {args.tier} tier: {dt_fix * 1e3:.1f} ms/step, "
             f"{len(mats) / dt_fix:.0f} matrices/s, fallback {len(fb_fix)}/{len(mats)}")
 
     roofline = None
     if args.workload == "uneven":
-        roofline = roofline_leg(args, mats, orders, plain, rt, stream, torch, np, capi, dev)
+        roofline = roofline_leg(args, mats, orders, rt, stream, torch, np, capi, dev)
         log("roofline", roofline)
 
     cpu = None if args.no_cpu else cpu_baseline(mats, args.cpu_budget)
@@ -292,8 +301,8 @@ def main():
                                  if use_selector else None),
                     "max_rel_error": err_low, "within_1e-6": pass_low,
                     "fallbacks": len(fallback),
-                    "mean_sweeps": float(np.mean([r["sweeps"] for r in rep_low])),
-                    "mean_corrections": float(np.mean([r["corrections"] for r in rep_low]))},
+                    "mean_sweeps": float(np.mean(rep_low.sweeps)),
+                    "mean_corrections": float(np.mean(rep_low.corrections))},
         "speedup_vs_plain_fp32": round(dt_plain / dt_low, 3),
         "fixed_tier": fixed,
         "roofline": roofline,

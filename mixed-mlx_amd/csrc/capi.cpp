@@ -823,6 +823,12 @@ int fp8_alpha(msq_runtime* rt, int tier, const Operand& a, const Operand& b, flo
 // One frozen-factor correction for a group (refineUpdateAndMeasure's product
 // chain, src/metal_runtime.cpp:867-928):
 //   W = V^T R ; Y = (W V) o H ; W2 = V Y ; E = W2 V^T   (E into cb.t2)
+// With x_update (the solve's binary64 iterate X) and upper (the group's plan
+// of the tiles on and above the diagonal): Y = (V^T R V) o H is symmetric, so
+// only its upper tiles are computed and stored mirrored, and the last product
+// adds its upper triangle to X at (i, j) and (j, i) instead of storing E
+// (working_add_symmetric_correction's X += sym(E), .metal:1955-1970, fused
+// into the product; E's two triangles differ only by the rounding of W2).
 // ov / ovt: V and V^T encoded in the group's tier; r32: R as float; rb: R
 // as BF16 codes (batch layout) or null.  In the BF16 tier the intermediate
 // products store their codes from the GEMM epilogue (the same RNE codec as
@@ -832,8 +838,10 @@ int correction_products(msq_runtime* rt, int tier, const Layout& L,
                         const std::vector<uint32_t>& members, const msq::Segment* dsegs,
                         const GemmPlan& plan, const Operand& ov, const Operand& ovt,
                         const float* r32, const uint16_t* rb, const float* roots,
-                        const int* active, CorrectionBuffers& cb) {
+                        const int* active, CorrectionBuffers& cb, double* x_update = nullptr,
+                        const GemmPlan* upper = nullptr) {
   const uint32_t count = static_cast<uint32_t>(L.order.size());
+  const bool fused = x_update && upper;
   const msq::GemmType type = gemm_type(tier);
   const bool direct = tier == MSQ_BF16;
   const float* alpha = nullptr;
@@ -872,8 +880,9 @@ int correction_products(msq_runtime* rt, int tier, const Layout& L,
   MSQ_TRY(fp8_alpha(rt, tier, opa, ov, cb.alpha, count, &alpha));
   g = product_args(opa, ov, cb.t2, alpha, active);
   g.aux = roots;
+  g.mirror = fused ? 1 : 0;
   out_to(g, cb.enc_b, cb.t2);
-  MSQ_TRY(run_gemm(rt, plan, type, msq::Epilogue::Hadamard, g));
+  MSQ_TRY(run_gemm(rt, fused ? *upper : plan, type, msq::Epilogue::Hadamard, g));
   // W2^T storage <- V Y, row-major
   MSQ_TRY(operand_of(cb.enc_b, cb.t2, opr));
   MSQ_TRY(fp8_alpha(rt, tier, ovt, opr, cb.alpha, count, &alpha));
@@ -884,8 +893,9 @@ int correction_products(msq_runtime* rt, int tier, const Layout& L,
   // E <- W2 V^T
   MSQ_TRY(operand_of(cb.enc_a, cb.t1, opa));
   MSQ_TRY(fp8_alpha(rt, tier, opa, ovt, cb.alpha, count, &alpha));
-  g = product_args(opa, ovt, cb.t2, alpha, active);
-  MSQ_TRY(run_gemm(rt, plan, type, msq::Epilogue::Store, g));
+  g = product_args(opa, ovt, fused ? static_cast<void*>(x_update) : cb.t2, alpha, active);
+  MSQ_TRY(run_gemm(rt, fused ? *upper : plan, type,
+                   fused ? msq::Epilogue::AddSym : msq::Epilogue::Store, g));
   return MSQ_OK;
 }
 
@@ -1559,6 +1569,7 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
     std::vector<uint32_t> members;
     std::vector<msq::Segment> segs;
     GemmPlan plan;
+    GemmPlan upper;  // tiles on and above the diagonal (symmetric products)
     msq::Segment* dsegs = nullptr;
   };
   std::vector<Group> gl;
@@ -1569,6 +1580,7 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
     g.members = kv.second;
     g.segs = segments_of(L, g.members);
     build_plan(L, g.members, false, g.plan, rt, gemm_type(g.corr));
+    build_plan(L, g.members, false, g.upper, rt, gemm_type(g.corr), true);
     gl.push_back(std::move(g));
   }
   struct CorrTier {
@@ -1630,6 +1642,7 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   for (Group& g : gl) {
     S.want(g.dsegs, g.members.size());
     plan_want(S, g.plan);
+    plan_want(S, g.upper);
   }
   for (CorrTier& t : tiers) {
     S.want(t.dsegs, t.members.size());
@@ -1658,6 +1671,7 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   for (Group& g : gl) {
     MSQ_TRY(upload(rt, g.dsegs, g.segs));
     MSQ_TRY(plan_upload(rt, g.plan));
+    MSQ_TRY(plan_upload(rt, g.upper));
   }
   for (CorrTier& t : tiers) MSQ_TRY(upload(rt, t.dsegs, t.segs));
 
@@ -1682,10 +1696,13 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
         const float* alpha = nullptr;
         MSQ_TRY(encode_operand(rt, g.basis, L, g.members, g.dsegs, V, cb.enc_a, ob));
         MSQ_TRY(fp8_alpha(rt, g.basis, ob, ob, cb.alpha, count, &alpha));
-        msq::GemmArgs gg = product_args(ob, ob, cb.t2, alpha, active);
-        MSQ_TRY(run_gemm(rt, g.plan, gemm_type(g.basis), msq::Epilogue::Store, gg));
+        // Gram product V_t^T V_t reduced to ||V_t^T V_t - I||_F^2 in its
+        // epilogue (symmetric: the upper tiles, off-diagonal ones twice).
         HIP_TRY(msq::launch_zero_f64(orth2, count, rt->stream));
-        HIP_TRY(msq::launch_orthogonality(g.dsegs, gm, L.max_ld, cb.t2, orth2, rt->stream));
+        msq::GemmArgs gg = product_args(ob, ob, nullptr, alpha, active);
+        gg.norms = orth2;
+        gg.mirror = 1;
+        MSQ_TRY(run_gemm(rt, g.upper, gemm_type(g.basis), msq::Epilogue::Gram, gg));
         HIP_TRY(msq::launch_basis_escalation(g.dsegs, gm, orth2, basis_tolerance(g.basis), active,
                                              esc, st, rt->stream));
       }
@@ -1714,10 +1731,7 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
         for (const CorrTier& c : tiers)
           if (c.tier == g.corr) t = &c;
         MSQ_TRY(correction_products(rt, g.corr, L, g.members, g.dsegs, g.plan, t->ov, t->ovt, R32,
-                                    Rb, sroot, active, cb));
-        HIP_TRY(msq::launch_add_symmetric(g.dsegs, static_cast<uint32_t>(g.members.size()),
-                                          pair_count(g.segs), cb.t2, roots_out, active,
-                                          rt->stream));
+                                    Rb, sroot, active, cb, roots_out, &g.upper));
       }
       MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, nullptr, R32, Rb, norms, active,
                              count));
@@ -1765,13 +1779,31 @@ int msq_matrix_features(msq_runtime* rt, uint32_t count, const uint32_t* orders,
     if (L.order[i] > static_cast<uint32_t>(msq::kMaxFeatureOrder))
       return set_error(MSQ_ERROR_INVALID_ARGUMENT, "order %u above %d", L.order[i],
                        msq::kMaxFeatureOrder);
-  const std::vector<msq::Segment> segs = segments_of(L, iota_members(count));
+  // Segments grouped by order range, one launch per range: the ranges run
+  // concurrently on the runtime's lanes.
+  static_assert(kLanes >= 4, "one lane per order range");
+  const int kinds[4] = {256, 128, 64, 0};
+  std::vector<uint32_t> members;
+  uint32_t bounds[5] = {0, 0, 0, 0, 0};
+  for (int k = 0; k < 4; ++k) {
+    for (uint32_t i = 0; i < count; ++i)
+      if (msq::feature_rows_for_order(L.order[i]) == kinds[k]) members.push_back(i);
+    bounds[k + 1] = static_cast<uint32_t>(members.size());
+  }
+  const std::vector<msq::Segment> segs = segments_of(L, members);
   Scratch S{rt};
   msq::Segment* dsegs = nullptr;
   S.want(dsegs, count);
   MSQ_TRY(S.commit());
   MSQ_TRY(upload(rt, dsegs, segs));
-  HIP_TRY(msq::launch_features(dsegs, count, matrices, features, rt->stream));
+  HIP_TRY(hipEventRecord(rt->fork, rt->stream));
+  for (int k = 0; k < 4; ++k) {
+    HIP_TRY(hipStreamWaitEvent(rt->lanes[k], rt->fork, 0));
+    HIP_TRY(msq::launch_features(kinds[k], dsegs + bounds[k], bounds[k + 1] - bounds[k], matrices,
+                                 features, rt->lanes[k]));
+    HIP_TRY(hipEventRecord(rt->joins[k], rt->lanes[k]));
+    HIP_TRY(hipStreamWaitEvent(rt->stream, rt->joins[k], 0));
+  }
   return MSQ_OK;
 }
 

@@ -85,8 +85,12 @@ constexpr int kPowerIterationsMax = 16;  // lambda_max: power iteration on A
 constexpr int kPowerIterationsMin = 24;  // lambda_min: on (c I - A), c > lambda_max
 
 struct Segment;
-hipError_t launch_features(const Segment* segs, uint32_t count, const double* a64,
+// Features of the segments of one order range: rows = 64 (orders 1..64),
+// 128 (65..128), 256 (129..256) or 0 (257..512); segments outside the range
+// are skipped.  Row sg.index of `features` receives segment sg's 16 values.
+hipError_t launch_features(int rows, const Segment* segs, uint32_t count, const double* a64,
                            float* features, hipStream_t stream);
+int feature_rows_for_order(uint32_t order);
 hipError_t launch_mlp_forward(const float* params, const float* x, int batch, float* logits,
                               float* h1, float* h2, hipStream_t stream);
 hipError_t launch_mlp_backward(const float* params, const float* x, const float* h1,
@@ -147,6 +151,11 @@ enum class Epilogue : int {
   Store = 0,     // C (or C^T) as float (double for F64), times alpha
   Hadamard = 1,  // C o H, H_ij = 1 / (s_i + s_j) inside the order, 0 outside
   Residual = 2,  // F64 only: R = A - C (double) and R as float, ||R||_F^2
+  AddSym = 3,    // X += C on and above the diagonal, mirrored below (X binary64
+                 // in c, inside the order); the plan lists the tiles with tile
+                 // row <= tile column
+  Gram = 4,      // no store: ||C - I||_F^2 inside the order into norms (with
+                 // mirror, off-diagonal tiles of the upper list count twice)
 };
 
 // One matrix of a batched GEMM launch: C = A B with both operands stored
@@ -184,7 +193,10 @@ struct GemmArgs {
                              // codec of msq_quantize)
   int mirror;                // Residual: the plan lists tiles with tile row <=
                              // tile column; off-diagonal tiles also store their
-                             // transpose and count twice in the norm
+                             // transpose and count twice in the norm.
+                             // Hadamard: the same tile list; entries on and
+                             // above the diagonal are stored at (i, j) and
+                             // (j, i) (a symmetric product)
 };
 
 hipError_t launch_gemm(GemmType type, Epilogue epi, int tile, const GemmArgs& args,
@@ -209,9 +221,6 @@ struct Segment {
 hipError_t launch_symmetrize_to_f64(const Segment* segs, uint32_t count, uint32_t pairs,
                                     const float* c, double* x, const int* active,
                                     hipStream_t stream);
-hipError_t launch_add_symmetric(const Segment* segs, uint32_t count, uint32_t pairs,
-                                const float* e, double* x, const int* active,
-                                hipStream_t stream);
 hipError_t launch_scale_rows_transpose(const Segment* segs, uint32_t count, uint32_t pairs,
                                        const float* v, const float* roots, int use_sqrt,
                                        float* out, hipStream_t stream);

@@ -152,23 +152,43 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
   const int kend = order < ld ? ((order + 31) & ~31) : ld;
   const size_t row_bytes = MX ? static_cast<size_t>(ld) / 2
                               : static_cast<size_t>(ld) * (Tr::BITS / 8);
+  // Software pipeline: the 16-byte chunks of the next K step are loaded into
+  // registers while the MFMAs of this step run on LDS.
+  constexpr int TOTAL = (BM + BN) * CHUNKS;
+  constexpr int CPT = (TOTAL + NT - 1) / NT;  // chunks per thread per step
+  uint4 pre[CPT];
+  auto global_load = [&](int kk) {
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+      const int c = tid + u * NT;
+      pre[u] = make_uint4(0, 0, 0, 0);
+      if (c < TOTAL) {
+        const int r = c / CHUNKS;
+        const int ch = c % CHUNKS;
+        const bool is_a = r < BM;
+        const int g = (is_a ? tm : tn) + (is_a ? r : r - BM);
+        const int k0 = kk + ch * (128 / Tr::BITS);  // first element of the chunk
+        if (g < ld && k0 < kend) {
+          const unsigned char* src = (is_a ? abase : bbase) + static_cast<size_t>(g) * row_bytes +
+                                     (MX ? static_cast<size_t>(k0) / 2
+                                         : static_cast<size_t>(k0) * (Tr::BITS / 8));
+          pre[u] = *reinterpret_cast<const uint4*>(src);
+        }
+      }
+    }
+  };
+  global_load(0);
   for (int kk = 0; kk < kend; kk += Tr::KS) {
     // Stage A (BM rows) and B (BN rows) of this K step.
-    for (int c = tid; c < (BM + BN) * CHUNKS; c += NT) {
-      const int r = c / CHUNKS;
-      const int ch = c % CHUNKS;
-      const bool is_a = r < BM;
-      const int rr = is_a ? r : r - BM;
-      const int g = (is_a ? tm : tn) + rr;
-      const int k0 = kk + ch * (128 / Tr::BITS);  // first element of the chunk
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (g < ld && k0 < kend) {
-        const unsigned char* src = (is_a ? abase : bbase) + static_cast<size_t>(g) * row_bytes +
-                                   (MX ? static_cast<size_t>(k0) / 2
-                                       : static_cast<size_t>(k0) * (Tr::BITS / 8));
-        v = *reinterpret_cast<const uint4*>(src);
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+      const int c = tid + u * NT;
+      if (c < TOTAL) {
+        const int r = c / CHUNKS;
+        const int ch = c % CHUNKS;
+        const bool is_a = r < BM;
+        *reinterpret_cast<uint4*>((is_a ? As : Bs) + (is_a ? r : r - BM) * RP + ch * 16) = pre[u];
       }
-      *reinterpret_cast<uint4*>((is_a ? As : Bs) + rr * RP + ch * 16) = v;
     }
     if constexpr (MX) {
       for (int c = tid; c < (BM + BN) * 4; c += NT) {
@@ -185,6 +205,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
       }
     }
     __syncthreads();
+    if (kk + Tr::KS < kend) global_load(kk + Tr::KS);
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int ar = wm + 16 * i + row16;
@@ -243,7 +264,21 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
             if (mirror) args.c_bf16[job.c_offset + midx] = encode_bf16(static_cast<float>(mres));
           }
           if (row < order && col < order) norm2 += res * res + mres * mres;
+        } else if constexpr (EPI == Epilogue::Gram) {
+          if (row >= order || col >= order) continue;
+          const double d = static_cast<double>(static_cast<float>(acc[i][j][r]) * alpha) -
+                           (row == col ? 1.0 : 0.0);
+          norm2 += (args.mirror && tm != tn ? 2.0 : 1.0) * d * d;
+        } else if constexpr (EPI == Epilogue::AddSym) {
+          // Upper tiles only: entry (row, col), row <= col, is added at
+          // (row, col) and (col, row), so X stays exactly symmetric.
+          if (row > col || col >= order) continue;
+          double* X = static_cast<double*>(args.c) + job.c_offset;
+          const double e = static_cast<double>(static_cast<float>(acc[i][j][r]) * alpha);
+          X[static_cast<size_t>(row) + static_cast<size_t>(col) * ld] += e;
+          if (row != col) X[static_cast<size_t>(col) + static_cast<size_t>(row) * ld] += e;
         } else {
+          if (EPI == Epilogue::Hadamard && args.mirror && row > col) continue;
           Out v = static_cast<Out>(acc[i][j][r]);
           if constexpr (T != GemmType::F64) v *= alpha;
           if constexpr (EPI == Epilogue::Hadamard) {
@@ -254,19 +289,26 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
           const size_t idx = args.transpose_c
                                  ? static_cast<size_t>(col) + static_cast<size_t>(row) * ld
                                  : static_cast<size_t>(row) + static_cast<size_t>(col) * ld;
+          // Symmetric Hadamard product: the mirror entry (col, row).
+          const bool mirrored = EPI == Epilogue::Hadamard && args.mirror && row != col;
+          const size_t midx = static_cast<size_t>(col) + static_cast<size_t>(row) * ld;
           if constexpr (T == GemmType::F64) {
             C[idx] = v;
           } else {
-            if (args.c_bf16)
-              args.c_bf16[job.c_offset + idx] = encode_bf16(static_cast<float>(v));
-            else
+            if (args.c_bf16) {
+              const uint16_t code = encode_bf16(static_cast<float>(v));
+              args.c_bf16[job.c_offset + idx] = code;
+              if (mirrored) args.c_bf16[job.c_offset + midx] = code;
+            } else {
               C[idx] = v;
+              if (mirrored) C[midx] = v;
+            }
           }
         }
       }
     }
   }
-  if constexpr (EPI == Epilogue::Residual) {
+  if constexpr (EPI == Epilogue::Residual || EPI == Epilogue::Gram) {
     for (int off = 32; off > 0; off >>= 1) norm2 += __shfl_xor(norm2, off);
     if (lane == 0) atomicAdd(args.norms + job.status_index, norm2);
   }
@@ -286,6 +328,29 @@ hipError_t launch_typed(int tile, const GemmArgs& args, uint32_t tiles, hipStrea
   return hipGetLastError();
 }
 
+template <GemmType T>
+hipError_t launch_epilogue(Epilogue epi, int tile, const GemmArgs& args, uint32_t tiles,
+                           hipStream_t stream) {
+  switch (epi) {
+    case Epilogue::Store: return launch_typed<T, Epilogue::Store>(tile, args, tiles, stream);
+    case Epilogue::Hadamard: return launch_typed<T, Epilogue::Hadamard>(tile, args, tiles, stream);
+    case Epilogue::AddSym: return launch_typed<T, Epilogue::AddSym>(tile, args, tiles, stream);
+    case Epilogue::Gram: return launch_typed<T, Epilogue::Gram>(tile, args, tiles, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t dispatch_epilogue(GemmType type, Epilogue epi, int tile, const GemmArgs& args,
+                             uint32_t tiles, hipStream_t stream) {
+  switch (type) {
+    case GemmType::F32: return launch_epilogue<GemmType::F32>(epi, tile, args, tiles, stream);
+    case GemmType::Bf16: return launch_epilogue<GemmType::Bf16>(epi, tile, args, tiles, stream);
+    case GemmType::Fp8: return launch_epilogue<GemmType::Fp8>(epi, tile, args, tiles, stream);
+    case GemmType::Mxfp4: return launch_epilogue<GemmType::Mxfp4>(epi, tile, args, tiles, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 }  // namespace
 
 hipError_t launch_gemm(GemmType type, Epilogue epi, int tile, const GemmArgs& args,
@@ -295,28 +360,11 @@ hipError_t launch_gemm(GemmType type, Epilogue epi, int tile, const GemmArgs& ar
     if (type != GemmType::F64) return hipErrorInvalidValue;
     return launch_typed<GemmType::F64, Epilogue::Residual>(tile, args, tiles, stream);
   }
-  switch (type) {
-    case GemmType::F64:
-      if (epi != Epilogue::Store) return hipErrorInvalidValue;
-      return launch_typed<GemmType::F64, Epilogue::Store>(tile, args, tiles, stream);
-    case GemmType::F32:
-      return epi == Epilogue::Store
-                 ? launch_typed<GemmType::F32, Epilogue::Store>(tile, args, tiles, stream)
-                 : launch_typed<GemmType::F32, Epilogue::Hadamard>(tile, args, tiles, stream);
-    case GemmType::Bf16:
-      return epi == Epilogue::Store
-                 ? launch_typed<GemmType::Bf16, Epilogue::Store>(tile, args, tiles, stream)
-                 : launch_typed<GemmType::Bf16, Epilogue::Hadamard>(tile, args, tiles, stream);
-    case GemmType::Fp8:
-      return epi == Epilogue::Store
-                 ? launch_typed<GemmType::Fp8, Epilogue::Store>(tile, args, tiles, stream)
-                 : launch_typed<GemmType::Fp8, Epilogue::Hadamard>(tile, args, tiles, stream);
-    case GemmType::Mxfp4:
-      return epi == Epilogue::Store
-                 ? launch_typed<GemmType::Mxfp4, Epilogue::Store>(tile, args, tiles, stream)
-                 : launch_typed<GemmType::Mxfp4, Epilogue::Hadamard>(tile, args, tiles, stream);
-  }
-  return hipErrorInvalidValue;
+  if (type == GemmType::F64)
+    return epi == Epilogue::Store
+               ? launch_typed<GemmType::F64, Epilogue::Store>(tile, args, tiles, stream)
+               : hipErrorInvalidValue;
+  return dispatch_epilogue(type, epi, tile, args, tiles, stream);
 }
 
 }  // namespace msq

@@ -1,9 +1,10 @@
 // Elementwise and reduction kernels of the solve, batched over uneven
 // segments (grid.y = segment).
 //
-// Reference counterparts: working_from_symmetric_f32 and
-// working_add_symmetric_correction (metal_mixed_sqrt.metal:1911-1925,
-// :1955-1970; float-float there, binary64 here), scale_columns_f32 (:21-37),
+// Reference counterparts: working_from_symmetric_f32 (metal_mixed_sqrt.metal:
+// 1911-1925; float-float there, binary64 here; the correction's
+// working_add_symmetric_correction, :1955-1970, is fused into the last
+// correction product, kernels_gemm.hip), scale_columns_f32 (:21-37),
 // transpose_square_f32 (:1413-1421), and the host reduction of
 // measureF32Orthogonality (src/metal_runtime.cpp:765-775), done on the device.
 #include <hip/hip_runtime.h>
@@ -96,30 +97,6 @@ __global__ __launch_bounds__(256) void symmetrize_to_f64_kernel(const Segment* _
                               static_cast<double>(P[j + kTilePitch * i]));
       xm[(kTile * pt.tj + i) + static_cast<size_t>(kTile * pt.ti + j) * ld] = t;
     }
-  }
-}
-
-// x += (e + e^T) / 2.
-__global__ __launch_bounds__(256) void add_symmetric_kernel(const Segment* __restrict__ segs,
-                                                            uint32_t count,
-                                                            const float* __restrict__ ecorr,
-                                                            double* __restrict__ x,
-                                                            const int* __restrict__ active) {
-  __shared__ float P[kTile * kTilePitch], Q[kTile * kTilePitch];
-  const PairTile pt = pair_tile(segs, count);
-  if (active && !active[pt.sg.index]) return;
-  load_pair(ecorr, pt, P, Q);
-  const size_t ld = pt.sg.ld;
-  double* xm = x + pt.sg.offset;
-  for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
-    const int i = e % kTile, j = e / kTile;
-    xm[(kTile * pt.ti + i) + static_cast<size_t>(kTile * pt.tj + j) * ld] +=
-        0.5 * (static_cast<double>(P[i + kTilePitch * j]) +
-               static_cast<double>(Q[j + kTilePitch * i]));
-    if (pt.ti != pt.tj)
-      xm[(kTile * pt.tj + i) + static_cast<size_t>(kTile * pt.ti + j) * ld] +=
-          0.5 * (static_cast<double>(Q[i + kTilePitch * j]) +
-                 static_cast<double>(P[j + kTilePitch * i]));
   }
 }
 
@@ -356,13 +333,6 @@ hipError_t launch_symmetrize_to_f64(const Segment* segs, uint32_t count, uint32_
                                     hipStream_t stream) {
   if (!count || !pairs) return hipSuccess;
   symmetrize_to_f64_kernel<<<pairs, kT, 0, stream>>>(segs, count, c, x, active);
-  return hipGetLastError();
-}
-hipError_t launch_add_symmetric(const Segment* segs, uint32_t count, uint32_t pairs,
-                                const float* e, double* x, const int* active,
-                                hipStream_t stream) {
-  if (!count || !pairs) return hipSuccess;
-  add_symmetric_kernel<<<pairs, kT, 0, stream>>>(segs, count, e, x, active);
   return hipGetLastError();
 }
 hipError_t launch_scale_rows_transpose(const Segment* segs, uint32_t count, uint32_t pairs,

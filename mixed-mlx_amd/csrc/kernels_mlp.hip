@@ -267,7 +267,7 @@ __global__ __launch_bounds__(feat_threads(ROWS)) void features_kernel(
     const double lmax_c = fmax(lmax, 1e-300);
     const double lmin_c = fmax(lmin, lmax_c * 1e-16);
     const double mean_logd = logd / n;
-    float* f = features + static_cast<size_t>(blockIdx.x) * kIn;
+    float* f = features + static_cast<size_t>(sg.index) * kIn;
     f[0] = static_cast<float>(n / 256.0);
     f[1] = static_cast<float>(log2(static_cast<double>(n)) / 8.0);
     f[2] = static_cast<float>(log10(lmax_c / lmin_c));
@@ -430,19 +430,20 @@ __global__ void mlp_adam_kernel(float* __restrict__ params, const float* __restr
 
 }  // namespace
 
-hipError_t launch_features(const Segment* segs, uint32_t count, const double* a64,
+int feature_rows_for_order(uint32_t order) {
+  return order <= 64 ? 64 : order <= 128 ? 128 : order <= 256 ? 256 : 0;
+}
+
+hipError_t launch_features(int rows, const Segment* segs, uint32_t count, const double* a64,
                            float* features, hipStream_t stream) {
   if (count == 0) return hipSuccess;
-  features_kernel<64><<<count, feat_threads(64), 0, stream>>>(segs, a64, features);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  features_kernel<128><<<count, feat_threads(128), 0, stream>>>(segs, a64, features);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  features_kernel<256><<<count, feat_threads(256), 0, stream>>>(segs, a64, features);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  features_kernel<0><<<count, feat_threads(0), 0, stream>>>(segs, a64, features);
+  switch (rows) {
+    case 64: features_kernel<64><<<count, feat_threads(64), 0, stream>>>(segs, a64, features); break;
+    case 128: features_kernel<128><<<count, feat_threads(128), 0, stream>>>(segs, a64, features); break;
+    case 256: features_kernel<256><<<count, feat_threads(256), 0, stream>>>(segs, a64, features); break;
+    case 0: features_kernel<0><<<count, feat_threads(0), 0, stream>>>(segs, a64, features); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

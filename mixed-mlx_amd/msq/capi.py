@@ -190,8 +190,8 @@ def _addr(x):
 
 
 def _u32_array(values):
-    arr = (ctypes.c_uint32 * len(values))(*[int(v) for v in values])
-    return arr
+    # The pointer keeps a reference to the array for the call's duration.
+    return np.ascontiguousarray(values, dtype=np.uint32).ctypes.data_as(_u32p)
 
 
 # ---------------------------------------------------------------- layout
@@ -251,21 +251,98 @@ def solve_options(basis=FP32, correction=FP32, jacobi=None, min_corrections=0,
                         residual_tolerance, forward_tolerance)
 
 
-def report_dict(r: SolveReport):
-    steps = r.corrections + 1
+def _struct_dtype(struct):
+    """numpy dtype with the field offsets of a ctypes structure."""
+    names, formats, offsets = [], [], []
+    for name, ctype in struct._fields_:
+        names.append(name)
+        offsets.append(getattr(struct, name).offset)
+        if issubclass(ctype, ctypes.Array):
+            formats.append((np.dtype(ctype._type_), (ctype._length_,)))
+        else:
+            formats.append(np.dtype(ctype))
+    return np.dtype({"names": names, "formats": formats, "offsets": offsets,
+                     "itemsize": ctypes.sizeof(struct)})
+
+
+REPORT_DTYPE = _struct_dtype(SolveReport)
+CONVERGED = 1
+
+
+def report_dict(r):
+    """One report as a dict (r: a SolveReport or a REPORT_DTYPE record)."""
+    corrections = int(r["corrections"] if isinstance(r, np.void) else r.corrections)
+    get = (lambda k: r[k]) if isinstance(r, np.void) else (lambda k: getattr(r, k))
+    steps = corrections + 1
+    sweeps = int(get("sweeps"))
+    status = int(get("status"))
     return {
-        "order": r.order, "sweeps": r.sweeps, "corrections": r.corrections,
-        "status": SOLVE_STATUS.get(r.status, str(r.status)),
-        "matrix_norm": r.matrix_norm, "min_root": r.min_root, "off_norm": r.off_norm,
-        "final_residual": r.final_residual,
-        "final_forward_estimate": r.final_forward_estimate,
-        "residuals": list(r.residuals[:steps]),
-        "forward_estimates": list(r.forward_estimates[:steps]),
-        "off_history": list(r.off_history[:r.sweeps]),
-        "correction_rate": r.correction_rate,
-        "basis_orthogonality": r.basis_orthogonality,
-        "precision_escalations": r.precision_escalations,
+        "order": int(get("order")), "sweeps": sweeps, "corrections": corrections,
+        "status": SOLVE_STATUS.get(status, str(status)),
+        "matrix_norm": float(get("matrix_norm")), "min_root": float(get("min_root")),
+        "off_norm": float(get("off_norm")),
+        "final_residual": float(get("final_residual")),
+        "final_forward_estimate": float(get("final_forward_estimate")),
+        "residuals": [float(x) for x in get("residuals")[:steps]],
+        "forward_estimates": [float(x) for x in get("forward_estimates")[:steps]],
+        "off_history": [float(x) for x in get("off_history")[:sweeps]],
+        "correction_rate": float(get("correction_rate")),
+        "basis_orthogonality": float(get("basis_orthogonality")),
+        "precision_escalations": int(get("precision_escalations")),
     }
+
+
+class Reports:
+    """The per-matrix reports of one batched solve.
+
+    A sequence of dicts to the caller (``reports[i]["status"]``, iteration),
+    stored as one numpy record array: building a dict per matrix costs more
+    host time than the solve of a small matrix, so it happens only for the
+    entries that are read.  Whole-batch fields are numpy arrays:
+    ``status``, ``converged``, ``sweeps``, ``corrections``."""
+
+    def __init__(self, records):
+        self.records = records
+        self.fallback_from = {}
+
+    def __len__(self):
+        return len(self.records)
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[k] for k in range(*i.indices(len(self)))]
+        i = int(i)
+        if i < 0:
+            i += len(self)
+        d = report_dict(self.records[i])
+        if i in self.fallback_from:
+            d["fallback_from"] = self.fallback_from[i]
+        return d
+
+    def __iter__(self):
+        return (self[i] for i in range(len(self)))
+
+    @property
+    def status(self):
+        return self.records["status"]
+
+    @property
+    def converged(self):
+        return self.records["status"] == CONVERGED
+
+    @property
+    def sweeps(self):
+        return self.records["sweeps"]
+
+    @property
+    def corrections(self):
+        return self.records["corrections"]
+
+    def replace(self, i, other, k):
+        """Entry i becomes entry k of ``other`` (a re-solve of the matrix);
+        the replaced report is kept as its ``fallback_from``."""
+        self.fallback_from[int(i)] = self[int(i)]
+        self.records[int(i)] = other.records[int(k)]
 
 
 # ---------------------------------------------------------------- runtime
@@ -382,10 +459,13 @@ class Runtime:
     def solve_batched(self, orders, matrices, roots, options, reports=True):
         opts = options if isinstance(options, (list, tuple)) else [options]
         arr = (SolveOptions * len(opts))(*opts)
-        reps = (SolveReport * len(orders))() if reports else None
+        """Reports: a ``Reports`` sequence (one dict-like entry per matrix)."""
+        recs = np.zeros(len(orders), dtype=REPORT_DTYPE) if reports else None
         check(lib.msq_solve_batched(self.handle, len(orders), _u32_array(orders),
-                                    _addr(matrices), _addr(roots), arr, len(opts), reps))
-        return [report_dict(r) for r in reps] if reports else None
+                                    _addr(matrices), _addr(roots), arr, len(opts),
+                                    recs.ctypes.data_as(ctypes.POINTER(SolveReport))
+                                    if reports else None))
+        return Reports(recs) if reports else None
 
     def last_factors(self):
         v = _p()

@@ -128,10 +128,10 @@ def label_batch(rt, mats, target=MS.TARGET, log=None):
             runs.append((time.perf_counter() - t0) * 1e3)
         tier_ms[k] = float(np.median(runs))
         err = true_errors(X, mats, offs, lds)
-        ok = np.array([r["status"] == "converged" for r in reps]) & (err <= target)
+        ok = reps.converged & (err <= target)
         success[:, k] = ok
-        sweeps[:, k] = [r["sweeps"] for r in reps]
-        corrections[:, k] = [r["corrections"] for r in reps]
+        sweeps[:, k] = reps.sweeps
+        corrections[:, k] = reps.corrections
         if log:
             log(f"  tier {TIERS[k][0]}: success {ok.mean():.3f}, mean sweeps "
                 f"{sweeps[:, k].mean():.2f}, mean corrections {corrections[:, k].mean():.2f}, "
@@ -299,6 +299,9 @@ def solve_selected(rt, selector, orders, matrices, roots, candidates=BATCH_CANDI
     else:
         tiers = selector.choose(prob, orders, candidates=candidates)
     opts_all = [tier_options(int(k)) for k in range(len(TIERS))]
-    opts = [opts_all[int(k)] for k in tiers]
+    if np.all(tiers == tiers[0]):  # one record for the whole batch
+        opts = opts_all[int(tiers[0])]
+    else:
+        opts = [opts_all[int(k)] for k in tiers]
     reports, fallback = MS.solve_with_fallback(rt, orders, matrices, roots, opts)
     return reports, fallback, tiers

@@ -67,7 +67,7 @@ def solve_with_fallback(runtime, orders, matrices, roots, options, packed_host=N
     import torch
 
     reports = runtime.solve_batched(orders, matrices, roots, options)
-    failed = [i for i, r in enumerate(reports) if r["status"] != "converged"]
+    failed = np.flatnonzero(~reports.converged).tolist()
     if not failed:
         return reports, failed
     lds, offs, _, _, _ = capi.batch_layout(orders)
@@ -82,6 +82,5 @@ def solve_with_fallback(runtime, orders, matrices, roots, options, packed_host=N
     for k, i in enumerate(failed):
         size = int(lds[i]) ** 2
         roots[offs[i]:offs[i] + size] = sub_x[soffs[k]:soffs[k] + size]
-        sub_reports[k]["fallback_from"] = reports[i]
-        reports[i] = sub_reports[k]
+        reports.replace(i, sub_reports, k)
     return reports, failed

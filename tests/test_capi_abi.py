@@ -43,3 +43,32 @@ def test_batch_layout():
     assert total == 1024 + 4096 + 4096 and vtotal == 160
     orders, flat = capi.pack_batch([np.eye(2) * 3 for np in [__import__("numpy")]])
     assert orders == [2] and flat[0] == 3 and flat[33] == 3
+
+
+def test_report_records_match_the_c_struct():
+    """The record array that msq_solve_batched fills has the C struct's layout,
+    and a record reads back as the same dict as the ctypes structure."""
+    import ctypes
+
+    import numpy as np
+
+    from msq import capi
+    dt = capi.REPORT_DTYPE
+    assert dt.itemsize == ctypes.sizeof(capi.SolveReport)
+    for name, _ in capi.SolveReport._fields_:
+        assert dt.fields[name][1] == getattr(capi.SolveReport, name).offset, name
+    r = capi.SolveReport()
+    r.order, r.sweeps, r.corrections, r.status = 40, 3, 2, 1
+    r.matrix_norm, r.final_residual = 2.5, 1e-9
+    for k in range(3):
+        r.residuals[k] = 10.0 ** -k
+        r.off_history[k] = 0.5 ** k
+    rec = np.frombuffer(ctypes.string_at(ctypes.addressof(r), ctypes.sizeof(r)), dtype=dt)
+    assert capi.report_dict(rec[0]) == capi.report_dict(r)
+    reps = capi.Reports(np.concatenate([rec, rec]))
+    reps.records["status"][1] = 2
+    assert list(reps.converged) == [True, False] and reps[1]["status"] == "budget_exhausted"
+    assert [d["corrections"] for d in reps] == [2, 2]
+    reps.replace(1, capi.Reports(rec.copy()), 0)
+    assert reps.converged.all() and reps[1]["fallback_from"]["status"] == "budget_exhausted"
+    assert reps[0]["residuals"] == [1.0, 0.1, 0.01] and reps[-1]["off_history"] == [1.0, 0.5, 0.25]
