@@ -19,6 +19,12 @@ Two paths are timed on the same device-resident batch:
 checked against the FP64 root Q diag(sqrt(lambda)) Q^T.  With the selector,
 the fixed BF16 tier (rate --rate) is timed as well and reported beside it.
 
+``--workload configs`` runs the single-matrix configurations instead (n = 64
+FP32; n = 1024 BF16; n = 4096 with scaled-FP8 and MXFP4 products): error per
+correction step against the construction root, solve latency, and at the
+n = 4096 shape the codecs' HBM GB/s and the reconstruction product's
+TFLOP/s against the MI355X peaks.
+
 ``--workload mixture`` runs the learned-tier workload instead (default 2000
 matrices, n uniform in [32, 512], spectra one third each log-uniform with
 cond up to 1e6, power law k^-alpha with alpha in [0.5, 3], four clusters):
@@ -129,6 +135,126 @@ def cpu_baseline(mats, budget_s):
                        "reference CPU path")}
 
 
+# MI355X dense peaks (TFLOP/s) of the product types, and HBM bandwidth.
+PRODUCT_PEAKS = {"fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0, "mxfp4": 10100.0}
+HBM_PEAK_GBS = 8000.0
+
+
+def configs_leg(rt, torch, np, capi, G, MS, cpu=True):
+    """The single-matrix configurations, one JSON record each: per-step
+    relative Frobenius error against the construction root (a solve stopped
+    after k corrections, k = 0..K: the Jacobi and the products are
+    deterministic, so the k-th solve is the k-th step of the K-step one),
+    the solve's latency, and for n = 4096 the codecs' HBM bandwidth and the
+    products' MFMA throughput."""
+    from oracle import solver as S
+
+    def event_time(fn, reps=5):
+        stream = torch.cuda.current_stream()
+        fn()
+        stream.synchronize()
+        out = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            stream.synchronize()
+            out.append(e0.elapsed_time(e1) * 1e-3)
+        return float(np.median(out))
+
+    def steps(A, Xc, basis, corr, K, jacobi):
+        o, aflat = capi.pack_batch([A])
+        _, _, _, total, _ = capi.batch_layout(o)
+        Ad = torch.from_numpy(aflat).cuda()
+        Xd = torch.zeros(total, dtype=torch.float64, device="cuda")
+        errs, rep = [], None
+        for k in range(K + 1):
+            opts = capi.solve_options(basis, corr, jacobi, min_corrections=k, max_corrections=k,
+                                      residual_tolerance=0.0)
+            rep = rt.solve_batched(o, Ad, Xd, opts)[0]
+            errs.append(float(S.relative_error(capi.unpack_batch(o, Xd.cpu().numpy())[0], Xc)))
+        opts = capi.solve_options(basis, corr, jacobi, min_corrections=K, max_corrections=K,
+                                  residual_tolerance=0.0)
+        t = event_time(lambda: rt.solve_batched(o, Ad, Xd, opts), reps=3)
+        return {"error_per_step": errs, "sweeps": rep["sweeps"],
+                "residual_per_step": rep["residuals"], "seconds": round(t, 5)}
+
+    jac = capi.jacobi_options(off_tol=1e-7)
+    records = []
+    # configs[0]: n = 64, cond 1e2, FP32 end to end, one correction.
+    A, Xc, _ = G.single_matrix(0, 64, 1e-2)
+    r = {"config": "n=64 cond 1e2 fp32, 1 correction",
+         **steps(A, Xc, capi.FP32, capi.FP32, 1, jac)}
+    if cpu:
+        import time as _t
+        t0 = _t.perf_counter()
+        for _ in range(20):
+            S.accelerate_mixed(A, None, max_corrections=1, tolerance=0.0)
+        r["cpu_reference_seconds"] = round((_t.perf_counter() - t0) / 20, 6)
+    records.append(r)
+    # configs[1]: n = 1024, cond 1e4, BF16 storage and products, FP64
+    # residual, two corrections.
+    A, Xc, _ = G.single_matrix(1, 1024, 1e-4)
+    r = {"config": "n=1024 cond 1e4 bf16, 2 corrections",
+         **steps(A, Xc, capi.BF16, capi.BF16, 2, jac)}
+    if cpu:
+        import time as _t
+        t0 = _t.perf_counter()
+        S.accelerate_mixed(A, None, max_corrections=2, tolerance=0.0)
+        r["cpu_reference_seconds"] = round(_t.perf_counter() - t0, 4)
+    records.append(r)
+    # configs[2]: n = 4096, cond 1e3, products in scaled FP8 E4M3 and in
+    # MXFP4, up to three corrections: every product in the tier, and the
+    # tier's reconstruction with BF16 corrections.
+    A, Xc, _ = G.single_matrix(2, 4096, 1e-3)
+    for name, p in (("fp8", capi.FP8_E4M3FN_SCALED), ("mxfp4", capi.MXFP4_E2M1)):
+        records.append({"config": f"n=4096 cond 1e3 {name} products, 3 corrections",
+                        **steps(A, Xc, p, p, 3, jac)})
+        records.append({"config": f"n=4096 cond 1e3 {name} reconstruction, bf16 corrections",
+                        **steps(A, Xc, p, capi.BF16, 3, jac)})
+    # Codecs at the n = 4096 shape: HBM bytes 4N (float) + N bits / 8 (+ N / 32
+    # scale bytes for MX) per direction.
+    n = 4096
+    N = n * n
+    g = torch.Generator(device="cuda").manual_seed(7)
+    src = torch.randn(N, generator=g, device="cuda", dtype=torch.float32) / n ** 0.5
+    dst = torch.empty_like(src)
+    codecs = {}
+    for name, p, bits in (("bf16", capi.BF16, 16), ("fp8_e4m3fn", capi.FP8_E4M3FN, 8),
+                          ("mxfp4", capi.MXFP4_E2M1, 4)):
+        vals = torch.empty(capi.storage_bytes(p, N), dtype=torch.uint8, device="cuda")
+        scl = torch.empty(max(1, capi.scale_bytes(p, N)), dtype=torch.uint8, device="cuda")
+        nbytes = 4 * N + N * bits // 8 + capi.scale_bytes(p, N)
+        te = event_time(lambda: rt.quantize(p, src, vals, scl))
+        td = event_time(lambda: rt.decode(p, vals, scl, dst))
+        codecs[name] = {"encode_gbs": round(nbytes / te / 1e9, 1),
+                        "decode_gbs": round(nbytes / td / 1e9, 1),
+                        "bytes": nbytes}
+    vals = torch.empty(N, dtype=torch.uint8, device="cuda")
+    sig = torch.empty(1, dtype=torch.float32, device="cuda")
+    te = event_time(lambda: rt.quantize_fp8_scaled(src, vals, sig))
+    codecs["fp8_e4m3fn_scaled"] = {"encode_gbs": round((5 * N) / te / 1e9, 1), "bytes": 5 * N}
+    # Products at n = 4096: the reconstruction X0 = G G^T (2 n^3 flops), the
+    # call's row scaling and operand encode included.
+    o = [n]
+    _, _, _, total, vtotal = capi.batch_layout(o)
+    V = torch.linalg.qr(torch.randn(n, n, generator=g, device="cuda", dtype=torch.float64))[0]
+    V = V.to(torch.float32).t().contiguous().reshape(-1)
+    roots = torch.rand(vtotal, generator=g, device="cuda", dtype=torch.float32) + 0.5
+    out = torch.empty(total, dtype=torch.float32, device="cuda")
+    products = {}
+    for name, p in (("fp32", capi.FP32), ("bf16", capi.BF16), ("fp8", capi.FP8_E4M3FN_SCALED),
+                    ("mxfp4", capi.MXFP4_E2M1)):
+        t = event_time(lambda: rt.reconstruct(p, o, V, roots, out))
+        tf = 2.0 * n ** 3 / t / 1e12
+        products[name] = {"seconds": round(t, 6), "tflops": round(tf, 1),
+                          "frac_of_peak": round(tf / PRODUCT_PEAKS[name], 4)}
+    return {"configs": records, "codecs_n4096": codecs, "hbm_peak_gbs": HBM_PEAK_GBS,
+            "reconstruction_n4096": products}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,7 +262,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--count", type=int, default=None,
                     help="matrices (default 4096; mixture workload 2000)")
-    ap.add_argument("--workload", choices=["uneven", "mixture"], default="uneven")
+    ap.add_argument("--workload", choices=["uneven", "mixture", "configs"], default="uneven")
     ap.add_argument("--tier", default="bf16")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -159,6 +285,20 @@ def main():
     import torch
 
     from msq import capi, generators as G, selector as SEL, solver as MS
+
+    if args.workload == "configs":
+        torch.cuda.set_device(0)
+        rt = capi.Runtime(0)
+        rec = configs_leg(rt, torch, np, capi, G, MS, cpu=not args.no_cpu)
+        for r in rec["configs"]:
+            log(r)
+        log("codecs", rec["codecs_n4096"])
+        log("reconstruction", rec["reconstruction_n4096"])
+        print(json.dumps({"metric": "single-matrix configurations: error per step, latency, "
+                                    "codec GB/s and product TFLOP/s at n = 4096",
+                          **rec}), flush=True)
+        rt.close()
+        return
 
     t = time.perf_counter()
     if args.workload == "uneven":

@@ -900,20 +900,50 @@ int correction_products(msq_runtime* rt, int tier, const Layout& L,
 }
 
 // X0 = G G^T, G = V diag(sqrt(s)) in the tier (into cb.t2).
+//
+// two_term (MXFP4): G is carried as a two-term expansion hi + lo, hi the
+// MXFP4 rounding of G and lo the MXFP4 rounding of G - hi, and the product is
+// hi hi^T + 2 hi lo^T (lo lo^T dropped): the symmetrisation that follows,
+// (C + C^T) / 2, turns it into hi hi^T + hi lo^T + lo hi^T.  One E2M1
+// mantissa bit leaves X0 about 0.17 off the root at n = 4096, outside the
+// range where three corrections reach 1e-6; the expansion brings it to about
+// 0.03 with one more scaled MFMA product.
 int reconstruction_product(msq_runtime* rt, int tier, const Layout& L,
                            const std::vector<uint32_t>& members, const msq::Segment* dsegs,
                            const GemmPlan& plan, const float* V, const float* roots,
-                           const int* active, CorrectionBuffers& cb) {
+                           const int* active, CorrectionBuffers& cb, bool two_term = false) {
   const uint32_t count = static_cast<uint32_t>(L.order.size());
   HIP_TRY(msq::launch_scale_rows_transpose(dsegs, static_cast<uint32_t>(members.size()),
                                            pair_count(L, members),
                                            V, roots, 1, cb.t1, rt->stream));
-  Operand og;
+  Operand og, olo;
   MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, cb.t1, cb.enc_a, og));
+  two_term = two_term && tier == MSQ_MXFP4_E2M1 && !members.empty();
+  if (two_term) {
+    // lo = mxfp4(G - dec(hi)) over the members' span of the batch (the span
+    // the encode covers); cb.t2 holds dec(hi) until the products overwrite it.
+    uint64_t begin = UINT64_MAX, end = 0;
+    for (uint32_t i : members) {
+      begin = std::min<uint64_t>(begin, L.off[i]);
+      end = std::max<uint64_t>(end, L.off[i] + static_cast<uint64_t>(L.ld[i]) * L.ld[i]);
+    }
+    HIP_TRY(msq::launch_decode(msq::Format::Mxfp4E2M1,
+                               static_cast<const uint8_t*>(cb.enc_a.codes) + begin / 2,
+                               cb.enc_a.scales + begin / 32, cb.t2 + begin, end - begin,
+                               rt->stream));
+    HIP_TRY(msq::launch_subtract(cb.t1 + begin, cb.t2 + begin, cb.t1 + begin, end - begin,
+                                 rt->stream));
+    MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, cb.t1, cb.enc_b, olo));
+  }
   const float* alpha = nullptr;
   MSQ_TRY(fp8_alpha(rt, tier, og, og, cb.alpha, count, &alpha));
   msq::GemmArgs g = product_args(og, og, cb.t2, alpha, active);
   MSQ_TRY(run_gemm(rt, plan, gemm_type(tier), msq::Epilogue::Store, g));
+  if (two_term) {
+    g = product_args(og, olo, cb.t2, nullptr, active);
+    g.c_scale = 2.0f;
+    MSQ_TRY(run_gemm(rt, plan, gemm_type(tier), msq::Epilogue::Accumulate, g));
+  }
   return MSQ_OK;
 }
 
@@ -1707,7 +1737,7 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
                                              esc, st, rt->stream));
       }
       MSQ_TRY(reconstruction_product(rt, g.basis, L, g.members, g.dsegs, g.plan, V, sroot, active,
-                                     cb));
+                                     cb, /*two_term=*/true));
       if (g.basis != MSQ_FP32)  // the rejected ones again, from the FP32 basis
         MSQ_TRY(reconstruction_product(rt, MSQ_FP32, L, g.members, g.dsegs, g.plan, V, sroot, esc,
                                        cb));

@@ -156,6 +156,7 @@ enum class Epilogue : int {
                  // row <= tile column
   Gram = 4,      // no store: ||C - I||_F^2 inside the order into norms (with
                  // mirror, off-diagonal tiles of the upper list count twice)
+  Accumulate = 5,  // C += c_scale C_new (float), times alpha
 };
 
 // One matrix of a batched GEMM launch: C = A B with both operands stored
@@ -197,6 +198,7 @@ struct GemmArgs {
                              // Hadamard: the same tile list; entries on and
                              // above the diagonal are stored at (i, j) and
                              // (j, i) (a symmetric product)
+  float c_scale;             // Accumulate: C += c_scale * alpha * A B
 };
 
 hipError_t launch_gemm(GemmType type, Epilogue epi, int tile, const GemmArgs& args,
@@ -227,6 +229,8 @@ hipError_t launch_scale_rows_transpose(const Segment* segs, uint32_t count, uint
 hipError_t launch_transpose(const Segment* segs, uint32_t count, uint32_t pairs,
                             const float* v, float* out, hipStream_t stream);
 hipError_t launch_zero_f64(double* p, size_t n, hipStream_t stream);
+hipError_t launch_subtract(const float* a, const float* b, float* out, size_t n,
+                           hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t value, hipStream_t stream);
 hipError_t launch_sumsq_f64(const Segment* segs, uint32_t count, uint32_t max_ld, const double* a,
                             double* out, hipStream_t stream);

@@ -10,6 +10,8 @@
 // shuffles.  All launches are grid-stride and HBM-bound.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_codecs.hpp"
 #include "kernels.hpp"
 
@@ -299,7 +301,16 @@ __global__ void amax_kernel(const float* __restrict__ src, size_t n,
   }
   for (int off = 32; off > 0; off >>= 1)
     m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), off)));
-  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+  // One atomic per block: the grid is capped, so a large tensor costs a few
+  // hundred atomics on the one address rather than one per wave.
+  __shared__ uint32_t wave_max[kThreads / 64];
+  if ((threadIdx.x & 63) == 0) wave_max[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t b = 0;
+    for (int w = 0; w < static_cast<int>(blockDim.x) / 64; ++w) b = max(b, wave_max[w]);
+    atomicMax(out, b);
+  }
 }
 
 // sigma = (float)(448 / (double)amax), 1 when amax is 0.
@@ -391,7 +402,7 @@ hipError_t launch_decode(Format fmt, const void* values, const uint8_t* scales,
 hipError_t launch_fp8_amax_sigma(const float* src, size_t n, uint32_t* amax_bits,
                                  float* sigma, hipStream_t stream) {
   zero_u32_kernel<<<1, 1, 0, stream>>>(amax_bits);
-  if (n > 0) amax_kernel<<<grid_for(n, 8), kThreads, 0, stream>>>(src, n, amax_bits);
+  if (n > 0) amax_kernel<<<std::min(grid_for(n, 8), 1024), kThreads, 0, stream>>>(src, n, amax_bits);
   fp8_sigma_kernel<<<1, 1, 0, stream>>>(amax_bits, sigma);
   return hipGetLastError();
 }

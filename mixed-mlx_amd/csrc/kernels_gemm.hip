@@ -295,7 +295,9 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
           if constexpr (T == GemmType::F64) {
             C[idx] = v;
           } else {
-            if (args.c_bf16) {
+            if (EPI == Epilogue::Accumulate) {
+              C[idx] += args.c_scale * v;
+            } else if (args.c_bf16) {
               const uint16_t code = encode_bf16(static_cast<float>(v));
               args.c_bf16[job.c_offset + idx] = code;
               if (mirrored) args.c_bf16[job.c_offset + midx] = code;
@@ -336,6 +338,8 @@ hipError_t launch_epilogue(Epilogue epi, int tile, const GemmArgs& args, uint32_
     case Epilogue::Hadamard: return launch_typed<T, Epilogue::Hadamard>(tile, args, tiles, stream);
     case Epilogue::AddSym: return launch_typed<T, Epilogue::AddSym>(tile, args, tiles, stream);
     case Epilogue::Gram: return launch_typed<T, Epilogue::Gram>(tile, args, tiles, stream);
+    case Epilogue::Accumulate:
+      return launch_typed<T, Epilogue::Accumulate>(tile, args, tiles, stream);
     default: return hipErrorInvalidValue;
   }
 }

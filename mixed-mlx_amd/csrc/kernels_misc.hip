@@ -150,6 +150,14 @@ __global__ void zero_f64_kernel(double* p, size_t n) {
     p[e] = 0.0;
 }
 
+// out = a - b, elementwise.
+__global__ void subtract_kernel(const float* __restrict__ a, const float* __restrict__ b,
+                                float* __restrict__ out, size_t n) {
+  for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+       e += static_cast<size_t>(gridDim.x) * blockDim.x)
+    out[e] = a[e] - b[e];
+}
+
 // sum (G_ij - delta_ij)^2 over the order x order block, per segment.
 __global__ void orthogonality_kernel(const Segment* __restrict__ segs,
                                      const float* __restrict__ gram,
@@ -346,6 +354,14 @@ hipError_t launch_transpose(const Segment* segs, uint32_t count, uint32_t pairs,
                             float* out, hipStream_t stream) {
   if (!count || !pairs) return hipSuccess;
   transpose_kernel<<<pairs, kT, 0, stream>>>(segs, count, v, out);
+  return hipGetLastError();
+}
+hipError_t launch_subtract(const float* a, const float* b, float* out, size_t n,
+                           hipStream_t stream) {
+  if (!n) return hipSuccess;
+  size_t g = (n + kT - 1) / kT;
+  if (g > 16384) g = 16384;
+  subtract_kernel<<<static_cast<int>(g), kT, 0, stream>>>(a, b, out, n);
   return hipGetLastError();
 }
 hipError_t launch_zero_f64(double* p, size_t n, hipStream_t stream) {

@@ -253,3 +253,46 @@ def test_orthogonality_measure(runtime):
     got = runtime.orthogonality(o, dev(vflat))
     for g, V in zip(got, Vs):
         assert abs(g - S.orthogonality_error(V)) <= 1e-6 + 1e-3 * S.orthogonality_error(V)
+
+
+def test_mxfp4_two_term_reconstruction(runtime):
+    """The solve's MXFP4 reconstruction: G = V diag(sqrt s) carried as hi + lo
+    (hi the MXFP4 rounding of G, lo that of G - hi in float), X0 =
+    sym(hi hi^T + 2 hi lo^T) = hi hi^T + hi lo^T + lo hi^T, against the same
+    expansion built on the host from the solve's own Jacobi basis; and the
+    expansion is much closer to the root than hi hi^T alone."""
+    orders = [32, 64, 100, 256]
+    mats = [G.single_matrix(15, n, 1e-3, index=i)[0] for i, n in enumerate(orders)]
+    jac = capi.jacobi_options(off_tol=1e-7)
+    opts = capi.solve_options(capi.MXFP4_E2M1, capi.BF16, jac, min_corrections=0,
+                              max_corrections=0, residual_tolerance=0.0)
+    o, aflat = capi.pack_batch(mats)
+    _, offs, voffs, total, vtotal = capi.batch_layout(o)
+    A = dev(aflat)
+    X = torch.zeros(total, dtype=torch.float64, device="cuda")
+    say("mxfp4 two-term reconstruction", orders)
+    runtime.solve_batched(o, A, X, opts)
+    V = torch.zeros(total, dtype=torch.float32, device="cuda")
+    D = torch.zeros(vtotal, dtype=torch.float32, device="cuda")
+    Rt = torch.zeros(vtotal, dtype=torch.float32, device="cuda")
+    runtime.jacobi_batched(o, A, V, D, Rt, jac)
+    Vh, Rh = host(V), host(Rt)
+    got = capi.unpack_batch(o, host(X))
+    for M, Xg, n, off, v0 in zip(mats, got, orders, offs, voffs):
+        ld = (n + 31) // 32 * 32
+        Vm = Vh[off:off + ld * ld].reshape(ld, ld, order="F")
+        s = Rh[v0:v0 + ld]
+        Gs = (Vm * np.sqrt(s)[None, :]).astype(np.float32)   # rows of G, K-contiguous
+        hi = quantized(capi.MXFP4_E2M1, Gs)
+        lo = quantized(capi.MXFP4_E2M1, (Gs - hi.astype(np.float32)).astype(np.float32))
+        C = hi @ hi.T + 2.0 * (hi @ lo.T)
+        want = (0.5 * (C + C.T))[:n, :n]
+        u = 2.0**-24
+        bound = 2 * ((2 * n + 8) * u * (np.linalg.norm(hi) ** 2 + 2 * np.linalg.norm(hi) *
+                                         np.linalg.norm(lo)) + 2 * u * np.linalg.norm(C))
+        assert np.linalg.norm(Xg - want) <= bound, (n, np.linalg.norm(Xg - want), bound)
+        root = S.reference_root(M)
+        one = (hi @ hi.T)[:n, :n]
+        e1, e2 = S.relative_error(one, root), S.relative_error(Xg, root)
+        say(n, "X0 error one term", e1, "two terms", e2)
+        assert e2 < 0.5 * e1
