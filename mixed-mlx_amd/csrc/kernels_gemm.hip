@@ -109,7 +109,13 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
   constexpr int CHUNKS = RB / 16;
   constexpr bool MX = T == GemmType::Mxfp4;
 
-  const uint32_t tile = blockIdx.x;
+  // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin
+  // (workgroup b to XCD b mod 8), so XCD x takes the x-th contiguous eighth
+  // of the tile list.  A matrix's tiles are adjacent in the list, so its
+  // operand panels are read into one XCD's L2 and reused there.
+  const uint32_t nb = gridDim.x, q = nb / 8, rem = nb % 8;
+  const uint32_t xcd = blockIdx.x % 8, k = blockIdx.x / 8;
+  const uint32_t tile = (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + k;
   const GemmJob job = args.jobs[args.tile_job[tile]];
   if (args.active && !args.active[job.status_index]) return;
   const int tm = static_cast<int>(args.tile_pos[tile] >> 16) * BM;
@@ -117,8 +123,17 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
   const int ld = static_cast<int>(job.ld);
   const int order = static_cast<int>(job.order);
 
-  __shared__ __align__(16) unsigned char As[BM * RP];
-  __shared__ __align__(16) unsigned char Bs[BN * RP];
+  typedef typename Tr::Out Out;
+  // LDS: the K-step stages of A and B, reused after the K loop for the
+  // output tile when the epilogue goes through LDS (column pitch TP).
+  constexpr int TP = BM + 1;
+  constexpr int STAGE_BYTES = (BM + BN) * RP;
+  constexpr int TILE_BYTES = BN * TP * static_cast<int>(sizeof(Out));
+  constexpr bool STAGED = EPI != Epilogue::Gram && TILE_BYTES <= 72 * 1024;
+  constexpr int SMEM = STAGED && TILE_BYTES > STAGE_BYTES ? TILE_BYTES : STAGE_BYTES;
+  __shared__ __align__(16) unsigned char smem[SMEM];
+  unsigned char* const As = smem;
+  unsigned char* const Bs = smem + BM * RP;
   __shared__ uint8_t Ss[MX ? (BM + BN) * 4 : 1];
 
   const unsigned char* abase;
@@ -230,82 +245,129 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
   // row16] in accumulator r of a lane; v_mfma_f64_16x16x4_f64 leaves
   // C[row = 4 r + quad][col = row16].
   const float alpha = args.alpha ? args.alpha[job.status_index] : 1.0f;
-  typedef typename Tr::Out Out;
   Out* C = static_cast<Out*>(args.c) + job.c_offset;
   double norm2 = 0.0;
+  const bool mirror_tile = args.mirror && tm != tn;
+  // Entry (row, col) of the product, v, written at (row, col) ("main") or,
+  // for the symmetric and transposed forms, at (col, row) ("mirror").
+  auto entry_main = [&](int row, int col, Out v) {
+    const size_t idx = static_cast<size_t>(row) + static_cast<size_t>(col) * ld;
+    const bool in = row < order && col < order;
+    if constexpr (EPI == Epilogue::Residual) {
+      const double* A = static_cast<const double*>(args.aux) + job.aux_offset;
+      const double res = A[idx] - static_cast<double>(v);
+      if (args.c) C[idx] = static_cast<Out>(res);
+      if (args.c_float) args.c_float[job.c_offset + idx] = static_cast<float>(res);
+      if (args.c_bf16) args.c_bf16[job.c_offset + idx] = encode_bf16(static_cast<float>(res));
+      if (in) norm2 += res * res;
+    } else if constexpr (EPI == Epilogue::Gram) {
+      if (!in) return;
+      const double d = static_cast<double>(static_cast<float>(v) * alpha) - (row == col ? 1.0 : 0.0);
+      norm2 += (mirror_tile ? 2.0 : 1.0) * d * d;
+    } else if constexpr (EPI == Epilogue::AddSym) {
+      // Upper entries only (row <= col), added at (row, col) here and at
+      // (col, row) by the mirror pass, so X stays exactly symmetric.
+      if (row > col || col >= order) return;
+      double* X = static_cast<double*>(args.c) + job.c_offset;
+      X[idx] += static_cast<double>(static_cast<float>(v) * alpha);
+    } else if constexpr (T == GemmType::F64) {
+      if (!args.transpose_c) C[idx] = v;
+    } else {
+      if (args.transpose_c) return;
+      if (EPI == Epilogue::Hadamard && args.mirror && row > col) return;
+      float w = static_cast<float>(v) * alpha;
+      if constexpr (EPI == Epilogue::Hadamard) {
+        const float* s = static_cast<const float*>(args.aux) + job.aux_offset;
+        w = in ? w * (1.0f / (s[row] + s[col])) : 0.0f;
+      }
+      if (EPI == Epilogue::Accumulate)
+        C[idx] += args.c_scale * w;
+      else if (args.c_bf16)
+        args.c_bf16[job.c_offset + idx] = encode_bf16(w);
+      else
+        C[idx] = w;
+    }
+  };
+  auto entry_mirror = [&](int row, int col, Out v) {
+    const size_t midx = static_cast<size_t>(col) + static_cast<size_t>(row) * ld;
+    const bool in = row < order && col < order;
+    if constexpr (EPI == Epilogue::Residual) {
+      // A_ji - (X X)_ij: (X X)_ji = (X X)_ij for the solve's symmetric X.
+      if (!mirror_tile) return;
+      const double* A = static_cast<const double*>(args.aux) + job.aux_offset;
+      const double res = A[midx] - static_cast<double>(v);
+      if (args.c) C[midx] = static_cast<Out>(res);
+      if (args.c_float) args.c_float[job.c_offset + midx] = static_cast<float>(res);
+      if (args.c_bf16) args.c_bf16[job.c_offset + midx] = encode_bf16(static_cast<float>(res));
+      if (in) norm2 += res * res;
+    } else if constexpr (EPI == Epilogue::AddSym) {
+      if (row >= col || col >= order) return;
+      double* X = static_cast<double*>(args.c) + job.c_offset;
+      X[midx] += static_cast<double>(static_cast<float>(v) * alpha);
+    } else if constexpr (EPI == Epilogue::Gram) {
+      return;
+    } else if constexpr (T == GemmType::F64) {
+      if (args.transpose_c) C[midx] = v;
+    } else {
+      // Transposed store, or the mirror of a symmetric Hadamard product.
+      const bool sym = EPI == Epilogue::Hadamard && args.mirror;
+      if (!args.transpose_c && !(sym && row < col)) return;
+      float w = static_cast<float>(v) * alpha;
+      if constexpr (EPI == Epilogue::Hadamard) {
+        const float* s = static_cast<const float*>(args.aux) + job.aux_offset;
+        w = in ? w * (1.0f / (s[row] + s[col])) : 0.0f;
+      }
+      if (EPI == Epilogue::Accumulate)
+        C[midx] += args.c_scale * w;
+      else if (args.c_bf16)
+        args.c_bf16[job.c_offset + midx] = encode_bf16(w);
+      else
+        C[midx] = w;
+    }
+  };
+  if constexpr (STAGED) {
+    // Through LDS: the tile goes to LDS column-major, then every thread
+    // walks it along columns for the (row, col) writes and along rows for
+    // the (col, row) ones, so both are coalesced in global memory.
+    Out* Tt = reinterpret_cast<Out*>(smem);
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int col = tn + wn + 16 * j + row16;
+      for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = tm + wm + 16 * i + (T == GemmType::F64 ? 4 * r + quad : 4 * quad + r);
-        if (row >= ld || col >= ld) continue;
-        if constexpr (EPI == Epilogue::Residual) {
-          const double* A = static_cast<const double*>(args.aux) + job.aux_offset;
-          const size_t idx = static_cast<size_t>(row) + static_cast<size_t>(col) * ld;
-          const size_t midx = static_cast<size_t>(col) + static_cast<size_t>(row) * ld;
-          const bool mirror = args.mirror && tm != tn;
-          const double xx = static_cast<double>(acc[i][j][r]);
-          const double res = A[idx] - xx;
-          // Mirror entry (col, row): A_ji - (X X)_ij, (X X)_ji = (X X)_ij.
-          const double mres = mirror ? A[midx] - xx : 0.0;
-          if (args.c) {
-            C[idx] = static_cast<Out>(res);
-            if (mirror) C[midx] = static_cast<Out>(mres);
-          }
-          if (args.c_float) {
-            args.c_float[job.c_offset + idx] = static_cast<float>(res);
-            if (mirror) args.c_float[job.c_offset + midx] = static_cast<float>(mres);
-          }
-          if (args.c_bf16) {
-            args.c_bf16[job.c_offset + idx] = encode_bf16(static_cast<float>(res));
-            if (mirror) args.c_bf16[job.c_offset + midx] = encode_bf16(static_cast<float>(mres));
-          }
-          if (row < order && col < order) norm2 += res * res + mres * mres;
-        } else if constexpr (EPI == Epilogue::Gram) {
-          if (row >= order || col >= order) continue;
-          const double d = static_cast<double>(static_cast<float>(acc[i][j][r]) * alpha) -
-                           (row == col ? 1.0 : 0.0);
-          norm2 += (args.mirror && tm != tn ? 2.0 : 1.0) * d * d;
-        } else if constexpr (EPI == Epilogue::AddSym) {
-          // Upper tiles only: entry (row, col), row <= col, is added at
-          // (row, col) and (col, row), so X stays exactly symmetric.
-          if (row > col || col >= order) continue;
-          double* X = static_cast<double*>(args.c) + job.c_offset;
-          const double e = static_cast<double>(static_cast<float>(acc[i][j][r]) * alpha);
-          X[static_cast<size_t>(row) + static_cast<size_t>(col) * ld] += e;
-          if (row != col) X[static_cast<size_t>(col) + static_cast<size_t>(row) * ld] += e;
-        } else {
-          if (EPI == Epilogue::Hadamard && args.mirror && row > col) continue;
-          Out v = static_cast<Out>(acc[i][j][r]);
-          if constexpr (T != GemmType::F64) v *= alpha;
-          if constexpr (EPI == Epilogue::Hadamard) {
-            const float* s = static_cast<const float*>(args.aux) + job.aux_offset;
-            const bool in = row < order && col < order;
-            v = in ? v * (1.0f / (s[row] + s[col])) : Out(0);
-          }
-          const size_t idx = args.transpose_c
-                                 ? static_cast<size_t>(col) + static_cast<size_t>(row) * ld
-                                 : static_cast<size_t>(row) + static_cast<size_t>(col) * ld;
-          // Symmetric Hadamard product: the mirror entry (col, row).
-          const bool mirrored = EPI == Epilogue::Hadamard && args.mirror && row != col;
-          const size_t midx = static_cast<size_t>(col) + static_cast<size_t>(row) * ld;
-          if constexpr (T == GemmType::F64) {
-            C[idx] = v;
-          } else {
-            if (EPI == Epilogue::Accumulate) {
-              C[idx] += args.c_scale * v;
-            } else if (args.c_bf16) {
-              const uint16_t code = encode_bf16(static_cast<float>(v));
-              args.c_bf16[job.c_offset + idx] = code;
-              if (mirrored) args.c_bf16[job.c_offset + midx] = code;
-            } else {
-              C[idx] = v;
-              if (mirrored) C[midx] = v;
-            }
-          }
+        for (int r = 0; r < 4; ++r) {
+          const int lr = wm + 16 * i + (T == GemmType::F64 ? 4 * r + quad : 4 * quad + r);
+          const int lc = wn + 16 * j + row16;
+          Tt[lc * TP + lr] = static_cast<Out>(acc[i][j][r]);
+        }
+    __syncthreads();
+    for (int e = tid; e < BM * BN; e += NT) {
+      const int lr = e % BM, lc = e / BM;
+      const int row = tm + lr, col = tn + lc;
+      if (row < ld && col < ld) entry_main(row, col, Tt[lc * TP + lr]);
+    }
+    const bool second = (EPI == Epilogue::Residual && mirror_tile) || EPI == Epilogue::AddSym ||
+                        args.transpose_c || (EPI == Epilogue::Hadamard && args.mirror);
+    if (second) {
+      for (int e = tid; e < BM * BN; e += NT) {
+        const int lc = e % BN, lr = e / BN;
+        const int row = tm + lr, col = tn + lc;
+        if (row < ld && col < ld) entry_mirror(row, col, Tt[lc * TP + lr]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int col = tn + wn + 16 * j + row16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = tm + wm + 16 * i + (T == GemmType::F64 ? 4 * r + quad : 4 * quad + r);
+          if (row >= ld || col >= ld) continue;
+          const Out v = static_cast<Out>(acc[i][j][r]);
+          entry_main(row, col, v);
+          entry_mirror(row, col, v);
         }
       }
     }
