@@ -191,7 +191,9 @@ size_t msq_storage_bytes(int precision, size_t elements);
 size_t msq_scale_bytes(int precision, size_t elements);
 
 /* Batch layout: ld_i = roundup(order_i, 32); offsets[i] = sum_{j<i} ld_j^2;
- * vector_offsets[i] = sum_{j<i} ld_j.  Totals in elements. */
+ * vector_offsets[i] = sum_{j<i} ld_j.  Totals in elements.  Every batched
+ * call validates its orders this way: count >= 1 and 1 <= order_i <= 65504,
+ * else MSQ_ERROR_INVALID_ARGUMENT. */
 int msq_batch_layout(uint32_t count, const uint32_t *orders, uint32_t *lds,
                      uint64_t *offsets, uint64_t *vector_offsets,
                      uint64_t *total_elements, uint64_t *total_vector_elements);

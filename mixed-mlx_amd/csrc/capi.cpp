@@ -156,8 +156,8 @@ int make_layout(uint32_t count, const uint32_t* orders, Layout& L) {
   L.off.resize(count);
   L.voff.resize(count);
   for (uint32_t i = 0; i < count; ++i) {
-    if (orders[i] < 2 || orders[i] > 65504)
-      return set_error(MSQ_ERROR_INVALID_ARGUMENT, "order %u of matrix %u is outside [2, 65504]",
+    if (orders[i] < 1 || orders[i] > 65504)
+      return set_error(MSQ_ERROR_INVALID_ARGUMENT, "order %u of matrix %u is outside [1, 65504]",
                        orders[i], i);
     L.ld[i] = round_up(orders[i], 32);
     L.off[i] = L.total;
@@ -1546,6 +1546,12 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
                       const double* matrices, double* roots_out, const msq_solve_options* options,
                       uint32_t options_count, msq_solve_report* reports) {
   MSQ_TRY(activate(rt));
+  Layout L;
+  MSQ_TRY(make_layout(count, orders, L));
+  if (!matrices || !roots_out || !options)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  if (options_count != 1 && options_count != count)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "options_count must be 1 or count");
   GraphEntry* graph = nullptr;
   if (rt->graphs_enabled) {
     graph = find_graph(rt, count, orders, matrices, roots_out, options, options_count);
@@ -1557,12 +1563,6 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
                           reports);
     }
   }
-  Layout L;
-  MSQ_TRY(make_layout(count, orders, L));
-  if (!matrices || !roots_out || !options)
-    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
-  if (options_count != 1 && options_count != count)
-    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "options_count must be 1 or count");
   auto opt = [&](uint32_t i) -> const msq_solve_options& {
     return options[options_count == 1 ? 0 : i];
   };

@@ -1026,7 +1026,7 @@ hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t
                                  const double* a64, float* vectors, float* diagonal,
                                  float* roots, JacobiStatus* status, hipStream_t stream) {
   if (count == 0) return hipSuccess;
-  if (order < 2 || order > kBatchedJacobiMaxOrder) return hipErrorInvalidValue;
+  if (order < 1 || order > kBatchedJacobiMaxOrder) return hipErrorInvalidValue;
   const int N = static_cast<int>(batched_jacobi_padded_order(order));
   const int S = slots_for_order(N);
   const int threads = batched_jacobi_threads(order);

@@ -72,3 +72,18 @@ def test_report_records_match_the_c_struct():
     reps.replace(1, capi.Reports(rec.copy()), 0)
     assert reps.converged.all() and reps[1]["fallback_from"]["status"] == "budget_exhausted"
     assert reps[0]["residuals"] == [1.0, 0.1, 0.01] and reps[-1]["off_history"] == [1.0, 0.5, 0.25]
+
+
+def test_batch_layout_validates_orders():
+    """Every batched entry point validates its batch through the layout:
+    an empty batch and orders outside [1, 65504] are errors, order 1 is a
+    matrix like any other (a 32 x 32 tile)."""
+    import pytest
+
+    from msq import capi
+    for bad in ([], [0], [16, 0, 32], [65505]):
+        with pytest.raises(capi.MsqError):
+            capi.batch_layout(bad)
+    lds, offs, voffs, total, vtotal = capi.batch_layout([1, 65504])
+    assert list(lds) == [32, 65504] and list(offs) == [0, 1024]
+    assert total == 1024 + 65504 ** 2 and vtotal == 32 + 65504

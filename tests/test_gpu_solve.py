@@ -172,6 +172,68 @@ def test_config2_n4096_low_bit_products(runtime, tier):
     assert not fallback
 
 
+def _edge_batch():
+    """Matrices whose roots are known exactly or nearly so, at orders that
+    hit the schedule's and the layout's edges: 1 x 1, 2 x 2, odd orders,
+    orders just below and at a multiple of 32, the largest order of the LDS
+    kernel and the first of the blocked one."""
+    rng = np.random.default_rng(5)
+    mats = [np.array([[9.0]]),                                   # root 3
+            np.array([[2.0, 1.0], [1.0, 2.0]]),                  # eigenvalues 1, 3
+            4.0 * np.eye(3),                                     # root 2 I
+            np.eye(17),                                          # root I
+            np.diag(np.geomspace(1e-3, 1.0, 31)),                # already diagonal
+            G.single_matrix(16, 33, 1e-2)[0]]
+    # Exactly repeated eigenvalues: Q diag(1 x 100, 0.01 x 155) Q^T.
+    Q = G.haar_orthogonal([G.matrix_seed(16, 1)], 255)[0]
+    lam = np.r_[np.ones(100), np.full(155, 1e-2)]
+    mats.append((Q * lam) @ Q.T)
+    mats.append(G.single_matrix(16, 256, 1e-3, index=2)[0])
+    mats.append(G.single_matrix(16, 257, 1e-3, index=3)[0])
+    return [0.5 * (M + M.T) for M in mats]
+
+
+@pytest.mark.parametrize("tier", ["plain", "bf16"])
+def test_edge_orders_and_structures(runtime, tier):
+    """Orders 1, 2, 3, 17, 31, 33, 255, 256, 257 in one batch (the LDS kernel
+    and the blocked one together), identity, scaled identity, diagonal and
+    exactly repeated spectra: every root meets 1e-6 against the FP64 LAPACK
+    root and is exactly symmetric, and the identity's root is the identity."""
+    mats = _edge_batch()
+    opts = MS.plain_fp32_options() if tier == "plain" else MS.tier_options("bf16")
+    say("edge batch", tier, [M.shape[0] for M in mats])
+    o, aflat = capi.pack_batch(mats)
+    _, _, _, total, _ = capi.batch_layout(o)
+    Ad = dev(aflat)
+    X = torch.zeros(total, dtype=torch.float64, device="cuda")
+    reps, fallback = MS.solve_with_fallback(runtime, o, Ad, X, opts)
+    got = capi.unpack_batch(o, host(X))
+    for M, Xg, rep in zip(mats, got, reps):
+        err = S.relative_error(Xg, S.reference_root(M))
+        say(M.shape[0], rep["status"], rep["sweeps"], rep["corrections"], err)
+        assert rep["status"] == "converged" and err <= 1e-6, (M.shape[0], rep, err)
+        assert np.abs(Xg - Xg.T).max() == 0.0
+    # The identity: V = I and every root 1, so every product is exact.
+    assert np.array_equal(got[3], np.eye(17))
+    assert not fallback
+
+
+def test_invalid_batches_are_errors_and_leave_the_runtime_usable(runtime):
+    """Empty batches, zero orders and null buffers are rejected before any
+    device work; the runtime then solves the next batch normally."""
+    A = torch.zeros(1024, dtype=torch.float64, device="cuda")
+    X = torch.zeros(1024, dtype=torch.float64, device="cuda")
+    opts = MS.plain_fp32_options()
+    with pytest.raises(capi.MsqError, match="empty batch"):
+        runtime.solve_batched([], A, X, opts)
+    with pytest.raises(capi.MsqError, match="outside"):
+        runtime.solve_batched([0], A, X, opts)
+    with pytest.raises(capi.MsqError, match="null"):
+        runtime.solve_batched([16], 0, X, opts)
+    (Xg,), (rep,) = solve(runtime, [np.diag([4.0, 9.0])], opts)
+    assert rep["status"] == "converged" and np.allclose(Xg, np.diag([2.0, 3.0]), atol=1e-12)
+
+
 def test_graph_replays_match_eager(runtime):
     """Graph mode: the first call of a signature runs eagerly, the second is
     captured, later ones replay.  Every replay -- with eager work, host
