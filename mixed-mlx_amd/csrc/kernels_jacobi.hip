@@ -352,20 +352,26 @@ __device__ __forceinline__ float& lds_at(float* base, int byte_offset) {
   return *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_offset);
 }
 
-// Per slot and round: byte-scaled participants p < q and their column bases
-// (one 16-byte read), and the rotation (c, s) in an array of its own
-// (consecutive slots stay 16 / 8 bytes apart: no LDS bank conflicts).
-struct __align__(16) SlotIdx {
-  int p4, q4, cbp4, cbq4;
+// Per slot and round, one 16-byte record: the rotation (c, s) of the pair
+// p < q and, for each of p and q, the byte-scaled participant x4 = 4 x in the
+// low 10 bits and the byte-scaled column base cb4 = 4 x (x + 1) / 2 above
+// them (x <= 255: 10 + 17 bits).  One ds_read_b128 per slot serves a block.
+struct __align__(16) SlotRec {
+  float c, s;
+  uint32_t p, q;  // x4 | cb4 << 10
 };
+__device__ __forceinline__ uint32_t slot_index(int x) {
+  return static_cast<uint32_t>(4 * x) | (static_cast<uint32_t>(2 * x * (x + 1)) << 10);
+}
+__device__ __forceinline__ int idx4(uint32_t w) { return static_cast<int>(w & 0x3ffu); }
+__device__ __forceinline__ int base4(uint32_t w) { return static_cast<int>(w >> 10); }
 
 // Rotates U off-diagonal blocks (k < l) in place.  All loads are issued before
 // any store (the blocks are disjoint, which the compiler cannot know); entries
 // of `code` equal to ~0 are padding: they load a real block and store nothing.
 template <int U>
 __device__ __forceinline__ void rotate_blocks(float* __restrict__ Ap,
-                                              const SlotIdx* __restrict__ sidx,
-                                              const float2* __restrict__ rot,
+                                              const SlotRec* __restrict__ rec,
                                               const uint32_t* code) {
   int i00[U], i01[U], i10[U], i11[U];
   Block blk[U];
@@ -374,12 +380,11 @@ __device__ __forceinline__ void rotate_blocks(float* __restrict__ Ap,
   for (int u = 0; u < U; ++u) {
     const uint32_t c = code[u] == ~0u ? (1u << 16) : code[u];
     const int k = static_cast<int>(c & 0xffffu), l = static_cast<int>(c >> 16);
-    const SlotIdx rk = sidx[k], rl = sidx[l];
-    const float2 ak = rot[k], al = rot[l];
-    i00[u] = packed_offset(rk.p4, rk.cbp4, rl.p4, rl.cbp4);
-    i01[u] = packed_offset(rk.p4, rk.cbp4, rl.q4, rl.cbq4);
-    i10[u] = packed_offset(rk.q4, rk.cbq4, rl.p4, rl.cbp4);
-    i11[u] = packed_offset(rk.q4, rk.cbq4, rl.q4, rl.cbq4);
+    const SlotRec rk = rec[k], rl = rec[l];
+    i00[u] = packed_offset(idx4(rk.p), base4(rk.p), idx4(rl.p), base4(rl.p));
+    i01[u] = packed_offset(idx4(rk.p), base4(rk.p), idx4(rl.q), base4(rl.q));
+    i10[u] = packed_offset(idx4(rk.q), base4(rk.q), idx4(rl.p), base4(rl.p));
+    i11[u] = packed_offset(idx4(rk.q), base4(rk.q), idx4(rl.q), base4(rl.q));
     blk[u].a00 = lds_at(Ap, i00[u]);
     blk[u].a01 = lds_at(Ap, i01[u]);
     blk[u].a10 = lds_at(Ap, i10[u]);
@@ -387,10 +392,10 @@ __device__ __forceinline__ void rotate_blocks(float* __restrict__ Ap,
     // The element formula's coefficient products: p sees (c, -s) and q
     // (c, s), so every coefficient is one of c_k c_l, s_k c_l, c_k s_l,
     // s_k s_l up to an exact sign.
-    cc[u] = ak.x * al.x;
-    sc[u] = ak.y * al.x;
-    cs[u] = ak.x * al.y;
-    ss[u] = ak.y * al.y;
+    cc[u] = rk.c * rl.c;
+    sc[u] = rk.s * rl.c;
+    cs[u] = rk.c * rl.s;
+    ss[u] = rk.s * rl.s;
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
@@ -415,26 +420,21 @@ __device__ __forceinline__ void rotate_blocks(float* __restrict__ Ap,
 // dropped).  All loads are issued before any store (the blocks of a round are
 // disjoint, which the compiler cannot know).
 __device__ __forceinline__ void rotate_super(float* __restrict__ Ap,
-                                             const SlotIdx* __restrict__ sidx,
-                                             const float2* __restrict__ rot, uint32_t code) {
+                                             const SlotRec* __restrict__ rec, uint32_t code) {
   const int K = static_cast<int>(code & 0xffffu), L = static_cast<int>(code >> 16);
-  const SlotIdx rk[2] = {sidx[2 * K], sidx[2 * K + 1]};
-  const SlotIdx rl[2] = {sidx[2 * L], sidx[2 * L + 1]};
-  const float4 ck = *reinterpret_cast<const float4*>(rot + 2 * K);  // (c, s) of 2K, 2K + 1
-  const float4 cl = *reinterpret_cast<const float4*>(rot + 2 * L);
-  const float2 ak[2] = {make_float2(ck.x, ck.y), make_float2(ck.z, ck.w)};
-  const float2 al[2] = {make_float2(cl.x, cl.y), make_float2(cl.z, cl.w)};
+  const SlotRec rk[2] = {rec[2 * K], rec[2 * K + 1]};
+  const SlotRec rl[2] = {rec[2 * L], rec[2 * L + 1]};
   int i00[4], i01[4], i10[4], i11[4];
   Block blk[4];
   float cc[4], sc[4], cs[4], ss[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const SlotIdx& x = rk[u >> 1];
-    const SlotIdx& y = rl[u & 1];
-    i00[u] = packed_offset(x.p4, x.cbp4, y.p4, y.cbp4);
-    i01[u] = packed_offset(x.p4, x.cbp4, y.q4, y.cbq4);
-    i10[u] = packed_offset(x.q4, x.cbq4, y.p4, y.cbp4);
-    i11[u] = packed_offset(x.q4, x.cbq4, y.q4, y.cbq4);
+    const SlotRec& x = rk[u >> 1];
+    const SlotRec& y = rl[u & 1];
+    i00[u] = packed_offset(idx4(x.p), base4(x.p), idx4(y.p), base4(y.p));
+    i01[u] = packed_offset(idx4(x.p), base4(x.p), idx4(y.q), base4(y.q));
+    i10[u] = packed_offset(idx4(x.q), base4(x.q), idx4(y.p), base4(y.p));
+    i11[u] = packed_offset(idx4(x.q), base4(x.q), idx4(y.q), base4(y.q));
     blk[u].a00 = lds_at(Ap, i00[u]);
     blk[u].a01 = lds_at(Ap, i01[u]);
     blk[u].a10 = lds_at(Ap, i10[u]);
@@ -442,11 +442,10 @@ __device__ __forceinline__ void rotate_super(float* __restrict__ Ap,
     // The element formula's coefficient products: p sees (c, -s) and q
     // (c, s), so every coefficient is one of c_k c_l, s_k c_l, c_k s_l,
     // s_k s_l up to an exact sign.
-    const float2 a = ak[u >> 1], b = al[u & 1];
-    cc[u] = a.x * b.x;
-    sc[u] = a.y * b.x;
-    cs[u] = a.x * b.y;
-    ss[u] = a.y * b.y;
+    cc[u] = x.c * y.c;
+    sc[u] = x.s * y.c;
+    cs[u] = x.c * y.s;
+    ss[u] = x.s * y.s;
   }
   const bool diag = K == L;
 #pragma unroll
@@ -568,8 +567,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   // Static LDS: every array at a compile-time offset (addresses fold into
   // the ds instructions' offset fields).
   __shared__ __align__(16) float Ap[(packed + 3) & ~3];
-  __shared__ SlotIdx sidx[2][m];  // round parity x slot
-  __shared__ __align__(16) float2 rot[2][m];  // (c, s), p < q
+  __shared__ SlotRec srec[2][m];  // round parity x slot
   __shared__ float2 hist[G * hrow];  // G rounds x 8 parts x SP of (c, s') for V
   __shared__ double red[32];
   __shared__ float redf[16];
@@ -697,13 +695,12 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     Ap[ipp] = o.a00;
     Ap[ipq] = o.a01;
     Ap[iqq] = o.a11;
-    SlotIdx rec;
-    rec.p4 = 4 * p;
-    rec.q4 = 4 * q;
-    rec.cbp4 = 4 * cbp;
-    rec.cbq4 = 4 * cbq;
-    sidx[tt & 1][k] = rec;
-    rot[tt & 1][k] = r;
+    SlotRec rec;
+    rec.c = r.x;
+    rec.s = r.y;
+    rec.p = slot_index(p);
+    rec.q = slot_index(q);
+    srec[tt & 1][k] = rec;
     // position k holds a: it is p (sees (c, -s)) when a < b
     rv[(k / S) * SP + k % S] = make_float2(r.x, a < b ? -r.y : r.y);
   };
@@ -737,13 +734,12 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
       Ap[cbp[u] + p[u]] = o.a00;
       Ap[cbq[u] + p[u]] = o.a01;
       Ap[cbq[u] + q[u]] = o.a11;
-      SlotIdx rec;
-      rec.p4 = 4 * p[u];
-      rec.q4 = 4 * q[u];
-      rec.cbp4 = 4 * cbp[u];
-      rec.cbq4 = 4 * cbq[u];
-      sidx[tt & 1][ks[u]] = rec;
-      rot[tt & 1][ks[u]] = r[u];
+      SlotRec rec;
+      rec.c = r[u].x;
+      rec.s = r[u].y;
+      rec.p = slot_index(p[u]);
+      rec.q = slot_index(q[u]);
+      srec[tt & 1][ks[u]] = rec;
       rv[(ks[u] / S) * SP + ks[u] % S] = make_float2(r[u].x, a[u] < b[u] ? -r[u].y : r[u].y);
     }
   };
@@ -755,12 +751,11 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     __syncthreads();
 #pragma unroll 1
     for (; t < group_end; ++t) {
-      const SlotIdx* cur = sidx[t & 1];
-      const float2* cur_rot = rot[t & 1];
+      const SlotRec* cur = srec[t & 1];
       if constexpr (SUPER) {
         // Owner super-blocks first (waves without owners skip them), then the
         // pair phases of round t + 1, then the dealt items.
-        if (code[0] != ~0u) rotate_super(Ap, cur, cur_rot, code[0]);
+        if (code[0] != ~0u) rotate_super(Ap, cur, code[0]);
         if (t + 1 < group_end) {
           float2* rv = hist + (t + 1 - group_start) * hrow;
           if (tid < m2 - 1) {
@@ -773,14 +768,14 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
         }
 #pragma unroll
         for (int i = 1; i <= IPT; ++i)
-          if (code[i] != ~0u) rotate_super(Ap, cur, cur_rot, code[i]);
+          if (code[i] != ~0u) rotate_super(Ap, cur, code[i]);
       } else {
 #pragma unroll
         for (int i = 0; i < BPT + 1; i += 2) {
           if (i + 1 < BPT + 1) {
-            rotate_blocks<2>(Ap, cur, cur_rot, code + i);
+            rotate_blocks<2>(Ap, cur, code + i);
           } else {
-            rotate_blocks<1>(Ap, cur, cur_rot, code + i);
+            rotate_blocks<1>(Ap, cur, code + i);
           }
         }
         if (tid < m && t + 1 < group_end)
@@ -1018,7 +1013,7 @@ size_t batched_jacobi_lds_bytes(uint32_t order) {
   const size_t packed = static_cast<size_t>(N) * (N + 1) / 2;
   const int S = slots_for_order(N);
   const size_t hist = static_cast<size_t>(group_for_slots(S)) * 8 * padded_slots(S);
-  return ((packed + 3) & ~size_t{3}) * sizeof(float) + 2 * (N / 2) * (sizeof(SlotIdx) + sizeof(float2)) +
+  return ((packed + 3) & ~size_t{3}) * sizeof(float) + 2 * (N / 2) * sizeof(SlotRec) +
          hist * sizeof(float2) + 32 * sizeof(double) + 16 * sizeof(float) + 4 * sizeof(int);
 }
 
