@@ -32,16 +32,21 @@ static_assert(kParams == kMlpParams, "parameter layout");
 // run on A rounded to float (an A-perturbation of 2^-24 relative, far below
 // what the features resolve) with binary64 accumulation:
 //  * orders up to 256: A lives in registers, four lanes per row (lane p of
-//    row i holds columns 4 k + p), so the 40 steps never touch memory again;
+//    row i holds columns 4 k + p; two rows per lane for orders 129..256),
+//    so the 40 steps never touch memory again;
 //    one instantiation per order range (1..64, 65..128, 129..256) with
 //    4 ROWS lanes, so small matrices take small workgroups;
 //  * orders 257..512: one row per lane, A re-read from memory each step.
-// Each instantiation runs over the whole batch; workgroups of matrices
-// outside its range return at once.
+// Each instantiation runs over its range's own segment list (a workgroup
+// given a matrix outside its range returns at once).
 constexpr int kFeatGlobalThreads = 512;
 
+// Register path: rows of A per lane.  Orders 129..256 hold two rows per lane
+// (512 lanes): with one row on 1024 lanes, A's 64 floats per lane leave too
+// few of the 128 registers a 1024-lane workgroup gets and the rest spills.
+__host__ __device__ constexpr int feat_rows_per_lane(int rows) { return rows == 256 ? 2 : 1; }
 __host__ __device__ constexpr int feat_threads(int rows) {
-  return rows ? 4 * rows : kFeatGlobalThreads;
+  return rows ? 4 * rows / feat_rows_per_lane(rows) : kFeatGlobalThreads;
 }
 
 // Block-wide reductions in a fixed order (a butterfly within each wave, then
@@ -120,72 +125,89 @@ __global__ __launch_bounds__(feat_threads(ROWS)) void features_kernel(
   const int ld = static_cast<int>(sg.ld);
   const double* A = a64 + sg.offset;
   const int tid = threadIdx.x;
-  // Row and column phase of this lane: register path (row tid / 4, columns
-  // 4 k + tid % 4), else row tid over all columns.
-  const int row = REGS ? tid >> 2 : tid;
+  // Rows and column phase of this lane: register path (rows tid / 4 + r T / 4,
+  // r < RPL, columns 4 k + tid % 4), else row tid over all columns.
+  constexpr int RPL = REGS ? feat_rows_per_lane(ROWS) : 1;
+  const int row0 = REGS ? tid >> 2 : tid;
   const int phase = REGS ? tid & 3 : 0;
-  const bool row_ok = row < n;
+  int rows[RPL];
+  bool rows_ok[RPL];
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    rows[r] = row0 + r * (T / 4);
+    rows_ok[r] = rows[r] < n;
+  }
 
   // Pass 1: diagonal, off-diagonal absolute row sums, Frobenius norms.
-  float a[COLS];
-  double diag = 0.0, rsum = 0.0, fro = 0.0, off = 0.0;
+  float a[RPL][COLS];
+  double diag[RPL], rsum[RPL];
+  double fro = 0.0, off = 0.0;
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) diag[r] = rsum[r] = 0.0;
   if constexpr (REGS) {
     // Chunks of 8 columns, kept apart (sched_barrier) so that the loads in
     // flight stay within the register budget.
 #pragma unroll
-    for (int k0 = 0; k0 < COLS; k0 += 8) {
+    for (int r = 0; r < RPL; ++r) {
+      const int row = rows[r];
 #pragma unroll
-      for (int k = k0; k < k0 + 8; ++k) {
-        // Unconditional in-bounds load (clamped indices), then a select; the
-        // empty asm keeps the load out of a branch (the compiler would
-        // otherwise make each load conditional, one basic block per column).
-        const int j = 4 * k + phase;
-        double raw = A[min(row, n - 1) + static_cast<size_t>(min(j, n - 1)) * ld];
-        asm volatile("" : "+v"(raw));
-        const double v = (row_ok && j < n) ? raw : 0.0;
-        a[k] = static_cast<float>(v);
-        fro += v * v;
-        if (j == row) {
-          diag = v;
-        } else {
-          rsum += fabs(v);
-          off += v * v;
+      for (int k0 = 0; k0 < COLS; k0 += 8) {
+#pragma unroll
+        for (int k = k0; k < k0 + 8; ++k) {
+          // Unconditional in-bounds load (clamped indices), then a select; the
+          // empty asm keeps the load out of a branch (the compiler would
+          // otherwise make each load conditional, one basic block per column).
+          const int j = 4 * k + phase;
+          double raw = A[min(row, n - 1) + static_cast<size_t>(min(j, n - 1)) * ld];
+          asm volatile("" : "+v"(raw));
+          const double v = (rows_ok[r] && j < n) ? raw : 0.0;
+          a[r][k] = static_cast<float>(v);
+          fro += v * v;
+          if (j == row) {
+            diag[r] = v;
+          } else {
+            rsum[r] += fabs(v);
+            off += v * v;
+          }
         }
+        __builtin_amdgcn_sched_barrier(0);
       }
-      __builtin_amdgcn_sched_barrier(0);
+      diag[r] = quad_sum(diag[r]);
+      rsum[r] = quad_sum(rsum[r]);
     }
-    diag = quad_sum(diag);
-    rsum = quad_sum(rsum);
   } else {
-    for (int j = 0; j < n && row_ok; ++j) {
+    const int row = rows[0];
+    for (int j = 0; j < n && rows_ok[0]; ++j) {
       const double v = A[row + static_cast<size_t>(j) * ld];
       fro += v * v;
       if (j == row) {
-        diag = v;
+        diag[0] = v;
       } else {
-        rsum += fabs(v);
+        rsum[0] += fabs(v);
         off += v * v;
       }
     }
   }
   // Per-row features from one lane of each row.
-  const bool lead = row_ok && phase == 0;
   double tr = 0.0, dmin = INFINITY, dmax = -INFINITY, gl = INFINITY, gh = -INFINITY;
   double logd = 0.0, logd2 = 0.0, dommin = INFINITY, domsum = 0.0, weak = 0.0;
-  if (lead) {
-    const double d = diag;
-    tr = d;
-    dmin = d;
-    dmax = d;
-    gl = d - rsum;
-    gh = d + rsum;
-    const double ld10 = log10(fmax(d, 1e-300));
-    logd = ld10;
-    logd2 = ld10 * ld10;
-    const double dom = fmin(fmax(d / fmax(rsum, 1e-300), 1e-6), 1e6);
-    dommin = dom;
-    domsum = dom;
-    weak = d < rsum ? 1.0 : 0.0;
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    if (rows_ok[r] && phase == 0) {
+      const double d = diag[r];
+      tr += d;
+      dmin = fmin(dmin, d);
+      dmax = fmax(dmax, d);
+      gl = fmin(gl, d - rsum[r]);
+      gh = fmax(gh, d + rsum[r]);
+      const double ld10 = log10(fmax(d, 1e-300));
+      logd += ld10;
+      logd2 += ld10 * ld10;
+      const double dom = fmin(fmax(d / fmax(rsum[r], 1e-300), 1e-6), 1e6);
+      dommin = fmin(dommin, dom);
+      domsum += dom;
+      weak += d < rsum[r] ? 1.0 : 0.0;
+    }
   }
   block_sum2<NW>(tr, fro, red);
   block_sum2<NW>(off, logd, red);
@@ -196,6 +218,24 @@ __global__ __launch_bounds__(feat_threads(ROWS)) void features_kernel(
   gl = block_min<NW>(gl, red);
   gh = block_max<NW>(gh, red);
   dommin = block_min<NW>(dommin, red);
+  // The reduced statistics are read again only by thread 0 at the end: kept
+  // in LDS meanwhile, so that the power steps get the registers (the binary64
+  // values would otherwise stay live in every lane).
+  __shared__ double stats[12];
+  if (tid == 0) {
+    stats[0] = tr;
+    stats[1] = fro;
+    stats[2] = off;
+    stats[3] = logd;
+    stats[4] = logd2;
+    stats[5] = domsum;
+    stats[6] = weak;
+    stats[7] = dmin;
+    stats[8] = dmax;
+    stats[9] = gl;
+    stats[10] = gh;
+    stats[11] = dommin;
+  }
 
   // Power iterations: kPowerIterationsMax steps of x <- y / ||y|| with
   // y = A x from x = 1 / sqrt(n) (lambda_max), then kPowerIterationsMin steps
@@ -221,33 +261,53 @@ __global__ __launch_bounds__(feat_threads(ROWS)) void features_kernel(
       for (int i = tid; i < n; i += T) xc[i] = start_vector(i) / sqrt(snorm);
       __syncthreads();
     }
-    double y = 0.0;
+    double y[RPL];
     if constexpr (REGS) {
       // The register copy of A is pinned as float for each step (empty asm, no
       // instruction): otherwise its widening to binary64 is hoisted out of the
       // loop and the doubled copy no longer fits the register file.  Four
-      // partial sums (columns 16 k0 + 4 u + p, u = k mod 4) shorten the
-      // dependent chain of binary64 FMAs.
+      // partial sums per row (columns 16 k0 + 4 u + p, u = k mod 4) shorten
+      // the dependent chain of binary64 FMAs; each x value read serves every
+      // row of the lane.
 #pragma unroll
-      for (int k = 0; k < COLS; ++k) asm volatile("" : "+v"(a[k]));
-      double ys[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int r = 0; r < RPL; ++r)
 #pragma unroll
-      for (int k0 = 0; k0 < COLS; k0 += 8) {
-        if (4 * k0 < n) {
+        for (int k = 0; k < COLS; ++k) asm volatile("" : "+v"(a[r][k]));
+      double ys[RPL][4];
 #pragma unroll
-          for (int k = k0; k < k0 + 8; ++k)
-            ys[k & 3] += static_cast<double>(a[k]) * xc[4 * k + phase];
+      for (int r = 0; r < RPL; ++r) ys[r][0] = ys[r][1] = ys[r][2] = ys[r][3] = 0.0;
+#pragma unroll
+      for (int r = 0; r < RPL; ++r) {
+#pragma unroll
+        for (int k0 = 0; k0 < COLS; k0 += 8) {
+          if (4 * k0 < n) {
+#pragma unroll
+            for (int k = k0; k < k0 + 8; ++k)
+              ys[r][k & 3] += static_cast<double>(a[r][k]) * xc[4 * k + phase];
+          }
+          __builtin_amdgcn_sched_barrier(0);
         }
-        __builtin_amdgcn_sched_barrier(0);
       }
-      y = quad_sum((ys[0] + ys[1]) + (ys[2] + ys[3]));
-    } else if (row_ok) {
-      for (int j = 0; j < n; ++j)
-        y += static_cast<double>(static_cast<float>(A[row + static_cast<size_t>(j) * ld])) * xc[j];
+#pragma unroll
+      for (int r = 0; r < RPL; ++r)
+        y[r] = quad_sum((ys[r][0] + ys[r][1]) + (ys[r][2] + ys[r][3]));
+    } else {
+      y[0] = 0.0;
+      if (rows_ok[0]) {
+        for (int j = 0; j < n; ++j)
+          y[0] += static_cast<double>(static_cast<float>(A[rows[0] + static_cast<size_t>(j) * ld])) *
+                  xc[j];
+      }
     }
-    if (c != 0.0 && row_ok) y = c * xc[row] - y;
-    double q = lead ? xc[row] * y : 0.0;
-    double yy = lead ? y * y : 0.0;
+    double q = 0.0, yy = 0.0;
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+      if (c != 0.0 && rows_ok[r]) y[r] = c * xc[rows[r]] - y[r];
+      if (rows_ok[r] && phase == 0) {
+        q += xc[rows[r]] * y[r];
+        yy += y[r] * y[r];
+      }
+    }
     block_sum2<NW>(q, yy, red);
     if (it < kPowerIterationsMax) {
       lmax_prev = lmax;
@@ -257,12 +317,26 @@ __global__ __launch_bounds__(feat_threads(ROWS)) void features_kernel(
       mu = q;
     }
     const double inv = yy > 0.0 ? 1.0 / sqrt(yy) : 0.0;
-    if (lead) xc[row] = y * inv;
+#pragma unroll
+    for (int r = 0; r < RPL; ++r)
+      if (rows_ok[r] && phase == 0) xc[rows[r]] = y[r] * inv;
     __syncthreads();
   }
   const double lmin = c - mu;
 
   if (tid == 0) {
+    tr = stats[0];
+    fro = stats[1];
+    off = stats[2];
+    logd = stats[3];
+    logd2 = stats[4];
+    domsum = stats[5];
+    weak = stats[6];
+    dmin = stats[7];
+    dmax = stats[8];
+    gl = stats[9];
+    gh = stats[10];
+    dommin = stats[11];
     const double t = tr / n;
     const double lmax_c = fmax(lmax, 1e-300);
     const double lmin_c = fmax(lmin, lmax_c * 1e-16);
