@@ -364,6 +364,16 @@ __device__ __forceinline__ uint32_t slot_index(int x) {
   return static_cast<uint32_t>(4 * x) | (static_cast<uint32_t>(2 * x * (x + 1)) << 10);
 }
 __device__ __forceinline__ int idx4(uint32_t w) { return static_cast<int>(w & 0x3ffu); }
+// Record position of slot k.  Super-blocks read slots 2L and 2L + 1 for
+// consecutive L on consecutive lanes, so their records are stored even slots
+// first, then odd ones: each of those reads is then 16-byte consecutive
+// across lanes (a stride of two records would put four lanes of every
+// 16-lane group on the same LDS banks).  Single blocks read slot l for
+// consecutive l: identity.
+template <bool SPLIT, int M>
+__device__ __forceinline__ int slot_pos(int k) {
+  return SPLIT ? (k & 1) * (M / 2) + (k >> 1) : k;
+}
 __device__ __forceinline__ int base4(uint32_t w) { return static_cast<int>(w >> 10); }
 
 // Rotates U off-diagonal blocks (k < l) in place.  All loads are issued before
@@ -419,11 +429,14 @@ __device__ __forceinline__ void rotate_blocks(float* __restrict__ Ap,
 // for K < L, only (2K, 2K + 1) for K = L (the other three load and are
 // dropped).  All loads are issued before any store (the blocks of a round are
 // disjoint, which the compiler cannot know).
+// rec: records in the split order of slot_pos (even slots, then odd ones,
+// m / 2 = M2 each).
+template <int M2>
 __device__ __forceinline__ void rotate_super(float* __restrict__ Ap,
                                              const SlotRec* __restrict__ rec, uint32_t code) {
   const int K = static_cast<int>(code & 0xffffu), L = static_cast<int>(code >> 16);
-  const SlotRec rk[2] = {rec[2 * K], rec[2 * K + 1]};
-  const SlotRec rl[2] = {rec[2 * L], rec[2 * L + 1]};
+  const SlotRec rk[2] = {rec[K], rec[M2 + K]};
+  const SlotRec rl[2] = {rec[L], rec[M2 + L]};
   int i00[4], i01[4], i10[4], i11[4];
   Block blk[4];
   float cc[4], sc[4], cs[4], ss[4];
@@ -700,7 +713,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     rec.s = r.y;
     rec.p = slot_index(p);
     rec.q = slot_index(q);
-    srec[tt & 1][k] = rec;
+    srec[tt & 1][slot_pos<SUPER, m>(k)] = rec;
     // position k holds a: it is p (sees (c, -s)) when a < b
     rv[(k / S) * SP + k % S] = make_float2(r.x, a < b ? -r.y : r.y);
   };
@@ -739,7 +752,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
       rec.s = r[u].y;
       rec.p = slot_index(p[u]);
       rec.q = slot_index(q[u]);
-      srec[tt & 1][ks[u]] = rec;
+      srec[tt & 1][slot_pos<SUPER, m>(ks[u])] = rec;
       rv[(ks[u] / S) * SP + ks[u] % S] = make_float2(r[u].x, a[u] < b[u] ? -r[u].y : r[u].y);
     }
   };
@@ -755,7 +768,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
       if constexpr (SUPER) {
         // Owner super-blocks first (waves without owners skip them), then the
         // pair phases of round t + 1, then the dealt items.
-        if (code[0] != ~0u) rotate_super(Ap, cur, code[0]);
+        if (code[0] != ~0u) rotate_super<m2>(Ap, cur, code[0]);
         if (t + 1 < group_end) {
           float2* rv = hist + (t + 1 - group_start) * hrow;
           if (tid < m2 - 1) {
@@ -768,7 +781,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
         }
 #pragma unroll
         for (int i = 1; i <= IPT; ++i)
-          if (code[i] != ~0u) rotate_super(Ap, cur, code[i]);
+          if (code[i] != ~0u) rotate_super<m2>(Ap, cur, code[i]);
       } else {
 #pragma unroll
         for (int i = 0; i < BPT + 1; i += 2) {
