@@ -608,12 +608,14 @@ struct GemmPlan {
 int tile_class(uint32_t ld) { return ld > 256 ? kTileClasses - 1 : static_cast<int>(ld / 32) - 1; }
 
 // Tile edge index for matrices of pitch ld in products of this type: the
-// autotuned choice if any, else 32-tiles for ld = 32 and 64-tiles above.
+// autotuned choice if any, else 32-tiles for ld = 32, 64-tiles up to 256 and
+// 128-tiles above (FP64: 64-tiles, whose 4 x 4 binary64 accumulators per
+// wave would leave one wave per SIMD).
 int tile_kind(const msq_runtime* rt, msq::GemmType type, uint32_t ld) {
   const uint8_t t = rt ? rt->tiles[static_cast<int>(type)][tile_class(ld)] : 0;
   for (int k = 0; k < 3; ++k)
     if (t == kTileEdges[k]) return k;
-  return ld == 32 ? 0 : 1;
+  return ld == 32 ? 0 : (ld > 256 && type != msq::GemmType::F64 ? 2 : 1);
 }
 
 // upper: only the tiles on and above the diagonal (tile row <= tile column),

@@ -18,9 +18,9 @@
 // major), so a 16-byte global load feeds LDS directly and every lane's
 // fragment is one contiguous LDS read.  An MXFP4 lane fragment is exactly one
 // 32-element block along K, so its E8M0 scale is one byte per lane.
-// A workgroup computes a (32 WM) x (32 WN) tile, each wave a 32 x 32 block as
-// 2 x 2 MFMA tiles; tiles of all matrices form one flat grid (the host's tile
-// list), so small and large orders share a launch.
+// A workgroup computes a (16 F WM) x (16 F WN) tile, each wave a 16 F x 16 F
+// block as F x F MFMA tiles; tiles of all matrices form one flat grid (the
+// host's tile list), so small and large orders share a launch.
 #include <hip/hip_runtime.h>
 
 #include "device_codecs.hpp"
@@ -99,11 +99,11 @@ __device__ __forceinline__ void mma_step(typename Traits<T>::Acc& acc, const uns
   }
 }
 
-template <GemmType T, int WM, int WN, Epilogue EPI>
+template <GemmType T, int WM, int WN, int F, Epilogue EPI>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
   typedef Traits<T> Tr;
   constexpr int NT = 64 * WM * WN;
-  constexpr int BM = 32 * WM, BN = 32 * WN;
+  constexpr int BM = 16 * F * WM, BN = 16 * F * WN;
   constexpr int RB = Tr::KS * Tr::BITS / 8;  // bytes of one row per stage
   constexpr int RP = RB + 16;                // padded LDS row pitch
   constexpr int CHUNKS = RB / 16;
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
   constexpr int TP = BM + 1;
   constexpr int STAGE_BYTES = (BM + BN) * RP;
   constexpr int TILE_BYTES = BN * TP * static_cast<int>(sizeof(Out));
-  constexpr bool STAGED = EPI != Epilogue::Gram && TILE_BYTES <= 72 * 1024;
+  constexpr bool STAGED = EPI != Epilogue::Gram && TILE_BYTES <= 36 * 1024;
   constexpr int SMEM = STAGED && TILE_BYTES > STAGE_BYTES ? TILE_BYTES : STAGE_BYTES;
   __shared__ __align__(16) unsigned char smem[SMEM];
   unsigned char* const As = smem;
@@ -151,16 +151,16 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = (wave % WM) * 32;
-  const int wn = (wave / WM) * 32;
+  const int wm = (wave % WM) * 16 * F;
+  const int wn = (wave / WM) * 16 * F;
   const int quad = lane >> 4;
   const int row16 = lane & 15;
 
-  typename Tr::Acc acc[2][2];
+  typename Tr::Acc acc[F][F];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < F; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < F; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[i][j][r] = 0;
 
@@ -222,11 +222,11 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
     __syncthreads();
     if (kk + Tr::KS < kend) global_load(kk + Tr::KS);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < F; ++i) {
       const int ar = wm + 16 * i + row16;
       const unsigned char* arow = As + ar * RP;
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
+      for (int j = 0; j < F; ++j) {
         const int br = wn + 16 * j + row16;
         const unsigned char* brow = Bs + br * RP;
         int sa = 0, sb = 0;
@@ -331,9 +331,9 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
     // the (col, row) ones, so both are coalesced in global memory.
     Out* Tt = reinterpret_cast<Out*>(smem);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < F; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < F; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int lr = wm + 16 * i + (T == GemmType::F64 ? 4 * r + quad : 4 * quad + r);
@@ -357,9 +357,9 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < F; ++i) {
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
+      for (int j = 0; j < F; ++j) {
         const int col = tn + wn + 16 * j + row16;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -380,12 +380,14 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
 
 template <GemmType T, Epilogue E>
 hipError_t launch_typed(int tile, const GemmArgs& args, uint32_t tiles, hipStream_t stream) {
+  // 32: one wave of 2 x 2 MFMA blocks; 64: four such waves; 128: four
+  // waves of 4 x 4 blocks (each fragment read from LDS serves four MFMAs).
   if (tile == 32) {
-    gemm_kernel<T, 1, 1, E><<<tiles, 64, 0, stream>>>(args);
+    gemm_kernel<T, 1, 1, 2, E><<<tiles, 64, 0, stream>>>(args);
   } else if (tile == 64) {
-    gemm_kernel<T, 2, 2, E><<<tiles, 256, 0, stream>>>(args);
+    gemm_kernel<T, 2, 2, 2, E><<<tiles, 256, 0, stream>>>(args);
   } else if (tile == 128) {
-    gemm_kernel<T, 4, 4, E><<<tiles, 1024, 0, stream>>>(args);
+    gemm_kernel<T, 2, 2, 4, E><<<tiles, 256, 0, stream>>>(args);
   } else {
     return hipErrorInvalidValue;
   }
