@@ -11,7 +11,7 @@
 //   F64     v_mfma_f64_16x16x4_f64                    4           double
 //   F32     v_mfma_f32_16x16x4_f32                    4           float
 //   Bf16    v_mfma_f32_16x16x32_bf16                  32          8 x bf16
-//   Fp8     v_mfma_f32_16x16x32_fp8_fp8               32          8 x E4M3FN
+//   Fp8     v_mfma_scale_f32_16x16x128_f8f6f4 (fp8)   128         32 x E4M3FN, unit scales
 //   Mxfp4   v_mfma_scale_f32_16x16x128_f8f6f4 (fp4)   128         32 x E2M1 + E8M0
 //
 // Both operands are stored K-contiguous (A as A^T column-major, B column-
@@ -36,6 +36,9 @@ typedef int i32x8 __attribute__((ext_vector_type(8)));
 
 template <GemmType T>
 struct Traits;
+// KS: K elements per stage; MPS: MFMA instructions per stage along K, each
+// covering KS / MPS.  Wider stages (128 bytes of a row for BF16 and FP32)
+// measured slower at n = 4096: the larger staging buffers cost occupancy.
 template <>
 struct Traits<GemmType::F64> {
   static constexpr int KS = 16, BITS = 64, MPS = 4;
@@ -56,7 +59,7 @@ struct Traits<GemmType::Bf16> {
 };
 template <>
 struct Traits<GemmType::Fp8> {
-  static constexpr int KS = 32, BITS = 8, MPS = 1;
+  static constexpr int KS = 128, BITS = 8, MPS = 1;
   typedef f32x4 Acc;
   typedef float Out;
 };
@@ -81,16 +84,27 @@ __device__ __forceinline__ void mma_step(typename Traits<T>::Acc& acc, const uns
     const float b = reinterpret_cast<const float*>(brow)[4 * sub + quad];
     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
   } else if constexpr (T == GemmType::Bf16) {
-    const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow + 16 * quad);
-    const bf16x8 b = *reinterpret_cast<const bf16x8*>(brow + 16 * quad);
+    // 32 K per instruction: lane quad q holds K 8 q .. 8 q + 7 (16 bytes).
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow + 64 * sub + 16 * quad);
+    const bf16x8 b = *reinterpret_cast<const bf16x8*>(brow + 64 * sub + 16 * quad);
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
   } else if constexpr (T == GemmType::Fp8) {
-    const long a = *reinterpret_cast<const long*>(arow + 8 * quad);
-    const long b = *reinterpret_cast<const long*>(brow + 8 * quad);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, acc, 0, 0, 0);
+    // The f8f6f4 instruction with E4M3 operands (format 0) and unit scales
+    // (E8M0 127): 128 K per instruction, lane quad q holds K 32 q .. 32 q + 31.
+    const uint4* pa = reinterpret_cast<const uint4*>(arow + 128 * sub + 32 * quad);
+    const uint4* pb = reinterpret_cast<const uint4*>(brow + 128 * sub + 32 * quad);
+    const uint4 a0 = pa[0], a1 = pa[1], b0 = pb[0], b1 = pb[1];
+    i32x8 a = {static_cast<int>(a0.x), static_cast<int>(a0.y), static_cast<int>(a0.z),
+               static_cast<int>(a0.w), static_cast<int>(a1.x), static_cast<int>(a1.y),
+               static_cast<int>(a1.z), static_cast<int>(a1.w)};
+    i32x8 b = {static_cast<int>(b0.x), static_cast<int>(b0.y), static_cast<int>(b0.z),
+               static_cast<int>(b0.w), static_cast<int>(b1.x), static_cast<int>(b1.y),
+               static_cast<int>(b1.z), static_cast<int>(b1.w)};
+    acc = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, acc, 0, 0, 0, 127, 0, 127);
   } else {
-    const uint4 a4 = *reinterpret_cast<const uint4*>(arow + 16 * quad);
-    const uint4 b4 = *reinterpret_cast<const uint4*>(brow + 16 * quad);
+    // 128 K per instruction: lane quad q holds the E2M1 block K 32 q .. 32 q + 31.
+    const uint4 a4 = *reinterpret_cast<const uint4*>(arow + 64 * sub + 16 * quad);
+    const uint4 b4 = *reinterpret_cast<const uint4*>(brow + 64 * sub + 16 * quad);
     i32x8 a = {static_cast<int>(a4.x), static_cast<int>(a4.y), static_cast<int>(a4.z),
                static_cast<int>(a4.w), 0, 0, 0, 0};
     i32x8 b = {static_cast<int>(b4.x), static_cast<int>(b4.y), static_cast<int>(b4.z),
@@ -134,7 +148,8 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
   __shared__ __align__(16) unsigned char smem[SMEM];
   unsigned char* const As = smem;
   unsigned char* const Bs = smem + BM * RP;
-  __shared__ uint8_t Ss[MX ? (BM + BN) * 4 : 1];
+  constexpr int NBLK = Tr::KS / 32;  // MX: 32-element scale blocks per row per stage
+  __shared__ uint8_t Ss[MX ? (BM + BN) * NBLK : 1];
 
   const unsigned char* abase;
   const unsigned char* bbase;
@@ -206,9 +221,9 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
       }
     }
     if constexpr (MX) {
-      for (int c = tid; c < (BM + BN) * 4; c += NT) {
-        const int r = c / 4;
-        const int blk = c % 4;
+      for (int c = tid; c < (BM + BN) * NBLK; c += NT) {
+        const int r = c / NBLK;
+        const int blk = c % NBLK;
         const bool is_a = r < BM;
         const int rr = is_a ? r : r - BM;
         const int g = (is_a ? tm : tn) + rr;
@@ -216,7 +231,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
         uint8_t s = 127;
         if (g < ld && k0 < kend)
           s = (is_a ? asc : bsc)[(static_cast<size_t>(g) * ld + k0) / 32];
-        Ss[r * 4 + blk] = s;
+        Ss[r * NBLK + blk] = s;
       }
     }
     __syncthreads();
@@ -229,13 +244,17 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
       for (int j = 0; j < F; ++j) {
         const int br = wn + 16 * j + row16;
         const unsigned char* brow = Bs + br * RP;
-        int sa = 0, sb = 0;
-        if constexpr (MX) {
-          sa = Ss[ar * 4 + quad];
-          sb = Ss[(BM + br) * 4 + quad];
-        }
 #pragma unroll
-        for (int s = 0; s < Tr::MPS; ++s) mma_step<T>(acc[i][j], arow, brow, s, quad, sa, sb);
+        for (int s = 0; s < Tr::MPS; ++s) {
+          // Sub-steps past the order's K range hold zeros: skipped.
+          if (s > 0 && kk + s * (Tr::KS / Tr::MPS) >= kend) break;
+          int sa = 0, sb = 0;
+          if constexpr (MX) {
+            sa = Ss[ar * NBLK + 4 * s + quad];
+            sb = Ss[(BM + br) * NBLK + 4 * s + quad];
+          }
+          mma_step<T>(acc[i][j], arow, brow, s, quad, sa, sb);
+        }
       }
     }
     __syncthreads();

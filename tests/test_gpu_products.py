@@ -57,8 +57,8 @@ def test_reconstruction_products(runtime, tier):
         Gq = quantized(tier, Gt)[:n, :n]
         want = Gq @ Gq.T
         # FP32 accumulation bound (3_spectral/product_benchmark.cpp:422-444 form);
-        # the FP8 MFMA accumulates its 32-term chunks more coarsely (measured
-        # about 2^-17 relative on MI355X), so FP8 gets u = 2^-16 per term.
+        # the FP8 MFMA (f8f6f4, 128 K per instruction) accumulates inside an
+        # instruction more coarsely than FP32, so FP8 gets u = 2^-16 per term.
         u = 2.0**-16 if tier == capi.FP8_E4M3FN_SCALED else 2.0**-24
         bound = (2 * n + 8) * u * np.linalg.norm(Gq) ** 2
         assert np.linalg.norm(X - want) <= bound, (tier, n)
