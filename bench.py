@@ -240,8 +240,9 @@ def configs_leg(rt, torch, np, capi, G, MS, cpu=True):
     # call's row scaling and operand encode included.
     o = [n]
     _, _, _, total, vtotal = capi.batch_layout(o)
-    V = torch.linalg.qr(torch.randn(n, n, generator=g, device="cuda", dtype=torch.float64))[0]
-    V = V.to(torch.float32).t().contiguous().reshape(-1)
+    # A random operand with the basis' scale (entries ~ 1 / sqrt(n)): the
+    # product's time does not depend on the operand being orthogonal.
+    V = (torch.randn(n * n, generator=g, device="cuda", dtype=torch.float32) / n ** 0.5)
     roots = torch.rand(vtotal, generator=g, device="cuda", dtype=torch.float32) + 0.5
     out = torch.empty(total, dtype=torch.float32, device="cuda")
     products = {}
