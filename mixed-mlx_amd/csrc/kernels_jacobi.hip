@@ -88,6 +88,23 @@ __device__ __forceinline__ Block rotate_block(Block b, float2 r0, float2 r1,
 }
 
 
+// Off-diagonal 2 x 2 block {p_k, q_k} x {p_l, q_l} rotated on both sides:
+// the reference's element formula A'[r][c] = rr.x cc.x A[r][c] + rr.y cc.x
+// A[r'][c] + rr.x cc.y A[r][c'] + rr.y cc.y A[r'][c'] (p sees (c, -s), q
+// (c, s)) evaluated as rows first, then columns: t = J_k^T B, o = t J_l,
+// 16 multiply-adds with no coefficient products.
+__device__ __forceinline__ void rotate_rows_cols(const Block& b, float ck, float sk, float cl,
+                                                 float sl, Block& o) {
+  const float t00 = ck * b.a00 - sk * b.a10;
+  const float t01 = ck * b.a01 - sk * b.a11;
+  const float t10 = ck * b.a10 + sk * b.a00;
+  const float t11 = ck * b.a11 + sk * b.a01;
+  o.a00 = cl * t00 - sl * t01;
+  o.a01 = cl * t01 + sl * t00;
+  o.a10 = cl * t10 - sl * t11;
+  o.a11 = cl * t11 + sl * t10;
+}
+
 // Launch shape for a padded order N (a multiple of 16 up to 256).
 //
 // V rows are split into H parts of S slots on adjacent lanes; a lane holds
@@ -385,7 +402,7 @@ __device__ __forceinline__ void rotate_blocks(float* __restrict__ Ap,
                                               const uint32_t* code) {
   int i00[U], i01[U], i10[U], i11[U];
   Block blk[U];
-  float cc[U], sc[U], cs[U], ss[U];
+  float ck[U], sk[U], cl[U], sl[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const uint32_t c = code[u] == ~0u ? (1u << 16) : code[u];
@@ -399,23 +416,17 @@ __device__ __forceinline__ void rotate_blocks(float* __restrict__ Ap,
     blk[u].a01 = lds_at(Ap, i01[u]);
     blk[u].a10 = lds_at(Ap, i10[u]);
     blk[u].a11 = lds_at(Ap, i11[u]);
-    // The element formula's coefficient products: p sees (c, -s) and q
-    // (c, s), so every coefficient is one of c_k c_l, s_k c_l, c_k s_l,
-    // s_k s_l up to an exact sign.
-    cc[u] = rk.c * rl.c;
-    sc[u] = rk.s * rl.c;
-    cs[u] = rk.c * rl.s;
-    ss[u] = rk.s * rl.s;
+    ck[u] = rk.c;
+    sk[u] = rk.s;
+    cl[u] = rl.c;
+    sl[u] = rl.s;
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const Block b = blk[u];
-    // Rows (p_k, q_k), columns (p_l, q_l); A'[r][c] = rr.x cc.x A[r][c] +
-    // rr.y cc.x A[r'][c] + rr.x cc.y A[r][c'] + rr.y cc.y A[r'][c'].
-    const float o00 = cc[u] * b.a00 - sc[u] * b.a10 - cs[u] * b.a01 + ss[u] * b.a11;
-    const float o01 = cc[u] * b.a01 - sc[u] * b.a11 + cs[u] * b.a00 - ss[u] * b.a10;
-    const float o10 = cc[u] * b.a10 + sc[u] * b.a00 - cs[u] * b.a11 - ss[u] * b.a01;
-    const float o11 = cc[u] * b.a11 + sc[u] * b.a01 + cs[u] * b.a10 + ss[u] * b.a00;
+    Block o;
+    rotate_rows_cols(b, ck[u], sk[u], cl[u], sl[u], o);
+    const float o00 = o.a00, o01 = o.a01, o10 = o.a10, o11 = o.a11;
     if (code[u] != ~0u) {
       lds_at(Ap, i00[u]) = o00;
       lds_at(Ap, i01[u]) = o01;
@@ -439,7 +450,6 @@ __device__ __forceinline__ void rotate_super(float* __restrict__ Ap,
   const SlotRec rl[2] = {rec[L], rec[M2 + L]};
   int i00[4], i01[4], i10[4], i11[4];
   Block blk[4];
-  float cc[4], sc[4], cs[4], ss[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const SlotRec& x = rk[u >> 1];
@@ -452,24 +462,15 @@ __device__ __forceinline__ void rotate_super(float* __restrict__ Ap,
     blk[u].a01 = lds_at(Ap, i01[u]);
     blk[u].a10 = lds_at(Ap, i10[u]);
     blk[u].a11 = lds_at(Ap, i11[u]);
-    // The element formula's coefficient products: p sees (c, -s) and q
-    // (c, s), so every coefficient is one of c_k c_l, s_k c_l, c_k s_l,
-    // s_k s_l up to an exact sign.
-    cc[u] = x.c * y.c;
-    sc[u] = x.s * y.c;
-    cs[u] = x.c * y.s;
-    ss[u] = x.s * y.s;
   }
   const bool diag = K == L;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const Block b = blk[u];
-    // Rows (p_k, q_k), columns (p_l, q_l); A'[r][c] = rr.x cc.x A[r][c] +
-    // rr.y cc.x A[r'][c] + rr.x cc.y A[r][c'] + rr.y cc.y A[r'][c'].
-    const float o00 = cc[u] * b.a00 - sc[u] * b.a10 - cs[u] * b.a01 + ss[u] * b.a11;
-    const float o01 = cc[u] * b.a01 - sc[u] * b.a11 + cs[u] * b.a00 - ss[u] * b.a10;
-    const float o10 = cc[u] * b.a10 + sc[u] * b.a00 - cs[u] * b.a11 - ss[u] * b.a01;
-    const float o11 = cc[u] * b.a11 + sc[u] * b.a01 + cs[u] * b.a10 + ss[u] * b.a00;
+    const SlotRec& x = rk[u >> 1];
+    const SlotRec& y = rl[u & 1];
+    Block o;
+    rotate_rows_cols(blk[u], x.c, x.s, y.c, y.s, o);
+    const float o00 = o.a00, o01 = o.a01, o10 = o.a10, o11 = o.a11;
     if (!diag || u == 1) {
       lds_at(Ap, i00[u]) = o00;
       lds_at(Ap, i01[u]) = o01;
