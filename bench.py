@@ -307,7 +307,11 @@ def main():
         workload = ("batched uneven: 4096 SPD matrices, n in {16,...,256}, cond log-uniform "
                     "in [1e1, 1e5], half clustered spectra")
     else:
-        mats = list(G.mixture(count=args.count or 2000, config=4, order_range=(32, 512)))
+        mats = []
+        for m in G.mixture(count=args.count or 2000, config=4, order_range=(32, 512)):
+            mats.append(m)
+            if len(mats) % 2000 == 0:
+                log(f"generated {len(mats)} matrices")
         workload = (f"learned tier selection: {len(mats)} SPD matrices, n uniform in [32, 512], "
                     "spectra log-uniform (cond <= 1e6) / power law k^-alpha (alpha in "
                     "[0.5, 3]) / 4 clusters")
