@@ -27,6 +27,8 @@ namespace {
 constexpr int kB = kBlockedWidth;  // 32
 constexpr int kP = 2 * kB;         // 64: order of a pair's subproblem
 constexpr int kPitch = kP + 1;     // LDS pitch (odd: conflict-free columns)
+constexpr int kThreads = 256;      // workgroup of the gather and apply kernels
+constexpr int kPerThread = kP * kP / kThreads;  // 64 x 64 elements per thread
 
 __device__ __forceinline__ int block_participant(int t, int x, int nb) {
   if (x == 0) return 0;
@@ -87,25 +89,31 @@ __device__ __forceinline__ void lds_gemm64(const float* X, const float* Y, float
         C[(r0 + 16 * a + 4 * kq + r) + kPitch * (c0 + 16 * b + m)] = acc[a][b][r];
 }
 
-__global__ __launch_bounds__(256) void blocked_gather_kernel(const uint32_t* __restrict__ active,
-                                                             const float* __restrict__ A, int N,
-                                                             int t, double* __restrict__ S) {
+__global__ __launch_bounds__(kThreads) void blocked_gather_kernel(
+    const uint32_t* __restrict__ active, const float* __restrict__ A, int N, int t,
+    double* __restrict__ S) {
   const int nb = N / kB, np = nb / 2;
   const int p = blockIdx.x;
   const float* Am = A + static_cast<size_t>(active[blockIdx.y]) * N * N;
   double* Sp = S + (static_cast<size_t>(blockIdx.y) * np + p) * kP * kP;
-  for (int e = threadIdx.x; e < kP * kP; e += blockDim.x) {
-    const int i = e % kP, j = e / kP;
-    Sp[e] = static_cast<double>(
-        Am[pair_index(t, p, nb, i) + static_cast<size_t>(N) * pair_index(t, p, nb, j)]);
+  // All of a thread's loads are issued before its first store (a load and
+  // store per iteration would wait out one memory latency each).
+  float v[kPerThread];
+#pragma unroll
+  for (int it = 0; it < kPerThread; ++it) {
+    const int e = threadIdx.x + kThreads * it, i = e % kP, j = e / kP;
+    v[it] = Am[pair_index(t, p, nb, i) + static_cast<size_t>(N) * pair_index(t, p, nb, j)];
   }
+#pragma unroll
+  for (int it = 0; it < kPerThread; ++it)
+    Sp[threadIdx.x + kThreads * it] = static_cast<double>(v[it]);
 }
 
 // Block (p, q), p <= q, of the pair grid: A[P_p, P_q] <- U_p^T A[P_p, P_q] U_q,
 // and its transpose into A[P_q, P_p].
-__global__ __launch_bounds__(256) void blocked_apply_a_kernel(const uint32_t* __restrict__ active,
-                                                              float* __restrict__ A, int N, int t,
-                                                              const float* __restrict__ U) {
+__global__ __launch_bounds__(kThreads) void blocked_apply_a_kernel(
+    const uint32_t* __restrict__ active, float* __restrict__ A, int N, int t,
+    const float* __restrict__ U) {
   __shared__ float Up[kP * kPitch], Uq[kP * kPitch], Bm[kP * kPitch];
   const int nb = N / kB, np = nb / 2;
   // blockIdx.x -> (p, q), p <= q, over np (np + 1) / 2 entries.
@@ -118,12 +126,32 @@ __global__ __launch_bounds__(256) void blocked_apply_a_kernel(const uint32_t* __
   float* Am = A + static_cast<size_t>(active[blockIdx.y]) * N * N;
   const float* up = U + (static_cast<size_t>(blockIdx.y) * np + p) * kP * kP;
   const float* uq = U + (static_cast<size_t>(blockIdx.y) * np + q) * kP * kP;
-  for (int e = threadIdx.x; e < kP * kP; e += blockDim.x) {
-    const int i = e % kP, j = e / kP;
-    Up[i + kPitch * j] = up[e];
-    Uq[i + kPitch * j] = uq[e];
-    Bm[i + kPitch * j] =
-        Am[pair_index(t, p, nb, i) + static_cast<size_t>(N) * pair_index(t, q, nb, j)];
+  // Loads in two batches (U_p and U_q, then the block), each issued before
+  // its LDS stores.
+  {
+    float a[kPerThread], b[kPerThread];
+#pragma unroll
+    for (int it = 0; it < kPerThread; ++it) {
+      const int e = threadIdx.x + kThreads * it;
+      a[it] = up[e];
+      b[it] = uq[e];
+    }
+#pragma unroll
+    for (int it = 0; it < kPerThread; ++it) {
+      const int e = threadIdx.x + kThreads * it, i = e % kP, j = e / kP;
+      Up[i + kPitch * j] = a[it];
+      Uq[i + kPitch * j] = b[it];
+    }
+#pragma unroll
+    for (int it = 0; it < kPerThread; ++it) {
+      const int e = threadIdx.x + kThreads * it, i = e % kP, j = e / kP;
+      a[it] = Am[pair_index(t, p, nb, i) + static_cast<size_t>(N) * pair_index(t, q, nb, j)];
+    }
+#pragma unroll
+    for (int it = 0; it < kPerThread; ++it) {
+      const int e = threadIdx.x + kThreads * it, i = e % kP, j = e / kP;
+      Bm[i + kPitch * j] = a[it];
+    }
   }
   __syncthreads();
   lds_gemm64<true>(Up, Bm, Up);  // T = U_p^T B, over U_p
@@ -141,18 +169,28 @@ __global__ __launch_bounds__(256) void blocked_apply_a_kernel(const uint32_t* __
 }
 
 // V[rows of chunk c, P_p] <- V[rows, P_p] U_p (64 rows per workgroup).
-__global__ __launch_bounds__(256) void blocked_apply_v_kernel(const uint32_t* __restrict__ active,
-                                                              float* __restrict__ V, int N, int t,
-                                                              const float* __restrict__ U) {
+__global__ __launch_bounds__(kThreads) void blocked_apply_v_kernel(
+    const uint32_t* __restrict__ active, float* __restrict__ V, int N, int t,
+    const float* __restrict__ U) {
   __shared__ float Up[kP * kPitch], Vc[kP * kPitch];
   const int nb = N / kB, np = nb / 2;
   const int c = blockIdx.x, p = blockIdx.y;
   float* Vm = V + static_cast<size_t>(active[blockIdx.z]) * N * N;
   const float* up = U + (static_cast<size_t>(blockIdx.z) * np + p) * kP * kP;
-  for (int e = threadIdx.x; e < kP * kP; e += blockDim.x) {
-    const int i = e % kP, j = e / kP;
-    Up[i + kPitch * j] = up[e];
-    Vc[i + kPitch * j] = Vm[(kP * c + i) + static_cast<size_t>(N) * pair_index(t, p, nb, j)];
+  {
+    float a[kPerThread], b[kPerThread];
+#pragma unroll
+    for (int it = 0; it < kPerThread; ++it) {
+      const int e = threadIdx.x + kThreads * it, i = e % kP, j = e / kP;
+      a[it] = up[e];
+      b[it] = Vm[(kP * c + i) + static_cast<size_t>(N) * pair_index(t, p, nb, j)];
+    }
+#pragma unroll
+    for (int it = 0; it < kPerThread; ++it) {
+      const int e = threadIdx.x + kThreads * it, i = e % kP, j = e / kP;
+      Up[i + kPitch * j] = a[it];
+      Vc[i + kPitch * j] = b[it];
+    }
   }
   __syncthreads();
   lds_gemm64<false>(Vc, Up, Vc);
@@ -330,16 +368,17 @@ hipError_t launch_blocked_round(const uint32_t* active, uint32_t count, float* A
   if (N % kP != 0 || nb < 2 || count > 65535) return hipErrorInvalidValue;
   if (count == 0) return hipSuccess;
   const int Ni = static_cast<int>(N), t = static_cast<int>(round);
-  blocked_gather_kernel<<<dim3(np, count), 256, 0, stream>>>(active, A, Ni, t, staging);
+  blocked_gather_kernel<<<dim3(np, count), kThreads, 0, stream>>>(active, A, Ni, t, staging);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   e = launch_batched_jacobi(jobs, count * np, kP, staging, U, pair_diag, pair_roots, pair_status,
                             stream);
   if (e != hipSuccess) return e;
-  blocked_apply_a_kernel<<<dim3(np * (np + 1) / 2, count), 256, 0, stream>>>(active, A, Ni, t, U);
+  const uint32_t blocks = np * (np + 1) / 2;
+  blocked_apply_a_kernel<<<dim3(blocks, count), kThreads, 0, stream>>>(active, A, Ni, t, U);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  blocked_apply_v_kernel<<<dim3(N / kP, np, count), 256, 0, stream>>>(active, V, Ni, t, U);
+  blocked_apply_v_kernel<<<dim3(N / kP, np, count), kThreads, 0, stream>>>(active, V, Ni, t, U);
   return hipGetLastError();
 }
 
