@@ -26,7 +26,8 @@
 //
 // Rotation angles: tau = (a_qq - a_pp) / (2 a_pq), t = sign(tau) / (|tau| +
 // sqrt(1 + tau^2)), c = 1 / sqrt(1 + t^2), s = t c, with p < q, evaluated in
-// binary64 and rounded once to float (the reference evaluates them in float);
+// binary64 (an equivalent form, rotation_angles below) and rounded once to
+// float (the reference evaluates them in float);
 // entries are updated with the reference's element formula
 //   A'[r][c] = rr.x cc.x A[r][c] + rr.y cc.x A[r'][c] + rr.x cc.y A[r][c'] +
 //              rr.y cc.y A[r'][c']
@@ -63,12 +64,32 @@ __device__ __forceinline__ int tri_col(int e) {
 }
 
 // Rotation (c, s) for the pair p < q from a_pp, a_qq, a_pq, in binary64.
+// With d = a_qq - a_pp and e = 2 a_pq (tau = d / e),
+//   t = sign(d e) |e| / (|d| + sqrt(d^2 + e^2)) = sign(tau) / (|tau| + sqrt(1 + tau^2)),
+// c = 1 / sqrt(1 + t^2), s = t c.  The square roots and the reciprocal come
+// from the hardware estimates (v_rsq_f64, v_rcp_f64) refined by two Newton
+// steps each: a binary64 result within a few units of the last place, which
+// rounds to the same float as the correctly rounded quotients and roots
+// except at float rounding ties, and one third of their dependent chain (the
+// pair phase is on every round's critical path).  d^2 + e^2 neither
+// overflows nor underflows in binary64 for float inputs, and t is in [-1, 1].
+__device__ __forceinline__ double rsqrt_refined(double w) {  // w > 0, normal
+  double y = __builtin_amdgcn_rsq(w);
+  y = y * fma(-0.5 * w * y, y, 1.5);
+  y = y * fma(-0.5 * w * y, y, 1.5);
+  return y;
+}
 __device__ __forceinline__ float2 rotation_angles(float app, float aqq, float apq) {
   if (apq == 0.0f) return make_float2(1.0f, 0.0f);
-  const double tau = (static_cast<double>(aqq) - static_cast<double>(app)) /
-                     (2.0 * static_cast<double>(apq));
-  const double t = copysign(1.0 / (fabs(tau) + sqrt(1.0 + tau * tau)), tau);
-  const double c = 1.0 / sqrt(1.0 + t * t);
+  const double d = static_cast<double>(aqq) - static_cast<double>(app);
+  const double e = 2.0 * static_cast<double>(apq);
+  const double w = fma(d, d, e * e);
+  const double den = fabs(d) + w * rsqrt_refined(w);
+  double r = __builtin_amdgcn_rcp(den);
+  r = r * fma(-den, r, 2.0);
+  r = r * fma(-den, r, 2.0);
+  const double t = copysign(fabs(e) * r, d * e);
+  const double c = rsqrt_refined(fma(t, t, 1.0));
   return make_float2(static_cast<float>(c), static_cast<float>(t * c));
 }
 
@@ -127,10 +148,11 @@ __host__ __device__ constexpr int threads_for_order(int N) {
   return N <= 128 ? ((N + (64 / (N / 16)) - 1) / (64 / (N / 16))) * 64 : 4 * N;
 }
 
-// Off-diagonal 2x2 blocks per thread when each thread takes single blocks
-// (m (m - 1) / 2 blocks over the launch's threads; checked on the host).
-__host__ __device__ constexpr int blocks_per_thread(int S) {
-  return S == 8 ? 2 : (8 * S - 1 + 15) / 16;
+// Off-diagonal 2x2 blocks per thread when each thread takes single blocks:
+// the m (m - 1) / 2 blocks over the launch's threads, m = N / 2, the
+// next-pair blocks among them (one per thread below m).
+__host__ __device__ constexpr int single_blocks_per_thread(int N) {
+  return ((N / 2) * (N / 2 - 1) / 2 + threads_for_order(N) - 1) / threads_for_order(N);
 }
 // Orders whose block phase is bound by LDS throughput rather than by the
 // latency of a round take super-blocks (below); smaller orders take single
@@ -498,8 +520,18 @@ __device__ __forceinline__ uint32_t next_pair_block(int kp, int m) {
   const int l = kp == 0 ? 1 : (kp == m - 1 ? m - 1 : kp + 1);
   return static_cast<uint32_t>(k) | (static_cast<uint32_t>(l) << 16);
 }
-__device__ __forceinline__ bool is_next_pair_block(int k, int l, int m) {
-  return l == k + 2 || (k == 0 && l == 1) || (k == m - 2 && l == m - 1);
+// Item r of the dealt single blocks, the ones no next_pair_block names:
+// (k, k + 1) for 0 < k < m - 2, then (k, l) with k + 3 <= l row-major;
+// m (m - 3) / 2 items in all.
+__device__ __forceinline__ uint32_t item_single(int r, int m) {
+  if (r < m - 3) return static_cast<uint32_t>(r + 1) | (static_cast<uint32_t>(r + 2) << 16);
+  r -= m - 3;
+  int k = 0;
+  while (r >= m - 3 - k) {
+    r -= m - 3 - k;
+    ++k;
+  }
+  return static_cast<uint32_t>(k) | (static_cast<uint32_t>(k + 3 + r) << 16);
 }
 __device__ __forceinline__ uint32_t super_code(int K, int L) {
   return static_cast<uint32_t>(K) | (static_cast<uint32_t>(L) << 16);
@@ -559,7 +591,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   constexpr int SP = padded_slots(S);
   constexpr bool SUPER = use_super_blocks(NP);
   constexpr int IPT = items_per_thread(NP);
-  constexpr int BPT = blocks_per_thread(S);
+  constexpr int BPT = single_blocks_per_thread(NP);
   const JacobiJob job = jobs[blockIdx.x];
   const int tid = threadIdx.x;
   const int nt = blockDim.x;
@@ -576,7 +608,6 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   constexpr int m2 = m / 2;  // slot pairs
   constexpr int items = super_items(NP);
   constexpr int nthreads = threads_for_order(NP);  // == blockDim.x (host check)
-  constexpr int offblocks = m * (m - 1) / 2;
 
   // Static LDS: every array at a compile-time offset (addresses fold into
   // the ds instructions' offset fields).
@@ -600,11 +631,13 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   // r mod nthreads: the empty entries (~0) gather in the last threads, so
   // whole waves skip them (LDS bandwidth is the block phase's bound).
   //
-  // Single blocks (smaller orders): (k | l << 16), first the block that
-  // completes the next round's pair at slot tid (threads below m), then the
-  // other blocks, row-major over k < l, dealt round-robin; empty entries (~0)
-  // load a real block and store nothing, keeping the waves in step.
-  constexpr int CODES = SUPER ? IPT + 1 : BPT + 1;
+  // Single blocks (smaller orders): (k | l << 16), BPT per thread, the first
+  // of a thread below m the block that completes the next round's pair at
+  // slot tid; the other blocks (item_single) fill the remaining entries in
+  // order, entry 0 of threads m .. nthreads - 1 first, then entry 1 of every
+  // thread, and so on.  Empty entries (~0) load a real block and store
+  // nothing, keeping the waves in step.
+  constexpr int CODES = SUPER ? IPT + 1 : BPT;
   uint32_t code[CODES];
   if constexpr (SUPER) {
     code[0] = owner_super(tid, m2);
@@ -614,21 +647,16 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
       code[i + 1] = r < items ? item_super(r, m2) : ~0u;
     }
   } else {
-    code[0] = tid < m ? next_pair_block(tid, m) : ~0u;
+    constexpr int dealt = m * (m - 3) / 2;
+    code[0] = tid < m ? next_pair_block(tid, m)
+                      : (tid - m < dealt ? item_single(tid - m, m) : ~0u);
 #pragma unroll
-    for (int i = 0; i < BPT; ++i) {
-      const int b = tid + i * nthreads;
-      code[i + 1] = ~0u;
-      if (b < offblocks) {
-        int k = 0, rem = b;
-        while (rem >= m - 1 - k) {
-          rem -= m - 1 - k;
-          ++k;
-        }
-        const int l = k + 1 + rem;
-        if (!is_next_pair_block(k, l, m))
-          code[i + 1] = static_cast<uint32_t>(k) | (static_cast<uint32_t>(l) << 16);
-      }
+    for (int i = 1; i < BPT; ++i) {
+      // The last entry in reverse thread order: its empty entries fall to the
+      // lowest threads, whose waves also run the pair phases, and a wave with
+      // every last entry empty skips it.
+      const int r = nthreads - m + (i - 1) * nthreads + (i == BPT - 1 ? nthreads - 1 - tid : tid);
+      code[i] = r < dealt ? item_single(r, m) : ~0u;
     }
   }
 
@@ -784,13 +812,25 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
         for (int i = 1; i <= IPT; ++i)
           if (code[i] != ~0u) rotate_super<m2>(Ap, cur, code[i]);
       } else {
+        auto blocks = [&](auto entries) {
+          constexpr int NE = decltype(entries)::value;
 #pragma unroll
-        for (int i = 0; i < BPT + 1; i += 2) {
-          if (i + 1 < BPT + 1) {
-            rotate_blocks<2>(Ap, cur, code + i);
-          } else {
-            rotate_blocks<1>(Ap, cur, code + i);
+          for (int i = 0; i < NE; i += 2) {
+            if (i + 1 < NE) {
+              rotate_blocks<2>(Ap, cur, code + i);
+            } else {
+              rotate_blocks<1>(Ap, cur, code + i);
+            }
           }
+        };
+        if constexpr (BPT > 1) {
+          if (__ballot(code[BPT - 1] != ~0u) == 0) {
+            blocks(std::integral_constant<int, BPT - 1>{});
+          } else {
+            blocks(std::integral_constant<int, BPT>{});
+          }
+        } else {
+          blocks(std::integral_constant<int, BPT>{});
         }
         if (tid < m && t + 1 < group_end)
           pair_phase(tid, t + 1, hist + (t + 1 - group_start) * hrow);
@@ -1050,7 +1090,7 @@ hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t
   if (parts_for_order(N) * S != m || threads > 1024 || threads % 64 != 0 || threads < m ||
       threads != threads_for_order(N) || threads < N / 4 + 1 ||
       super_items(N) > items_per_thread(N) * threads ||
-      (offblocks + threads - 1) / threads > blocks_per_thread(S) || lds > 160 * 1024 ||
+      single_blocks_per_thread(N) * threads < offblocks || m < 8 || lds > 160 * 1024 ||
       (v_in_memory(S) && threads != 8 * m))
     return hipErrorInvalidValue;
   switch (N) {
