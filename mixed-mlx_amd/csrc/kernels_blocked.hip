@@ -28,7 +28,24 @@ constexpr int kB = kBlockedWidth;  // 32
 constexpr int kP = 2 * kB;         // 64: order of a pair's subproblem
 constexpr int kPitch = kP + 1;     // LDS pitch (odd: conflict-free columns)
 constexpr int kThreads = 256;      // workgroup of the gather and apply kernels
-constexpr int kPerThread = kP * kP / kThreads;  // 64 x 64 elements per thread
+// A 64 x 64 tile moves as 16-byte vectors of 4 consecutive rows of a column
+// (a column of a pair's tile is two runs of 32 rows, so no vector straddles
+// them): vector v is rows 4 (v % 16) .. + 3 of column v / 16.
+constexpr int kVecs = kP * kP / 4 / kThreads;  // vectors per thread (4)
+__device__ __forceinline__ int vec_row(int v) { return 4 * (v % (kP / 4)); }
+__device__ __forceinline__ int vec_col(int v) { return v / (kP / 4); }
+// Four LDS floats at stride 1 (a column run) or kPitch (a row run), and four
+// consecutive ones stored from a vector (the odd pitch rules out 16-byte LDS
+// accesses).
+__device__ __forceinline__ float4 load4(const float* x, int stride) {
+  return make_float4(x[0], x[stride], x[2 * stride], x[3 * stride]);
+}
+__device__ __forceinline__ void store4(float* x, float4 v) {
+  x[0] = v.x;
+  x[1] = v.y;
+  x[2] = v.z;
+  x[3] = v.w;
+}
 
 __device__ __forceinline__ int block_participant(int t, int x, int nb) {
   if (x == 0) return 0;
@@ -98,23 +115,30 @@ __global__ __launch_bounds__(kThreads) void blocked_gather_kernel(
   double* Sp = S + (static_cast<size_t>(blockIdx.y) * np + p) * kP * kP;
   // All of a thread's loads are issued before its first store (a load and
   // store per iteration would wait out one memory latency each).
-  float v[kPerThread];
+  float4 v[kVecs];
 #pragma unroll
-  for (int it = 0; it < kPerThread; ++it) {
-    const int e = threadIdx.x + kThreads * it, i = e % kP, j = e / kP;
-    v[it] = Am[pair_index(t, p, nb, i) + static_cast<size_t>(N) * pair_index(t, p, nb, j)];
+  for (int it = 0; it < kVecs; ++it) {
+    const int e = threadIdx.x + kThreads * it, i = vec_row(e), j = vec_col(e);
+    v[it] = *reinterpret_cast<const float4*>(
+        Am + pair_index(t, p, nb, i) + static_cast<size_t>(N) * pair_index(t, p, nb, j));
   }
 #pragma unroll
-  for (int it = 0; it < kPerThread; ++it)
-    Sp[threadIdx.x + kThreads * it] = static_cast<double>(v[it]);
+  for (int it = 0; it < kVecs; ++it) {
+    double* d = Sp + 4 * (threadIdx.x + kThreads * it);
+    *reinterpret_cast<double2*>(d) = make_double2(v[it].x, v[it].y);
+    *reinterpret_cast<double2*>(d + 2) = make_double2(v[it].z, v[it].w);
+  }
 }
 
 // Block (p, q), p <= q, of the pair grid: A[P_p, P_q] <- U_p^T A[P_p, P_q] U_q,
-// and its transpose into A[P_q, P_p].
+// and its transpose into A[P_q, P_p].  Two LDS tiles (four workgroups per
+// CU): U_p and the block first, T = U_p^T B over U_p, U_q (held in registers
+// since the start) over the block, then B' = T U_q over U_q.  Every thread
+// issues all its global loads before its first LDS store.
 __global__ __launch_bounds__(kThreads) void blocked_apply_a_kernel(
     const uint32_t* __restrict__ active, float* __restrict__ A, int N, int t,
     const float* __restrict__ U) {
-  __shared__ float Up[kP * kPitch], Uq[kP * kPitch], Bm[kP * kPitch];
+  __shared__ float X[kP * kPitch], Y[kP * kPitch];
   const int nb = N / kB, np = nb / 2;
   // blockIdx.x -> (p, q), p <= q, over np (np + 1) / 2 entries.
   int p = 0, rem = blockIdx.x;
@@ -126,45 +150,41 @@ __global__ __launch_bounds__(kThreads) void blocked_apply_a_kernel(
   float* Am = A + static_cast<size_t>(active[blockIdx.y]) * N * N;
   const float* up = U + (static_cast<size_t>(blockIdx.y) * np + p) * kP * kP;
   const float* uq = U + (static_cast<size_t>(blockIdx.y) * np + q) * kP * kP;
-  // Loads in two batches (U_p and U_q, then the block), each issued before
-  // its LDS stores.
-  {
-    float a[kPerThread], b[kPerThread];
+  float4 a[kVecs], b[kVecs], c[kVecs];
 #pragma unroll
-    for (int it = 0; it < kPerThread; ++it) {
-      const int e = threadIdx.x + kThreads * it;
-      a[it] = up[e];
-      b[it] = uq[e];
-    }
+  for (int it = 0; it < kVecs; ++it) {
+    const int e = threadIdx.x + kThreads * it, i = vec_row(e), j = vec_col(e);
+    a[it] = reinterpret_cast<const float4*>(up)[e];
+    b[it] = *reinterpret_cast<const float4*>(
+        Am + pair_index(t, p, nb, i) + static_cast<size_t>(N) * pair_index(t, q, nb, j));
+    c[it] = reinterpret_cast<const float4*>(uq)[e];
+  }
 #pragma unroll
-    for (int it = 0; it < kPerThread; ++it) {
-      const int e = threadIdx.x + kThreads * it, i = e % kP, j = e / kP;
-      Up[i + kPitch * j] = a[it];
-      Uq[i + kPitch * j] = b[it];
-    }
-#pragma unroll
-    for (int it = 0; it < kPerThread; ++it) {
-      const int e = threadIdx.x + kThreads * it, i = e % kP, j = e / kP;
-      a[it] = Am[pair_index(t, p, nb, i) + static_cast<size_t>(N) * pair_index(t, q, nb, j)];
-    }
-#pragma unroll
-    for (int it = 0; it < kPerThread; ++it) {
-      const int e = threadIdx.x + kThreads * it, i = e % kP, j = e / kP;
-      Bm[i + kPitch * j] = a[it];
-    }
+  for (int it = 0; it < kVecs; ++it) {
+    const int e = threadIdx.x + kThreads * it, o = vec_row(e) + kPitch * vec_col(e);
+    store4(X + o, a[it]);
+    store4(Y + o, b[it]);
   }
   __syncthreads();
-  lds_gemm64<true>(Up, Bm, Up);  // T = U_p^T B, over U_p
+  lds_gemm64<true>(X, Y, X);  // T = U_p^T B, over U_p (B read before the store)
+#pragma unroll
+  for (int it = 0; it < kVecs; ++it) {
+    const int e = threadIdx.x + kThreads * it;
+    store4(Y + vec_row(e) + kPitch * vec_col(e), c[it]);
+  }
   __syncthreads();
-  lds_gemm64<false>(Up, Uq, Bm);  // B' = T U_q, over B
+  lds_gemm64<false>(X, Y, Y);  // B' = T U_q, over U_q
   __syncthreads();
-  for (int e = threadIdx.x; e < kP * kP; e += blockDim.x) {
-    const int i = e % kP, j = e / kP;
-    Am[pair_index(t, p, nb, i) + static_cast<size_t>(N) * pair_index(t, q, nb, j)] =
-        Bm[i + kPitch * j];
+#pragma unroll
+  for (int it = 0; it < kVecs; ++it) {
+    const int e = threadIdx.x + kThreads * it, i = vec_row(e), j = vec_col(e);
+    *reinterpret_cast<float4*>(Am + pair_index(t, p, nb, i) +
+                               static_cast<size_t>(N) * pair_index(t, q, nb, j)) =
+        load4(Y + i + kPitch * j, 1);
     if (p != q)
-      Am[pair_index(t, q, nb, i) + static_cast<size_t>(N) * pair_index(t, p, nb, j)] =
-          Bm[j + kPitch * i];
+      *reinterpret_cast<float4*>(Am + pair_index(t, q, nb, i) +
+                                 static_cast<size_t>(N) * pair_index(t, p, nb, j)) =
+          load4(Y + j + kPitch * i, kPitch);
   }
 }
 
@@ -177,27 +197,29 @@ __global__ __launch_bounds__(kThreads) void blocked_apply_v_kernel(
   const int c = blockIdx.x, p = blockIdx.y;
   float* Vm = V + static_cast<size_t>(active[blockIdx.z]) * N * N;
   const float* up = U + (static_cast<size_t>(blockIdx.z) * np + p) * kP * kP;
-  {
-    float a[kPerThread], b[kPerThread];
+  float4 a[kVecs], b[kVecs];
 #pragma unroll
-    for (int it = 0; it < kPerThread; ++it) {
-      const int e = threadIdx.x + kThreads * it, i = e % kP, j = e / kP;
-      a[it] = up[e];
-      b[it] = Vm[(kP * c + i) + static_cast<size_t>(N) * pair_index(t, p, nb, j)];
-    }
+  for (int it = 0; it < kVecs; ++it) {
+    const int e = threadIdx.x + kThreads * it, i = vec_row(e), j = vec_col(e);
+    a[it] = reinterpret_cast<const float4*>(up)[e];
+    b[it] = *reinterpret_cast<const float4*>(
+        Vm + (kP * c + i) + static_cast<size_t>(N) * pair_index(t, p, nb, j));
+  }
 #pragma unroll
-    for (int it = 0; it < kPerThread; ++it) {
-      const int e = threadIdx.x + kThreads * it, i = e % kP, j = e / kP;
-      Up[i + kPitch * j] = a[it];
-      Vc[i + kPitch * j] = b[it];
-    }
+  for (int it = 0; it < kVecs; ++it) {
+    const int e = threadIdx.x + kThreads * it, o = vec_row(e) + kPitch * vec_col(e);
+    store4(Up + o, a[it]);
+    store4(Vc + o, b[it]);
   }
   __syncthreads();
   lds_gemm64<false>(Vc, Up, Vc);
   __syncthreads();
-  for (int e = threadIdx.x; e < kP * kP; e += blockDim.x) {
-    const int i = e % kP, j = e / kP;
-    Vm[(kP * c + i) + static_cast<size_t>(N) * pair_index(t, p, nb, j)] = Vc[i + kPitch * j];
+#pragma unroll
+  for (int it = 0; it < kVecs; ++it) {
+    const int e = threadIdx.x + kThreads * it, i = vec_row(e), j = vec_col(e);
+    *reinterpret_cast<float4*>(Vm + (kP * c + i) +
+                               static_cast<size_t>(N) * pair_index(t, p, nb, j)) =
+        load4(Vc + i + kPitch * j, 1);
   }
 }
 
