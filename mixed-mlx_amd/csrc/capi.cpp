@@ -326,7 +326,7 @@ int run_large_jacobi(msq_runtime* rt, const Layout& L, const double* a64, float*
                carve(subs * 4096 * sizeof(double)),     carve(subs * 4096 * sizeof(float)),
                carve(subs * 64 * sizeof(float)),        carve(subs * 64 * sizeof(float)),
                carve(subs * sizeof(msq::JacobiStatus)), carve(subs * sizeof(msq::JacobiJob)),
-               carve(cnt * (2 * msq::kOffnormParts + 1) * sizeof(double)),
+               carve(cnt * (3 * msq::kOffnormParts) * sizeof(double)),
                carve(cnt * msq::kOffnormParts * sizeof(float)),
                carve(cnt * sizeof(uint32_t)),           0};
   }
@@ -382,7 +382,7 @@ int run_large_jacobi(msq_runtime* rt, const Layout& L, const double* a64, float*
     c.previous.assign(cnt, INFINITY);
     c.h_off.resize(cnt * msq::kOffnormParts);
     c.h_g.resize(cnt * msq::kOffnormParts);
-    c.h_norm.resize(cnt);
+    c.h_norm.resize(cnt * msq::kOffnormParts);
     c.h_min.resize(cnt * msq::kOffnormParts);
     c.h_bad.resize(cnt);
     c.active.resize(cnt);
@@ -393,11 +393,16 @@ int run_large_jacobi(msq_runtime* rt, const Layout& L, const double* a64, float*
                            hipMemcpyHostToDevice, c.stream));
     HIP_TRY(msq::launch_blocked_init(c.dmem, static_cast<uint32_t>(cnt), a64, c.A, c.Vw, c.N,
                                      c.norm2, c.stream));
-    HIP_TRY(hipMemcpyAsync(c.h_norm.data(), c.norm2, cnt * sizeof(double), hipMemcpyDeviceToHost,
-                           c.stream));
+    HIP_TRY(hipMemcpyAsync(c.h_norm.data(), c.norm2, cnt * msq::kOffnormParts * sizeof(double),
+                           hipMemcpyDeviceToHost, c.stream));
     // hm and jobs are host temporaries: their copies finish before they go.
     HIP_TRY(hipStreamSynchronize(c.stream));
-    for (size_t j = 0; j < cnt; ++j) c.status[j].matrix_norm = std::sqrt(c.h_norm[j]);
+    const uint32_t nparts = msq::blocked_offnorm_parts(static_cast<uint32_t>(cnt));
+    for (size_t j = 0; j < cnt; ++j) {
+      double s2 = 0.0;
+      for (uint32_t k = 0; k < nparts; ++k) s2 += c.h_norm[j * msq::kOffnormParts + k];
+      c.status[j].matrix_norm = std::sqrt(s2);
+    }
   }
   for (uint32_t sweep = 0;; ++sweep) {
     bool any = false;

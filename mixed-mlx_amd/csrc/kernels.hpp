@@ -114,7 +114,12 @@ struct BlockedMember {
   uint32_t order;
   uint32_t ld;
 };
-// Working copies of every member and norm2[k] = ||A_k||_F^2 (binary64).
+// Partial results of the per-member reductions below: part k of member j at
+// [j * kOffnormParts + k], blocked_offnorm_parts(count) parts per member,
+// added in part order by the caller.
+constexpr int kOffnormParts = 64;
+uint32_t blocked_offnorm_parts(uint32_t count);
+// Working copies of every member and ||A_k||_F^2 (binary64) in partial sums.
 hipError_t launch_blocked_init(const BlockedMember* members, uint32_t count, const double* a64,
                                float* A, float* V, uint32_t N, double* norm2, hipStream_t stream);
 // One block round of the `count` members listed in `active`: gather each
@@ -126,11 +131,7 @@ hipError_t launch_blocked_round(const uint32_t* active, uint32_t count, float* A
                                 uint32_t N, uint32_t round, double* staging, float* U,
                                 const JacobiJob* jobs, float* pair_diag, float* pair_roots,
                                 JacobiStatus* pair_status, hipStream_t stream);
-// off(A)^2, ||G||_F^2 and min diag of each active member in
-// blocked_offnorm_parts(count) partial results at [member][part] (arrays of
-// kOffnormParts entries per member).
-constexpr int kOffnormParts = 64;
-uint32_t blocked_offnorm_parts(uint32_t count);
+// off(A)^2, ||G||_F^2 and min diag of each active member in partial results.
 hipError_t launch_blocked_offnorm(const uint32_t* active, uint32_t count,
                                   const BlockedMember* members, const float* A, uint32_t N,
                                   double* off2, double* g2, float* dmin, hipStream_t stream);

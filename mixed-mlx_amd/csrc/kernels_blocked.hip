@@ -266,20 +266,24 @@ __device__ float block_min(float v, float* red) {
 }
 
 // ||A||_F^2 of member blockIdx.x from its binary64 input.
+// Part blockIdx.x of gridDim.x of ||A||_F^2 of member blockIdx.y, from the
+// binary64 input, at [member][part].
 __global__ __launch_bounds__(512) void blocked_norm_kernel(const BlockedMember* __restrict__ members,
                                                            const double* __restrict__ a64,
                                                            double* __restrict__ norm2) {
   __shared__ double red[16];
-  const BlockedMember mb = members[blockIdx.x];
+  const BlockedMember mb = members[blockIdx.y];
   const int n = static_cast<int>(mb.order), ld = static_cast<int>(mb.ld);
   const double* a = a64 + mb.offset;
+  const size_t total = static_cast<size_t>(n) * n;
   double s = 0.0;
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-    const double x = a[(e % n) + static_cast<size_t>(e / n) * ld];
+  for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+       e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const double x = a[(e % n) + (e / n) * ld];
     s += x * x;
   }
   s = block_sum(s, red);
-  if (threadIdx.x == 0) norm2[blockIdx.x] = s;
+  if (threadIdx.x == 0) norm2[static_cast<size_t>(blockIdx.y) * kOffnormParts + blockIdx.x] = s;
 }
 
 // Stop-rule measures of active member active[blockIdx.y] (entries 0..n-1 of
@@ -377,7 +381,8 @@ hipError_t launch_blocked_init(const BlockedMember* members, uint32_t count, con
       members, a64, A, V, static_cast<int>(N));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  blocked_norm_kernel<<<count, 512, 0, stream>>>(members, a64, norm2);
+  blocked_norm_kernel<<<dim3(blocked_offnorm_parts(count), count), 512, 0, stream>>>(members, a64,
+                                                                                 norm2);
   return hipGetLastError();
 }
 
