@@ -138,12 +138,33 @@ __device__ __forceinline__ void rotate_rows_cols(const Block& b, float ck, float
 __host__ __device__ constexpr int slots_for_order(int N) { return N <= 128 ? 8 : N / 16; }
 __host__ __device__ constexpr bool v_in_memory(int S) { return S != 8; }
 __host__ __device__ constexpr int parts_for_order(int N) { return N <= 128 ? N / 16 : 8; }
-// Rounds buffered before V is updated: a whole phase cycle (S) for the
-// register-resident V, 16 for V in device memory.
-__host__ __device__ constexpr int group_for_slots(int S) { return S == 8 ? 8 : 16; }
 // History row stride per part: odd, so the H parts' broadcast reads of one
 // slot fall in distinct LDS banks.
 __host__ __device__ constexpr int padded_slots(int S) { return S | 1; }
+// LDS of the kernel but for the rotation history: packed A, slot records
+// (two rounds of m 16-byte records), reduction scratch.
+__host__ __device__ constexpr int lds_without_history(int N) {
+  return (((N * (N + 1) / 2) + 3) & ~3) * 4 + 2 * (N / 2) * 16 + 32 * 8 + 16 * 4 + 4 * 4;
+}
+constexpr int kLdsPerCu = 160 * 1024;
+// Rounds buffered before V is updated.
+//  * Register-resident V: a whole phase cycle, S rounds.
+//  * V in device memory: V is loaded and stored once per group, so the group
+//    is as long as the LDS left by A holds history for (every such order runs
+//    one workgroup per CU, bound by its registers), in chunks of 16 rounds,
+//    and at most a sweep.  Groups end at sweep boundaries, so a group is a
+//    multiple of 16 rounds or, the last of a sweep, 15 more than one (a
+//    sweep is N - 1 = 15 mod 16 rounds).  N = 256: 16; 240: 32; 224: 48;
+//    208: 80; 192: 96; 176: 128; 160: 144; 144: 144.
+__host__ __device__ constexpr int group_for_order(int N) {
+  const int S = slots_for_order(N);
+  if (!v_in_memory(S)) return S;
+  const int chunk = 16 * 8 * padded_slots(S) * 8;  // bytes of 16 rounds of history
+  int l = (kLdsPerCu - 512 - lds_without_history(N)) / chunk;  // 512: alignment slack
+  const int sweep_chunks = (N - 1 + 15) / 16;
+  l = l < sweep_chunks ? l : sweep_chunks;
+  return 16 * (l < 1 ? 1 : l);
+}
 __host__ __device__ constexpr int threads_for_order(int N) {
   return N <= 128 ? ((N + (64 / (N / 16)) - 1) / (64 / (N / 16))) * 64 : 4 * N;
 }
@@ -587,7 +608,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     float* __restrict__ roots, JacobiStatus* __restrict__ status) {
   constexpr int S = slots_for_order(NP);
   constexpr bool VMEM = v_in_memory(S);
-  constexpr int G = group_for_slots(S);
+  constexpr int G = group_for_order(NP);
   constexpr int SP = padded_slots(S);
   constexpr bool SUPER = use_super_blocks(NP);
   constexpr int IPT = items_per_thread(NP);
@@ -708,7 +729,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   float min_diag = 0.0f;
   uint32_t sweeps = 0;
   int t = 0;   // rounds applied to A
-  int tv = 0;  // rounds applied to V (whole groups; t rounded up to G)
+  int tv = 0;  // rounds applied to V (register V: whole groups, t rounded up to G)
   JacobiStatus* st = status + job.status_index;
 
   // One group: up to G rounds on A (rotation parameters buffered), the G
@@ -788,7 +809,10 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
 
   auto group = [&]() -> bool {
     const int group_start = t;
-    const int group_end = min(t + G, limit);
+    // V in device memory: groups end at sweep boundaries (the stop rules are
+    // checked there, and V applies exactly the group's rounds).
+    const int sweep_end = (t / rounds_per_sweep + 1) * rounds_per_sweep;
+    const int group_end = VMEM ? min(min(t + G, limit), sweep_end) : min(t + G, limit);
     for (int k = tid; k < m; k += nt) pair_phase(k, t, hist);
     __syncthreads();
 #pragma unroll 1
@@ -837,11 +861,18 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
       }
       __syncthreads();
     }
-    // Rounds the group did not reach: identity rotations (V's positions still
-    // advance; the output maps them through V's own round count).
-    for (int e = (group_end - group_start) * hrow + tid; e < G * hrow; e += nt)
-      hist[e] = make_float2(1.0f, 0.0f);
+    // Register V: rounds the group did not reach take identity rotations (V's
+    // positions still advance; the output maps them through V's own round
+    // count).  V in device memory applies the group's rounds only.
+    if constexpr (!VMEM) {
+      for (int e = (group_end - group_start) * hrow + tid; e < G * hrow; e += nt)
+        hist[e] = make_float2(1.0f, 0.0f);
+    }
     __syncthreads();
+    // Rounds of this group for V in device memory: whole chunks of 16, then 15
+    // at the end of a sweep.
+    const int vchunks = (group_end - group_start) / 16;
+    const bool vtail = (group_end - group_start) % 16 != 0;
     if constexpr (!VMEM) {
       // Every lane runs the V code (lanes past the last whole row of a wave
       // compute on zeros that no valid lane reads): no divergent branch around
@@ -872,7 +903,10 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
           vl1[j] = Vg[lo_off + m + j * step];
           vh1[j] = Vg[hi_off + m - j * step];
         }
-        v_group_shift2<S, G>(vl0, vh0, vl1, vh1, hist, hrow, vh);
+#pragma unroll 1
+        for (int c = 0; c < vchunks; ++c)
+          v_group_shift2<S, 16>(vl0, vh0, vl1, vh1, hist + c * 16 * hrow, hrow, vh);
+        if (vtail) v_group_shift2<S, 15>(vl0, vh0, vl1, vh1, hist + vchunks * 16 * hrow, hrow, vh);
         asm volatile("" : "+v"(lo_off), "+v"(hi_off), "+s"(step));
 #pragma unroll
         for (int j = 0; j < S; ++j) {
@@ -897,7 +931,10 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
             vl[j] = Vg[lo_off + j * step];
             vhv[j] = Vg[hi_off - j * step];
           }
-          v_group_shift<S, G>(vl, vhv, hist, hrow, vh);
+#pragma unroll 1
+          for (int c = 0; c < vchunks; ++c)
+            v_group_shift<S, 16>(vl, vhv, hist + c * 16 * hrow, hrow, vh);
+          if (vtail) v_group_shift<S, 15>(vl, vhv, hist + vchunks * 16 * hrow, hrow, vh);
           asm volatile("" : "+v"(lo_off), "+v"(hi_off), "+s"(step));
 #pragma unroll
           for (int j = 0; j < S; ++j) {
@@ -907,7 +944,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
         }
       }
     }
-    tv += G;
+    tv += VMEM ? group_end - group_start : G;
     __syncthreads();  // the next group overwrites the history (and V)
 
     // Convergence, at the first group boundary after each finished sweep.
@@ -1066,7 +1103,7 @@ size_t batched_jacobi_lds_bytes(uint32_t order) {
   const int N = static_cast<int>(batched_jacobi_padded_order(order));
   const size_t packed = static_cast<size_t>(N) * (N + 1) / 2;
   const int S = slots_for_order(N);
-  const size_t hist = static_cast<size_t>(group_for_slots(S)) * 8 * padded_slots(S);
+  const size_t hist = static_cast<size_t>(group_for_order(N)) * 8 * padded_slots(S);
   return ((packed + 3) & ~size_t{3}) * sizeof(float) + 2 * (N / 2) * sizeof(SlotRec) +
          hist * sizeof(float2) + 32 * sizeof(double) + 16 * sizeof(float) + 4 * sizeof(int);
 }
