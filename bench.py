@@ -56,7 +56,7 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch of the roofline kernel on the default workload: two
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a
 # 256 MiB copy gives), profiles/jacobi_n256_hbm_traffic_pmc.md.
-ROOFLINE_TRAFFIC_BYTES = 1.752e10
+ROOFLINE_TRAFFIC_BYTES = 1.213e10
 
 
 def log(*a):

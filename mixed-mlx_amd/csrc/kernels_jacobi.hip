@@ -147,23 +147,28 @@ __host__ __device__ constexpr int lds_without_history(int N) {
   return (((N * (N + 1) / 2) + 3) & ~3) * 4 + 2 * (N / 2) * 16 + 32 * 8 + 16 * 4 + 4 * 4;
 }
 constexpr int kLdsPerCu = 160 * 1024;
+// V in device memory is updated in chunks of rounds, each unrolled (the
+// values shift through fixed registers): 16 rounds, or 8 at N = 256, the one
+// order whose A leaves LDS for a single chunk of 16 but three of 8.
+__host__ __device__ constexpr int v_chunk_for_order(int N) { return N == 256 ? 8 : 16; }
 // Rounds buffered before V is updated.
 //  * Register-resident V: a whole phase cycle, S rounds.
 //  * V in device memory: V is loaded and stored once per group, so the group
 //    is as long as the LDS left by A holds history for (every such order runs
-//    one workgroup per CU, bound by its registers), in chunks of 16 rounds,
-//    and at most a sweep.  Groups end at sweep boundaries, so a group is a
-//    multiple of 16 rounds or, the last of a sweep, 15 more than one (a
-//    sweep is N - 1 = 15 mod 16 rounds).  N = 256: 16; 240: 32; 224: 48;
-//    208: 80; 192: 96; 176: 128; 160: 144; 144: 144.
+//    one workgroup per CU, bound by its registers), in whole chunks, and at
+//    most a sweep.  Groups end at sweep boundaries, so a group is a multiple
+//    of the chunk or, the last of a sweep, one round short of one (a sweep is
+//    N - 1 = 15 mod 16 rounds).  N = 256: 24; 240: 32; 224: 48; 208: 80;
+//    192: 96; 176: 128; 160: 144; 144: 144.
 __host__ __device__ constexpr int group_for_order(int N) {
   const int S = slots_for_order(N);
   if (!v_in_memory(S)) return S;
-  const int chunk = 16 * 8 * padded_slots(S) * 8;  // bytes of 16 rounds of history
+  const int C = v_chunk_for_order(N);
+  const int chunk = C * 8 * padded_slots(S) * 8;  // bytes of a chunk of history
   int l = (kLdsPerCu - 512 - lds_without_history(N)) / chunk;  // 512: alignment slack
-  const int sweep_chunks = (N - 1 + 15) / 16;
+  const int sweep_chunks = (N - 1 + C - 1) / C;
   l = l < sweep_chunks ? l : sweep_chunks;
-  return 16 * (l < 1 ? 1 : l);
+  return C * (l < 1 ? 1 : l);
 }
 __host__ __device__ constexpr int threads_for_order(int N) {
   return N <= 128 ? ((N + (64 / (N / 16)) - 1) / (64 / (N / 16))) * 64 : 4 * N;
@@ -609,6 +614,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   constexpr int S = slots_for_order(NP);
   constexpr bool VMEM = v_in_memory(S);
   constexpr int G = group_for_order(NP);
+  constexpr int VC = v_chunk_for_order(NP);
   constexpr int SP = padded_slots(S);
   constexpr bool SUPER = use_super_blocks(NP);
   constexpr int IPT = items_per_thread(NP);
@@ -869,10 +875,10 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
         hist[e] = make_float2(1.0f, 0.0f);
     }
     __syncthreads();
-    // Rounds of this group for V in device memory: whole chunks of 16, then 15
-    // at the end of a sweep.
-    const int vchunks = (group_end - group_start) / 16;
-    const bool vtail = (group_end - group_start) % 16 != 0;
+    // Rounds of this group for V in device memory: whole chunks, then one
+    // chunk less a round at the end of a sweep.
+    const int vchunks = (group_end - group_start) / VC;
+    const bool vtail = (group_end - group_start) % VC != 0;
     if constexpr (!VMEM) {
       // Every lane runs the V code (lanes past the last whole row of a wave
       // compute on zeros that no valid lane reads): no divergent branch around
@@ -905,8 +911,9 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
         }
 #pragma unroll 1
         for (int c = 0; c < vchunks; ++c)
-          v_group_shift2<S, 16>(vl0, vh0, vl1, vh1, hist + c * 16 * hrow, hrow, vh);
-        if (vtail) v_group_shift2<S, 15>(vl0, vh0, vl1, vh1, hist + vchunks * 16 * hrow, hrow, vh);
+          v_group_shift2<S, VC>(vl0, vh0, vl1, vh1, hist + c * VC * hrow, hrow, vh);
+        if (vtail)
+          v_group_shift2<S, VC - 1>(vl0, vh0, vl1, vh1, hist + vchunks * VC * hrow, hrow, vh);
         asm volatile("" : "+v"(lo_off), "+v"(hi_off), "+s"(step));
 #pragma unroll
         for (int j = 0; j < S; ++j) {
@@ -933,8 +940,8 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
           }
 #pragma unroll 1
           for (int c = 0; c < vchunks; ++c)
-            v_group_shift<S, 16>(vl, vhv, hist + c * 16 * hrow, hrow, vh);
-          if (vtail) v_group_shift<S, 15>(vl, vhv, hist + vchunks * 16 * hrow, hrow, vh);
+            v_group_shift<S, VC>(vl, vhv, hist + c * VC * hrow, hrow, vh);
+          if (vtail) v_group_shift<S, VC - 1>(vl, vhv, hist + vchunks * VC * hrow, hrow, vh);
           asm volatile("" : "+v"(lo_off), "+v"(hi_off), "+s"(step));
 #pragma unroll
           for (int j = 0; j < S; ++j) {
