@@ -133,8 +133,8 @@ __device__ __forceinline__ void rotate_rows_cols(const Block& b, float ck, float
 //  * N <= 128: S = 8, H = N / 16.  V stays in registers for the whole solve
 //    (one row per H lanes, all rows at once).
 //  * N > 128: S = N / 16 (9..16), H = 8.  V lives in device memory between
-//    groups of rounds; a group is applied in two passes of N / 2 rows, so a
-//    lane never holds more than 32 V values.
+//    groups of rounds; a group is applied with rows r and r + N / 2 on one
+//    lane (4 S V values per lane, N / 2 x 8 lanes).
 __host__ __device__ constexpr int slots_for_order(int N) { return N <= 128 ? 8 : N / 16; }
 __host__ __device__ constexpr bool v_in_memory(int S) { return S != 8; }
 __host__ __device__ constexpr int parts_for_order(int N) { return N <= 128 ? N / 16 : 8; }
@@ -186,14 +186,12 @@ __host__ __device__ constexpr int single_blocks_per_thread(int N) {
 // MI355X (one launch per padded order of a 4096-matrix batch): N = 256 runs
 // 5.5% faster with super-blocks, N = 240 4% slower, N <= 128 about 2x slower.
 __host__ __device__ constexpr bool use_super_blocks(int N) { return N >= 256; }
-// V in device memory: two rows per lane (each history read serves both) or
-// one row per lane in two passes.  Two rows hold 4 S values per lane, which
-// spill at S >= 11; measured per order on MI355X (one launch per padded order
-// of a 4096-matrix batch), two rows run 2-5% faster at N = 144, 160, 176
-// and 256, and 2-4% slower at N = 192 .. 240.
-__host__ __device__ constexpr bool two_v_rows(int N) {
-  return N == 144 || N == 160 || N == 176 || N == 256;
-}
+// V in device memory: two rows per lane (rows r and r + N / 2; each history
+// read serves both), 4 S values per lane, which spill a little at
+// N = 192 .. 240 (36 to 84 bytes per lane).  Measured on MI355X (256
+// matrices per order, 4 fixed sweeps) with V groups as long as the LDS
+// allows, two rows run 2-3% faster than one row per lane in two passes at
+// N = 144 .. 176 and 224 .. 240, 0.3-1% at 192, 208 and 256.
 
 // Large orders handle off-diagonal blocks as 2 x 2 super-blocks of slot pairs:
 // super-block (K, L), K <= L < m / 2, covers slots {2K, 2K + 1} x {2L, 2L + 1}
@@ -285,61 +283,12 @@ __device__ __forceinline__ void v_group(float (&lo)[S], float (&hi)[S],
   }
 }
 
-// One round on V rows held slot-aligned (8 parts per row): lo[j] / hi[j] are
-// slot j's two positions, rotated together, then shifted one position along
-// the round-robin circle (straight-line code turns the shifts into register
-// renames; the caller stores every register back to its own position).
-template <int S>
-__device__ __forceinline__ void v_round_shift(float (&lo)[S], float (&hi)[S],
-                                              const float2* __restrict__ rotv, int h) {
-#pragma unroll
-  for (int j0 = 0; j0 < S; j0 += 4) {
-    float2 r4[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (j0 + u < S) r4[u] = rotv[h * padded_slots(S) + j0 + u];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int j = j0 + u;
-      if (j < S) {
-        const float x = lo[j], y = hi[j];
-        lo[j] = r4[u].x * x + r4[u].y * y;
-        hi[j] = r4[u].x * y - r4[u].y * x;
-      }
-    }
-  }
-  const float out_up = lo[S - 1];  // to the next part's lo[0] (last part: own hi[S-1])
-  const float out_dn = hi[0];      // to the previous part's hi[S-1] (part 0: own lo[1])
-  const float keep0 = lo[0];
-  const float in_up = from_lane_below(out_up);
-  const float in_dn = from_lane_above(out_dn);
-#pragma unroll
-  for (int j = S - 1; j >= 2; --j) lo[j] = lo[j - 1];
-  const bool first = h == 0;
-  lo[1] = first ? out_dn : lo[0];
-  lo[0] = first ? keep0 : in_up;
-#pragma unroll
-  for (int j = 0; j < S - 1; ++j) hi[j] = hi[j + 1];
-  hi[S - 1] = h == 7 ? out_up : in_dn;
-  // Round boundary (empty asm, no instruction): V values pinned to registers
-  // and memory accesses kept on their side, so rounds do not overlap.
-#pragma unroll
-  for (int r = 0; r < S; ++r) asm volatile("" : "+v"(lo[r]), "+v"(hi[r]));
-  asm volatile("" ::: "memory");
-}
-
-template <int S, int G, int I = 0>
-__device__ __forceinline__ void v_group_shift(float (&lo)[S], float (&hi)[S],
-                                              const float2* __restrict__ hist, int row, int h) {
-  if constexpr (I < G) {
-    v_round_shift<S>(lo, hi, hist + I * row, h);
-    v_group_shift<S, G, I + 1>(lo, hi, hist, row, h);
-  }
-}
-
-// v_round_shift on two rows of the same part: each rotation is read from LDS
-// once and applied to both (the history's LDS traffic is per lane, so two
-// rows per lane halve it).
+// One round on two V rows held slot-aligned (8 parts per row, the same
+// part of rows r and r + N / 2): lo[j] / hi[j] are slot j's two positions,
+// rotated together, then shifted one position along the round-robin circle
+// (straight-line code turns the shifts into register renames; the caller
+// stores every register back to its own position).  Each rotation is read
+// from LDS once and applied to both rows.
 template <int S>
 __device__ __forceinline__ void v_round_shift2(float (&lo0)[S], float (&hi0)[S],
                                                float (&lo1)[S], float (&hi1)[S],
@@ -388,7 +337,8 @@ __device__ __forceinline__ void v_round_shift2(float (&lo0)[S], float (&hi0)[S],
     for (int j = 0; j < S - 1; ++j) hi1[j] = hi1[j + 1];
     hi1[S - 1] = last ? out_up : in_dn;
   }
-  // Round boundary (empty asm, no instruction), as in v_round_shift.
+  // Round boundary (empty asm, no instruction): V values pinned to registers
+  // and memory accesses kept on their side, so rounds do not overlap.
 #pragma unroll
   for (int r = 0; r < S; ++r) {
     asm volatile("" : "+v"(lo0[r]), "+v"(hi0[r]));
@@ -892,7 +842,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
       // treat as new in every group (empty asm): otherwise it hoists all the
       // addresses out of the loop and keeps them live (spilled) through the A
       // update.
-      if constexpr (two_v_rows(NP)) {
+      {
         // Rows r and r + N / 2 on the same lane (8 lanes per row), so each
         // rotation read from the history serves two rows.
         const int vrow = tid >> 3;
@@ -921,33 +871,6 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
           Vg[hi_off - j * step] = vh0[j];
           Vg[lo_off + m + j * step] = vl1[j];
           Vg[hi_off + m - j * step] = vh1[j];
-        }
-      } else {
-        // Two passes of N / 2 rows (8 lanes per row).
-#pragma unroll 1
-        for (int pass = 0; pass < 2; ++pass) {
-          const int vrow = pass * m + (tid >> 3);
-          const int vh = tid & 7;
-          uint32_t lo_off = static_cast<uint32_t>(vrow + vh * S * ld);
-          uint32_t hi_off = static_cast<uint32_t>(vrow + (N - 1 - vh * S) * ld);
-          uint32_t step = static_cast<uint32_t>(ld);
-          asm volatile("" : "+v"(lo_off), "+v"(hi_off), "+s"(step));
-          float vl[S], vhv[S];
-#pragma unroll
-          for (int j = 0; j < S; ++j) {
-            vl[j] = Vg[lo_off + j * step];
-            vhv[j] = Vg[hi_off - j * step];
-          }
-#pragma unroll 1
-          for (int c = 0; c < vchunks; ++c)
-            v_group_shift<S, VC>(vl, vhv, hist + c * VC * hrow, hrow, vh);
-          if (vtail) v_group_shift<S, VC - 1>(vl, vhv, hist + vchunks * VC * hrow, hrow, vh);
-          asm volatile("" : "+v"(lo_off), "+v"(hi_off), "+s"(step));
-#pragma unroll
-          for (int j = 0; j < S; ++j) {
-            Vg[lo_off + j * step] = vl[j];
-            Vg[hi_off - j * step] = vhv[j];
-          }
         }
       }
     }
@@ -1128,8 +1051,8 @@ hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t
   // The kernel's shape assumptions: whole parts per row, at most 1024 lanes
   // (the count the kernel's item dealing is compiled for), a thread for each
   // of the N / 4 + 1 pair-phase owners, every dealt super-block in some
-  // thread's list, LDS within 160 KiB, and for V in device memory two passes
-  // of N / 2 rows of 8 lanes.
+  // thread's list, LDS within 160 KiB, and for V in device memory N / 2 rows
+  // of 8 lanes (two rows per lane in the updates, two passes in the output).
   const int offblocks = m * (m - 1) / 2;
   if (parts_for_order(N) * S != m || threads > 1024 || threads % 64 != 0 || threads < m ||
       threads != threads_for_order(N) || threads < N / 4 + 1 ||
