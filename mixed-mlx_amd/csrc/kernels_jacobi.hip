@@ -312,6 +312,9 @@ __device__ __forceinline__ void v_round_shift2(float (&lo0)[S], float (&hi0)[S],
       }
     }
   }
+  // The shift: lo[S - 1] goes to the next part's lo[0] (the last part's to
+  // its own hi[S - 1]), hi[0] to the previous part's hi[S - 1] (part 0's to
+  // its own lo[1]); position 0 stays.
   const bool first = h == 0, last = h == 7;
   {
     const float out_up = lo0[S - 1], out_dn = hi0[0], keep0 = lo0[0];
