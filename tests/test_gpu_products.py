@@ -199,6 +199,50 @@ def test_jacobi_tolerance_mode(runtime):
         assert off <= 2e-5 * np.linalg.norm(A)
 
 
+@pytest.mark.parametrize("fixed", [3, 0])
+def test_jacobi_orders_with_v_in_device_memory(runtime, fixed):
+    """Padded orders 144..256 keep V in device memory and update it once per
+    group of rounds, as many as the LDS beside A holds history for (24 to 144
+    rounds), groups ending at sweep boundaries.  One matrix per padded order
+    (odd orders, so the padding is live), with fixed sweeps (the last group
+    of each sweep is one round short of whole chunks) and with the tolerance
+    stop.  V must stay in step with A's rounds: the off-norm of V^T A V
+    (binary64, from the returned V) equals the kernel's own off-norm of its
+    rotated A to the FP32 drift, V is orthogonal and V diag(d) V^T
+    reproduces A up to the off-norm left.  After 3 of the oracle's sweeps of
+    the same padded schedule the off-norm is of the same size; it is not
+    compared closer, because the early iterates are chaotic (float32 against
+    binary64 angles, or a 1e-7 relative perturbation of A, move the oracle's
+    own off-norm after 3 sweeps by 2 to 9% on these matrices)."""
+    orders = [137, 150, 170, 185, 200, 215, 230, 250]  # padded 144, 160, ..., 256
+    mats = [G.single_matrix(12, n, 1e-3, index=i)[0] for i, n in enumerate(orders)]
+    say("jacobi V in device memory", orders, "fixed", fixed)
+    opts = capi.jacobi_options(fixed_sweeps=fixed) if fixed else capi.jacobi_options(off_tol=1e-7)
+    Vs, Ds, _, reps = _jacobi_batch(runtime, mats, opts)
+    for A, V, d, rep in zip(mats, Vs, Ds, reps):
+        n = A.shape[0]
+        N = (n + 15) // 16 * 16
+        normA = np.linalg.norm(A)
+        drift = 2.0 * 2.0**-23 * normA * np.sqrt(max(rep.sweeps, 1) * N)
+        V64 = V.astype(np.float64)
+        VAV = V64.T @ A @ V64
+        off = np.sqrt(max(np.sum(VAV**2) - np.sum(np.diag(VAV) ** 2), 0.0))
+        assert abs(off - rep.off_norm) <= 2.0 * drift, (n, off, rep.off_norm, drift)
+        if fixed:
+            assert rep.sweeps == fixed
+            _, _, info = S.jacobi_f32(A, sweeps=fixed, pad_to=N)
+            assert 0.75 * info["off_norm"] <= rep.off_norm <= 1.33 * info["off_norm"], (
+                n, rep.off_norm, info["off_norm"])
+        else:
+            assert 3 <= rep.sweeps < 30
+            assert rep.off_history[rep.sweeps - 1] <= 1e-5
+        lam = np.linalg.eigvalsh(A)
+        assert np.linalg.norm(np.sort(d) - lam) <= rep.off_norm + drift
+        assert S.orthogonality_error(V) < 2e-6 * np.sqrt(n)
+        err = np.linalg.norm((V * d) @ V.T - A)
+        assert err <= rep.off_norm + 4e-6 * np.sqrt(n) * normA, (n, err, rep.off_norm)
+
+
 @pytest.mark.parametrize("n", [257, 300, 520])
 def test_jacobi_large_order_blocked_path(runtime, n):
     """Orders above 256: the blocked Jacobi (64 x 64 subproblems, rotations as
