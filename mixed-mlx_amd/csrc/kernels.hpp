@@ -68,7 +68,7 @@ struct JacobiStatus {
 // One launch covers matrices sharing the padded order (order rounded up to
 // 16: the launch's block size and LDS size follow it).
 uint32_t batched_jacobi_padded_order(uint32_t order);
-int batched_jacobi_threads(uint32_t order);
+int batched_jacobi_threads(uint32_t order, uint32_t count);
 size_t batched_jacobi_lds_bytes(uint32_t order);
 hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t order,
                                  const double* a64, float* vectors, float* diagonal,

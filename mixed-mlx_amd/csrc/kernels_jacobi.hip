@@ -170,15 +170,36 @@ __host__ __device__ constexpr int group_for_order(int N) {
   l = l < sweep_chunks ? l : sweep_chunks;
   return C * (l < 1 ? 1 : l);
 }
-__host__ __device__ constexpr int threads_for_order(int N) {
-  return N <= 128 ? ((N + (64 / (N / 16)) - 1) / (64 / (N / 16))) * 64 : 4 * N;
+// Lanes of the register-resident V: one row per H = N / 16 lanes, whole waves.
+__host__ __device__ constexpr int v_threads_for_order(int N) {
+  return ((N + (64 / (N / 16)) - 1) / (64 / (N / 16))) * 64;
+}
+// Threads per workgroup.  N > 128: 4 N.  N <= 128: the V lanes, or (wide)
+// one thread per off-diagonal block where that fits 1024 lanes and exceeds
+// the V lanes (N = 32 .. 80), so that a round's block phase is a single
+// block per thread; the waves past the V lanes skip the V updates.  A
+// launch whose workgroups are all resident at once is bound by the latency
+// of a round (the pair phase runs after its thread's block), which the wide
+// shape shortens (one 64 x 64 sweep, as in the subproblems of the blocked
+// Jacobi: 75 -> 55 us); a launch with more workgroups than that is bound by
+// throughput, where the narrow shape's fuller CUs win (measured on MI355X:
+// 20,000 mixed orders up to 512 ran 13-19% slower all wide).
+__host__ __device__ constexpr int threads_for_order(int N, bool wide = false) {
+  if (N > 128) return 4 * N;
+  const int v = v_threads_for_order(N);
+  const int blocks = ((N / 2) * (N / 2 - 1) / 2 + 63) / 64 * 64;
+  return wide && blocks <= 1024 && blocks > v ? blocks : v;
+}
+__host__ __device__ constexpr bool has_wide_shape(int N) {
+  return threads_for_order(N, true) != threads_for_order(N, false);
 }
 
 // Off-diagonal 2x2 blocks per thread when each thread takes single blocks:
 // the m (m - 1) / 2 blocks over the launch's threads, m = N / 2, the
 // next-pair blocks among them (one per thread below m).
-__host__ __device__ constexpr int single_blocks_per_thread(int N) {
-  return ((N / 2) * (N / 2 - 1) / 2 + threads_for_order(N) - 1) / threads_for_order(N);
+__host__ __device__ constexpr int single_blocks_per_thread(int N, bool wide = false) {
+  return ((N / 2) * (N / 2 - 1) / 2 + threads_for_order(N, wide) - 1) /
+         threads_for_order(N, wide);
 }
 // Orders whose block phase is bound by LDS throughput rather than by the
 // latency of a round take super-blocks (below); smaller orders take single
@@ -559,7 +580,7 @@ __device__ __forceinline__ uint32_t item_super(int r, int m2) {
 // down, and one value crosses to each neighbouring part of the row (adjacent
 // lanes).  The rotations of a group of rounds are buffered in LDS and applied
 // to V after the group.
-template <int NP>
+template <int NP, bool WIDE>
 __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     const JacobiJob* __restrict__ jobs, const double* __restrict__ a64,
     float* __restrict__ vectors, float* __restrict__ diagonal,
@@ -571,7 +592,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   constexpr int SP = padded_slots(S);
   constexpr bool SUPER = use_super_blocks(NP);
   constexpr int IPT = items_per_thread(NP);
-  constexpr int BPT = single_blocks_per_thread(NP);
+  constexpr int BPT = single_blocks_per_thread(NP, WIDE);
   const JacobiJob job = jobs[blockIdx.x];
   const int tid = threadIdx.x;
   const int nt = blockDim.x;
@@ -587,7 +608,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   constexpr int packed = N * (N + 1) / 2;
   constexpr int m2 = m / 2;  // slot pairs
   constexpr int items = super_items(NP);
-  constexpr int nthreads = threads_for_order(NP);  // == blockDim.x (host check)
+  constexpr int nthreads = threads_for_order(NP, WIDE);  // == blockDim.x (host check)
 
   // Static LDS: every array at a compile-time offset (addresses fold into
   // the ds instructions' offset fields).
@@ -833,10 +854,11 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     const int vchunks = (group_end - group_start) / VC;
     const bool vtail = (group_end - group_start) % VC != 0;
     if constexpr (!VMEM) {
-      // Every lane runs the V code (lanes past the last whole row of a wave
-      // compute on zeros that no valid lane reads): no divergent branch around
-      // the 2 S live V registers.  G = S: every group starts at phase 0.
-      v_group<S, G, 0>(lo, hi, hist, hrow, h, H, lane);
+      // Every lane of the V waves runs the V code (lanes past the last whole
+      // row of a wave compute on zeros that no valid lane reads): no divergent
+      // branch around the 2 S live V registers.  Waves past the V lanes skip
+      // it (a wave-uniform branch).  G = S: every group starts at phase 0.
+      if (wave * rows_per_wave < N) v_group<S, G, 0>(lo, hi, hist, hrow, h, H, lane);
     } else {
       // Each lane loads its positions, applies the group's rounds with the
       // values shifting along the circle in registers, and stores every
@@ -1016,20 +1038,36 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
 }
 
 template <int NP>
-hipError_t launch_order(const JacobiJob* jobs, uint32_t count, int threads, const double* a64,
-                        float* vectors, float* diagonal, float* roots, JacobiStatus* status,
-                        hipStream_t stream) {
-  batched_jacobi_kernel<NP><<<count, threads, 0, stream>>>(jobs, a64, vectors, diagonal, roots,
-                                                           status);
+hipError_t launch_order(const JacobiJob* jobs, uint32_t count, int threads, bool wide,
+                        const double* a64, float* vectors, float* diagonal, float* roots,
+                        JacobiStatus* status, hipStream_t stream) {
+  if constexpr (has_wide_shape(NP)) {
+    if (wide) {
+      batched_jacobi_kernel<NP, true><<<count, threads, 0, stream>>>(jobs, a64, vectors, diagonal,
+                                                                     roots, status);
+      return hipGetLastError();
+    }
+  }
+  batched_jacobi_kernel<NP, false><<<count, threads, 0, stream>>>(jobs, a64, vectors, diagonal,
+                                                                  roots, status);
   return hipGetLastError();
+}
+
+// Wide shape when every workgroup of the launch is resident at once: the
+// kernels of these orders use about 100 VGPRs, 20 waves per CU.
+bool use_wide_shape(int N, uint32_t count) {
+  if (!has_wide_shape(N)) return false;
+  const uint32_t waves = static_cast<uint32_t>(threads_for_order(N, true) / 64);
+  return count * waves <= 256u * 20u;
 }
 
 }  // namespace
 
 uint32_t batched_jacobi_padded_order(uint32_t order) { return (order + 15u) & ~15u; }
 
-int batched_jacobi_threads(uint32_t order) {
-  return threads_for_order(static_cast<int>(batched_jacobi_padded_order(order)));
+int batched_jacobi_threads(uint32_t order, uint32_t count) {
+  const int N = static_cast<int>(batched_jacobi_padded_order(order));
+  return threads_for_order(N, use_wide_shape(N, count));
 }
 
 size_t batched_jacobi_lds_bytes(uint32_t order) {
@@ -1048,7 +1086,8 @@ hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t
   if (order < 1 || order > kBatchedJacobiMaxOrder) return hipErrorInvalidValue;
   const int N = static_cast<int>(batched_jacobi_padded_order(order));
   const int S = slots_for_order(N);
-  const int threads = batched_jacobi_threads(order);
+  const bool wide = use_wide_shape(N, count);
+  const int threads = batched_jacobi_threads(order, count);
   const size_t lds = batched_jacobi_lds_bytes(order);
   const int m = N / 2;
   // The kernel's shape assumptions: whole parts per row, at most 1024 lanes
@@ -1058,14 +1097,16 @@ hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t
   // of 8 lanes (two rows per lane in the updates, two passes in the output).
   const int offblocks = m * (m - 1) / 2;
   if (parts_for_order(N) * S != m || threads > 1024 || threads % 64 != 0 || threads < m ||
-      threads != threads_for_order(N) || threads < N / 4 + 1 ||
+      threads != threads_for_order(N, wide) || threads < N / 4 + 1 ||
       super_items(N) > items_per_thread(N) * threads ||
-      single_blocks_per_thread(N) * threads < offblocks || m < 8 || lds > 160 * 1024 ||
+      single_blocks_per_thread(N, wide) * threads < offblocks || m < 8 || lds > 160 * 1024 ||
       (v_in_memory(S) && threads != 8 * m))
     return hipErrorInvalidValue;
   switch (N) {
 #define MSQ_ORDER(NV) \
-    case NV: return launch_order<NV>(jobs, count, threads, a64, vectors, diagonal, roots, status, stream);
+    case NV:                                                                                  \
+      return launch_order<NV>(jobs, count, threads, wide, a64, vectors, diagonal, roots, status, \
+                              stream);
     MSQ_ORDER(16) MSQ_ORDER(32) MSQ_ORDER(48) MSQ_ORDER(64) MSQ_ORDER(80) MSQ_ORDER(96)
     MSQ_ORDER(112) MSQ_ORDER(128) MSQ_ORDER(144) MSQ_ORDER(160) MSQ_ORDER(176) MSQ_ORDER(192)
     MSQ_ORDER(208) MSQ_ORDER(224) MSQ_ORDER(240) MSQ_ORDER(256)
