@@ -225,6 +225,31 @@ __host__ __device__ constexpr bool use_super_blocks(int N) { return N >= 256; }
 __host__ __device__ constexpr int super_items(int N) {
   return (N / 4 - 1) * (N / 4 - 2) / 2 + (N / 4 - 2);
 }
+// Sums of one or two doubles per thread in a fixed order: thread values in
+// part0 / part1 (nt entries each), then a tree over thread indices (stride
+// halving from the power of two below nt).  The result does not depend on
+// which waves arrive first, so a matrix's norms and stop decisions are the
+// same in every launch and whatever else shares the batch.  Every thread
+// returns the sums; part0 / part1 are free again on return.
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* part0, double* part1,
+                                           int tid, int nt) {
+  part0[tid] = a;
+  part1[tid] = b;
+  __syncthreads();
+  int s = 1;
+  while (2 * s < nt) s *= 2;
+  for (; s > 0; s >>= 1) {
+    if (tid < s && tid + s < nt) {
+      part0[tid] += part0[tid + s];
+      part1[tid] += part1[tid + s];
+    }
+    __syncthreads();
+  }
+  a = part0[0];
+  b = part1[0];
+  __syncthreads();
+}
+
 __host__ __device__ constexpr int items_per_thread(int N) {
   return super_items(N) / threads_for_order(N) + 1;
 }
@@ -615,9 +640,22 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   __shared__ __align__(16) float Ap[(packed + 3) & ~3];
   __shared__ SlotRec srec[2][m];  // round parity x slot
   __shared__ float2 hist[G * hrow];  // G rounds x 8 parts x SP of (c, s') for V
-  __shared__ double red[32];
   __shared__ float redf[16];
   __shared__ int ctrl[4];
+  // Fixed-order reductions (block_sum2) need two doubles per thread.  Above
+  // N = 128 they use the rotation history, idle at both reduction points
+  // (after the s_i = sqrt(a_ii) values the convergence test keeps at its
+  // start); below, where LDS is plentiful, an array of their own.
+  constexpr bool kPartInHistory = NP > 128;
+  constexpr int kSrootBytes = (4 * NP + 7) / 8 * 8;
+  static_assert(!kPartInHistory ||
+                    sizeof(float2) * G * hrow >= kSrootBytes + 2 * sizeof(double) * nthreads,
+                "room for the reduction partials in the rotation history");
+  __shared__ double partbuf[kPartInHistory ? 2 : 2 * nthreads];
+  double* const part0 =
+      kPartInHistory ? reinterpret_cast<double*>(reinterpret_cast<char*>(hist) + kSrootBytes)
+                     : partbuf;
+  double* const part1 = part0 + nthreads;
 
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -662,7 +700,6 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   }
 
   const double* A = a64 + job.a_offset;
-  if (tid == 0) red[0] = 0.0;
   double norm2 = 0.0;
   for (int e = tid; e < packed; e += nt) {
     const int j = tri_col(e);
@@ -693,12 +730,11 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
       Vg[i + static_cast<size_t>(x) * ld] = i == x ? 1.0f : 0.0f;
     }
   }
-  // Block reductions go through LDS atomics: wave shuffles would keep their
-  // lane-address registers live through the whole kernel.
-  __syncthreads();
-  atomicAdd(&red[0], norm2);
-  __syncthreads();
-  const double matrix_norm = sqrt(red[0]);
+  // Block reductions go through LDS (wave shuffles would keep their
+  // lane-address registers live through the whole kernel), in a fixed order.
+  double unused = 0.0;
+  block_sum2(norm2, unused, part0, part1, tid, nt);
+  const double matrix_norm = sqrt(norm2);
 
   const uint32_t fixed = job.fixed_sweeps;
   const uint32_t cap = fixed ? fixed : job.max_sweeps;
@@ -911,11 +947,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     // s_i = sqrt(a_ii) in the history buffer (idle until the next group).
     float* sroot = reinterpret_cast<float*>(hist);
     for (int i = tid; i < N; i += nt) sroot[i] = i < n ? sqrtf(fmaxf(Ap[tri(i, i)], 0.0f)) : 0.0f;
-    if (tid == 0) {
-      red[1] = 0.0;
-      red[2] = 0.0;
-      redf[0] = INFINITY;
-    }
+    if (tid == 0) redf[0] = INFINITY;
     __syncthreads();
     for (int e = tid; e < packed; e += nt) {
       const int j = tri_col(e);
@@ -932,16 +964,14 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
         g2 += 2.0 * static_cast<double>(g * g);
       }
     }
-    if (off2 != 0.0) atomicAdd(&red[1], off2);
-    if (g2 != 0.0) atomicAdd(&red[2], g2);
+    // (The minimum is exact in any order.)
     if (dmin < INFINITY) atomicMin(&redf[0], dmin);
-    __syncthreads();
+    block_sum2(off2, g2, part0, part1, tid, nt);
     if (tid == 0) {
-      const double s2 = red[1];
       const float d = redf[0];
-      off = sqrt(s2);
+      off = sqrt(off2);
       min_diag = d;
-      rate = d > 0.0f ? sqrt(red[2]) / sqrt(static_cast<double>(d)) : INFINITY;
+      rate = d > 0.0f ? sqrt(g2) / sqrt(static_cast<double>(d)) : INFINITY;
       if (sweeps >= 1 && sweeps <= kJacobiHistory)
         st->off_history[sweeps - 1] = static_cast<float>(off / matrix_norm);
       int stop = at_limit ? 1 : 0;
@@ -1075,8 +1105,12 @@ size_t batched_jacobi_lds_bytes(uint32_t order) {
   const size_t packed = static_cast<size_t>(N) * (N + 1) / 2;
   const int S = slots_for_order(N);
   const size_t hist = static_cast<size_t>(group_for_order(N)) * 8 * padded_slots(S);
+  // Reduction partials: in the history above N = 128, an array of their own
+  // (two doubles per thread of the widest shape) below.
+  const size_t part = N > 128 ? 2 * sizeof(double)
+                              : 2 * sizeof(double) * static_cast<size_t>(threads_for_order(N, true));
   return ((packed + 3) & ~size_t{3}) * sizeof(float) + 2 * (N / 2) * sizeof(SlotRec) +
-         hist * sizeof(float2) + 32 * sizeof(double) + 16 * sizeof(float) + 4 * sizeof(int);
+         hist * sizeof(float2) + part + 16 * sizeof(float) + 4 * sizeof(int);
 }
 
 hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t order,
