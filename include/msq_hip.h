@@ -234,6 +234,24 @@ int msq_jacobi_batched(msq_runtime *runtime, uint32_t count, const uint32_t *ord
                        float *roots, const msq_jacobi_options *options,
                        uint32_t options_count, msq_jacobi_report *reports);
 
+/* No counterpart in the reference.  msq_jacobi_batched for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Same arguments, results and reports as
+ * msq_jacobi_batched (bit for bit), for orders 1..256 and at most
+ * MSQ_NOALLOC_MAX_MATRICES matrices per call; larger orders or counts are
+ * rejected with MSQ_ERROR_INVALID_ARGUMENT.  It allocates nothing: its launch
+ * plan and per-matrix status live in one static device array of
+ * MSQ_NOALLOC_SCRATCH_BYTES per GPU, which the library holds until the process
+ * ends.  Every call on a GPU shares that GPU's array, so calls that overlap on
+ * one GPU (two runtimes on it, or two host threads) race; calls in sequence do
+ * not (each call rewrites the entries it reads). */
+#define MSQ_NOALLOC_MAX_MATRICES 4096
+#define MSQ_NOALLOC_SCRATCH_BYTES 1703936
+int msq_jacobi_batched_noalloc(msq_runtime *runtime, uint32_t count, const uint32_t *orders,
+                               const double *matrices, float *vectors, float *diagonal,
+                               float *roots, const msq_jacobi_options *options,
+                               uint32_t options_count, msq_jacobi_report *reports);
+
 /* --------------------------------------------------------------- products */
 
 /* reconstructTiled / reconstructLowFactorTiled (src/metal_runtime.cpp:429-534)

@@ -28,7 +28,8 @@ FLAGS = [
 ]
 
 SOURCES = ["kernels_codec.hip", "kernels_jacobi.hip", "kernels_gemm.hip",
-           "kernels_misc.hip", "kernels_mlp.hip", "kernels_blocked.hip", "capi.cpp"]
+           "kernels_misc.hip", "kernels_mlp.hip", "kernels_blocked.hip", "noalloc.hip",
+           "capi.cpp"]
 
 
 def compile_one(src: str) -> pathlib.Path:

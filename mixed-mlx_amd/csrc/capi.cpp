@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "noalloc.hpp"
 
 namespace {
 
@@ -536,43 +537,77 @@ void plan_jacobi(const Layout& L, const msq_jacobi_options* opts, uint32_t opts_
   P.launches.push_back({P.jobs.size(), 0});
 }
 
-// The batched launches of a plan whose jobs are already on the device
-// (device work only: no host synchronisation, capturable in a graph).
-int enqueue_jacobi(msq_runtime* rt, const JacobiPlan& P, const msq::JacobiJob* djobs,
-                   const double* a64, float* V, float* diag, float* roots,
-                   msq::JacobiStatus* jstatus) {
-  if (P.jobs.empty()) return MSQ_OK;
-  // Order classes run concurrently on the runtime's lanes: a launch lasts as
-  // long as its slowest matrix, and the CUs its faster matrices free take
-  // work from the other classes.  Each lane is in order, so the classes are
-  // dealt longest first to the lane with the least work so far (work: sum of
-  // n^3 over the class), which keeps the lanes' totals level.
-  const size_t classes = P.launches.size() - 1;
-  const int used = classes > 1 ? static_cast<int>(std::min<size_t>(classes, kLanes)) : 0;
+// Launch classes of a batched Jacobi (padded orders 16..256: at most 16).
+constexpr int kMaxJacobiClasses = 16;
+struct JacobiClass {
+  size_t first = 0;      // first job of the class
+  uint32_t count = 0;    // jobs
+  uint32_t order = 0;    // padded order
+  double cost = 0.0;     // sum of n^3
+};
+
+// The batched launches of `classes` (jobs already on the device): device work
+// only, no host synchronisation and no allocation (capturable in a graph).
+// Order classes run concurrently on the runtime's lanes: a launch lasts as
+// long as its slowest matrix, and the CUs its faster matrices free take work
+// from the other classes.  Each lane is in order, so the classes are dealt
+// longest first to the lane with the least work so far (work: sum of n^3 over
+// the class), which keeps the lanes' totals level.
+int enqueue_jacobi_classes(msq_runtime* rt, const JacobiClass* cls, int classes,
+                           const msq::JacobiJob* djobs, const double* a64, float* V, float* diag,
+                           float* roots, msq::JacobiStatus* jstatus) {
+  if (classes <= 0) return MSQ_OK;
+  if (classes > kMaxJacobiClasses)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "%d Jacobi classes", classes);
+  const int used = classes > 1 ? std::min(classes, kLanes) : 0;
   if (used) {
     HIP_TRY(hipEventRecord(rt->fork, rt->stream));
     for (int l = 0; l < used; ++l) HIP_TRY(hipStreamWaitEvent(rt->lanes[l], rt->fork, 0));
   }
-  std::vector<size_t> order(classes);
-  for (size_t c = 0; c < classes; ++c) order[c] = c;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](size_t x, size_t y) { return P.cost[x] > P.cost[y]; });
+  // Stable order by decreasing cost (insertion sort on the stack).
+  int order[kMaxJacobiClasses];
+  for (int c = 0; c < classes; ++c) {
+    int k = c;
+    while (k > 0 && cls[order[k - 1]].cost < cls[c].cost) {
+      order[k] = order[k - 1];
+      --k;
+    }
+    order[k] = c;
+  }
   double load[kLanes] = {};
-  for (size_t c : order) {
-    const size_t b = P.launches[c].first, e = P.launches[c + 1].first;
+  for (int q = 0; q < classes; ++q) {
+    const JacobiClass& c = cls[order[q]];
     int lane = 0;
     for (int l = 1; l < used; ++l)
       if (load[l] < load[lane]) lane = l;
-    if (used) load[lane] += P.cost[c];
+    if (used) load[lane] += c.cost;
     hipStream_t s = used ? rt->lanes[lane] : rt->stream;
-    HIP_TRY(msq::launch_batched_jacobi(djobs + b, static_cast<uint32_t>(e - b),
-                                       P.launches[c].second, a64, V, diag, roots, jstatus, s));
+    HIP_TRY(msq::launch_batched_jacobi(djobs + c.first, c.count, c.order, a64, V, diag, roots,
+                                       jstatus, s));
   }
   for (int l = 0; l < used; ++l) {
     HIP_TRY(hipEventRecord(rt->joins[l], rt->lanes[l]));
     HIP_TRY(hipStreamWaitEvent(rt->stream, rt->joins[l], 0));
   }
   return MSQ_OK;
+}
+
+// The batched launches of a plan whose jobs are already on the device.
+int enqueue_jacobi(msq_runtime* rt, const JacobiPlan& P, const msq::JacobiJob* djobs,
+                   const double* a64, float* V, float* diag, float* roots,
+                   msq::JacobiStatus* jstatus) {
+  if (P.jobs.empty()) return MSQ_OK;
+  const int classes = static_cast<int>(P.launches.size() - 1);
+  JacobiClass cls[kMaxJacobiClasses];
+  if (classes > kMaxJacobiClasses)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "%d Jacobi classes", classes);
+  for (int c = 0; c < classes; ++c) {
+    cls[c].first = P.launches[c].first;
+    cls[c].count = static_cast<uint32_t>(P.launches[c + 1].first - P.launches[c].first);
+    cls[c].order = P.launches[c].second;
+    cls[c].cost = P.cost[c];
+  }
+  return enqueue_jacobi_classes(rt, cls, classes, djobs, a64, V, diag, roots, jstatus);
 }
 
 // Jacobi on a whole batch.  djobs: device room for `count` jobs; jstatus:
@@ -1188,6 +1223,9 @@ int msq_runtime_create(int device, msq_runtime** out) {
   if (e == hipSuccess) e = hipEventCreateWithFlags(&rt->fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&rt->graph_in, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&rt->graph_out, hipEventDisableTiming);
+  // The allocation-free entry point's scratch address on this device (once
+  // per device; the device is current here).
+  if (e == hipSuccess) e = msq::noalloc_register(device);
   if (e != hipSuccess) {
     msq_runtime_destroy(rt);
     return set_error(MSQ_ERROR_HIP, "runtime streams: %s", hipGetErrorString(e));
@@ -1410,6 +1448,72 @@ int msq_jacobi_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   HIP_TRY(hipStreamSynchronize(rt->stream));
   if (reports)
     for (uint32_t i = 0; i < count; ++i) fill_jacobi_report(hs[i], reports[i]);
+  return MSQ_OK;
+}
+
+// msq_jacobi_batched without allocation: plan, status and reports in the
+// per-device static scratch (noalloc.hip); no hipMalloc, hipFree or host heap
+// use.  The plan kernel lays the jobs out as plan_jacobi does (classes of
+// padded order, largest first, matrices in index order within a class), so
+// the launches are the same as the allocating entry point's.
+int msq_jacobi_batched_noalloc(msq_runtime* rt, uint32_t count, const uint32_t* orders,
+                               const double* matrices, float* vectors, float* diagonal,
+                               float* roots, const msq_jacobi_options* options,
+                               uint32_t options_count, msq_jacobi_report* reports) {
+  MSQ_TRY(activate(rt));
+  if (count == 0 || count > msq::kNoallocMaxMatrices)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT,
+                     "count %u outside [1, %u] (the allocation-free scratch holds %u matrices)",
+                     count, msq::kNoallocMaxMatrices, msq::kNoallocMaxMatrices);
+  if (!orders || !matrices || !vectors || !diagonal || !roots || !options)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  if (options_count != 1 && options_count != count)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "options_count must be 1 or count");
+  uint32_t class_count[msq::kNoallocClasses] = {};
+  double class_cost[msq::kNoallocClasses] = {};
+  for (uint32_t i = 0; i < count; ++i) {
+    const uint32_t o = orders[i];
+    if (o < 1 || o > msq::kBatchedJacobiMaxOrder)
+      return set_error(MSQ_ERROR_INVALID_ARGUMENT,
+                       "order %u of matrix %u is outside [1, %u] (the allocation-free entry "
+                       "point runs the LDS-resident Jacobi only)",
+                       o, i, msq::kBatchedJacobiMaxOrder);
+    const int c = msq::noalloc_class(o);
+    ++class_count[c];
+    class_cost[c] += static_cast<double>(o) * o * o;
+  }
+  const msq::NoallocScratch s = msq::noalloc_scratch(rt->device);
+  if (!s.jobs)
+    return set_error(MSQ_ERROR_HIP, "device %d has no allocation-free scratch", rt->device);
+  // Classes largest first, as plan_jacobi lays them out.
+  msq::NoallocClasses starts{};
+  JacobiClass cls[kMaxJacobiClasses];
+  int classes = 0;
+  uint32_t pos = 0;
+  for (int c = msq::kNoallocClasses - 1; c >= 0; --c) {
+    starts.start[c] = pos;
+    if (class_count[c]) {
+      cls[classes].first = pos;
+      cls[classes].count = class_count[c];
+      cls[classes].order = static_cast<uint32_t>(16 * (c + 1));
+      cls[classes].cost = class_cost[c];
+      ++classes;
+    }
+    pos += class_count[c];
+  }
+  HIP_TRY(hipMemcpyAsync(s.orders, orders, count * sizeof(uint32_t), hipMemcpyHostToDevice,
+                         rt->stream));
+  HIP_TRY(hipMemcpyAsync(s.options, options, options_count * sizeof(msq_jacobi_options),
+                         hipMemcpyHostToDevice, rt->stream));
+  HIP_TRY(msq::launch_noalloc_plan(s, options_count, count, starts, rt->stream));
+  MSQ_TRY(enqueue_jacobi_classes(rt, cls, classes, s.jobs, matrices, vectors, diagonal, roots,
+                                 s.status));
+  if (reports) {
+    HIP_TRY(msq::launch_noalloc_reports(s, count, rt->stream));
+    HIP_TRY(hipMemcpyAsync(reports, s.reports, count * sizeof(msq_jacobi_report),
+                           hipMemcpyDeviceToHost, rt->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(rt->stream));
   return MSQ_OK;
 }
 

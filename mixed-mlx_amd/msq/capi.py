@@ -56,6 +56,7 @@ MLP_HIDDEN = 64
 MLP_OUTPUTS = 8
 MLP_PARAMS = 5768
 FEATURE_MAX_ORDER = 512
+NOALLOC_MAX_MATRICES = 4096
 
 # Every symbol include/msq_hip.h declares.
 EXPORTED_SYMBOLS = (
@@ -65,7 +66,7 @@ EXPORTED_SYMBOLS = (
     "msq_copy_to_device", "msq_copy_to_host", "msq_memset", "msq_storage_bytes",
     "msq_scale_bytes", "msq_batch_layout", "msq_quantize", "msq_decode",
     "msq_quantize_and_decode", "msq_quantize_fp8_scaled", "msq_decode_fp8_scaled",
-    "msq_jacobi_batched", "msq_reconstruct", "msq_residual_fp64", "msq_correction",
+    "msq_jacobi_batched", "msq_jacobi_batched_noalloc", "msq_reconstruct", "msq_residual_fp64", "msq_correction",
     "msq_orthogonality", "msq_solve_batched", "msq_last_factors",
     "msq_matrix_features", "msq_mlp_forward", "msq_mlp_backward", "msq_mlp_adam",
     "msq_autotune", "msq_set_tile_table",
@@ -149,6 +150,8 @@ _sig("msq_quantize_and_decode", _i, _p, _i, _p, _p, _p, _p, _sz)
 _sig("msq_quantize_fp8_scaled", _i, _p, _p, _p, _p, _sz)
 _sig("msq_decode_fp8_scaled", _i, _p, _p, _p, _p, _sz)
 _sig("msq_jacobi_batched", _i, _p, _u32, _u32p, _p, _p, _p, _p,
+     ctypes.POINTER(JacobiOptions), _u32, ctypes.POINTER(JacobiReport))
+_sig("msq_jacobi_batched_noalloc", _i, _p, _u32, _u32p, _p, _p, _p, _p,
      ctypes.POINTER(JacobiOptions), _u32, ctypes.POINTER(JacobiReport))
 _sig("msq_reconstruct", _i, _p, _i, _u32, _u32p, _p, _p, _p)
 _sig("msq_residual_fp64", _i, _p, _u32, _u32p, _p, _p, _p, _p, _f64p)
@@ -433,6 +436,17 @@ class Runtime:
         check(lib.msq_jacobi_batched(self.handle, len(orders), _u32_array(orders),
                                      _addr(matrices), _addr(vectors), _addr(diagonal),
                                      _addr(roots), arr, len(opts), reports))
+        return list(reports)
+
+    def jacobi_batched_noalloc(self, orders, matrices, vectors, diagonal, roots, options):
+        """jacobi_batched through the allocation-free entry point (orders <= 256,
+        at most NOALLOC_MAX_MATRICES matrices)."""
+        opts = options if isinstance(options, (list, tuple)) else [options]
+        arr = (JacobiOptions * len(opts))(*opts)
+        reports = (JacobiReport * len(orders))()
+        check(lib.msq_jacobi_batched_noalloc(self.handle, len(orders), _u32_array(orders),
+                                             _addr(matrices), _addr(vectors), _addr(diagonal),
+                                             _addr(roots), arr, len(opts), reports))
         return list(reports)
 
     def reconstruct(self, precision, orders, vectors, roots, out):
