@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define MSQ_ABI_VERSION 4
+#define MSQ_ABI_VERSION 5
 
 enum msq_status {
   MSQ_OK = 0,
@@ -63,20 +63,47 @@ enum msq_precision {
   MSQ_FP8_E4M3FN_SCALED = 16,
 };
 
+/* RoundingMode (src/precision.hpp:22): how quantization rounds. */
+enum msq_rounding_mode {
+  MSQ_ROUNDING_NEAREST = 0,
+  MSQ_ROUNDING_STOCHASTIC = 1,
+};
+
 typedef struct msq_runtime msq_runtime;
 
-/* Solver entry state of one matrix (solve status). */
+/* Solver entry state of one matrix (solve status): the reference's
+ * termination reasons (src/metal_solver.cpp:922-934) plus two of this
+ * backend's. */
 enum msq_solve_status {
   MSQ_SOLVE_RUNNING = 0,
   MSQ_SOLVE_CONVERGED = 1,
   MSQ_SOLVE_BUDGET_EXHAUSTED = 2,
   MSQ_SOLVE_DIVERGED = 3,
   MSQ_SOLVE_NONFINITE = 4,
+  /* a Jacobi eigenvalue that is not positive and finite */
   MSQ_SOLVE_INVALID_SPECTRUM = 5,
+  /* MSQ_STOPPING_STAGNATION stopped the corrections (the best iterate is
+   * returned) */
+  MSQ_SOLVE_STAGNATED = 6,
+  /* The residual met its target, but the smallest Jacobi eigenvalue is below
+   * what the FP32 eigensolve resolves (msq_solve_options.resolution), so the
+   * forward-error estimate that stopped the corrections is not reliable and
+   * the error may exceed the target.  refine_unresolved = 1 resolves it. */
+  MSQ_SOLVE_UNRESOLVED = 7,
 };
 
+/* KernelPath, CorrectionMode, NativeStoppingPolicy and NativeDispatchPolicy
+ * (src/precision.hpp:126-146), in the reference's order. */
+enum msq_kernel_path { MSQ_KERNEL_PATH_LEGACY = 0, MSQ_KERNEL_PATH_TILED = 1 };
+enum msq_correction_mode {
+  MSQ_CORRECTION_FP32_REORTHOGONALIZED = 0,
+  MSQ_CORRECTION_NATIVE_QUANTIZED = 1,
+};
+enum msq_stopping_policy { MSQ_STOPPING_FIXED_BUDGET = 0, MSQ_STOPPING_STAGNATION = 1 };
+enum msq_dispatch_policy { MSQ_DISPATCH_PER_STAGE = 0, MSQ_DISPATCH_FUSED = 1 };
+
 #define MSQ_JACOBI_HISTORY 32
-#define MSQ_SOLVE_HISTORY 20
+#define MSQ_SOLVE_HISTORY 65
 
 /* Jacobi options of one matrix.  The reference runs a fixed 8 sweeps
  * (src/metal_solver.cpp:210-213); fixed_sweeps = 8 reproduces that.  With
@@ -111,15 +138,42 @@ typedef struct msq_jacobi_report {
   float correction_rate;                 /* ||G||_F / min s after the last sweep */
 } msq_jacobi_report;
 
-/* Options of one matrix of msq_solve_batched.  The precisions select the
- * products: reconstruction X0 = G G^T with G = V diag(sqrt(s)) rounded to
- * basis_precision, and every correction product W = V^T R, Y = (W V) o H,
- * W2 = V Y, E = W2 V^T in correction_precision; MSQ_FP32 for both is the
- * reference's FP32 path.  Supported: MSQ_FP32, MSQ_BF16,
- * MSQ_FP8_E4M3FN_SCALED, MSQ_MXFP4_E2M1.  Corrections stop when the relative
- * FP64 residual ||A - X^2||_F / ||A||_F <= residual_tolerance or the forward
- * error estimate ||R||_F / (2 s_min ||X||_F) <= forward_tolerance, after at
- * least min_corrections and at most max_corrections (< MSQ_SOLVE_HISTORY). */
+/* Options of one matrix of msq_solve_batched: MetalSolverConfiguration and
+ * MetalProblem (src/metal_solver.hpp:12-31) for this backend.
+ * msq_solve_options_init fills the reference's defaults.
+ *
+ * Products: reconstruction X0 = G G^T with G = V diag(sqrt(s)) rounded to
+ * basis_precision, s the roots rounded to spectrum_precision (both decided per
+ * matrix: a basis whose ||V_t^T V_t - I||_F / sqrt(n) exceeds basis_tolerance,
+ * or a spectrum with a decoded root that is not positive and finite, falls
+ * back to FP32, as src/metal_solver.cpp:217-263 does); every correction product
+ * W = V^T R, Y = (W V) o H, W2 = V Y, E = W2 V^T in correction_precision, on
+ * the FP32 basis and roots.  MSQ_FP32 for all three is the reference's
+ * GPU-resident FP32 path.  basis / correction precisions: MSQ_FP32, MSQ_BF16,
+ * MSQ_FP8_E4M3FN_SCALED, MSQ_MXFP4_E2M1; spectrum: every msq_precision but
+ * MSQ_FP8_E4M3FN_SCALED.
+ *
+ * correction_mode MSQ_CORRECTION_NATIVE_QUANTIZED is the reference's native
+ * path (refineNativeStaged, src/metal_runtime.cpp:312-411): basis, roots,
+ * residual and every intermediate product are stored in basis_precision (FP8
+ * E4M3FN / E5M2, MXFP8, FP4 E2M1, MXFP4, the reference's unscaled codecs on
+ * its linear n x n layout), and the update is X = sym(X + E).  As in the
+ * reference (validateCorrectionMode, src/configuration.cpp:322-346) it needs
+ * basis_precision == spectrum_precision == correction_precision, one of those
+ * six, and kernel_path MSQ_KERNEL_PATH_LEGACY; stopping_policy
+ * MSQ_STOPPING_STAGNATION and dispatch_policy MSQ_DISPATCH_PER_STAGE need it.
+ * On this backend both kernel paths and both dispatch policies run the same
+ * device kernels on the runtime's stream (the reference's Legacy / Tiled and
+ * PerStage / Fused differ in kernel granularity and command buffers, not in
+ * the values computed).
+ *
+ * Corrections stop when the relative FP64 residual ||A - X^2||_F / ||A||_F <=
+ * residual_tolerance or the forward-error estimate ||R||_F / (2 s_min ||X||_F)
+ * <= forward_tolerance, after at least min_corrections and at most
+ * max_corrections (< MSQ_SOLVE_HISTORY); with MSQ_STOPPING_STAGNATION also
+ * after stagnation_window corrections in a row that improve the best residual
+ * by no more than minimum_relative_improvement (src/metal_solver.cpp:889-902),
+ * and the best iterate is returned. */
 typedef struct msq_solve_options {
   int32_t basis_precision;
   int32_t correction_precision;
@@ -128,6 +182,27 @@ typedef struct msq_solve_options {
   uint32_t max_corrections;
   double residual_tolerance;
   double forward_tolerance;
+  int32_t spectrum_precision;
+  int32_t kernel_path;      /* enum msq_kernel_path */
+  int32_t correction_mode;  /* enum msq_correction_mode */
+  int32_t stopping_policy;  /* enum msq_stopping_policy */
+  int32_t dispatch_policy;  /* enum msq_dispatch_policy */
+  uint32_t stagnation_window;
+  double minimum_relative_improvement;
+  /* Basis acceptance tolerance; 0: the tier's basisOrthogonalityTolerance
+   * (src/configuration.cpp:109-118: BF16 5e-3, 8-bit 0.15, 4-bit 0.5). */
+  double basis_tolerance;
+  /* No reference counterpart.  A matrix whose smallest Jacobi eigenvalue is
+   * below resolution * (largest) is unresolved: its forward-error estimate is
+   * not reliable (0: no test).  With refine_unresolved = 1 such a matrix gets
+   * a second Jacobi pass on V^T A V formed in binary64, whose small
+   * eigenvalues FP32 resolves to their own relative accuracy, and V <- V W;
+   * otherwise a matrix that stops unresolved reports MSQ_SOLVE_UNRESOLVED. */
+  float resolution;
+  uint32_t refine_unresolved;
+  /* 1: measure ||V^T V - I||_F / sqrt(n) of the FP32 basis for the certificate
+   * (runMetalCase measures it, src/metal_solver.cpp:511-524). */
+  uint32_t certificate;
 } msq_solve_options;
 
 typedef struct msq_solve_report {
@@ -145,12 +220,44 @@ typedef struct msq_solve_report {
   float off_history[MSQ_JACOBI_HISTORY];
   float correction_rate; /* Jacobi's ||G||_F / min s (see msq_jacobi_options) */
   /* Basis tier acceptance (src/metal_solver.cpp:217-236): ||V_t^T V_t - I||_F /
-   * sqrt(n) of V rounded to the basis tier (0 for an FP32 basis), and 1 when it
-   * exceeded the tier's tolerance (BF16 5e-3, 8-bit 0.15, 4-bit 0.5;
-   * src/configuration.cpp:109-118) and X0 was formed from the FP32 basis. */
+   * sqrt(n) of V rounded to the basis tier (0 for an FP32 basis). */
   float basis_orthogonality;
+  /* Tiers rejected for FP32: basis_rejected + spectrum_rejected. */
   uint32_t precision_escalations;
+  uint32_t basis_rejected;
+  uint32_t spectrum_rejected;
+  /* Frozen-factor certificate (src/metal_solver.cpp:511-524):
+   * 1 / (2 min s) over the frozen roots X0 was formed with; the basis
+   * orthogonality it rests on (the selected tier's, or the FP32 basis' when
+   * options.certificate = 1, else NaN); usable when that is below 1e-3.  The
+   * contraction bound 3 ||X0 - X*||_F / (2 min s) needs the FP64 root X*: the
+   * caller forms it from msq_solve_outputs.initial_roots. */
+  double frozen_sylvester_inverse_norm;
+  double frozen_basis_orthogonality;
+  uint32_t certificate_usable;
+  /* Smallest Jacobi eigenvalue below options.resolution (after the second
+   * pass, when it ran); refined: the second pass ran (its sweeps are in
+   * `sweeps` too). */
+  uint32_t unresolved;
+  uint32_t refined;
+  /* Best residual iterate (MSQ_STOPPING_STAGNATION returns it). */
+  uint32_t best_iteration;
+  uint32_t stagnation_steps;
+  uint32_t reserved;
+  double best_residual;
 } msq_solve_report;
+
+/* Outputs of msq_solve_batched_ex: `roots` (X, binary64, batch layout) is
+ * required; the others may be NULL.  initial_roots: X0 (binary64, batch
+ * layout) before the first correction; vectors: the FP32 eigenvectors the
+ * corrections used (float, batch layout); root_eigenvalues: their roots s =
+ * sqrt(diag) (float, vector layout). */
+typedef struct msq_solve_outputs {
+  double *roots;
+  double *initial_roots;
+  float *vectors;
+  float *root_eigenvalues;
+} msq_solve_outputs;
 
 /* ---------------------------------------------------------------- runtime */
 
@@ -170,6 +277,14 @@ int msq_runtime_destroy(msq_runtime *runtime);
 #define MSQ_OWN_STREAM ((void *)(~(uintptr_t)0))
 int msq_runtime_set_stream(msq_runtime *runtime, void *hip_stream);
 int msq_synchronize(msq_runtime *runtime);
+/* The rounding mode and seed of MetalRuntime's constructor
+ * (src/metal_runtime.cpp:41-126; default nearest, seed 0).  With
+ * MSQ_ROUNDING_STOCHASTIC every quantization of this runtime (the codec calls
+ * and a solve's basis and spectrum tiers) rounds each element up or down with
+ * probability given by its distance to the two neighbouring codes, decided by
+ * a hash of (element index, seed, rounding stream)
+ * (metal_mixed_sqrt.metal:394-405): results are reproducible for a seed. */
+int msq_runtime_set_rounding(msq_runtime *runtime, int mode, uint32_t seed);
 /* No counterpart in the reference.  enable != 0: msq_solve_batched records
  * the device work of each batch signature (orders, options, matrices and
  * roots pointers) in a HIP graph -- the first call runs eagerly, the second
@@ -189,6 +304,9 @@ int msq_memset(msq_runtime *runtime, void *dst, int value, size_t bytes);
 /* storageBytes / scaleBytes (src/configuration.cpp:188-213). */
 size_t msq_storage_bytes(int precision, size_t elements);
 size_t msq_scale_bytes(int precision, size_t elements);
+/* spectrumMetadataBytes (src/storage_codecs.cpp:387-398): two floats per block
+ * of 8 elements for the log-spectrum precisions, else msq_scale_bytes. */
+size_t msq_spectrum_metadata_bytes(int precision, size_t elements);
 
 /* Batch layout: ld_i = roundup(order_i, 32); offsets[i] = sum_{j<i} ld_j^2;
  * vector_offsets[i] = sum_{j<i} ld_j.  Totals in elements.  Every batched
@@ -207,7 +325,17 @@ int msq_batch_layout(uint32_t count, const uint32_t *orders, uint32_t *lds,
  * (src/storage_codecs.cpp:16-198).  scales may be NULL for scalar formats. */
 int msq_quantize(msq_runtime *runtime, int precision, const float *src, void *values,
                  void *scales, size_t elements);
-/* The decode half (decode_*_to_f32, metal_mixed_sqrt.metal:1661-1685). */
+/* msq_quantize with the quantizeAndDecode rounding stream argument
+ * (src/metal_runtime.cpp:672-740): stochastic rounding (msq_runtime_set_rounding)
+ * hashes (element index, seed, stream).  Also takes the log-spectrum
+ * precisions MSQ_FP8_E4M3FN_LOG_S and MSQ_FP4_E2M1_LOG_S, whose `scales` is the
+ * metadata plane (msq_spectrum_metadata_bytes: per block of 8 elements the
+ * floats min log2 v and the log2 span; log_spectrum_metadata and
+ * quantize_*_log_s, metal_mixed_sqrt.metal:1689-1799). */
+int msq_quantize_stream(msq_runtime *runtime, int precision, const float *src, void *values,
+                        void *scales, size_t elements, uint32_t rounding_stream);
+/* The decode half (decode_*_to_f32, metal_mixed_sqrt.metal:1661-1685,
+ * :1801-1817; for the log-spectrum precisions `scales` is the metadata). */
 int msq_decode(msq_runtime *runtime, int precision, const void *values,
                const void *scales, float *dst, size_t elements);
 /* quantizeAndDecode as one call. */
@@ -276,6 +404,17 @@ int msq_correction(msq_runtime *runtime, int precision, uint32_t count,
                    const uint32_t *orders, const float *vectors, const float *roots,
                    const float *residual_f32, float *correction);
 
+/* refineNativeStaged (src/metal_runtime.cpp:312-411): one native-quantized
+ * correction.  V (and V^T, stored separately), the roots and the residual
+ * (float, batch / vector layout) are stored in `precision` (FP8 E4M3FN / E5M2,
+ * MXFP8 E4M3FN / E5M2, FP4 E2M1, MXFP4 E2M1; the reference's host codecs on
+ * its linear n x n layout); then U = Q_t R, C = U Q, Y_ij = C_ij / (s_i + s_j),
+ * V2 = Q Y, E = V2 Q_t, each formed in FP32 from decoded operands and stored
+ * in the format; correction (float, batch layout) receives decoded E. */
+int msq_refine_native_staged(msq_runtime *runtime, int precision, uint32_t count,
+                             const uint32_t *orders, const float *vectors, const float *roots,
+                             const float *residual_f32, float *correction);
+
 /* measureF32Orthogonality (src/metal_runtime.cpp:742-776):
  * ||V^T V - I||_F / sqrt(n) per matrix into results (host, count). */
 int msq_orthogonality(msq_runtime *runtime, uint32_t count, const uint32_t *orders,
@@ -294,10 +433,23 @@ int msq_solve_batched(msq_runtime *runtime, uint32_t count, const uint32_t *orde
                       const double *matrices, double *roots,
                       const msq_solve_options *options, uint32_t options_count,
                       msq_solve_report *reports);
+/* msq_solve_batched with the outputs of an msq_solve_outputs record; in graph
+ * mode the output pointers are part of the batch signature. */
+int msq_solve_batched_ex(msq_runtime *runtime, uint32_t count, const uint32_t *orders,
+                         const double *matrices, const msq_solve_outputs *outputs,
+                         const msq_solve_options *options, uint32_t options_count,
+                         msq_solve_report *reports);
+/* The reference's defaults (MetalSolverConfiguration, MetalProblem and
+ * src/metal_backend.cpp:105-113): FP32 tiers, Tiled, FP32-reorthogonalized,
+ * fixed budget, Fused, stagnation window 4 and improvement 1e-3, Jacobi to
+ * off(A) <= 1e-7 ||A||_F, at most 12 corrections, residual tolerance 1e-12. */
+void msq_solve_options_init(msq_solve_options *options);
 
-/* Optional eigenvector / root-eigenvalue outputs of the last msq_solve_batched
- * call on this runtime (device pointers owned by the runtime, valid until the
- * next solve; float, batch / vector layout). */
+/* Eigenvectors and roots of the last msq_solve_batched call on this runtime
+ * (device pointers into the runtime's scratch, float, batch / vector layout).
+ * Valid until the next call on this runtime that uses scratch (any solve,
+ * Jacobi, product or feature call) or that releases graphs; msq_solve_batched_ex
+ * copies them to caller memory instead. */
 int msq_last_factors(msq_runtime *runtime, const float **vectors, const float **roots);
 
 /* -------------------------------------------------------------- autotuner */

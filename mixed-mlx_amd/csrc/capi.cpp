@@ -91,12 +91,16 @@ struct msq_runtime {
   hipEvent_t graph_in = nullptr, graph_out = nullptr;
   Arena arena;
   Arena large;  // blocked Jacobi workspace (orders above 256)
+  Arena refine;  // second Jacobi pass of unresolved matrices (eager solves)
   bool graphs_enabled = false;
   // Autotuned GEMM tile edge per (GemmType, ld class); 0: the default.
   uint8_t tiles[kGemmTypes][kTileClasses] = {};
   std::vector<std::unique_ptr<GraphEntry>> graphs;
   float* last_vectors = nullptr;
   float* last_roots = nullptr;
+  // RoundingMode and seed of the reference's runtime (msq_runtime_set_rounding).
+  int rounding_mode = MSQ_ROUNDING_NEAREST;
+  uint32_t rounding_seed = 0;
 };
 
 namespace {
@@ -112,6 +116,7 @@ int activate(msq_runtime* rt) {
 // arena is only regrown between calls, after a synchronize.
 struct Scratch {
   msq_runtime* rt;
+  Arena* target = nullptr;  // null: the runtime's arena
   std::vector<std::pair<void**, size_t>> wants;
   template <typename T>
   void want(T*& ptr, size_t count) {
@@ -120,10 +125,16 @@ struct Scratch {
   int commit() {
     size_t total = 0;
     for (auto& w : wants) total += (w.second + 255) / 256 * 256;
-    Arena& a = rt->arena;
+    Arena& a = target ? *target : rt->arena;
     if (total > a.capacity) {
       const size_t grow = std::max(total, a.capacity * 3 / 2);
       HIP_TRY(hipStreamSynchronize(rt->stream));
+      // msq_last_factors' pointers into this arena die with it.
+      if (rt->last_vectors && rt->last_vectors >= reinterpret_cast<float*>(a.base) &&
+          rt->last_vectors < reinterpret_cast<float*>(a.base + a.capacity)) {
+        rt->last_vectors = nullptr;
+        rt->last_roots = nullptr;
+      }
       if (a.base) HIP_TRY(hipFree(a.base));
       a.base = nullptr;
       a.capacity = 0;
@@ -187,6 +198,10 @@ msq::Format format_of(int precision, bool* ok) {
 bool is_mx(int precision) {
   return precision == MSQ_MXFP8_E4M3FN || precision == MSQ_MXFP8_E5M2 ||
          precision == MSQ_MXFP4_E2M1;
+}
+
+bool is_log_spectrum(int precision) {
+  return precision == MSQ_FP8_E4M3FN_LOG_S || precision == MSQ_FP4_E2M1_LOG_S;
 }
 
 // Products supported by the MFMA GEMMs.
@@ -640,9 +655,17 @@ struct GemmPlan {
   std::vector<msq::GemmJob> jobs;        // one per matrix of the group
   std::vector<uint32_t> tile_job[3];     // per tile edge kTileEdges[t]
   std::vector<uint32_t> tile_pos[3];
+  // Norm epilogues: per-wave slots, tile kind t's from slot_base[t]; job k's
+  // slots are [norm_first[k], norm_first[k] + norm_count[k]).
+  uint32_t slot_base[3] = {0, 0, 0};
+  uint32_t slots = 0;
+  std::vector<uint32_t> norm_first, norm_count;
   msq::GemmJob* d_jobs = nullptr;
   uint32_t* d_tile_job[3] = {nullptr, nullptr, nullptr};
   uint32_t* d_tile_pos[3] = {nullptr, nullptr, nullptr};
+  uint32_t* d_norm_first = nullptr;
+  uint32_t* d_norm_count = nullptr;
+  double* d_partials = nullptr;
 };
 
 int tile_class(uint32_t ld) { return ld > 256 ? kTileClasses - 1 : static_cast<int>(ld / 32) - 1; }
@@ -664,6 +687,10 @@ void build_plan(const Layout& L, const std::vector<uint32_t>& members, bool aux_
                 GemmPlan& P, const msq_runtime* rt = nullptr,
                 msq::GemmType type = msq::GemmType::F32, bool upper = false) {
   P.jobs.clear();
+  P.norm_first.clear();
+  P.norm_count.clear();
+  std::vector<int> kind;
+  std::vector<uint32_t> first_tile;
   for (int t = 0; t < 3; ++t) {
     P.tile_job[t].clear();
     P.tile_pos[t].clear();
@@ -680,11 +707,33 @@ void build_plan(const Layout& L, const std::vector<uint32_t>& members, bool aux_
     const int t = tile_kind(rt, type, L.ld[i]);
     const uint32_t tile = static_cast<uint32_t>(kTileEdges[t]);
     const uint32_t nt = (L.order[i] + tile - 1) / tile;
+    kind.push_back(t);
+    first_tile.push_back(static_cast<uint32_t>(P.tile_job[t].size()));
     for (uint32_t a = 0; a < nt; ++a)
       for (uint32_t b = upper ? a : 0; b < nt; ++b) {
         P.tile_job[t].push_back(jid);
         P.tile_pos[t].push_back((a << 16) | b);
       }
+  }
+  uint32_t slots = 0;
+  for (int t = 0; t < 3; ++t) {
+    P.slot_base[t] = slots;
+    slots += static_cast<uint32_t>(P.tile_job[t].size()) * msq::gemm_waves(kTileEdges[t]);
+  }
+  P.slots = slots;
+  for (size_t j = 0; j < P.jobs.size(); ++j) {
+    const int t = kind[j];
+    const uint32_t w = static_cast<uint32_t>(msq::gemm_waves(kTileEdges[t]));
+    // Tiles of one job are consecutive in its kind's list; the next job of
+    // the same kind starts where they end.
+    uint32_t e = static_cast<uint32_t>(P.tile_job[t].size());
+    for (size_t q = j + 1; q < P.jobs.size(); ++q)
+      if (kind[q] == t) {
+        e = first_tile[q];
+        break;
+      }
+    P.norm_first.push_back(P.slot_base[t] + first_tile[j] * w);
+    P.norm_count.push_back((e - first_tile[j]) * w);
   }
 }
 
@@ -694,6 +743,9 @@ void plan_want(Scratch& S, GemmPlan& P) {
     S.want(P.d_tile_job[t], std::max<size_t>(1, P.tile_job[t].size()));
     S.want(P.d_tile_pos[t], std::max<size_t>(1, P.tile_pos[t].size()));
   }
+  S.want(P.d_norm_first, std::max<size_t>(1, P.norm_first.size()));
+  S.want(P.d_norm_count, std::max<size_t>(1, P.norm_count.size()));
+  S.want(P.d_partials, std::max<size_t>(1, P.slots));
 }
 
 int plan_upload(msq_runtime* rt, GemmPlan& P) {
@@ -702,19 +754,30 @@ int plan_upload(msq_runtime* rt, GemmPlan& P) {
     MSQ_TRY(upload(rt, P.d_tile_job[t], P.tile_job[t]));
     MSQ_TRY(upload(rt, P.d_tile_pos[t], P.tile_pos[t]));
   }
+  MSQ_TRY(upload(rt, P.d_norm_first, P.norm_first));
+  MSQ_TRY(upload(rt, P.d_norm_count, P.norm_count));
   return MSQ_OK;
 }
 
+// The plan's launches (one per tile edge in use); a Residual or Gram epilogue
+// then sums its per-wave slots into args.norms in tile order.
 int run_gemm(msq_runtime* rt, const GemmPlan& P, msq::GemmType type, msq::Epilogue epi,
              msq::GemmArgs args) {
   args.jobs = P.d_jobs;
+  const bool norm = epi == msq::Epilogue::Residual || epi == msq::Epilogue::Gram;
+  args.partials = P.d_partials;
   for (int t = 0; t < 3; ++t) {
     if (P.tile_job[t].empty()) continue;
     args.tile_job = P.d_tile_job[t];
     args.tile_pos = P.d_tile_pos[t];
+    args.partial_base = P.slot_base[t];
     HIP_TRY(msq::launch_gemm(type, epi, kTileEdges[t], args,
                              static_cast<uint32_t>(P.tile_job[t].size()), rt->stream));
   }
+  if (norm)
+    HIP_TRY(msq::launch_gemm_norms(P.d_jobs, P.d_norm_first, P.d_norm_count,
+                                   static_cast<uint32_t>(P.jobs.size()), P.d_partials,
+                                   args.active, args.norms, rt->stream));
   return MSQ_OK;
 }
 
@@ -1054,6 +1117,275 @@ int validate_tier(int p, const char* what) {
   return MSQ_OK;
 }
 
+// ---------------------------------------------------------- native stages
+// One set of matrices corrected in a native storage format (the reference's
+// refineNativeStaged, src/metal_runtime.cpp:312-411): every operand and
+// product stored in the format on the compact n x n layout, products formed
+// in FP32 on the decoded values.
+struct NativeSet {
+  int format = 0;
+  uint32_t count = 0;
+  uint32_t max_order = 0;
+  const msq::Segment* dsegs = nullptr;
+  const uint64_t* dcoff = nullptr;
+  const GemmPlan* plan = nullptr;  // F32 tiles of the set's matrices
+  uint8_t* codes = nullptr;        // compact scratch of one stage
+  uint8_t* scales = nullptr;
+};
+
+// `in` rounded to the set's format (read transposed when in_t), decoded into
+// `out` (stored transposed when out_t): one matmul + quantize pair's output.
+int native_round(msq_runtime* rt, const Layout& L, const NativeSet& g, const float* in, bool in_t,
+                 float* out, bool out_t, const int* active) {
+  HIP_TRY(msq::launch_compact_quantize(g.dsegs, g.dcoff, g.count, g.max_order, g.format, in, in_t,
+                                       g.codes, g.scales, active, rt->stream));
+  HIP_TRY(msq::launch_compact_dequantize(g.dsegs, g.dcoff, g.count, L.max_ld, g.format, g.codes,
+                                         g.scales, out, out_t, active, rt->stream));
+  return MSQ_OK;
+}
+
+// C = A B on decoded float operands: `at` holds A^T column-major (the GEMM's
+// K-contiguous left operand), b holds B.
+int native_product(msq_runtime* rt, const NativeSet& g, const float* at, const float* b, float* c,
+                   const int* active) {
+  Operand oa, ob;
+  oa.data = at;
+  ob.data = b;
+  return run_gemm(rt, *g.plan, msq::GemmType::F32, msq::Epilogue::Store,
+                  product_args(oa, ob, c, nullptr, active));
+}
+
+// The stored basis: Q = quant(V) and Q_t = quant(V^T) (the reference stores
+// the transposed eigenvectors separately, src/metal_solver.cpp:707-711),
+// each decoded as is and transposed.
+int native_basis(msq_runtime* rt, const Layout& L, const NativeSet& g, const float* V, float* q,
+                 float* qT, float* qt, float* qtT, const int* active) {
+  MSQ_TRY(native_round(rt, L, g, V, false, q, false, active));
+  MSQ_TRY(native_round(rt, L, g, V, false, qT, true, active));
+  MSQ_TRY(native_round(rt, L, g, V, true, qt, false, active));
+  MSQ_TRY(native_round(rt, L, g, V, true, qtT, true, active));
+  return MSQ_OK;
+}
+
+// One staged correction from R (float): U = Q_t R, C = U Q, Y = C / (s_i +
+// s_j), V2 = Q Y, E = V2 Q_t, each stored in the format; E decoded into t2.
+int native_correction(msq_runtime* rt, const Layout& L, const NativeSet& g, const float* r32,
+                      const float* q, const float* qT, const float* qt, const float* qtT,
+                      const float* s, float* t1, float* t2, const int* active) {
+  MSQ_TRY(native_round(rt, L, g, r32, false, t2, false, active));  // R
+  MSQ_TRY(native_product(rt, g, qtT, t2, t1, active));             // U = Q_t R
+  MSQ_TRY(native_round(rt, L, g, t1, false, t2, true, active));    // U, transposed
+  MSQ_TRY(native_product(rt, g, t2, q, t1, active));               // C = U Q
+  MSQ_TRY(native_round(rt, L, g, t1, false, t2, false, active));
+  HIP_TRY(msq::launch_sylvester_divide(g.dsegs, g.count, L.max_ld, t2, s, t1, active, rt->stream));
+  MSQ_TRY(native_round(rt, L, g, t1, false, t2, false, active));   // Y
+  MSQ_TRY(native_product(rt, g, qT, t2, t1, active));              // V2 = Q Y
+  MSQ_TRY(native_round(rt, L, g, t1, false, t2, true, active));    // V2, transposed
+  MSQ_TRY(native_product(rt, g, t2, qt, t1, active));              // E = V2 Q_t
+  MSQ_TRY(native_round(rt, L, g, t1, false, t2, false, active));   // E
+  return MSQ_OK;
+}
+
+// Compact element offsets of the members (n^2 from a 64-element boundary)
+// and the largest order; returns the compact total.
+uint64_t compact_offsets(const Layout& L, const std::vector<uint32_t>& members,
+                         std::vector<uint64_t>& coff, uint32_t& max_order) {
+  uint64_t c = 0;
+  coff.clear();
+  max_order = 0;
+  for (uint32_t i : members) {
+    coff.push_back(c);
+    c += (static_cast<uint64_t>(L.order[i]) * L.order[i] + 63) / 64 * 64;
+    max_order = std::max(max_order, L.order[i]);
+  }
+  return c;
+}
+
+bool native_format(int p) { return p >= MSQ_FP8_E4M3FN && p <= MSQ_MXFP4_E2M1; }
+
+// Host-side facts of a solve that its reports need (per matrix).
+struct SolveMeta {
+  std::vector<uint8_t> stagnation;  // MSQ_STOPPING_STAGNATION: the best iterate is the result
+  std::vector<uint8_t> native;      // native-quantized corrections (no certificate)
+  std::vector<uint8_t> low_basis;   // a low-bit basis tier was tried
+  std::vector<uint8_t> refined;     // the second Jacobi pass ran
+  std::vector<double> residual_tolerance;
+};
+
+// The reference's checks of a solver configuration (runMetalCase,
+// src/metal_solver.cpp:359-370; validateNativeStoppingPolicy,
+// validateNativeDispatchPolicy and validateCorrectionMode,
+// src/configuration.cpp:290-346) and this backend's limits.
+int validate_solve_options(const msq_solve_options& o) {
+  auto bad = [](const char* what) { return set_error(MSQ_ERROR_INVALID_ARGUMENT, "%s", what); };
+  if (o.kernel_path != MSQ_KERNEL_PATH_LEGACY && o.kernel_path != MSQ_KERNEL_PATH_TILED)
+    return bad("kernel_path must be MSQ_KERNEL_PATH_LEGACY or MSQ_KERNEL_PATH_TILED");
+  if (o.correction_mode != MSQ_CORRECTION_FP32_REORTHOGONALIZED &&
+      o.correction_mode != MSQ_CORRECTION_NATIVE_QUANTIZED)
+    return bad("correction_mode must be fp32_reorthogonalized or native_quantized");
+  if (o.stopping_policy != MSQ_STOPPING_FIXED_BUDGET && o.stopping_policy != MSQ_STOPPING_STAGNATION)
+    return bad("stopping_policy must be fixed_budget or stagnation");
+  if (o.dispatch_policy != MSQ_DISPATCH_PER_STAGE && o.dispatch_policy != MSQ_DISPATCH_FUSED)
+    return bad("dispatch_policy must be per_stage or fused");
+  const bool native = o.correction_mode == MSQ_CORRECTION_NATIVE_QUANTIZED;
+  if (o.stopping_policy == MSQ_STOPPING_STAGNATION && !native)
+    return bad("stagnation stopping requires native_quantized correction");
+  if (o.dispatch_policy == MSQ_DISPATCH_PER_STAGE && !native)
+    return bad("per_stage dispatch requires native_quantized correction");
+  if (o.stagnation_window == 0) return bad("stagnation_window must be a positive integer");
+  if (!std::isfinite(o.minimum_relative_improvement) || o.minimum_relative_improvement < 0.0)
+    return bad("minimum_relative_improvement must be finite and nonnegative");
+  if (o.max_corrections >= msq::kSolveHistory)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "max_corrections must be < %u",
+                     msq::kSolveHistory);
+  if (!(o.resolution >= 0.0f) || !std::isfinite(o.resolution))
+    return bad("resolution must be finite and nonnegative");
+  if (!(o.basis_tolerance >= 0.0)) return bad("basis_tolerance must be nonnegative");
+  if (o.spectrum_precision < MSQ_FP32 || o.spectrum_precision > MSQ_FP4_E2M1_LOG_S)
+    return set_error(MSQ_ERROR_UNSUPPORTED, "spectrum precision %d", o.spectrum_precision);
+  if (is_log_spectrum(o.basis_precision))
+    return bad("a log-spectrum format cannot store the frozen basis");
+  if (native) {
+    if (is_log_spectrum(o.spectrum_precision))
+      return bad("native quantized correction does not support log-spectrum storage");
+    if (o.basis_precision != o.spectrum_precision)
+      return bad("native quantized correction requires matching basis and spectrum precisions");
+    if (o.kernel_path != MSQ_KERNEL_PATH_LEGACY)
+      return bad("native quantized correction requires the legacy kernel path");
+    if (o.basis_precision < MSQ_FP8_E4M3FN || o.basis_precision > MSQ_MXFP4_E2M1)
+      return bad("native quantized correction supports FP8 E4M3FN, FP8 E5M2, MXFP8 E4M3FN, "
+                 "MXFP8 E5M2, FP4 E2M1, or MXFP4 E2M1 storage");
+    if (o.correction_precision != o.basis_precision)
+      return bad("native quantized correction runs its products in the basis storage format: "
+                 "correction_precision must equal basis_precision");
+    return MSQ_OK;
+  }
+  MSQ_TRY(validate_tier(o.basis_precision, "basis"));
+  return validate_tier(o.correction_precision, "correction");
+}
+
+// Second Jacobi pass of the matrices whose smallest eigenvalue the FP32
+// Jacobi does not resolve (solve state `unresolved`, options with
+// refine_unresolved).  B = V^T A V is formed in binary64 (two FP64 MFMA
+// products on V widened exactly), so its entries carry the small eigenvalues
+// to full float precision, where the first pass only reached eps32 ||A||;
+// the batched Jacobi then diagonalises B (stopping once the frozen-factor
+// correction is predicted to contract by 1e-2), V <- V W, and the solve
+// state is rebuilt from the new diagonal.  Host-driven (it reads the flags
+// back), so such solves run eagerly.
+int refine_unresolved(msq_runtime* rt, const Layout& L, const std::vector<msq::SolveLimits>& limits,
+                      const double* matrices, float* V, float* diag, float* sroot,
+                      msq::JacobiStatus* jst, msq::SolveState* st, const msq::Segment* dsegs_all,
+                      const msq::SolveLimits* dlim, int* active, SolveMeta& meta) {
+  const uint32_t count = static_cast<uint32_t>(L.order.size());
+  std::vector<msq::SolveState> hs(count);
+  HIP_TRY(hipMemcpyAsync(hs.data(), st, count * sizeof(msq::SolveState), hipMemcpyDeviceToHost,
+                         rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  std::vector<uint32_t> F;
+  for (uint32_t i = 0; i < count; ++i)
+    if (limits[i].refine && hs[i].unresolved && hs[i].status == msq::kSolveRunning) F.push_back(i);
+  if (F.empty()) return MSQ_OK;
+  const uint32_t nf = static_cast<uint32_t>(F.size());
+  std::vector<uint32_t> orders_f(nf);
+  for (uint32_t k = 0; k < nf; ++k) orders_f[k] = L.order[F[k]];
+  Layout LF;
+  MSQ_TRY(make_layout(nf, orders_f.data(), LF));
+  const std::vector<uint32_t> sub = iota_members(nf);
+  const std::vector<msq::Segment> segs_from = segments_of(L, F), segs_to = segments_of(LF, sub);
+  GemmPlan p64, p32;
+  build_plan(LF, sub, false, p64, rt, msq::GemmType::F64);
+  build_plan(LF, sub, false, p32, rt, msq::GemmType::F32);
+  Scratch S{rt, &rt->refine};
+  double *a64 = nullptr, *v64 = nullptr, *c64 = nullptr, *b64 = nullptr;
+  float *vs = nullptr, *vst = nullptr, *w = nullptr, *vn = nullptr, *dg = nullptr, *rs = nullptr;
+  msq::JacobiStatus *js = nullptr, *jcomb = nullptr;
+  msq::JacobiJob* jobs = nullptr;
+  msq::Segment *dfrom = nullptr, *dto = nullptr;
+  uint32_t* dindex = nullptr;
+  S.want(a64, LF.total);
+  S.want(v64, LF.total);
+  S.want(c64, LF.total);
+  S.want(b64, LF.total);
+  S.want(vs, LF.total);
+  S.want(vst, LF.total);
+  S.want(w, LF.total);
+  S.want(vn, LF.total);
+  S.want(dg, LF.vtotal);
+  S.want(rs, LF.vtotal);
+  S.want(js, nf);
+  S.want(jcomb, nf);
+  S.want(jobs, nf);
+  S.want(dfrom, nf);
+  S.want(dto, nf);
+  S.want(dindex, nf);
+  plan_want(S, p64);
+  plan_want(S, p32);
+  MSQ_TRY(S.commit());
+  MSQ_TRY(upload(rt, dfrom, segs_from));
+  MSQ_TRY(upload(rt, dto, segs_to));
+  MSQ_TRY(upload(rt, dindex, F));
+  MSQ_TRY(plan_upload(rt, p64));
+  MSQ_TRY(plan_upload(rt, p32));
+  const uint32_t max_ld = L.max_ld;
+  HIP_TRY(msq::launch_move_tiles(dfrom, dto, nf, max_ld, matrices, a64, sizeof(double), rt->stream));
+  HIP_TRY(msq::launch_move_tiles(dfrom, dto, nf, max_ld, V, vs, sizeof(float), rt->stream));
+  HIP_TRY(msq::launch_widen_f64(vs, v64, LF.total, rt->stream));
+  Operand oa, ob;
+  oa.data = a64;  // A symmetric: its K-contiguous storage is itself
+  ob.data = v64;
+  MSQ_TRY(run_gemm(rt, p64, msq::GemmType::F64, msq::Epilogue::Store,
+                   product_args(oa, ob, c64, nullptr, nullptr)));  // C = A V
+  oa.data = v64;  // V^T's K-contiguous storage is V column-major
+  ob.data = c64;
+  MSQ_TRY(run_gemm(rt, p64, msq::GemmType::F64, msq::Epilogue::Store,
+                   product_args(oa, ob, b64, nullptr, nullptr)));  // B = V^T C
+  msq_jacobi_options jo{};
+  jo.max_sweeps = 30;
+  jo.off_tol = 0.0f;      // off(B) is at the FP32 floor of A already
+  jo.stagnation = 0.5f;
+  jo.floor_tol = 1e-30f;  // no floor: the small block converges below it
+  jo.correction_rate = 1e-2f;
+  MSQ_TRY(run_jacobi(rt, LF, b64, w, dg, rs, &jo, 1, js, jobs));
+  HIP_TRY(msq::launch_transpose(dto, nf, pair_count(segs_to), vs, vst, rt->stream));
+  oa.data = vst;
+  ob.data = w;
+  MSQ_TRY(run_gemm(rt, p32, msq::GemmType::F32, msq::Epilogue::Store,
+                   product_args(oa, ob, vn, nullptr, nullptr)));  // V W
+  HIP_TRY(msq::launch_move_tiles(dto, dfrom, nf, max_ld, vn, V, sizeof(float), rt->stream));
+  HIP_TRY(msq::launch_move_vectors(dto, dfrom, nf, max_ld, dg, diag, rt->stream));
+  HIP_TRY(msq::launch_move_vectors(dto, dfrom, nf, max_ld, rs, sroot, rt->stream));
+  // Jacobi reports: the two passes' sweeps and histories in sequence, the
+  // second pass' final measures.
+  std::vector<msq::JacobiStatus> first(count), second(nf);
+  HIP_TRY(hipMemcpyAsync(first.data(), jst, count * sizeof(msq::JacobiStatus),
+                         hipMemcpyDeviceToHost, rt->stream));
+  HIP_TRY(hipMemcpyAsync(second.data(), js, nf * sizeof(msq::JacobiStatus), hipMemcpyDeviceToHost,
+                         rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  std::vector<msq::JacobiStatus> comb(nf);
+  for (uint32_t k = 0; k < nf; ++k) {
+    msq::JacobiStatus c = first[F[k]];
+    const msq::JacobiStatus& b = second[k];
+    for (uint32_t q = 0; q < b.sweeps && c.sweeps + q < msq::kJacobiHistory; ++q)
+      c.off_history[c.sweeps + q] = b.off_history[q];
+    c.sweeps += b.sweeps;
+    c.flags = b.flags;  // the second pass decides the spectrum
+    c.off_norm = b.off_norm;
+    c.min_diagonal = b.min_diagonal;
+    c.correction_rate = b.correction_rate;
+    comb[k] = c;
+    meta.refined[F[k]] = 1;
+  }
+  MSQ_TRY(upload(rt, jcomb, comb));
+  HIP_TRY(msq::launch_scatter_status(jcomb, dindex, nf, jst, rt->stream));
+  HIP_TRY(msq::launch_solve_state(dsegs_all, jst, diag, dlim, st, active, count, false,
+                                  rt->stream));
+  // comb is a host temporary read by the upload above.
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  return MSQ_OK;
+}
+
 }  // namespace
 
 // ============================================================== C entry points
@@ -1065,7 +1397,7 @@ struct GraphEntry {
   std::vector<uint32_t> orders;
   std::vector<unsigned char> options;
   const void* matrices = nullptr;
-  void* roots = nullptr;
+  msq_solve_outputs outputs{};
   Arena arena;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1079,35 +1411,45 @@ struct GraphEntry {
 
 namespace {
 
-void release_graph(GraphEntry& g) {
+void release_graph(msq_runtime* rt, GraphEntry& g) {
+  // msq_last_factors' pointers into this arena die with it.
+  if (rt->last_vectors && rt->last_vectors == g.V) {
+    rt->last_vectors = nullptr;
+    rt->last_roots = nullptr;
+  }
   if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (g.graph) (void)hipGraphDestroy(g.graph);
   if (g.arena.base) (void)hipFree(g.arena.base);
   g = GraphEntry{};
 }
 
+bool same_outputs(const msq_solve_outputs& a, const msq_solve_outputs& b) {
+  return a.roots == b.roots && a.initial_roots == b.initial_roots && a.vectors == b.vectors &&
+         a.root_eigenvalues == b.root_eigenvalues;
+}
+
 // The entry of this signature (created on first use; at most kMaxGraphs,
 // the oldest evicted).
 GraphEntry* find_graph(msq_runtime* rt, uint32_t count, const uint32_t* orders,
-                       const double* matrices, double* roots, const msq_solve_options* options,
-                       uint32_t options_count) {
+                       const double* matrices, const msq_solve_outputs& outputs,
+                       const msq_solve_options* options, uint32_t options_count) {
   const unsigned char* ob = reinterpret_cast<const unsigned char*>(options);
   const size_t obytes = static_cast<size_t>(options_count) * sizeof(msq_solve_options);
   for (auto& g : rt->graphs)
-    if (g->matrices == matrices && g->roots == roots && g->orders.size() == count &&
+    if (g->matrices == matrices && same_outputs(g->outputs, outputs) && g->orders.size() == count &&
         std::equal(g->orders.begin(), g->orders.end(), orders) && g->options.size() == obytes &&
         std::equal(g->options.begin(), g->options.end(), ob))
       return g.get();
   if (rt->graphs.size() >= kMaxGraphs) {
     (void)hipStreamSynchronize(rt->stream);
-    release_graph(*rt->graphs.front());
+    release_graph(rt, *rt->graphs.front());
     rt->graphs.erase(rt->graphs.begin());
   }
   auto g = std::make_unique<GraphEntry>();
   g->orders.assign(orders, orders + count);
   g->options.assign(ob, ob + obytes);
   g->matrices = matrices;
-  g->roots = roots;
+  g->outputs = outputs;
   rt->graphs.push_back(std::move(g));
   return rt->graphs.back().get();
 }
@@ -1151,10 +1493,15 @@ int launch_graph(msq_runtime* rt, GraphEntry& g) {
   return MSQ_OK;
 }
 
+
 // Reports of a solve whose device work is enqueued: copies the per-matrix
-// states back and waits.
+// states back and waits.  Status mapping: a running matrix has exhausted its
+// budget; under the stagnation policy the best iterate decides convergence
+// and the reference's termination reasons (src/metal_solver.cpp:904-935); a
+// matrix that stops converged while unresolved reports MSQ_SOLVE_UNRESOLVED.
 int finish_solve(msq_runtime* rt, const Layout& L, const msq::SolveState* st,
-                 const msq::JacobiStatus* jst, float* V, float* sroot, msq_solve_report* reports) {
+                 const msq::JacobiStatus* jst, float* V, float* sroot, msq_solve_report* reports,
+                 const SolveMeta& meta) {
   const uint32_t count = static_cast<uint32_t>(L.order.size());
   std::vector<msq::SolveState> hs(count);
   std::vector<msq::JacobiStatus> hj(count);
@@ -1171,21 +1518,50 @@ int finish_solve(msq_runtime* rt, const Layout& L, const msq::SolveState* st,
       r.order = L.order[i];
       r.sweeps = hj[i].sweeps;
       r.corrections = s.corrections;
-      r.status = s.status == msq::kSolveRunning ? msq::kSolveBudgetExhausted : s.status;
+      uint32_t status = s.status == msq::kSolveRunning ? msq::kSolveBudgetExhausted : s.status;
+      r.final_residual = s.final_residual;
+      r.final_forward_estimate = s.final_forward_estimate;
+      if (meta.stagnation[i] && status != msq::kSolveNonfinite &&
+          status != msq::kSolveInvalidSpectrum) {
+        r.final_residual = s.best_residual;
+        const uint32_t last = s.steps > 0 ? s.steps - 1 : 0;
+        if (s.best_residual <= meta.residual_tolerance[i])
+          status = msq::kSolveConverged;
+        else if (status != msq::kSolveStagnated)
+          status = last < msq::kSolveHistory && s.residuals[last] > s.residuals[0]
+                       ? msq::kSolveDiverged
+                       : msq::kSolveBudgetExhausted;
+      }
+      const bool unresolved = s.unresolved && !meta.refined[i];
+      if (status == msq::kSolveConverged && unresolved) status = msq::kSolveUnresolved;
+      r.status = status;
       r.matrix_norm = s.matrix_norm;
       r.min_root = s.min_root;
       r.off_norm = hj[i].off_norm;
       r.correction_rate = hj[i].correction_rate;
-      r.final_residual = s.final_residual;
-      r.final_forward_estimate = s.final_forward_estimate;
       r.basis_orthogonality = static_cast<float>(s.basis_orthogonality);
-      r.precision_escalations = s.escalated;
+      r.basis_rejected = s.escalated;
+      r.spectrum_rejected = s.spectrum_rejected;
+      r.precision_escalations = s.escalated + s.spectrum_rejected;
+      r.unresolved = unresolved ? 1u : 0u;
+      r.refined = meta.refined[i];
+      r.best_iteration = s.best_iteration;
+      r.stagnation_steps = s.stagnation_steps;
+      r.best_residual = s.best_residual;
+      // Certificate (src/metal_solver.cpp:511-524): the accepted tier's basis
+      // orthogonality, else the FP32 basis' when measured.
+      const bool tier_basis = meta.low_basis[i] && !s.escalated;
+      const double orth = tier_basis ? s.basis_orthogonality : s.fp32_orthogonality;
+      r.frozen_basis_orthogonality = orth;
+      r.frozen_sylvester_inverse_norm =
+          meta.native[i] ? NAN : 1.0 / (2.0 * s.frozen_min_root);
+      r.certificate_usable = !meta.native[i] && std::isfinite(orth) && orth < 1.0e-3 ? 1u : 0u;
       for (uint32_t k = 0; k < MSQ_SOLVE_HISTORY && k < msq::kSolveHistory; ++k) {
         r.residuals[k] = k < s.steps ? s.residuals[k] : 0.0;
         r.forward_estimates[k] = k < s.steps ? s.forward_estimates[k] : 0.0;
       }
       for (uint32_t k = 0; k < MSQ_JACOBI_HISTORY; ++k)
-        r.off_history[k] = k < hj[i].sweeps ? hj[i].off_history[k] : 0.0f;
+        r.off_history[k] = k < hj[i].sweeps && k < msq::kJacobiHistory ? hj[i].off_history[k] : 0.0f;
     }
   }
   rt->last_vectors = V;
@@ -1242,11 +1618,13 @@ int msq_runtime_destroy(msq_runtime* rt) {
   if (e == hipSuccess) e = hipStreamSynchronize(rt->stream);
   if (e != hipSuccess) status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
   if (rt->own_stream) (void)hipStreamSynchronize(rt->own_stream);
-  for (auto& g : rt->graphs) release_graph(*g);
+  for (auto& g : rt->graphs) release_graph(rt, *g);
   rt->graphs.clear();
   if (rt->arena.base && (e = hipFree(rt->arena.base)) != hipSuccess && status == MSQ_OK)
     status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
   if (rt->large.base && (e = hipFree(rt->large.base)) != hipSuccess && status == MSQ_OK)
+    status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
+  if (rt->refine.base && (e = hipFree(rt->refine.base)) != hipSuccess && status == MSQ_OK)
     status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
   for (int l = 0; l < kLanes; ++l) {
     if (rt->lanes[l] && (e = hipStreamSynchronize(rt->lanes[l])) == hipSuccess)
@@ -1278,10 +1656,19 @@ int msq_runtime_set_graphs(msq_runtime* rt, int enable) {
   if (!enable && !rt->graphs.empty()) {
     HIP_TRY(hipStreamSynchronize(rt->stream));
     HIP_TRY(hipStreamSynchronize(rt->own_stream));
-    for (auto& g : rt->graphs) release_graph(*g);
+    for (auto& g : rt->graphs) release_graph(rt, *g);
     rt->graphs.clear();
   }
   rt->graphs_enabled = enable != 0;
+  return MSQ_OK;
+}
+
+int msq_runtime_set_rounding(msq_runtime* rt, int mode, uint32_t seed) {
+  MSQ_TRY(activate(rt));
+  if (mode != MSQ_ROUNDING_NEAREST && mode != MSQ_ROUNDING_STOCHASTIC)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "rounding mode %d", mode);
+  rt->rounding_mode = mode;
+  rt->rounding_seed = seed;
   return MSQ_OK;
 }
 
@@ -1344,6 +1731,11 @@ size_t msq_scale_bytes(int precision, size_t elements) {
   return is_mx(precision) ? (elements + 31) / 32 : 0;
 }
 
+size_t msq_spectrum_metadata_bytes(int precision, size_t elements) {
+  if (is_log_spectrum(precision)) return 2 * sizeof(float) * ((elements + 7) / 8);
+  return msq_scale_bytes(precision, elements);
+}
+
 int msq_batch_layout(uint32_t count, const uint32_t* orders, uint32_t* lds, uint64_t* offsets,
                      uint64_t* vector_offsets, uint64_t* total, uint64_t* vtotal) {
   Layout L;
@@ -1358,25 +1750,60 @@ int msq_batch_layout(uint32_t count, const uint32_t* orders, uint32_t* lds, uint
   return MSQ_OK;
 }
 
-int msq_quantize(msq_runtime* rt, int precision, const float* src, void* values, void* scales,
-                 size_t elements) {
+int msq_quantize_stream(msq_runtime* rt, int precision, const float* src, void* values,
+                        void* scales, size_t elements, uint32_t rounding_stream) {
   MSQ_TRY(activate(rt));
+  const bool stochastic = rt->rounding_mode == MSQ_ROUNDING_STOCHASTIC;
+  if (elements && (!src || !values)) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  if (reinterpret_cast<uintptr_t>(src) % 16 || reinterpret_cast<uintptr_t>(values) % 16)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "buffers must be 16-byte aligned");
+  if (is_log_spectrum(precision)) {
+    if (!scales) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "log-spectrum storage requires metadata");
+    if (elements > UINT32_MAX)
+      return set_error(MSQ_ERROR_INVALID_ARGUMENT, "more than 2^32 spectrum elements");
+    HIP_TRY(msq::launch_encode_log(precision == MSQ_FP8_E4M3FN_LOG_S, src,
+                                   static_cast<uint8_t*>(values), static_cast<float*>(scales),
+                                   elements, stochastic, rt->rounding_seed, rounding_stream,
+                                   rt->stream));
+    HIP_TRY(hipStreamSynchronize(rt->stream));
+    return MSQ_OK;
+  }
   bool ok = false;
   const msq::Format f = format_of(precision, &ok);
   if (!ok) return set_error(MSQ_ERROR_UNSUPPORTED, "precision %d has no linear codec", precision);
   if (is_mx(precision) && !scales)
     return set_error(MSQ_ERROR_INVALID_ARGUMENT, "MX storage requires a scale plane");
-  if (elements && (!src || !values)) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
-  if (reinterpret_cast<uintptr_t>(src) % 16 || reinterpret_cast<uintptr_t>(values) % 16)
-    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "buffers must be 16-byte aligned");
-  HIP_TRY(msq::launch_encode(f, src, values, static_cast<uint8_t*>(scales), elements, rt->stream));
+  if (stochastic) {
+    // The hash takes 32-bit element indices, as the reference's kernels do.
+    if (elements > UINT32_MAX)
+      return set_error(MSQ_ERROR_INVALID_ARGUMENT, "more than 2^32 elements to round stochastically");
+    HIP_TRY(msq::launch_encode_stochastic(f, src, values, static_cast<uint8_t*>(scales), elements,
+                                          rt->rounding_seed, rounding_stream, rt->stream));
+  } else {
+    HIP_TRY(msq::launch_encode(f, src, values, static_cast<uint8_t*>(scales), elements,
+                               rt->stream));
+  }
   HIP_TRY(hipStreamSynchronize(rt->stream));
   return MSQ_OK;
+}
+
+int msq_quantize(msq_runtime* rt, int precision, const float* src, void* values, void* scales,
+                 size_t elements) {
+  return msq_quantize_stream(rt, precision, src, values, scales, elements, 0u);
 }
 
 int msq_decode(msq_runtime* rt, int precision, const void* values, const void* scales,
                float* dst, size_t elements) {
   MSQ_TRY(activate(rt));
+  if (is_log_spectrum(precision)) {
+    if (!scales) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "log-spectrum storage requires metadata");
+    if (elements && (!dst || !values)) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+    HIP_TRY(msq::launch_decode_log(precision == MSQ_FP8_E4M3FN_LOG_S,
+                                   static_cast<const uint8_t*>(values),
+                                   static_cast<const float*>(scales), dst, elements, rt->stream));
+    HIP_TRY(hipStreamSynchronize(rt->stream));
+    return MSQ_OK;
+  }
   bool ok = false;
   const msq::Format f = format_of(precision, &ok);
   if (!ok) return set_error(MSQ_ERROR_UNSUPPORTED, "precision %d has no linear codec", precision);
@@ -1619,6 +2046,80 @@ int msq_correction(msq_runtime* rt, int precision, uint32_t count, const uint32_
   return MSQ_OK;
 }
 
+int msq_refine_native_staged(msq_runtime* rt, int precision, uint32_t count,
+                             const uint32_t* orders, const float* vectors, const float* roots,
+                             const float* residual_f32, float* correction) {
+  MSQ_TRY(activate(rt));
+  if (!native_format(precision))
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT,
+                     "native staged correction supports FP8 E4M3FN, FP8 E5M2, MXFP8 E4M3FN, MXFP8 "
+                     "E5M2, FP4 E2M1, or MXFP4 E2M1 storage (precision %d)",
+                     precision);
+  Layout L;
+  MSQ_TRY(make_layout(count, orders, L));
+  if (!vectors || !roots || !residual_f32 || !correction)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  const std::vector<uint32_t> members = iota_members(count);
+  const std::vector<msq::Segment> segs = segments_of(L, members);
+  std::vector<uint64_t> coff;
+  uint32_t max_order = 0;
+  const uint64_t compact = compact_offsets(L, members, coff, max_order);
+  GemmPlan plan;
+  build_plan(L, members, false, plan, rt, msq::GemmType::F32);
+  Scratch S{rt};
+  msq::Segment* dsegs = nullptr;
+  uint64_t* dcoff = nullptr;
+  float *q = nullptr, *qT = nullptr, *qt = nullptr, *qtT = nullptr, *qs = nullptr, *t1 = nullptr,
+        *t2 = nullptr;
+  int* rejected = nullptr;
+  uint8_t *codes = nullptr, *scales = nullptr;
+  S.want(dsegs, count);
+  S.want(dcoff, count);
+  S.want(q, L.total);
+  S.want(qT, L.total);
+  S.want(qt, L.total);
+  S.want(qtT, L.total);
+  S.want(t1, L.total);
+  S.want(t2, L.total);
+  S.want(qs, L.vtotal);
+  S.want(rejected, count);
+  S.want(codes, compact);
+  S.want(scales, compact / 32 + 1);
+  plan_want(S, plan);
+  MSQ_TRY(S.commit());
+  MSQ_TRY(upload(rt, dsegs, segs));
+  MSQ_TRY(upload(rt, dcoff, coff));
+  MSQ_TRY(plan_upload(rt, plan));
+  NativeSet g;
+  g.format = precision;
+  g.count = count;
+  g.max_order = max_order;
+  g.dsegs = dsegs;
+  g.dcoff = dcoff;
+  g.plan = &plan;
+  g.codes = codes;
+  g.scales = scales;
+  // Roots stored in the format (copySpectrumToStorage); the reference fails on
+  // a stored root that is not positive and finite.
+  HIP_TRY(msq::launch_segment_spectrum(dsegs, count, precision, roots, qs, rejected, nullptr,
+                                       false, 0u, 1u, rt->stream));
+  std::vector<int> hrej(count);
+  HIP_TRY(hipMemcpyAsync(hrej.data(), rejected, count * sizeof(int), hipMemcpyDeviceToHost,
+                         rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  for (uint32_t i = 0; i < count; ++i)
+    if (hrej[i])
+      return set_error(MSQ_ERROR_INVALID_ARGUMENT,
+                       "the stored roots of matrix %u contain a value that is not positive and "
+                       "finite",
+                       i);
+  MSQ_TRY(native_basis(rt, L, g, vectors, q, qT, qt, qtT, nullptr));
+  MSQ_TRY(native_correction(rt, L, g, residual_f32, q, qT, qt, qtT, qs, t1, t2, nullptr));
+  HIP_TRY(msq::launch_copy_words(t2, correction, L.total, rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  return MSQ_OK;
+}
+
 int msq_orthogonality(msq_runtime* rt, uint32_t count, const uint32_t* orders,
                       const float* vectors, double* results) {
   MSQ_TRY(activate(rt));
@@ -1653,50 +2154,115 @@ int msq_orthogonality(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   return MSQ_OK;
 }
 
+void msq_solve_options_init(msq_solve_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->basis_precision = MSQ_FP32;
+  o->correction_precision = MSQ_FP32;
+  o->jacobi.max_sweeps = 30;
+  o->jacobi.off_tol = 1e-7f;
+  o->jacobi.stagnation = 0.5f;
+  o->jacobi.floor_tol = 1e-5f;
+  o->min_corrections = 0;
+  o->max_corrections = 12;
+  o->residual_tolerance = 1e-12;
+  o->forward_tolerance = 0.0;
+  o->spectrum_precision = MSQ_FP32;
+  o->kernel_path = MSQ_KERNEL_PATH_TILED;
+  o->correction_mode = MSQ_CORRECTION_FP32_REORTHOGONALIZED;
+  o->stopping_policy = MSQ_STOPPING_FIXED_BUDGET;
+  o->dispatch_policy = MSQ_DISPATCH_FUSED;
+  o->stagnation_window = 4;
+  o->minimum_relative_improvement = 1e-3;
+}
+
 int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
                       const double* matrices, double* roots_out, const msq_solve_options* options,
                       uint32_t options_count, msq_solve_report* reports) {
+  msq_solve_outputs out{};
+  out.roots = roots_out;
+  return msq_solve_batched_ex(rt, count, orders, matrices, &out, options, options_count, reports);
+}
+
+int msq_solve_batched_ex(msq_runtime* rt, uint32_t count, const uint32_t* orders,
+                         const double* matrices, const msq_solve_outputs* outputs,
+                         const msq_solve_options* options, uint32_t options_count,
+                         msq_solve_report* reports) {
   MSQ_TRY(activate(rt));
   Layout L;
   MSQ_TRY(make_layout(count, orders, L));
-  if (!matrices || !roots_out || !options)
+  if (!matrices || !outputs || !outputs->roots || !options)
     return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
   if (options_count != 1 && options_count != count)
     return set_error(MSQ_ERROR_INVALID_ARGUMENT, "options_count must be 1 or count");
+  double* roots_out = outputs->roots;
+  auto opt = [&](uint32_t i) -> const msq_solve_options& {
+    return options[options_count == 1 ? 0 : i];
+  };
+  // Validation: the reference's checks of a configuration (runMetalCase,
+  // src/metal_solver.cpp:359-370; validateCorrectionMode and the policy
+  // validators, src/configuration.cpp:290-346) and this backend's limits.
+  for (uint32_t k = 0; k < options_count; ++k) MSQ_TRY(validate_solve_options(options[k]));
+  SolveMeta meta;
+  meta.stagnation.assign(count, 0);
+  meta.native.assign(count, 0);
+  meta.low_basis.assign(count, 0);
+  meta.refined.assign(count, 0);
+  meta.residual_tolerance.assign(count, 0.0);
+  bool refine_any = false, stagnation_any = false, certificate_any = false;
+  for (uint32_t i = 0; i < count; ++i) {
+    const msq_solve_options& o = opt(i);
+    const bool native = o.correction_mode == MSQ_CORRECTION_NATIVE_QUANTIZED;
+    meta.native[i] = native;
+    meta.stagnation[i] = o.stopping_policy == MSQ_STOPPING_STAGNATION;
+    meta.low_basis[i] = !native && o.basis_precision != MSQ_FP32;
+    meta.residual_tolerance[i] = o.residual_tolerance;
+    refine_any = refine_any || (o.refine_unresolved && o.resolution > 0.0f);
+    stagnation_any = stagnation_any || meta.stagnation[i];
+    certificate_any = certificate_any || (o.certificate && !native);
+  }
   GraphEntry* graph = nullptr;
   if (rt->graphs_enabled) {
-    graph = find_graph(rt, count, orders, matrices, roots_out, options, options_count);
+    graph = find_graph(rt, count, orders, matrices, *outputs, options, options_count);
     if (graph && graph->exec) {
       // Replay: the plans and uploads of this signature live in the entry's
       // arena; only the device work runs again.
       MSQ_TRY(launch_graph(rt, *graph));
       return finish_solve(rt, graph->layout, graph->st, graph->jst, graph->V, graph->sroot,
-                          reports);
+                          reports, meta);
     }
   }
-  auto opt = [&](uint32_t i) -> const msq_solve_options& {
-    return options[options_count == 1 ? 0 : i];
-  };
-  // Groups of matrices sharing (basis, correction) precisions.
+  // Groups: matrices sharing (basis, correction) tiers of the reorthogonalized
+  // path, and native-quantized matrices by storage format.
   std::map<std::pair<int, int>, std::vector<uint32_t>> groups;
-  std::map<int, std::vector<uint32_t>> by_corr;
+  std::map<int, std::vector<uint32_t>> by_corr, by_spectrum, by_native;
+  std::vector<uint32_t> cert_members, stag_members;
   uint32_t max_corrections = 0;
   std::vector<msq_jacobi_options> jopts(options_count);
   std::vector<msq::SolveLimits> limits(count);
   for (uint32_t i = 0; i < count; ++i) {
     const msq_solve_options& o = opt(i);
-    MSQ_TRY(validate_tier(o.basis_precision, "basis"));
-    MSQ_TRY(validate_tier(o.correction_precision, "correction"));
-    if (o.max_corrections >= msq::kSolveHistory)
-      return set_error(MSQ_ERROR_INVALID_ARGUMENT, "max_corrections must be < %u",
-                       msq::kSolveHistory);
-    groups[{o.basis_precision, o.correction_precision}].push_back(i);
-    by_corr[o.correction_precision].push_back(i);
+    if (meta.native[i]) {
+      by_native[o.basis_precision].push_back(i);
+    } else {
+      groups[{o.basis_precision, o.correction_precision}].push_back(i);
+      by_corr[o.correction_precision].push_back(i);
+      if (o.spectrum_precision != MSQ_FP32) by_spectrum[o.spectrum_precision].push_back(i);
+      if (o.certificate) cert_members.push_back(i);
+    }
+    if (meta.stagnation[i]) stag_members.push_back(i);
     max_corrections = std::max(max_corrections, o.max_corrections);
-    limits[i].residual_tolerance = o.residual_tolerance;
-    limits[i].forward_tolerance = o.forward_tolerance;
-    limits[i].min_corrections = std::min(o.min_corrections, o.max_corrections);
-    limits[i].max_corrections = o.max_corrections;
+    msq::SolveLimits& lim = limits[i];
+    lim = msq::SolveLimits{};
+    lim.residual_tolerance = o.residual_tolerance;
+    lim.forward_tolerance = o.forward_tolerance;
+    lim.minimum_relative_improvement = o.minimum_relative_improvement;
+    lim.basis_tolerance = o.basis_tolerance;
+    lim.min_corrections = std::min(o.min_corrections, o.max_corrections);
+    lim.max_corrections = o.max_corrections;
+    lim.stagnation_window = meta.stagnation[i] ? o.stagnation_window : 0u;
+    lim.resolution = o.resolution;
+    lim.refine = o.refine_unresolved;
   }
   for (uint32_t k = 0; k < options_count; ++k) jopts[k] = options[k].jacobi;
 
@@ -1740,46 +2306,117 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
     t.segs = segments_of(L, t.members);
     tiers.push_back(std::move(t));
   }
+  struct SpectrumTier {
+    int precision;
+    std::vector<msq::Segment> segs;
+    msq::Segment* dsegs = nullptr;
+  };
+  std::vector<SpectrumTier> stiers;
+  for (auto& kv : by_spectrum) {
+    SpectrumTier t;
+    t.precision = kv.first;
+    t.segs = segments_of(L, kv.second);
+    stiers.push_back(std::move(t));
+  }
+  // Native groups: compact storage of every member (n^2 elements from a
+  // 64-element boundary), F32 products on decoded operands.
+  struct NativeGroup {
+    int format;
+    std::vector<uint32_t> members;
+    std::vector<msq::Segment> segs;
+    std::vector<uint64_t> coff;
+    GemmPlan plan;
+    msq::Segment* dsegs = nullptr;
+    uint64_t* dcoff = nullptr;
+    uint32_t max_order = 0;
+    NativeSet set() const {
+      NativeSet n;
+      n.format = format;
+      n.count = static_cast<uint32_t>(members.size());
+      n.max_order = max_order;
+      n.dsegs = dsegs;
+      n.dcoff = dcoff;
+      n.plan = &plan;
+      return n;
+    }
+  };
+  std::vector<NativeGroup> nl;
+  uint64_t compact_total = 0;
+  for (auto& kv : by_native) {
+    NativeGroup g;
+    g.format = kv.first;
+    g.members = kv.second;
+    g.segs = segments_of(L, g.members);
+    compact_total = std::max(compact_total, compact_offsets(L, g.members, g.coff, g.max_order));
+    build_plan(L, g.members, false, g.plan, rt, msq::GemmType::F32);
+    nl.push_back(std::move(g));
+  }
+  GemmPlan plan_cert;
+  std::vector<msq::Segment> segs_cert = segments_of(L, cert_members);
+  if (!cert_members.empty())
+    build_plan(L, cert_members, false, plan_cert, rt, msq::GemmType::F32, true);
+  std::vector<msq::Segment> segs_stag = segments_of(L, stag_members);
+  std::vector<int> stag_flags(count, 0);
+  for (uint32_t i : stag_members) stag_flags[i] = 1;
 
   // Device buffers.
   Scratch S{rt};
-  float *V = nullptr, *Vt = nullptr, *diag = nullptr, *sroot = nullptr, *R32 = nullptr;
+  float *V = nullptr, *Vt = nullptr, *diag = nullptr, *sroot = nullptr, *qroot = nullptr,
+        *R32 = nullptr;
   uint16_t* Rb = nullptr;
   double* norms = nullptr;
   double* orth2 = nullptr;  // per matrix: ||V_t^T V_t - I||_F^2 (basis acceptance)
+  double* cert2 = nullptr;  // per matrix: ||V^T V - I||_F^2 of the FP32 basis (certificate)
   int* esc = nullptr;       // per matrix: basis tier rejected
+  int* srej = nullptr;      // per matrix: spectrum tier rejected
   int* active = nullptr;
+  int* save = nullptr;       // per matrix: this step's iterate is the best (stagnation policy)
+  int* dstag = nullptr;      // per matrix: stagnation policy
+  double* xbest = nullptr;   // best iterates (stagnation policy)
   msq::JacobiStatus* jst = nullptr;
   msq::JacobiJob* djobs = nullptr;
   msq::SolveState* st = nullptr;
   msq::SolveLimits* dlim = nullptr;
   msq::Segment* dsegs_all = nullptr;
+  msq::Segment* dsegs_cert = nullptr;
+  msq::Segment* dsegs_stag = nullptr;
+  float *nq = nullptr, *nqT = nullptr, *nqt = nullptr, *nqtT = nullptr;  // native basis operands
+  uint8_t *ncodes = nullptr, *nscales = nullptr;
   CorrectionBuffers cb;
   S.want(V, L.total);
   S.want(Vt, L.total);
   S.want(diag, L.vtotal);
   S.want(sroot, L.vtotal);
-  // R feeds the corrections as float (FP32 / FP8 / MXFP4 tiers) and as BF16
-  // codes (BF16 tier); the binary64 R is not kept.
-  bool any_bf16 = false, any_float = false;
+  S.want(qroot, L.vtotal);
+  // R feeds the corrections as float (FP32 / FP8 / MXFP4 tiers, native) and
+  // as BF16 codes (BF16 tier); the binary64 R is not kept.
+  bool any_bf16 = false, any_float = !nl.empty();
   for (const Group& g : gl) (g.corr == MSQ_BF16 ? any_bf16 : any_float) = true;
   if (any_float) S.want(R32, L.total);
   if (any_bf16) S.want(Rb, L.total);
   S.want(norms, count);
   S.want(orth2, count);
+  S.want(cert2, count);
   S.want(esc, count);
+  S.want(srej, count);
   S.want(active, count);
+  S.want(save, count);
+  S.want(dstag, count);
+  if (stagnation_any) S.want(xbest, L.total);
   S.want(jst, count);
   S.want(djobs, count);
   S.want(st, count);
   S.want(dlim, count);
   S.want(dsegs_all, count);
+  S.want(dsegs_cert, std::max<size_t>(1, cert_members.size()));
+  S.want(dsegs_stag, std::max<size_t>(1, stag_members.size()));
   S.want(cb.t1, L.total);
   S.want(cb.t2, L.total);
   S.want(cb.alpha, count);
   tier_want_all(S, cb.enc_a, L);
   tier_want_all(S, cb.enc_b, L);
   plan_want(S, plan_res);
+  if (!cert_members.empty()) plan_want(S, plan_cert);
   for (Group& g : gl) {
     S.want(g.dsegs, g.members.size());
     plan_want(S, g.plan);
@@ -1789,6 +2426,20 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
     S.want(t.dsegs, t.members.size());
     tier_want(S, t.ev, t.tier, L);
     tier_want(S, t.evt, t.tier, L);
+  }
+  for (SpectrumTier& t : stiers) S.want(t.dsegs, t.segs.size());
+  if (!nl.empty()) {
+    S.want(nq, L.total);
+    S.want(nqT, L.total);
+    S.want(nqt, L.total);
+    S.want(nqtT, L.total);
+    S.want(ncodes, compact_total);
+    S.want(nscales, compact_total / 32 + 1);
+    for (NativeGroup& g : nl) {
+      S.want(g.dsegs, g.members.size());
+      S.want(g.dcoff, g.members.size());
+      plan_want(S, g.plan);
+    }
   }
   // Graph mode: the call's buffers come from the signature's own arena, so
   // that a captured launch sequence keeps valid pointers.
@@ -1804,32 +2455,65 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   JacobiPlan jplan;
   plan_jacobi(L, jopts.data(), options_count, jplan);
   MSQ_TRY(upload(rt, djobs, jplan.jobs));
-
-
   MSQ_TRY(upload(rt, dsegs_all, segs_all));
   MSQ_TRY(upload(rt, dlim, limits));
+  MSQ_TRY(upload(rt, dstag, stag_flags));
+  MSQ_TRY(upload(rt, dsegs_cert, segs_cert));
+  MSQ_TRY(upload(rt, dsegs_stag, segs_stag));
   MSQ_TRY(plan_upload(rt, plan_res));
+  if (!cert_members.empty()) MSQ_TRY(plan_upload(rt, plan_cert));
   for (Group& g : gl) {
     MSQ_TRY(upload(rt, g.dsegs, g.segs));
     MSQ_TRY(plan_upload(rt, g.plan));
     MSQ_TRY(plan_upload(rt, g.upper));
   }
   for (CorrTier& t : tiers) MSQ_TRY(upload(rt, t.dsegs, t.segs));
+  for (SpectrumTier& t : stiers) MSQ_TRY(upload(rt, t.dsegs, t.segs));
+  for (NativeGroup& g : nl) {
+    MSQ_TRY(upload(rt, g.dsegs, g.segs));
+    MSQ_TRY(upload(rt, g.dcoff, g.coff));
+    MSQ_TRY(plan_upload(rt, g.plan));
+  }
+  const bool stochastic = rt->rounding_mode == MSQ_ROUNDING_STOCHASTIC;
+
+  auto native_set = [&](const NativeGroup& g) {
+    NativeSet n = g.set();
+    n.codes = ncodes;
+    n.scales = nscales;
+    return n;
+  };
 
   // Device work: Jacobi, reconstruction, frozen factor, residual and
-  // corrections (no host synchronisation below, except for orders above 256).
+  // corrections (no host synchronisation below, except for orders above 256
+  // and the second Jacobi pass of unresolved matrices).
   auto enqueue = [&]() -> int {
     // 1. Jacobi (symmetricJacobi).
     MSQ_TRY(enqueue_jacobi(rt, jplan, djobs, matrices, V, diag, sroot, jst));
     if (jplan.has_large)
       MSQ_TRY(run_large_jacobi(rt, L, matrices, V, diag, sroot, jopts.data(), options_count, jst));
-    HIP_TRY(msq::launch_solve_state(dsegs_all, jst, diag, dlim, st, active, count, rt->stream));
-    // 2. Reconstruction X0 = G G^T in the basis tier, X = sym(X0) in binary64.
-    //    A low-bit basis is accepted per matrix as the reference's
-    //    quantizeAndDecode + measureF32Orthogonality do (src/metal_solver.cpp:
-    //    217-236): ||V_t^T V_t - I||_F / sqrt(n) of V rounded to the tier
-    //    within the tier's tolerance (src/configuration.cpp:109-118); the
-    //    matrices that miss it get X0 from the FP32 basis instead.
+    HIP_TRY(msq::launch_solve_state(dsegs_all, jst, diag, dlim, st, active, count, refine_any,
+                                    rt->stream));
+    // 1b. Matrices whose smallest eigenvalue the FP32 Jacobi does not
+    //     resolve: a second pass on V^T A V formed in binary64.
+    if (refine_any) MSQ_TRY(refine_unresolved(rt, L, limits, matrices, V, diag, sroot, jst, st,
+                                              dsegs_all, dlim, active, meta));
+    // 2. Spectrum tier: the roots rounded to the spectrum precision, accepted
+    //    per matrix when every decoded root is positive and finite
+    //    (src/metal_solver.cpp:240-263); the others keep the FP32 roots.
+    HIP_TRY(msq::launch_copy_words(sroot, qroot, L.vtotal, rt->stream));
+    HIP_TRY(msq::launch_fill_u32(reinterpret_cast<uint32_t*>(srej), count, 0u, rt->stream));
+    for (SpectrumTier& t : stiers)
+      HIP_TRY(msq::launch_segment_spectrum(t.dsegs, static_cast<uint32_t>(t.segs.size()),
+                                           t.precision, sroot, qroot, srej, active, stochastic,
+                                           rt->rounding_seed, 1u, rt->stream));
+    // 3. Reconstruction X0 = G G^T in the basis tier (with the frozen
+    //    roots), X = sym(X0) in binary64.  A low-bit basis is accepted per
+    //    matrix as the reference's quantizeAndDecode + measureF32Orthogonality
+    //    do (src/metal_solver.cpp:217-236): ||V_t^T V_t - I||_F / sqrt(n) of V
+    //    rounded to the tier within the tolerance (src/configuration.cpp:
+    //    109-118, or the options' override); the matrices that miss it get X0
+    //    from the FP32 basis instead.
+    HIP_TRY(msq::launch_fill_u32(reinterpret_cast<uint32_t*>(esc), count, 0u, rt->stream));
     for (Group& g : gl) {
       const uint32_t gm = static_cast<uint32_t>(g.members.size());
       if (g.basis != MSQ_FP32) {
@@ -1844,28 +2528,87 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
         gg.norms = orth2;
         gg.mirror = 1;
         MSQ_TRY(run_gemm(rt, g.upper, gemm_type(g.basis), msq::Epilogue::Gram, gg));
-        HIP_TRY(msq::launch_basis_escalation(g.dsegs, gm, orth2, basis_tolerance(g.basis), active,
-                                             esc, st, rt->stream));
+        HIP_TRY(msq::launch_basis_escalation(g.dsegs, gm, orth2, basis_tolerance(g.basis), dlim,
+                                             active, esc, st, rt->stream));
       }
-      MSQ_TRY(reconstruction_product(rt, g.basis, L, g.members, g.dsegs, g.plan, V, sroot, active,
+      MSQ_TRY(reconstruction_product(rt, g.basis, L, g.members, g.dsegs, g.plan, V, qroot, active,
                                      cb, /*two_term=*/true));
       if (g.basis != MSQ_FP32)  // the rejected ones again, from the FP32 basis
-        MSQ_TRY(reconstruction_product(rt, MSQ_FP32, L, g.members, g.dsegs, g.plan, V, sroot, esc,
-                                       cb));
-      HIP_TRY(msq::launch_symmetrize_to_f64(g.dsegs, static_cast<uint32_t>(g.members.size()),
-                                            pair_count(g.segs), cb.t2, roots_out, active,
-                                            rt->stream));
+        MSQ_TRY(reconstruction_product(rt, MSQ_FP32, L, g.members, g.dsegs, g.plan, V, qroot,
+                                       esc, cb));
+      HIP_TRY(msq::launch_symmetrize_to_f64(g.dsegs, gm, pair_count(g.segs), cb.t2, roots_out,
+                                            active, rt->stream));
+      HIP_TRY(msq::launch_frozen_roots(g.dsegs, gm, qroot, srej, st, rt->stream));
     }
-    // 3. Frozen factor in each correction tier: V and V^T.
+    // Certificate: ||V^T V - I||_F of the FP32 basis.
+    if (!cert_members.empty()) {
+      HIP_TRY(msq::launch_zero_f64(cert2, count, rt->stream));
+      Operand ov;
+      ov.data = V;
+      msq::GemmArgs gg = product_args(ov, ov, nullptr, nullptr, active);
+      gg.norms = cert2;
+      gg.mirror = 1;
+      MSQ_TRY(run_gemm(rt, plan_cert, msq::GemmType::F32, msq::Epilogue::Gram, gg));
+      HIP_TRY(msq::launch_record_orthogonality(dsegs_cert,
+                                               static_cast<uint32_t>(cert_members.size()), cert2,
+                                               st, rt->stream));
+    }
+    // 3b. Native groups: basis, transposed basis and roots in the storage
+    //     format (copyToStorage / copySpectrumToStorage); X0 = quant(quant(Q
+    //     diag(s)) Q_t) (scaleColumns + matmul, src/metal_solver.cpp:
+    //     767-787), the spectrum rejected as an invalid spectrum.
+    for (NativeGroup& g : nl) {
+      const uint32_t gm = static_cast<uint32_t>(g.members.size());
+      const NativeSet ns = native_set(g);
+      HIP_TRY(msq::launch_segment_spectrum(g.dsegs, gm, g.format, sroot, qroot, srej, active,
+                                           false, 0u, 1u, rt->stream));
+      HIP_TRY(msq::launch_reject_spectrum(g.dsegs, gm, srej, st, active, rt->stream));
+      HIP_TRY(msq::launch_frozen_roots(g.dsegs, gm, qroot, srej, st, rt->stream));
+      MSQ_TRY(native_basis(rt, L, ns, V, nq, nqT, nqt, nqtT, active));
+      // Basis orthogonality of Q for the report.
+      HIP_TRY(msq::launch_zero_f64(orth2, count, rt->stream));
+      {
+        Operand oq;
+        oq.data = nq;
+        msq::GemmArgs gg = product_args(oq, oq, nullptr, nullptr, active);
+        gg.norms = orth2;
+        MSQ_TRY(run_gemm(rt, g.plan, msq::GemmType::F32, msq::Epilogue::Gram, gg));
+        // Recorded, never rejected (the native path has no basis acceptance).
+        HIP_TRY(msq::launch_basis_escalation(g.dsegs, gm, orth2, INFINITY, nullptr, active, esc,
+                                             st, rt->stream));
+      }
+      HIP_TRY(msq::launch_scale_columns(g.dsegs, gm, L.max_ld, nq, qroot, cb.t1, active,
+                                        rt->stream));
+      MSQ_TRY(native_round(rt, L, ns, cb.t1, false, cb.t2, true, active));
+      MSQ_TRY(native_product(rt, ns, cb.t2, nqt, cb.t1, active));
+      MSQ_TRY(native_round(rt, L, ns, cb.t1, false, cb.t2, false, active));
+      HIP_TRY(msq::launch_symmetrize_to_f64(g.dsegs, gm, pair_count(g.segs), cb.t2, roots_out,
+                                            active, rt->stream));
+    }
+    if (outputs->initial_roots)
+      HIP_TRY(msq::launch_copy_words(roots_out, outputs->initial_roots, 2 * L.total, rt->stream));
+    // The best iterate starts as X0 (stagnation policy).
+    if (stagnation_any)
+      HIP_TRY(msq::launch_copy_tiles_f64(dsegs_stag, static_cast<uint32_t>(stag_members.size()),
+                                         L.max_ld, roots_out, xbest, nullptr, rt->stream));
+    // 4. Frozen factor in each correction tier: V and V^T.
     HIP_TRY(msq::launch_transpose(dsegs_all, count, pair_count(segs_all), V, Vt, rt->stream));
     for (CorrTier& t : tiers) {
       MSQ_TRY(encode_operand(rt, t.tier, L, t.members, t.dsegs, V, t.ev, t.ov));
       MSQ_TRY(encode_operand(rt, t.tier, L, t.members, t.dsegs, Vt, t.evt, t.ovt));
     }
-    // 4. Residual of X0, then corrections until each matrix stops.
-    MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, nullptr, R32, Rb, norms, active,
-                             count));
-    HIP_TRY(msq::launch_convergence(st, norms, active, count, 0, rt->stream));
+    // 5. Residual of X0, then corrections until each matrix stops.
+    auto residual_step = [&](uint32_t step) -> int {
+      MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, nullptr, R32, Rb, norms, active,
+                               count));
+      HIP_TRY(msq::launch_convergence(st, norms, active, stagnation_any ? save : nullptr, count,
+                                      step, rt->stream));
+      if (stagnation_any)
+        HIP_TRY(msq::launch_copy_tiles_f64(dsegs_stag, static_cast<uint32_t>(stag_members.size()),
+                                           L.max_ld, roots_out, xbest, save, rt->stream));
+      return MSQ_OK;
+    };
+    MSQ_TRY(residual_step(0));
     for (uint32_t step = 1; step <= max_corrections; ++step) {
       for (Group& g : gl) {
         const CorrTier* t = nullptr;
@@ -1874,13 +2617,27 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
         MSQ_TRY(correction_products(rt, g.corr, L, g.members, g.dsegs, g.plan, t->ov, t->ovt, R32,
                                     Rb, sroot, active, cb, roots_out, &g.upper));
       }
-      MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, nullptr, R32, Rb, norms, active,
-                             count));
-      HIP_TRY(msq::launch_convergence(st, norms, active, count, step, rt->stream));
+      // Native: U = Q_t R, C = U Q, Y = C / (s_i + s_j), V2 = Q Y, E = V2 Q_t,
+      // each rounded to the format; X = sym(X + E).
+      for (NativeGroup& g : nl) {
+        const uint32_t gm = static_cast<uint32_t>(g.members.size());
+        MSQ_TRY(native_correction(rt, L, native_set(g), R32, nq, nqT, nqt, nqtT, qroot, cb.t1,
+                                  cb.t2, active));
+        HIP_TRY(msq::launch_add_symmetrize_f64(g.dsegs, gm, L.max_ld, roots_out, cb.t2, active,
+                                               rt->stream));
+      }
+      MSQ_TRY(residual_step(step));
     }
+    // The stagnation policy returns the best iterate.
+    if (stagnation_any)
+      HIP_TRY(msq::launch_copy_tiles_f64(dsegs_stag, static_cast<uint32_t>(stag_members.size()),
+                                         L.max_ld, xbest, roots_out, dstag, rt->stream));
+    if (outputs->vectors) HIP_TRY(msq::launch_copy_words(V, outputs->vectors, L.total, rt->stream));
+    if (outputs->root_eigenvalues)
+      HIP_TRY(msq::launch_copy_words(sroot, outputs->root_eigenvalues, L.vtotal, rt->stream));
     return MSQ_OK;
   };
-  if (graph && !jplan.has_large && graph->calls >= 1) {
+  if (graph && !jplan.has_large && !refine_any && graph->calls >= 1) {
     MSQ_TRY(capture_graph(rt, *graph, enqueue));
     graph->layout = L;
     graph->st = st;
@@ -1892,7 +2649,7 @@ int msq_solve_batched(msq_runtime* rt, uint32_t count, const uint32_t* orders,
     MSQ_TRY(enqueue());
     if (graph) ++graph->calls;
   }
-  return finish_solve(rt, L, st, jst, V, sroot, reports);
+  return finish_solve(rt, L, st, jst, V, sroot, reports, meta);
 }
 
 int msq_last_factors(msq_runtime* rt, const float** vectors, const float** roots) {

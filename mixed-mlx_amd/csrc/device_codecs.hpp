@@ -15,6 +15,7 @@
 // needs no floating-point mode and is exact for subnormal inputs too.
 #pragma once
 
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -231,6 +232,136 @@ MSQ_HD int mx_scale_exponent(uint32_t max_abs_bits, int emax) {
 MSQ_HD uint32_t finite_abs_bits(float v) {
   const uint32_t a = f32_bits(v) & 0x7fffffffu;
   return a < 0x7f800000u ? a : 0u;
+}
+
+// ------------------------------------------------------ stochastic rounding
+// The reference's stochastic mode (metal_mixed_sqrt.metal:394-405, :462-483,
+// :545-567, :740-762, :1888-1910): a value between two adjacent codes rounds
+// up with probability (|v| - lower) / (upper - lower), decided by a counter
+// hash of (element index, seed, stream); values the nearest mode maps
+// exactly, zeros, NaN and the saturating range use the nearest-mode encoder.
+// The probability is exact in float (the numerator is exact by Sterbenz's
+// lemma or is |v| itself, the denominator a power of two), so the decision is
+// the same on every IEEE implementation.
+MSQ_HD uint32_t stochastic_hash(uint32_t index, uint32_t seed, uint32_t stream) {
+  const uint32_t state = index * 747796405u + seed + stream * 2891336453u;
+  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+MSQ_HD bool stochastic_round_up(float probability, uint32_t index, uint32_t seed,
+                                uint32_t stream) {
+  // 24 hash bits as a float in [0, 1): exact.
+  const float sample =
+      static_cast<float>(stochastic_hash(index, seed, stream) >> 8u) * 5.9604644775390625e-08f;
+  return sample < probability;
+}
+
+MSQ_HD uint16_t encode_bf16_stochastic(float value, uint32_t index, uint32_t seed,
+                                       uint32_t stream) {
+  const uint32_t bits = f32_bits(value);
+  if ((bits & 0x7f800000u) == 0x7f800000u || (bits & 0x7fffffffu) == 0u)
+    return encode_bf16(value);
+  const uint32_t discarded = bits & 0xffffu;
+  uint32_t rounded = bits & 0xffff0000u;
+  if ((stochastic_hash(index, seed, stream) & 0xffffu) < discarded) rounded += 0x00010000u;
+  return static_cast<uint16_t>(rounded >> 16);
+}
+
+MSQ_HD uint8_t encode_e4m3_stochastic(float value, uint32_t index, uint32_t seed,
+                                      uint32_t stream) {
+  const uint32_t bits = f32_bits(value);
+  const uint8_t sign = (bits >> 31) ? 0x80u : 0u;
+  const float mag = bits_f32(bits & 0x7fffffffu);
+  if (!(mag == mag) || mag == 0.0f || mag >= 448.0f) return encode_e4m3(value);
+  const uint8_t nearest = encode_e4m3(mag);
+  const float nv = decode_e4m3(nearest);
+  if (nv == mag) return sign | nearest;
+  const uint8_t lower = nv < mag ? nearest : static_cast<uint8_t>(nearest - 1u);
+  const uint8_t upper = static_cast<uint8_t>(lower + 1u);
+  const float lv = decode_e4m3(lower), uv = decode_e4m3(upper);
+  const float p = (mag - lv) / (uv - lv);
+  return sign | (stochastic_round_up(p, index, seed, stream) ? upper : lower);
+}
+
+MSQ_HD uint8_t encode_e5m2_stochastic(float value, uint32_t index, uint32_t seed,
+                                      uint32_t stream) {
+  const uint32_t bits = f32_bits(value);
+  const uint8_t sign = (bits >> 31) ? 0x80u : 0u;
+  const float mag = bits_f32(bits & 0x7fffffffu);
+  if (!(mag == mag) || mag == 0.0f || mag >= 57344.0f) return encode_e5m2(value);  // inf too
+  const uint8_t nearest = encode_e5m2(mag);
+  const float nv = decode_e5m2(nearest);
+  if (nv == mag) return sign | nearest;
+  const uint8_t lower = nv < mag ? nearest : static_cast<uint8_t>(nearest - 1u);
+  const uint8_t upper = static_cast<uint8_t>(lower + 1u);
+  const float lv = decode_e5m2(lower), uv = decode_e5m2(upper);
+  const float p = (mag - lv) / (uv - lv);
+  return sign | (stochastic_round_up(p, index, seed, stream) ? upper : lower);
+}
+
+// FP4 E2M1 nibble.
+MSQ_HD uint8_t encode_e2m1_stochastic(float value, uint32_t index, uint32_t seed,
+                                      uint32_t stream) {
+  const uint32_t bits = f32_bits(value);
+  const uint8_t sign = (bits >> 31) ? 0x8u : 0u;
+  const float mag = bits_f32(bits & 0x7fffffffu);
+  if (!(mag == mag) || mag == 0.0f || mag >= 6.0f) return encode_e2m1(value);  // inf too
+  uint8_t lower = 0;
+  while (lower < 7u && decode_e2m1(static_cast<uint8_t>(lower + 1u)) <= mag) ++lower;
+  const float lv = decode_e2m1(lower);
+  if (lv == mag || lower == 7u) return sign | lower;
+  const uint8_t upper = static_cast<uint8_t>(lower + 1u);
+  const float uv = decode_e2m1(upper);
+  const float p = (mag - lv) / (uv - lv);
+  return sign | (stochastic_round_up(p, index, seed, stream) ? upper : lower);
+}
+
+// 2^e as a float, inf for e = 128 (the MX scale byte 255).
+MSQ_HD float pow2f(int e) { return ldexp_exact(1.0f, e); }
+
+// ------------------------------------------------------ log-affine spectrum
+// The reference's log-affine spectrum storage for FP8 E4M3FN and FP4 E2M1
+// (precisions *_log_s; metal_mixed_sqrt.metal:1689-1817, blocks of 8): per
+// block the float pair (min log2 s, span), span the block's log2 range
+// rounded down by one ulp when min + span would exceed the block maximum;
+// code = encode(M (log2 s - min) / span), M = 448 (E4M3) or 6 (E2M1), 0 for a
+// constant block; decode exp2(min + span * code / M).  exp2 is evaluated in
+// binary64 and rounded to float; log2 in binary64 and rounded toward -inf, so
+// that exp2 of a block's largest log2 never exceeds its largest value (to
+// nearest, log2 of the largest float is 128 and the top code would decode to
+// infinity; the reference's one-ulp step of the span aims at the same bound).
+constexpr int kLogBlock = 8;
+MSQ_HD float log2_f32(float x) {
+  const double l = ::log2(static_cast<double>(x));
+  float f = static_cast<float>(l);
+  if (static_cast<double>(f) > l) f = bits_f32(f > 0.0f ? f32_bits(f) - 1u : f32_bits(f) + 1u);
+  return f;
+}
+MSQ_HD float exp2_f32(float y) { return static_cast<float>(::exp2(static_cast<double>(y))); }
+// The next float toward zero (span > 0, finite).
+MSQ_HD float toward_zero_f32(float v) {
+  const uint32_t b = f32_bits(v);
+  return (b & 0x7fffffffu) == 0u ? v : bits_f32(b - 1u);
+}
+// Metadata (min, span) of the block of values[begin, end).
+MSQ_HD void log_block_metadata(const float* values, uint32_t begin, uint32_t end, float& minimum,
+                               float& span) {
+  float mn = INFINITY, mx = -INFINITY;
+  for (uint32_t k = begin; k < end; ++k) {
+    const float l = log2_f32(values[k]);
+    mn = l < mn ? l : mn;  // the reference's min / max (NaN-ignoring as fmin / fmax)
+    mx = l > mx ? l : mx;
+  }
+  float sp = mx - mn;
+  if (mn + sp > mx) sp = toward_zero_f32(sp);
+  minimum = mn;
+  span = sp;
+}
+MSQ_HD float log_normalized(float value, float minimum, float span, float maximum) {
+  return span == 0.0f ? 0.0f : maximum * (log2_f32(value) - minimum) / span;
+}
+MSQ_HD float log_decode(float decoded_code, float minimum, float span, float maximum) {
+  return exp2_f32(minimum + span * decoded_code / maximum);
 }
 
 }  // namespace msq

@@ -262,7 +262,8 @@ std::vector<double> makeSpdRoot(size_t order, double condition) {
 namespace {
 
 msq_solve_options solve_options(const HipSolverConfiguration& c, const HipProblem& p) {
-  msq_solve_options o{};
+  msq_solve_options o;
+  msq_solve_options_init(&o);
   o.basis_precision = static_cast<int32_t>(c.basis_precision);
   o.correction_precision = static_cast<int32_t>(c.correction_precision);
   o.jacobi.fixed_sweeps = c.fixed_sweeps;

@@ -32,6 +32,17 @@ hipError_t launch_encode_fp8_scaled(const float* src, uint8_t* values,
                                     const float* sigma, size_t n, hipStream_t stream);
 hipError_t launch_decode_fp8_scaled(const uint8_t* values, const float* sigma,
                                     float* dst, size_t n, hipStream_t stream);
+// Stochastic rounding of the linear formats (the hash of element index, seed
+// and stream decides each rounding direction).
+hipError_t launch_encode_stochastic(Format fmt, const float* src, void* values, uint8_t* scales,
+                                   size_t n, uint32_t seed, uint32_t stream, hipStream_t s);
+// Log-affine spectrum codec (E4M3FN or E2M1 codes, two floats of metadata per
+// block of 8 elements), nearest or stochastic.
+hipError_t launch_encode_log(bool e4m3, const float* src, uint8_t* values, float* metadata,
+                             size_t n, bool stochastic, uint32_t seed, uint32_t stream,
+                             hipStream_t s);
+hipError_t launch_decode_log(bool e4m3, const uint8_t* values, const float* metadata, float* dst,
+                             size_t n, hipStream_t s);
 
 // ------------------------------------------------------------------ Jacobi
 constexpr uint32_t kBatchedJacobiMaxOrder = 256;
@@ -200,10 +211,23 @@ struct GemmArgs {
                              // above the diagonal are stored at (i, j) and
                              // (j, i) (a symmetric product)
   float c_scale;             // Accumulate: C += c_scale * alpha * A B
+  // Residual / Gram: each wave's share of ||.||^2 goes to partials[partial_base
+  // + tile * waves + wave] (tile: the position in this launch's tile list,
+  // waves: gemm_waves(tile edge)); launch_gemm_norms then adds each matrix's
+  // slots in order, so the norms do not depend on which tile finishes first.
+  double* partials;
+  uint32_t partial_base;
 };
 
 hipError_t launch_gemm(GemmType type, Epilogue epi, int tile, const GemmArgs& args,
                        uint32_t tiles, hipStream_t stream);
+// Waves per workgroup of a tile edge (32: 1, 64 and 128: 4).
+int gemm_waves(int tile);
+// norms[jobs[k].status_index] = sum of partials[first[k] .. first[k] + count[k])
+// in order, for the jobs whose matrix is active (active null: all).
+hipError_t launch_gemm_norms(const GemmJob* jobs, const uint32_t* first, const uint32_t* count,
+                             uint32_t njobs, const double* partials, const int* active,
+                             double* norms, hipStream_t stream);
 
 // ------------------------------------------------------ elementwise helpers
 // All per-matrix arrays of a batch share one segment descriptor: an ld x ld
@@ -242,9 +266,30 @@ hipError_t launch_segment_fp8(const Segment* segs, uint32_t count, uint32_t max_
                               hipStream_t stream);
 hipError_t launch_alpha(const float* sa, const float* sb, float* alpha, uint32_t count,
                         hipStream_t stream);
+// Compact matrix storage of the native-quantized correction: segment k's
+// n x n elements in column-major linear order at compact element offset
+// coff[k] (a multiple of 64), in an msq_precision codec (the reference's host
+// codecs, nearest rounding); the float side is the batch layout, optionally
+// transposed.  active: per matrix index, or null.
+hipError_t launch_compact_quantize(const Segment* segs, const uint64_t* coff, uint32_t count,
+                                  uint32_t max_order, int precision, const float* src,
+                                  bool src_transposed, uint8_t* codes, uint8_t* scales,
+                                  const int* active, hipStream_t s);
+hipError_t launch_compact_dequantize(const Segment* segs, const uint64_t* coff, uint32_t count,
+                                    uint32_t max_ld, int precision, const uint8_t* codes,
+                                    const uint8_t* scales, float* dst, bool dst_transposed,
+                                    const int* active, hipStream_t s);
+// Spectrum tier of a solve: each segment's roots (order entries at
+// vec_offset) rounded to `precision` (an msq_precision value) and decoded into
+// qroots; rejected[index] = 1 (and qroots = roots) when a decoded root is not
+// positive and finite.  One workgroup per segment.
+hipError_t launch_segment_spectrum(const Segment* segs, uint32_t count, int precision,
+                                   const float* roots, float* qroots, int* rejected,
+                                   const int* active, bool stochastic, uint32_t seed,
+                                   uint32_t stream, hipStream_t s);
 
 // ------------------------------------------------------------ solve state
-constexpr uint32_t kSolveHistory = 20;
+constexpr uint32_t kSolveHistory = 65;
 enum : uint32_t {
   kSolveRunning = 0,
   kSolveConverged = 1,
@@ -252,13 +297,21 @@ enum : uint32_t {
   kSolveDiverged = 3,
   kSolveNonfinite = 4,
   kSolveInvalidSpectrum = 5,
+  kSolveStagnated = 6,
+  kSolveUnresolved = 7,
 };
 
 struct SolveLimits {
   double residual_tolerance;  // stop when ||A - X^2||_F / ||A||_F <= this ...
   double forward_tolerance;   // ... or the forward-error estimate is <= this
+  double minimum_relative_improvement;  // stagnation policy
+  double basis_tolerance;     // basis acceptance (0: the group's tier default)
   uint32_t min_corrections;
   uint32_t max_corrections;
+  uint32_t stagnation_window;  // 0: fixed budget
+  float resolution;            // unresolved: min diag < resolution * max diag
+  uint32_t refine;             // the second Jacobi pass runs for unresolved ones
+  uint32_t reserved;
 };
 
 struct SolveState {
@@ -267,30 +320,92 @@ struct SolveState {
   double min_root;               // sqrt of the smallest Jacobi diagonal entry
   double residual_tolerance;
   double forward_tolerance;
+  double minimum_relative_improvement;
   uint32_t min_corrections;
   uint32_t max_corrections;
+  uint32_t stagnation_window;
   uint32_t steps;                // residuals recorded (initial root + corrections)
   uint32_t corrections;          // corrections applied
   uint32_t status;
+  uint32_t unresolved;           // smallest eigenvalue below the resolution
+  uint32_t best_iteration;
+  uint32_t stagnation_steps;
+  uint32_t escalated;            // basis tier rejected, X0 rebuilt in FP32
+  uint32_t spectrum_rejected;    // spectrum tier rejected, X0 from FP32 roots
   uint32_t reserved;
+  double best_residual;
   double final_residual;
   double final_forward_estimate;
+  double basis_orthogonality;    // ||V_t^T V_t - I||_F / sqrt(n) of the basis-tier V
+  double frozen_min_root;        // min of the roots X0 was formed with
+  double fp32_orthogonality;     // ||V^T V - I||_F / sqrt(n) of the FP32 basis (NaN: not measured)
   double residuals[kSolveHistory];          // after X0, then after each correction
   double forward_estimates[kSolveHistory];  // ||R||_F / (2 s_min ||X||_F)
-  double basis_orthogonality;    // ||V_t^T V_t - I||_F / sqrt(n) of the basis-tier V
-  uint32_t escalated;            // 1: the basis tier was rejected, X0 rebuilt in FP32
-  uint32_t reserved2;
 };
 
+// allow_refine: matrices unresolved with refine set stay running even with a
+// diagonal entry that is not positive (the second Jacobi pass decides them).
 hipError_t launch_solve_state(const Segment* segs, const JacobiStatus* js,
                               const float* diagonal, const SolveLimits* limits, SolveState* st,
-                              int* active, uint32_t count, hipStream_t stream);
-hipError_t launch_convergence(SolveState* st, const double* norms, int* active, uint32_t count,
-                              uint32_t step, hipStream_t stream);
+                              int* active, uint32_t count, bool allow_refine,
+                              hipStream_t stream);
+// After R = A - X^2 of step `step` (0: X0): history, stop rules, next active
+// flag; save[i] = 1 when matrix i's residual is its best so far under the
+// stagnation policy (the caller keeps that iterate).
+hipError_t launch_convergence(SolveState* st, const double* norms, int* active, int* save,
+                              uint32_t count, uint32_t step, hipStream_t stream);
 // Basis escalation of the segments' matrices: orthogonality o = sqrt(orth2 /
-// n) into the state, and esc = active && !(o <= tolerance).
+// n) into the state, and esc = active && !(o <= tolerance), the tolerance
+// limits[i].basis_tolerance or else `tolerance`.
 hipError_t launch_basis_escalation(const Segment* segs, uint32_t count, const double* orth2,
-                                   double tolerance, const int* active, int* esc,
-                                   SolveState* st, hipStream_t stream);
+                                   double tolerance, const SolveLimits* limits, const int* active,
+                                   int* esc, SolveState* st, hipStream_t stream);
+// ||V^T V - I||_F / sqrt(n) = sqrt(orth2 / n) of each segment's FP32 basis
+// into the state's fp32_orthogonality.
+hipError_t launch_record_orthogonality(const Segment* segs, uint32_t count, const double* orth2,
+                                       SolveState* st, hipStream_t stream);
+// Native-quantized matrices whose frozen roots were rejected stop as an
+// invalid spectrum (the reference fails, src/metal_solver.cpp:720-727).
+hipError_t launch_reject_spectrum(const Segment* segs, uint32_t count, const int* rejected,
+                                  SolveState* st, int* active, hipStream_t stream);
+// Frozen roots of each segment's matrix: min over qroots (the spectrum tier's,
+// or the FP32 roots when that was rejected) into the state; rejected[i] into
+// the state's spectrum_rejected.
+hipError_t launch_frozen_roots(const Segment* segs, uint32_t count, const float* qroots,
+                               const int* rejected, SolveState* st, hipStream_t stream);
+// Copies the ld x ld tiles of the segments whose flag is set from src to dst
+// (binary64); flag null: all.
+hipError_t launch_copy_tiles_f64(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                 const double* src, double* dst, const int* flag,
+                                 hipStream_t stream);
+// x = ((x + d) + (x + d)^T) / 2 over each active segment's order (the
+// reference's addCorrectionFromStorage then symmetrizeInPlace); d float.
+hipError_t launch_add_symmetrize_f64(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                     double* x, const float* d, const int* active,
+                                     hipStream_t stream);
+// Native-quantized stages on float tiles (zero outside the order):
+// out_ij = in_ij / (s_i + s_j) in float (diagonal_sylvester_*,
+// metal_mixed_sqrt.metal:58-74, :913-932) and out_ij = in_ij * s_j
+// (scale_columns_*, :21-37), s the segment's roots (vector layout).
+hipError_t launch_sylvester_divide(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                   const float* in, const float* roots, float* out,
+                                   const int* active, hipStream_t stream);
+hipError_t launch_scale_columns(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                const float* in, const float* roots, float* out,
+                                const int* active, hipStream_t stream);
+// dst[0, words) = src[0, words), 4-byte words (float or binary64 data).
+hipError_t launch_copy_words(const void* src, void* dst, size_t words, hipStream_t stream);
+// Elementwise float -> binary64.
+hipError_t launch_widen_f64(const float* src, double* dst, size_t n, hipStream_t stream);
+// Tiles of the segments `from` (source batch layout) into consecutive tiles of
+// `to` (destination batch layout), element type by size: gather (from =
+// batch, to = sub-batch) or scatter (the reverse).  Both lists in the same
+// order, same orders.
+hipError_t launch_move_tiles(const Segment* from, const Segment* to, uint32_t count,
+                             uint32_t max_ld, const void* src, void* dst, size_t elem_bytes,
+                             hipStream_t stream);
+hipError_t launch_move_vectors(const Segment* from, const Segment* to, uint32_t count,
+                               uint32_t max_ld, const float* src, float* dst,
+                               hipStream_t stream);
 
 }  // namespace msq

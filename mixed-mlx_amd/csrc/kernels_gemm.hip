@@ -392,9 +392,25 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(GemmArgs args) {
     }
   }
   if constexpr (EPI == Epilogue::Residual || EPI == Epilogue::Gram) {
+    // A fixed butterfly within the wave, then one slot per wave.
     for (int off = 32; off > 0; off >>= 1) norm2 += __shfl_xor(norm2, off);
-    if (lane == 0) atomicAdd(args.norms + job.status_index, norm2);
+    if (lane == 0)
+      args.partials[args.partial_base + static_cast<size_t>(tile) * (NT / 64) + wave] = norm2;
   }
+}
+
+__global__ void gemm_norms_kernel(const GemmJob* __restrict__ jobs,
+                                  const uint32_t* __restrict__ first,
+                                  const uint32_t* __restrict__ count, uint32_t njobs,
+                                  const double* __restrict__ partials,
+                                  const int* __restrict__ active, double* __restrict__ norms) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= njobs) return;
+  const uint32_t i = jobs[k].status_index;
+  if (active && !active[i]) return;
+  double s = 0.0;
+  for (uint32_t q = 0; q < count[k]; ++q) s += partials[static_cast<size_t>(first[k]) + q];
+  norms[i] = s;
 }
 
 template <GemmType T, Epilogue E>
@@ -439,6 +455,17 @@ hipError_t dispatch_epilogue(GemmType type, Epilogue epi, int tile, const GemmAr
 }
 
 }  // namespace
+
+int gemm_waves(int tile) { return tile == 32 ? 1 : 4; }
+
+hipError_t launch_gemm_norms(const GemmJob* jobs, const uint32_t* first, const uint32_t* count,
+                             uint32_t njobs, const double* partials, const int* active,
+                             double* norms, hipStream_t stream) {
+  if (njobs == 0) return hipSuccess;
+  gemm_norms_kernel<<<(njobs + 255) / 256, 256, 0, stream>>>(jobs, first, count, njobs, partials,
+                                                              active, norms);
+  return hipGetLastError();
+}
 
 hipError_t launch_gemm(GemmType type, Epilogue epi, int tile, const GemmArgs& args,
                        uint32_t tiles, hipStream_t stream) {

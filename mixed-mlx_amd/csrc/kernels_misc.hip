@@ -214,12 +214,20 @@ __global__ void alpha_kernel(const float* __restrict__ sa, const float* __restri
 }
 
 // Residual bookkeeping after R = A - X^2: relative residual, forward-error
-// estimate ||R||_F / (2 s_min ||X||_F) with ||X||_F^2 = trace(A), the next
-// iteration's active flag.
+// estimate ||R||_F / (2 s_min ||X||_F) with ||X||_F^2 = trace(A), the best
+// iterate, the stop rules and the next iteration's active flag.
+//  * Fixed budget: stop on the residual or estimate target, the budget, or
+//    divergence (the residual grew past 4x the initial root's).
+//  * Stagnation (stagnation_window > 0; src/metal_solver.cpp:883-902): also
+//    stop after `window` corrections in a row that improve the best residual
+//    by no more than minimum_relative_improvement; the caller keeps the best
+//    iterate (save[i] = 1 when this step's is the new best).
 __global__ void convergence_kernel(SolveState* __restrict__ st, const double* __restrict__ norms,
-                                   int* __restrict__ active, uint32_t count, uint32_t step) {
+                                   int* __restrict__ active, int* __restrict__ save,
+                                   uint32_t count, uint32_t step) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
+  if (save) save[i] = 0;
   SolveState& s = st[i];
   if (!active[i]) return;
   const double rel = sqrt(norms[i]) / s.matrix_norm;
@@ -232,6 +240,13 @@ __global__ void convergence_kernel(SolveState* __restrict__ st, const double* __
   s.final_residual = rel;
   s.final_forward_estimate = fwd;
   const bool finite = isfinite(rel);
+  const double prior = s.best_residual;
+  const double improvement = step >= 1 && prior > 0.0 ? (prior - rel) / prior : 0.0;
+  if (finite && rel < prior) {
+    s.best_residual = rel;
+    s.best_iteration = step;
+    if (save) save[i] = 1;
+  }
   const bool met = rel <= s.residual_tolerance || fwd <= s.forward_tolerance;
   const uint32_t done_corrections = step;  // step 0 is the initial root
   int next = 1;
@@ -241,12 +256,22 @@ __global__ void convergence_kernel(SolveState* __restrict__ st, const double* __
   } else if (met && done_corrections >= s.min_corrections) {
     s.status = kSolveConverged;
     next = 0;
-  } else if (done_corrections >= s.max_corrections) {
+  } else if (s.stagnation_window > 0 && step >= 1 && !(rel <= s.residual_tolerance)) {
+    if (improvement > s.minimum_relative_improvement)
+      s.stagnation_steps = 0;
+    else
+      ++s.stagnation_steps;
+    if (s.stagnation_steps >= s.stagnation_window) {
+      s.status = kSolveStagnated;
+      next = 0;
+    }
+  }
+  if (next && done_corrections >= s.max_corrections) {
     s.status = met ? kSolveConverged : kSolveBudgetExhausted;
     next = 0;
   }
   // Diverged: the residual grew well past the initial root's.
-  if (next && step >= 1 && rel > 4.0 * s.residuals[0]) {
+  if (next && s.stagnation_window == 0 && step >= 1 && rel > 4.0 * s.residuals[0]) {
     s.status = kSolveDiverged;
     next = 0;
   }
@@ -255,53 +280,196 @@ __global__ void convergence_kernel(SolveState* __restrict__ st, const double* __
 }
 
 // Per-matrix solve state from the Jacobi results: ||A||_F, sqrt(trace A) =
-// ||A^(1/2)||_F, the smallest root; tolerances and budgets from the options.
+// ||A^(1/2)||_F, the smallest root, the resolution test; tolerances and
+// budgets from the options.
+// allow_refine: a matrix whose only fault is a diagonal entry below the
+// resolution (zero or negative included) stays running when its options ask
+// for the second Jacobi pass, which decides it.
 __global__ void solve_state_kernel(const Segment* __restrict__ segs,
                                    const JacobiStatus* __restrict__ js,
                                    const float* __restrict__ diagonal,
                                    const SolveLimits* __restrict__ limits,
                                    SolveState* __restrict__ st, int* __restrict__ active,
-                                   uint32_t count) {
+                                   uint32_t count, int allow_refine) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const Segment sg = segs[i];
   double trace = 0.0;
-  float dmin = INFINITY;
+  float dmin = INFINITY, dmax = 0.0f;
+  bool finite = true;
   for (uint32_t k = 0; k < sg.order; ++k) {
     const float d = diagonal[sg.vec_offset + k];
     trace += d;
     dmin = fminf(dmin, d);
+    dmax = fmaxf(dmax, d);
+    finite = finite && isfinite(d);
   }
+  const SolveLimits& lim = limits[i];
   SolveState s = {};
   s.matrix_norm = js[i].matrix_norm;
   s.root_norm = sqrt(trace > 0.0 ? trace : 0.0);
   s.min_root = dmin > 0.0f ? sqrt(static_cast<double>(dmin)) : 0.0;
-  s.residual_tolerance = limits[i].residual_tolerance;
-  s.forward_tolerance = limits[i].forward_tolerance;
-  s.min_corrections = limits[i].min_corrections;
-  s.max_corrections = limits[i].max_corrections;
-  const bool valid = (js[i].flags & kJacobiInvalidSpectrum) == 0 && dmin > 0.0f &&
-                     isfinite(s.matrix_norm) && s.matrix_norm > 0.0;
-  s.status = valid ? kSolveRunning : kSolveInvalidSpectrum;
+  s.residual_tolerance = lim.residual_tolerance;
+  s.forward_tolerance = lim.forward_tolerance;
+  s.minimum_relative_improvement = lim.minimum_relative_improvement;
+  s.min_corrections = lim.min_corrections;
+  s.max_corrections = lim.max_corrections;
+  s.stagnation_window = lim.stagnation_window;
+  s.best_residual = INFINITY;
+  s.basis_orthogonality = 0.0;
+  s.fp32_orthogonality = NAN;
+  s.frozen_min_root = s.min_root;
+  const bool norm_ok = isfinite(s.matrix_norm) && s.matrix_norm > 0.0;
+  const bool valid = (js[i].flags & kJacobiInvalidSpectrum) == 0 && dmin > 0.0f && norm_ok;
+  s.unresolved =
+      finite && norm_ok && dmax > 0.0f && lim.resolution > 0.0f && dmin < lim.resolution * dmax
+          ? 1u
+          : 0u;
+  const bool running = valid || (allow_refine && lim.refine && s.unresolved);
+  s.status = running ? kSolveRunning : kSolveInvalidSpectrum;
   st[i] = s;
-  active[i] = valid ? 1 : 0;
+  active[i] = running ? 1 : 0;
 }
 
 // measureF32Orthogonality's acceptance test (src/metal_solver.cpp:217-236):
 // the basis tier is kept when ||V_t^T V_t - I||_F / sqrt(n) <= tolerance.
 __global__ void basis_escalation_kernel(const Segment* __restrict__ segs, uint32_t count,
                                         const double* __restrict__ orth2, double tolerance,
+                                        const SolveLimits* __restrict__ limits,
                                         const int* __restrict__ active, int* __restrict__ esc,
                                         SolveState* __restrict__ st) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= count) return;
   const Segment sg = segs[k];
   const uint32_t i = sg.index;
+  const double tol =
+      limits && limits[i].basis_tolerance > 0.0 ? limits[i].basis_tolerance : tolerance;
   const double o = sqrt(orth2[i] / static_cast<double>(sg.order));
-  const int e = (active[i] && !(o <= tolerance)) ? 1 : 0;
+  const int e = (active[i] && !(o <= tol)) ? 1 : 0;
   esc[i] = e;
   st[i].basis_orthogonality = o;
   st[i].escalated = static_cast<uint32_t>(e);
+}
+
+__global__ void record_orthogonality_kernel(const Segment* __restrict__ segs, uint32_t count,
+                                            const double* __restrict__ orth2,
+                                            SolveState* __restrict__ st) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const Segment sg = segs[k];
+  st[sg.index].fp32_orthogonality = sqrt(orth2[sg.index] / static_cast<double>(sg.order));
+}
+
+__global__ void reject_spectrum_kernel(const Segment* __restrict__ segs, uint32_t count,
+                                       const int* __restrict__ rejected,
+                                       SolveState* __restrict__ st, int* __restrict__ active) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const uint32_t i = segs[k].index;
+  if (rejected[i] && active[i]) {
+    st[i].status = kSolveInvalidSpectrum;
+    active[i] = 0;
+  }
+}
+
+// Smallest frozen root of each segment (the roots X0 is formed with).
+__global__ void frozen_roots_kernel(const Segment* __restrict__ segs, uint32_t count,
+                                    const float* __restrict__ qroots,
+                                    const int* __restrict__ rejected,
+                                    SolveState* __restrict__ st) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const Segment sg = segs[k];
+  float m = INFINITY;
+  for (uint32_t r = 0; r < sg.order; ++r) m = fminf(m, qroots[sg.vec_offset + r]);
+  st[sg.index].frozen_min_root = m;
+  st[sg.index].spectrum_rejected = rejected ? static_cast<uint32_t>(rejected[sg.index]) : 0u;
+}
+
+__global__ void copy_tiles_f64_kernel(const Segment* __restrict__ segs,
+                                      const double* __restrict__ src, double* __restrict__ dst,
+                                      const int* __restrict__ flag) {
+  if (flag && !flag[segs[blockIdx.y].index]) return;
+  SEGMENT_LOOP(sg, e) { dst[sg.offset + e] = src[sg.offset + e]; }
+}
+
+// The upper triangle (i <= j) of each active segment's order: x_ij and x_ji
+// both become ((x_ij + d_ij) + (x_ji + d_ji)) / 2, the diagonal x_ii + d_ii.
+__global__ void add_symmetrize_f64_kernel(const Segment* __restrict__ segs,
+                                          double* __restrict__ x, const float* __restrict__ d,
+                                          const int* __restrict__ active) {
+  if (active && !active[segs[blockIdx.y].index]) return;
+  SEGMENT_LOOP(sg, e) {
+    const uint32_t i = static_cast<uint32_t>(e % sg.ld), j = static_cast<uint32_t>(e / sg.ld);
+    if (i > j || j >= sg.order) continue;
+    const size_t ij = sg.offset + e, ji = sg.offset + j + static_cast<size_t>(i) * sg.ld;
+    if (i == j) {
+      x[ij] = x[ij] + static_cast<double>(d[ij]);
+    } else {
+      const double a = x[ij] + static_cast<double>(d[ij]);
+      const double b = x[ji] + static_cast<double>(d[ji]);
+      const double v = 0.5 * (a + b);
+      x[ij] = v;
+      x[ji] = v;
+    }
+  }
+}
+
+__global__ void sylvester_divide_kernel(const Segment* __restrict__ segs,
+                                        const float* __restrict__ in,
+                                        const float* __restrict__ roots, float* __restrict__ out,
+                                        const int* __restrict__ active) {
+  if (active && !active[segs[blockIdx.y].index]) return;
+  SEGMENT_LOOP(sg, e) {
+    const uint32_t i = static_cast<uint32_t>(e % sg.ld), j = static_cast<uint32_t>(e / sg.ld);
+    float v = 0.0f;
+    if (i < sg.order && j < sg.order)
+      v = in[sg.offset + e] / (roots[sg.vec_offset + i] + roots[sg.vec_offset + j]);
+    out[sg.offset + e] = v;
+  }
+}
+
+__global__ void scale_columns_kernel(const Segment* __restrict__ segs, const float* __restrict__ in,
+                                     const float* __restrict__ roots, float* __restrict__ out,
+                                     const int* __restrict__ active) {
+  if (active && !active[segs[blockIdx.y].index]) return;
+  SEGMENT_LOOP(sg, e) {
+    const uint32_t i = static_cast<uint32_t>(e % sg.ld), j = static_cast<uint32_t>(e / sg.ld);
+    out[sg.offset + e] =
+        i < sg.order && j < sg.order ? in[sg.offset + e] * roots[sg.vec_offset + j] : 0.0f;
+  }
+}
+
+__global__ void copy_words_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                  size_t n) {
+  for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+       e += static_cast<size_t>(gridDim.x) * blockDim.x)
+    dst[e] = src[e];
+}
+
+__global__ void widen_f64_kernel(const float* __restrict__ src, double* __restrict__ dst,
+                                 size_t n) {
+  for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+       e += static_cast<size_t>(gridDim.x) * blockDim.x)
+    dst[e] = static_cast<double>(src[e]);
+}
+
+template <typename T>
+__global__ void move_tiles_kernel(const Segment* __restrict__ from, const Segment* __restrict__ to,
+                                  const T* __restrict__ src, T* __restrict__ dst) {
+  const Segment f = from[blockIdx.y], t = to[blockIdx.y];
+  const size_t n = static_cast<size_t>(f.ld) * f.ld;
+  for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+       e += static_cast<size_t>(gridDim.x) * blockDim.x)
+    dst[t.offset + e] = src[f.offset + e];
+}
+
+__global__ void move_vectors_kernel(const Segment* __restrict__ from,
+                                    const Segment* __restrict__ to, const float* __restrict__ src,
+                                    float* __restrict__ dst) {
+  const Segment f = from[blockIdx.y], t = to[blockIdx.y];
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < f.ld; e += gridDim.x * blockDim.x)
+    dst[t.vec_offset + e] = src[f.vec_offset + e];
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t value) {
@@ -408,24 +576,113 @@ hipError_t launch_alpha(const float* sa, const float* sb, float* alpha, uint32_t
 }
 hipError_t launch_solve_state(const Segment* segs, const JacobiStatus* js,
                               const float* diagonal, const SolveLimits* limits, SolveState* st,
-                              int* active, uint32_t count, hipStream_t stream) {
+                              int* active, uint32_t count, bool allow_refine,
+                              hipStream_t stream) {
   if (!count) return hipSuccess;
   solve_state_kernel<<<(count + kT - 1) / kT, kT, 0, stream>>>(segs, js, diagonal, limits, st,
-                                                              active, count);
+                                                              active, count, allow_refine ? 1 : 0);
   return hipGetLastError();
 }
 hipError_t launch_basis_escalation(const Segment* segs, uint32_t count, const double* orth2,
-                                   double tolerance, const int* active, int* esc,
-                                   SolveState* st, hipStream_t stream) {
+                                   double tolerance, const SolveLimits* limits, const int* active,
+                                   int* esc, SolveState* st, hipStream_t stream) {
   if (!count) return hipSuccess;
   basis_escalation_kernel<<<(count + kT - 1) / kT, kT, 0, stream>>>(segs, count, orth2, tolerance,
-                                                                   active, esc, st);
+                                                                   limits, active, esc, st);
   return hipGetLastError();
 }
-hipError_t launch_convergence(SolveState* st, const double* norms, int* active, uint32_t count,
-                              uint32_t step, hipStream_t stream) {
+hipError_t launch_convergence(SolveState* st, const double* norms, int* active, int* save,
+                              uint32_t count, uint32_t step, hipStream_t stream) {
   if (!count) return hipSuccess;
-  convergence_kernel<<<(count + kT - 1) / kT, kT, 0, stream>>>(st, norms, active, count, step);
+  convergence_kernel<<<(count + kT - 1) / kT, kT, 0, stream>>>(st, norms, active, save, count,
+                                                              step);
+  return hipGetLastError();
+}
+hipError_t launch_record_orthogonality(const Segment* segs, uint32_t count, const double* orth2,
+                                       SolveState* st, hipStream_t stream) {
+  if (!count) return hipSuccess;
+  record_orthogonality_kernel<<<(count + kT - 1) / kT, kT, 0, stream>>>(segs, count, orth2, st);
+  return hipGetLastError();
+}
+hipError_t launch_reject_spectrum(const Segment* segs, uint32_t count, const int* rejected,
+                                  SolveState* st, int* active, hipStream_t stream) {
+  if (!count) return hipSuccess;
+  reject_spectrum_kernel<<<(count + kT - 1) / kT, kT, 0, stream>>>(segs, count, rejected, st,
+                                                                  active);
+  return hipGetLastError();
+}
+hipError_t launch_frozen_roots(const Segment* segs, uint32_t count, const float* qroots,
+                               const int* rejected, SolveState* st, hipStream_t stream) {
+  if (!count) return hipSuccess;
+  frozen_roots_kernel<<<(count + kT - 1) / kT, kT, 0, stream>>>(segs, count, qroots, rejected, st);
+  return hipGetLastError();
+}
+hipError_t launch_copy_tiles_f64(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                 const double* src, double* dst, const int* flag,
+                                 hipStream_t stream) {
+  if (!count) return hipSuccess;
+  copy_tiles_f64_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(segs, src, dst, flag);
+  return hipGetLastError();
+}
+hipError_t launch_add_symmetrize_f64(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                     double* x, const float* d, const int* active,
+                                     hipStream_t stream) {
+  if (!count) return hipSuccess;
+  add_symmetrize_f64_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(segs, x, d,
+                                                                               active);
+  return hipGetLastError();
+}
+hipError_t launch_sylvester_divide(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                   const float* in, const float* roots, float* out,
+                                   const int* active, hipStream_t stream) {
+  if (!count) return hipSuccess;
+  sylvester_divide_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(segs, in, roots, out,
+                                                                            active);
+  return hipGetLastError();
+}
+hipError_t launch_scale_columns(const Segment* segs, uint32_t count, uint32_t max_ld,
+                                const float* in, const float* roots, float* out,
+                                const int* active, hipStream_t stream) {
+  if (!count) return hipSuccess;
+  scale_columns_kernel<<<dim3(seg_grid_x(max_ld), count), kT, 0, stream>>>(segs, in, roots, out,
+                                                                         active);
+  return hipGetLastError();
+}
+hipError_t launch_copy_words(const void* src, void* dst, size_t words, hipStream_t stream) {
+  if (!words) return hipSuccess;
+  size_t g = (words + kT - 1) / kT;
+  if (g > 16384) g = 16384;
+  copy_words_kernel<<<static_cast<int>(g), kT, 0, stream>>>(static_cast<const uint32_t*>(src),
+                                                           static_cast<uint32_t*>(dst), words);
+  return hipGetLastError();
+}
+hipError_t launch_widen_f64(const float* src, double* dst, size_t n, hipStream_t stream) {
+  if (!n) return hipSuccess;
+  size_t g = (n + kT - 1) / kT;
+  if (g > 16384) g = 16384;
+  widen_f64_kernel<<<static_cast<int>(g), kT, 0, stream>>>(src, dst, n);
+  return hipGetLastError();
+}
+hipError_t launch_move_tiles(const Segment* from, const Segment* to, uint32_t count,
+                             uint32_t max_ld, const void* src, void* dst, size_t elem_bytes,
+                             hipStream_t stream) {
+  if (!count) return hipSuccess;
+  const dim3 grid(seg_grid_x(max_ld), count);
+  if (elem_bytes == 8)
+    move_tiles_kernel<double><<<grid, kT, 0, stream>>>(from, to, static_cast<const double*>(src),
+                                                      static_cast<double*>(dst));
+  else if (elem_bytes == 4)
+    move_tiles_kernel<float><<<grid, kT, 0, stream>>>(from, to, static_cast<const float*>(src),
+                                                     static_cast<float*>(dst));
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+hipError_t launch_move_vectors(const Segment* from, const Segment* to, uint32_t count,
+                               uint32_t max_ld, const float* src, float* dst,
+                               hipStream_t stream) {
+  if (!count) return hipSuccess;
+  move_vectors_kernel<<<dim3((max_ld + kT - 1) / kT, count), kT, 0, stream>>>(from, to, src, dst);
   return hipGetLastError();
 }
 

@@ -47,9 +47,18 @@ PRECISION_NAMES = {
 PRECISION_BY_NAME = {v: k for k, v in PRECISION_NAMES.items()}
 
 SOLVE_STATUS = {0: "running", 1: "converged", 2: "budget_exhausted",
-                3: "diverged", 4: "nonfinite", 5: "invalid_spectrum"}
+                3: "diverged", 4: "nonfinite", 5: "invalid_spectrum", 6: "stagnated",
+                7: "unresolved"}
 JACOBI_HISTORY = 32
-SOLVE_HISTORY = 20
+SOLVE_HISTORY = 65
+ABI_VERSION = 5
+# KernelPath / CorrectionMode / NativeStoppingPolicy / NativeDispatchPolicy /
+# RoundingMode (src/precision.hpp:22, :126-146), the reference's order.
+KERNEL_PATH_LEGACY, KERNEL_PATH_TILED = 0, 1
+CORRECTION_FP32_REORTHOGONALIZED, CORRECTION_NATIVE_QUANTIZED = 0, 1
+STOPPING_FIXED_BUDGET, STOPPING_STAGNATION = 0, 1
+DISPATCH_PER_STAGE, DISPATCH_FUSED = 0, 1
+ROUNDING_NEAREST, ROUNDING_STOCHASTIC = 0, 1
 OWN_STREAM = 2**64 - 1  # MSQ_OWN_STREAM
 MLP_INPUTS = 16
 MLP_HIDDEN = 64
@@ -61,13 +70,16 @@ NOALLOC_MAX_MATRICES = 4096
 # Every symbol include/msq_hip.h declares.
 EXPORTED_SYMBOLS = (
     "msq_last_error", "msq_abi_version", "msq_runtime_create", "msq_runtime_destroy",
-    "msq_runtime_set_stream", "msq_synchronize", "msq_runtime_set_graphs", "msq_alloc",
-    "msq_free",
+    "msq_runtime_set_stream", "msq_synchronize", "msq_runtime_set_graphs",
+    "msq_runtime_set_rounding", "msq_alloc", "msq_free",
     "msq_copy_to_device", "msq_copy_to_host", "msq_memset", "msq_storage_bytes",
-    "msq_scale_bytes", "msq_batch_layout", "msq_quantize", "msq_decode",
+    "msq_scale_bytes", "msq_spectrum_metadata_bytes", "msq_batch_layout", "msq_quantize",
+    "msq_quantize_stream", "msq_decode",
     "msq_quantize_and_decode", "msq_quantize_fp8_scaled", "msq_decode_fp8_scaled",
-    "msq_jacobi_batched", "msq_jacobi_batched_noalloc", "msq_reconstruct", "msq_residual_fp64", "msq_correction",
-    "msq_orthogonality", "msq_solve_batched", "msq_last_factors",
+    "msq_jacobi_batched", "msq_jacobi_batched_noalloc", "msq_reconstruct", "msq_residual_fp64",
+    "msq_correction", "msq_refine_native_staged",
+    "msq_orthogonality", "msq_solve_batched", "msq_solve_batched_ex", "msq_solve_options_init",
+    "msq_last_factors",
     "msq_matrix_features", "msq_mlp_forward", "msq_mlp_backward", "msq_mlp_adam",
     "msq_autotune", "msq_set_tile_table",
 )
@@ -94,7 +106,13 @@ class SolveOptions(ctypes.Structure):
     _fields_ = [("basis_precision", ctypes.c_int32), ("correction_precision", ctypes.c_int32),
                 ("jacobi", JacobiOptions), ("min_corrections", ctypes.c_uint32),
                 ("max_corrections", ctypes.c_uint32), ("residual_tolerance", ctypes.c_double),
-                ("forward_tolerance", ctypes.c_double)]
+                ("forward_tolerance", ctypes.c_double),
+                ("spectrum_precision", ctypes.c_int32), ("kernel_path", ctypes.c_int32),
+                ("correction_mode", ctypes.c_int32), ("stopping_policy", ctypes.c_int32),
+                ("dispatch_policy", ctypes.c_int32), ("stagnation_window", ctypes.c_uint32),
+                ("minimum_relative_improvement", ctypes.c_double),
+                ("basis_tolerance", ctypes.c_double), ("resolution", ctypes.c_float),
+                ("refine_unresolved", ctypes.c_uint32), ("certificate", ctypes.c_uint32)]
 
 
 class SolveReport(ctypes.Structure):
@@ -108,7 +126,19 @@ class SolveReport(ctypes.Structure):
                 ("off_history", ctypes.c_float * JACOBI_HISTORY),
                 ("correction_rate", ctypes.c_float),
                 ("basis_orthogonality", ctypes.c_float),
-                ("precision_escalations", ctypes.c_uint32)]
+                ("precision_escalations", ctypes.c_uint32),
+                ("basis_rejected", ctypes.c_uint32), ("spectrum_rejected", ctypes.c_uint32),
+                ("frozen_sylvester_inverse_norm", ctypes.c_double),
+                ("frozen_basis_orthogonality", ctypes.c_double),
+                ("certificate_usable", ctypes.c_uint32), ("unresolved", ctypes.c_uint32),
+                ("refined", ctypes.c_uint32), ("best_iteration", ctypes.c_uint32),
+                ("stagnation_steps", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("best_residual", ctypes.c_double)]
+
+
+class SolveOutputs(ctypes.Structure):
+    _fields_ = [("roots", ctypes.c_void_p), ("initial_roots", ctypes.c_void_p),
+                ("vectors", ctypes.c_void_p), ("root_eigenvalues", ctypes.c_void_p)]
 
 
 _p = ctypes.c_void_p
@@ -159,6 +189,13 @@ _sig("msq_correction", _i, _p, _i, _u32, _u32p, _p, _p, _p, _p)
 _sig("msq_orthogonality", _i, _p, _u32, _u32p, _p, _f64p)
 _sig("msq_solve_batched", _i, _p, _u32, _u32p, _p, _p, ctypes.POINTER(SolveOptions), _u32,
      ctypes.POINTER(SolveReport))
+_sig("msq_solve_batched_ex", _i, _p, _u32, _u32p, _p, ctypes.POINTER(SolveOutputs),
+     ctypes.POINTER(SolveOptions), _u32, ctypes.POINTER(SolveReport))
+_sig("msq_solve_options_init", None, ctypes.POINTER(SolveOptions))
+_sig("msq_refine_native_staged", _i, _p, _i, _u32, _u32p, _p, _p, _p, _p)
+_sig("msq_runtime_set_rounding", _i, _p, _i, _u32)
+_sig("msq_quantize_stream", _i, _p, _i, _p, _p, _p, _sz, _u32)
+_sig("msq_spectrum_metadata_bytes", _sz, _i, _sz)
 _sig("msq_last_factors", _i, _p, ctypes.POINTER(_p), ctypes.POINTER(_p))
 _sig("msq_matrix_features", _i, _p, _u32, _u32p, _p, _p)
 _sig("msq_mlp_forward", _i, _p, _p, _p, _u32, _p, _p, _p)
@@ -182,6 +219,10 @@ def storage_bytes(precision, elements):
 
 def scale_bytes(precision, elements):
     return lib.msq_scale_bytes(precision, elements)
+
+
+def spectrum_metadata_bytes(precision, elements):
+    return lib.msq_spectrum_metadata_bytes(precision, elements)
 
 
 def _addr(x):
@@ -247,11 +288,27 @@ def jacobi_options(fixed_sweeps=0, max_sweeps=30, off_tol=1e-7, off_tol_lambda=0
 
 
 def solve_options(basis=FP32, correction=FP32, jacobi=None, min_corrections=0,
-                  max_corrections=12, residual_tolerance=1e-12, forward_tolerance=0.0):
-    if jacobi is None:
-        jacobi = jacobi_options()
-    return SolveOptions(basis, correction, jacobi, min_corrections, max_corrections,
-                        residual_tolerance, forward_tolerance)
+                  max_corrections=12, residual_tolerance=1e-12, forward_tolerance=0.0,
+                  **fields):
+    """An msq_solve_options record: msq_solve_options_init's reference
+    defaults, then these values; `fields` sets any other field by name
+    (spectrum_precision, kernel_path, correction_mode, stopping_policy,
+    dispatch_policy, stagnation_window, minimum_relative_improvement,
+    basis_tolerance, resolution, refine_unresolved, certificate)."""
+    o = SolveOptions()
+    lib.msq_solve_options_init(ctypes.byref(o))
+    o.basis_precision = basis
+    o.correction_precision = correction
+    o.jacobi = jacobi if jacobi is not None else jacobi_options()
+    o.min_corrections = min_corrections
+    o.max_corrections = max_corrections
+    o.residual_tolerance = residual_tolerance
+    o.forward_tolerance = forward_tolerance
+    for k, v in fields.items():
+        if k not in dict(SolveOptions._fields_):
+            raise TypeError(f"no solve option {k}")
+        setattr(o, k, v)
+    return o
 
 
 def _struct_dtype(struct):
@@ -292,6 +349,15 @@ def report_dict(r):
         "correction_rate": float(get("correction_rate")),
         "basis_orthogonality": float(get("basis_orthogonality")),
         "precision_escalations": int(get("precision_escalations")),
+        "basis_rejected": int(get("basis_rejected")),
+        "spectrum_rejected": int(get("spectrum_rejected")),
+        "frozen_sylvester_inverse_norm": float(get("frozen_sylvester_inverse_norm")),
+        "frozen_basis_orthogonality": float(get("frozen_basis_orthogonality")),
+        "certificate_usable": int(get("certificate_usable")),
+        "unresolved": int(get("unresolved")), "refined": int(get("refined")),
+        "best_iteration": int(get("best_iteration")),
+        "stagnation_steps": int(get("stagnation_steps")),
+        "best_residual": float(get("best_residual")),
     }
 
 
@@ -407,7 +473,17 @@ class Runtime:
         """Record each batch signature's solve in a HIP graph and replay it."""
         check(lib.msq_runtime_set_graphs(self.handle, 1 if enable else 0))
 
+    def set_rounding(self, mode=ROUNDING_NEAREST, seed=0):
+        """MetalRuntime's rounding mode and seed (nearest or stochastic)."""
+        check(lib.msq_runtime_set_rounding(self.handle, mode, seed))
+
     # codecs -----------------------------------------------------------
+    def quantize_stream(self, precision, src, values, scales=None, stream=0):
+        """quantize with the quantizeAndDecode rounding stream; the log-spectrum
+        precisions take their metadata plane as `scales`."""
+        check(lib.msq_quantize_stream(self.handle, precision, _addr(src), _addr(values),
+                                      _addr(scales), src.numel(), stream))
+
     def quantize(self, precision, src, values, scales=None):
         check(lib.msq_quantize(self.handle, precision, _addr(src), _addr(values),
                                _addr(scales), src.numel()))
@@ -464,21 +540,32 @@ class Runtime:
         check(lib.msq_correction(self.handle, precision, len(orders), _u32_array(orders),
                                  _addr(vectors), _addr(roots), _addr(residual_f32), _addr(out)))
 
+    def refine_native_staged(self, precision, orders, vectors, roots, residual_f32, out):
+        """refineNativeStaged: one native-quantized correction, decoded into out."""
+        check(lib.msq_refine_native_staged(self.handle, precision, len(orders),
+                                           _u32_array(orders), _addr(vectors), _addr(roots),
+                                           _addr(residual_f32), _addr(out)))
+
     def orthogonality(self, orders, vectors):
         res = (ctypes.c_double * len(orders))()
         check(lib.msq_orthogonality(self.handle, len(orders), _u32_array(orders),
                                     _addr(vectors), res))
         return np.array(res[:])
 
-    def solve_batched(self, orders, matrices, roots, options, reports=True):
+    def solve_batched(self, orders, matrices, roots, options, reports=True,
+                      initial_roots=None, vectors=None, root_eigenvalues=None):
+        """Reports: a ``Reports`` sequence (one dict-like entry per matrix).
+        initial_roots / vectors / root_eigenvalues: optional device outputs
+        (msq_solve_batched_ex)."""
         opts = options if isinstance(options, (list, tuple)) else [options]
         arr = (SolveOptions * len(opts))(*opts)
-        """Reports: a ``Reports`` sequence (one dict-like entry per matrix)."""
         recs = np.zeros(len(orders), dtype=REPORT_DTYPE) if reports else None
-        check(lib.msq_solve_batched(self.handle, len(orders), _u32_array(orders),
-                                    _addr(matrices), _addr(roots), arr, len(opts),
-                                    recs.ctypes.data_as(ctypes.POINTER(SolveReport))
-                                    if reports else None))
+        outs = SolveOutputs(_addr(roots), _addr(initial_roots), _addr(vectors),
+                            _addr(root_eigenvalues))
+        check(lib.msq_solve_batched_ex(self.handle, len(orders), _u32_array(orders),
+                                       _addr(matrices), ctypes.byref(outs), arr, len(opts),
+                                       recs.ctypes.data_as(ctypes.POINTER(SolveReport))
+                                       if reports else None))
         return Reports(recs) if reports else None
 
     def last_factors(self):

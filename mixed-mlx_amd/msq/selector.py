@@ -57,7 +57,7 @@ assert len(TIERS) == capi.MLP_OUTPUTS
 def tier_options(k, max_corrections=12):
     name, tier, rate = TIERS[k]
     if tier is None:
-        return MS.plain_fp32_options()
+        return MS.fallback_options()
     return MS.tier_options(tier, rate=rate, max_corrections=max_corrections)
 
 

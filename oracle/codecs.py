@@ -103,3 +103,55 @@ def fp8_scaled(values):
 def decode_fp8_scaled(codes, sigma, n):
     vals = decode("fp8_e4m3fn", codes, None, n)
     return (vals / np.float32(sigma)).astype(np.float32)
+
+
+_lib.oracle_stochastic_hash.argtypes = [ctypes.c_uint32] * 3
+_lib.oracle_stochastic_hash.restype = ctypes.c_uint32
+_lib.oracle_encode_stochastic.argtypes = [ctypes.c_int, _fp, _u8, _u8, _sz, ctypes.c_uint32,
+                                          ctypes.c_uint32]
+_lib.oracle_encode_stochastic.restype = ctypes.c_int
+_lib.oracle_encode_log.argtypes = [ctypes.c_int, _fp, _u8, _fp, _sz, ctypes.c_int,
+                                   ctypes.c_uint32, ctypes.c_uint32]
+_lib.oracle_encode_log.restype = None
+_lib.oracle_decode_log.argtypes = [ctypes.c_int, _u8, _fp, _fp, _sz]
+_lib.oracle_decode_log.restype = None
+
+
+def stochastic_hash(index, seed, stream):
+    return int(_lib.oracle_stochastic_hash(index, seed, stream))
+
+
+def encode_stochastic(fmt: str, values, seed, stream):
+    """Stochastically rounded codes (and MX scales or None)."""
+    x, p = _f(values)
+    n = x.size
+    fid = DECODE_ID[fmt]
+    nbytes = {1: 2 * n, 6: (n + 1) // 2, 7: (n + 1) // 2}.get(fid, n)
+    out = np.zeros(max(nbytes, 1), np.uint8)
+    sc = np.zeros(max((n + 31) // 32, 1), np.uint8)
+    rc = _lib.oracle_encode_stochastic(fid, p, out.ctypes.data_as(_u8), sc.ctypes.data_as(_u8),
+                                       n, seed, stream)
+    if rc != 0:
+        raise ValueError(fmt)
+    codes = out[:nbytes].view(np.uint16) if fid == 1 else out[:nbytes]
+    return codes, (sc if fmt.startswith("mx") else None)
+
+
+def encode_log(e4m3: bool, values, stochastic=False, seed=0, stream=0):
+    """Log-affine spectrum codes and metadata (min, span per block of 8)."""
+    x, p = _f(values)
+    n = x.size
+    codes = np.zeros(n if e4m3 else (n + 1) // 2, np.uint8)
+    meta = np.zeros(2 * ((n + 7) // 8), np.float32)
+    _lib.oracle_encode_log(1 if e4m3 else 0, p, codes.ctypes.data_as(_u8),
+                           meta.ctypes.data_as(_fp), n, 1 if stochastic else 0, seed, stream)
+    return codes, meta
+
+
+def decode_log(e4m3: bool, codes, metadata, n):
+    out = np.zeros(n, np.float32)
+    c = np.ascontiguousarray(codes, np.uint8)
+    m = np.ascontiguousarray(metadata, np.float32)
+    _lib.oracle_decode_log(1 if e4m3 else 0, c.ctypes.data_as(_u8), m.ctypes.data_as(_fp),
+                           out.ctypes.data_as(_fp), n)
+    return out
