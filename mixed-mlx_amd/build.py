@@ -66,21 +66,47 @@ def build(jobs: int = 4) -> pathlib.Path:
 
 
 CLI = HERE / "msq" / "msq_cli"
-CLI_SOURCES = ["hip_runtime.cpp", "msq_cli.cpp"]
+MIRROR = HERE / "msq" / "libmsq_mirror.so"
+MIRROR_SOURCES = ["hip_runtime.cpp", "host_reference.cpp"]
+CLI_SOURCES = ["msq_cli.cpp"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fPIC"]
 
 
-def build_cli() -> pathlib.Path:
-    """The C++ mirror of the solver entry points (hip_runtime.cpp) and its
-    command-line driver: host code over the C ABI, built with g++."""
-    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
-    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Wextra",
-           *[str(CSRC / s) for s in CLI_SOURCES], "-o", str(CLI) + ".tmp",
-           "-L", str(OUT.parent), "-lmsq_hip", "-Wl,-rpath,$ORIGIN",
-           f"-Wl,-rpath-link,{rocm}/lib"]
+def host_lapack() -> list[str]:
+    """Link arguments of the host's binary64 LAPACK/BLAS: the OpenBLAS build
+    that scipy ships (its symbols carry a scipy_ prefix)."""
+    import glob
+    import scipy
+    libs = pathlib.Path(scipy.__file__).resolve().parent.parent / "scipy.libs"
+    found = sorted(glob.glob(str(libs / "libscipy_openblas*.so")))
+    if not found:
+        raise RuntimeError(f"no OpenBLAS library under {libs}")
+    lib = pathlib.Path(found[0])
+    return ["-L", str(lib.parent), f"-l:{lib.name}", f"-Wl,-rpath,{lib.parent}"]
+
+
+def _run(cmd: list[str], what: str) -> None:
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("msq_cli build failed")
+        raise RuntimeError(f"{what} build failed")
+
+
+def build_cli(extra: list[str] | None = None) -> pathlib.Path:
+    """The C++ mirror of the solver entry points (libmsq_mirror.so:
+    hip_runtime.cpp + host_reference.cpp) and its command-line driver: host
+    code over the C ABI, built with g++ (-Werror).  `extra` adds compiler
+    flags (a sanitizer build passes its own)."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cxx = os.environ.get("CXX", "g++")
+    flags = HOST_FLAGS + list(extra or [])
+    _run([cxx, *flags, "-shared", *[str(CSRC / s) for s in MIRROR_SOURCES],
+          "-o", str(MIRROR) + ".tmp", "-L", str(OUT.parent), "-lmsq_hip", "-Wl,-rpath,$ORIGIN",
+          f"-Wl,-rpath-link,{rocm}/lib", *host_lapack()], "libmsq_mirror.so")
+    os.replace(str(MIRROR) + ".tmp", MIRROR)
+    _run([cxx, *flags, *[str(CSRC / s) for s in CLI_SOURCES], "-o", str(CLI) + ".tmp",
+          "-L", str(OUT.parent), "-lmsq_mirror", "-lmsq_hip", "-Wl,-rpath,$ORIGIN",
+          f"-Wl,-rpath-link,{rocm}/lib", *host_lapack()], "msq_cli")
     os.replace(str(CLI) + ".tmp", CLI)
     return CLI
 

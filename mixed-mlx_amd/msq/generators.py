@@ -196,3 +196,80 @@ def mixture(count=20000, config=4, order_range=(32, 512), max_log_cond=6.0,
         A, X = assemble(Q, lam)
         yield {"index": i, "order": n, "condition": cond, "kind": kind,
                "A": A, "X": X, "lam": lam, "seed": seed}
+
+
+# ----------------------------------------------------------- paper test sets
+PAPER_ORDERS = {"V": 4096, "VI": 4096, "VII": 1024}
+_PAPER_ORDINALS = {"V": 5, "VI": 6, "VII": 7}
+
+
+class _ScalarStream:
+    """One xorshift64* stream drawn value by value (uniform on [0, 1) with 53
+    bits; Box-Muller normals in pairs, cos first, the sin value kept)."""
+
+    def __init__(self, seed: int):
+        self.state = seed or 0x9E3779B97F4A7C15
+        self.spare = None
+
+    def uniform(self) -> float:
+        x = self.state
+        x ^= x >> 12
+        x ^= (x << 25) & 0xFFFFFFFFFFFFFFFF
+        x ^= x >> 27
+        self.state = x
+        return (((x * 2685821657736338717) & 0xFFFFFFFFFFFFFFFF) >> 11) * 2.0**-53
+
+    def normal(self) -> float:
+        if self.spare is not None:
+            out, self.spare = self.spare, None
+            return out
+        u1 = max(self.uniform(), np.finfo(np.float64).tiny)
+        u2 = self.uniform()
+        radius = np.sqrt(-2.0 * np.log(u1))
+        angle = 6.283185307179586476925286766559 * u2
+        self.spare = radius * np.sin(angle)
+        return radius * np.cos(angle)
+
+
+def paper_matrix(test_set: str, order: int | None = None) -> dict:
+    """The symmetric paper test sets of the reference
+    (preparePaperMatrix, src/accelerate_backend.cpp:690-801):
+
+    * V / VI: a symmetric matrix with uniform [0, 1) (V) or standard normal
+      (VI) entries drawn down each column from the diagonal (upper triangle
+      mirrored), from the stream seeded 0x243f6a8885a308d3 ^
+      (0x9e3779b97f4a7c15 * ordinal) (ordinal 5 or 6), then shifted by
+      max(0, max(1, 0.01 ||A||_inf) - lambda_min) on the diagonal;
+    * VII: the Hilbert matrix 1 / (i + j + 1) plus 1e-6 I.
+
+    Returns A (column-major n x n as a numpy array), its condition number,
+    the diagonal shift, the seed and the labels the reference records."""
+    if test_set not in PAPER_ORDERS:
+        raise ValueError("symmetric paper test sets are V, VI and VII")
+    n = int(order or PAPER_ORDERS[test_set])
+    out = {"paper_test_set": test_set, "order": n, "seed": 0}
+    if test_set == "VII":
+        i = np.arange(n, dtype=np.float64)
+        A = 1.0 / (i[:, None] + i[None, :] + 1.0)
+        A[np.diag_indices(n)] += 1.0e-6
+        lam = np.linalg.eigvalsh(A)
+        out.update(A=A, shift=1.0e-6, condition=lam[-1] / lam[0], family="hilbert_shifted",
+                   shift_rule="fixed_hilbert_tau")
+        return out
+    seed = (0x243F6A8885A308D3 ^ ((0x9E3779B97F4A7C15 * _PAPER_ORDINALS[test_set])
+                                  & 0xFFFFFFFFFFFFFFFF))
+    st = _ScalarStream(seed)
+    draw = st.uniform if test_set == "V" else st.normal
+    A = np.empty((n, n))
+    for j in range(n):
+        col = np.array([draw() for _ in range(n - j)])
+        A[j:, j] = col
+        A[j, j:] = col
+    separation = max(1.0, 0.01 * np.abs(A).sum(axis=1).max())
+    lam = np.linalg.eigvalsh(A)
+    shift = max(0.0, separation - lam[0])
+    A[np.diag_indices(n)] += shift
+    out.update(A=A, shift=shift, condition=(lam[-1] + shift) / (lam[0] + shift), seed=seed,
+               family="uniform_random" if test_set == "V" else "normal_random",
+               shift_rule="set_min_real_part_to_max_1_or_0.01_inf_norm")
+    return out

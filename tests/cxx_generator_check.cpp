@@ -1,9 +1,9 @@
-// Prints makeSpdMatrix / makeSpdRoot of the C++ mirror (hip_runtime.cpp) for
-// the CPU test that compares them with the oracle's restatement.
+// Prints makeSpdMatrix / makeSpdRoot of the C++ mirror (host_reference.cpp)
+// for the CPU test that compares them with the oracle's restatement.
 #include <cstdio>
 #include <cstdlib>
 
-#include "hip_runtime.hpp"
+#include "host_reference.hpp"
 
 int main(int argc, char** argv) {
   if (argc != 3) return 2;
