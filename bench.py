@@ -31,13 +31,26 @@ cond up to 1e6, power law k^-alpha with alpha in [0.5, 3], four clusters):
 the same paths, with the pass rate (<= 1e-6 against the known root) and the
 fallback rate of each; no roofline leg.
 
-Also reported: the roofline of the dominant kernel (batched_jacobi_kernel<256>
-on 256 of the batch's matrices of orders 241..256, one per CU, with the
-reference's fixed 8 sweeps, run on its own and timed with HIP events on the
-library's stream; ``--roofline-only`` runs just these launches, for the
-profiler) and a CPU baseline: the reference's CPU plain path (1_baseline/spd.cpp:348-430,
-LAPACK ssyevd + BLAS, restated in oracle/solver.py) on a bounded sample of
-the same matrices, one core.
+``value`` counts the matrices within 1e-6 of their FP64 root (every matrix of
+the batch when all pass), per second of the low-bit path.
+
+Also reported:
+  * the roofline of the dominant kernel (batched_jacobi_kernel<256> on 256 of
+    the batch's matrices of orders 241..256, one per CU, with the reference's
+    fixed 8 sweeps, run on its own and timed with HIP events on the library's
+    stream; ``--roofline-only`` runs just these launches, for the profiler).
+    Its HBM traffic comes from the rocprofv3 PMC summary in profiles/, used
+    only when that summary was taken on the kernel source being run (the
+    SHA-256 of kernels_jacobi.hip recorded there);
+  * the whole-batch roofline fraction sum_i t_roof,i / t_measured of each
+    path, t_roof,i = 9 S n^2 (n - 1) / P_fp32 + (2 + 8 k) n^3 / P_tier +
+    2 k n^3 / P_fp64 with the sweeps S and corrections k each matrix ran;
+  * a control: FP32 products with the low-bit tier's Jacobi stop rule, which
+    separates the stop rule's gain from the low-bit products';
+  * a CPU baseline: the reference's CPU plain path (1_baseline/spd.cpp:348-430,
+    LAPACK ssyevd + BLAS, restated in C++ in oracle/cpu_baseline.cpp) over the
+    whole batch, one matrix per host core with single-threaded BLAS, timed in
+    whole passes for about --cpu-budget seconds.
 """
 from __future__ import annotations
 
@@ -53,10 +66,43 @@ sys.path[:0] = [str(ROOT), str(ROOT / "mixed-mlx_amd")]
 
 METRIC = "SPD sqrts/sec at ≤1e-6 rel err vs FP64: low-bit+refine vs plain FP32, 1 GPU"
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per launch of the roofline kernel on the default workload: two
-# rocprofv3 --pmc runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a
-# 256 MiB copy gives), profiles/jacobi_n256_hbm_traffic_pmc.md.
-ROOFLINE_TRAFFIC_BYTES = 1.213e10
+# HBM traffic of the roofline kernel: profiles/jacobi_n256_hbm_traffic_pmc.md
+# records the bytes per launch and the SHA-256 of the kernel source it was
+# measured on.
+TRAFFIC_PROFILE = ROOT / "profiles" / "jacobi_n256_hbm_traffic_pmc.md"
+JACOBI_SOURCE = ROOT / "mixed-mlx_amd" / "csrc" / "kernels_jacobi.hip"
+# MI355X peaks (TFLOP/s): FP32 vector and matrix, FP64, and the dense MFMA
+# rates of the product formats.
+P_FP32, P_FP64 = 157.3, 78.6
+TIER_PEAKS = {"fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0, "mxfp4": 10100.0}
+
+
+def measured_traffic():
+    """(bytes per launch, source hash) from the PMC summary when it was taken
+    on the kernel source of this tree, else (None, hash)."""
+    import hashlib
+    digest = hashlib.sha256(JACOBI_SOURCE.read_bytes()).hexdigest()
+    try:
+        fields = dict(line.split(":", 1) for line in TRAFFIC_PROFILE.read_text().splitlines()
+                      if line.startswith(("source_sha256:", "traffic_bytes_per_launch:")))
+    except (OSError, ValueError):
+        return None, digest
+    if fields.get("source_sha256", "").strip() != digest:
+        return None, digest
+    return float(fields["traffic_bytes_per_launch"]), digest
+
+
+def batch_roofline(orders, sweeps, corrections, tiers, seconds):
+    """sum_i t_roof,i / t_measured: per matrix, the Jacobi at the FP32 peak,
+    reconstruction and corrections at the tier's product peak, the binary64
+    residuals at the FP64 peak."""
+    t = 0.0
+    for n, S, k, tier in zip(orders, sweeps, corrections, tiers):
+        n = float(n)
+        t += (9.0 * S * n * n * (n - 1) / (P_FP32 * 1e12)
+              + (2.0 + 8.0 * k) * n ** 3 / (TIER_PEAKS[tier] * 1e12)
+              + 2.0 * k * n ** 3 / (P_FP64 * 1e12))
+    return {"sum_t_roof_s": t, "t_measured_s": seconds, "frac": t / seconds}
 
 
 def log(*a):
@@ -102,9 +148,11 @@ def roofline_leg(args, mats, orders, rt, stream, torch, np, capi, dev):
     sweeps = [r.sweeps for r in reps]
     flops = jacobi_flops(sub_orders, sweeps)
     achieved = flops / jt / 1e12
-    return {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+    traffic, digest = measured_traffic()
+    return {"bound": "fp32 valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-            "traffic": ROOFLINE_TRAFFIC_BYTES if len(mats) == 4096 else None,
+            "traffic": traffic if len(mats) == 4096 else None,
+            "kernel_source_sha256": digest,
             "kernel": (f"batched_jacobi_kernel<256> ({len(big)} matrices of orders 241..256, "
                        f"one per CU, {ROOFLINE_SWEEPS} fixed sweeps; "
                        f"{flops / 1e12:.3f} TFLOP per launch at 9 S n^2 (n - 1), "
@@ -112,27 +160,31 @@ def roofline_leg(args, mats, orders, rt, stream, torch, np, capi, dev):
 
 
 def cpu_baseline(mats, budget_s):
+    """Whole passes over the batch with one matrix per host thread until about
+    budget_s seconds are spent (at least one pass); matrices/s over all passes,
+    and the last pass checked against the known roots."""
     import numpy as np
-    from threadpoolctl import threadpool_limits
 
-    from oracle import solver as S
+    from oracle import cpu_baseline as CB
 
-    done = 0
-    orders = []
-    t0 = time.perf_counter()
-    with threadpool_limits(limits=1):
-        for m in mats:
-            S.accelerate_mixed(m["A"], None, max_corrections=12, tolerance=1e-12)
-            done += 1
-            orders.append(m["order"])
-            if time.perf_counter() - t0 >= budget_s:
-                break
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "matrices/s", "cores": 1, "kind": "port",
-            "sample": (f"first {done} matrices of the batch (mean order "
-                       f"{float(np.mean(orders)):.0f}), LAPACK ssyevd + BLAS via scipy, "
-                       f"1 thread, {dt:.1f} s; corrections to residual 1e-12 as the "
-                       "reference CPU path")}
+    threads = CB.host_threads()
+    batch = CB.Batch([m["A"] for m in mats])
+    passes, total = 0, 0.0
+    while passes == 0 or total < budget_s:
+        secs, corr, res = CB.solve_batch(batch, threads, max_corrections=12, tolerance=1e-12)
+        passes += 1
+        total += secs
+    errs = np.array([np.linalg.norm(batch.root(i) - m["X"]) / np.linalg.norm(m["X"])
+                     for i, m in enumerate(mats)])
+    return {"value": round(passes * len(mats) / total, 2), "unit": "matrices/s",
+            "cores": threads, "kind": "port",
+            "sample": (f"the whole batch ({len(mats)} matrices) x {passes} passes in "
+                       f"{total:.1f} s: the reference CPU plain path (ssyevd, sgemm, dgemm "
+                       f"residual, FP32 corrections to residual 1e-12, at most 12) in C++, one "
+                       f"matrix per thread, {threads} threads, single-threaded BLAS"),
+            "within_1e-6": int((errs <= 1e-6).sum()), "max_rel_error": float(errs.max()),
+            "mean_corrections": float(corr.mean()), "failed": int((corr < 0).sum()),
+            "nproc": os.cpu_count(), "cpu_model": CB.cpu_model(), "blas": CB.blas_config()}
 
 
 # MI355X dense peaks (TFLOP/s) of the product types, and HBM bandwidth.
@@ -402,12 +454,41 @@ def main():
         dt_fix, (rep_fix, fb_fix) = timed(fixed_tier)
         err_fix, pass_fix = check_all(X, f"fixed {args.tier} tier")
         fixed = {"tier": args.tier, "correction_rate": args.rate,
-                 "matrices_per_s": round(len(mats) / dt_fix, 2),
+                 "matrices_per_s": round(pass_fix / dt_fix, 2),
                  "ms_per_step": round(dt_fix * 1e3, 3), "max_rel_error": err_fix,
                  "within_1e-6": pass_fix, "fallbacks": len(fb_fix),
                  "mean_sweeps": float(np.mean(rep_fix.sweeps))}
         log(f"fixed {args.tier} tier: {dt_fix * 1e3:.1f} ms/step, "
             f"{len(mats) / dt_fix:.0f} matrices/s, fallback {len(fb_fix)}/{len(mats)}")
+    # Control: FP32 products with the same Jacobi stop rule as the fixed tier,
+    # so the low-bit products' share of the gain is measured on its own.
+    control_opts = MS.tier_options("fp32", rate=args.rate, max_corrections=12)
+    dt_ctl, (rep_ctl, fb_ctl) = timed(
+        lambda: MS.solve_with_fallback(rt, orders, A, X, control_opts))
+    err_ctl, pass_ctl = check_all(X, f"fp32 products, rate {args.rate}")
+    control = {"products": "fp32", "correction_rate": args.rate,
+               "matrices_per_s": round(pass_ctl / dt_ctl, 2),
+               "ms_per_step": round(dt_ctl * 1e3, 3), "max_rel_error": err_ctl,
+               "within_1e-6": pass_ctl, "fallbacks": len(fb_ctl),
+               "mean_sweeps": float(np.mean(rep_ctl.sweeps)),
+               "mean_corrections": float(np.mean(rep_ctl.corrections))}
+    log(f"fp32 products, rate {args.rate}: {dt_ctl * 1e3:.1f} ms/step")
+
+    # Whole-batch roofline fraction of each path with the S and k each matrix
+    # ran (a matrix re-solved on the plain path counts with that solve).
+    if use_selector:
+        low_tiers = [SEL.TIERS[k][1] or "fp32" for k in chosen[0]]
+    else:
+        low_tiers = [args.tier] * len(mats)
+    for i in fallback:
+        low_tiers[i] = "fp32"
+    batch_roof = {
+        "plain_fp32": batch_roofline(orders, rep_plain.sweeps, rep_plain.corrections,
+                                     ["fp32"] * len(mats), dt_plain),
+        "low_bit": batch_roofline(orders, rep_low.sweeps, rep_low.corrections, low_tiers,
+                                  dt_low),
+    }
+    log("batch roofline", batch_roof)
 
     roofline = None
     if args.workload == "uneven":
@@ -420,7 +501,7 @@ def main():
 
     line = {
         "metric": METRIC,
-        "value": round(len(mats) / dt_low, 2),
+        "value": round(pass_low / dt_low, 2),
         "unit": "matrices/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -435,11 +516,12 @@ def main():
                    "count": len(mats), "tier": args.tier, "correction_rate": args.rate,
                    "hip_graphs": not args.no_graphs,
                    "tile_table": tile_table.tolist() if tile_table is not None else None},
-        "plain_fp32": {"matrices_per_s": round(len(mats) / dt_plain, 2),
+        "plain_fp32": {"matrices_per_s": round(pass_plain / dt_plain, 2),
                        "ms_per_step": round(dt_plain * 1e3, 3),
                        "max_rel_error": err_plain, "converged": ok_plain,
                        "within_1e-6": pass_plain},
-        "low_bit": {"matrices_per_s": round(len(mats) / dt_low, 2),
+        "low_bit": {"matrices_per_s": round(pass_low / dt_low, 2),
+                    "all_matrices_per_s": round(len(mats) / dt_low, 2),
                     "selector": ({"tier_counts": dict(zip(
                         [t[0] for t in SEL.TIERS],
                         np.bincount(chosen[0], minlength=len(SEL.TIERS)).tolist()))}
@@ -448,8 +530,10 @@ def main():
                     "fallbacks": len(fallback),
                     "mean_sweeps": float(np.mean(rep_low.sweeps)),
                     "mean_corrections": float(np.mean(rep_low.corrections))},
-        "speedup_vs_plain_fp32": round(dt_plain / dt_low, 3),
+        "speedup_vs_plain_fp32": round((pass_low / dt_low) / max(pass_plain / dt_plain, 1e-30), 3),
         "fixed_tier": fixed,
+        "fp32_products_same_stop": control,
+        "batch_roofline": batch_roof,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
