@@ -101,6 +101,10 @@ struct msq_runtime {
   // RoundingMode and seed of the reference's runtime (msq_runtime_set_rounding).
   int rounding_mode = MSQ_ROUNDING_NEAREST;
   uint32_t rounding_seed = 0;
+  // Job counter of the persistent Jacobi launch (device memory, allocated
+  // with the runtime) and the device's CU count (its grid).
+  uint32_t* jacobi_counter = nullptr;
+  int cu_count = 0;
 };
 
 namespace {
@@ -563,42 +567,79 @@ struct JacobiClass {
 
 // The batched launches of `classes` (jobs already on the device): device work
 // only, no host synchronisation and no allocation (capturable in a graph).
-// Order classes run concurrently on the runtime's lanes: a launch lasts as
-// long as its slowest matrix, and the CUs its faster matrices free take work
-// from the other classes.  Each lane is in order, so the classes are dealt
-// longest first to the lane with the least work so far (work: sum of n^3 over
-// the class), which keeps the lanes' totals level.
+// Orders 129..256 (padded 144..256, whose workgroups each fill a CU) run as
+// one persistent launch over their jobs, largest order first: the classes
+// lie in djobs largest first and contiguous (plan_jacobi and the
+// allocation-free plan both lay them out so).  The smaller orders run one
+// launch per class.  The launches run concurrently on the runtime's lanes,
+// dealt longest first to the lane with the least work so far (work: sum of
+// n^3), which keeps the lanes' totals level.
 int enqueue_jacobi_classes(msq_runtime* rt, const JacobiClass* cls, int classes,
                            const msq::JacobiJob* djobs, const double* a64, float* V, float* diag,
                            float* roots, msq::JacobiStatus* jstatus) {
   if (classes <= 0) return MSQ_OK;
   if (classes > kMaxJacobiClasses)
     return set_error(MSQ_ERROR_INVALID_ARGUMENT, "%d Jacobi classes", classes);
-  const int used = classes > 1 ? std::min(classes, kLanes) : 0;
+  // Launch list: the persistent family as one entry (order 0), then the
+  // smaller classes.
+  struct Launch {
+    size_t first;
+    uint32_t count, order;
+    double cost;
+  };
+  Launch launches[kMaxJacobiClasses + 1];
+  int nl = 0;
+  size_t pfirst = 0, pend = 0;
+  uint32_t pcount = 0;
+  double pcost = 0.0;
+  bool contiguous = true;
+  for (int c = 0; c < classes; ++c) {
+    if (cls[c].order < msq::batched_jacobi_padded_order(msq::kPersistentMinOrder)) continue;
+    if (pcount == 0) {
+      pfirst = cls[c].first;
+    } else if (cls[c].first != pend) {
+      contiguous = false;
+    }
+    pend = cls[c].first + cls[c].count;
+    pcount += cls[c].count;
+    pcost += cls[c].cost;
+  }
+  const bool persistent = pcount > 0 && contiguous && rt->jacobi_counter && rt->cu_count > 0;
+  if (persistent) launches[nl++] = {pfirst, pcount, 0u, pcost};
+  for (int c = 0; c < classes; ++c)
+    if (!persistent ||
+        cls[c].order < msq::batched_jacobi_padded_order(msq::kPersistentMinOrder))
+      launches[nl++] = {cls[c].first, cls[c].count, cls[c].order, cls[c].cost};
+  const int used = nl > 1 ? std::min(nl, kLanes) : 0;
   if (used) {
     HIP_TRY(hipEventRecord(rt->fork, rt->stream));
     for (int l = 0; l < used; ++l) HIP_TRY(hipStreamWaitEvent(rt->lanes[l], rt->fork, 0));
   }
   // Stable order by decreasing cost (insertion sort on the stack).
-  int order[kMaxJacobiClasses];
-  for (int c = 0; c < classes; ++c) {
+  int order[kMaxJacobiClasses + 1];
+  for (int c = 0; c < nl; ++c) {
     int k = c;
-    while (k > 0 && cls[order[k - 1]].cost < cls[c].cost) {
+    while (k > 0 && launches[order[k - 1]].cost < launches[c].cost) {
       order[k] = order[k - 1];
       --k;
     }
     order[k] = c;
   }
   double load[kLanes] = {};
-  for (int q = 0; q < classes; ++q) {
-    const JacobiClass& c = cls[order[q]];
+  for (int q = 0; q < nl; ++q) {
+    const Launch& c = launches[order[q]];
     int lane = 0;
     for (int l = 1; l < used; ++l)
       if (load[l] < load[lane]) lane = l;
     if (used) load[lane] += c.cost;
     hipStream_t s = used ? rt->lanes[lane] : rt->stream;
-    HIP_TRY(msq::launch_batched_jacobi(djobs + c.first, c.count, c.order, a64, V, diag, roots,
-                                       jstatus, s));
+    if (c.order == 0) {
+      HIP_TRY(msq::launch_persistent_jacobi(djobs + c.first, c.count, rt->jacobi_counter,
+                                            rt->cu_count, a64, V, diag, roots, jstatus, s));
+    } else {
+      HIP_TRY(msq::launch_batched_jacobi(djobs + c.first, c.count, c.order, a64, V, diag, roots,
+                                         jstatus, s));
+    }
   }
   for (int l = 0; l < used; ++l) {
     HIP_TRY(hipEventRecord(rt->joins[l], rt->lanes[l]));
@@ -1599,6 +1640,9 @@ int msq_runtime_create(int device, msq_runtime** out) {
   if (e == hipSuccess) e = hipEventCreateWithFlags(&rt->fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&rt->graph_in, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&rt->graph_out, hipEventDisableTiming);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&rt->cu_count, hipDeviceAttributeMultiprocessorCount, device);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&rt->jacobi_counter), 256);
   // The allocation-free entry point's scratch address on this device (once
   // per device; the device is current here).
   if (e == hipSuccess) e = msq::noalloc_register(device);
@@ -1625,6 +1669,8 @@ int msq_runtime_destroy(msq_runtime* rt) {
   if (rt->large.base && (e = hipFree(rt->large.base)) != hipSuccess && status == MSQ_OK)
     status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
   if (rt->refine.base && (e = hipFree(rt->refine.base)) != hipSuccess && status == MSQ_OK)
+    status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
+  if (rt->jacobi_counter && (e = hipFree(rt->jacobi_counter)) != hipSuccess && status == MSQ_OK)
     status = set_error(MSQ_ERROR_HIP, "destroy: %s", hipGetErrorString(e));
   for (int l = 0; l < kLanes; ++l) {
     if (rt->lanes[l] && (e = hipStreamSynchronize(rt->lanes[l])) == hipSuccess)

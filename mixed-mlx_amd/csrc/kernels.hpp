@@ -84,6 +84,16 @@ size_t batched_jacobi_lds_bytes(uint32_t order);
 hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t order,
                                  const double* a64, float* vectors, float* diagonal,
                                  float* roots, JacobiStatus* status, hipStream_t stream);
+// Orders 129..256 in one persistent launch: `grid` workgroups (one per CU)
+// take jobs[0, count) in order from `counter` (one uint32 of device memory,
+// cleared here by a kernel before the launch; one launch at a time per
+// counter).  Every job's order must be in [kPersistentMinOrder, 256]; the
+// results equal launch_batched_jacobi's, bit for bit.
+constexpr uint32_t kPersistentMinOrder = 129;
+hipError_t launch_persistent_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t* counter,
+                                    int grid, const double* a64, float* vectors,
+                                    float* diagonal, float* roots, JacobiStatus* status,
+                                    hipStream_t stream);
 
 // ------------------------------------------------------ learned selector
 constexpr int kMlpInputs = 16;

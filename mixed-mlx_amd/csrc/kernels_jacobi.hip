@@ -225,16 +225,19 @@ __host__ __device__ constexpr bool use_super_blocks(int N) { return N >= 256; }
 __host__ __device__ constexpr int super_items(int N) {
   return (N / 4 - 1) * (N / 4 - 2) / 2 + (N / 4 - 2);
 }
-// Sums of one or two doubles per thread in a fixed order: thread values in
-// part0 / part1 (nt entries each), then a tree over thread indices (stride
+// Sums of one or two doubles per thread in a fixed order: the values of
+// threads 0 .. nt - 1 in part0 / part1 (nt entries each; threads beyond nt
+// take part in the barriers only), then a tree over thread indices (stride
 // halving from the power of two below nt).  The result does not depend on
 // which waves arrive first, so a matrix's norms and stop decisions are the
 // same in every launch and whatever else shares the batch.  Every thread
 // returns the sums; part0 / part1 are free again on return.
 __device__ __forceinline__ void block_sum2(double& a, double& b, double* part0, double* part1,
                                            int tid, int nt) {
-  part0[tid] = a;
-  part1[tid] = b;
+  if (tid < nt) {
+    part0[tid] = a;
+    part1[tid] = b;
+  }
   __syncthreads();
   int s = 1;
   while (2 * s < nt) s *= 2;
@@ -605,11 +608,48 @@ __device__ __forceinline__ uint32_t item_super(int r, int m2) {
 // down, and one value crosses to each neighbouring part of the row (adjacent
 // lanes).  The rotations of a group of rounds are buffered in LDS and applied
 // to V after the group.
+// LDS of one solve of padded order NP, carved from one byte array at
+// compile-time offsets (addresses fold into the ds instructions' offset
+// fields): packed A, slot records (two rounds x m), the rotation history (G
+// rounds x 8 parts x SP of (c, s') for V), the fixed-order reduction
+// partials (above N = 128 they live in the history, idle at both reduction
+// points, after the s_i = sqrt(a_ii) values the convergence test keeps at its
+// start; below, where LDS is plentiful, in an array of their own), and
+// scratch for the minimum and the stop flag.
 template <int NP, bool WIDE>
-__global__ __launch_bounds__(1024) void batched_jacobi_kernel(
-    const JacobiJob* __restrict__ jobs, const double* __restrict__ a64,
-    float* __restrict__ vectors, float* __restrict__ diagonal,
-    float* __restrict__ roots, JacobiStatus* __restrict__ status) {
+struct JacobiLds {
+  static constexpr int kS = slots_for_order(NP);
+  static constexpr int kPacked = NP * (NP + 1) / 2;
+  static constexpr int kHistEntries = group_for_order(NP) * 8 * padded_slots(kS);
+  static constexpr int kThreads = threads_for_order(NP, WIDE);
+  static constexpr bool kPartInHistory = NP > 128;
+  static constexpr int kSrootBytes = (4 * NP + 7) / 8 * 8;
+  static constexpr size_t kAp = 0;
+  static constexpr size_t kRec = kAp + static_cast<size_t>((kPacked + 3) & ~3) * sizeof(float);
+  static constexpr size_t kHist = kRec + 2 * static_cast<size_t>(NP / 2) * sizeof(SlotRec);
+  static constexpr size_t kPart = kHist + static_cast<size_t>(kHistEntries) * sizeof(float2);
+  static constexpr size_t kRedf =
+      kPart + (kPartInHistory ? 0 : 2 * static_cast<size_t>(kThreads) * sizeof(double));
+  static constexpr size_t kCtrl = kRedf + 16 * sizeof(float);
+  static constexpr size_t kBytes = kCtrl + 4 * sizeof(int);
+  static_assert(kRec % 16 == 0 && kHist % 16 == 0 && kPart % 8 == 0, "LDS alignment");
+  static_assert(!kPartInHistory || sizeof(float2) * static_cast<size_t>(kHistEntries) >=
+                                       kSrootBytes + 2 * sizeof(double) * kThreads,
+                "room for the reduction partials in the rotation history");
+};
+
+// One matrix on one workgroup.  NT = threads_for_order(NP, WIDE) threads do
+// the work; a workgroup may have more (the persistent launch runs every order
+// on 1024 threads): those take part in the barriers only, every work loop
+// strides by NT, and the results do not depend on the workgroup's size.
+template <int NP, bool WIDE>
+__device__ __forceinline__ void jacobi_solve(const JacobiJob& job, const double* __restrict__ a64,
+                                             float* __restrict__ vectors,
+                                             float* __restrict__ diagonal,
+                                             float* __restrict__ roots,
+                                             JacobiStatus* __restrict__ status,
+                                             unsigned char* __restrict__ smem, const int nt) {
+  using L = JacobiLds<NP, WIDE>;
   constexpr int S = slots_for_order(NP);
   constexpr bool VMEM = v_in_memory(S);
   constexpr int G = group_for_order(NP);
@@ -618,9 +658,12 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   constexpr bool SUPER = use_super_blocks(NP);
   constexpr int IPT = items_per_thread(NP);
   constexpr int BPT = single_blocks_per_thread(NP, WIDE);
-  const JacobiJob job = jobs[blockIdx.x];
   const int tid = threadIdx.x;
-  const int nt = blockDim.x;
+  // nt: the working threads, threads_for_order(NP, WIDE), passed in as a
+  // run-time value (the work loops' stride: with a compile-time stride the
+  // compiler reshapes them and spills in the block phase).
+  constexpr int nthreads = threads_for_order(NP, WIDE);
+  const bool worker = tid < nt;  // whole waves: nt is a multiple of 64
   const int n = static_cast<int>(job.order);
   const int ld = static_cast<int>(job.ld);
   constexpr int N = NP;  // padded order: zero rows/columns beyond n
@@ -633,28 +676,15 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   constexpr int packed = N * (N + 1) / 2;
   constexpr int m2 = m / 2;  // slot pairs
   constexpr int items = super_items(NP);
-  constexpr int nthreads = threads_for_order(NP, WIDE);  // == blockDim.x (host check)
 
-  // Static LDS: every array at a compile-time offset (addresses fold into
-  // the ds instructions' offset fields).
-  __shared__ __align__(16) float Ap[(packed + 3) & ~3];
-  __shared__ SlotRec srec[2][m];  // round parity x slot
-  __shared__ float2 hist[G * hrow];  // G rounds x 8 parts x SP of (c, s') for V
-  __shared__ float redf[16];
-  __shared__ int ctrl[4];
-  // Fixed-order reductions (block_sum2) need two doubles per thread.  Above
-  // N = 128 they use the rotation history, idle at both reduction points
-  // (after the s_i = sqrt(a_ii) values the convergence test keeps at its
-  // start); below, where LDS is plentiful, an array of their own.
-  constexpr bool kPartInHistory = NP > 128;
-  constexpr int kSrootBytes = (4 * NP + 7) / 8 * 8;
-  static_assert(!kPartInHistory ||
-                    sizeof(float2) * G * hrow >= kSrootBytes + 2 * sizeof(double) * nthreads,
-                "room for the reduction partials in the rotation history");
-  __shared__ double partbuf[kPartInHistory ? 2 : 2 * nthreads];
-  double* const part0 =
-      kPartInHistory ? reinterpret_cast<double*>(reinterpret_cast<char*>(hist) + kSrootBytes)
-                     : partbuf;
+  float* const Ap = reinterpret_cast<float*>(smem + L::kAp);
+  SlotRec (*const srec)[m] = reinterpret_cast<SlotRec(*)[m]>(smem + L::kRec);
+  float2* const hist = reinterpret_cast<float2*>(smem + L::kHist);
+  float* const redf = reinterpret_cast<float*>(smem + L::kRedf);
+  int* const ctrl = reinterpret_cast<int*>(smem + L::kCtrl);
+  double* const part0 = L::kPartInHistory
+                            ? reinterpret_cast<double*>(smem + L::kHist + L::kSrootBytes)
+                            : reinterpret_cast<double*>(smem + L::kPart);
   double* const part1 = part0 + nthreads;
 
   const int lane = tid & 63;
@@ -678,7 +708,10 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   // nothing, keeping the waves in step.
   constexpr int CODES = SUPER ? IPT + 1 : BPT;
   uint32_t code[CODES];
-  if constexpr (SUPER) {
+  if (!worker) {
+#pragma unroll
+    for (int i = 0; i < CODES; ++i) code[i] = ~0u;
+  } else if constexpr (SUPER) {
     code[0] = owner_super(tid, m2);
 #pragma unroll
     for (int i = 0; i < IPT; ++i) {
@@ -701,7 +734,8 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
 
   const double* A = a64 + job.a_offset;
   double norm2 = 0.0;
-  for (int e = tid; e < packed; e += nt) {
+  #pragma unroll 1
+  for (int e = tid; worker && e < packed; e += nt) {
     const int j = tri_col(e);
     const int i = e - (j * (j + 1)) / 2;
     float v = 0.0f;
@@ -724,7 +758,8 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
       hi[j] = (vlane && row == N - 1 - k) ? 1.0f : 0.0f;
     }
   } else {
-    for (int e = tid; e < N * N; e += nt) {
+    #pragma unroll 1
+    for (int e = tid; worker && e < N * N; e += nt) {
       const int i = e % N;
       const int x = e / N;
       Vg[i + static_cast<size_t>(x) * ld] = i == x ? 1.0f : 0.0f;
@@ -863,14 +898,16 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
             }
           }
         };
-        if constexpr (BPT > 1) {
-          if (__ballot(code[BPT - 1] != ~0u) == 0) {
-            blocks(std::integral_constant<int, BPT - 1>{});
+        if (worker) {
+          if constexpr (BPT > 1) {
+            if (__ballot(code[BPT - 1] != ~0u) == 0) {
+              blocks(std::integral_constant<int, BPT - 1>{});
+            } else {
+              blocks(std::integral_constant<int, BPT>{});
+            }
           } else {
             blocks(std::integral_constant<int, BPT>{});
           }
-        } else {
-          blocks(std::integral_constant<int, BPT>{});
         }
         if (tid < m && t + 1 < group_end)
           pair_phase(tid, t + 1, hist + (t + 1 - group_start) * hrow);
@@ -881,7 +918,8 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     // positions still advance; the output maps them through V's own round
     // count).  V in device memory applies the group's rounds only.
     if constexpr (!VMEM) {
-      for (int e = (group_end - group_start) * hrow + tid; e < G * hrow; e += nt)
+      #pragma unroll 1
+      for (int e = (group_end - group_start) * hrow + tid; worker && e < G * hrow; e += nt)
         hist[e] = make_float2(1.0f, 0.0f);
     }
     __syncthreads();
@@ -894,7 +932,7 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
       // row of a wave compute on zeros that no valid lane reads): no divergent
       // branch around the 2 S live V registers.  Waves past the V lanes skip
       // it (a wave-uniform branch).  G = S: every group starts at phase 0.
-      if (wave * rows_per_wave < N) v_group<S, G, 0>(lo, hi, hist, hrow, h, H, lane);
+      if (worker && wave * rows_per_wave < N) v_group<S, G, 0>(lo, hi, hist, hrow, h, H, lane);
     } else {
       // Each lane loads its positions, applies the group's rounds with the
       // values shifting along the circle in registers, and stores every
@@ -903,14 +941,14 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
       // treat as new in every group (empty asm): otherwise it hoists all the
       // addresses out of the loop and keeps them live (spilled) through the A
       // update.
-      {
+      if (worker) {
         // Rows r and r + N / 2 on the same lane (8 lanes per row), so each
         // rotation read from the history serves two rows.
         const int vrow = tid >> 3;
         const int vh = tid & 7;
         uint32_t lo_off = static_cast<uint32_t>(vrow + vh * S * ld);
         uint32_t hi_off = static_cast<uint32_t>(vrow + (N - 1 - vh * S) * ld);
-        uint32_t step = static_cast<uint32_t>(ld);
+        uint32_t step = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(ld));
         asm volatile("" : "+v"(lo_off), "+v"(hi_off), "+s"(step));
         float vl0[S], vh0[S], vl1[S], vh1[S];
 #pragma unroll
@@ -946,10 +984,13 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     float dmin = INFINITY;
     // s_i = sqrt(a_ii) in the history buffer (idle until the next group).
     float* sroot = reinterpret_cast<float*>(hist);
-    for (int i = tid; i < N; i += nt) sroot[i] = i < n ? sqrtf(fmaxf(Ap[tri(i, i)], 0.0f)) : 0.0f;
+    #pragma unroll 1
+    for (int i = tid; worker && i < N; i += nt)
+      sroot[i] = i < n ? sqrtf(fmaxf(Ap[tri(i, i)], 0.0f)) : 0.0f;
     if (tid == 0) redf[0] = INFINITY;
     __syncthreads();
-    for (int e = tid; e < packed; e += nt) {
+    #pragma unroll 1
+    for (int e = tid; worker && e < packed; e += nt) {
       const int j = tri_col(e);
       const int i = e - (j * (j + 1)) / 2;
       if (j >= n) continue;
@@ -1005,7 +1046,8 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
   float* dg = diagonal + job.s_offset;
   float* rt = roots + job.s_offset;
   int bad = 0;
-  for (int i = tid; i < ld; i += nt) {
+  #pragma unroll 1
+  for (int i = tid; worker && i < ld; i += nt) {
     const float d = i < n ? Ap[tri(i, i)] : 0.0f;
     dg[i] = d;
     rt[i] = i < n ? sqrtf(d) : 0.0f;
@@ -1020,28 +1062,33 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
       const int vh = tid & 7;
       float vl[S], vhv[S];
       const uint32_t base = static_cast<uint32_t>(vrow);
+      if (worker) {
 #pragma unroll
-      for (int j = 0; j < S; ++j) {
-        vl[j] = Vg[base + static_cast<uint32_t>((vh * S + j) * ld)];
-        vhv[j] = Vg[base + static_cast<uint32_t>((N - 1 - (vh * S + j)) * ld)];
+        for (int j = 0; j < S; ++j) {
+          vl[j] = Vg[base + static_cast<uint32_t>((vh * S + j) * ld)];
+          vhv[j] = Vg[base + static_cast<uint32_t>((N - 1 - (vh * S + j)) * ld)];
+        }
       }
       __syncthreads();
+      if (worker) {
 #pragma unroll
-      for (int j = 0; j < S; ++j) {
-        const int kl = participant(round, vh * S + j, N);
-        const int kh = participant(round, N - 1 - (vh * S + j), N);
-        Vg[base + static_cast<uint32_t>(kl * ld)] = vl[j];
-        Vg[base + static_cast<uint32_t>(kh * ld)] = vhv[j];
+        for (int j = 0; j < S; ++j) {
+          const int kl = participant(round, vh * S + j, N);
+          const int kh = participant(round, N - 1 - (vh * S + j), N);
+          Vg[base + static_cast<uint32_t>(kl * ld)] = vl[j];
+          Vg[base + static_cast<uint32_t>(kh * ld)] = vhv[j];
+        }
       }
       __syncthreads();
     }
   }
-  for (int e = tid; e < ld * ld; e += nt) {
+  #pragma unroll 1
+  for (int e = tid; worker && e < ld * ld; e += nt) {
     const int i = e % ld;
     const int x = e / ld;
     if (i >= n || x >= n) Vg[e] = 0.0f;
   }
-  if (!VMEM && vlane && row < n) {
+  if (!VMEM && worker && vlane && row < n) {
     // G = S, so V's phase is 0: register r holds slot h S + r; the position
     // x of a slot holds matrix index P_t(x) of the round reached.
 #pragma unroll
@@ -1064,6 +1111,84 @@ __global__ __launch_bounds__(1024) void batched_jacobi_kernel(
     st->matrix_norm = matrix_norm;
     st->min_diagonal = min_diag;
     st->correction_rate = static_cast<float>(rate);
+  }
+}
+
+// One workgroup per matrix of one padded order.
+template <int NP, bool WIDE>
+__global__ __launch_bounds__(1024) void batched_jacobi_kernel(
+    const JacobiJob* __restrict__ jobs, const double* __restrict__ a64,
+    float* __restrict__ vectors, float* __restrict__ diagonal,
+    float* __restrict__ roots, JacobiStatus* __restrict__ status) {
+  __shared__ __align__(16) unsigned char smem[JacobiLds<NP, WIDE>::kBytes];
+  jacobi_solve<NP, WIDE>(jobs[blockIdx.x], a64, vectors, diagonal, roots, status, smem,
+                         static_cast<int>(blockDim.x));
+}
+
+// Persistent launch of the orders 129..256 (padded 144..256), whose
+// workgroups each take a whole CU (A's packed triangle and the rotation
+// history fill the LDS): one 1024-thread workgroup per CU takes the next
+// job from a device counter until the jobs run out, so the CUs stay busy
+// across order classes instead of waiting for each class's slowest matrix.
+// The jobs come largest order first (longest processing time first).  Every
+// workgroup reads the counter once per job and leaves when it passes the
+// job count, so the grid drains.  A job's results are those of
+// batched_jacobi_kernel for it, bit for bit.
+constexpr int kPersistentMinPadded = 144;
+constexpr size_t persistent_lds_bytes() {
+  size_t b = 0;
+  b = JacobiLds<144, false>::kBytes > b ? JacobiLds<144, false>::kBytes : b;
+  b = JacobiLds<160, false>::kBytes > b ? JacobiLds<160, false>::kBytes : b;
+  b = JacobiLds<176, false>::kBytes > b ? JacobiLds<176, false>::kBytes : b;
+  b = JacobiLds<192, false>::kBytes > b ? JacobiLds<192, false>::kBytes : b;
+  b = JacobiLds<208, false>::kBytes > b ? JacobiLds<208, false>::kBytes : b;
+  b = JacobiLds<224, false>::kBytes > b ? JacobiLds<224, false>::kBytes : b;
+  b = JacobiLds<240, false>::kBytes > b ? JacobiLds<240, false>::kBytes : b;
+  b = JacobiLds<256, false>::kBytes > b ? JacobiLds<256, false>::kBytes : b;
+  return b;
+}
+static_assert(persistent_lds_bytes() + 16 <= 160 * 1024, "persistent Jacobi LDS");
+
+// One job of the persistent launch: a call, not inlined, so that the eight
+// orders' bodies do not share one register allocation and stack; the LDS
+// pointer keeps its address space across the call.
+template <int NP>
+__device__ __attribute__((noinline)) void persistent_job(
+    const JacobiJob& job, const double* __restrict__ a64, float* __restrict__ vectors,
+    float* __restrict__ diagonal, float* __restrict__ roots, JacobiStatus* __restrict__ status,
+    __attribute__((address_space(3))) unsigned char* lds) {
+  // The 1024-thread workgroup's first threads_for_order(NP) threads work
+  // (blockDim.x - (1024 - that): equal, but not a compile-time constant).
+  jacobi_solve<NP, false>(job, a64, vectors, diagonal, roots, status, (unsigned char*)lds,
+                          static_cast<int>(blockDim.x) - (1024 - threads_for_order(NP, false)));
+}
+
+__global__ __launch_bounds__(1024) void persistent_jacobi_kernel(
+    const JacobiJob* __restrict__ jobs, uint32_t count, uint32_t* __restrict__ counter,
+    const double* __restrict__ a64, float* __restrict__ vectors, float* __restrict__ diagonal,
+    float* __restrict__ roots, JacobiStatus* __restrict__ status) {
+  __shared__ __align__(16) unsigned char smem[persistent_lds_bytes()];
+  __shared__ uint32_t next;
+  auto* lds = (__attribute__((address_space(3))) unsigned char*)smem;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t i = next;
+    __syncthreads();  // every lane has read `next` before it is reused
+    if (i >= count) return;
+    const JacobiJob job = jobs[i];
+    switch ((job.order + 15u) & ~15u) {
+      case 144: persistent_job<144>(job, a64, vectors, diagonal, roots, status, lds); break;
+      case 160: persistent_job<160>(job, a64, vectors, diagonal, roots, status, lds); break;
+      case 176: persistent_job<176>(job, a64, vectors, diagonal, roots, status, lds); break;
+      case 192: persistent_job<192>(job, a64, vectors, diagonal, roots, status, lds); break;
+      case 208: persistent_job<208>(job, a64, vectors, diagonal, roots, status, lds); break;
+      case 224: persistent_job<224>(job, a64, vectors, diagonal, roots, status, lds); break;
+      case 240: persistent_job<240>(job, a64, vectors, diagonal, roots, status, lds); break;
+      case 256: persistent_job<256>(job, a64, vectors, diagonal, roots, status, lds); break;
+      default: break;  // the host launches orders 129..256 only
+    }
+    __syncthreads();  // the next job reuses the LDS
   }
 }
 
@@ -1111,6 +1236,22 @@ size_t batched_jacobi_lds_bytes(uint32_t order) {
                               : 2 * sizeof(double) * static_cast<size_t>(threads_for_order(N, true));
   return ((packed + 3) & ~size_t{3}) * sizeof(float) + 2 * (N / 2) * sizeof(SlotRec) +
          hist * sizeof(float2) + part + 16 * sizeof(float) + 4 * sizeof(int);
+}
+
+hipError_t launch_persistent_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t* counter,
+                                    int grid, const double* a64, float* vectors,
+                                    float* diagonal, float* roots, JacobiStatus* status,
+                                    hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  if (!jobs || !counter || grid < 1) return hipErrorInvalidValue;
+  static_assert(kPersistentMinPadded == ((kPersistentMinOrder + 15u) & ~15u),
+                "persistent orders");
+  const uint32_t blocks = static_cast<uint32_t>(grid) < count ? static_cast<uint32_t>(grid) : count;
+  hipError_t e = launch_fill_u32(counter, 1, 0u, stream);
+  if (e != hipSuccess) return e;
+  persistent_jacobi_kernel<<<blocks, 1024, 0, stream>>>(jobs, count, counter, a64, vectors,
+                                                        diagonal, roots, status);
+  return hipGetLastError();
 }
 
 hipError_t launch_batched_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t order,
