@@ -477,7 +477,7 @@ def main():
     # Whole-batch roofline fraction of each path with the S and k each matrix
     # ran (a matrix re-solved on the plain path counts with that solve).
     if use_selector:
-        low_tiers = [SEL.TIERS[k][1] or "fp32" for k in chosen[0]]
+        low_tiers = [SEL.choice_format(int(k)) for k in chosen[0]]
     else:
         low_tiers = [args.tier] * len(mats)
     for i in fallback:
@@ -523,8 +523,8 @@ def main():
         "low_bit": {"matrices_per_s": round(pass_low / dt_low, 2),
                     "all_matrices_per_s": round(len(mats) / dt_low, 2),
                     "selector": ({"tier_counts": dict(zip(
-                        [t[0] for t in SEL.TIERS],
-                        np.bincount(chosen[0], minlength=len(SEL.TIERS)).tolist()))}
+                        SEL.CHOICES,
+                        np.bincount(chosen[0], minlength=len(SEL.CHOICES)).tolist()))}
                                  if use_selector else None),
                     "max_rel_error": err_low, "within_1e-6": pass_low,
                     "fallbacks": len(fallback),

@@ -495,6 +495,15 @@ int msq_matrix_features(msq_runtime *runtime, uint32_t count, const uint32_t *or
 int msq_mlp_forward(msq_runtime *runtime, const float *params, const float *x, uint32_t batch,
                     float *logits, float *h1, float *h2);
 
+/* BF16 inference: logits (device, batch x MSQ_MLP_OUTPUTS, float) of the
+ * network whose weights `params` (device, MSQ_MLP_PARAMS BF16 codes, e.g. from
+ * msq_quantize with MSQ_BF16) are stored in BF16, with the inputs and the
+ * hidden activations rounded to BF16 (nearest even, after the ReLU) and float
+ * accumulation in index order.  No reference counterpart (the learned tier
+ * selector is new). */
+int msq_mlp_forward_bf16(msq_runtime *runtime, const uint16_t *params, const float *x,
+                         uint32_t batch, float *logits);
+
 /* Gradient (device, MSQ_MLP_PARAMS) of the masked binary cross-entropy
  *   L = sum_{b,k} mask_bk (softplus(z_bk) - y_bk z_bk) / mask_total
  * at the forward pass (x, h1, h2, logits) with targets y and mask (device,

@@ -2761,6 +2761,15 @@ int msq_mlp_forward(msq_runtime* rt, const float* params, const float* x, uint32
   return MSQ_OK;
 }
 
+int msq_mlp_forward_bf16(msq_runtime* rt, const uint16_t* params, const float* x, uint32_t batch,
+                         float* logits) {
+  MSQ_TRY(activate(rt));
+  if (!params || !x || !logits) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
+  if (batch > (1u << 24)) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "batch too large");
+  HIP_TRY(msq::launch_mlp_forward_bf16(params, x, static_cast<int>(batch), logits, rt->stream));
+  return MSQ_OK;
+}
+
 int msq_mlp_backward(msq_runtime* rt, const float* params, const float* x, const float* h1,
                      const float* h2, const float* logits, const float* targets,
                      const float* mask, uint32_t batch, float mask_total, float* grads,

@@ -112,6 +112,8 @@ struct Segment;
 hipError_t launch_features(int rows, const Segment* segs, uint32_t count, const double* a64,
                            float* features, hipStream_t stream);
 int feature_rows_for_order(uint32_t order);
+hipError_t launch_mlp_forward_bf16(const uint16_t* params, const float* x, int batch,
+                                   float* logits, hipStream_t stream);
 hipError_t launch_mlp_forward(const float* params, const float* x, int batch, float* logits,
                               float* h1, float* h2, hipStream_t stream);
 hipError_t launch_mlp_backward(const float* params, const float* x, const float* h1,

@@ -16,6 +16,7 @@
 
 #include <math.h>
 
+#include "device_codecs.hpp"
 #include "kernels.hpp"
 
 namespace msq {
@@ -405,6 +406,47 @@ __global__ __launch_bounds__(kMlpThreads) void mlp_forward_kernel(
   for (int k = 0; k < kOut; ++k) logits[static_cast<size_t>(b) * kOut + k] = z[k];
 }
 
+// BF16 inference: weights stored in BF16, inputs and hidden activations
+// rounded to BF16 (round to nearest even, after the ReLU), products
+// accumulated in float with fused multiply-adds in index order (a product of
+// two BF16 values is exact in float, so each step rounds once), logits in
+// float.  One thread per sample, the same order of operations as
+// mlp_forward_kernel.
+__global__ __launch_bounds__(kMlpThreads) void mlp_forward_bf16_kernel(
+    const uint16_t* __restrict__ params, const float* __restrict__ x, int batch,
+    float* __restrict__ logits) {
+  __shared__ float p[kParams];
+  for (int e = threadIdx.x; e < kParams; e += blockDim.x) p[e] = decode_bf16(params[e]);
+  __syncthreads();
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  float xi[kIn];
+#pragma unroll
+  for (int i = 0; i < kIn; ++i)
+    xi[i] = decode_bf16(encode_bf16(x[static_cast<size_t>(b) * kIn + i]));
+  float h1[kHid];
+#pragma unroll
+  for (int j = 0; j < kHid; ++j) {
+    float acc = p[kB1 + j];
+#pragma unroll
+    for (int i = 0; i < kIn; ++i) acc = fmaf(p[kW1 + j * kIn + i], xi[i], acc);
+    h1[j] = decode_bf16(encode_bf16(fmaxf(acc, 0.0f)));
+  }
+  float z[kOut];
+#pragma unroll
+  for (int k = 0; k < kOut; ++k) z[k] = p[kB3 + k];
+  for (int j = 0; j < kHid; ++j) {
+    float acc = p[kB2 + j];
+#pragma unroll
+    for (int i = 0; i < kHid; ++i) acc = fmaf(p[kW2 + j * kHid + i], h1[i], acc);
+    const float h2 = decode_bf16(encode_bf16(fmaxf(acc, 0.0f)));
+#pragma unroll
+    for (int k = 0; k < kOut; ++k) z[k] = fmaf(p[kW3 + k * kHid + j], h2, z[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < kOut; ++k) logits[static_cast<size_t>(b) * kOut + k] = z[k];
+}
+
 // Per-sample deltas: dz = mask (sigmoid(z) - y) / mask_total,
 // dh2 = (W3^T dz) o [h2 > 0], dh1 = (W2^T dh2) o [h1 > 0]; and the sample's
 // loss sum_k mask (softplus(z) - y z).
@@ -526,6 +568,14 @@ hipError_t launch_mlp_forward(const float* params, const float* x, int batch, fl
   if (batch <= 0) return hipSuccess;
   mlp_forward_kernel<<<(batch + kMlpThreads - 1) / kMlpThreads, kMlpThreads, 0, stream>>>(
       params, x, batch, logits, h1, h2);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_forward_bf16(const uint16_t* params, const float* x, int batch,
+                                   float* logits, hipStream_t stream) {
+  if (batch <= 0) return hipSuccess;
+  mlp_forward_bf16_kernel<<<(batch + kMlpThreads - 1) / kMlpThreads, kMlpThreads, 0, stream>>>(
+      params, x, batch, logits);
   return hipGetLastError();
 }
 

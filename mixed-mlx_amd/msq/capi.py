@@ -80,8 +80,8 @@ EXPORTED_SYMBOLS = (
     "msq_correction", "msq_refine_native_staged",
     "msq_orthogonality", "msq_solve_batched", "msq_solve_batched_ex", "msq_solve_options_init",
     "msq_last_factors",
-    "msq_matrix_features", "msq_mlp_forward", "msq_mlp_backward", "msq_mlp_adam",
-    "msq_autotune", "msq_set_tile_table",
+    "msq_matrix_features", "msq_mlp_forward", "msq_mlp_forward_bf16", "msq_mlp_backward",
+    "msq_mlp_adam", "msq_autotune", "msq_set_tile_table",
 )
 TILE_TYPES = ("fp64_residual", "fp32", "bf16", "fp8_scaled", "mxfp4")
 TILE_CLASSES = (32, 64, 96, 128, 160, 192, 224, 256, 512)
@@ -199,6 +199,7 @@ _sig("msq_spectrum_metadata_bytes", _sz, _i, _sz)
 _sig("msq_last_factors", _i, _p, ctypes.POINTER(_p), ctypes.POINTER(_p))
 _sig("msq_matrix_features", _i, _p, _u32, _u32p, _p, _p)
 _sig("msq_mlp_forward", _i, _p, _p, _p, _u32, _p, _p, _p)
+_sig("msq_mlp_forward_bf16", _i, _p, _p, _p, _u32, _p)
 _sig("msq_mlp_backward", _i, _p, _p, _p, _p, _p, _p, _p, _p, _u32, ctypes.c_float, _p, _p)
 _sig("msq_mlp_adam", _i, _p, _p, _p, _p, _p, _u32, ctypes.c_float, ctypes.c_float,
      ctypes.c_float, ctypes.c_float, ctypes.c_float, _u32)
@@ -583,6 +584,12 @@ class Runtime:
         batch = x.numel() // MLP_INPUTS
         check(lib.msq_mlp_forward(self.handle, _addr(params), _addr(x), batch, _addr(logits),
                                   _addr(h1), _addr(h2)))
+
+    def mlp_forward_bf16(self, params16, x, logits):
+        """params16: device uint8/int16 tensor of MLP_PARAMS BF16 codes."""
+        batch = x.numel() // MLP_INPUTS
+        check(lib.msq_mlp_forward_bf16(self.handle, _addr(params16), _addr(x), batch,
+                                       _addr(logits)))
 
     def mlp_backward(self, params, x, h1, h2, logits, targets, mask, mask_total, grads,
                      losses=None):

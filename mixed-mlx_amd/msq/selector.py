@@ -1,29 +1,49 @@
 """Learned tier selector of the batched SPD square-root solver.
 
 No counterpart in the reference.  For each matrix of a batch the selector
-chooses one of eight solver configurations ("tiers"): the plain FP32 path,
-which always meets the target and is the fallback, and seven low-bit tiers
-that stop Jacobi early (``correction_rate``) and leave the rest of the
-accuracy to the Sylvester corrections.  A low-bit tier that misses the
-binary64 residual check is re-solved on the plain path, so a wrong choice
-costs time, never accuracy.
+chooses one of eight solver configurations ("tiers"): the product format of
+the reconstruction and corrections -- MXFP4, scaled FP8 or BF16 (both with
+BF16 corrections), or FP32 -- crossed with two Jacobi stop rules, stopping
+once the frozen-factor corrections are predicted to contract by 0.5 per step
+(fewer corrections) or by 2 (the Jacobi stops earlier, more corrections).
+The plain FP32 path is a ninth choice the network does not score: it always
+meets the target (with its second Jacobi pass for unresolved spectra) and is
+the fallback of any tier that misses the binary64 residual check, so a wrong
+choice costs time, never accuracy.
+
+The tier space follows the format list of the learned-tier workload
+({MXFP4, FP8, BF16, FP32}); its second axis is the Jacobi stop rule instead
+of a budget of one or two corrections.  Measured on MI355X (4096-matrix
+uneven batch and a 1000-matrix 32..512 mixture): capping the corrections at
+two makes the Jacobi run 2-4 more sweeps (each costing about ten correction
+steps) and sends 78% of the uneven batch (BF16 products) or 70% of the
+mixture to the fallback; the two stop rules cover the same trade between
+Jacobi sweeps and corrections without that cliff.
 
 * features: ``Runtime.matrix_features`` (HIP kernel; 16 per matrix);
 * network: 16 -> 64 -> 64 -> 8 ReLU MLP, output k = logit of "tier k meets
-  the target within its correction budget"; forward, backward and Adam are the
-  library's HIP kernels (``Runtime.mlp_*``);
-* labels: the solver's own runs of every tier on seeded synthetic batches
-  (``label_batch``), success = converged and relative Frobenius error against
-  the construction-known root <= 1e-6;
-* decision: the tier of least expected cost c_t + (1 - p_t) F c_plain, with
-  p_t the predicted success and c_t the tier's measured cost: the wall time of
-  the tier's batched solve of each labelled batch divided by the batch's work
-  sum (n / 256)^3, pooled over the batches (milliseconds per unit of work; a
-  matrix of order n costs c_t (n / 256)^3, so the factor cancels in the
-  choice).  A failed choice pays the plain path again, in a second solve call
-  whose latency is that of its slowest matrix rather than the matrix's share
-  of a batch: F = FALLBACK_WEIGHT weights it accordingly, so a low-bit tier is
-  chosen only when its predicted success is high.
+  the target within its correction budget"; trained in float32 by the
+  library's forward, backward and Adam kernels; inference in BF16
+  (``Runtime.mlp_forward_bf16``: BF16 weights and activations, float
+  accumulation);
+* labels: the solver's own runs of every tier and of the plain path on
+  seeded synthetic batches (``label_batch``), success = converged and
+  relative Frobenius error against the construction-known root <= 1e-6;
+* decision: per matrix of order n, the choice of least expected time
+  w c_t + (1 - p_t) (L(n) + w c_plain), w = (n / 256)^3, with p_t the
+  predicted success, c_t the measured cost of choice t (the wall time of its
+  batched solve of each labelled batch divided by the batch's work sum,
+  pooled over the batches) and L(n) the measured latency of the plain-path
+  solve of one matrix of order n (fallback_latency.py): a failed choice is
+  solved again in a second batched call, whose time is that of its slowest
+  matrix rather than the matrix's share of a batch.  The plain path costs
+  w c_plain.  Each product format used in a batch adds its own launches to
+  every correction step of the batched solve, which the per-matrix costs do
+  not model: formats chosen for less than GROUP_SHARE_MIN of the batch's work
+  are dropped (their matrices take their best remaining choice), and a batch
+  whose best single choice is expected within SINGLE_MARGIN of the
+  per-matrix choices runs that single choice.  A batch expected to pass on
+  the fixed BF16 tier (GATE_EXPECTED_FAILURES) runs it.
 
 Feature standardisation is folded into the first layer when the weights are
 saved, so inference runs the raw features through the network unchanged.
@@ -38,26 +58,46 @@ from . import capi
 from . import solver as MS
 
 WEIGHTS = pathlib.Path(__file__).resolve().parent / "data" / "selector.npz"
-FALLBACK_WEIGHT = 10.0
+# Weight of the expected fallback latency (1: as measured).
+FALLBACK_WEIGHT = 1.0
 
-# (name, basis/correction tier, correction_rate); tier 0 is the plain path.
+# (name, product format, correction_rate); the network's output k scores
+# TIERS[k].  PLAIN (= 8) is the plain FP32 path.
 TIERS = [
-    ("plain_fp32", None, 0.0),
+    ("mxfp4_r0.5", "mxfp4", 0.5),
+    ("mxfp4_r2", "mxfp4", 2.0),
+    ("fp8_r0.5", "fp8", 0.5),
+    ("fp8_r2", "fp8", 2.0),
     ("bf16_r0.5", "bf16", 0.5),
-    ("bf16_r1", "bf16", 1.0),
     ("bf16_r2", "bf16", 2.0),
-    ("bf16_r4", "bf16", 4.0),
-    ("bf16_r8", "bf16", 8.0),
-    ("fp8_r1", "fp8", 1.0),
-    ("mxfp4_r1", "mxfp4", 1.0),
+    ("fp32_r0.5", "fp32", 0.5),
+    ("fp32_r2", "fp32", 2.0),
 ]
 assert len(TIERS) == capi.MLP_OUTPUTS
+PLAIN = len(TIERS)
+CHOICES = [t[0] for t in TIERS] + ["plain_fp32"]
+FIXED = 4  # bf16_r0.5: the fixed tier the selector is compared with
+GROUP_SHARE_MIN = 0.1
+# Measured on MI355X (held-out batches, same process): choices that mix two
+# product formats ran 4.9 ms (8%) slower on the 4096-matrix uneven batch than
+# their best single choice, so a single choice within 10% is preferred.
+SINGLE_MARGIN = 0.1
+# A batch the network expects to pass on the fixed tier (fewer than half a
+# matrix expected to fail) runs the fixed tier: on the uneven batch the
+# tiers' costs lie within a few percent of each other, below what the cost
+# model resolves.
+GATE_EXPECTED_FAILURES = 0.5
+
+
+def choice_format(k):
+    """Product format of choice k ("fp32" for the plain path)."""
+    return "fp32" if k == PLAIN else TIERS[k][1]
 
 
 def tier_options(k, max_corrections=12):
-    name, tier, rate = TIERS[k]
-    if tier is None:
+    if k == PLAIN:
         return MS.fallback_options()
+    name, tier, rate = TIERS[k]
     return MS.tier_options(tier, rate=rate, max_corrections=max_corrections)
 
 
@@ -71,15 +111,21 @@ def unit_costs(tier_ms, orders):
     return np.asarray(tier_ms, dtype=np.float64) / float(np.sum(work_units(orders)))
 
 
-def policy_cost(choice, success, unit_cost, orders):
-    """Modelled time (ms) of a per-matrix tier choice on a labelled batch: a
-    matrix costs its tier's unit cost times (n / 256)^3, and every choice that
-    failed pays the plain path again; returns (cost, failures)."""
+def policy_cost(choice, success, unit_cost, orders, latency=None):
+    """Modelled time (ms) of a per-matrix choice on a labelled batch: a
+    matrix costs its choice's unit cost times (n / 256)^3; the choices that
+    failed pay the plain path again (unit_cost[PLAIN] times their work) in
+    one second call, which takes at least `latency(n)` of its largest failed
+    matrix when `latency` is given.  success and unit_cost hold the eight
+    tiers, then the plain path.  Returns (cost, failures)."""
     rows = np.arange(len(choice))
     w = work_units(orders)
     c = np.asarray(unit_cost, dtype=np.float64)
     ok = success[rows, choice] > 0
-    return float(np.sum(w * (c[choice] + np.where(ok, 0.0, c[0])))), int(np.sum(~ok))
+    total = float(np.sum(w * (c[choice] + np.where(ok, 0.0, c[PLAIN]))))
+    if latency is not None and not ok.all():
+        total += float(np.max(latency(np.asarray(orders)[~ok])))
+    return total, int(np.sum(~ok))
 
 
 # -------------------------------------------------------------- labelling
@@ -94,12 +140,12 @@ def true_errors(roots, mats, offs, lds):
     return out
 
 
-def label_batch(rt, mats, target=MS.TARGET, log=None):
-    """Runs every tier on the batch.  Returns (features, success, sweeps,
-    corrections, orders, tier_ms): arrays of shape (count, 16), (count, 8) x 3,
-    (count,), (8,); tier_ms is the wall time of each tier's batched solve (the
-    median of three runs after a first, so graph capture and workspace growth
-    are excluded)."""
+def label_batch(rt, mats, target=MS.TARGET, log=None, runs=3):
+    """Runs every tier and the plain path on the batch.  Returns (features,
+    success, sweeps, corrections, orders, tier_ms): arrays of shape (count,
+    16), (count, 9) x 3, (count,), (9,); tier_ms is the wall time of each
+    choice's batched solve (the median of `runs` runs after a first, so graph
+    capture and workspace growth are excluded)."""
     import time
 
     import torch
@@ -111,7 +157,7 @@ def label_batch(rt, mats, target=MS.TARGET, log=None):
     X = torch.zeros(total, dtype=torch.float64, device="cuda")
     feats = torch.zeros(len(mats) * capi.MLP_INPUTS, dtype=torch.float32, device="cuda")
     rt.matrix_features(orders, A, feats)
-    count, tiers = len(mats), len(TIERS)
+    count, tiers = len(mats), len(CHOICES)
     success = np.zeros((count, tiers), dtype=np.float32)
     sweeps = np.zeros((count, tiers))
     corrections = np.zeros((count, tiers))
@@ -119,21 +165,21 @@ def label_batch(rt, mats, target=MS.TARGET, log=None):
     for k in range(tiers):
         opts = tier_options(k)
         rt.solve_batched(orders, A, X, opts)
-        runs = []
-        for _ in range(3):
+        times = []
+        for _ in range(runs):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             reps = rt.solve_batched(orders, A, X, opts)
             torch.cuda.synchronize()
-            runs.append((time.perf_counter() - t0) * 1e3)
-        tier_ms[k] = float(np.median(runs))
+            times.append((time.perf_counter() - t0) * 1e3)
+        tier_ms[k] = float(np.median(times))
         err = true_errors(X, mats, offs, lds)
         ok = reps.converged & (err <= target)
         success[:, k] = ok
         sweeps[:, k] = reps.sweeps
         corrections[:, k] = reps.corrections
         if log:
-            log(f"  tier {TIERS[k][0]}: success {ok.mean():.3f}, mean sweeps "
+            log(f"  {CHOICES[k]}: success {ok.mean():.3f}, mean sweeps "
                 f"{sweeps[:, k].mean():.2f}, mean corrections {corrections[:, k].mean():.2f}, "
                 f"{tier_ms[k]:.1f} ms")
     return (feats.cpu().numpy().reshape(count, capi.MLP_INPUTS), success, sweeps, corrections,
@@ -213,95 +259,128 @@ def train(rt, x, y, mask=None, epochs=200, batch=512, lr=3e-3, decay=1e-4, seed=
 
 # ------------------------------------------------------------------ selector
 class Selector:
-    """Trained network (raw-feature weights) and per-tier unit costs (ms per
-    unit of work, see unit_costs)."""
+    """Trained network (raw-feature weights), per-choice unit costs (ms per
+    unit of work, see unit_costs: the eight tiers, then the plain path) and
+    the measured latency of a plain-path fallback solve per order."""
 
-    def __init__(self, params, tier_cost):
+    def __init__(self, params, choice_cost, fallback_orders=None, fallback_ms=None):
         self.params = np.asarray(params, dtype=np.float32)
-        self.tier_cost = np.asarray(tier_cost, dtype=np.float64)
+        self.choice_cost = np.asarray(choice_cost, dtype=np.float64)
+        assert self.choice_cost.shape == (len(CHOICES),)
+        self.fallback_orders = (np.array([1.0]) if fallback_orders is None
+                                else np.asarray(fallback_orders, dtype=np.float64))
+        self.fallback_ms = (np.array([0.0]) if fallback_ms is None
+                            else np.asarray(fallback_ms, dtype=np.float64))
+        self._p16 = None
 
     @classmethod
     def load(cls, path=WEIGHTS):
         with np.load(path, allow_pickle=False) as z:
-            return cls(z["params"], z["tier_cost"])
+            lat = (z["fallback_orders"], z["fallback_ms"]) if "fallback_ms" in z.files else (None, None)
+            return cls(z["params"], z["choice_cost"], *lat)
 
     def save(self, path=WEIGHTS, **extra):
         path = pathlib.Path(path)
         path.parent.mkdir(parents=True, exist_ok=True)
-        np.savez(path, params=self.params, tier_cost=self.tier_cost, **extra)
+        np.savez(path, params=self.params, choice_cost=self.choice_cost,
+                 fallback_orders=self.fallback_orders, fallback_ms=self.fallback_ms, **extra)
+
+    def latency(self, orders):
+        """Measured latency (ms) of the plain-path solve of one matrix, per
+        order (linear interpolation over the measured grid)."""
+        return np.interp(np.asarray(orders, dtype=np.float64), self.fallback_orders,
+                         self.fallback_ms)
+
+    def bf16_params(self, rt, dev):
+        """The weights as BF16 codes on the device (encoded once, by the
+        library's BF16 codec)."""
+        import torch
+
+        if self._p16 is None or self._p16.device != dev:
+            P = torch.from_numpy(self.params).to(dev)
+            p16 = torch.zeros(capi.MLP_PARAMS, dtype=torch.int16, device=dev)
+            rt.quantize(capi.BF16, P, p16, None)
+            self._p16 = p16
+        return self._p16
 
     def probabilities(self, rt, orders, matrices):
-        """P(tier k succeeds) per matrix, (count, 8), from device matrices."""
+        """P(tier k succeeds) per matrix, (count, 8), from device matrices:
+        features, then BF16 inference."""
         import torch
 
         count = len(orders)
         dev = matrices.device
         feats = torch.zeros(count * capi.MLP_INPUTS, dtype=torch.float32, device=dev)
         rt.matrix_features(orders, matrices, feats)
-        P = torch.from_numpy(self.params).to(dev)
         logits = torch.zeros(count * capi.MLP_OUTPUTS, dtype=torch.float32, device=dev)
-        rt.mlp_forward(P, feats, logits)
+        rt.mlp_forward_bf16(self.bf16_params(rt, dev), feats, logits)
         z = logits.cpu().numpy().reshape(count, capi.MLP_OUTPUTS).astype(np.float64)
         return 1.0 / (1.0 + np.exp(-z))
 
-    def choose(self, prob, orders=None, fallback_weight=FALLBACK_WEIGHT, candidates=None):
-        """Tier per matrix minimising c_t + (1 - p_t) F c_plain (tier 0: p = 1),
-        over `candidates` (tier indices; all tiers when None; the plain tier
-        is always a candidate)."""
-        p = np.array(prob, dtype=np.float64)
-        p[:, 0] = 1.0
-        expected = self.tier_cost[None, :] + (1.0 - p) * fallback_weight * self.tier_cost[0]
+    def expected(self, prob, orders, fallback_weight=FALLBACK_WEIGHT):
+        """Expected time (ms) of every choice for every matrix, (count, 9):
+        w c_t + (1 - p_t) (F L(n) + w c_plain) for the tiers, w c_plain for
+        the plain path."""
+        w = work_units(orders)[:, None]
+        p = np.ones((prob.shape[0], len(CHOICES)))
+        p[:, :PLAIN] = prob
+        c = self.choice_cost[None, :]
+        lat = fallback_weight * self.latency(orders)[:, None]
+        return w * c + (1.0 - p) * (lat + w * c[:, PLAIN:PLAIN + 1])
+
+    def choose(self, prob, orders, fallback_weight=FALLBACK_WEIGHT, candidates=None):
+        """Choice per matrix (0..7 a tier, PLAIN the plain path) minimising
+        the expected time, over `candidates` (choice indices; all when None;
+        the plain path is always a candidate)."""
+        expected = self.expected(prob, orders, fallback_weight)
         if candidates is not None:
-            allowed = np.zeros(len(self.tier_cost), dtype=bool)
+            allowed = np.zeros(len(CHOICES), dtype=bool)
             allowed[list(candidates)] = True
-            allowed[0] = True
+            allowed[PLAIN] = True
             expected = np.where(allowed[None, :], expected, np.inf)
         return np.argmin(expected, axis=1)
 
-    def options(self, rt, orders, matrices, candidates=None):
-        tiers = self.choose(self.probabilities(rt, orders, matrices), orders,
-                            candidates=candidates)
-        opts = [tier_options(int(k)) for k in range(len(TIERS))]
-        return [opts[int(k)] for k in tiers], tiers
+    def choose_batch(self, prob, orders, fallback_weight=FALLBACK_WEIGHT):
+        """Per-matrix choices for one batched solve: the fixed tier for a
+        batch expected to pass on it; otherwise formats chosen for less than
+        GROUP_SHARE_MIN of the work are dropped, and a single choice is taken
+        when it is expected within SINGLE_MARGIN of the per-matrix ones."""
+        if float(np.sum(1.0 - prob[:, FIXED])) < GATE_EXPECTED_FAILURES:
+            return np.full(prob.shape[0], FIXED)
+        w = work_units(orders)
+        expected = self.expected(prob, orders, fallback_weight)
+        allowed = list(range(len(CHOICES)))
+        while True:
+            mask = np.isin(np.arange(len(CHOICES)), allowed)[None, :]
+            choice = np.argmin(np.where(mask, expected, np.inf), axis=1)
+            share = {}
+            for k, wk in zip(choice, w):
+                f = choice_format(int(k))
+                share[f] = share.get(f, 0.0) + wk
+            total = float(np.sum(w))
+            small = [f for f, v in share.items()
+                     if v < GROUP_SHARE_MIN * total and len(share) > 1]
+            if not small:
+                break
+            drop = min(small, key=lambda f: share[f])
+            allowed = [k for k in allowed if choice_format(k) != drop or k == PLAIN]
+        per_matrix = float(np.sum(expected[np.arange(len(choice)), choice]))
+        single = np.sum(expected, axis=0)
+        best = int(np.argmin(single))
+        if single[best] <= (1.0 + SINGLE_MARGIN) * per_matrix:
+            return np.full(len(choice), best)
+        return choice
 
 
-# Tiers the batched solve runs as one precision group: BF16 basis and
-# corrections at stop rates 0.5 to 4.  A second low-bit precision group (FP8,
-# MXFP4) costs more in one batched solve than the per-matrix cost model
-# predicts, because each group adds its own launches to every correction step.
-# Measured on MI355X, in the same process on the same batches:
-#   * 1000-matrix mixture, orders 32..512: 364.7 ms against 378.4 ms for the
-#     fixed BF16 tier, with 17 fallbacks against 45; all eight tiers give
-#     376.0 ms;
-#   * 4096-matrix uneven batch: 89.1 ms against 75.8 ms.  The fixed tier
-#     never fails there, so the features and the matrices sent to the plain
-#     path are pure cost.
-BATCH_CANDIDATES = (1, 2, 3, 4)
-
-
-# Batch gate: when the network expects less than this fraction of the batch
-# to fail on the gentlest BF16 tier (mean of 1 - p), the whole batch runs
-# that tier.  The few fallback re-solves it may save cost less than routing
-# matrices to other tiers adds to the batched solve.  Measured: the uneven
-# 4096-matrix batch expects 0.3% (12.5 matrices; none fail) and gates to the
-# fixed tier; the 32..512 mixture expects 4.6% (45.9; 45 fail) and is routed
-# per matrix.
-GATE_FAILURE_FRACTION = 0.005
-
-
-def solve_selected(rt, selector, orders, matrices, roots, candidates=BATCH_CANDIDATES,
-                   gate=GATE_FAILURE_FRACTION):
+def solve_selected(rt, selector, orders, matrices, roots):
     """Selector-chosen tiers with the plain-path fallback.  Returns
-    (reports, fallback_indices, tiers)."""
+    (reports, fallback_indices, choices)."""
     prob = selector.probabilities(rt, orders, matrices)
-    if gate is not None and float(np.mean(1.0 - prob[:, 1])) < gate:
-        tiers = np.ones(len(orders), dtype=np.int64)
+    choice = selector.choose_batch(prob, orders)
+    opts_all = [tier_options(k) for k in range(len(CHOICES))]
+    if np.all(choice == choice[0]):  # one record for the whole batch
+        opts = opts_all[int(choice[0])]
     else:
-        tiers = selector.choose(prob, orders, candidates=candidates)
-    opts_all = [tier_options(int(k)) for k in range(len(TIERS))]
-    if np.all(tiers == tiers[0]):  # one record for the whole batch
-        opts = opts_all[int(tiers[0])]
-    else:
-        opts = [opts_all[int(k)] for k in tiers]
+        opts = [opts_all[int(k)] for k in choice]
     reports, fallback = MS.solve_with_fallback(rt, orders, matrices, roots, opts)
-    return reports, fallback, tiers
+    return reports, fallback, choice

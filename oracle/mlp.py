@@ -11,6 +11,11 @@ numpy so that the GPU kernels can be checked:
                    (c I - A), c = 1.0001 lambda_max, from a fixed hash start
                    vector; every other feature from A in binary64);
 * ``forward``   -- logits and hidden activations;
+* ``forward_bf16`` -- the BF16 inference kernel: weights, inputs and hidden
+                   activations rounded to BF16 (nearest even), float32
+                   accumulation in index order (each BF16 x BF16 product is
+                   exact in float32, so a fused multiply-add is one rounded
+                   float32 addition);
 * ``backward``  -- the gradient of the masked BCE loss;
 * ``adam``      -- one bias-corrected Adam step.
 
@@ -110,6 +115,35 @@ def forward(params, x):
     h2 = np.maximum(h1 @ p["W2"].T + p["b2"], 0.0)
     z = h2 @ p["W3"].T + p["b3"]
     return z, h1, h2
+
+
+def _bf16(v):
+    """Round float32 values to BF16 (nearest even; NaN quieted) as float32."""
+    b = np.asarray(v, np.float32).view(np.uint32).astype(np.uint64)
+    r = b >> np.uint64(16)
+    rem = b & np.uint64(0xFFFF)
+    up = (rem > 0x8000) | ((rem == 0x8000) & ((r & np.uint64(1)) == 1))
+    r = r + up.astype(np.uint64)
+    nan = ((b & np.uint64(0x7F800000)) == 0x7F800000) & ((b & np.uint64(0x7FFFFF)) != 0)
+    r = np.where(nan, (b >> np.uint64(16)) | np.uint64(0x40), r)
+    return (r.astype(np.uint32) << np.uint32(16)).view(np.float32)
+
+
+def forward_bf16(params, x):
+    """Logits (float32) of the BF16 inference kernel for inputs x (batch x 16)."""
+    p = {k: _bf16(np.asarray(v, np.float32)) for k, v in unpack(np.asarray(params, np.float32)).items()}
+    x = _bf16(np.asarray(x, np.float32))
+    f32 = np.float32
+
+    def layer(inp, W, b):
+        acc = np.broadcast_to(b, (inp.shape[0], b.size)).astype(f32).copy()
+        for i in range(inp.shape[1]):
+            acc = (acc + (inp[:, i:i + 1] * W[None, :, i]).astype(f32)).astype(f32)
+        return acc
+
+    h1 = _bf16(np.maximum(layer(x, p["W1"], p["b1"]), f32(0)))
+    h2 = _bf16(np.maximum(layer(h1, p["W2"], p["b2"]), f32(0)))
+    return layer(h2, p["W3"], p["b3"])
 
 
 def loss(params, x, y, m):

@@ -85,6 +85,29 @@ def test_forward_and_backward_match_oracle(runtime, b):
     np.testing.assert_allclose(host(losses).sum() / m.sum(), lo, rtol=1e-4)
 
 
+@pytest.mark.parametrize("b", [1, 300, 5000])
+def test_forward_bf16_matches_oracle(runtime, b):
+    """BF16 inference (weights, inputs and activations in BF16, float
+    accumulation in index order) bit for bit against the oracle, and within
+    BF16 rounding of the float32 network."""
+    p, x, _, _ = _batch(50 + b, b)
+    x[: min(b, 7)] *= 1e3  # large inputs: rounding ties and saturating sums
+    p16 = torch.zeros(M.PARAMS, dtype=torch.int16, device="cuda")
+    runtime.quantize(capi.BF16, dev(p), p16, None)
+    z = torch.zeros(b * M.OUTPUTS, device="cuda")
+    runtime.mlp_forward_bf16(p16, dev(x), z)
+    got = host(z).reshape(b, M.OUTPUTS)
+    want = M.forward_bf16(p, x)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # Unscaled rows: within BF16 rounding of the float32 network.
+    k = min(b, 7)
+    if b > k:
+        z32, _, _ = M.forward(p.astype(np.float64), x[k:].astype(np.float64))
+        diff = np.abs(got[k:] - z32).max() / np.abs(z32).max()
+        say("bf16 vs float32 network: max difference / max |z|", float(diff))
+        assert diff < 0.05
+
+
 def test_backward_is_deterministic(runtime):
     p, x, y, m = _batch(31, 300)
     P, X, Y, Mk = dev(p), dev(x), dev(y), dev(m)
@@ -129,24 +152,30 @@ def test_training_fits_a_learnable_rule(runtime):
 
 def test_trained_selector_beats_the_fixed_tier(runtime):
     """Held-out batch (mixture seed 13, never used in training): the committed
-    network's tier choices cost less than the fixed BF16 tier with the same
+    network's batch choices cost less than the fixed BF16 tier with the same
     plain-path fallback, in the modelled time of the batch's own measured
-    tier costs (every tier is run on the batch and timed)."""
+    choice costs (every choice is run on the batch and timed), and the BF16
+    inference agrees with the float32 network on the choices."""
     if not SEL.WEIGHTS.exists():
         pytest.skip("no trained selector weights")
     sel = SEL.Selector.load()
     mats = list(G.mixture(count=768, config=13, order_range=(32, 256)))
     say("selector held-out batch", len(mats))
-    x, success, sweeps, corr, orders, tier_ms = SEL.label_batch(runtime, mats, log=say)
+    x, success, sweeps, corr, orders, choice_ms = SEL.label_batch(runtime, mats, log=say)
+    o, flat = capi.pack_batch([m["A"] for m in mats])
+    prob = sel.probabilities(runtime, o, dev(flat))
     logits = torch.zeros(len(mats) * M.OUTPUTS, device="cuda")
     runtime.mlp_forward(dev(sel.params), dev(x), logits)
-    prob = 1.0 / (1.0 + np.exp(-host(logits).reshape(len(mats), -1).astype(np.float64)))
-    choice = sel.choose(prob)
-    uc = SEL.unit_costs(tier_ms, orders)
-    c_sel, f_sel = SEL.policy_cost(choice, success, uc, orders)
-    c_fix, f_fix = SEL.policy_cost(np.ones(len(mats), int), success, uc, orders)
-    c_plain, _ = SEL.policy_cost(np.zeros(len(mats), int), success, uc, orders)
+    p32 = 1.0 / (1.0 + np.exp(-host(logits).reshape(len(mats), -1).astype(np.float64)))
+    agree = np.mean(sel.choose(prob, o) == sel.choose(p32, o))
+    choice = sel.choose_batch(prob, o)
+    uc = SEL.unit_costs(choice_ms, orders)
+    c_sel, f_sel = SEL.policy_cost(choice, success, uc, orders, sel.latency)
+    c_fix, f_fix = SEL.policy_cost(np.full(len(mats), SEL.FIXED), success, uc, orders, sel.latency)
+    c_plain, _ = SEL.policy_cost(np.full(len(mats), SEL.PLAIN), success, uc, orders, sel.latency)
     say("modelled ms: selector", c_sel, f_sel, "fixed", c_fix, f_fix, "plain", c_plain,
-        "choices", np.bincount(choice, minlength=8).tolist())
-    assert success[:, 0].mean() >= 0.99  # the fallback meets the target (FP32 range)
-    assert c_sel < c_fix < c_plain
+        "choices", np.bincount(choice, minlength=len(SEL.CHOICES)).tolist(),
+        "bf16/fp32 inference agreement", agree)
+    assert success[:, SEL.PLAIN].mean() >= 0.99  # the fallback meets the target
+    assert agree >= 0.95
+    assert c_sel <= c_fix < c_plain

@@ -118,22 +118,97 @@ def test_fold_standardisation_equals_standardised_inputs():
 
 def test_choice_minimises_expected_cost_and_policy_cost():
     from msq import selector as SEL
-    cost = np.array([10.0, 7.0, 6.0, 5.0, 4.0, 3.0, 7.5, 7.5])
-    sel = SEL.Selector(np.zeros(M.PARAMS, np.float32), cost)
-    prob = np.array([[0.0, 1.0, 1.0, 1.0, 1.0, 0.2, 1.0, 1.0],    # r4 sure, r8 unlikely
+    # eight tiers, then the plain path (index 8, always succeeds); unit costs
+    # in ms per (n / 256)^3; a failed choice pays L(n) = 20 ms plus the plain
+    # path's share again.
+    cost = np.array([3.0, 4.0, 5.0, 6.0, 6.0, 7.0, 7.5, 7.5, 10.0])
+    sel = SEL.Selector(np.zeros(M.PARAMS, np.float32), cost, [1, 1000], [20.0, 20.0])
+    orders = [256, 256]
+    prob = np.array([[0.2, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0],    # tier 1 sure, 0 unlikely
                      [0.0, 0.3, 0.1, 0.0, 0.0, 0.0, 0.0, 0.0]])   # nothing likely: plain
-    ch = sel.choose(prob, fallback_weight=1.0)
-    assert ch.tolist() == [4, 0]
-    # A heavier fallback weight turns away from the uncertain tiers only.
-    assert sel.choose(prob, fallback_weight=10.0).tolist() == [4, 0]
-    prob[0, 4] = 0.95
-    assert sel.choose(prob, fallback_weight=1.0).tolist() == [4, 0]
-    assert sel.choose(prob, fallback_weight=10.0).tolist() == [3, 0]
-    success = np.array([[1, 1, 1, 1, 1, 0, 1, 1], [1, 0, 0, 0, 0, 0, 0, 0]], dtype=float)
-    # Orders 256 and 128: work units 1 and 1/8.
-    total, fails = SEL.policy_cost(np.array([5, 1]), success, cost, [256, 128])
-    assert fails == 2 and total == (3 + 10) + (7 + 10) / 8
-    total, fails = SEL.policy_cost(ch, success, cost, [256, 128])
-    assert fails == 0 and total == 4 + 10 / 8
-    # Unit costs: batch time over total work.
+    assert sel.choose(prob, orders).tolist() == [1, SEL.PLAIN]
+    e = sel.expected(prob, orders)
+    assert e[0, 0] == 3.0 + 0.8 * (20.0 + 10.0) and e[0, 1] == 4.0 and e[1, SEL.PLAIN] == 10.0
+    prob[0, 1] = 0.97          # 4 + 0.03 * 30 = 4.9 < 5: still tier 1
+    assert sel.choose(prob, orders).tolist() == [1, SEL.PLAIN]
+    prob[0, 1] = 0.95          # 4 + 1.5 > 5: tier 2
+    assert sel.choose(prob, orders).tolist() == [2, SEL.PLAIN]
+    # candidates restrict the tiers; the plain path stays a candidate
+    assert sel.choose(prob, orders, candidates=[6]).tolist() == [6, SEL.PLAIN]
+    # a small matrix (n = 128, work 1/8): the latency term dominates
+    assert sel.choose(np.array([[0.9] * 8]), [128]).tolist() == [SEL.PLAIN]
+    success = np.array([[0, 1, 1, 1, 1, 1, 1, 1, 1], [0, 0, 0, 0, 0, 0, 0, 0, 1]], dtype=float)
+    # Orders 256 and 128: work units 1 and 1/8; failures pay the plain path
+    # in one second call of latency max L.
+    total, fails = SEL.policy_cost(np.array([0, 1]), success, cost, [256, 128])
+    assert fails == 2 and total == (3 + 10) + (4 + 10) / 8
+    total, fails = SEL.policy_cost(np.array([0, 1]), success, cost, [256, 128],
+                                   lambda o: np.asarray(o) / 10.0)
+    assert total == (3 + 10) + (4 + 10) / 8 + 25.6
+    total, fails = SEL.policy_cost(np.array([2, SEL.PLAIN]), success, cost, [256, 128])
+    assert fails == 0 and total == 5 + 10 / 8
     np.testing.assert_allclose(SEL.unit_costs([9.0, 4.5], [256, 128]), [8.0, 4.0])
+
+
+def test_batch_choice_drops_small_format_groups_and_prefers_a_single_choice():
+    from msq import selector as SEL
+    cost = np.array([1.0, 1.0, 1.0, 1.0, 2.0, 2.0, 3.0, 3.0, 6.0])
+    sel = SEL.Selector(np.zeros(M.PARAMS, np.float32), cost)
+    n = 100
+    orders = [256] * n
+    prob = np.zeros((n, 8))
+    prob[:, 4] = 1.0           # the fixed (BF16) tier always works: the gate
+    prob[:40, 0] = 1.0
+    assert (sel.choose_batch(prob, orders) == SEL.FIXED).all()
+    prob[:, 4] = 0.99          # one failure expected on the fixed tier
+    prob[:40, 0] = 0.0
+    prob[:3, 0] = 1.0          # MXFP4 works for 3% of the work only
+    ch = sel.choose(prob, orders)
+    assert (ch[:3] == 0).all() and (ch[3:] == 4).all()
+    # MXFP4's group (3% of the work) is dropped: one choice for the batch
+    assert (sel.choose_batch(prob, orders) == 4).all()
+    # 40% MXFP4: a saving beyond the single-choice margin, kept per matrix
+    prob[:40, 0] = 1.0
+    bc = sel.choose_batch(prob, orders)
+    assert (bc[:40] == 0).all() and (bc[40:] == 4).all()
+
+
+def test_forward_bf16_rounding_and_closed_forms():
+    """Pins of the BF16 forward: the BF16 rounding of float32 values (ties to
+    even, NaN quieted, exhaustive against the codec oracle on a sample); an
+    exact network on small integers; and the rounding of a hidden activation
+    at 1 + 2^-9 (a tie between BF16 1 and 1 + 2^-8: rounds to 1)."""
+    from oracle import codecs as C
+    rng = np.random.default_rng(8)
+    v = rng.standard_normal(20000).astype(np.float32) * np.float32(1e3)
+    v[:4] = [np.nan, np.inf, -0.0, 1.0 + 2.0 ** -9]
+    codes, _ = C.encode("bf16", v)
+    want = C.decode("bf16", codes, None, v.size)
+    got = M._bf16(v)
+    assert np.array_equal(got.view(np.uint32), np.asarray(want, np.float32).view(np.uint32))
+    # W1 = identity on the first 16 hidden units, W2 = identity, W3 sums the
+    # first 16 hidden units into output 0; integer inputs: exact.
+    parts = {k: np.zeros(s) for k, s in M.SIZES}
+    parts["W1"][:16, :16] = np.eye(16)
+    parts["W2"][:16, :16] = np.eye(16)
+    parts["W3"][0, :16] = 1.0
+    parts["b3"][1] = 3.0
+    params = M.pack(parts).astype(np.float32)
+    x = np.arange(32, dtype=np.float32).reshape(2, 16) - 8.0
+    z = M.forward_bf16(params, x)
+    relu = np.maximum(x, 0)
+    assert z[0, 0] == relu[0].sum() and z[1, 0] == relu[1].sum() and np.all(z[:, 1] == 3.0)
+    parts = {k: np.zeros(s) for k, s in M.SIZES}
+    parts["b1"][0] = 1.0 + 2.0 ** -9   # bias rounds to 1 in BF16
+    parts["W2"][0, 0] = 1.0
+    parts["W3"][0, 0] = 1.0
+    z = M.forward_bf16(M.pack(parts).astype(np.float32), np.zeros((1, 16), np.float32))
+    assert z[0, 0] == 1.0
+    # float32 network with the same (representable) weights differs only by
+    # the activations' BF16 rounding
+    p = (rng.standard_normal(M.PARAMS) * 0.3).astype(np.float32)
+    p = M._bf16(p)
+    x = M._bf16(rng.standard_normal((64, 16)).astype(np.float32))
+    z16 = M.forward_bf16(p, x)
+    z64, _, _ = M.forward(p.astype(np.float64), x.astype(np.float64))
+    assert np.max(np.abs(z16 - z64)) <= 0.05 * (np.abs(z64).max() + 1)
