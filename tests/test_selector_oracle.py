@@ -160,7 +160,7 @@ def test_batch_choice_drops_small_format_groups_and_prefers_a_single_choice():
     prob[:, 4] = 1.0           # the fixed (BF16) tier always works: the gate
     prob[:40, 0] = 1.0
     assert (sel.choose_batch(prob, orders) == SEL.FIXED).all()
-    prob[:, 4] = 0.99          # one failure expected on the fixed tier
+    prob[:, 4] = 0.99          # 1% expected to fail on the fixed tier
     prob[:40, 0] = 0.0
     prob[:3, 0] = 1.0          # MXFP4 works for 3% of the work only
     ch = sel.choose(prob, orders)
