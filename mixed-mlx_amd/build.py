@@ -24,7 +24,7 @@ ARCH = os.environ.get("MSQ_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-fno-gpu-flush-denormals-to-zero", "-Wall", "-Wno-unused-function",
+    "-fno-gpu-flush-denormals-to-zero", "-Wall", "-Werror", "-Wno-unused-function",
 ]
 
 SOURCES = ["kernels_codec.hip", "kernels_jacobi.hip", "kernels_gemm.hip",
