@@ -35,10 +35,11 @@ fallback rate of each; no roofline leg.
 the batch when all pass), per second of the low-bit path.
 
 Also reported:
-  * the roofline of the dominant kernel (batched_jacobi_kernel<256> on 256 of
-    the batch's matrices of orders 241..256, one per CU, with the reference's
-    fixed 8 sweeps, run on its own and timed with HIP events on the library's
-    stream; ``--roofline-only`` runs just these launches, for the profiler).
+  * the roofline of the dominant kernel (the Jacobi of orders 129..256,
+    persistent_jacobi_kernel, on 256 of the batch's matrices of orders
+    241..256, one per CU, with the reference's fixed 8 sweeps, run on its own
+    and timed with HIP events on the library's stream; ``--roofline-only``
+    runs just these launches, for the profiler).
     Its HBM traffic comes from the rocprofv3 PMC summary in profiles/, used
     only when that summary was taken on the kernel source being run (the
     SHA-256 of kernels_jacobi.hip recorded there);
@@ -119,12 +120,13 @@ ROOFLINE_SWEEPS = 8      # the reference's fixed sweep count (src/metal_solver.c
 
 
 def roofline_leg(args, mats, orders, rt, stream, torch, np, capi, dev):
-    """Dominant kernel: batched_jacobi_kernel<256> (one workgroup per matrix,
-    the whole solve in LDS) on 256 of the batch's matrices of orders 241..256
-    -- one per CU, a single wave of workgroups -- with the reference's fixed 8
-    sweeps, so that every workgroup does the same work and the launch time is
-    the kernel's time per matrix.  In the batched solve the same kernel stops
-    per matrix and shares the machine with the other order classes."""
+    """Dominant kernel: the Jacobi of orders 129..256 (persistent_jacobi_kernel:
+    one workgroup per CU, the whole solve of a matrix in LDS) on 256 of the
+    batch's matrices of orders 241..256 -- one per CU, so every workgroup takes
+    one job -- with the reference's fixed 8 sweeps, so that every workgroup
+    does the same work and the launch time is the kernel's time per matrix.
+    In the batched solve the same kernel takes the jobs of every order 129..256
+    and stops per matrix."""
     big = [i for i, n in enumerate(orders) if n > 240][:ROOFLINE_MATRICES]
     sub_orders = [orders[i] for i in big]
     _, sub_flat = capi.pack_batch([mats[i]["A"] for i in big])
@@ -153,7 +155,7 @@ def roofline_leg(args, mats, orders, rt, stream, torch, np, capi, dev):
             "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
             "traffic": traffic if len(mats) == 4096 else None,
             "kernel_source_sha256": digest,
-            "kernel": (f"batched_jacobi_kernel<256> ({len(big)} matrices of orders 241..256, "
+            "kernel": (f"persistent_jacobi_kernel ({len(big)} matrices of orders 241..256, "
                        f"one per CU, {ROOFLINE_SWEEPS} fixed sweeps; "
                        f"{flops / 1e12:.3f} TFLOP per launch at 9 S n^2 (n - 1), "
                        f"{jt * 1e3:.2f} ms mean of {len(durations)} launches)")}
@@ -321,6 +323,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rate", type=float, default=0.5)
     ap.add_argument("--roofline-only", action="store_true")
+    ap.add_argument("--profile-path", choices=["plain", "fixed", "learned"], default=None,
+                    help="time only this path (warm-up and steps), for the profiler")
     ap.add_argument("--roofline-reps", type=int, default=5)
     ap.add_argument("--no-selector", action="store_true",
                     help="fixed tier instead of the learned tier selector")
@@ -417,6 +421,19 @@ def main():
     plain = MS.plain_fp32_options()
     if args.roofline_only:
         log("roofline", roofline_leg(args, mats, orders, rt, stream, torch, np, capi, dev))
+        rt.close()
+        return
+    if args.profile_path:
+        if args.profile_path == "plain":
+            fn = lambda: rt.solve_batched(orders, A, X, plain)  # noqa: E731
+        elif args.profile_path == "fixed":
+            opts = MS.tier_options(args.tier, rate=args.rate, max_corrections=12)
+            fn = lambda: MS.solve_with_fallback(rt, orders, A, X, opts)  # noqa: E731
+        else:
+            sel = SEL.Selector.load()
+            fn = lambda: SEL.solve_selected(rt, sel, orders, A, X)  # noqa: E731
+        dt, _ = timed(fn)
+        log(f"{args.profile_path}: {dt * 1e3:.1f} ms/step")
         rt.close()
         return
     dt_plain, rep_plain = timed(lambda: rt.solve_batched(orders, A, X, plain))
