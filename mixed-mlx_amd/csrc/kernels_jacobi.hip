@@ -34,6 +34,8 @@
 // where index p sees (c, -s) and index q sees (c, s).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <math.h>
 
 #include <type_traits>
@@ -1208,12 +1210,30 @@ hipError_t launch_order(const JacobiJob* jobs, uint32_t count, int threads, bool
   return hipGetLastError();
 }
 
+// CUs of the current device (queried once per device; 256 on MI355X).
+uint32_t device_cu_count() {
+  constexpr int kMaxDevices = 64;
+  static std::atomic<uint32_t> cache[kMaxDevices];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return 256u;
+  uint32_t cu = cache[dev].load(std::memory_order_relaxed);
+  if (cu == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        v <= 0)
+      return 256u;
+    cu = static_cast<uint32_t>(v);
+    cache[dev].store(cu, std::memory_order_relaxed);
+  }
+  return cu;
+}
+
 // Wide shape when every workgroup of the launch is resident at once: the
 // kernels of these orders use about 100 VGPRs, 20 waves per CU.
 bool use_wide_shape(int N, uint32_t count) {
   if (!has_wide_shape(N)) return false;
   const uint32_t waves = static_cast<uint32_t>(threads_for_order(N, true) / 64);
-  return count * waves <= 256u * 20u;
+  return count * waves <= device_cu_count() * 20u;
 }
 
 }  // namespace
