@@ -355,8 +355,10 @@ int msq_decode_fp8_scaled(msq_runtime *runtime, const void *values, const float 
  * (float, batch layout) receive V, diagonal and roots (float, vector layout)
  * receive diag(V^T A V) and its square roots.  options has 1 or count
  * entries; reports (host, count entries) may be NULL.  Orders up to 256 run
- * one workgroup per matrix in one launch; larger orders run the round-by-round
- * device-memory solver, one matrix at a time. */
+ * one workgroup per matrix, all sweeps in LDS (orders 129..256 as one
+ * persistent launch, smaller orders one launch per padded order); larger
+ * orders run the blocked solver (64 x 64 pair subproblems, rotations applied
+ * as MFMA products), the matrices of one padded order advancing together. */
 int msq_jacobi_batched(msq_runtime *runtime, uint32_t count, const uint32_t *orders,
                        const double *matrices, float *vectors, float *diagonal,
                        float *roots, const msq_jacobi_options *options,

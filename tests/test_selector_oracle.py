@@ -63,6 +63,70 @@ def test_adam_first_step_is_signed_learning_rate():
     np.testing.assert_allclose(p, want, rtol=1e-12)
 
 
+def test_adam_decay_enters_the_gradient():
+    """Weight decay d adds d p to the gradient before the moments: the first
+    bias-corrected step is -lr sign(g + d p) (chosen so that every sign
+    differs from sign(g)); the second is -lr m1_hat / sqrt(m2_hat) with the
+    moments of the two decayed gradients, in exact arithmetic."""
+    from fractions import Fraction as Fr
+    p0 = np.array([1.0, -2.0, 0.5])
+    g = np.array([-0.5, 0.5, -0.375])
+    decay, lr = 1.0, 0.125
+    p1, m1, m2 = M.adam(p0, g, np.zeros(3), np.zeros(3), lr=lr, step=1, eps=0.0, decay=decay)
+    assert np.all(np.sign(g + decay * p0) != np.sign(g))
+    np.testing.assert_allclose(p1, p0 - lr * np.sign(g + decay * p0), rtol=1e-15)
+    g2 = np.array([0.25, -0.25, 0.5])
+    p2, _, _ = M.adam(p1, g2, m1, m2, lr=lr, step=2, eps=0.0, decay=decay)
+    b1, b2 = Fr(9, 10), Fr(999, 1000)
+    for i in range(3):
+        ga = Fr(g[i]) + Fr(decay) * Fr(p0[i])
+        gb = Fr(g2[i]) + Fr(decay) * Fr(p1[i])
+        a = b1 * (1 - b1) * ga + (1 - b1) * gb
+        v = b2 * (1 - b2) * ga * ga + (1 - b2) * gb * gb
+        want = float(Fr(p1[i])) - lr * float(a / (1 - b1 * b1)) / np.sqrt(float(v / (1 - b2 * b2)))
+        assert p2[i] == pytest.approx(want, rel=1e-13)
+
+
+def test_feature_8_is_the_spread_of_the_log_diagonal():
+    """f8 = standard deviation of log10 diag(A): half the diagonal 1, half 100
+    gives exactly 1; a constant diagonal gives 0."""
+    A = np.diag([1.0, 100.0, 1.0, 100.0]) + 0.01 * (np.ones((4, 4)) - np.eye(4))
+    assert M.features(A)[8] == pytest.approx(1.0, abs=1e-14)
+    assert M.features(5.0 * np.eye(6) + 0.1 * np.ones((6, 6)))[8] == pytest.approx(0.0, abs=1e-7)
+
+
+def test_features_13_14_power_step_changes_in_closed_form():
+    """f13 / f14 = log10 of the relative change of the last power step.  For
+    a diagonal matrix the Rayleigh quotients of both iterations are exact
+    rationals: q_k = sum_i w_i e_i^(2k+1) / sum_i w_i e_i^(2k) with e = d,
+    w = 1 (lambda_max, 16 steps from the ones vector) and e = c - d,
+    w = s_i^2 (lambda_min, 24 steps on c I - A from the hash start s,
+    c = 1.0001 lambda_max)."""
+    from fractions import Fraction as Fr
+    d = [2.0, 1.0, 0.875]  # exact in float32
+    f = M.features(np.diag(d))
+
+    def quotients(e, w, steps):
+        q = []
+        for k in range(steps):
+            num = sum(wi * ei ** (2 * k + 1) for ei, wi in zip(e, w))
+            den = sum(wi * ei ** (2 * k) for ei, wi in zip(e, w))
+            q.append(num / den)
+        return q
+
+    e_max = [Fr(x) for x in d]
+    q = quotients(e_max, [Fr(1)] * 3, M.POWER_MAX)
+    lmax = q[-1]
+    want13 = np.log10(float(abs(q[-1] - q[-2]) / lmax))
+    assert f[13] == pytest.approx(want13, abs=1e-6)
+    c = Fr(1.0001) * lmax
+    s = [Fr(x) for x in M.start_vector(3)]
+    mu = quotients([c - x for x in e_max], [si * si for si in s], M.POWER_MIN)
+    want14 = np.log10(float(abs(mu[-1] - mu[-2]) / abs(mu[-1])))
+    assert -6.0 < want14 < -1.0  # a change the features resolve, not the 1e-30 floor
+    assert f[14] == pytest.approx(want14, abs=1e-6)
+
+
 def test_features_of_a_diagonal_matrix():
     # Separated extremes.  The lambda_max iteration contracts by 5 / 10 per
     # step: after 16 steps its Rayleigh quotient is within ~30 * 2^-32 of 10
