@@ -2724,12 +2724,12 @@ int msq_matrix_features(msq_runtime* rt, uint32_t count, const uint32_t* orders,
       return set_error(MSQ_ERROR_INVALID_ARGUMENT, "order %u above %d", L.order[i],
                        msq::kMaxFeatureOrder);
   // Segments grouped by order range, one launch per range: the ranges run
-  // concurrently on the runtime's lanes.
-  static_assert(kLanes >= 4, "one lane per order range");
-  const int kinds[4] = {256, 128, 64, 0};
+  // concurrently on the runtime's lanes (the fifth on the first lane).
+  constexpr int kKinds = 5;
+  const int kinds[kKinds] = {256, 192, 128, 64, 0};
   std::vector<uint32_t> members;
-  uint32_t bounds[5] = {0, 0, 0, 0, 0};
-  for (int k = 0; k < 4; ++k) {
+  uint32_t bounds[kKinds + 1] = {};
+  for (int k = 0; k < kKinds; ++k) {
     for (uint32_t i = 0; i < count; ++i)
       if (msq::feature_rows_for_order(L.order[i]) == kinds[k]) members.push_back(i);
     bounds[k + 1] = static_cast<uint32_t>(members.size());
@@ -2741,12 +2741,13 @@ int msq_matrix_features(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   MSQ_TRY(S.commit());
   MSQ_TRY(upload(rt, dsegs, segs));
   HIP_TRY(hipEventRecord(rt->fork, rt->stream));
-  for (int k = 0; k < 4; ++k) {
-    HIP_TRY(hipStreamWaitEvent(rt->lanes[k], rt->fork, 0));
+  for (int l = 0; l < kLanes; ++l) HIP_TRY(hipStreamWaitEvent(rt->lanes[l], rt->fork, 0));
+  for (int k = 0; k < kKinds; ++k)
     HIP_TRY(msq::launch_features(kinds[k], dsegs + bounds[k], bounds[k + 1] - bounds[k], matrices,
-                                 features, rt->lanes[k]));
-    HIP_TRY(hipEventRecord(rt->joins[k], rt->lanes[k]));
-    HIP_TRY(hipStreamWaitEvent(rt->stream, rt->joins[k], 0));
+                                 features, rt->lanes[k % kLanes]));
+  for (int l = 0; l < kLanes; ++l) {
+    HIP_TRY(hipEventRecord(rt->joins[l], rt->lanes[l]));
+    HIP_TRY(hipStreamWaitEvent(rt->stream, rt->joins[l], 0));
   }
   return MSQ_OK;
 }

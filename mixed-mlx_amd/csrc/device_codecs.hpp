@@ -11,8 +11,9 @@
 //   * MX: scale exponent clamp(ilogb(max finite |v|) - emax, -127, 128), stored
 //     biased by 127, elements encoded as v / 2^exponent (:148-198).
 // The reference finds the nearest code by a scan over all codes; here the same
-// result comes from integer round-half-even on the float's significand, which
-// needs no floating-point mode and is exact for subnormal inputs too.
+// result comes from round-half-even on the float's bit pattern (normal codes)
+// and rintf of an exact power-of-two multiple (subnormal codes): a few integer
+// operations and selects per element, no branches.
 #pragma once
 
 #include <math.h>
@@ -60,113 +61,76 @@ MSQ_HD float decode_bf16(uint16_t code) {
   return bits_f32(static_cast<uint32_t>(code) << 16);
 }
 
+// x * 2^k exactly as std::ldexp does for these few-bit values: the product is
+// representable unless it overflows (then inf); two steps keep 2^k finite.
+MSQ_HD float ldexp_exact(float x, int k) {
+  if (k > 127) {
+    x *= bits_f32(0x7f000000u);  // 2^127
+    k -= 127;
+  } else if (k < -126) {
+    x *= bits_f32(0x00800000u);  // 2^-126
+    k += 126;
+  }
+  return x * bits_f32(static_cast<uint32_t>(k + 127) << 23);
+}
+
 // ------------------------------------------------- minifloat round to nearest
-// Rounds the finite positive value sig * 2^e2 (sig < 2^24) to the code of a
-// minifloat with `mant` mantissa bits and exponent bias `bias`, where codes
-// count up monotonically with value: code = ((E + bias - 1) << mant) + k with
-// E = max(floor(log2 value), 1 - bias) and k = round_half_even(value /
-// 2^(E - mant)).  A carry out of k walks into the next binade correctly.
-MSQ_HD uint32_t minifloat_round(uint32_t sig, int e2, int mant, int bias) {
-  int msb = 31 - __builtin_clz(sig);       // sig != 0
-  int log2v = e2 + msb;
-  int E = log2v > 1 - bias ? log2v : 1 - bias;
-  int shift = (E - mant) - e2;  // value / 2^(E - mant) = sig / 2^shift
-  uint32_t k;
-  if (shift <= 0) {
-    k = sig << (-shift);
-  } else if (shift > 25) {
-    k = 0;  // value < 2^(E - mant - 1): below half the quantum
-  } else {
-    k = sig >> shift;
-    const uint32_t rem = sig & ((1u << shift) - 1u);
-    const uint32_t half = 1u << (shift - 1);
-    if (rem > half || (rem == half && (k & 1u))) ++k;
-  }
-  return (static_cast<uint32_t>(E + bias - 1) << mant) + k;
+// Round to nearest, ties to even, of v / 2^scale_exp to a minifloat whose
+// codes count up monotonically with value.  Normal codes: the float's
+// exponent and top mantissa bits, rounded in the integer domain (adding
+// half a quantum minus one plus the kept lowest bit carries exactly when the
+// dropped bits are above half, or at half with an odd kept bit; a carry walks
+// into the next binade), then rebased to the minifloat's exponent bias.
+// Subnormal codes: |v| as an integer multiple of the smallest subnormal,
+// rounded by rintf (ties to even; the product by a power of two is exact).
+// The scaling by 2^-scale_exp is exact wherever the result can be non-zero:
+// a scaled value below 2^-126 (where a float product could round) is far
+// below half the smallest subnormal of every format here, so its code is 0.
+MSQ_HD float scale_pow2(float v, int scale_exp) {
+  return scale_exp ? ldexp_exact(v, -scale_exp) : v;
 }
 
-// Splits |v| (finite, non-zero) into sig * 2^e2, applying an extra power of
-// two `scale_exp` exactly (value / 2^scale_exp).
-MSQ_HD void split_abs(uint32_t bits, int scale_exp, uint32_t &sig, int &e2) {
-  const uint32_t ef = (bits >> 23) & 0xffu;
-  const uint32_t mf = bits & 0x7fffffu;
-  if (ef == 0) {
-    sig = mf;
-    e2 = -149;
-  } else {
-    sig = mf | 0x800000u;
-    e2 = static_cast<int>(ef) - 150;
-  }
-  e2 -= scale_exp;
-}
-
-// E4M3FN of v / 2^scale_exp (scale_exp = 0 for the scalar codec).
+// E4M3FN of v / 2^scale_exp (scale_exp = 0 for the scalar codec): above 448
+// (infinity included) saturates to 0x7e, NaN is 0x7f, the sign is kept.
 MSQ_HD uint8_t encode_e4m3(float v, int scale_exp = 0) {
-  const uint32_t bits = f32_bits(v);
-  const uint8_t sign = (bits >> 31) ? 0x80u : 0u;
-  const uint32_t a = bits & 0x7fffffffu;
-  if (a > 0x7f800000u) return sign | 0x7fu;   // NaN
-  if (a == 0) return sign;                     // +-0
-  if (a == 0x7f800000u) return sign | 0x7eu;  // inf saturates
-  uint32_t sig;
-  int e2;
-  split_abs(bits, scale_exp, sig, e2);
-  // |v| / 2^s > 448  <=>  sig * 2^e2 > 7 * 2^6
-  const int msb = 31 - __builtin_clz(sig);
-  if (e2 + msb > 8) return sign | 0x7eu;
-  if (e2 + msb == 8) {
-    // value in [256, 512): compare with 448 = 0b111 << 6 exactly
-    const int sh = msb - 2;  // keep the top three bits
-    const uint32_t top = sig >> sh;
-    const bool above = top > 7u || (top == 7u && (sig & ((1u << sh) - 1u)));
-    if (above) return sign | 0x7eu;
-  }
-  return sign | static_cast<uint8_t>(minifloat_round(sig, e2, 3, 7));
+  const uint32_t sign = (f32_bits(v) >> 24) & 0x80u;
+  const uint32_t a = f32_bits(scale_pow2(v, scale_exp)) & 0x7fffffffu;
+  const uint32_t sub = a < 0x3c800000u ? a : 0x3c800000u;  // below 2^-6
+  uint32_t code = a >= 0x3c800000u ? ((a + 0x7ffffu + ((a >> 20) & 1u)) >> 20) - 960u
+                                   : static_cast<uint32_t>(rintf(bits_f32(sub) * 512.0f));
+  if (a > 0x43e00000u) code = 0x7eu;
+  if (a > 0x7f800000u) code = 0x7fu;
+  return static_cast<uint8_t>(sign | code);
 }
 
-// E5M2 of v / 2^scale_exp.
+// E5M2 of v / 2^scale_exp: finite values above 57344 saturate to 0x7b,
+// infinity stays 0x7c, NaN is 0x7e.
 MSQ_HD uint8_t encode_e5m2(float v, int scale_exp = 0) {
   const uint32_t bits = f32_bits(v);
-  const uint8_t sign = (bits >> 31) ? 0x80u : 0u;
-  const uint32_t a = bits & 0x7fffffffu;
-  if (a > 0x7f800000u) return sign | 0x7eu;   // NaN
-  if (a == 0) return sign;
-  if (a == 0x7f800000u) return sign | 0x7cu;  // inf stays inf
-  uint32_t sig;
-  int e2;
-  split_abs(bits, scale_exp, sig, e2);
-  // finite |v| / 2^s > 57344 = 0b111 << 13 saturates to 0x7b
-  const int msb = 31 - __builtin_clz(sig);
-  if (e2 + msb > 15) return sign | 0x7bu;
-  if (e2 + msb == 15) {
-    const int sh = msb - 2;
-    const uint32_t top = sig >> sh;
-    const bool above = top > 7u || (top == 7u && (sig & ((1u << sh) - 1u)));
-    if (above) return sign | 0x7bu;
-  }
-  return sign | static_cast<uint8_t>(minifloat_round(sig, e2, 2, 15));
+  const uint32_t sign = (bits >> 24) & 0x80u;
+  const uint32_t av = bits & 0x7fffffffu;
+  const uint32_t a = f32_bits(scale_pow2(v, scale_exp)) & 0x7fffffffu;
+  const uint32_t sub = a < 0x38800000u ? a : 0x38800000u;  // below 2^-14
+  uint32_t code = a >= 0x38800000u ? ((a + 0xfffffu + ((a >> 21) & 1u)) >> 21) - 448u
+                                   : static_cast<uint32_t>(rintf(bits_f32(sub) * 65536.0f));
+  if (a > 0x47600000u) code = 0x7bu;
+  if (av == 0x7f800000u) code = 0x7cu;
+  if (av > 0x7f800000u) code = 0x7eu;
+  return static_cast<uint8_t>(sign | code);
 }
 
-// FP4 E2M1 of v / 2^scale_exp (a nibble in the low four bits).
+// FP4 E2M1 of v / 2^scale_exp (a nibble in the low four bits): above 6
+// (infinity included) saturates to 0x7, NaN encodes as a signed zero.
 MSQ_HD uint8_t encode_e2m1(float v, int scale_exp = 0) {
   const uint32_t bits = f32_bits(v);
-  const uint8_t sign = (bits >> 31) ? 0x8u : 0u;
-  const uint32_t a = bits & 0x7fffffffu;
-  if (a > 0x7f800000u || a == 0) return sign;  // NaN and zero
-  if (a == 0x7f800000u) return sign | 0x7u;
-  uint32_t sig;
-  int e2;
-  split_abs(bits, scale_exp, sig, e2);
-  // > 6 = 0b11 << 1 saturates
-  const int msb = 31 - __builtin_clz(sig);
-  if (e2 + msb > 2) return sign | 0x7u;
-  if (e2 + msb == 2) {
-    const int sh = msb - 1;  // keep the top two bits
-    const uint32_t top = sig >> sh;
-    const bool above = top > 3u || (top == 3u && (sig & ((1u << sh) - 1u)));
-    if (above) return sign | 0x7u;
-  }
-  return sign | static_cast<uint8_t>(minifloat_round(sig, e2, 1, 1));
+  const uint32_t sign = (bits >> 28) & 0x8u;
+  const uint32_t a = f32_bits(scale_pow2(v, scale_exp)) & 0x7fffffffu;
+  const uint32_t sub = a < 0x3f800000u ? a : 0x3f800000u;  // below 1
+  uint32_t code = a >= 0x3f800000u ? ((a + 0x1fffffu + ((a >> 22) & 1u)) >> 22) - 252u
+                                   : static_cast<uint32_t>(rintf(bits_f32(sub) * 2.0f));
+  if (a > 0x40c00000u) code = 0x7u;
+  if ((bits & 0x7fffffffu) > 0x7f800000u) code = 0u;
+  return static_cast<uint8_t>(sign | code);
 }
 
 // ------------------------------------------------------------------ decoders
@@ -199,19 +163,6 @@ MSQ_HD float decode_e2m1(uint8_t nibble) {
   const uint32_t f = nibble & 0x1u;
   if (e == 0) return bits_f32(sign | (f ? 0x3f000000u : 0u));  // 0 or 0.5
   return bits_f32(sign | ((e + 126u) << 23) | (f << 22));
-}
-
-// x * 2^k exactly as std::ldexp does for these few-bit values: the product is
-// representable unless it overflows (then inf); two steps keep 2^k finite.
-MSQ_HD float ldexp_exact(float x, int k) {
-  if (k > 127) {
-    x *= bits_f32(0x7f000000u);  // 2^127
-    k -= 127;
-  } else if (k < -126) {
-    x *= bits_f32(0x00800000u);  // 2^-126
-    k += 126;
-  }
-  return x * bits_f32(static_cast<uint32_t>(k + 127) << 23);
 }
 
 // MX scale exponent from the largest finite magnitude's bit pattern (bit

@@ -107,8 +107,8 @@ constexpr int kPowerIterationsMin = 24;  // lambda_min: on (c I - A), c > lambda
 
 struct Segment;
 // Features of the segments of one order range: rows = 64 (orders 1..64),
-// 128 (65..128), 256 (129..256) or 0 (257..512); segments outside the range
-// are skipped.  Row sg.index of `features` receives segment sg's 16 values.
+// 128 (65..128), 192 (129..192), 256 (193..256) or 0 (257..512); segments
+// outside the range are skipped.  Row sg.index of `features` receives segment sg's 16 values.
 hipError_t launch_features(int rows, const Segment* segs, uint32_t count, const double* a64,
                            float* features, hipStream_t stream);
 int feature_rows_for_order(uint32_t order);

@@ -32,27 +32,336 @@ static_assert(kParams == kMlpParams, "parameter layout");
 // diagonal, Gershgorin, dominance and norm features.  The power iterations
 // run on A rounded to float (an A-perturbation of 2^-24 relative, far below
 // what the features resolve) with binary64 accumulation:
-//  * orders up to 256: A lives in registers, four lanes per row (lane p of
-//    row i holds columns 4 k + p; two rows per lane for orders 129..256),
-//    so the 40 steps never touch memory again;
-//    one instantiation per order range (1..64, 65..128, 129..256) with
-//    4 ROWS lanes, so small matrices take small workgroups;
-//  * orders 257..512: one row per lane, A re-read from memory each step.
-// Each instantiation runs over its range's own segment list (a workgroup
-// given a matrix outside its range returns at once).
+//  * orders up to 256 (features_regs_kernel): A lives in registers, four
+//    lanes per row (lane p of row i holds columns [p C, p C + C), C = ROWS /
+//    4), so the 40 steps never touch memory again; one instantiation per
+//    order range (1..64, 65..128, 129..192, 193..256) with 4 ROWS lanes;
+//  * orders 257..512 (features_global_kernel): one row per lane, A re-read
+//    from memory each step.
+// Each launch runs over its range's own segment list (a workgroup given a
+// matrix outside its range returns at once).
 constexpr int kFeatGlobalThreads = 512;
+constexpr int kStats = 13;
 
-// Register path: rows of A per lane.  Orders 129..256 hold two rows per lane
-// (512 lanes): with one row on 1024 lanes, A's 64 floats per lane leave too
-// few of the 128 registers a 1024-lane workgroup gets and the rest spills.
-__host__ __device__ constexpr int feat_rows_per_lane(int rows) { return rows == 256 ? 2 : 1; }
-__host__ __device__ constexpr int feat_threads(int rows) {
-  return rows ? 4 * rows / feat_rows_per_lane(rows) : kFeatGlobalThreads;
+// Statistic s of the per-row features: 0..6 and 12 are sums (trace, ||A||_F^2,
+// off-diagonal ||.||_F^2, sum and sum of squares of log10 d, dominance sum,
+// weak-row count, start-vector norm^2), 7 / 9 / 11 minima (diagonal,
+// Gershgorin low, dominance), 8 / 10 maxima (diagonal, Gershgorin high).
+__device__ __forceinline__ double stat_combine(int s, double a, double b) {
+  if (s == 7 || s == 9 || s == 11) return fmin(a, b);
+  if (s == 8 || s == 10) return fmax(a, b);
+  return a + b;
 }
 
-// Block-wide reductions in a fixed order (a butterfly within each wave, then
-// a butterfly over the NW wave partials, which every lane reads back): the
-// results do not depend on timing, and few values are live at a time.
+// The kStats statistics of a workgroup in a fixed order: a butterfly within
+// each wave (each lane combines the same two values, so every lane of a wave
+// holds the same result), then thread s < kStats combines statistic s over
+// the NW wave partials in wave order into out[s] (visible after the caller's
+// next barrier).
+template <int NW>
+__device__ void reduce_stats(double v[kStats], double* part, double* out) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int s = 0; s < kStats; ++s) v[s] = stat_combine(s, v[s], __shfl_xor(v[s], off));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int s = 0; s < kStats; ++s) part[wave * kStats + s] = v[s];
+  __syncthreads();
+  if (threadIdx.x < kStats) {
+    const int s = threadIdx.x;
+    double acc = part[s];
+    for (int w = 1; w < NW; ++w) acc = stat_combine(s, acc, part[w * kStats + s]);
+    out[s] = acc;
+  }
+}
+
+// Per-row statistics of row d (diagonal), r (off-diagonal absolute row sum)
+// into v (neutral elements elsewhere); `own`: this lane reports the row.
+__device__ __forceinline__ void row_stats(bool own, double d, double r, double fro, double off,
+                                          double v[kStats]) {
+#pragma unroll
+  for (int s = 0; s < kStats; ++s) v[s] = 0.0;
+  v[7] = v[9] = v[11] = INFINITY;
+  v[8] = v[10] = -INFINITY;
+  v[1] = fro;
+  v[2] = off;
+  if (own) {
+    v[0] = d;
+    v[7] = v[8] = d;
+    v[9] = d - r;
+    v[10] = d + r;
+    const double ld10 = log10(fmax(d, 1e-300));
+    v[3] = ld10;
+    v[4] = ld10 * ld10;
+    const double dom = fmin(fmax(d / fmax(r, 1e-300), 1e-6), 1e6);
+    v[5] = dom;
+    v[11] = dom;
+    v[6] = d < r ? 1.0 : 0.0;
+  }
+}
+
+// Sum over the four lanes of a row (lanes 4 r .. 4 r + 3), by DPP quad
+// permutations (lane ^ 1, then lane ^ 2): each lane adds the same two values,
+// so all four hold the same sum.
+__device__ __forceinline__ double dpp_quad(double v, int ctrl_xor1) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int plo = ctrl_xor1 ? __builtin_amdgcn_mov_dpp(lo, 0xb1, 0xf, 0xf, false)
+                            : __builtin_amdgcn_mov_dpp(lo, 0x4e, 0xf, 0xf, false);
+  const int phi = ctrl_xor1 ? __builtin_amdgcn_mov_dpp(hi, 0xb1, 0xf, 0xf, false)
+                            : __builtin_amdgcn_mov_dpp(hi, 0x4e, 0xf, 0xf, false);
+  return __hiloint2double(phi, plo);
+}
+__device__ __forceinline__ double quad_sum(double v) {
+  v += dpp_quad(v, 1);  // quad_perm [1, 0, 3, 2]
+  v += dpp_quad(v, 0);  // quad_perm [2, 3, 0, 1]
+  return v;
+}
+
+// Start vector of the lambda_min iteration: a fixed hash of the index in
+// [-0.5, 0.5) (any vector with a component along the bottom eigenvector).
+__device__ __forceinline__ double start_vector(int j) {
+  const uint32_t h = static_cast<uint32_t>(j + 1) * 2654435761u;
+  return static_cast<double>((h >> 16) & 0xffffu) / 65536.0 - 0.5;
+}
+
+// The 16 features from the reduced statistics and the power iterations'
+// last two Rayleigh quotients of each phase (thread 0).
+__device__ void write_features(const double* st, int n, double lmax, double lmax_prev, double mu,
+                               double mu_prev, double c, float* f) {
+  const double tr = st[0], fro = st[1], off = st[2], logd = st[3], logd2 = st[4];
+  const double domsum = st[5], weak = st[6], dmin = st[7], dmax = st[8], gl = st[9];
+  const double gh = st[10], dommin = st[11];
+  const double lmin = c - mu;
+  const double t = tr / n;
+  const double lmax_c = fmax(lmax, 1e-300);
+  const double lmin_c = fmax(lmin, lmax_c * 1e-16);
+  const double mean_logd = logd / n;
+  f[0] = static_cast<float>(n / 256.0);
+  f[1] = static_cast<float>(log2(static_cast<double>(n)) / 8.0);
+  f[2] = static_cast<float>(log10(lmax_c / lmin_c));
+  f[3] = static_cast<float>(log10(lmax_c / t));
+  f[4] = static_cast<float>(log10(lmin_c / t));
+  f[5] = static_cast<float>(fmin(fmax(gl / lmax_c, -16.0), 1.0));
+  f[6] = static_cast<float>(log10(fmax(gh, 1e-300) / lmax_c));
+  f[7] = static_cast<float>(log10(fmax(dmin, 1e-300) / fmax(dmax, 1e-300)));
+  f[8] = static_cast<float>(sqrt(fmax(logd2 / n - mean_logd * mean_logd, 0.0)));
+  f[9] = static_cast<float>(log10(dommin));
+  f[10] = static_cast<float>(log10(domsum / n));
+  f[11] = static_cast<float>(fro > 0.0 ? sqrt(off / fro) : 0.0);
+  f[12] = static_cast<float>(log10(fmax(sqrt(fro) * sqrt(static_cast<double>(n)) / tr, 1e-300)));
+  f[13] = static_cast<float>(log10(fabs(lmax - lmax_prev) / lmax_c + 1e-30));
+  f[14] = static_cast<float>(log10(fabs(mu - mu_prev) / fmax(fabs(mu), 1e-300) + 1e-30));
+  f[15] = static_cast<float>(weak / n);
+}
+
+// Orders (ROWS - 64, ROWS], A (binary64, column-major, pitch ld) held in
+// registers as float.  The power steps keep the vector unnormalised in LDS
+// (double-buffered) with its scale s = 1 / ||y|| in registers: step k reads
+// buffer k mod 2, forms y = s A b (or s (c b - A b) for lambda_min), writes y
+// and the wave partials of x^T y and y^T y to the other buffer, and meets the
+// other waves at one barrier; every lane then sums the NW partials in wave
+// order.  The last lambda_max step writes the start vector instead of y.
+// For orders 193..256 the lane's last C - R columns sit in LDS instead
+// (float4 k-major per lane, conflict-free): 64 register columns per lane do
+// not fit the 128 registers of a 1024-lane workgroup beside the rest.
+template <int ROWS>
+__global__ __launch_bounds__(4 * ROWS) void features_regs_kernel(
+    const Segment* __restrict__ segs, const double* __restrict__ a64,
+    float* __restrict__ features) {
+  constexpr int T = 4 * ROWS;
+  constexpr int NW = T / 64;
+  constexpr int C = ROWS / 4;                 // columns per lane
+  constexpr int R = ROWS == 256 ? 40 : C;     // of which in registers
+  constexpr int LQ = (C - R) / 4;             // float4 groups in LDS
+  static_assert(C % 8 == 0 && R % 8 == 0, "chunks of 8 columns");
+  // The vector b, double-buffered, each lane-phase's C columns padded by two
+  // doubles (16 bytes) so that the four phases' broadcast reads of a wave
+  // fall in different LDS banks: column j at j + 2 (j / C).
+  constexpr int XP = ROWS + 8;
+  __shared__ __align__(16) double xb[2][XP];
+  __shared__ float4 al[LQ > 0 ? LQ : 1][LQ > 0 ? T : 1];
+  __shared__ double pw[2][NW][2];
+  __shared__ double part[NW * kStats];
+  __shared__ double stats[kStats];
+  const Segment sg = segs[blockIdx.x];
+  const int n = static_cast<int>(sg.order);
+  if (n > ROWS || n <= ROWS - 64) return;
+  const int ld = static_cast<int>(sg.ld);
+  const double* A = a64 + sg.offset;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int row = tid >> 2, phase = tid & 3, col0 = phase * C;
+  const bool row_ok = row < n;
+
+  // Pass 1: this lane's columns of its row, rounded to float for the power
+  // steps; diagonal, off-diagonal absolute row sum, Frobenius norms.  Chunks
+  // of 8 columns, kept apart (sched_barrier) so that the loads in flight stay
+  // within the register budget.
+  float a[R];
+  double diag = 0.0, rsum = 0.0, fro = 0.0, off = 0.0;
+#pragma unroll
+  for (int k0 = 0; k0 < C; k0 += 8) {
+    float t[8];
+    // Unconditional in-bounds loads (clamped indices) of the chunk, all
+    // issued before one empty asm that keeps them out of a branch (the
+    // selects below zero what lies outside the order).
+    double raw[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      raw[k] = A[min(row, n - 1) + static_cast<size_t>(min(col0 + k0 + k, n - 1)) * ld];
+    asm volatile("" : "+v"(raw[0]), "+v"(raw[1]), "+v"(raw[2]), "+v"(raw[3]), "+v"(raw[4]),
+                 "+v"(raw[5]), "+v"(raw[6]), "+v"(raw[7]));
+#pragma unroll
+    for (int k = k0; k < k0 + 8; ++k) {
+      const int j = col0 + k;
+      const double v = (row_ok && j < n) ? raw[k - k0] : 0.0;
+      // The float is formed here (pinned), so the binary64 value dies.
+      float f = static_cast<float>(v);
+      asm volatile("" : "+v"(f));
+      if (k < R) {
+        a[k < R ? k : 0] = f;
+      } else {
+        t[k - k0] = f;
+      }
+      fro += v * v;
+      if (j == row) {
+        diag = v;
+      } else {
+        rsum += fabs(v);
+        off += v * v;
+      }
+    }
+    if (k0 >= R) {
+      al[(k0 - R) / 4][tid] = make_float4(t[0], t[1], t[2], t[3]);
+      al[(k0 - R) / 4 + 1][tid] = make_float4(t[4], t[5], t[6], t[7]);
+    }
+    asm volatile("" : "+v"(diag), "+v"(rsum), "+v"(fro), "+v"(off));
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  diag = quad_sum(diag);
+  rsum = quad_sum(rsum);
+  // Pinned here (empty asm): the statistics are reduced after the power
+  // steps, and without the pin their sums sink there too, keeping every
+  // loaded binary64 value alive through the loop.
+  asm volatile("" : "+v"(diag), "+v"(rsum), "+v"(fro), "+v"(off));
+  const bool own = row_ok && phase == 0;
+  // The start vector's squared norm now (its wave partials, summed by every
+  // lane when the lambda_min phase starts); the other statistics after the
+  // power steps, when A's registers are free.
+  {
+    double sv = own ? start_vector(row) * start_vector(row) : 0.0;
+#pragma unroll
+    for (int o = 32; o >= 4; o >>= 1) sv += __shfl_xor(sv, o);
+    if (lane == 0) part[wave * kStats + 12] = sv;
+  }
+
+  // Power iterations (x = s b, b the buffer): kPowerIterationsMax steps with
+  // y = A x from x = 1 / sqrt(n), then kPowerIterationsMin steps with
+  // y = c x - A x, c just above lambda_max, from the start vector.  Beyond
+  // the order the buffers stay zero.
+  for (int i = tid; i < XP; i += T) {
+    const int j = i - 2 * (i / (C + 2));  // the column padded index i holds
+    const bool col = i % (C + 2) < C;
+    xb[0][i] = col && j < n ? 1.0 : 0.0;
+    xb[1][i] = 0.0;
+  }
+  __syncthreads();
+  double s = 1.0 / sqrt(static_cast<double>(n));
+  double c = 0.0, lmax = 0.0, lmax_prev = 0.0, mu = 0.0, mu_prev = 0.0;
+  int cur = 0;
+  const int total_iters = kPowerIterationsMax + kPowerIterationsMin;
+#pragma unroll 1
+  for (int it = 0; it < total_iters; ++it) {
+    // The float copy of A is pinned for each step (empty asm, no
+    // instruction): otherwise its widening to binary64 is hoisted out of the
+    // loop and the doubled copy no longer fits the register file.
+#pragma unroll
+    for (int k = 0; k < R; ++k) asm volatile("" : "+v"(a[k]));
+    const double* xc = xb[cur];
+    double zs[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k0 = 0; k0 < C; k0 += 8) {
+      if (col0 + k0 < n) {
+        float w[8];
+        if (k0 < R) {
+#pragma unroll
+          for (int h = 0; h < 8; ++h) w[h] = a[k0 < R ? k0 + h : 0];
+        } else {
+          const float4 lo = al[(k0 - R) / 4][tid], hi = al[(k0 - R) / 4 + 1][tid];
+          w[0] = lo.x; w[1] = lo.y; w[2] = lo.z; w[3] = lo.w;
+          w[4] = hi.x; w[5] = hi.y; w[6] = hi.z; w[7] = hi.w;
+        }
+        const double2* xv = reinterpret_cast<const double2*>(xc + col0 + 2 * phase + k0);
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+          const double2 x2 = xv[h];
+          zs[(2 * h) & 3] += static_cast<double>(w[2 * h]) * x2.x;
+          zs[(2 * h + 1) & 3] += static_cast<double>(w[2 * h + 1]) * x2.y;
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    const double z = quad_sum((zs[0] + zs[1]) + (zs[2] + zs[3]));
+    const int prow = row + 2 * (row / C);
+    const double br = row_ok ? xc[prow] : 0.0;
+    const double y = c != 0.0 ? s * (c * br - z) : s * z;
+    double q = 0.0, yy = 0.0;
+    if (row_ok && phase == 0) {
+      q = (s * br) * y;
+      yy = y * y;
+    }
+    // Only lanes 4 r carry values: the butterfly over offsets 32 .. 4.
+#pragma unroll
+    for (int o = 32; o >= 4; o >>= 1) {
+      q += __shfl_xor(q, o);
+      yy += __shfl_xor(yy, o);
+    }
+    const int nxt = cur ^ 1;
+    if (row_ok && phase == 0)
+      xb[nxt][prow] = it == kPowerIterationsMax - 1 ? start_vector(row) : y;
+    if (lane == 0) {
+      pw[nxt][wave][0] = q;
+      pw[nxt][wave][1] = yy;
+    }
+    __syncthreads();
+    double qs = 0.0, ys = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      qs += pw[nxt][w][0];
+      ys += pw[nxt][w][1];
+    }
+    if (it < kPowerIterationsMax) {
+      lmax_prev = lmax;
+      lmax = qs;
+    } else {
+      mu_prev = mu;
+      mu = qs;
+    }
+    if (it == kPowerIterationsMax - 1) {
+      c = 1.0001 * fmax(lmax, 1e-300);
+      double sv = 0.0;
+      for (int w = 0; w < NW; ++w) sv += part[w * kStats + 12];
+      s = 1.0 / sqrt(sv);
+    } else {
+      s = ys > 0.0 ? 1.0 / sqrt(ys) : 0.0;
+    }
+    cur = nxt;
+  }
+  {
+    double v[kStats];
+    row_stats(own, diag, rsum, fro, off, v);
+    reduce_stats<NW>(v, part, stats);
+  }
+  __syncthreads();
+  if (tid == 0)
+    write_features(stats, n, lmax, lmax_prev, mu, mu_prev, c,
+                   features + static_cast<size_t>(sg.index) * kIn);
+}
+
+// Block-wide sums in a fixed order (a butterfly within each wave, then a
+// butterfly over the NW wave partials, which every lane reads back).
 template <int NW>
 __device__ void block_sum2(double& a, double& b, double* red) {
   static_assert(NW >= 1 && NW <= 64 && (NW & (NW - 1)) == 0, "waves: a power of two");
@@ -75,175 +384,44 @@ __device__ void block_sum2(double& a, double& b, double* red) {
     b += __shfl_xor(b, off);
   }
 }
-template <int NW>
-__device__ double block_sum(double v, double* red) {
-  double z = 0.0;
-  block_sum2<NW>(v, z, red);
-  return v;
-}
-template <int NW>
-__device__ double block_min(double v, double* red) {
-  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) red[wave] = v;
-  __syncthreads();
-  v = red[lane & (NW - 1)];
-  for (int off = NW / 2; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
-  return v;
-}
-template <int NW>
-__device__ double block_max(double v, double* red) { return -block_min<NW>(-v, red); }
-// Sum over the four lanes of a row (lanes 4 r .. 4 r + 3).
-__device__ __forceinline__ double quad_sum(double v) {
-  v += __shfl_xor(v, 1);
-  v += __shfl_xor(v, 2);
-  return v;
-}
 
-// Start vector of the lambda_min iteration: a fixed hash of the index in
-// [-0.5, 0.5) (any vector with a component along the bottom eigenvector).
-__device__ __forceinline__ double start_vector(int j) {
-  const uint32_t h = static_cast<uint32_t>(j + 1) * 2654435761u;
-  return static_cast<double>((h >> 16) & 0xffffu) / 65536.0 - 0.5;
-}
-
-// A binary64 column-major (pitch ld), order <= 512.  ROWS: orders
-// (ROWS / 2, ROWS] on the register path, or 0 for orders 257..512.
-template <int ROWS>
-__global__ __launch_bounds__(feat_threads(ROWS)) void features_kernel(
+// Orders 257..512: one row per lane, A re-read from memory each step.
+__global__ __launch_bounds__(kFeatGlobalThreads) void features_global_kernel(
     const Segment* __restrict__ segs, const double* __restrict__ a64,
     float* __restrict__ features) {
-  constexpr bool REGS = ROWS != 0;
-  constexpr int T = feat_threads(ROWS);
+  constexpr int T = kFeatGlobalThreads;
   constexpr int NW = T / 64;
-  constexpr int COLS = REGS ? ROWS / 4 : 1;  // register values per lane
   __shared__ double x[kMaxFeatureOrder];  // power-step vector
   __shared__ double red[2 * NW];
+  __shared__ double part[NW * kStats];
+  __shared__ double stats[kStats];
   const Segment sg = segs[blockIdx.x];
   const int n = static_cast<int>(sg.order);
-  if (REGS ? (n > ROWS || (ROWS > 64 && n <= ROWS / 2)) : n <= 256) return;
+  if (n <= 256 || n > kMaxFeatureOrder) return;
   const int ld = static_cast<int>(sg.ld);
   const double* A = a64 + sg.offset;
   const int tid = threadIdx.x;
-  // Rows and column phase of this lane: register path (rows tid / 4 + r T / 4,
-  // r < RPL, columns 4 k + tid % 4), else row tid over all columns.
-  constexpr int RPL = REGS ? feat_rows_per_lane(ROWS) : 1;
-  const int row0 = REGS ? tid >> 2 : tid;
-  const int phase = REGS ? tid & 3 : 0;
-  int rows[RPL];
-  bool rows_ok[RPL];
-#pragma unroll
-  for (int r = 0; r < RPL; ++r) {
-    rows[r] = row0 + r * (T / 4);
-    rows_ok[r] = rows[r] < n;
+  const int row = tid;
+  const bool row_ok = row < n;
+
+  double diag = 0.0, rsum = 0.0, fro = 0.0, off = 0.0;
+  for (int j = 0; j < n && row_ok; ++j) {
+    const double v = A[row + static_cast<size_t>(j) * ld];
+    fro += v * v;
+    if (j == row) {
+      diag = v;
+    } else {
+      rsum += fabs(v);
+      off += v * v;
+    }
+  }
+  {
+    double v[kStats];
+    row_stats(row_ok, diag, rsum, fro, off, v);
+    v[12] = 0.0;
+    reduce_stats<NW>(v, part, stats);
   }
 
-  // Pass 1: diagonal, off-diagonal absolute row sums, Frobenius norms.
-  float a[RPL][COLS];
-  double diag[RPL], rsum[RPL];
-  double fro = 0.0, off = 0.0;
-#pragma unroll
-  for (int r = 0; r < RPL; ++r) diag[r] = rsum[r] = 0.0;
-  if constexpr (REGS) {
-    // Chunks of 8 columns, kept apart (sched_barrier) so that the loads in
-    // flight stay within the register budget.
-#pragma unroll
-    for (int r = 0; r < RPL; ++r) {
-      const int row = rows[r];
-#pragma unroll
-      for (int k0 = 0; k0 < COLS; k0 += 8) {
-#pragma unroll
-        for (int k = k0; k < k0 + 8; ++k) {
-          // Unconditional in-bounds load (clamped indices), then a select; the
-          // empty asm keeps the load out of a branch (the compiler would
-          // otherwise make each load conditional, one basic block per column).
-          const int j = 4 * k + phase;
-          double raw = A[min(row, n - 1) + static_cast<size_t>(min(j, n - 1)) * ld];
-          asm volatile("" : "+v"(raw));
-          const double v = (rows_ok[r] && j < n) ? raw : 0.0;
-          a[r][k] = static_cast<float>(v);
-          fro += v * v;
-          if (j == row) {
-            diag[r] = v;
-          } else {
-            rsum[r] += fabs(v);
-            off += v * v;
-          }
-        }
-        __builtin_amdgcn_sched_barrier(0);
-      }
-      diag[r] = quad_sum(diag[r]);
-      rsum[r] = quad_sum(rsum[r]);
-    }
-  } else {
-    const int row = rows[0];
-    for (int j = 0; j < n && rows_ok[0]; ++j) {
-      const double v = A[row + static_cast<size_t>(j) * ld];
-      fro += v * v;
-      if (j == row) {
-        diag[0] = v;
-      } else {
-        rsum[0] += fabs(v);
-        off += v * v;
-      }
-    }
-  }
-  // Per-row features from one lane of each row.
-  double tr = 0.0, dmin = INFINITY, dmax = -INFINITY, gl = INFINITY, gh = -INFINITY;
-  double logd = 0.0, logd2 = 0.0, dommin = INFINITY, domsum = 0.0, weak = 0.0;
-#pragma unroll
-  for (int r = 0; r < RPL; ++r) {
-    if (rows_ok[r] && phase == 0) {
-      const double d = diag[r];
-      tr += d;
-      dmin = fmin(dmin, d);
-      dmax = fmax(dmax, d);
-      gl = fmin(gl, d - rsum[r]);
-      gh = fmax(gh, d + rsum[r]);
-      const double ld10 = log10(fmax(d, 1e-300));
-      logd += ld10;
-      logd2 += ld10 * ld10;
-      const double dom = fmin(fmax(d / fmax(rsum[r], 1e-300), 1e-6), 1e6);
-      dommin = fmin(dommin, dom);
-      domsum += dom;
-      weak += d < rsum[r] ? 1.0 : 0.0;
-    }
-  }
-  block_sum2<NW>(tr, fro, red);
-  block_sum2<NW>(off, logd, red);
-  block_sum2<NW>(logd2, domsum, red);
-  weak = block_sum<NW>(weak, red);
-  dmin = block_min<NW>(dmin, red);
-  dmax = block_max<NW>(dmax, red);
-  gl = block_min<NW>(gl, red);
-  gh = block_max<NW>(gh, red);
-  dommin = block_min<NW>(dommin, red);
-  // The reduced statistics are read again only by thread 0 at the end: kept
-  // in LDS meanwhile, so that the power steps get the registers (the binary64
-  // values would otherwise stay live in every lane).
-  __shared__ double stats[12];
-  if (tid == 0) {
-    stats[0] = tr;
-    stats[1] = fro;
-    stats[2] = off;
-    stats[3] = logd;
-    stats[4] = logd2;
-    stats[5] = domsum;
-    stats[6] = weak;
-    stats[7] = dmin;
-    stats[8] = dmax;
-    stats[9] = gl;
-    stats[10] = gh;
-    stats[11] = dommin;
-  }
-
-  // Power iterations: kPowerIterationsMax steps of x <- y / ||y|| with
-  // y = A x from x = 1 / sqrt(n) (lambda_max), then kPowerIterationsMin steps
-  // with y = c x - A x, c just above lambda_max, from the start vector (top of
-  // c I - A: lambda_min = c - mu).  The last two Rayleigh quotients x^T y of
-  // each phase are kept.  x beyond the order stays zero (the register path
-  // reads columns up to the next multiple of 4).
   const double inv_sqrt_n = 1.0 / sqrt(static_cast<double>(n));
   for (int i = tid; i < kMaxFeatureOrder; i += T) x[i] = i < n ? inv_sqrt_n : 0.0;
   __syncthreads();
@@ -253,61 +431,24 @@ __global__ __launch_bounds__(feat_threads(ROWS)) void features_kernel(
   for (int it = 0; it < total_iters; ++it) {
     // Every read of x in a step precedes the barriers of its reduction, so
     // the new x is written in place after it.
-    double* xc = x;
     if (it == kPowerIterationsMax) {
       c = 1.0001 * fmax(lmax, 1e-300);
-      double snorm = 0.0;
+      double snorm = 0.0, unused = 0.0;
       for (int i = tid; i < n; i += T) snorm += start_vector(i) * start_vector(i);
-      snorm = block_sum<NW>(snorm, red);
-      for (int i = tid; i < n; i += T) xc[i] = start_vector(i) / sqrt(snorm);
+      block_sum2<NW>(snorm, unused, red);
+      for (int i = tid; i < n; i += T) x[i] = start_vector(i) / sqrt(snorm);
       __syncthreads();
     }
-    double y[RPL];
-    if constexpr (REGS) {
-      // The register copy of A is pinned as float for each step (empty asm, no
-      // instruction): otherwise its widening to binary64 is hoisted out of the
-      // loop and the doubled copy no longer fits the register file.  Four
-      // partial sums per row (columns 16 k0 + 4 u + p, u = k mod 4) shorten
-      // the dependent chain of binary64 FMAs; each x value read serves every
-      // row of the lane.
-#pragma unroll
-      for (int r = 0; r < RPL; ++r)
-#pragma unroll
-        for (int k = 0; k < COLS; ++k) asm volatile("" : "+v"(a[r][k]));
-      double ys[RPL][4];
-#pragma unroll
-      for (int r = 0; r < RPL; ++r) ys[r][0] = ys[r][1] = ys[r][2] = ys[r][3] = 0.0;
-#pragma unroll
-      for (int r = 0; r < RPL; ++r) {
-#pragma unroll
-        for (int k0 = 0; k0 < COLS; k0 += 8) {
-          if (4 * k0 < n) {
-#pragma unroll
-            for (int k = k0; k < k0 + 8; ++k)
-              ys[r][k & 3] += static_cast<double>(a[r][k]) * xc[4 * k + phase];
-          }
-          __builtin_amdgcn_sched_barrier(0);
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < RPL; ++r)
-        y[r] = quad_sum((ys[r][0] + ys[r][1]) + (ys[r][2] + ys[r][3]));
-    } else {
-      y[0] = 0.0;
-      if (rows_ok[0]) {
-        for (int j = 0; j < n; ++j)
-          y[0] += static_cast<double>(static_cast<float>(A[rows[0] + static_cast<size_t>(j) * ld])) *
-                  xc[j];
-      }
+    double y = 0.0;
+    if (row_ok) {
+      for (int j = 0; j < n; ++j)
+        y += static_cast<double>(static_cast<float>(A[row + static_cast<size_t>(j) * ld])) * x[j];
     }
     double q = 0.0, yy = 0.0;
-#pragma unroll
-    for (int r = 0; r < RPL; ++r) {
-      if (c != 0.0 && rows_ok[r]) y[r] = c * xc[rows[r]] - y[r];
-      if (rows_ok[r] && phase == 0) {
-        q += xc[rows[r]] * y[r];
-        yy += y[r] * y[r];
-      }
+    if (row_ok) {
+      if (c != 0.0) y = c * x[row] - y;
+      q = x[row] * y;
+      yy = y * y;
     }
     block_sum2<NW>(q, yy, red);
     if (it < kPowerIterationsMax) {
@@ -318,48 +459,12 @@ __global__ __launch_bounds__(feat_threads(ROWS)) void features_kernel(
       mu = q;
     }
     const double inv = yy > 0.0 ? 1.0 / sqrt(yy) : 0.0;
-#pragma unroll
-    for (int r = 0; r < RPL; ++r)
-      if (rows_ok[r] && phase == 0) xc[rows[r]] = y[r] * inv;
+    if (row_ok) x[row] = y * inv;
     __syncthreads();
   }
-  const double lmin = c - mu;
-
-  if (tid == 0) {
-    tr = stats[0];
-    fro = stats[1];
-    off = stats[2];
-    logd = stats[3];
-    logd2 = stats[4];
-    domsum = stats[5];
-    weak = stats[6];
-    dmin = stats[7];
-    dmax = stats[8];
-    gl = stats[9];
-    gh = stats[10];
-    dommin = stats[11];
-    const double t = tr / n;
-    const double lmax_c = fmax(lmax, 1e-300);
-    const double lmin_c = fmax(lmin, lmax_c * 1e-16);
-    const double mean_logd = logd / n;
-    float* f = features + static_cast<size_t>(sg.index) * kIn;
-    f[0] = static_cast<float>(n / 256.0);
-    f[1] = static_cast<float>(log2(static_cast<double>(n)) / 8.0);
-    f[2] = static_cast<float>(log10(lmax_c / lmin_c));
-    f[3] = static_cast<float>(log10(lmax_c / t));
-    f[4] = static_cast<float>(log10(lmin_c / t));
-    f[5] = static_cast<float>(fmin(fmax(gl / lmax_c, -16.0), 1.0));
-    f[6] = static_cast<float>(log10(fmax(gh, 1e-300) / lmax_c));
-    f[7] = static_cast<float>(log10(fmax(dmin, 1e-300) / fmax(dmax, 1e-300)));
-    f[8] = static_cast<float>(sqrt(fmax(logd2 / n - mean_logd * mean_logd, 0.0)));
-    f[9] = static_cast<float>(log10(dommin));
-    f[10] = static_cast<float>(log10(domsum / n));
-    f[11] = static_cast<float>(fro > 0.0 ? sqrt(off / fro) : 0.0);
-    f[12] = static_cast<float>(log10(fmax(sqrt(fro) * sqrt(static_cast<double>(n)) / tr, 1e-300)));
-    f[13] = static_cast<float>(log10(fabs(lmax - lmax_prev) / lmax_c + 1e-30));
-    f[14] = static_cast<float>(log10(fabs(mu - mu_prev) / fmax(fabs(mu), 1e-300) + 1e-30));
-    f[15] = static_cast<float>(weak / n);
-  }
+  if (tid == 0)
+    write_features(stats, n, lmax, lmax_prev, mu, mu_prev, c,
+                   features + static_cast<size_t>(sg.index) * kIn);
 }
 
 // ----------------------------------------------------------------------- MLP
@@ -547,17 +652,20 @@ __global__ void mlp_adam_kernel(float* __restrict__ params, const float* __restr
 }  // namespace
 
 int feature_rows_for_order(uint32_t order) {
-  return order <= 64 ? 64 : order <= 128 ? 128 : order <= 256 ? 256 : 0;
+  return order <= 64 ? 64 : order <= 128 ? 128 : order <= 192 ? 192 : order <= 256 ? 256 : 0;
 }
 
 hipError_t launch_features(int rows, const Segment* segs, uint32_t count, const double* a64,
                            float* features, hipStream_t stream) {
   if (count == 0) return hipSuccess;
   switch (rows) {
-    case 64: features_kernel<64><<<count, feat_threads(64), 0, stream>>>(segs, a64, features); break;
-    case 128: features_kernel<128><<<count, feat_threads(128), 0, stream>>>(segs, a64, features); break;
-    case 256: features_kernel<256><<<count, feat_threads(256), 0, stream>>>(segs, a64, features); break;
-    case 0: features_kernel<0><<<count, feat_threads(0), 0, stream>>>(segs, a64, features); break;
+    case 64: features_regs_kernel<64><<<count, 4 * 64, 0, stream>>>(segs, a64, features); break;
+    case 128: features_regs_kernel<128><<<count, 4 * 128, 0, stream>>>(segs, a64, features); break;
+    case 192: features_regs_kernel<192><<<count, 4 * 192, 0, stream>>>(segs, a64, features); break;
+    case 256: features_regs_kernel<256><<<count, 4 * 256, 0, stream>>>(segs, a64, features); break;
+    case 0:
+      features_global_kernel<<<count, kFeatGlobalThreads, 0, stream>>>(segs, a64, features);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -182,10 +182,12 @@ def test_fp8_scaled_oracle():
 
 def test_device_codecs_match_oracle_on_host(tmp_path):
     exe = tmp_path / "host_codec_check"
-    subprocess.run(["g++", "-O2", "-std=c++17", "-I", str(ROOT / "mixed-mlx_amd" / "csrc"),
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", "-I", str(ROOT / "mixed-mlx_amd" / "csrc"),
                     "-x", "c++", str(ROOT / "tests" / "host_codec_check.cpp"),
                     "-x", "c", str(ROOT / "oracle" / "codecs_ref.c"), "-o", str(exe), "-lm"],
                    check=True)
-    out = subprocess.run([str(exe), "99991", "997"], capture_output=True, text=True, timeout=300)
+    # "full": every float bit pattern against the generic formulation too.
+    out = subprocess.run([str(exe), "99991", "997", "full"], capture_output=True, text=True,
+                         timeout=600)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.startswith("ok")
