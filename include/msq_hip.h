@@ -490,6 +490,13 @@ int msq_set_tile_table(msq_runtime *runtime, const uint8_t *table);
  * diagonal spread and dominance, norms. */
 int msq_matrix_features(msq_runtime *runtime, uint32_t count, const uint32_t *orders,
                         const double *matrices, float *features);
+/* msq_matrix_features for the batch positions in members[0 .. member_count)
+ * only (each < count): row members[q] of features receives matrix
+ * members[q]'s features, the other rows are left as they are.  The selector
+ * estimates its batch gate from a sample this way. */
+int msq_matrix_features_subset(msq_runtime *runtime, uint32_t count, const uint32_t *orders,
+                               const double *matrices, uint32_t member_count,
+                               const uint32_t *members, float *features);
 
 /* logits (device, batch x MSQ_MLP_OUTPUTS) of the network at params for the
  * inputs x (device, batch x MSQ_MLP_INPUTS).  h1 / h2 (device, batch x

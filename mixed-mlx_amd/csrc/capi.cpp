@@ -2713,16 +2713,25 @@ static_assert(msq::kMlpParams == MSQ_MLP_PARAMS && msq::kMlpInputs == MSQ_MLP_IN
                   msq::kMaxFeatureOrder == MSQ_FEATURE_MAX_ORDER,
               "selector constants of the C ABI");
 
-int msq_matrix_features(msq_runtime* rt, uint32_t count, const uint32_t* orders,
-                        const double* matrices, float* features) {
+namespace {
+
+// Features of the listed members of a batch (rows of `features` indexed by
+// batch position; other rows untouched).
+int matrix_features_of(msq_runtime* rt, uint32_t count, const uint32_t* orders,
+                       const double* matrices, uint32_t member_count, const uint32_t* member_list,
+                       float* features) {
   MSQ_TRY(activate(rt));
   Layout L;
   MSQ_TRY(make_layout(count, orders, L));
   if (!matrices || !features) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null buffer");
-  for (uint32_t i = 0; i < count; ++i)
+  for (uint32_t q = 0; q < member_count; ++q) {
+    const uint32_t i = member_list ? member_list[q] : q;
+    if (i >= count) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "member %u of %u", i, count);
     if (L.order[i] > static_cast<uint32_t>(msq::kMaxFeatureOrder))
       return set_error(MSQ_ERROR_INVALID_ARGUMENT, "order %u above %d", L.order[i],
                        msq::kMaxFeatureOrder);
+  }
+  if (member_count == 0) return MSQ_OK;
   // Segments grouped by order range, one launch per range: the ranges run
   // concurrently on the runtime's lanes (the fifth on the first lane).
   constexpr int kKinds = 5;
@@ -2730,14 +2739,16 @@ int msq_matrix_features(msq_runtime* rt, uint32_t count, const uint32_t* orders,
   std::vector<uint32_t> members;
   uint32_t bounds[kKinds + 1] = {};
   for (int k = 0; k < kKinds; ++k) {
-    for (uint32_t i = 0; i < count; ++i)
+    for (uint32_t q = 0; q < member_count; ++q) {
+      const uint32_t i = member_list ? member_list[q] : q;
       if (msq::feature_rows_for_order(L.order[i]) == kinds[k]) members.push_back(i);
+    }
     bounds[k + 1] = static_cast<uint32_t>(members.size());
   }
   const std::vector<msq::Segment> segs = segments_of(L, members);
   Scratch S{rt};
   msq::Segment* dsegs = nullptr;
-  S.want(dsegs, count);
+  S.want(dsegs, members.size());
   MSQ_TRY(S.commit());
   MSQ_TRY(upload(rt, dsegs, segs));
   HIP_TRY(hipEventRecord(rt->fork, rt->stream));
@@ -2750,6 +2761,20 @@ int msq_matrix_features(msq_runtime* rt, uint32_t count, const uint32_t* orders,
     HIP_TRY(hipStreamWaitEvent(rt->stream, rt->joins[l], 0));
   }
   return MSQ_OK;
+}
+
+}  // namespace
+
+int msq_matrix_features(msq_runtime* rt, uint32_t count, const uint32_t* orders,
+                        const double* matrices, float* features) {
+  return matrix_features_of(rt, count, orders, matrices, count, nullptr, features);
+}
+
+int msq_matrix_features_subset(msq_runtime* rt, uint32_t count, const uint32_t* orders,
+                               const double* matrices, uint32_t member_count,
+                               const uint32_t* members, float* features) {
+  if (member_count && !members) return set_error(MSQ_ERROR_INVALID_ARGUMENT, "null member list");
+  return matrix_features_of(rt, count, orders, matrices, member_count, members, features);
 }
 
 int msq_mlp_forward(msq_runtime* rt, const float* params, const float* x, uint32_t batch,

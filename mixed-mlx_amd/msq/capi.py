@@ -80,7 +80,8 @@ EXPORTED_SYMBOLS = (
     "msq_correction", "msq_refine_native_staged",
     "msq_orthogonality", "msq_solve_batched", "msq_solve_batched_ex", "msq_solve_options_init",
     "msq_last_factors",
-    "msq_matrix_features", "msq_mlp_forward", "msq_mlp_forward_bf16", "msq_mlp_backward",
+    "msq_matrix_features", "msq_matrix_features_subset", "msq_mlp_forward",
+    "msq_mlp_forward_bf16", "msq_mlp_backward",
     "msq_mlp_adam", "msq_autotune", "msq_set_tile_table",
 )
 TILE_TYPES = ("fp64_residual", "fp32", "bf16", "fp8_scaled", "mxfp4")
@@ -198,6 +199,7 @@ _sig("msq_quantize_stream", _i, _p, _i, _p, _p, _p, _sz, _u32)
 _sig("msq_spectrum_metadata_bytes", _sz, _i, _sz)
 _sig("msq_last_factors", _i, _p, ctypes.POINTER(_p), ctypes.POINTER(_p))
 _sig("msq_matrix_features", _i, _p, _u32, _u32p, _p, _p)
+_sig("msq_matrix_features_subset", _i, _p, _u32, _u32p, _p, _u32, _u32p, _p)
 _sig("msq_mlp_forward", _i, _p, _p, _p, _u32, _p, _p, _p)
 _sig("msq_mlp_forward_bf16", _i, _p, _p, _p, _u32, _p)
 _sig("msq_mlp_backward", _i, _p, _p, _p, _p, _p, _p, _p, _p, _u32, ctypes.c_float, _p, _p)
@@ -576,9 +578,16 @@ class Runtime:
         return v.value, s.value
 
     # tier selector ------------------------------------------------------
-    def matrix_features(self, orders, matrices, features):
-        check(lib.msq_matrix_features(self.handle, len(orders), _u32_array(orders),
-                                      _addr(matrices), _addr(features)))
+    def matrix_features(self, orders, matrices, features, members=None):
+        """Features of every matrix, or of the batch positions in `members`
+        only (their rows of `features`; the other rows untouched)."""
+        if members is None:
+            check(lib.msq_matrix_features(self.handle, len(orders), _u32_array(orders),
+                                          _addr(matrices), _addr(features)))
+        else:
+            check(lib.msq_matrix_features_subset(self.handle, len(orders), _u32_array(orders),
+                                                 _addr(matrices), len(members),
+                                                 _u32_array(members), _addr(features)))
 
     def mlp_forward(self, params, x, logits, h1=None, h2=None):
         batch = x.numel() // MLP_INPUTS

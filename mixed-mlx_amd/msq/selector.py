@@ -43,7 +43,14 @@ Jacobi sweeps and corrections without that cliff.
   are dropped (their matrices take their best remaining choice), and a batch
   whose best single choice is expected within SINGLE_MARGIN of the
   per-matrix choices runs that single choice.  A batch expected to pass on
-  the fixed BF16 tier (GATE_FAILURE_FRACTION) runs it.
+  the fixed BF16 tier (GATE_FAILURE_FRACTION) runs it;
+* batch gate from a sample: batches of GATE_SAMPLE_MIN matrices or more
+  first score one matrix in GATE_SAMPLE_STRIDE (stratified by order,
+  ``gate_sample``); when the sample's mean predicted failure of the fixed tier
+  is below the gate (times GATE_SAMPLE_MARGIN), the batch runs the fixed tier
+  without the other matrices' features (the features are most of the
+  selector's cost: 1.5 ms on the 4096-matrix uneven batch); otherwise the
+  rest of the batch is scored and the full decision above runs.
 
 Feature standardisation is folded into the first layer when the weights are
 saved, so inference runs the raw features through the network unchanged.
@@ -88,6 +95,27 @@ SINGLE_MARGIN = 0.1
 # model resolves (measured: the uneven batch expects 0.3% and none fail; the
 # 32..512 mixture expects about 5% and 5.6% fail).
 GATE_FAILURE_FRACTION = 0.005
+# The gate estimated from a stratified sample of one matrix in 8 (batches of
+# 1024 or more), held to the same threshold: near the gate the fixed tier and
+# the per-matrix choices cost about the same, so an estimate that lands on
+# the wrong side of it costs little; far from it (the 32..512 mixture expects
+# 10 times the gate) a sample of 128 or more stratified matrices does not
+# mistake the batch.  A misjudged batch costs time (fallback solves), never
+# accuracy.
+GATE_SAMPLE_STRIDE = 8
+GATE_SAMPLE_MIN = 1024
+GATE_SAMPLE_MARGIN = 1.0
+
+
+def gate_sample(orders):
+    """Batch positions of the gate's sample: positions sorted by order (stable),
+    every GATE_SAMPLE_STRIDE-th from the middle of the first stride, in
+    position order; None for batches below GATE_SAMPLE_MIN."""
+    count = len(orders)
+    if count < GATE_SAMPLE_MIN:
+        return None
+    by_order = np.argsort(np.asarray(orders), kind="stable")
+    return np.sort(by_order[GATE_SAMPLE_STRIDE // 2::GATE_SAMPLE_STRIDE])
 
 
 def choice_format(k):
@@ -304,19 +332,49 @@ class Selector:
             self._p16 = p16
         return self._p16
 
+    def infer(self, rt, feats):
+        """P(tier k succeeds), (rows, 8), for device features (rows x 16):
+        BF16 inference."""
+        import torch
+
+        rows = feats.numel() // capi.MLP_INPUTS
+        logits = torch.zeros(rows * capi.MLP_OUTPUTS, dtype=torch.float32, device=feats.device)
+        rt.mlp_forward_bf16(self.bf16_params(rt, feats.device), feats, logits)
+        z = logits.cpu().numpy().reshape(rows, capi.MLP_OUTPUTS).astype(np.float64)
+        return 1.0 / (1.0 + np.exp(-z))
+
     def probabilities(self, rt, orders, matrices):
         """P(tier k succeeds) per matrix, (count, 8), from device matrices:
         features, then BF16 inference."""
         import torch
 
         count = len(orders)
+        feats = torch.zeros(count * capi.MLP_INPUTS, dtype=torch.float32, device=matrices.device)
+        rt.matrix_features(orders, matrices, feats)
+        return self.infer(rt, feats)
+
+    def choose_for_batch(self, rt, orders, matrices, fallback_weight=FALLBACK_WEIGHT):
+        """Choices for one batched solve of device matrices: the gate from
+        the sample first (large batches), else every matrix scored and
+        ``choose_batch``.  Returns (choices, number of matrices scored)."""
+        import torch
+
+        count = len(orders)
         dev = matrices.device
         feats = torch.zeros(count * capi.MLP_INPUTS, dtype=torch.float32, device=dev)
-        rt.matrix_features(orders, matrices, feats)
-        logits = torch.zeros(count * capi.MLP_OUTPUTS, dtype=torch.float32, device=dev)
-        rt.mlp_forward_bf16(self.bf16_params(rt, dev), feats, logits)
-        z = logits.cpu().numpy().reshape(count, capi.MLP_OUTPUTS).astype(np.float64)
-        return 1.0 / (1.0 + np.exp(-z))
+        sample = gate_sample(orders)
+        if sample is not None:
+            rt.matrix_features(orders, matrices, feats, members=sample.tolist())
+            rows = torch.as_tensor(sample, dtype=torch.long, device=dev)
+            ps = self.infer(rt, feats.view(count, capi.MLP_INPUTS)[rows].reshape(-1))
+            if float(np.mean(1.0 - ps[:, FIXED])) < GATE_SAMPLE_MARGIN * GATE_FAILURE_FRACTION:
+                return np.full(count, FIXED), len(sample)
+            rest = np.setdiff1d(np.arange(count), sample)
+            rt.matrix_features(orders, matrices, feats, members=rest.tolist())
+        else:
+            rt.matrix_features(orders, matrices, feats)
+        prob = self.infer(rt, feats)
+        return self.choose_batch(prob, orders, fallback_weight), count
 
     def expected(self, prob, orders, fallback_weight=FALLBACK_WEIGHT):
         """Expected time (ms) of every choice for every matrix, (count, 9):
@@ -376,8 +434,7 @@ class Selector:
 def solve_selected(rt, selector, orders, matrices, roots):
     """Selector-chosen tiers with the plain-path fallback.  Returns
     (reports, fallback_indices, choices)."""
-    prob = selector.probabilities(rt, orders, matrices)
-    choice = selector.choose_batch(prob, orders)
+    choice, _ = selector.choose_for_batch(rt, orders, matrices)
     opts_all = [tier_options(k) for k in range(len(CHOICES))]
     if np.all(choice == choice[0]):  # one record for the whole batch
         opts = opts_all[int(choice[0])]

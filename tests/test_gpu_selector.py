@@ -45,6 +45,50 @@ def test_features_match_oracle(runtime):
                                    rtol=1e-5, atol=8 * n * 2.0**-52)
 
 
+def test_feature_subset_rows_equal_the_full_call(runtime):
+    """msq_matrix_features_subset: the listed rows bit for bit as the full
+    call computes them, the other rows untouched."""
+    rng = np.random.default_rng(12)
+    orders = [int(n) for n in rng.integers(1, 513, 40)]
+    mats = [_spd(rng, n, 10.0 ** rng.uniform(1, 5)) for n in orders]
+    o, aflat = capi.pack_batch(mats)
+    A = dev(aflat)
+    full = torch.zeros(len(o) * capi.MLP_INPUTS, dtype=torch.float32, device="cuda")
+    runtime.matrix_features(o, A, full)
+    members = sorted(rng.choice(len(o), 17, replace=False).tolist())
+    part = torch.full((len(o) * capi.MLP_INPUTS,), 7.0, dtype=torch.float32, device="cuda")
+    runtime.matrix_features(o, A, part, members=members)
+    f, p = host(full).reshape(len(o), -1), host(part).reshape(len(o), -1)
+    say("feature subset", len(members), "of", len(o))
+    assert np.array_equal(p[members].view(np.uint32), f[members].view(np.uint32))
+    rest = np.setdiff1d(np.arange(len(o)), members)
+    assert np.all(p[rest] == 7.0)
+    with pytest.raises(capi.MsqError):
+        runtime.matrix_features(o, A, part, members=[len(o)])
+
+
+def test_sampled_gate_agrees_with_the_full_batch(runtime):
+    """The batched uneven workload (first 2048 matrices): the gate's sample
+    sends the batch to the fixed tier, as the full batch's mean predicted
+    failure does; the sample scores one matrix in 8."""
+    if not SEL.WEIGHTS.exists():
+        pytest.skip("no trained selector weights")
+    sel = SEL.Selector.load()
+    mats = G.batched_uneven(count=2048)
+    o, flat = capi.pack_batch([m["A"] for m in mats])
+    A = dev(flat)
+    choice, scored = sel.choose_for_batch(runtime, o, A)
+    prob = sel.probabilities(runtime, o, A)
+    full_fail = float(np.mean(1.0 - prob[:, SEL.FIXED]))
+    sample = SEL.gate_sample(o)
+    say("gate: full-batch predicted failure", full_fail, "sample",
+        float(np.mean(1.0 - prob[sample, SEL.FIXED])), "scored", scored)
+    assert full_fail < SEL.GATE_FAILURE_FRACTION
+    assert scored == len(sample) == len(o) // SEL.GATE_SAMPLE_STRIDE
+    assert (choice == SEL.FIXED).all()
+    assert (sel.choose_batch(prob, o) == SEL.FIXED).all()
+
+
 def test_features_reject_orders_above_512(runtime):
     A = np.eye(520)
     o, aflat = capi.pack_batch([A])

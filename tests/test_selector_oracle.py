@@ -237,6 +237,27 @@ def test_batch_choice_drops_small_format_groups_and_prefers_a_single_choice():
     assert (bc[:40] == 0).all() and (bc[40:] == 4).all()
 
 
+def test_gate_sample_is_stratified_by_order():
+    """One matrix in 8 of every order class, deterministic, in position
+    order; small batches are scored whole."""
+    from msq import selector as SEL
+    assert SEL.gate_sample([64] * (SEL.GATE_SAMPLE_MIN - 1)) is None
+    rng = np.random.default_rng(4)
+    orders = (16 * (1 + rng.integers(0, 16, 4096))).tolist()
+    sample = SEL.gate_sample(orders)
+    assert np.array_equal(sample, SEL.gate_sample(orders))
+    assert np.all(np.diff(sample) > 0) and len(sample) == 4096 // SEL.GATE_SAMPLE_STRIDE
+    o = np.asarray(orders)
+    for n in np.unique(o):
+        members = int(np.sum(o == n))
+        picked = int(np.sum(o[sample] == n))
+        assert abs(picked - members / SEL.GATE_SAMPLE_STRIDE) <= 1
+    # a batch sorted by order is sampled across its whole range too
+    sorted_orders = sorted(orders)
+    s2 = SEL.gate_sample(sorted_orders)
+    assert s2[0] < SEL.GATE_SAMPLE_STRIDE and s2[-1] >= 4096 - SEL.GATE_SAMPLE_STRIDE
+
+
 def test_forward_bf16_rounding_and_closed_forms():
     """Pins of the BF16 forward: the BF16 rounding of float32 values (ties to
     even, NaN quieted, exhaustive against the codec oracle on a sample); an
