@@ -331,6 +331,9 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="eager launches instead of per-batch HIP graph replay")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--jacobi", choices=["blocked", "scalar"], default="scalar",
+                    help="eigensolver of the orders 129..256: blocked (8-index blocks, "
+                         "FP32 MFMA products) or scalar (one rotation per pair)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the batched SPD solve shards by matrix with no "
@@ -380,6 +383,8 @@ def main():
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
     rt = capi.Runtime(0, stream=stream)
+    rt.set_jacobi_method(capi.JACOBI_BLOCKED_MFMA if args.jacobi == "blocked"
+                         else capi.JACOBI_SCALAR)
     tile_table = None
     if not args.no_autotune:
         t = time.perf_counter()

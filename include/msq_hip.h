@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define MSQ_ABI_VERSION 5
+#define MSQ_ABI_VERSION 6
 
 enum msq_status {
   MSQ_OK = 0,
@@ -292,6 +292,19 @@ int msq_runtime_set_rounding(msq_runtime *runtime, int mode, uint32_t seed);
  * of the signature's own).  Batches with orders above 256 always run
  * eagerly.  At most 4 signatures are kept; enable = 0 releases them. */
 int msq_runtime_set_graphs(msq_runtime *runtime, int enable);
+
+/* Eigensolver of the orders 129..256 (no counterpart in the reference, whose
+ * Jacobi is one scalar kernel for every order):
+ *  - MSQ_JACOBI_SCALAR: one rotation at a time per index pair, the reference's
+ *    element formulas on the padded circle schedule, V in device memory;
+ *  - MSQ_JACOBI_BLOCKED_MFMA: blocks of 8 indices paired on the circle, each
+ *    16 x 16 pair subproblem swept once by scalar rotations, the rest of the
+ *    round applied as FP32 MFMA products (V in registers); same stop rules,
+ *    a sweep being one pass over the block pairs.
+ * Orders up to 128 and above 256 are not affected.  Changing the method
+ * releases the runtime's HIP graphs.  Default: MSQ_JACOBI_SCALAR. */
+enum msq_jacobi_method { MSQ_JACOBI_SCALAR = 0, MSQ_JACOBI_BLOCKED_MFMA = 1 };
+int msq_runtime_set_jacobi_method(msq_runtime *runtime, int method);
 
 /* makeSharedBuffer / releaseBuffers (src/metal_runtime.cpp:1426-1432).  No
  * unified memory here: device allocations plus explicit copies. */

@@ -27,7 +27,7 @@ FLAGS = [
     "-fno-gpu-flush-denormals-to-zero", "-Wall", "-Werror", "-Wno-unused-function",
 ]
 
-SOURCES = ["kernels_codec.hip", "kernels_jacobi.hip", "kernels_gemm.hip",
+SOURCES = ["kernels_codec.hip", "kernels_jacobi.hip", "kernels_tile_jacobi.hip", "kernels_gemm.hip",
            "kernels_misc.hip", "kernels_mlp.hip", "kernels_blocked.hip", "noalloc.hip",
            "capi.cpp"]
 

@@ -105,6 +105,8 @@ struct msq_runtime {
   // with the runtime) and the device's CU count (its grid).
   uint32_t* jacobi_counter = nullptr;
   int cu_count = 0;
+  // Eigensolver of the persistent family (msq_runtime_set_jacobi_method).
+  int jacobi_method = MSQ_JACOBI_SCALAR;
 };
 
 namespace {
@@ -633,7 +635,10 @@ int enqueue_jacobi_classes(msq_runtime* rt, const JacobiClass* cls, int classes,
       if (load[l] < load[lane]) lane = l;
     if (used) load[lane] += c.cost;
     hipStream_t s = used ? rt->lanes[lane] : rt->stream;
-    if (c.order == 0) {
+    if (c.order == 0 && rt->jacobi_method == MSQ_JACOBI_BLOCKED_MFMA) {
+      HIP_TRY(msq::launch_tile_jacobi(djobs + c.first, c.count, rt->jacobi_counter, rt->cu_count,
+                                      a64, V, diag, roots, jstatus, s));
+    } else if (c.order == 0) {
       HIP_TRY(msq::launch_persistent_jacobi(djobs + c.first, c.count, rt->jacobi_counter,
                                             rt->cu_count, a64, V, diag, roots, jstatus, s));
     } else {
@@ -1715,6 +1720,20 @@ int msq_runtime_set_rounding(msq_runtime* rt, int mode, uint32_t seed) {
     return set_error(MSQ_ERROR_INVALID_ARGUMENT, "rounding mode %d", mode);
   rt->rounding_mode = mode;
   rt->rounding_seed = seed;
+  return MSQ_OK;
+}
+
+int msq_runtime_set_jacobi_method(msq_runtime* rt, int method) {
+  MSQ_TRY(activate(rt));
+  if (method != MSQ_JACOBI_SCALAR && method != MSQ_JACOBI_BLOCKED_MFMA)
+    return set_error(MSQ_ERROR_INVALID_ARGUMENT, "Jacobi method %d", method);
+  if (method != rt->jacobi_method && !rt->graphs.empty()) {
+    HIP_TRY(hipStreamSynchronize(rt->stream));
+    HIP_TRY(hipStreamSynchronize(rt->own_stream));
+    for (auto& g : rt->graphs) release_graph(rt, *g);
+    rt->graphs.clear();
+  }
+  rt->jacobi_method = method;
   return MSQ_OK;
 }
 

@@ -40,17 +40,17 @@
 
 #include <type_traits>
 
+#include "jacobi_math.hpp"
 #include "kernels.hpp"
 
 namespace msq {
 namespace {
 
-__device__ __forceinline__ int participant(int t, int x, int np) {
-  if (x == 0) return 0;
-  int r = x - 1 - t;
-  if (r < 0) r += np - 1;
-  return 1 + r;
-}
+using jmath::Block;
+using jmath::participant;
+using jmath::rotate_block;
+using jmath::rotate_rows_cols;
+using jmath::rotation_angles;
 
 __device__ __forceinline__ int tri(int i, int j) {  // i <= j
   return (j * (j + 1)) / 2 + i;
@@ -63,69 +63,6 @@ __device__ __forceinline__ int tri_col(int e) {
   while (j > 0 && (j * (j + 1)) / 2 > e) --j;
   while (((j + 1) * (j + 2)) / 2 <= e) ++j;
   return j;
-}
-
-// Rotation (c, s) for the pair p < q from a_pp, a_qq, a_pq, in binary64.
-// With d = a_qq - a_pp and e = 2 a_pq (tau = d / e),
-//   t = sign(d e) |e| / (|d| + sqrt(d^2 + e^2)) = sign(tau) / (|tau| + sqrt(1 + tau^2)),
-// c = 1 / sqrt(1 + t^2), s = t c.  The square roots and the reciprocal come
-// from the hardware estimates (v_rsq_f64, v_rcp_f64) refined by two Newton
-// steps each: a binary64 result within a few units of the last place, which
-// rounds to the same float as the correctly rounded quotients and roots
-// except at float rounding ties, and one third of their dependent chain (the
-// pair phase is on every round's critical path).  d^2 + e^2 neither
-// overflows nor underflows in binary64 for float inputs, and t is in [-1, 1].
-__device__ __forceinline__ double rsqrt_refined(double w) {  // w > 0, normal
-  double y = __builtin_amdgcn_rsq(w);
-  y = y * fma(-0.5 * w * y, y, 1.5);
-  y = y * fma(-0.5 * w * y, y, 1.5);
-  return y;
-}
-__device__ __forceinline__ float2 rotation_angles(float app, float aqq, float apq) {
-  if (apq == 0.0f) return make_float2(1.0f, 0.0f);
-  const double d = static_cast<double>(aqq) - static_cast<double>(app);
-  const double e = 2.0 * static_cast<double>(apq);
-  const double w = fma(d, d, e * e);
-  const double den = fabs(d) + w * rsqrt_refined(w);
-  double r = __builtin_amdgcn_rcp(den);
-  r = r * fma(-den, r, 2.0);
-  r = r * fma(-den, r, 2.0);
-  const double t = copysign(fabs(e) * r, d * e);
-  const double c = rsqrt_refined(fma(t, t, 1.0));
-  return make_float2(static_cast<float>(c), static_cast<float>(t * c));
-}
-
-// Rotated 2x2 block.  Rows (r0, r1) see (cr0, sr0) and (cr1, sr1); columns
-// (c0, c1) likewise; a_ij is the entry at (row i, column j).
-struct Block {
-  float a00, a01, a10, a11;
-};
-__device__ __forceinline__ Block rotate_block(Block b, float2 r0, float2 r1,
-                                              float2 c0, float2 c1) {
-  Block o;
-  o.a00 = r0.x * c0.x * b.a00 + r0.y * c0.x * b.a10 + r0.x * c0.y * b.a01 + r0.y * c0.y * b.a11;
-  o.a01 = r0.x * c1.x * b.a01 + r0.y * c1.x * b.a11 + r0.x * c1.y * b.a00 + r0.y * c1.y * b.a10;
-  o.a10 = r1.x * c0.x * b.a10 + r1.y * c0.x * b.a00 + r1.x * c0.y * b.a11 + r1.y * c0.y * b.a01;
-  o.a11 = r1.x * c1.x * b.a11 + r1.y * c1.x * b.a01 + r1.x * c1.y * b.a10 + r1.y * c1.y * b.a00;
-  return o;
-}
-
-
-// Off-diagonal 2 x 2 block {p_k, q_k} x {p_l, q_l} rotated on both sides:
-// the reference's element formula A'[r][c] = rr.x cc.x A[r][c] + rr.y cc.x
-// A[r'][c] + rr.x cc.y A[r][c'] + rr.y cc.y A[r'][c'] (p sees (c, -s), q
-// (c, s)) evaluated as rows first, then columns: t = J_k^T B, o = t J_l,
-// 16 multiply-adds with no coefficient products.
-__device__ __forceinline__ void rotate_rows_cols(const Block& b, float ck, float sk, float cl,
-                                                 float sl, Block& o) {
-  const float t00 = ck * b.a00 - sk * b.a10;
-  const float t01 = ck * b.a01 - sk * b.a11;
-  const float t10 = ck * b.a10 + sk * b.a00;
-  const float t11 = ck * b.a11 + sk * b.a01;
-  o.a00 = cl * t00 - sl * t01;
-  o.a01 = cl * t01 + sl * t00;
-  o.a10 = cl * t10 - sl * t11;
-  o.a11 = cl * t11 + sl * t10;
 }
 
 // Launch shape for a padded order N (a multiple of 16 up to 256).

@@ -51,7 +51,7 @@ SOLVE_STATUS = {0: "running", 1: "converged", 2: "budget_exhausted",
                 7: "unresolved"}
 JACOBI_HISTORY = 32
 SOLVE_HISTORY = 65
-ABI_VERSION = 5
+ABI_VERSION = 6
 # KernelPath / CorrectionMode / NativeStoppingPolicy / NativeDispatchPolicy /
 # RoundingMode (src/precision.hpp:22, :126-146), the reference's order.
 KERNEL_PATH_LEGACY, KERNEL_PATH_TILED = 0, 1
@@ -59,6 +59,8 @@ CORRECTION_FP32_REORTHOGONALIZED, CORRECTION_NATIVE_QUANTIZED = 0, 1
 STOPPING_FIXED_BUDGET, STOPPING_STAGNATION = 0, 1
 DISPATCH_PER_STAGE, DISPATCH_FUSED = 0, 1
 ROUNDING_NEAREST, ROUNDING_STOCHASTIC = 0, 1
+# msq_jacobi_method: eigensolver of the orders 129..256.
+JACOBI_SCALAR, JACOBI_BLOCKED_MFMA = 0, 1
 OWN_STREAM = 2**64 - 1  # MSQ_OWN_STREAM
 MLP_INPUTS = 16
 MLP_HIDDEN = 64
@@ -71,7 +73,7 @@ NOALLOC_MAX_MATRICES = 4096
 EXPORTED_SYMBOLS = (
     "msq_last_error", "msq_abi_version", "msq_runtime_create", "msq_runtime_destroy",
     "msq_runtime_set_stream", "msq_synchronize", "msq_runtime_set_graphs",
-    "msq_runtime_set_rounding", "msq_alloc", "msq_free",
+    "msq_runtime_set_rounding", "msq_runtime_set_jacobi_method", "msq_alloc", "msq_free",
     "msq_copy_to_device", "msq_copy_to_host", "msq_memset", "msq_storage_bytes",
     "msq_scale_bytes", "msq_spectrum_metadata_bytes", "msq_batch_layout", "msq_quantize",
     "msq_quantize_stream", "msq_decode",
@@ -195,6 +197,7 @@ _sig("msq_solve_batched_ex", _i, _p, _u32, _u32p, _p, ctypes.POINTER(SolveOutput
 _sig("msq_solve_options_init", None, ctypes.POINTER(SolveOptions))
 _sig("msq_refine_native_staged", _i, _p, _i, _u32, _u32p, _p, _p, _p, _p)
 _sig("msq_runtime_set_rounding", _i, _p, _i, _u32)
+_sig("msq_runtime_set_jacobi_method", _i, _p, _i)
 _sig("msq_quantize_stream", _i, _p, _i, _p, _p, _p, _sz, _u32)
 _sig("msq_spectrum_metadata_bytes", _sz, _i, _sz)
 _sig("msq_last_factors", _i, _p, ctypes.POINTER(_p), ctypes.POINTER(_p))
@@ -479,6 +482,12 @@ class Runtime:
     def set_rounding(self, mode=ROUNDING_NEAREST, seed=0):
         """MetalRuntime's rounding mode and seed (nearest or stochastic)."""
         check(lib.msq_runtime_set_rounding(self.handle, mode, seed))
+
+    def set_jacobi_method(self, method=JACOBI_BLOCKED_MFMA):
+        """Eigensolver of the orders 129..256: JACOBI_SCALAR (the reference's
+        rotations on the padded circle schedule) or JACOBI_BLOCKED_MFMA
+        (8-index blocks, pair subproblems, FP32 MFMA products)."""
+        check(lib.msq_runtime_set_jacobi_method(self.handle, method))
 
     # codecs -----------------------------------------------------------
     def quantize_stream(self, precision, src, values, scales=None, stream=0):

@@ -12,6 +12,14 @@ Functions and the reference code they follow:
                              jacobi_rotations_f32 / jacobi_similarity_vectors_f32,
                              metal_mixed_sqrt.metal:1331-1411 (whole round applied
                              out of place, as the reference's kernels do)
+* ``jacobi_tile``         -- the same rotations on blocks of 8 indices paired on the
+                             circle (the HIP library's blocked eigensolver of orders
+                             129..256): each 16 x 16 pair subproblem rotated by the
+                             formula of jacobi_rotations_f32 (all its pairs in a
+                             sweep's first round, its cross-block pairs after), the
+                             rest of the round applied as Q^T A Q; binary64 (no
+                             reference counterpart: the reference sweeps single
+                             pairs only)
 * ``frozen_factor``       -- prepareFrozenSpectralFactor,
                              src/metal_runtime.cpp:648-670, metal_mixed_sqrt.metal:1413-1434
 * ``reconstruct_f32``     -- reconstructTiled, src/metal_runtime.cpp:429-459
@@ -32,6 +40,9 @@ Pinning (what keeps this restatement honest):
 * schedule: every round is a perfect matching and every pair appears exactly
   once per sweep (closed-form round-robin property), checked in tests;
 * Jacobi: eigenvalues against LAPACK, orthogonality of V, off-norm decrease;
+* jacobi_tile: at order 16 (one block pair) a sweep is one scalar sweep of the
+  circle schedule, checked against jacobi_f32; at larger orders eigenvalues
+  against LAPACK, orthogonality, V^T A V against the rotated A;
 * products/residual/correction: closed forms against float64 numpy, and the
   reference's own known answer (identity basis, rhs 2, H = 0.5 gives 1
   exactly, src/metal_checks.cpp:537-601);
@@ -188,6 +199,92 @@ def jacobi_f32(A, sweeps=8, off_tolerance=None, max_sweeps=None,
     if return_history:
         info["history"] = history
     return V, roots, info
+
+
+def _circle(t, x, np_):
+    """Participant at slot x of round t (closed form of makeJacobiSchedule)."""
+    return 0 if x == 0 else 1 + ((x - 1 - t) % (np_ - 1))
+
+
+def _angles64(app, aqq, apq):
+    """(c, s) of the pair p < q in binary64: t = sign(d e) |e| / (|d| +
+    sqrt(d^2 + e^2)), d = a_qq - a_pp, e = 2 a_pq, c = 1 / sqrt(1 + t^2),
+    s = t c; (1, 0) where a_pq = 0 (.metal:1331-1361 in binary64)."""
+    d = aqq - app
+    e = 2.0 * apq
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = np.copysign(np.abs(e) / (np.abs(d) + np.sqrt(d * d + e * e)), d * e)
+        c = 1.0 / np.sqrt(1.0 + t * t)
+    nz = apq != 0
+    return np.where(nz, c, 1.0), np.where(nz, t * c, 0.0)
+
+
+def jacobi_tile(A, sweeps=1, off_tolerance=None, max_sweeps=30, return_history=False):
+    """Blocked cyclic Jacobi in binary64, the schedule of the HIP library's
+    blocked eigensolver.
+
+    The order is padded to N, a multiple of 16 (zero rows and columns), and
+    cut into NB = N / 8 blocks of 8 indices.  Round t of a sweep (NB - 1
+    rounds) pairs block P_t(k) with block P_t(NB - 1 - k) on the circle; the
+    pair's 16 indices (local 0..7 the first block, 8..15 the second) form the
+    subproblem S = A[pair, pair].  In round 0 it is swept by the circle
+    schedule over its 16 indices (15 rounds of 8 disjoint rotations), in the
+    other rounds over the 64 pairs across its two blocks only (8 rounds, local
+    u with 8 + (7 - u - r) mod 8 in round r): every pair of indices is rotated
+    exactly once per sweep, as in a scalar sweep.  Each rotation is A' = J^T A
+    J with J_pp = J_qq = c, J_pq = s, J_qp = -s, accumulated as Q <- Q J.  Then
+    A <- J^T A J and V <- V J for the block-diagonal J of the round's Q.
+
+    Returns (V, diagonal, info) like jacobi_f32, in binary64."""
+    A0 = np.asarray(A, F64)
+    n = A0.shape[0]
+    N = (n + 15) // 16 * 16
+    NB, M = N // 8, N // 16
+    B = np.zeros((N, N))
+    B[:n, :n] = A0
+    V = np.eye(N)
+    norm_a = float(np.linalg.norm(A0))
+    total = sweeps if off_tolerance is None else max_sweeps
+    history = []
+    done = 0
+    for _ in range(total):
+        for t in range(NB - 1):
+            idx = np.array([np.r_[8 * _circle(t, k, NB):8 * _circle(t, k, NB) + 8,
+                                  8 * _circle(t, NB - 1 - k, NB):
+                                  8 * _circle(t, NB - 1 - k, NB) + 8] for k in range(M)])
+            S = B[idx[:, :, None], idx[:, None, :]].copy()
+            Q = np.tile(np.eye(16), (M, 1, 1))
+            if t == 0:  # every pair of the 16 indices: the circle over 16
+                inner = [[(_circle(tau, u, 16), _circle(tau, 15 - u, 16)) for u in range(8)]
+                         for tau in range(15)]
+            else:  # the pairs across the two blocks only
+                inner = [[(u, 8 + (7 - u - r) % 8) for u in range(8)] for r in range(8)]
+            for pairs in inner:
+                for a, b in pairs:
+                    p, q = min(a, b), max(a, b)
+                    c, s = _angles64(S[:, p, p], S[:, q, q], S[:, p, q])
+                    c, s = c[:, None], s[:, None]
+                    sp, sq = S[:, p, :].copy(), S[:, q, :].copy()
+                    S[:, p, :], S[:, q, :] = c * sp - s * sq, s * sp + c * sq
+                    sp, sq = S[:, :, p].copy(), S[:, :, q].copy()
+                    S[:, :, p], S[:, :, q] = c * sp - s * sq, s * sp + c * sq
+                    qp, qq = Q[:, :, p].copy(), Q[:, :, q].copy()
+                    Q[:, :, p], Q[:, :, q] = c * qp - s * qq, s * qp + c * qq
+            J = np.zeros((N, N))
+            for k in range(M):
+                J[np.ix_(idx[k], idx[k])] = Q[k]
+            B = J.T @ B @ J
+            V = V @ J
+        done += 1
+        off = float(np.linalg.norm(B - np.diag(np.diag(B))))
+        history.append(off)
+        if off_tolerance is not None and off <= off_tolerance * norm_a:
+            break
+    info = {"sweeps": done, "off_norm": history[-1] if history else 0.0,
+            "matrix_norm": norm_a, "diagonal": np.diag(B)[:n].copy(), "rotated": B[:n, :n]}
+    if return_history:
+        info["history"] = history
+    return V[:n, :n], np.diag(B)[:n].copy(), info
 
 
 # ----------------------------------------------------------------------------

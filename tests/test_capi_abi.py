@@ -19,7 +19,7 @@ def test_library_exports_every_declared_symbol():
     assert sorted(capi.EXPORTED_SYMBOLS) == declared
     for name in declared:
         assert hasattr(capi.lib, name), name
-    assert capi.lib.msq_abi_version() == capi.ABI_VERSION == 5
+    assert capi.lib.msq_abi_version() == capi.ABI_VERSION == 6
 
 
 def test_storage_bytes_match_reference_formulas():

@@ -129,3 +129,31 @@ def test_accelerate_mixed_controlled_problem():
     assert 1e-7 < res[0] < 1e-5
     assert res[1] < 1e-10
     assert steps[-1][1] < 1e-12
+
+
+@pytest.mark.parametrize("sweeps", [1, 2, 3])
+def test_tile_jacobi_at_order_16_is_one_scalar_sweep(sweeps):
+    """One block pair: a blocked sweep is one sweep of the circle schedule over
+    all 16 indices, so the blocked oracle must follow jacobi_f32 to float
+    rounding (a wrong sign, index or accumulation order would not)."""
+    A = G.single_matrix(9, 16, 1e-2, index=0)[0]
+    V1, d1, _ = S.jacobi_tile(A, sweeps=sweeps)
+    V2, _, info = S.jacobi_f32(A, sweeps=sweeps, angle_dtype=np.float64)
+    scale = np.linalg.norm(A)
+    assert np.max(np.abs(d1 - info["diagonal"])) <= 1e-5 * scale
+    assert np.max(np.abs(V1 - V2)) <= 1e-4
+
+
+@pytest.mark.parametrize("n", [40, 150])
+def test_tile_jacobi_converges_to_lapack(n):
+    """Larger orders: eigenvalues against LAPACK, V orthogonal, V^T A V equal to
+    the rotated matrix, off(A) decreasing sweep by sweep."""
+    A = G.single_matrix(9, n, 1e-3, index=1)[0]
+    V, d, info = S.jacobi_tile(A, off_tolerance=1e-12, return_history=True)
+    lam = np.linalg.eigvalsh(A)
+    assert np.max(np.abs(np.sort(d) - lam)) <= 1e-11 * np.linalg.norm(A)
+    assert np.max(np.abs(V.T @ V - np.eye(n))) <= 1e-12
+    assert np.max(np.abs(V.T @ A @ V - info["rotated"])) <= 1e-12
+    h = info["history"]
+    assert all(b < a for a, b in zip(h, h[1:]))
+    assert info["sweeps"] <= 12
