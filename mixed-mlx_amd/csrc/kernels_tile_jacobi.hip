@@ -28,8 +28,8 @@
 // Storage.  A: the packed upper triangle of 8 x 8 tiles, tile (I, J), I <= J,
 // row-major (diagonal tiles use their upper triangle only), so every entry
 // a_ij = a_ji has one LDS word: N = 256 takes 528 tiles, 132 KiB.  Q_k: 16 x
-// 16 floats with entry (a, b) at b * 16 + 4 (a % 4) + a / 4, so each lane's
-// four MFMA operands of a product are one 16-byte read.  V lives in registers
+// 16 floats, each column four 16-byte chunks of the rows congruent mod 4
+// (q_at), so each lane's four MFMA operands of a product are one read.  V lives in registers
 // for the whole solve: lane (j, g) of wave w (j = lane % 16, g = lane / 16)
 // holds row 16 w + j of V at the columns of every block that are congruent
 // to g mod 4, two per block, by slot (vl[k] = slot k, vh[k] = slot NB-1-k);
@@ -61,14 +61,28 @@ using jmath::rotate_rows_cols;
 using jmath::rotation_angles;
 
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+// LDS pointers stay in the local address space (32-bit ds_* addressing).
+#define MSQ_LDS __attribute__((address_space(3)))
 
-constexpr int kTileThreads = 1024;
+#ifndef PAIRS_PER_WAVE
+#define PAIRS_PER_WAVE 1
+#endif
+// Pair subproblems (and 16-row groups of V) per wave.
+constexpr int kPairsPerWave = PAIRS_PER_WAVE;
+constexpr int kTileThreads = 1024 / kPairsPerWave;
 constexpr int kTileWaves = kTileThreads / 64;
 
 __host__ __device__ constexpr int tile_index(int I, int J) { return J * (J + 1) / 2 + I; }
 __device__ __forceinline__ int sigma16(int i) { return ((i & 3) << 2) | (i >> 2); }
-// Offset of Q entry (a, b) inside a pair's 256 floats.
-__device__ __forceinline__ int q_at(int a, int b) { return b * 16 + ((a & 3) << 2) + (a >> 2); }
+// Offset of Q entry (a, b) inside a pair's 256 floats: column b holds four
+// 16-byte chunks, chunk g the rows g, g + 4, g + 8, g + 12, at position
+// g ^ (b / 4) (so the 16-byte reads of 16 different columns by one lane group
+// fall in different LDS banks).
+__device__ __forceinline__ int q_at(int a, int b) {
+  return b * 16 + (((a & 3) ^ (b >> 2)) << 2) + (a >> 2);
+}
+__device__ __forceinline__ int q_chunk(int b, int g) { return b * 16 + ((g ^ (b >> 2)) << 2); }
 
 template <int NP>
 struct TileLds {
@@ -78,7 +92,7 @@ struct TileLds {
   static constexpr size_t kA = 0;
   static constexpr size_t kQ = kA + static_cast<size_t>(kTiles) * 64 * sizeof(float);
   static constexpr size_t kRot = kQ + static_cast<size_t>(M) * 256 * sizeof(float);
-  static constexpr size_t kRed = kRot + static_cast<size_t>(M) * 8 * sizeof(float2);
+  static constexpr size_t kRed = kRot + static_cast<size_t>(2 * kTileWaves) * 8 * sizeof(float2);
   static constexpr size_t kRedf = kRed + 2 * kTileWaves * sizeof(double);
   static constexpr size_t kCtrl = kRedf + 16 * sizeof(float);
   static constexpr size_t kBytes = kCtrl + 4 * sizeof(int);
@@ -97,7 +111,7 @@ __device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcn
 // Sums of two doubles over the workgroup in a fixed order: a butterfly inside
 // each wave, then the wave totals in wave order.  Every thread returns the
 // sums.
-__device__ __forceinline__ void tile_sum2(double& a, double& b, double* red) {
+__device__ __forceinline__ void tile_sum2(double& a, double& b, MSQ_LDS double* red) {
   for (int s = 32; s > 0; s >>= 1) {
     a += __shfl_xor(a, s);
     b += __shfl_xor(b, s);
@@ -159,24 +173,37 @@ __device__ __forceinline__ float dpp_shr1(float v) {  // lane x - 1 (lane 0: 0)
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
 }
 
-// One inner round.  FULL: the circle over 16 (slot y >= 1 holds local index
-// 1 + ((y - 1 - tau) mod 15), slot 0 index 0; the slots then move y -> y + 1,
-// 15 -> 1).  Otherwise: slot k < 8 holds local k and slot y >= 8 holds
-// 8 + ((y - 8 - tau) mod 8), so slot k pairs local k with 8 + ((7 - k - tau)
-// mod 8) (the pairs across the two blocks); then the slots 8..15 move
-// y -> y + 1, 15 -> 8.  rw: this wave's 8 (c, sigma) records, sigma the
-// coefficient of the pair's lower slot k (new row k = c row k + sigma row
-// 15 - k; the upper slot has -sigma).
-template <bool FULL>
-__device__ __forceinline__ void sub_round(SubState& st, int tau, int x, int g, float2* rw) {
-  // Rotation inputs, held by lane (x, g) with g = min(x, 15 - x) / 2: its
-  // diagonal entry and the entry of its row in its partner's column.
+// One inner round of two subproblems (independent, interleaved for
+// instruction-level parallelism).  FULL: the circle over 16 (slot y >= 1 holds
+// local index 1 + ((y - 1 - tau) mod 15), slot 0 index 0; the slots then move
+// y -> y + 1, 15 -> 1).  Otherwise: slot k < 8 holds local k and slot y >= 8
+// holds 8 + ((y - 8 - tau) mod 8), so slot k pairs local k with
+// 8 + ((7 - k - tau) mod 8) (the pairs across the two blocks); then the slots
+// 8..15 move y -> y + 1, 15 -> 8.  rw[u]: subproblem u's 8 (c, sigma)
+// records, sigma the coefficient of the pair's lower slot k (new row k =
+// c row k + sigma row 15 - k; the upper slot has -sigma).
+template <bool FULL, int P>
+__device__ __forceinline__ void sub_round(SubState (&st)[P], int tau, int x, int g,
+                                          MSQ_LDS f2* const (&rw)[P]) {
   const int kx = min(x, 15 - x);
   const bool even = (kx & 1) == 0, lo = x < 8;
-  const float diag = lo ? (even ? st.s0 : st.s2) : (even ? st.s1 : st.s3);
-  const float off = lo ? (even ? st.s1 : st.s3) : (even ? st.s0 : st.s2);
-  const float pdiag = dpp_mirror(diag);
-  const float poff = dpp_mirror(off);
+  // Rotation inputs, held by lane (x, g) with g = min(x, 15 - x) / 2: its
+  // diagonal entry and the entry of its row in its partner's column.  Values,
+  // not struct fields, go into the selects (a select between two fields
+  // becomes a load from a select of their addresses, which keeps the state in
+  // private memory).
+  // The partner's values come by DPP with every lane active (a DPP read of an
+  // inactive lane returns nothing).
+  float diag[P], off[P], pdiag[P], poff[P];
+#pragma unroll
+  for (int u = 0; u < P; ++u) {
+    float v0 = st[u].s0, v1 = st[u].s1, v2 = st[u].s2, v3 = st[u].s3;
+    asm volatile("" : "+v"(v0), "+v"(v1), "+v"(v2), "+v"(v3));
+    diag[u] = lo ? (even ? v0 : v2) : (even ? v1 : v3);
+    off[u] = lo ? (even ? v1 : v3) : (even ? v0 : v2);
+    pdiag[u] = dpp_mirror(diag[u]);
+    poff[u] = dpp_mirror(off[u]);
+  }
   if (lo && g == (x >> 1)) {
     int a, b;  // local indices at slots x and 15 - x
     if constexpr (FULL) {
@@ -188,91 +215,106 @@ __device__ __forceinline__ void sub_round(SubState& st, int tau, int x, int g, f
     }
     // The pair p < q: a_pp, a_qq, and a_pq from row p.
     const bool xp = a < b;
-    const float2 r = rotation_angles(xp ? diag : pdiag, xp ? pdiag : diag, xp ? off : poff);
-    rw[x] = make_float2(r.x, xp ? -r.y : r.y);
+#pragma unroll
+    for (int u = 0; u < P; ++u) {
+      const float2 r = rotation_angles(xp ? diag[u] : pdiag[u], xp ? pdiag[u] : diag[u],
+                                       xp ? off[u] : poff[u]);
+      rw[u][x] = f2{r.x, xp ? -r.y : r.y};
+    }
   }
   wave_lds_sync();
-  const float2 rr = rw[kx], ra = rw[2 * g], rb = rw[2 * g + 1];
+  f2 rr[P], ra[P], rb[P];
+#pragma unroll
+  for (int u = 0; u < P; ++u) {
+    rr[u] = rw[u][kx];
+    ra[u] = rw[u][2 * g];
+    rb[u] = rw[u][2 * g + 1];
+  }
   wave_lds_sync();  // the next round's records overwrite these
-  // Rows: new row x = c row x + sigma_x row 15 - x.
-  {
-    const float sg = lo ? rr.y : -rr.y;
-    const float p0 = dpp_mirror(st.s0), p1 = dpp_mirror(st.s1);
-    const float p2 = dpp_mirror(st.s2), p3 = dpp_mirror(st.s3);
-    st.s0 = rr.x * st.s0 + sg * p0;
-    st.s1 = rr.x * st.s1 + sg * p1;
-    st.s2 = rr.x * st.s2 + sg * p2;
-    st.s3 = rr.x * st.s3 + sg * p3;
-  }
-  // Columns of S and Q: slots 2g / 15 - 2g with ra, 2g + 1 / 14 - 2g with rb.
-  {
-    float t0 = ra.x * st.s0 + ra.y * st.s1;
-    float t1 = ra.x * st.s1 - ra.y * st.s0;
-    float t2 = rb.x * st.s2 + rb.y * st.s3;
-    float t3 = rb.x * st.s3 - rb.y * st.s2;
-    st.s0 = t0;
-    st.s1 = t1;
-    st.s2 = t2;
-    st.s3 = t3;
-    t0 = ra.x * st.q0 + ra.y * st.q1;
-    t1 = ra.x * st.q1 - ra.y * st.q0;
-    t2 = rb.x * st.q2 + rb.y * st.q3;
-    t3 = rb.x * st.q3 - rb.y * st.q2;
-    st.q0 = t0;
-    st.q1 = t1;
-    st.q2 = t2;
-    st.q3 = t3;
-  }
-  // Slots move for the next round: rows of S (lanes), columns of S and Q
-  // (registers, and lane groups g +- 1).
   const int lane = threadIdx.x & 63;
-  if constexpr (FULL) {
-    auto rows = [&](float v) {
-      const float r1 = dpp_ror1(v);
-      const float w = dpp_shr1(r1);
-      return x == 0 ? v : (x == 1 ? w : r1);
-    };
-    st.s0 = rows(st.s0);
-    st.s1 = rows(st.s1);
-    st.s2 = rows(st.s2);
-    st.s3 = rows(st.s3);
-    // new slot 2g <- 2g - 1 (group g - 1's c2; slot 0 stays), 2g + 1 <- 2g
-    // (slot 1 <- 15), 15 - 2g <- 14 - 2g, 14 - 2g <- 13 - 2g (group g + 1's
-    // c1; slot 8 <- 7).
-    const int dn = lane >= 16 ? lane - 16 : lane, up = lane < 48 ? lane + 16 : lane;
-    auto cols = [&](float& v0, float& v1, float& v2, float& v3) {
-      const float from_dn = __shfl(v2, dn);
-      const float from_up = __shfl(v1, up);
-      const float n0 = g == 0 ? v0 : from_dn;
-      const float n2 = g == 0 ? v1 : v0;
-      const float n1 = v3;
-      const float n3 = g == 3 ? v2 : from_up;
-      v0 = n0;
-      v1 = n1;
-      v2 = n2;
-      v3 = n3;
-    };
-    cols(st.s0, st.s1, st.s2, st.s3);
-    cols(st.q0, st.q1, st.q2, st.q3);
-  } else {
-    auto rows = [&](float v) {
-      const float r1 = dpp_ror1(v);
-      const float r9 = dpp_ror9(v);
-      return x < 8 ? v : (x == 8 ? r9 : r1);
-    };
-    st.s0 = rows(st.s0);
-    st.s1 = rows(st.s1);
-    st.s2 = rows(st.s2);
-    st.s3 = rows(st.s3);
-    // new slot 15 - 2g <- 14 - 2g (own c3), 14 - 2g <- 13 - 2g (group
-    // g + 1's c1; slot 8 <- 15, group 0's c1); slots 0..7 stay.
-    const int up = (lane + 16) & 63;
-    const float su = __shfl(st.s1, up), qu = __shfl(st.q1, up);
-    st.s1 = st.s3;
-    st.s3 = su;
-    st.q1 = st.q3;
-    st.q3 = qu;
+#pragma unroll
+  for (int u = 0; u < P; ++u) {
+    SubState& q = st[u];
+    // Rows: new row x = c row x + sigma_x row 15 - x.
+    const float sg = lo ? rr[u].y : -rr[u].y;
+    const float p0 = dpp_mirror(q.s0), p1 = dpp_mirror(q.s1);
+    const float p2 = dpp_mirror(q.s2), p3 = dpp_mirror(q.s3);
+    float s0 = rr[u].x * q.s0 + sg * p0;
+    float s1 = rr[u].x * q.s1 + sg * p1;
+    float s2 = rr[u].x * q.s2 + sg * p2;
+    float s3 = rr[u].x * q.s3 + sg * p3;
+    // Columns of S and Q: slots 2g / 15 - 2g with ra, 2g + 1 / 14 - 2g with rb.
+    q.s0 = ra[u].x * s0 + ra[u].y * s1;
+    q.s1 = ra[u].x * s1 - ra[u].y * s0;
+    q.s2 = rb[u].x * s2 + rb[u].y * s3;
+    q.s3 = rb[u].x * s3 - rb[u].y * s2;
+    const float t0 = ra[u].x * q.q0 + ra[u].y * q.q1;
+    const float t1 = ra[u].x * q.q1 - ra[u].y * q.q0;
+    const float t2 = rb[u].x * q.q2 + rb[u].y * q.q3;
+    const float t3 = rb[u].x * q.q3 - rb[u].y * q.q2;
+    q.q0 = t0;
+    q.q1 = t1;
+    q.q2 = t2;
+    q.q3 = t3;
+    // Slots move for the next round: rows of S (lanes), columns of S and Q
+    // (registers, and lane groups g +- 1).
+    if constexpr (FULL) {
+      auto rows = [x](float v) {
+        const float r1 = dpp_ror1(v);
+        const float w = dpp_shr1(r1);
+        return x == 0 ? v : (x == 1 ? w : r1);
+      };
+      q.s0 = rows(q.s0);
+      q.s1 = rows(q.s1);
+      q.s2 = rows(q.s2);
+      q.s3 = rows(q.s3);
+      // new slot 2g <- 2g - 1 (group g - 1's c2; slot 0 stays), 2g + 1 <- 2g
+      // (slot 1 <- 15), 15 - 2g <- 14 - 2g, 14 - 2g <- 13 - 2g (group g + 1's
+      // c1; slot 8 <- 7).
+      const int dn = lane >= 16 ? lane - 16 : lane, up = lane < 48 ? lane + 16 : lane;
+      auto cols = [g, dn, up](float& v0, float& v1, float& v2, float& v3) {
+        const float from_dn = __shfl(v2, dn);
+        const float from_up = __shfl(v1, up);
+        const float n0 = g == 0 ? v0 : from_dn;
+        const float n2 = g == 0 ? v1 : v0;
+        const float n1 = v3;
+        const float n3 = g == 3 ? v2 : from_up;
+        v0 = n0;
+        v1 = n1;
+        v2 = n2;
+        v3 = n3;
+      };
+      cols(q.s0, q.s1, q.s2, q.s3);
+      cols(q.q0, q.q1, q.q2, q.q3);
+    } else {
+      auto rows = [x](float v) {
+        const float r1 = dpp_ror1(v);
+        const float r9 = dpp_ror9(v);
+        return x < 8 ? v : (x == 8 ? r9 : r1);
+      };
+      q.s0 = rows(q.s0);
+      q.s1 = rows(q.s1);
+      q.s2 = rows(q.s2);
+      q.s3 = rows(q.s3);
+      // new slot 15 - 2g <- 14 - 2g (own c3), 14 - 2g <- 13 - 2g (group
+      // g + 1's c1; slot 8 <- 15, group 0's c1); slots 0..7 stay.
+      const int up = (lane + 16) & 63;
+      const float su = __shfl(q.s1, up), qu = __shfl(q.q1, up);
+      q.s1 = q.s3;
+      q.s3 = su;
+      q.q1 = q.q3;
+      q.q3 = qu;
+    }
   }
+}
+
+// Offset of s_ab of a pair subproblem, local indices a, b (0..7: block bp,
+// 8..15: block bq; tpp, tqq: the offsets of the two diagonal tiles); s_ab
+// and s_ba share one word.
+__device__ __forceinline__ int sub_at(int a, int b, int bp, int bq, int tpp, int tqq) {
+  const int ia = a & 7, ib = b & 7;
+  if ((a >> 3) == (b >> 3)) return (a < 8 ? tpp : tqq) + min(ia, ib) * 8 + max(ia, ib);
+  return a < 8 ? cross_at(bp, ia, bq, ib) : cross_at(bp, ib, bq, ia);
 }
 
 template <int NP>
@@ -282,7 +324,7 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
                                                   float* __restrict__ diagonal,
                                                   float* __restrict__ roots,
                                                   JacobiStatus* __restrict__ status,
-                                                  unsigned char* __restrict__ smem) {
+                                                  MSQ_LDS unsigned char* __restrict__ smem) {
   using L = TileLds<NP>;
   constexpr int NB = L::NB;
   constexpr int M = L::M;
@@ -290,17 +332,18 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
   constexpr int kItems = M * (M - 1) / 2;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  // Wave-uniform (scalar registers for everything derived from it).
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nthreads = static_cast<int>(blockDim.x);
   const int nwaves = nthreads >> 6;
   const int n = static_cast<int>(job.order);
   const int ld = static_cast<int>(job.ld);
-  float* const At = reinterpret_cast<float*>(smem + L::kA);
-  float* const Qs = reinterpret_cast<float*>(smem + L::kQ);
-  float2* const rot = reinterpret_cast<float2*>(smem + L::kRot);
-  double* const red = reinterpret_cast<double*>(smem + L::kRed);
-  float* const redf = reinterpret_cast<float*>(smem + L::kRedf);
-  int* const ctrl = reinterpret_cast<int*>(smem + L::kCtrl);
+  MSQ_LDS float* const At = reinterpret_cast<MSQ_LDS float*>(smem + L::kA);
+  MSQ_LDS float* const Qs = reinterpret_cast<MSQ_LDS float*>(smem + L::kQ);
+  MSQ_LDS f2* const rot = reinterpret_cast<MSQ_LDS f2*>(smem + L::kRot);
+  MSQ_LDS double* const red = reinterpret_cast<MSQ_LDS double*>(smem + L::kRed);
+  MSQ_LDS float* const redf = reinterpret_cast<MSQ_LDS float*>(smem + L::kRedf);
+  MSQ_LDS int* const ctrl = reinterpret_cast<MSQ_LDS int*>(smem + L::kCtrl);
 
   // A into the tiles: element (r, c) of tile (I, J) = a_{8I+r, 8J+c}; rows
   // fastest, so that 8 lanes read 64 consecutive bytes of a column.
@@ -324,19 +367,28 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
   tile_sum2(norm2, unused, red);
   const double matrix_norm = sqrt(norm2);
 
-  // V = I in registers.  Lane (j, g) of wave w < M: row 16 w + j, columns
-  // 8 b + 4 h + g of the block b at each slot (round 0: slot x holds block x).
+  // V = I in registers.  Lane (j, g) of wave w: rows 16 (2 w + h2) + j,
+  // h2 = 0, 1 (row groups below M), columns 8 b + 4 h + g of the block b at
+  // each slot (round 0: slot x holds block x).
   const int vj = lane & 15, vg = lane >> 4;
-  const int vrow = 16 * wave + vj;
-  float vl[M][2], vh[M][2];
+  constexpr int P = kPairsPerWave;
+  float vl[P][M][2], vh[P][M][2];
 #pragma unroll
-  for (int k = 0; k < M; ++k) {
+  for (int h2 = 0; h2 < P; ++h2) {
+    const int vrow = 16 * (P * wave + h2) + vj;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      vl[k][h] = vrow == 8 * k + 4 * h + vg ? 1.0f : 0.0f;
-      vh[k][h] = vrow == 8 * (NB - 1 - k) + 4 * h + vg ? 1.0f : 0.0f;
+    for (int k = 0; k < M; ++k) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        vl[h2][k][h] = vrow == 8 * k + 4 * h + vg ? 1.0f : 0.0f;
+        vh[h2][k][h] = vrow == 8 * (NB - 1 - k) + 4 * h + vg ? 1.0f : 0.0f;
+      }
     }
   }
+  // This wave's pairs (the second one a copy of the first, storing nothing,
+  // when M is odd and this is the last wave with pairs).
+  const bool has_pairs = P * wave < M;
+  const int pairs = min(P, M - P * wave);
 
   const uint32_t fixed = job.fixed_sweeps;
   const uint32_t cap = fixed ? fixed : job.max_sweeps;
@@ -351,130 +403,177 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
 #pragma unroll 1
   for (;;) {
     const int round = t % rounds_per_sweep;
-    // ---- 1. subproblems: wave k rotates pair k's 16 x 16 block in registers
-    // (round 0 of a sweep: all its index pairs; later rounds: the pairs
-    // across its two blocks), writes it back and stores Q_k.
-    if (wave < M) {
-      const int k = wave;
-      const int bp = participant(round, k, NB);
-      const int bq = participant(round, NB - 1 - k, NB);
-      const int tpp = tile_index(bp, bp) * 64, tqq = tile_index(bq, bq) * 64;
-      // Offset of s_ab, local indices a, b (0..7: block bp, 8..15: block bq);
-      // s_ab and s_ba share one word.
-      auto s_at = [&](int a, int b) -> int {
-        const int ia = a & 7, ib = b & 7;
-        if ((a >> 3) == (b >> 3)) return (a < 8 ? tpp : tqq) + min(ia, ib) * 8 + max(ia, ib);
-        return a < 8 ? cross_at(bp, ia, bq, ib) : cross_at(bp, ib, bq, ia);
-      };
+    // ---- 1. subproblems: wave w rotates pairs 2w and 2w + 1 in registers
+    // (round 0 of a sweep: all their index pairs; later rounds: the pairs
+    // across their two blocks), writes them back and stores Q_k.
+    if (has_pairs) {
       const int x = lane & 15, g = lane >> 4;
       const int c0 = 2 * g, c1 = 15 - 2 * g, c2 = 2 * g + 1, c3 = 14 - 2 * g;
-      SubState st{At[s_at(x, c0)], At[s_at(x, c1)], At[s_at(x, c2)], At[s_at(x, c3)],
-                  x == c0 ? 1.0f : 0.0f, x == c1 ? 1.0f : 0.0f,
-                  x == c2 ? 1.0f : 0.0f, x == c3 ? 1.0f : 0.0f};
-      float2* const rw = rot + k * 8;
+      int bp[P], bq[P], tpp[P], tqq[P];
+      SubState sub[P];
+#pragma unroll
+      for (int u = 0; u < P; ++u) {
+        const int k = P * wave + (u < pairs ? u : 0);
+        bp[u] = participant(round, k, NB);
+        bq[u] = participant(round, NB - 1 - k, NB);
+        tpp[u] = tile_index(bp[u], bp[u]) * 64;
+        tqq[u] = tile_index(bq[u], bq[u]) * 64;
+        sub[u] = SubState{At[sub_at(x, c0, bp[u], bq[u], tpp[u], tqq[u])],
+                          At[sub_at(x, c1, bp[u], bq[u], tpp[u], tqq[u])],
+                          At[sub_at(x, c2, bp[u], bq[u], tpp[u], tqq[u])],
+                          At[sub_at(x, c3, bp[u], bq[u], tpp[u], tqq[u])],
+                          x == c0 ? 1.0f : 0.0f, x == c1 ? 1.0f : 0.0f,
+                          x == c2 ? 1.0f : 0.0f, x == c3 ? 1.0f : 0.0f};
+      }
+      MSQ_LDS f2* rw[P];
+#pragma unroll
+      for (int u = 0; u < P; ++u) rw[u] = rot + (P * wave + u) * 8;
       if (round == 0) {
 #pragma unroll 1
-        for (int tau = 0; tau < 15; ++tau) sub_round<true>(st, tau, x, g, rw);
+        for (int tau = 0; tau < 15; ++tau) sub_round<true, P>(sub, tau, x, g, rw);
       } else {
 #pragma unroll 1
-        for (int tau = 0; tau < 8; ++tau) sub_round<false>(st, tau, x, g, rw);
+        for (int tau = 0; tau < 8; ++tau) sub_round<false, P>(sub, tau, x, g, rw);
       }
       // Slot order is local order again (15 circle rounds, 8 half-rotations).
-      const float sv[4] = {st.s0, st.s1, st.s2, st.s3};
-      const float qv[4] = {st.q0, st.q1, st.q2, st.q3};
-      const int cols[4] = {c0, c1, c2, c3};
-      float* const Q = Qs + k * 256;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int y = cols[j];
-        const bool own = (x >> 3) == (y >> 3) ? x <= y : x < y;
-        if (own) At[s_at(x, y)] = sv[j];
-        Q[q_at(x, y)] = qv[j];
+      for (int u = 0; u < P; ++u) {
+        if (u >= pairs) break;
+        const float sv[4] = {sub[u].s0, sub[u].s1, sub[u].s2, sub[u].s3};
+        const float qv[4] = {sub[u].q0, sub[u].q1, sub[u].q2, sub[u].q3};
+        const int cols[4] = {c0, c1, c2, c3};
+        MSQ_LDS float* const Q = Qs + (P * wave + u) * 256;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int y = cols[j];
+          const bool own = (x >> 3) == (y >> 3) ? x <= y : x < y;
+          if (own) At[sub_at(x, y, bp[u], bq[u], tpp[u], tqq[u])] = sv[j];
+          Q[q_at(x, y)] = qv[j];
+        }
       }
     }
     __syncthreads();
 
-    // ---- 2. products.  V: this wave's 16 rows, every pair.
-    if (wave < M) {
+    // ---- 2. products.  V: this wave's two row groups, every pair (one Q
+    // operand read serves both).
+    if (has_pairs) {
       const int i = lane & 15, g = lane >> 4;
 #pragma unroll
       for (int k = 0; k < M; ++k) {
-        const f4 qa = *reinterpret_cast<const f4*>(Qs + k * 256 + sigma16(i) * 16 + 4 * g);
-        f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[0], vl[k][0], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[1], vl[k][1], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[2], vh[k][0], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[3], vh[k][1], acc, 0, 0, 0);
-        vl[k][0] = acc[0];
-        vl[k][1] = acc[1];
-        vh[k][0] = acc[2];
-        vh[k][1] = acc[3];
+        const f4 qa = *reinterpret_cast<const MSQ_LDS f4*>(Qs + k * 256 + q_chunk(sigma16(i), g));
+#pragma unroll
+        for (int h2 = 0; h2 < P; ++h2) {
+          f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[0], vl[h2][k][0], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[1], vl[h2][k][1], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[2], vh[h2][k][0], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[3], vh[h2][k][1], acc, 0, 0, 0);
+          vl[h2][k][0] = acc[0];
+          vl[h2][k][1] = acc[1];
+          vh[h2][k][0] = acc[2];
+          vh[h2][k][1] = acc[3];
+        }
       }
     }
-    // Off-diagonal tile-blocks (k, l), k < l, dealt to every wave.
+    // Off-diagonal tile-blocks (k, l), k < l, dealt to every wave.  Per item
+    // the four tiles (row half a: block bk[a], column half b: block bl[b])
+    // have wave-uniform offsets and orientations; a lane only selects among
+    // them by its halves.
     {
       const int i = lane & 15, g = lane >> 4;
       const int rho = sigma16(i);
+      const int hr = rho >> 3;  // row half of this lane's A-operand row
+      // In-tile offsets of T[rho][4 s + g] when the stored tile is (row
+      // block, column block) (offr) or (column block, row block) (offc).
+      int offr[4], offc[4];
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2) {
+        const int y8 = 4 * (s2 & 1) + g;
+        offr[s2] = (rho & 7) * 8 + y8;
+        offc[s2] = y8 * 8 + (rho & 7);
+      }
+      const int hs = g >> 1, hc = i >> 3, r8 = 4 * (g & 1), i8 = i & 7;
+      // This wave's items it = wave + j nwaves, row-major over k < l; (k, l)
+      // advanced in place (scalar registers).
+      int k = 0, l = wave;
+      while (k < M - 1 && l >= M - 1 - k) {
+        l -= M - 1 - k;
+        ++k;
+      }
+      l += k + 1;
 #pragma unroll 1
       for (int it = wave; it < kItems; it += nwaves) {
-        // item -> (k, l), row-major over k < l
-        int k = 0, rem = it;
-        while (rem >= M - 1 - k) {
-          rem -= M - 1 - k;
-          ++k;
-        }
-        const int l = k + 1 + rem;
-        const int bpk = participant(round, k, NB), bqk = participant(round, NB - 1 - k, NB);
-        const int bpl = participant(round, l, NB), bql = participant(round, NB - 1 - l, NB);
+        const int bk[2] = {participant(round, k, NB), participant(round, NB - 1 - k, NB)};
+        const int bl[2] = {participant(round, l, NB), participant(round, NB - 1 - l, NB)};
+        int base[4], upm = 0;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            const int X = bk[a], Y = bl[b];
+            const int lo = min(X, Y), hi = max(X, Y);
+            base[2 * a + b] = (hi * (hi + 1) / 2 + lo) * 64;
+            upm |= (X < Y ? 1 : 0) << (2 * a + b);
+          }
         // Z = T' Q_l: A operand T[sigma(i)][4 s + g].
-        const int rb = rho < 8 ? bpk : bqk;
         float ta[4];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int cb = s < 2 ? bpl : bql;
-          ta[s] = At[cross_at(rb, rho & 7, cb, 4 * (s & 1) + g)];
+        for (int s2 = 0; s2 < 4; ++s2) {
+          const int t = 2 * hr + (s2 >> 1);
+          const int b0 = hr ? base[2 + (s2 >> 1)] : base[s2 >> 1];
+          ta[s2] = At[b0 + (((upm >> t) & 1) ? offr[s2] : offc[s2])];
         }
-        const f4 ql = *reinterpret_cast<const f4*>(Qs + l * 256 + i * 16 + 4 * g);
-        const f4 qk = *reinterpret_cast<const f4*>(Qs + k * 256 + i * 16 + 4 * g);
+        const f4 ql = *reinterpret_cast<const MSQ_LDS f4*>(Qs + l * 256 + q_chunk(i, g));
+        const f4 qk = *reinterpret_cast<const MSQ_LDS f4*>(Qs + k * 256 + q_chunk(i, g));
         f4 z = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int s = 0; s < 4; ++s) z = __builtin_amdgcn_mfma_f32_16x16x4f32(ta[s], ql[s], z, 0, 0, 0);
+        for (int s2 = 0; s2 < 4; ++s2)
+          z = __builtin_amdgcn_mfma_f32_16x16x4f32(ta[s2], ql[s2], z, 0, 0, 0);
         f4 o = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int s = 0; s < 4; ++s) o = __builtin_amdgcn_mfma_f32_16x16x4f32(qk[s], z[s], o, 0, 0, 0);
-        // o[r] = T_new[4 g + r][j], j = i.
-        const int rb2 = g < 2 ? bpk : bqk;
-        const int cb2 = i < 8 ? bpl : bql;
-        const int r8 = 4 * (g & 1);
-        if (rb2 < cb2) {
-          float* dst = At + tile_index(rb2, cb2) * 64 + r8 * 8 + (i & 7);
+        for (int s2 = 0; s2 < 4; ++s2)
+          o = __builtin_amdgcn_mfma_f32_16x16x4f32(qk[s2], z[s2], o, 0, 0, 0);
+        // o[r] = T_new[4 g + r][j], j = i: row half g / 2, column half i / 8.
+        {
+          const int t = 2 * hs + hc;
+          const int b0 = hs ? (hc ? base[3] : base[2]) : (hc ? base[1] : base[0]);
+          if ((upm >> t) & 1) {
+            MSQ_LDS float* dst = At + b0 + r8 * 8 + i8;
 #pragma unroll
-          for (int r = 0; r < 4; ++r) dst[8 * r] = o[r];
-        } else {
-          *reinterpret_cast<f4*>(At + tile_index(cb2, rb2) * 64 + (i & 7) * 8 + r8) = o;
+            for (int r = 0; r < 4; ++r) dst[8 * r] = o[r];
+          } else {
+            *reinterpret_cast<MSQ_LDS f4*>(At + b0 + i8 * 8 + r8) = o;
+          }
+        }
+        // Next item: nwaves further along the rows.
+        l += nwaves;
+        while (k < M - 2 && l > M - 1) {
+          l += k + 2 - M;
+          ++k;
         }
       }
     }
     __syncthreads();
     // V: every block moves one slot along the circle (slot x -> x + 1,
     // NB - 1 -> 1, 0 stays).
-    {
-      float l1[2] = {vh[0][0], vh[0][1]};
-      float hl[2] = {vl[M - 1][0], vl[M - 1][1]};
+#pragma unroll
+    for (int h2 = 0; h2 < P; ++h2) {
+      const float l1[2] = {vh[h2][0][0], vh[h2][0][1]};
+      const float hl[2] = {vl[h2][M - 1][0], vl[h2][M - 1][1]};
 #pragma unroll
       for (int k = M - 1; k >= 2; --k) {
-        vl[k][0] = vl[k - 1][0];
-        vl[k][1] = vl[k - 1][1];
+        vl[h2][k][0] = vl[h2][k - 1][0];
+        vl[h2][k][1] = vl[h2][k - 1][1];
       }
-      vl[1][0] = l1[0];
-      vl[1][1] = l1[1];
+      vl[h2][1][0] = l1[0];
+      vl[h2][1][1] = l1[1];
 #pragma unroll
       for (int k = 0; k < M - 1; ++k) {
-        vh[k][0] = vh[k + 1][0];
-        vh[k][1] = vh[k + 1][1];
+        vh[h2][k][0] = vh[h2][k + 1][0];
+        vh[h2][k][1] = vh[h2][k + 1][1];
       }
-      vh[M - 1][0] = hl[0];
-      vh[M - 1][1] = hl[1];
+      vh[h2][M - 1][0] = hl[0];
+      vh[h2][M - 1][1] = hl[1];
     }
     ++t;
     if (t % rounds_per_sweep != 0) continue;
@@ -482,7 +581,7 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
     // ---- Convergence after each sweep.
     sweeps = static_cast<uint32_t>(t / rounds_per_sweep);
     const bool at_limit = sweeps >= cap;
-    float* sroot = Qs;  // idle until the next round
+    MSQ_LDS float* sroot = Qs;  // idle until the next round
 #pragma unroll 1
     for (int i = tid; i < NP; i += nthreads) {
       const float d = At[tile_index(i >> 3, i >> 3) * 64 + (i & 7) * 9];
@@ -509,7 +608,7 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
         g2 += 2.0 * static_cast<double>(gg * gg);
       }
     }
-    if (dmin < INFINITY) atomicMin(&redf[0], dmin);
+    if (dmin < INFINITY) atomicMin((float*)redf, dmin);
     tile_sum2(off2, g2, red);
     if (tid == 0) {
       const float d = redf[0];
@@ -558,16 +657,21 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
     const int x = e / ld;
     if (i >= n || x >= n) Vg[e] = 0.0f;
   }
-  if (wave < M && vrow < n) {
+  {
     const int round = t % rounds_per_sweep;
 #pragma unroll
-    for (int k = 0; k < M; ++k) {
-      const int bl = participant(round, k, NB), bh = participant(round, NB - 1 - k, NB);
+    for (int h2 = 0; h2 < P; ++h2) {
+      const int vrow = 16 * (P * wave + h2) + vj;
+      if (vrow >= n) continue;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int cl = 8 * bl + 4 * h + vg, ch = 8 * bh + 4 * h + vg;
-        if (cl < n) Vg[vrow + static_cast<size_t>(cl) * ld] = vl[k][h];
-        if (ch < n) Vg[vrow + static_cast<size_t>(ch) * ld] = vh[k][h];
+      for (int k = 0; k < M; ++k) {
+        const int bl = participant(round, k, NB), bh = participant(round, NB - 1 - k, NB);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int cl = 8 * bl + 4 * h + vg, ch = 8 * bh + 4 * h + vg;
+          if (cl < n) Vg[vrow + static_cast<size_t>(cl) * ld] = vl[h2][k][h];
+          if (ch < n) Vg[vrow + static_cast<size_t>(ch) * ld] = vh[h2][k][h];
+        }
       }
     }
   }
@@ -590,10 +694,10 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
 // register allocation.
 template <int NP>
 __device__ __attribute__((noinline)) void tile_job(
-    const JacobiJob& job, const double* __restrict__ a64, float* __restrict__ vectors,
+    const JacobiJob* __restrict__ job, const double* __restrict__ a64, float* __restrict__ vectors,
     float* __restrict__ diagonal, float* __restrict__ roots, JacobiStatus* __restrict__ status,
     __attribute__((address_space(3))) unsigned char* lds) {
-  tile_jacobi_solve<NP>(job, a64, vectors, diagonal, roots, status, (unsigned char*)lds);
+  tile_jacobi_solve<NP>(*job, a64, vectors, diagonal, roots, status, lds);
 }
 
 // Persistent launch: one 1024-thread workgroup per CU takes the next job
@@ -610,11 +714,13 @@ __global__ __launch_bounds__(kTileThreads) void tile_jacobi_kernel(
   for (;;) {
     if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
     __syncthreads();
-    const uint32_t i = next;
+    // The job's fields are read from device memory where they are used, at a
+    // wave-uniform address (scalar loads, no private copy).
+    const uint32_t i = __builtin_amdgcn_readfirstlane(next);
     __syncthreads();
     if (i >= count) return;
-    const JacobiJob job = jobs[i];
-    switch ((job.order + 15u) & ~15u) {
+    const JacobiJob* job = jobs + i;
+    switch ((job->order + 15u) & ~15u) {
       case 144: tile_job<144>(job, a64, vectors, diagonal, roots, status, lds); break;
       case 160: tile_job<160>(job, a64, vectors, diagonal, roots, status, lds); break;
       case 176: tile_job<176>(job, a64, vectors, diagonal, roots, status, lds); break;
