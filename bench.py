@@ -36,13 +36,14 @@ the batch when all pass), per second of the low-bit path.
 
 Also reported:
   * the roofline of the dominant kernel (the Jacobi of orders 129..256,
-    persistent_jacobi_kernel, on 256 of the batch's matrices of orders
-    241..256, one per CU, with the reference's fixed 8 sweeps, run on its own
-    and timed with HIP events on the library's stream; ``--roofline-only``
-    runs just these launches, for the profiler).
+    tile_jacobi_kernel -- the blocked eigensolver on FP32 MFMA; --jacobi
+    scalar runs persistent_jacobi_kernel instead -- on 256 of the batch's
+    matrices of orders 241..256, one per CU, with the reference's fixed 8
+    sweeps, run on its own and timed with HIP events on the library's
+    stream; ``--roofline-only`` runs just these launches, for the profiler).
     Its HBM traffic comes from the rocprofv3 PMC summary in profiles/, used
     only when that summary was taken on the kernel source being run (the
-    SHA-256 of kernels_jacobi.hip recorded there);
+    SHA-256 of kernels_tile_jacobi.hip recorded there);
   * the whole-batch roofline fraction sum_i t_roof,i / t_measured of each
     path, t_roof,i = 9 S n^2 (n - 1) / P_fp32 + (2 + 8 k) n^3 / P_tier +
     2 k n^3 / P_fp64 with the sweeps S and corrections k each matrix ran;
@@ -71,7 +72,7 @@ FP32_PEAK_TFLOPS = 157.3
 # records the bytes per launch and the SHA-256 of the kernel source it was
 # measured on.
 TRAFFIC_PROFILE = ROOT / "profiles" / "jacobi_n256_hbm_traffic_pmc.md"
-JACOBI_SOURCE = ROOT / "mixed-mlx_amd" / "csrc" / "kernels_jacobi.hip"
+JACOBI_SOURCE = ROOT / "mixed-mlx_amd" / "csrc" / "kernels_tile_jacobi.hip"
 # MI355X peaks (TFLOP/s): FP32 vector and matrix, FP64, and the dense MFMA
 # rates of the product formats.
 P_FP32, P_FP64 = 157.3, 78.6
@@ -120,8 +121,9 @@ ROOFLINE_SWEEPS = 8      # the reference's fixed sweep count (src/metal_solver.c
 
 
 def roofline_leg(args, mats, orders, rt, stream, torch, np, capi, dev):
-    """Dominant kernel: the Jacobi of orders 129..256 (persistent_jacobi_kernel:
-    one workgroup per CU, the whole solve of a matrix in LDS) on 256 of the
+    """Dominant kernel: the Jacobi of orders 129..256 (tile_jacobi_kernel, the
+    blocked eigensolver, or persistent_jacobi_kernel with --jacobi scalar: one
+    workgroup per CU, the whole solve of a matrix in LDS) on 256 of the
     batch's matrices of orders 241..256 -- one per CU, so every workgroup takes
     one job -- with the reference's fixed 8 sweeps, so that every workgroup
     does the same work and the launch time is the kernel's time per matrix.
@@ -151,11 +153,14 @@ def roofline_leg(args, mats, orders, rt, stream, torch, np, capi, dev):
     flops = jacobi_flops(sub_orders, sweeps)
     achieved = flops / jt / 1e12
     traffic, digest = measured_traffic()
-    return {"bound": "fp32 valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+    blocked = args.jacobi == "blocked"
+    return {"bound": "fp32 mfma" if blocked else "fp32 valu", "achieved": round(achieved, 3),
+            "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-            "traffic": traffic if len(mats) == 4096 else None,
+            "traffic": traffic if len(mats) == 4096 and blocked else None,
             "kernel_source_sha256": digest,
-            "kernel": (f"persistent_jacobi_kernel ({len(big)} matrices of orders 241..256, "
+            "kernel": (f"{'tile_jacobi_kernel' if blocked else 'persistent_jacobi_kernel'} "
+                       f"({len(big)} matrices of orders 241..256, "
                        f"one per CU, {ROOFLINE_SWEEPS} fixed sweeps; "
                        f"{flops / 1e12:.3f} TFLOP per launch at 9 S n^2 (n - 1), "
                        f"{jt * 1e3:.2f} ms mean of {len(durations)} launches)")}
@@ -331,7 +336,7 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="eager launches instead of per-batch HIP graph replay")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--jacobi", choices=["blocked", "scalar"], default="scalar",
+    ap.add_argument("--jacobi", choices=["blocked", "scalar"], default="blocked",
                     help="eigensolver of the orders 129..256: blocked (8-index blocks, "
                          "FP32 MFMA products) or scalar (one rotation per pair)")
     args = ap.parse_args()

@@ -302,7 +302,7 @@ int msq_runtime_set_graphs(msq_runtime *runtime, int enable);
  *    round applied as FP32 MFMA products (V in registers); same stop rules,
  *    a sweep being one pass over the block pairs.
  * Orders up to 128 and above 256 are not affected.  Changing the method
- * releases the runtime's HIP graphs.  Default: MSQ_JACOBI_SCALAR. */
+ * releases the runtime's HIP graphs.  Default: MSQ_JACOBI_BLOCKED_MFMA. */
 enum msq_jacobi_method { MSQ_JACOBI_SCALAR = 0, MSQ_JACOBI_BLOCKED_MFMA = 1 };
 int msq_runtime_set_jacobi_method(msq_runtime *runtime, int method);
 

@@ -106,7 +106,7 @@ struct msq_runtime {
   uint32_t* jacobi_counter = nullptr;
   int cu_count = 0;
   // Eigensolver of the persistent family (msq_runtime_set_jacobi_method).
-  int jacobi_method = MSQ_JACOBI_SCALAR;
+  int jacobi_method = MSQ_JACOBI_BLOCKED_MFMA;
 };
 
 namespace {

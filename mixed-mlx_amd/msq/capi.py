@@ -61,6 +61,7 @@ DISPATCH_PER_STAGE, DISPATCH_FUSED = 0, 1
 ROUNDING_NEAREST, ROUNDING_STOCHASTIC = 0, 1
 # msq_jacobi_method: eigensolver of the orders 129..256.
 JACOBI_SCALAR, JACOBI_BLOCKED_MFMA = 0, 1
+JACOBI_DEFAULT = JACOBI_BLOCKED_MFMA
 OWN_STREAM = 2**64 - 1  # MSQ_OWN_STREAM
 MLP_INPUTS = 16
 MLP_HIDDEN = 64

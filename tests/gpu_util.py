@@ -1,4 +1,5 @@
 """Helpers shared by the GPU tests (device buffers are torch tensors)."""
+import contextlib
 import sys
 
 import numpy as np
@@ -24,3 +25,15 @@ def empty(nbytes):
 def host(t):
     torch.cuda.synchronize()
     return t.cpu().numpy()
+
+
+@contextlib.contextmanager
+def jacobi_method(runtime, method):
+    """Run the block with this eigensolver for orders 129..256, then restore
+    the library's default."""
+    from msq import capi
+    runtime.set_jacobi_method(method)
+    try:
+        yield runtime
+    finally:
+        runtime.set_jacobi_method(capi.JACOBI_DEFAULT)

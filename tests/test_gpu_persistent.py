@@ -1,12 +1,13 @@
-"""The persistent Jacobi launch of orders 129..256 (one workgroup per CU
-taking jobs from a device counter): more jobs than workgroups, mixed orders,
+"""The persistent Jacobi launches of orders 129..256 (one workgroup per CU
+taking jobs from a device counter), scalar and blocked eigensolvers: more
+jobs than workgroups, mixed orders,
 every result equal bit for bit to the same matrix solved alone (a grid of
 one), to the batch's graph replay, and close to LAPACK's eigenvalues."""
 import numpy as np
 import pytest
 import torch
 
-from gpu_util import dev, host, say
+from gpu_util import dev, host, jacobi_method, say
 from msq import capi
 from msq import generators as G
 
@@ -25,8 +26,16 @@ def jacobi(runtime, mats, opt):
     return Vs, [d[o:o + n] for o, n in zip(voffs, orders)], [r.sweeps for r in reps]
 
 
+@pytest.fixture(params=["scalar", "blocked"])
+def method(request, runtime):
+    m = capi.JACOBI_SCALAR if request.param == "scalar" else capi.JACOBI_BLOCKED_MFMA
+    with jacobi_method(runtime, m):
+        yield runtime
+
+
 @pytest.mark.parametrize("fixed", [0, 3])
-def test_queue_with_more_jobs_than_workgroups(runtime, fixed):
+def test_queue_with_more_jobs_than_workgroups(method, fixed):
+    runtime = method
     rng = np.random.default_rng(41)
     orders = rng.integers(129, 257, size=600).tolist() + [16, 64, 100, 128]
     rng.shuffle(orders)
@@ -51,7 +60,8 @@ def test_queue_with_more_jobs_than_workgroups(runtime, fixed):
         assert np.linalg.norm(Vi.T @ Vi - np.eye(orders[i])) <= 1e-4
 
 
-def test_graph_replay_of_the_persistent_launch(runtime):
+def test_graph_replay_of_the_persistent_launch(method):
+    runtime = method
     """The counter is cleared by a kernel inside the captured work, so every
     replay runs the whole queue again."""
     rng = np.random.default_rng(5)

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 import torch
 
-from gpu_util import dev, host, say
+from gpu_util import dev, host, say, jacobi_method
 from msq import capi
 from msq import generators as G
 from oracle import codecs as C
@@ -152,7 +152,9 @@ def test_jacobi_fixed_sweeps_parity(runtime):
     orders = [5, 16, 31, 64, 100, 128, 150, 256]
     mats = [G.single_matrix(7, n, 1e-2, index=i)[0] for i, n in enumerate(orders)]
     say("jacobi fixed 8 sweeps", orders)
-    Vs, Ds, Rts, reps = _jacobi_batch(runtime, mats, capi.jacobi_options(fixed_sweeps=8))
+    # The scalar eigensolver at every order: the oracle's schedule.
+    with jacobi_method(runtime, capi.JACOBI_SCALAR):
+        Vs, Ds, Rts, reps = _jacobi_batch(runtime, mats, capi.jacobi_options(fixed_sweeps=8))
     for A, V, d, rt, rep in zip(mats, Vs, Ds, Rts, reps):
         n = A.shape[0]
         assert rep.sweeps == 8 and not rep.invalid_spectrum
@@ -201,7 +203,8 @@ def test_jacobi_tolerance_mode(runtime):
 
 @pytest.mark.parametrize("fixed", [3, 0])
 def test_jacobi_orders_with_v_in_device_memory(runtime, fixed):
-    """Padded orders 144..256 keep V in device memory and update it once per
+    """The scalar eigensolver (MSQ_JACOBI_SCALAR).  Padded orders 144..256 keep
+    V in device memory and update it once per
     group of rounds, as many as the LDS beside A holds history for (24 to 144
     rounds), groups ending at sweep boundaries.  One matrix per padded order
     (odd orders, so the padding is live), with fixed sweeps (the last group
@@ -218,7 +221,8 @@ def test_jacobi_orders_with_v_in_device_memory(runtime, fixed):
     mats = [G.single_matrix(12, n, 1e-3, index=i)[0] for i, n in enumerate(orders)]
     say("jacobi V in device memory", orders, "fixed", fixed)
     opts = capi.jacobi_options(fixed_sweeps=fixed) if fixed else capi.jacobi_options(off_tol=1e-7)
-    Vs, Ds, _, reps = _jacobi_batch(runtime, mats, opts)
+    with jacobi_method(runtime, capi.JACOBI_SCALAR):
+        Vs, Ds, _, reps = _jacobi_batch(runtime, mats, opts)
     for A, V, d, rep in zip(mats, Vs, Ds, reps):
         n = A.shape[0]
         N = (n + 15) // 16 * 16

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 import torch
 
-from gpu_util import dev, host, say
+from gpu_util import dev, host, jacobi_method, say
 from msq import capi
 from msq import generators as G
 from msq import solver as MS
@@ -26,9 +26,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture
 def blocked(runtime):
-    runtime.set_jacobi_method(capi.JACOBI_BLOCKED_MFMA)
-    yield runtime
-    runtime.set_jacobi_method(capi.JACOBI_SCALAR)
+    with jacobi_method(runtime, capi.JACOBI_BLOCKED_MFMA):
+        yield runtime
 
 
 def jacobi(runtime, mats, opt):
