@@ -34,23 +34,28 @@ __device__ __forceinline__ int participant(int t, int x, int np) {
 // except at float rounding ties, and one third of their dependent chain (the
 // pair phase is on every round's critical path).  d^2 + e^2 neither
 // overflows nor underflows in binary64 for float inputs, and t is in [-1, 1].
+template <int STEPS = 2>
 __device__ __forceinline__ double rsqrt_refined(double w) {  // w > 0, normal
   double y = __builtin_amdgcn_rsq(w);
-  y = y * fma(-0.5 * w * y, y, 1.5);
-  y = y * fma(-0.5 * w * y, y, 1.5);
+#pragma unroll
+  for (int i = 0; i < STEPS; ++i) y = y * fma(-0.5 * w * y, y, 1.5);
   return y;
 }
+// STEPS = 1: one Newton step per estimate, still far below the float
+// rounding of the result (the blocked eigensolver, whose products round
+// differently from the oracle anyway, uses it).
+template <int STEPS = 2>
 __device__ __forceinline__ float2 rotation_angles(float app, float aqq, float apq) {
   if (apq == 0.0f) return make_float2(1.0f, 0.0f);
   const double d = static_cast<double>(aqq) - static_cast<double>(app);
   const double e = 2.0 * static_cast<double>(apq);
   const double w = fma(d, d, e * e);
-  const double den = fabs(d) + w * rsqrt_refined(w);
+  const double den = fabs(d) + w * rsqrt_refined<STEPS>(w);
   double r = __builtin_amdgcn_rcp(den);
-  r = r * fma(-den, r, 2.0);
-  r = r * fma(-den, r, 2.0);
+#pragma unroll
+  for (int i = 0; i < STEPS; ++i) r = r * fma(-den, r, 2.0);
   const double t = copysign(fabs(e) * r, d * e);
-  const double c = rsqrt_refined(fma(t, t, 1.0));
+  const double c = rsqrt_refined<STEPS>(fma(t, t, 1.0));
   return make_float2(static_cast<float>(c), static_cast<float>(t * c));
 }
 

@@ -95,7 +95,9 @@ struct TileLds {
   static constexpr size_t kRed = kRot + static_cast<size_t>(2 * kTileWaves) * 8 * sizeof(float2);
   static constexpr size_t kRedf = kRed + 2 * kTileWaves * sizeof(double);
   static constexpr size_t kCtrl = kRedf + 16 * sizeof(float);
-  static constexpr size_t kBytes = kCtrl + 4 * sizeof(int);
+  // Off-diagonal tile-blocks (k, l), k < l, row-major: k | l << 8.
+  static constexpr size_t kItemTable = kCtrl + 4 * sizeof(int);
+  static constexpr size_t kBytes = kItemTable + static_cast<size_t>(M * (M - 1) / 2) * 2;
   static_assert(kQ % 16 == 0 && kRot % 16 == 0 && kRed % 8 == 0, "LDS alignment");
   static_assert(NP * sizeof(float) <= static_cast<size_t>(M) * 256 * sizeof(float),
                 "square roots fit the Q area");
@@ -217,7 +219,7 @@ __device__ __forceinline__ void sub_round(SubState (&st)[P], int tau, int x, int
     const bool xp = a < b;
 #pragma unroll
     for (int u = 0; u < P; ++u) {
-      const float2 r = rotation_angles(xp ? diag[u] : pdiag[u], xp ? pdiag[u] : diag[u],
+      const float2 r = rotation_angles<1>(xp ? diag[u] : pdiag[u], xp ? pdiag[u] : diag[u],
                                        xp ? off[u] : poff[u]);
       rw[u][x] = f2{r.x, xp ? -r.y : r.y};
     }
@@ -335,7 +337,6 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
   // Wave-uniform (scalar registers for everything derived from it).
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nthreads = static_cast<int>(blockDim.x);
-  const int nwaves = nthreads >> 6;
   const int n = static_cast<int>(job.order);
   const int ld = static_cast<int>(job.ld);
   MSQ_LDS float* const At = reinterpret_cast<MSQ_LDS float*>(smem + L::kA);
@@ -344,6 +345,15 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
   MSQ_LDS double* const red = reinterpret_cast<MSQ_LDS double*>(smem + L::kRed);
   MSQ_LDS float* const redf = reinterpret_cast<MSQ_LDS float*>(smem + L::kRedf);
   MSQ_LDS int* const ctrl = reinterpret_cast<MSQ_LDS int*>(smem + L::kCtrl);
+  MSQ_LDS uint16_t* const items = reinterpret_cast<MSQ_LDS uint16_t*>(smem + L::kItemTable);
+  for (int it = tid; it < kItems; it += nthreads) {
+    int k = 0, l = it;
+    while (l >= M - 1 - k) {
+      l -= M - 1 - k;
+      ++k;
+    }
+    items[it] = static_cast<uint16_t>(k | (k + 1 + l) << 8);
+  }
 
   // A into the tiles: element (r, c) of tile (I, J) = a_{8I+r, 8J+c}; rows
   // fastest, so that 8 lanes read 64 consecutive bytes of a column.
@@ -452,6 +462,7 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
         }
       }
     }
+    if (tid == 0) ctrl[1] = 0;  // the products' item counter
     __syncthreads();
 
     // ---- 2. products.  V: this wave's two row groups, every pair (one Q
@@ -475,7 +486,7 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
         }
       }
     }
-    // Off-diagonal tile-blocks (k, l), k < l, dealt to every wave.  Per item
+    // Off-diagonal tile-blocks (k, l), k < l, shared by every wave.  Per item
     // the four tiles (row half a: block bk[a], column half b: block bl[b])
     // have wave-uniform offsets and orientations; a lane only selects among
     // them by its halves.
@@ -493,16 +504,16 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
         offc[s2] = y8 * 8 + (rho & 7);
       }
       const int hs = g >> 1, hc = i >> 3, r8 = 4 * (g & 1), i8 = i & 7;
-      // This wave's items it = wave + j nwaves, row-major over k < l; (k, l)
-      // advanced in place (scalar registers).
-      int k = 0, l = wave;
-      while (k < M - 1 && l >= M - 1 - k) {
-        l -= M - 1 - k;
-        ++k;
-      }
-      l += k + 1;
+      // Items are taken from a counter in LDS (cleared before the phase's
+      // barrier), so that waves that finish their share early take more.
 #pragma unroll 1
-      for (int it = wave; it < kItems; it += nwaves) {
+      for (;;) {
+        int it = 0;
+        if (lane == 0) it = atomicAdd((int*)ctrl + 1, 1);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= kItems) break;
+        const int kl = __builtin_amdgcn_readfirstlane(static_cast<int>(items[it]));
+        const int k = kl & 0xff, l = kl >> 8;
         const int bk[2] = {participant(round, k, NB), participant(round, NB - 1 - k, NB)};
         const int bl[2] = {participant(round, l, NB), participant(round, NB - 1 - l, NB)};
         int base[4], upm = 0;
@@ -544,12 +555,6 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
           } else {
             *reinterpret_cast<MSQ_LDS f4*>(At + b0 + i8 * 8 + r8) = o;
           }
-        }
-        // Next item: nwaves further along the rows.
-        l += nwaves;
-        while (k < M - 2 && l > M - 1) {
-          l += k + 2 - M;
-          ++k;
         }
       }
     }
