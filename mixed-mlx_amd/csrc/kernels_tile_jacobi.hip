@@ -310,6 +310,27 @@ __device__ __forceinline__ void sub_round(SubState (&st)[P], int tau, int x, int
   }
 }
 
+// V <- V Q_k for pair k on this wave's row groups: D = P W with P[i][a] =
+// Q_k[a][sigma(i)] and W[a][j] = V[j][pair column a] leaves W'[4 r + g] in
+// accumulator r of lane group g, the registers the B operand came from.
+template <int P, int M>
+__device__ __forceinline__ void v_update(float (&vl)[P][M][2], float (&vh)[P][M][2],
+                                         const MSQ_LDS float* Qs, int k, int qoff) {
+  const f4 qa = *reinterpret_cast<const MSQ_LDS f4*>(Qs + k * 256 + qoff);
+#pragma unroll
+  for (int h2 = 0; h2 < P; ++h2) {
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[0], vl[h2][k][0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[1], vl[h2][k][1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[2], vh[h2][k][0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[3], vh[h2][k][1], acc, 0, 0, 0);
+    vl[h2][k][0] = acc[0];
+    vl[h2][k][1] = acc[1];
+    vh[h2][k][0] = acc[2];
+    vh[h2][k][1] = acc[3];
+  }
+}
+
 // Offset of s_ab of a pair subproblem, local indices a, b (0..7: block bp,
 // 8..15: block bq; tpp, tqq: the offsets of the two diagonal tiles); s_ab
 // and s_ba share one word.
@@ -462,34 +483,14 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
         }
       }
     }
-    if (tid == 0) ctrl[1] = 0;  // the products' item counter
     __syncthreads();
 
-    // ---- 2. products.  V: this wave's two row groups, every pair (one Q
-    // operand read serves both).
-    if (has_pairs) {
-      const int i = lane & 15, g = lane >> 4;
-#pragma unroll
-      for (int k = 0; k < M; ++k) {
-        const f4 qa = *reinterpret_cast<const MSQ_LDS f4*>(Qs + k * 256 + q_chunk(sigma16(i), g));
-#pragma unroll
-        for (int h2 = 0; h2 < P; ++h2) {
-          f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[0], vl[h2][k][0], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[1], vl[h2][k][1], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[2], vh[h2][k][0], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qa[3], vh[h2][k][1], acc, 0, 0, 0);
-          vl[h2][k][0] = acc[0];
-          vl[h2][k][1] = acc[1];
-          vh[h2][k][0] = acc[2];
-          vh[h2][k][1] = acc[3];
-        }
-      }
-    }
-    // Off-diagonal tile-blocks (k, l), k < l, shared by every wave.  Per item
-    // the four tiles (row half a: block bk[a], column half b: block bl[b])
-    // have wave-uniform offsets and orientations; a lane only selects among
-    // them by its halves.
+    // ---- 2. products: the off-diagonal tile-blocks (k, l), k < l (item
+    // wave + 16 j in pass j), and this wave's V update, VPJ pairs per pass
+    // beside the item (independent MFMA chains in one instruction stream).
+    // Per item the four tiles (row half a: block bk[a], column half b: block
+    // bl[b]) have wave-uniform offsets and orientations; a lane only selects
+    // among them by its halves.
     {
       const int i = lane & 15, g = lane >> 4;
       const int rho = sigma16(i);
@@ -504,48 +505,46 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
         offc[s2] = y8 * 8 + (rho & 7);
       }
       const int hs = g >> 1, hc = i >> 3, r8 = 4 * (g & 1), i8 = i & 7;
-      // Items are taken from a counter in LDS (cleared before the phase's
-      // barrier), so that waves that finish their share early take more.
-#pragma unroll 1
-      for (;;) {
-        int it = 0;
-        if (lane == 0) it = atomicAdd((int*)ctrl + 1, 1);
-        it = __builtin_amdgcn_readfirstlane(it);
-        if (it >= kItems) break;
-        const int kl = __builtin_amdgcn_readfirstlane(static_cast<int>(items[it]));
-        const int k = kl & 0xff, l = kl >> 8;
-        const int bk[2] = {participant(round, k, NB), participant(round, NB - 1 - k, NB)};
-        const int bl[2] = {participant(round, l, NB), participant(round, NB - 1 - l, NB)};
-        int base[4], upm = 0;
+      const int qoff = q_chunk(rho, g);
+      constexpr int J = (kItems + kTileWaves - 1) / kTileWaves;
+      constexpr int VPJ = (M + J - 1) / J;
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+      for (int j = 0; j < J; ++j) {
+        const int it = wave + kTileWaves * j;
+        if (it < kItems) {
+          const int kl = __builtin_amdgcn_readfirstlane(static_cast<int>(items[it]));
+          const int k = kl & 0xff, l = kl >> 8;
+          const int bk[2] = {participant(round, k, NB), participant(round, NB - 1 - k, NB)};
+          const int bl[2] = {participant(round, l, NB), participant(round, NB - 1 - l, NB)};
+          int base[4], upm = 0;
 #pragma unroll
-          for (int b = 0; b < 2; ++b) {
-            const int X = bk[a], Y = bl[b];
-            const int lo = min(X, Y), hi = max(X, Y);
-            base[2 * a + b] = (hi * (hi + 1) / 2 + lo) * 64;
-            upm |= (X < Y ? 1 : 0) << (2 * a + b);
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+              const int X = bk[a], Y = bl[b];
+              const int lo = min(X, Y), hi = max(X, Y);
+              base[2 * a + b] = (hi * (hi + 1) / 2 + lo) * 64;
+              upm |= (X < Y ? 1 : 0) << (2 * a + b);
+            }
+          // Z = T' Q_l: A operand T[sigma(i)][4 s + g].
+          float ta[4];
+#pragma unroll
+          for (int s2 = 0; s2 < 4; ++s2) {
+            const int t = 2 * hr + (s2 >> 1);
+            const int b0 = hr ? base[2 + (s2 >> 1)] : base[s2 >> 1];
+            ta[s2] = At[b0 + (((upm >> t) & 1) ? offr[s2] : offc[s2])];
           }
-        // Z = T' Q_l: A operand T[sigma(i)][4 s + g].
-        float ta[4];
+          const f4 ql = *reinterpret_cast<const MSQ_LDS f4*>(Qs + l * 256 + q_chunk(i, g));
+          const f4 qk = *reinterpret_cast<const MSQ_LDS f4*>(Qs + k * 256 + q_chunk(i, g));
+          f4 z = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int s2 = 0; s2 < 4; ++s2) {
-          const int t = 2 * hr + (s2 >> 1);
-          const int b0 = hr ? base[2 + (s2 >> 1)] : base[s2 >> 1];
-          ta[s2] = At[b0 + (((upm >> t) & 1) ? offr[s2] : offc[s2])];
-        }
-        const f4 ql = *reinterpret_cast<const MSQ_LDS f4*>(Qs + l * 256 + q_chunk(i, g));
-        const f4 qk = *reinterpret_cast<const MSQ_LDS f4*>(Qs + k * 256 + q_chunk(i, g));
-        f4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+          for (int s2 = 0; s2 < 4; ++s2)
+            z = __builtin_amdgcn_mfma_f32_16x16x4f32(ta[s2], ql[s2], z, 0, 0, 0);
+          f4 o = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int s2 = 0; s2 < 4; ++s2)
-          z = __builtin_amdgcn_mfma_f32_16x16x4f32(ta[s2], ql[s2], z, 0, 0, 0);
-        f4 o = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int s2 = 0; s2 < 4; ++s2)
-          o = __builtin_amdgcn_mfma_f32_16x16x4f32(qk[s2], z[s2], o, 0, 0, 0);
-        // o[r] = T_new[4 g + r][j], j = i: row half g / 2, column half i / 8.
-        {
+          for (int s2 = 0; s2 < 4; ++s2)
+            o = __builtin_amdgcn_mfma_f32_16x16x4f32(qk[s2], z[s2], o, 0, 0, 0);
+          // o[r] = T_new[4 g + r][j], j = i: row half g / 2, column half i / 8.
           const int t = 2 * hs + hc;
           const int b0 = hs ? (hc ? base[3] : base[2]) : (hc ? base[1] : base[0]);
           if ((upm >> t) & 1) {
@@ -555,6 +554,11 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
           } else {
             *reinterpret_cast<MSQ_LDS f4*>(At + b0 + i8 * 8 + r8) = o;
           }
+        }
+        if (has_pairs) {
+#pragma unroll
+          for (int v = 0; v < VPJ; ++v)
+            if (j * VPJ + v < M) v_update<P, M>(vl, vh, Qs, j * VPJ + v, qoff);
         }
       }
     }
