@@ -163,16 +163,16 @@ struct SubState {
 };
 
 __device__ __forceinline__ float dpp_mirror(float v) {  // lane 15 - x of the row
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float dpp_ror1(float v) {  // lane (x - 1) mod 16
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x121, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x121, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float dpp_ror9(float v) {  // lane (x - 9) mod 16
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x129, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x129, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float dpp_shr1(float v) {  // lane x - 1 (lane 0: 0)
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x111, 0xf, 0xf, true));
 }
 
 // One inner round of two subproblems (independent, interleaved for
@@ -600,11 +600,20 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
     __syncthreads();
     double off2 = 0.0, g2 = 0.0;
     float dmin = INFINITY;
+    // Thread tid takes entry tid % 64 of the tiles tid / 64 + (blockDim / 64) q,
+    // the tile coordinates advanced in place.
+    int I, J;
+    tile_coords(tid >> 6, I, J);
 #pragma unroll 1
     for (int e = tid; e < kTiles * 64; e += nthreads) {
-      const int tt = e >> 6, r = (e >> 3) & 7, c = e & 7;
-      int I, J;
-      tile_coords(tt, I, J);
+      const int r = (e >> 3) & 7, c = e & 7;
+      if (e != tid) {
+        I += nthreads >> 6;
+        while (I > J) {
+          I -= J + 1;
+          ++J;
+        }
+      }
       const int i = 8 * I + r, j = 8 * J + c;
       if (i >= n || j >= n || (I == J && r > c)) continue;
       const float v = At[e];
