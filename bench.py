@@ -337,7 +337,7 @@ def main():
                     help="eager launches instead of per-batch HIP graph replay")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--jacobi", choices=["blocked", "scalar"], default="blocked",
-                    help="eigensolver of the orders 129..256: blocked (8-index blocks, "
+                    help="eigensolver of the orders up to 256: blocked (8-index blocks, "
                          "FP32 MFMA products) or scalar (one rotation per pair)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -293,15 +293,17 @@ int msq_runtime_set_rounding(msq_runtime *runtime, int mode, uint32_t seed);
  * eagerly.  At most 4 signatures are kept; enable = 0 releases them. */
 int msq_runtime_set_graphs(msq_runtime *runtime, int enable);
 
-/* Eigensolver of the orders 129..256 (no counterpart in the reference, whose
+/* Eigensolver of the orders up to 256 (no counterpart in the reference, whose
  * Jacobi is one scalar kernel for every order):
  *  - MSQ_JACOBI_SCALAR: one rotation at a time per index pair, the reference's
- *    element formulas on the padded circle schedule, V in device memory;
+ *    element formulas on the padded circle schedule (orders 129..256 in one
+ *    persistent launch, smaller ones one launch per padded order);
  *  - MSQ_JACOBI_BLOCKED_MFMA: blocks of 8 indices paired on the circle, each
  *    16 x 16 pair subproblem swept once by scalar rotations, the rest of the
  *    round applied as FP32 MFMA products (V in registers); same stop rules,
- *    a sweep being one pass over the block pairs.
- * Orders up to 128 and above 256 are not affected.  Changing the method
+ *    a sweep being one pass over the block pairs; orders 17..256 in one
+ *    persistent launch (orders up to 16 as with MSQ_JACOBI_SCALAR).
+ * Orders above 256 (the blocked 64 x 64 solver) are not affected.  Changing the method
  * releases the runtime's HIP graphs.  Default: MSQ_JACOBI_BLOCKED_MFMA. */
 enum msq_jacobi_method { MSQ_JACOBI_SCALAR = 0, MSQ_JACOBI_BLOCKED_MFMA = 1 };
 int msq_runtime_set_jacobi_method(msq_runtime *runtime, int method);
@@ -368,8 +370,8 @@ int msq_decode_fp8_scaled(msq_runtime *runtime, const void *values, const float 
  * (float, batch layout) receive V, diagonal and roots (float, vector layout)
  * receive diag(V^T A V) and its square roots.  options has 1 or count
  * entries; reports (host, count entries) may be NULL.  Orders up to 256 run
- * one workgroup per matrix, all sweeps in LDS (orders 129..256 as one
- * persistent launch, smaller orders one launch per padded order); larger
+ * one workgroup per matrix, all sweeps in LDS (msq_runtime_set_jacobi_method
+ * picks the eigensolver and which orders share one persistent launch); larger
  * orders run the blocked solver (64 x 64 pair subproblems, rotations applied
  * as MFMA products), the matrices of one padded order advancing together. */
 int msq_jacobi_batched(msq_runtime *runtime, uint32_t count, const uint32_t *orders,

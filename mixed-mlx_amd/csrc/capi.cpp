@@ -595,8 +595,13 @@ int enqueue_jacobi_classes(msq_runtime* rt, const JacobiClass* cls, int classes,
   uint32_t pcount = 0;
   double pcost = 0.0;
   bool contiguous = true;
+  // The persistent family: padded orders 144..256 on the scalar solver, from
+  // msq::tile_jacobi_min_padded() on the blocked one.
+  const uint32_t pmin = rt->jacobi_method == MSQ_JACOBI_BLOCKED_MFMA
+                            ? msq::tile_jacobi_min_padded()
+                            : msq::batched_jacobi_padded_order(msq::kPersistentMinOrder);
   for (int c = 0; c < classes; ++c) {
-    if (cls[c].order < msq::batched_jacobi_padded_order(msq::kPersistentMinOrder)) continue;
+    if (cls[c].order < pmin) continue;
     if (pcount == 0) {
       pfirst = cls[c].first;
     } else if (cls[c].first != pend) {
@@ -609,8 +614,7 @@ int enqueue_jacobi_classes(msq_runtime* rt, const JacobiClass* cls, int classes,
   const bool persistent = pcount > 0 && contiguous && rt->jacobi_counter && rt->cu_count > 0;
   if (persistent) launches[nl++] = {pfirst, pcount, 0u, pcost};
   for (int c = 0; c < classes; ++c)
-    if (!persistent ||
-        cls[c].order < msq::batched_jacobi_padded_order(msq::kPersistentMinOrder))
+    if (!persistent || cls[c].order < pmin)
       launches[nl++] = {cls[c].first, cls[c].count, cls[c].order, cls[c].cost};
   const int used = nl > 1 ? std::min(nl, kLanes) : 0;
   if (used) {

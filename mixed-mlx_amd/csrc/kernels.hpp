@@ -95,11 +95,14 @@ hipError_t launch_persistent_jacobi(const JacobiJob* jobs, uint32_t count, uint3
                                     float* diagonal, float* roots, JacobiStatus* status,
                                     hipStream_t stream);
 
-// The same jobs (orders 129..256) on the blocked LDS solver: blocks of 8
+// Orders 17..256 (padded from tile_jacobi_min_padded()) on the blocked LDS
+// solver, one persistent launch like launch_persistent_jacobi: blocks of 8
 // indices paired on the circle, 16 x 16 pair subproblems diagonalised by
 // scalar rotations and the rest of each round applied as FP32 MFMA products,
 // V in registers.  Same persistent launch shape, counter use, outputs and
 // stop rules as launch_persistent_jacobi; a sweep is N / 8 - 1 block rounds.
+// Smallest padded order of the blocked solver's persistent launch.
+uint32_t tile_jacobi_min_padded();
 hipError_t launch_tile_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t* counter, int grid,
                               const double* a64, float* vectors, float* diagonal, float* roots,
                               JacobiStatus* status, hipStream_t stream);

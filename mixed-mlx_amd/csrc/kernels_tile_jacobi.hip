@@ -1,4 +1,4 @@
-// Blocked Jacobi in LDS for orders 129..256: pair subproblems diagonalised by
+// Blocked Jacobi in LDS for orders 17..256: pair subproblems diagonalised by
 // scalar rotations, the rest of each rotation applied as FP32 MFMA products.
 //
 // Reference: symmetricJacobi (src/metal_runtime.cpp:584-646) with the
@@ -65,6 +65,9 @@ typedef float f2 __attribute__((ext_vector_type(2)));
 // LDS pointers stay in the local address space (32-bit ds_* addressing).
 #define MSQ_LDS __attribute__((address_space(3)))
 
+#ifndef TILE_MIN_PADDED
+#define TILE_MIN_PADDED 32
+#endif
 #ifndef PAIRS_PER_WAVE
 #define PAIRS_PER_WAVE 1
 #endif
@@ -721,7 +724,12 @@ __device__ __attribute__((noinline)) void tile_job(
 // Persistent launch: one 1024-thread workgroup per CU takes the next job
 // from a device counter until the jobs run out (largest order first); every
 // workgroup reads the counter once per job and leaves when it passes the job
-// count, so the grid drains.
+// count, so the grid drains.  Padded orders from TILE_MIN_PADDED (32: the
+// smaller orders keep the scalar per-class launch) to 256; below 256 some
+// waves have no pair and only share the tile-blocks, which still beats the
+// scalar kernel's per-class launches at every padded order from 32 (measured
+// on MI355X over the batched uneven workload: its Jacobi phase 28.6 ms with
+// the persistent family from 144, 25.3 ms from 32).
 __global__ __launch_bounds__(kTileThreads) void tile_jacobi_kernel(
     const JacobiJob* __restrict__ jobs, uint32_t count, uint32_t* __restrict__ counter,
     const double* __restrict__ a64, float* __restrict__ vectors, float* __restrict__ diagonal,
@@ -739,6 +747,27 @@ __global__ __launch_bounds__(kTileThreads) void tile_jacobi_kernel(
     if (i >= count) return;
     const JacobiJob* job = jobs + i;
     switch ((job->order + 15u) & ~15u) {
+#if TILE_MIN_PADDED <= 32
+      case 32: tile_job<32>(job, a64, vectors, diagonal, roots, status, lds); break;
+#endif
+#if TILE_MIN_PADDED <= 48
+      case 48: tile_job<48>(job, a64, vectors, diagonal, roots, status, lds); break;
+#endif
+#if TILE_MIN_PADDED <= 64
+      case 64: tile_job<64>(job, a64, vectors, diagonal, roots, status, lds); break;
+#endif
+#if TILE_MIN_PADDED <= 80
+      case 80: tile_job<80>(job, a64, vectors, diagonal, roots, status, lds); break;
+#endif
+#if TILE_MIN_PADDED <= 96
+      case 96: tile_job<96>(job, a64, vectors, diagonal, roots, status, lds); break;
+#endif
+#if TILE_MIN_PADDED <= 112
+      case 112: tile_job<112>(job, a64, vectors, diagonal, roots, status, lds); break;
+#endif
+#if TILE_MIN_PADDED <= 128
+      case 128: tile_job<128>(job, a64, vectors, diagonal, roots, status, lds); break;
+#endif
       case 144: tile_job<144>(job, a64, vectors, diagonal, roots, status, lds); break;
       case 160: tile_job<160>(job, a64, vectors, diagonal, roots, status, lds); break;
       case 176: tile_job<176>(job, a64, vectors, diagonal, roots, status, lds); break;
@@ -747,13 +776,15 @@ __global__ __launch_bounds__(kTileThreads) void tile_jacobi_kernel(
       case 224: tile_job<224>(job, a64, vectors, diagonal, roots, status, lds); break;
       case 240: tile_job<240>(job, a64, vectors, diagonal, roots, status, lds); break;
       case 256: tile_job<256>(job, a64, vectors, diagonal, roots, status, lds); break;
-      default: break;  // the host launches orders 129..256 only
+      default: break;  // the host launches the orders above tile_jacobi_min_padded() - 16 only
     }
     __syncthreads();
   }
 }
 
 }  // namespace
+
+uint32_t tile_jacobi_min_padded() { return TILE_MIN_PADDED; }
 
 hipError_t launch_tile_jacobi(const JacobiJob* jobs, uint32_t count, uint32_t* counter, int grid,
                               const double* a64, float* vectors, float* diagonal, float* roots,

@@ -59,7 +59,7 @@ CORRECTION_FP32_REORTHOGONALIZED, CORRECTION_NATIVE_QUANTIZED = 0, 1
 STOPPING_FIXED_BUDGET, STOPPING_STAGNATION = 0, 1
 DISPATCH_PER_STAGE, DISPATCH_FUSED = 0, 1
 ROUNDING_NEAREST, ROUNDING_STOCHASTIC = 0, 1
-# msq_jacobi_method: eigensolver of the orders 129..256.
+# msq_jacobi_method: eigensolver of the orders up to 256.
 JACOBI_SCALAR, JACOBI_BLOCKED_MFMA = 0, 1
 JACOBI_DEFAULT = JACOBI_BLOCKED_MFMA
 OWN_STREAM = 2**64 - 1  # MSQ_OWN_STREAM
@@ -485,7 +485,7 @@ class Runtime:
         check(lib.msq_runtime_set_rounding(self.handle, mode, seed))
 
     def set_jacobi_method(self, method=JACOBI_BLOCKED_MFMA):
-        """Eigensolver of the orders 129..256: JACOBI_SCALAR (the reference's
+        """Eigensolver of the orders up to 256: JACOBI_SCALAR (the reference's
         rotations on the padded circle schedule) or JACOBI_BLOCKED_MFMA
         (8-index blocks, pair subproblems, FP32 MFMA products)."""
         check(lib.msq_runtime_set_jacobi_method(self.handle, method))

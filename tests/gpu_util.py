@@ -29,7 +29,7 @@ def host(t):
 
 @contextlib.contextmanager
 def jacobi_method(runtime, method):
-    """Run the block with this eigensolver for orders 129..256, then restore
+    """Run the block with this eigensolver (orders up to 256), then restore
     the library's default."""
     from msq import capi
     runtime.set_jacobi_method(method)

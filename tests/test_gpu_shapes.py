@@ -1,4 +1,5 @@
-"""The two launch shapes of the LDS-resident Jacobi for padded orders 32..80:
+"""The two launch shapes of the scalar LDS-resident Jacobi (MSQ_JACOBI_SCALAR)
+for padded orders 32..80:
 "wide" (one thread per off-diagonal block, used while every workgroup of the
 launch is resident) and "narrow" (several blocks per thread, used above
 256 CUs x 20 waves of workgroups).  A batch above each order's threshold
@@ -9,7 +10,7 @@ import numpy as np
 import pytest
 import torch
 
-from gpu_util import dev, host, say
+from gpu_util import dev, host, jacobi_method, say
 from msq import capi
 from msq import generators as G
 
@@ -26,7 +27,8 @@ def run(runtime, mats, opt):
     V = torch.zeros(total, dtype=torch.float32, device="cuda")
     D = torch.zeros(vtotal, dtype=torch.float32, device="cuda")
     R = torch.zeros(vtotal, dtype=torch.float32, device="cuda")
-    runtime.jacobi_batched(orders, dev(flat), V, D, R, opt)
+    with jacobi_method(runtime, capi.JACOBI_SCALAR):
+        runtime.jacobi_batched(orders, dev(flat), V, D, R, opt)
     return capi.unpack_batch(orders, host(V)), host(D), voffs
 
 

@@ -1,7 +1,7 @@
-"""The blocked eigensolver of orders 129..256 (msq_runtime_set_jacobi_method
-MSQ_JACOBI_BLOCKED_MFMA: 8-index blocks paired on the circle, one sweep of
-each 16 x 16 pair subproblem by scalar rotations, the rest of the round as
-FP32 MFMA products, V in registers), through the C ABI:
+"""The blocked eigensolver of orders 17..256 (msq_runtime_set_jacobi_method
+MSQ_JACOBI_BLOCKED_MFMA: 8-index blocks paired on the circle, each 16 x 16
+pair subproblem rotated by scalar rotations, the rest of the round as FP32
+MFMA products, V in registers), through the C ABI:
 
 * sweep by sweep against the binary64 oracle of the same schedule
   (oracle.solver.jacobi_tile): diagonal, V and off(A) after 1 and 2 sweeps;
@@ -9,7 +9,7 @@ FP32 MFMA products, V in registers), through the C ABI:
 * a matrix's results do not depend on its batch (persistent queue), nor on
   graph capture and replay;
 * whole solves with the blocked eigensolver meet 1e-6 on the batched uneven
-  workload's orders 129..256.
+  workload's orders 129..256 and on a sample of all its orders.
 """
 import numpy as np
 import pytest
@@ -60,7 +60,7 @@ def graded(n, seed):
 
 @pytest.mark.parametrize("sweeps", [1, 2])
 def test_sweeps_follow_the_oracle(blocked, sweeps):
-    orders = [129, 144, 200, 256]
+    orders = [17, 40, 100, 129, 144, 200, 256]
     mats = [graded(n, i) for i, n in enumerate(orders)]
     say(f"tile jacobi vs oracle: orders {orders}, {sweeps} sweeps")
     Vs, ds, reps = jacobi(blocked, mats, capi.jacobi_options(fixed_sweeps=sweeps))
@@ -80,8 +80,8 @@ def test_sweeps_follow_the_oracle(blocked, sweeps):
 def test_sweeps_are_similarity_transforms(blocked, sweeps):
     """Random dense matrices after a few sweeps: V orthogonal, diag(V^T A V)
     equal to the returned diagonal, off(V^T A V) to the reported off(A)."""
-    orders = [129, 176, 256]
-    mats = [G.single_matrix(9, n, 10.0 ** -(1 + i), index=i)[0] for i, n in enumerate(orders)]
+    orders = [20, 64, 129, 176, 256]
+    mats = [G.single_matrix(9, n, 10.0 ** -(1 + i % 3), index=i)[0] for i, n in enumerate(orders)]
     Vs, ds, reps = jacobi(blocked, mats, capi.jacobi_options(fixed_sweeps=sweeps))
     for A, V, d, rep in zip(mats, Vs, ds, reps):
         n = A.shape[0]
@@ -98,7 +98,7 @@ def test_sweeps_are_similarity_transforms(blocked, sweeps):
 
 def test_converged_solves_against_lapack(blocked):
     rng = np.random.default_rng(3)
-    orders = [129, 160, 191, 224, 255, 256]
+    orders = [17, 33, 97, 129, 160, 191, 224, 255, 256]
     mats = [G.single_matrix(9, n, 10.0 ** -rng.uniform(1, 5), index=i)[0]
             for i, n in enumerate(orders)]
     opt = capi.jacobi_options(off_tol=1e-7, max_sweeps=30, stagnation=0.5, floor_tol=1e-5)
@@ -127,15 +127,16 @@ def test_results_do_not_depend_on_the_batch(blocked):
     opt = capi.jacobi_options(off_tol=1e-7, max_sweeps=30, stagnation=0.5, floor_tol=1e-5)
     say(f"tile jacobi queue: {len(orders)} matrices")
     Vb, db, rb = jacobi(blocked, mats, opt)
-    pick = [i for i, n in enumerate(orders) if n > 128][:10]
+    pick = [i for i, n in enumerate(orders) if n > 128][:10] + [orders.index(o) for o in (64, 100, 128)]
     for i in pick:
         Va, da, ra = jacobi(blocked, [mats[i]], opt)
         assert ra[0].sweeps == rb[i].sweeps
         assert np.array_equal(Va[0], Vb[i]) and np.array_equal(da[0], db[i]), (i, orders[i])
 
 
-def test_solves_meet_the_target_and_replay_bit_for_bit(blocked):
-    mats = [m for m in G.batched_uneven(count=4096) if m["order"] > 128][:240]
+@pytest.mark.parametrize("low", [16, 128])
+def test_solves_meet_the_target_and_replay_bit_for_bit(blocked, low):
+    mats = [m for m in G.batched_uneven(count=4096) if m["order"] > low][:240]
     orders = [m["order"] for m in mats]
     o, aflat = capi.pack_batch([m["A"] for m in mats])
     lds, offs, _, total, _ = capi.batch_layout(o)
