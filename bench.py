@@ -68,10 +68,10 @@ sys.path[:0] = [str(ROOT), str(ROOT / "mixed-mlx_amd")]
 
 METRIC = "SPD sqrts/sec at ≤1e-6 rel err vs FP64: low-bit+refine vs plain FP32, 1 GPU"
 FP32_PEAK_TFLOPS = 157.3
-# HBM traffic of the roofline kernel: profiles/jacobi_n256_hbm_traffic_pmc.md
+# HBM traffic of the roofline kernel: profiles/tile_jacobi_n256_hbm_traffic_pmc.md
 # records the bytes per launch and the SHA-256 of the kernel source it was
 # measured on.
-TRAFFIC_PROFILE = ROOT / "profiles" / "jacobi_n256_hbm_traffic_pmc.md"
+TRAFFIC_PROFILE = ROOT / "profiles" / "tile_jacobi_n256_hbm_traffic_pmc.md"
 JACOBI_SOURCE = ROOT / "mixed-mlx_amd" / "csrc" / "kernels_tile_jacobi.hip"
 # MI355X peaks (TFLOP/s): FP32 vector and matrix, FP64, and the dense MFMA
 # rates of the product formats.
