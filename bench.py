@@ -382,6 +382,9 @@ def main():
     orders = [m["order"] for m in mats]
     _, aflat = capi.pack_batch([m["A"] for m in mats])
     lds, offs, voffs, total, vtotal = capi.batch_layout(orders)
+    # The orders as one uint32 array for the timed calls (a list is converted
+    # again by every call).
+    batch_orders = np.ascontiguousarray(orders, dtype=np.uint32)
     log(f"generated {len(mats)} matrices in {time.perf_counter() - t:.1f} s")
 
     dev = torch.device("cuda:0")
@@ -435,18 +438,18 @@ def main():
         return
     if args.profile_path:
         if args.profile_path == "plain":
-            fn = lambda: rt.solve_batched(orders, A, X, plain)  # noqa: E731
+            fn = lambda: rt.solve_batched(batch_orders, A, X, plain)  # noqa: E731
         elif args.profile_path == "fixed":
             opts = MS.tier_options(args.tier, rate=args.rate, max_corrections=12)
-            fn = lambda: MS.solve_with_fallback(rt, orders, A, X, opts)  # noqa: E731
+            fn = lambda: MS.solve_with_fallback(rt, batch_orders, A, X, opts)  # noqa: E731
         else:
             sel = SEL.Selector.load()
-            fn = lambda: SEL.solve_selected(rt, sel, orders, A, X)  # noqa: E731
+            fn = lambda: SEL.solve_selected(rt, sel, batch_orders, A, X)  # noqa: E731
         dt, _ = timed(fn)
         log(f"{args.profile_path}: {dt * 1e3:.1f} ms/step")
         rt.close()
         return
-    dt_plain, rep_plain = timed(lambda: rt.solve_batched(orders, A, X, plain))
+    dt_plain, rep_plain = timed(lambda: rt.solve_batched(batch_orders, A, X, plain))
     err_plain, pass_plain = check_all(X, "plain fp32")
     ok_plain = int(rep_plain.converged.sum())
     log(f"plain fp32: {dt_plain * 1e3:.1f} ms/step, {len(mats) / dt_plain:.0f} matrices/s, "
@@ -460,13 +463,13 @@ def main():
         chosen = []
 
         def low_bit():
-            reports, fb, tiers = SEL.solve_selected(rt, sel, orders, A, X)
+            reports, fb, tiers = SEL.solve_selected(rt, sel, batch_orders, A, X)
             chosen[:] = [tiers]
             return reports, fb
     tier = MS.tier_options(args.tier, rate=args.rate, max_corrections=12)
 
     def fixed_tier():
-        return MS.solve_with_fallback(rt, orders, A, X, tier)
+        return MS.solve_with_fallback(rt, batch_orders, A, X, tier)
     if not use_selector:
         low_bit = fixed_tier
     label = "learned tier" if use_selector else f"fixed {args.tier} tier, rate {args.rate}"
@@ -491,7 +494,7 @@ def main():
     # so the low-bit products' share of the gain is measured on its own.
     control_opts = MS.tier_options("fp32", rate=args.rate, max_corrections=12)
     dt_ctl, (rep_ctl, fb_ctl) = timed(
-        lambda: MS.solve_with_fallback(rt, orders, A, X, control_opts))
+        lambda: MS.solve_with_fallback(rt, batch_orders, A, X, control_opts))
     err_ctl, pass_ctl = check_all(X, f"fp32 products, rate {args.rate}")
     control = {"products": "fp32", "correction_rate": args.rate,
                "matrices_per_s": round(pass_ctl / dt_ctl, 2),

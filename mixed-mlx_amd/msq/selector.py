@@ -364,13 +364,13 @@ class Selector:
         feats = torch.zeros(count * capi.MLP_INPUTS, dtype=torch.float32, device=dev)
         sample = gate_sample(orders)
         if sample is not None:
-            rt.matrix_features(orders, matrices, feats, members=sample.tolist())
+            rt.matrix_features(orders, matrices, feats, members=sample)
             rows = torch.as_tensor(sample, dtype=torch.long, device=dev)
             ps = self.infer(rt, feats.view(count, capi.MLP_INPUTS)[rows].reshape(-1))
             if float(np.mean(1.0 - ps[:, FIXED])) < GATE_SAMPLE_MARGIN * GATE_FAILURE_FRACTION:
                 return np.full(count, FIXED), len(sample)
             rest = np.setdiff1d(np.arange(count), sample)
-            rt.matrix_features(orders, matrices, feats, members=rest.tolist())
+            rt.matrix_features(orders, matrices, feats, members=rest)
         else:
             rt.matrix_features(orders, matrices, feats)
         prob = self.infer(rt, feats)
@@ -431,11 +431,24 @@ class Selector:
         return choice
 
 
+_CHOICE_OPTIONS = []
+
+
+def choice_options():
+    """Solve options of every choice (built once: ctypes records)."""
+    if not _CHOICE_OPTIONS:
+        _CHOICE_OPTIONS.extend(tier_options(k) for k in range(len(CHOICES)))
+    return _CHOICE_OPTIONS
+
+
 def solve_selected(rt, selector, orders, matrices, roots):
     """Selector-chosen tiers with the plain-path fallback.  Returns
     (reports, fallback_indices, choices)."""
+    # One array of orders for every call below (a list would be converted
+    # by each).
+    orders = np.ascontiguousarray(orders, dtype=np.uint32)
     choice, _ = selector.choose_for_batch(rt, orders, matrices)
-    opts_all = [tier_options(k) for k in range(len(CHOICES))]
+    opts_all = choice_options()
     if np.all(choice == choice[0]):  # one record for the whole batch
         opts = opts_all[int(choice[0])]
     else:
