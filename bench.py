@@ -554,7 +554,13 @@ def main():
                     "all_matrices_per_s": round(len(mats) / dt_low, 2),
                     "selector": ({"tier_counts": dict(zip(
                         SEL.CHOICES,
-                        np.bincount(chosen[0], minlength=len(SEL.CHOICES)).tolist()))}
+                        np.bincount(chosen[0], minlength=len(SEL.CHOICES)).tolist())),
+                        # A batch run on the fixed tier as a whole stops its
+                        # large orders at the looser rate (batch_options).
+                        "fixed_batch_large_orders_at_rate": (
+                            {str(SEL.FIXED_BATCH_LARGE_RATE): int(np.sum(
+                                np.asarray(orders) > SEL.FIXED_BATCH_LARGE_ORDER))}
+                            if np.all(chosen[0] == SEL.FIXED) else None)}
                                  if use_selector else None),
                     "max_rel_error": err_low, "within_1e-6": pass_low,
                     "fallbacks": len(fallback),

@@ -43,7 +43,9 @@ Jacobi sweeps and corrections without that cliff.
   are dropped (their matrices take their best remaining choice), and a batch
   whose best single choice is expected within SINGLE_MARGIN of the
   per-matrix choices runs that single choice.  A batch expected to pass on
-  the fixed BF16 tier (GATE_FAILURE_FRACTION) runs it;
+  the fixed BF16 tier (GATE_FAILURE_FRACTION) runs it, and a batch that runs
+  the fixed tier as a whole stops its orders above FIXED_BATCH_LARGE_ORDER at
+  stop rate FIXED_BATCH_LARGE_RATE (``batch_options``);
 * batch gate from a sample: batches of GATE_SAMPLE_MIN matrices or more
   first score one matrix in GATE_SAMPLE_STRIDE (stratified by order,
   ``gate_sample``); when the sample's mean predicted failure of the fixed tier
@@ -105,6 +107,18 @@ GATE_FAILURE_FRACTION = 0.005
 GATE_SAMPLE_STRIDE = 8
 GATE_SAMPLE_MIN = 1024
 GATE_SAMPLE_MARGIN = 1.0
+# A batch that runs the fixed tier as a whole stops the Jacobi of its orders
+# above FIXED_BATCH_LARGE_ORDER at stop rate FIXED_BATCH_LARGE_RATE instead of
+# the tier's 0.5: those orders carry most of the Jacobi's work, and the extra
+# correction they need on average runs inside the correction steps that the
+# batch's smaller matrices take anyway.  Measured on MI355X (blocked
+# eigensolver, graphs), two independently seeded 4096-matrix uneven batches:
+# 36.45 -> 35.43 ms and 36.26 -> 35.31 ms, no fallbacks, the largest
+# correction count unchanged (6 and 7); thresholds 96..160 came within 0.1 ms
+# of each other, rate 1.5 raised the largest count to 8, and rate 1 on the
+# orders up to 128 instead lengthened the correction tail (38.4 ms).
+FIXED_BATCH_LARGE_ORDER = 128
+FIXED_BATCH_LARGE_RATE = 1.0
 
 
 def gate_sample(orders):
@@ -435,10 +449,28 @@ _CHOICE_OPTIONS = []
 
 
 def choice_options():
-    """Solve options of every choice (built once: ctypes records)."""
+    """Solve options of every choice, then the fixed tier's format at
+    FIXED_BATCH_LARGE_RATE (built once: ctypes records)."""
     if not _CHOICE_OPTIONS:
         _CHOICE_OPTIONS.extend(tier_options(k) for k in range(len(CHOICES)))
+        _CHOICE_OPTIONS.append(MS.tier_options(TIERS[FIXED][1], rate=FIXED_BATCH_LARGE_RATE,
+                                               max_corrections=12))
     return _CHOICE_OPTIONS
+
+
+def batch_options(choice, orders):
+    """Options for one batched solve: one record when the batch makes one
+    choice (the fixed tier's with its large orders at FIXED_BATCH_LARGE_RATE),
+    else per-matrix records."""
+    opts = choice_options()
+    if np.all(choice == choice[0]):
+        if int(choice[0]) != FIXED:
+            return opts[int(choice[0])]
+        large = np.asarray(orders) > FIXED_BATCH_LARGE_ORDER
+        if not large.any():
+            return opts[FIXED]
+        return capi.options_array([opts[FIXED], opts[-1]], large)
+    return capi.options_array(opts, choice)
 
 
 def solve_selected(rt, selector, orders, matrices, roots):
@@ -448,10 +480,6 @@ def solve_selected(rt, selector, orders, matrices, roots):
     # by each).
     orders = np.ascontiguousarray(orders, dtype=np.uint32)
     choice, _ = selector.choose_for_batch(rt, orders, matrices)
-    opts_all = choice_options()
-    if np.all(choice == choice[0]):  # one record for the whole batch
-        opts = opts_all[int(choice[0])]
-    else:
-        opts = [opts_all[int(k)] for k in choice]
-    reports, fallback = MS.solve_with_fallback(rt, orders, matrices, roots, opts)
+    reports, fallback = MS.solve_with_fallback(rt, orders, matrices, roots,
+                                               batch_options(choice, orders))
     return reports, fallback, choice

@@ -297,3 +297,42 @@ def test_forward_bf16_rounding_and_closed_forms():
     z16 = M.forward_bf16(p, x)
     z64, _, _ = M.forward(p.astype(np.float64), x.astype(np.float64))
     assert np.max(np.abs(z16 - z64)) <= 0.05 * (np.abs(z64).max() + 1)
+
+
+def test_batch_options_records():
+    """Per-matrix options reach the library as raw msq_solve_options records
+    equal byte for byte to the ctypes structures; a batch that runs the fixed
+    tier as a whole stops its orders above FIXED_BATCH_LARGE_ORDER at
+    FIXED_BATCH_LARGE_RATE and the rest at the tier's rate; any other single
+    choice is one record."""
+    import ctypes
+    from msq import capi, selector as SEL
+
+    opts = SEL.choice_options()
+    assert len(opts) == len(SEL.CHOICES) + 1
+
+    def rec(raw):
+        return capi.SolveOptions.from_buffer_copy(raw.tobytes())
+
+    orders = np.array([16, 128, 129, 144, 256, 64], dtype=np.uint32)
+    fixed = np.full(len(orders), SEL.FIXED)
+    arr = SEL.batch_options(fixed, orders)
+    assert arr.dtype == capi.OPTIONS_RECORD and arr.itemsize == ctypes.sizeof(capi.SolveOptions)
+    for n, r in zip(orders, arr):
+        o = rec(r)
+        want = SEL.FIXED_BATCH_LARGE_RATE if n > SEL.FIXED_BATCH_LARGE_ORDER else SEL.TIERS[SEL.FIXED][2]
+        assert o.jacobi.correction_rate == np.float32(want)
+        assert o.correction_precision == opts[SEL.FIXED].correction_precision
+        assert o.max_corrections == opts[SEL.FIXED].max_corrections
+    # small orders only: the tier's own record
+    small = SEL.batch_options(fixed[:2], orders[:2])
+    assert isinstance(small, capi.SolveOptions)
+    assert bytes(small) == bytes(opts[SEL.FIXED])
+    # another single choice: its record
+    one = SEL.batch_options(np.full(len(orders), SEL.PLAIN), orders)
+    assert bytes(one) == bytes(opts[SEL.PLAIN])
+    # mixed choices: record k for matrix i, byte for byte
+    mixed = np.array([0, 4, 8, 5, 6, 1])
+    arr = SEL.batch_options(mixed, orders)
+    for k, r in zip(mixed, arr):
+        assert r.tobytes() == bytes(opts[k])
