@@ -1068,9 +1068,11 @@ int reconstruction_product(msq_runtime* rt, int tier, const Layout& L,
                            const GemmPlan& plan, const float* V, const float* roots,
                            const int* active, CorrectionBuffers& cb, bool two_term = false) {
   const uint32_t count = static_cast<uint32_t>(L.order.size());
+  // Only the matrices the product computes (active: the solve's running
+  // matrices, or those whose low-bit basis was rejected).
   HIP_TRY(msq::launch_scale_rows_transpose(dsegs, static_cast<uint32_t>(members.size()),
                                            pair_count(L, members),
-                                           V, roots, 1, cb.t1, rt->stream));
+                                           V, roots, 1, cb.t1, active, rt->stream));
   Operand og, olo;
   MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, cb.t1, cb.enc_a, og));
   two_term = two_term && tier == MSQ_MXFP4_E2M1 && !members.empty();

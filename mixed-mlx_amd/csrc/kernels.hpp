@@ -272,9 +272,10 @@ struct Segment {
 hipError_t launch_symmetrize_to_f64(const Segment* segs, uint32_t count, uint32_t pairs,
                                     const float* c, double* x, const int* active,
                                     hipStream_t stream);
+// active (may be null): matrices whose flag is 0 are skipped (out untouched).
 hipError_t launch_scale_rows_transpose(const Segment* segs, uint32_t count, uint32_t pairs,
                                        const float* v, const float* roots, int use_sqrt,
-                                       float* out, hipStream_t stream);
+                                       float* out, const int* active, hipStream_t stream);
 hipError_t launch_transpose(const Segment* segs, uint32_t count, uint32_t pairs,
                             const float* v, float* out, hipStream_t stream);
 hipError_t launch_zero_f64(double* p, size_t n, hipStream_t stream);

@@ -104,9 +104,11 @@ __global__ __launch_bounds__(256) void symmetrize_to_f64_kernel(const Segment* _
 // f = sqrt (use_sqrt) or identity.  Entries outside the order are zero.
 __global__ __launch_bounds__(256) void scale_rows_transpose_kernel(
     const Segment* __restrict__ segs, uint32_t count, const float* __restrict__ v,
-    const float* __restrict__ roots, int use_sqrt, float* __restrict__ out) {
+    const float* __restrict__ roots, int use_sqrt, float* __restrict__ out,
+    const int* __restrict__ active) {
   __shared__ float P[kTile * kTilePitch], Q[kTile * kTilePitch];
   const PairTile pt = pair_tile(segs, count);
+  if (active && !active[pt.sg.index]) return;
   load_pair(v, pt, P, Q);
   const size_t ld = pt.sg.ld;
   const int n = static_cast<int>(pt.sg.order);
@@ -513,9 +515,10 @@ hipError_t launch_symmetrize_to_f64(const Segment* segs, uint32_t count, uint32_
 }
 hipError_t launch_scale_rows_transpose(const Segment* segs, uint32_t count, uint32_t pairs,
                                        const float* v, const float* roots, int use_sqrt,
-                                       float* out, hipStream_t stream) {
+                                       float* out, const int* active, hipStream_t stream) {
   if (!count || !pairs) return hipSuccess;
-  scale_rows_transpose_kernel<<<pairs, kT, 0, stream>>>(segs, count, v, roots, use_sqrt, out);
+  scale_rows_transpose_kernel<<<pairs, kT, 0, stream>>>(segs, count, v, roots, use_sqrt, out,
+                                                        active);
   return hipGetLastError();
 }
 hipError_t launch_transpose(const Segment* segs, uint32_t count, uint32_t pairs, const float* v,

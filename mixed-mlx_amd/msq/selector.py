@@ -51,8 +51,9 @@ Jacobi sweeps and corrections without that cliff.
   ``gate_sample``); when the sample's mean predicted failure of the fixed tier
   is below the gate (times GATE_SAMPLE_MARGIN), the batch runs the fixed tier
   without the other matrices' features (the features are most of the
-  selector's cost: 1.5 ms on the 4096-matrix uneven batch); otherwise the
-  rest of the batch is scored and the full decision above runs.
+  selector's cost: 1.5 ms for the whole 4096-matrix uneven batch, 0.3 ms for
+  its sample); otherwise the rest of the batch is scored and the full
+  decision above runs.
 
 Feature standardisation is folded into the first layer when the weights are
 saved, so inference runs the raw features through the network unchanged.
@@ -97,15 +98,20 @@ SINGLE_MARGIN = 0.1
 # model resolves (measured: the uneven batch expects 0.3% and none fail; the
 # 32..512 mixture expects about 5% and 5.6% fail).
 GATE_FAILURE_FRACTION = 0.005
-# The gate estimated from a stratified sample of one matrix in 8 (batches of
-# 1024 or more), held to the same threshold: near the gate the fixed tier and
-# the per-matrix choices cost about the same, so an estimate that lands on
-# the wrong side of it costs little; far from it (the 32..512 mixture expects
-# 10 times the gate) a sample of 128 or more stratified matrices does not
-# mistake the batch.  A misjudged batch costs time (fallback solves), never
-# accuracy.
-GATE_SAMPLE_STRIDE = 8
-GATE_SAMPLE_MIN = 1024
+# The gate estimated from a stratified sample of one matrix in 16 (batches of
+# 2048 or more, so at least 128 matrices), held to the same threshold: near
+# the gate the fixed tier and the per-matrix choices cost about the same, so
+# an estimate that lands on the wrong side of it costs little; far from it a
+# stratified sample does not mistake the batch.  Measured on MI355X from the
+# network's per-matrix predictions over 4096-matrix batches, resampled 4000
+# times: the uneven batch's sample mean at one in 16 is 0.0025 +- 0.00044
+# (5.7 standard deviations below the gate; two seeds), the 32..512
+# mixture's 0.065 +- 0.013 (4.5 above); neither was misjudged.  One in 16
+# instead of one in 8 takes the sample's features from two waves of
+# workgroups to one: the gate's features and inference 0.41 -> 0.30 ms.  A
+# misjudged batch costs time (fallback solves), never accuracy.
+GATE_SAMPLE_STRIDE = 16
+GATE_SAMPLE_MIN = 2048
 GATE_SAMPLE_MARGIN = 1.0
 # A batch that runs the fixed tier as a whole stops the Jacobi of its orders
 # above FIXED_BATCH_LARGE_ORDER at stop rate FIXED_BATCH_LARGE_RATE instead of
@@ -128,7 +134,10 @@ def gate_sample(orders):
     count = len(orders)
     if count < GATE_SAMPLE_MIN:
         return None
-    by_order = np.argsort(np.asarray(orders), kind="stable")
+    # Orders are at most 512: as uint16 the stable sort is numpy's radix sort
+    # (4x faster than on 32-bit keys; the gate's host time precedes its
+    # kernels).
+    by_order = np.argsort(np.asarray(orders).astype(np.uint16), kind="stable")
     return np.sort(by_order[GATE_SAMPLE_STRIDE // 2::GATE_SAMPLE_STRIDE])
 
 
@@ -346,16 +355,20 @@ class Selector:
             self._p16 = p16
         return self._p16
 
-    def infer(self, rt, feats):
-        """P(tier k succeeds), (rows, 8), for device features (rows x 16):
-        BF16 inference."""
+    def infer(self, rt, feats, rows=None):
+        """P(tier k succeeds), (count, 8), for device features (count x 16):
+        BF16 inference; with `rows` (host indices), the probabilities of those
+        rows only (the network runs on every row: cheaper than a device
+        gather and its index upload)."""
         import torch
 
-        rows = feats.numel() // capi.MLP_INPUTS
-        logits = torch.zeros(rows * capi.MLP_OUTPUTS, dtype=torch.float32, device=feats.device)
+        count = feats.numel() // capi.MLP_INPUTS
+        logits = torch.empty(count * capi.MLP_OUTPUTS, dtype=torch.float32, device=feats.device)
         rt.mlp_forward_bf16(self.bf16_params(rt, feats.device), feats, logits)
-        z = logits.cpu().numpy().reshape(rows, capi.MLP_OUTPUTS).astype(np.float64)
-        return 1.0 / (1.0 + np.exp(-z))
+        z = logits.cpu().numpy().reshape(count, capi.MLP_OUTPUTS)
+        if rows is not None:
+            z = z[rows]
+        return 1.0 / (1.0 + np.exp(-z.astype(np.float64)))
 
     def probabilities(self, rt, orders, matrices):
         """P(tier k succeeds) per matrix, (count, 8), from device matrices:
@@ -379,8 +392,7 @@ class Selector:
         sample = gate_sample(orders)
         if sample is not None:
             rt.matrix_features(orders, matrices, feats, members=sample)
-            rows = torch.as_tensor(sample, dtype=torch.long, device=dev)
-            ps = self.infer(rt, feats.view(count, capi.MLP_INPUTS)[rows].reshape(-1))
+            ps = self.infer(rt, feats, rows=sample)
             if float(np.mean(1.0 - ps[:, FIXED])) < GATE_SAMPLE_MARGIN * GATE_FAILURE_FRACTION:
                 return np.full(count, FIXED), len(sample)
             rest = np.setdiff1d(np.arange(count), sample)

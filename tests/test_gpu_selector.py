@@ -70,7 +70,7 @@ def test_feature_subset_rows_equal_the_full_call(runtime):
 def test_sampled_gate_agrees_with_the_full_batch(runtime):
     """The batched uneven workload (first 2048 matrices): the gate's sample
     sends the batch to the fixed tier, as the full batch's mean predicted
-    failure does; the sample scores one matrix in 8."""
+    failure does; the sample scores one matrix in GATE_SAMPLE_STRIDE."""
     if not SEL.WEIGHTS.exists():
         pytest.skip("no trained selector weights")
     sel = SEL.Selector.load()

@@ -238,8 +238,8 @@ def test_batch_choice_drops_small_format_groups_and_prefers_a_single_choice():
 
 
 def test_gate_sample_is_stratified_by_order():
-    """One matrix in 8 of every order class, deterministic, in position
-    order; small batches are scored whole."""
+    """One matrix in GATE_SAMPLE_STRIDE of every order class, deterministic,
+    in position order; small batches are scored whole."""
     from msq import selector as SEL
     assert SEL.gate_sample([64] * (SEL.GATE_SAMPLE_MIN - 1)) is None
     rng = np.random.default_rng(4)
