@@ -519,23 +519,28 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
           const int k = kl & 0xff, l = kl >> 8;
           const int bk[2] = {participant(round, k, NB), participant(round, NB - 1 - k, NB)};
           const int bl[2] = {participant(round, l, NB), participant(round, NB - 1 - l, NB)};
-          int base[4], upm = 0;
-#pragma unroll
-          for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-              const int X = bk[a], Y = bl[b];
-              const int lo = min(X, Y), hi = max(X, Y);
-              base[2 * a + b] = (hi * (hi + 1) / 2 + lo) * 64;
-              upm |= (X < Y ? 1 : 0) << (2 * a + b);
-            }
+          // Tile (row half a, column half b): its offset and whether it is
+          // stored as (row block, column block) (X < Y) or transposed.
+          auto tile_base = [](int X, int Y) {
+            const int lo = min(X, Y), hi = max(X, Y);
+            return (hi * (hi + 1) / 2 + lo) * 64;
+          };
+          const int base0 = tile_base(bk[0], bl[0]), base1 = tile_base(bk[0], bl[1]);
+          const int base2 = tile_base(bk[1], bl[0]), base3 = tile_base(bk[1], bl[1]);
+          const int upm = (bk[0] < bl[0] ? 1 : 0) | (bk[0] < bl[1] ? 2 : 0) |
+                          (bk[1] < bl[0] ? 4 : 0) | (bk[1] < bl[1] ? 8 : 0);
+          // This lane's row half hr: its two tiles and their orientations,
+          // selected once (explicit selects, pinned: a select between array
+          // elements by a lane value becomes a private-memory array access).
+          int rb0 = hr ? base2 : base0, rb1 = hr ? base3 : base1;
+          int ru = hr ? upm >> 2 : upm;
+          asm volatile("" : "+v"(rb0), "+v"(rb1), "+v"(ru));
           // Z = T' Q_l: A operand T[sigma(i)][4 s + g].
           float ta[4];
 #pragma unroll
           for (int s2 = 0; s2 < 4; ++s2) {
-            const int t = 2 * hr + (s2 >> 1);
-            const int b0 = hr ? base[2 + (s2 >> 1)] : base[s2 >> 1];
-            ta[s2] = At[b0 + (((upm >> t) & 1) ? offr[s2] : offc[s2])];
+            const int b0 = (s2 >> 1) ? rb1 : rb0;
+            ta[s2] = At[b0 + (((ru >> (s2 >> 1)) & 1) ? offr[s2] : offc[s2])];
           }
           const f4 ql = *reinterpret_cast<const MSQ_LDS f4*>(Qs + l * 256 + q_chunk(i, g));
           const f4 qk = *reinterpret_cast<const MSQ_LDS f4*>(Qs + k * 256 + q_chunk(i, g));
@@ -549,7 +554,7 @@ __device__ __forceinline__ void tile_jacobi_solve(const JacobiJob& job,
             o = __builtin_amdgcn_mfma_f32_16x16x4f32(qk[s2], z[s2], o, 0, 0, 0);
           // o[r] = T_new[4 g + r][j], j = i: row half g / 2, column half i / 8.
           const int t = 2 * hs + hc;
-          const int b0 = hs ? (hc ? base[3] : base[2]) : (hc ? base[1] : base[0]);
+          const int b0 = hs ? (hc ? base3 : base2) : (hc ? base1 : base0);
           if ((upm >> t) & 1) {
             MSQ_LDS float* dst = At + b0 + r8 * 8 + i8;
 #pragma unroll
