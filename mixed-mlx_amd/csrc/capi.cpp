@@ -2662,9 +2662,22 @@ int msq_solve_batched_ex(msq_runtime* rt, uint32_t count, const uint32_t* orders
     if (stagnation_any)
       HIP_TRY(msq::launch_copy_tiles_f64(dsegs_stag, static_cast<uint32_t>(stag_members.size()),
                                          L.max_ld, roots_out, xbest, nullptr, rt->stream));
-    // 4. Frozen factor in each correction tier: V and V^T.
-    HIP_TRY(msq::launch_transpose(dsegs_all, count, pair_count(segs_all), V, Vt, rt->stream));
+    // 4. Frozen factor in each correction tier: V and V^T.  A BF16 tier
+    //    takes both codes from one pass over V; the float V^T is formed
+    //    only for the other tiers.
+    bool float_vt = false;
+    for (const CorrTier& t : tiers) float_vt = float_vt || t.tier != MSQ_BF16;
+    if (float_vt)
+      HIP_TRY(msq::launch_transpose(dsegs_all, count, pair_count(segs_all), V, Vt, rt->stream));
     for (CorrTier& t : tiers) {
+      if (t.tier == MSQ_BF16) {
+        HIP_TRY(msq::launch_transpose_encode_bf16(
+            t.dsegs, static_cast<uint32_t>(t.members.size()), pair_count(t.segs), V,
+            static_cast<uint16_t*>(t.ev.codes), static_cast<uint16_t*>(t.evt.codes), rt->stream));
+        t.ov.data = t.ev.codes;
+        t.ovt.data = t.evt.codes;
+        continue;
+      }
       MSQ_TRY(encode_operand(rt, t.tier, L, t.members, t.dsegs, V, t.ev, t.ov));
       MSQ_TRY(encode_operand(rt, t.tier, L, t.members, t.dsegs, Vt, t.evt, t.ovt));
     }

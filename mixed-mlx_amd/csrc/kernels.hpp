@@ -278,6 +278,10 @@ hipError_t launch_scale_rows_transpose(const Segment* segs, uint32_t count, uint
                                        float* out, const int* active, hipStream_t stream);
 hipError_t launch_transpose(const Segment* segs, uint32_t count, uint32_t pairs,
                             const float* v, float* out, hipStream_t stream);
+// BF16 codes of v (cv) and of its per-matrix transpose (cvt), same layout.
+hipError_t launch_transpose_encode_bf16(const Segment* segs, uint32_t count, uint32_t pairs,
+                                        const float* v, uint16_t* cv, uint16_t* cvt,
+                                        hipStream_t stream);
 hipError_t launch_zero_f64(double* p, size_t n, hipStream_t stream);
 hipError_t launch_subtract(const float* a, const float* b, float* out, size_t n,
                            hipStream_t stream);

@@ -146,6 +146,32 @@ __global__ __launch_bounds__(256) void transpose_kernel(const Segment* __restric
   }
 }
 
+// BF16 codes of V and of V^T from one read of V: the tile pair's two tiles
+// are written as they are into cv and transposed into cvt (the codec's
+// round-to-nearest-even encode, so the codes equal an encode of V and of a
+// transpose of V).
+__global__ __launch_bounds__(256) void transpose_encode_bf16_kernel(
+    const Segment* __restrict__ segs, uint32_t count, const float* __restrict__ v,
+    uint16_t* __restrict__ cv, uint16_t* __restrict__ cvt) {
+  __shared__ float P[kTile * kTilePitch], Q[kTile * kTilePitch];
+  const PairTile pt = pair_tile(segs, count);
+  load_pair(v, pt, P, Q);
+  const size_t ld = pt.sg.ld;
+  uint16_t* const vm = cv + pt.sg.offset;
+  uint16_t* const tm = cvt + pt.sg.offset;
+  for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+    const int i = e % kTile, j = e / kTile;
+    const size_t at = (kTile * pt.ti + i) + static_cast<size_t>(kTile * pt.tj + j) * ld;
+    vm[at] = encode_bf16(P[i + kTilePitch * j]);
+    tm[at] = encode_bf16(Q[j + kTilePitch * i]);
+    if (pt.ti != pt.tj) {
+      const size_t ta = (kTile * pt.tj + i) + static_cast<size_t>(kTile * pt.ti + j) * ld;
+      vm[ta] = encode_bf16(Q[i + kTilePitch * j]);
+      tm[ta] = encode_bf16(P[j + kTilePitch * i]);
+    }
+  }
+}
+
 __global__ void zero_f64_kernel(double* p, size_t n) {
   for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
        e += static_cast<size_t>(gridDim.x) * blockDim.x)
@@ -293,19 +319,29 @@ __global__ void solve_state_kernel(const Segment* __restrict__ segs,
                                    const SolveLimits* __restrict__ limits,
                                    SolveState* __restrict__ st, int* __restrict__ active,
                                    uint32_t count, int allow_refine) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
+  // One wave per matrix: the lanes stride over the diagonal, then a
+  // butterfly (a fixed order) gives every lane the sums; lane 0 writes.
+  const uint32_t i = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  if (i >= count) return;  // whole waves
   const Segment sg = segs[i];
   double trace = 0.0;
   float dmin = INFINITY, dmax = 0.0f;
-  bool finite = true;
-  for (uint32_t k = 0; k < sg.order; ++k) {
+  int finite = 1;
+  for (uint32_t k = static_cast<uint32_t>(lane); k < sg.order; k += 64) {
     const float d = diagonal[sg.vec_offset + k];
     trace += d;
     dmin = fminf(dmin, d);
     dmax = fmaxf(dmax, d);
     finite = finite && isfinite(d);
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    trace += __shfl_xor(trace, o);
+    dmin = fminf(dmin, __shfl_xor(dmin, o));
+    dmax = fmaxf(dmax, __shfl_xor(dmax, o));
+    finite = finite & __shfl_xor(finite, o);
+  }
+  if (lane != 0) return;
   const SolveLimits& lim = limits[i];
   SolveState s = {};
   s.matrix_norm = js[i].matrix_norm;
@@ -513,6 +549,13 @@ hipError_t launch_symmetrize_to_f64(const Segment* segs, uint32_t count, uint32_
   symmetrize_to_f64_kernel<<<pairs, kT, 0, stream>>>(segs, count, c, x, active);
   return hipGetLastError();
 }
+hipError_t launch_transpose_encode_bf16(const Segment* segs, uint32_t count, uint32_t pairs,
+                                        const float* v, uint16_t* cv, uint16_t* cvt,
+                                        hipStream_t stream) {
+  if (!count || !pairs) return hipSuccess;
+  transpose_encode_bf16_kernel<<<pairs, kT, 0, stream>>>(segs, count, v, cv, cvt);
+  return hipGetLastError();
+}
 hipError_t launch_scale_rows_transpose(const Segment* segs, uint32_t count, uint32_t pairs,
                                        const float* v, const float* roots, int use_sqrt,
                                        float* out, const int* active, hipStream_t stream) {
@@ -582,8 +625,9 @@ hipError_t launch_solve_state(const Segment* segs, const JacobiStatus* js,
                               int* active, uint32_t count, bool allow_refine,
                               hipStream_t stream) {
   if (!count) return hipSuccess;
-  solve_state_kernel<<<(count + kT - 1) / kT, kT, 0, stream>>>(segs, js, diagonal, limits, st,
-                                                              active, count, allow_refine ? 1 : 0);
+  constexpr uint32_t kPerBlock = kT / 64;  // one wave per matrix
+  solve_state_kernel<<<(count + kPerBlock - 1) / kPerBlock, kT, 0, stream>>>(
+      segs, js, diagonal, limits, st, active, count, allow_refine ? 1 : 0);
   return hipGetLastError();
 }
 hipError_t launch_basis_escalation(const Segment* segs, uint32_t count, const double* orth2,
