@@ -274,7 +274,10 @@ def configs_leg(rt, torch, np, capi, G, MS, cpu=True):
         records.append({"config": f"n=4096 cond 1e3 {name} reconstruction, bf16 corrections",
                         **steps(A, Xc, p, capi.BF16, 3, jac)})
     # Codecs at the n = 4096 shape: HBM bytes 4N (float) + N bits / 8 (+ N / 32
-    # scale bytes for MX) per direction.
+    # scale bytes for MX) per direction, over the time of one call.  The codec
+    # entry points wait for their kernels, so a call's time is the kernel's
+    # (about 15 us at this size) plus the host's launch and wait; the kernels'
+    # own rates come from a kernel trace.
     n = 4096
     N = n * n
     g = torch.Generator(device="cuda").manual_seed(7)

@@ -92,6 +92,37 @@ __global__ void decode_bf16_kernel(const uint16_t* __restrict__ src,
   }
 }
 
+// E4M3FN codes of eight floats, packed (element j in byte j), by the
+// hardware conversion v_cvt_pk_fp8_f32 (OCP E4M3FN, round to nearest even),
+// with encode_e4m3's rules for what the conversion does differently:
+// magnitudes above 448 (infinity included) are clamped to 448 first (code
+// 0x7e), and a NaN becomes sign | 0x7f.
+__device__ __forceinline__ uint2 encode_e4m3_x8(const float (&x)[8]) {
+  float c[8];
+  bool nan = false;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    c[j] = fabsf(x[j]) > 448.0f ? copysignf(448.0f, x[j]) : x[j];
+    nan = nan || x[j] != x[j];
+  }
+  int lo = __builtin_amdgcn_cvt_pk_fp8_f32(c[0], c[1], 0, false);
+  lo = __builtin_amdgcn_cvt_pk_fp8_f32(c[2], c[3], lo, true);
+  int hi = __builtin_amdgcn_cvt_pk_fp8_f32(c[4], c[5], 0, false);
+  hi = __builtin_amdgcn_cvt_pk_fp8_f32(c[6], c[7], hi, true);
+  uint2 w = make_uint2(static_cast<uint32_t>(lo), static_cast<uint32_t>(hi));
+  if (nan) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (x[j] == x[j]) continue;
+      const uint32_t code = ((f32_bits(x[j]) >> 24) & 0x80u) | 0x7fu;
+      const int s = 8 * (j & 3);
+      uint32_t& word = j < 4 ? w.x : w.y;
+      word = (word & ~(0xffu << s)) | (code << s);
+    }
+  }
+  return w;
+}
+
 // FP8, scalar or per-tensor scaled.  fmt: 0 E4M3FN, 1 E5M2.  When `sigma` is
 // non-null the value encoded is v * (*sigma) (one IEEE product).
 __global__ void encode_fp8_kernel(const float* __restrict__ src,
@@ -103,6 +134,13 @@ __global__ void encode_fp8_kernel(const float* __restrict__ src,
        i < n; i += stride) {
     float v[8];
     load8(src, i, n, v);
+    if (fmt == 0 && i + 8 <= n) {
+      float x[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[j] = sigma ? v[j] * scale : v[j];
+      *reinterpret_cast<uint2*>(dst + i) = encode_e4m3_x8(x);
+      continue;
+    }
     uint8_t c[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -290,14 +328,24 @@ __global__ void decode_mxfp8_kernel(const uint8_t* __restrict__ src,
 __global__ void amax_kernel(const float* __restrict__ src, size_t n,
                             uint32_t* __restrict__ out) {
   uint32_t m = 0;
+  // Two groups of eight per iteration, both loaded before either is reduced
+  // (four 16-byte loads in flight per lane: the grid is capped, so each lane
+  // walks the tensor in several steps).
   const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x * 8;
   for (size_t i = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) * 8;
-       i < n; i += stride) {
-    float v[8];
+       i < n; i += 2 * stride) {
+    float v[8], w[8];
     load8(src, i, n, v);
+    const size_t i2 = i + stride;
+    if (i2 < n) load8(src, i2, n, w);
 #pragma unroll
     for (int j = 0; j < 8; ++j)
       if (i + j < n) m = max(m, finite_abs_bits(v[j]));
+    if (i2 < n) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (i2 + j < n) m = max(m, finite_abs_bits(w[j]));
+    }
   }
   for (int off = 32; off > 0; off >>= 1)
     m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), off)));
