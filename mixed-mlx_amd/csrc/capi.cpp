@@ -1069,12 +1069,21 @@ int reconstruction_product(msq_runtime* rt, int tier, const Layout& L,
                            const int* active, CorrectionBuffers& cb, bool two_term = false) {
   const uint32_t count = static_cast<uint32_t>(L.order.size());
   // Only the matrices the product computes (active: the solve's running
-  // matrices, or those whose low-bit basis was rejected).
-  HIP_TRY(msq::launch_scale_rows_transpose(dsegs, static_cast<uint32_t>(members.size()),
-                                           pair_count(L, members),
-                                           V, roots, 1, cb.t1, active, rt->stream));
+  // matrices, or those whose low-bit basis was rejected).  BF16: G goes
+  // straight to its codes.
   Operand og, olo;
-  MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, cb.t1, cb.enc_a, og));
+  if (tier == MSQ_BF16) {
+    HIP_TRY(msq::launch_scale_rows_transpose_bf16(dsegs, static_cast<uint32_t>(members.size()),
+                                                  pair_count(L, members), V, roots, 1,
+                                                  static_cast<uint16_t*>(cb.enc_a.codes), active,
+                                                  rt->stream));
+    og.data = cb.enc_a.codes;
+  } else {
+    HIP_TRY(msq::launch_scale_rows_transpose(dsegs, static_cast<uint32_t>(members.size()),
+                                             pair_count(L, members),
+                                             V, roots, 1, cb.t1, active, rt->stream));
+    MSQ_TRY(encode_operand(rt, tier, L, members, dsegs, cb.t1, cb.enc_a, og));
+  }
   two_term = two_term && tier == MSQ_MXFP4_E2M1 && !members.empty();
   if (two_term) {
     // lo = mxfp4(G - dec(hi)) over the members' span of the batch (the span

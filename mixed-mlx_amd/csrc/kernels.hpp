@@ -272,6 +272,10 @@ struct Segment {
 hipError_t launch_symmetrize_to_f64(const Segment* segs, uint32_t count, uint32_t pairs,
                                     const float* c, double* x, const int* active,
                                     hipStream_t stream);
+// The same as BF16 codes.
+hipError_t launch_scale_rows_transpose_bf16(const Segment* segs, uint32_t count, uint32_t pairs,
+                                            const float* v, const float* roots, int use_sqrt,
+                                            uint16_t* out, const int* active, hipStream_t stream);
 // active (may be null): matrices whose flag is 0 are skipped (out untouched).
 hipError_t launch_scale_rows_transpose(const Segment* segs, uint32_t count, uint32_t pairs,
                                        const float* v, const float* roots, int use_sqrt,

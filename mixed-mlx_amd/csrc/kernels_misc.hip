@@ -128,6 +128,34 @@ __global__ __launch_bounds__(256) void scale_rows_transpose_kernel(
   if (pt.ti != pt.tj) put(pt.tj, pt.ti, P);
 }
 
+// scale_rows_transpose_kernel with the result as BF16 codes (the codec's
+// round-to-nearest-even encode of the same float values).
+__global__ __launch_bounds__(256) void scale_rows_transpose_bf16_kernel(
+    const Segment* __restrict__ segs, uint32_t count, const float* __restrict__ v,
+    const float* __restrict__ roots, int use_sqrt, uint16_t* __restrict__ out,
+    const int* __restrict__ active) {
+  __shared__ float P[kTile * kTilePitch], Q[kTile * kTilePitch];
+  const PairTile pt = pair_tile(segs, count);
+  if (active && !active[pt.sg.index]) return;
+  load_pair(v, pt, P, Q);
+  const size_t ld = pt.sg.ld;
+  const int n = static_cast<int>(pt.sg.order);
+  uint16_t* om = out + pt.sg.offset;
+  auto put = [&](int tk, int tm, const float* T) {  // out tile (tk, tm) from V^T
+    for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+      const int kk = kTile * tk + e % kTile, mm = kTile * tm + e / kTile;
+      float val = 0.0f;
+      if (kk < n && mm < n) {
+        const float s = roots[pt.sg.vec_offset + kk];
+        val = T[(e / kTile) + kTilePitch * (e % kTile)] * (use_sqrt ? sqrtf(s) : s);
+      }
+      om[kk + static_cast<size_t>(mm) * ld] = encode_bf16(val);
+    }
+  };
+  put(pt.ti, pt.tj, Q);
+  if (pt.ti != pt.tj) put(pt.tj, pt.ti, P);
+}
+
 __global__ __launch_bounds__(256) void transpose_kernel(const Segment* __restrict__ segs,
                                                         uint32_t count,
                                                         const float* __restrict__ v,
@@ -554,6 +582,14 @@ hipError_t launch_transpose_encode_bf16(const Segment* segs, uint32_t count, uin
                                         hipStream_t stream) {
   if (!count || !pairs) return hipSuccess;
   transpose_encode_bf16_kernel<<<pairs, kT, 0, stream>>>(segs, count, v, cv, cvt);
+  return hipGetLastError();
+}
+hipError_t launch_scale_rows_transpose_bf16(const Segment* segs, uint32_t count, uint32_t pairs,
+                                            const float* v, const float* roots, int use_sqrt,
+                                            uint16_t* out, const int* active, hipStream_t stream) {
+  if (!count || !pairs) return hipSuccess;
+  scale_rows_transpose_bf16_kernel<<<pairs, kT, 0, stream>>>(segs, count, v, roots, use_sqrt,
+                                                             out, active);
   return hipGetLastError();
 }
 hipError_t launch_scale_rows_transpose(const Segment* segs, uint32_t count, uint32_t pairs,
