@@ -42,18 +42,21 @@ __device__ __forceinline__ double rsqrt_refined(double w) {  // w > 0, normal
   return y;
 }
 // STEPS = 1: one Newton step per estimate, still far below the float
-// rounding of the result (the blocked eigensolver, whose products round
-// differently from the oracle anyway, uses it).
-template <int STEPS = 2>
+// rounding of the result.  T_STEPS (the blocked eigensolver: 0) refines only
+// the estimates that set t: an error there changes the angle, so the
+// rotation leaves a_pq at that relative size instead of zero, far below the
+// FP32 rounding of the update itself; c = 1 / sqrt(1 + t^2) keeps STEPS, so
+// c^2 + s^2 = 1 to binary64 accuracy for whatever t came out.
+template <int STEPS = 2, int T_STEPS = STEPS>
 __device__ __forceinline__ float2 rotation_angles(float app, float aqq, float apq) {
   if (apq == 0.0f) return make_float2(1.0f, 0.0f);
   const double d = static_cast<double>(aqq) - static_cast<double>(app);
   const double e = 2.0 * static_cast<double>(apq);
   const double w = fma(d, d, e * e);
-  const double den = fabs(d) + w * rsqrt_refined<STEPS>(w);
+  const double den = fabs(d) + w * rsqrt_refined<T_STEPS>(w);
   double r = __builtin_amdgcn_rcp(den);
 #pragma unroll
-  for (int i = 0; i < STEPS; ++i) r = r * fma(-den, r, 2.0);
+  for (int i = 0; i < T_STEPS; ++i) r = r * fma(-den, r, 2.0);
   const double t = copysign(fabs(e) * r, d * e);
   const double c = rsqrt_refined<STEPS>(fma(t, t, 1.0));
   return make_float2(static_cast<float>(c), static_cast<float>(t * c));

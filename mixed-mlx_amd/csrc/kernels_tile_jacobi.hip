@@ -222,7 +222,7 @@ __device__ __forceinline__ void sub_round(SubState (&st)[P], int tau, int x, int
     const bool xp = a < b;
 #pragma unroll
     for (int u = 0; u < P; ++u) {
-      const float2 r = rotation_angles<1>(xp ? diag[u] : pdiag[u], xp ? pdiag[u] : diag[u],
+      const float2 r = rotation_angles<1, 0>(xp ? diag[u] : pdiag[u], xp ? pdiag[u] : diag[u],
                                        xp ? off[u] : poff[u]);
       rw[u][x] = f2{r.x, xp ? -r.y : r.y};
     }
