@@ -1462,12 +1462,20 @@ struct GraphEntry {
   Arena arena;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // Correction steps after `split` in a graph of their own, launched only
+  // while some matrix is still running after the first graph; `split` is
+  // the last step that the eager call needed.
+  hipGraph_t tail_graph = nullptr;
+  hipGraphExec_t tail_exec = nullptr;
+  uint32_t split = UINT32_MAX;
   int calls = 0;  // eager calls so far (the second one is captured)
   Layout layout;
   msq::SolveState* st = nullptr;
   msq::JacobiStatus* jst = nullptr;
   float* V = nullptr;
   float* sroot = nullptr;
+  int* active = nullptr;
+  std::vector<int> flags;  // host copy of `active` at the split
 };
 
 namespace {
@@ -1480,6 +1488,8 @@ void release_graph(msq_runtime* rt, GraphEntry& g) {
   }
   if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (g.graph) (void)hipGraphDestroy(g.graph);
+  if (g.tail_exec) (void)hipGraphExecDestroy(g.tail_exec);
+  if (g.tail_graph) (void)hipGraphDestroy(g.tail_graph);
   if (g.arena.base) (void)hipFree(g.arena.base);
   g = GraphEntry{};
 }
@@ -1519,7 +1529,7 @@ GraphEntry* find_graph(msq_runtime* rt, uint32_t count, const uint32_t* orders,
 // null stream cannot be captured), ordered after and before the caller's
 // stream with events.
 template <typename F>
-int capture_graph(msq_runtime* rt, GraphEntry& g, F&& enqueue) {
+int capture_graph(msq_runtime* rt, hipGraph_t& out_graph, hipGraphExec_t& out_exec, F&& enqueue) {
   hipStream_t caller = rt->stream;
   HIP_TRY(hipEventRecord(rt->graph_in, caller));
   HIP_TRY(hipStreamWaitEvent(rt->own_stream, rt->graph_in, 0));
@@ -1540,17 +1550,32 @@ int capture_graph(msq_runtime* rt, GraphEntry& g, F&& enqueue) {
     (void)hipGraphDestroy(graph);
     return set_error(MSQ_ERROR_HIP, "graph instantiate: %s", hipGetErrorString(e));
   }
-  g.graph = graph;
-  g.exec = exec;
+  out_graph = graph;
+  out_exec = exec;
   return MSQ_OK;
 }
 
-int launch_graph(msq_runtime* rt, GraphEntry& g) {
+int launch_exec(msq_runtime* rt, hipGraphExec_t exec) {
   HIP_TRY(hipEventRecord(rt->graph_in, rt->stream));
   HIP_TRY(hipStreamWaitEvent(rt->own_stream, rt->graph_in, 0));
-  HIP_TRY(hipGraphLaunch(g.exec, rt->own_stream));
+  HIP_TRY(hipGraphLaunch(exec, rt->own_stream));
   HIP_TRY(hipEventRecord(rt->graph_out, rt->own_stream));
   HIP_TRY(hipStreamWaitEvent(rt->stream, rt->graph_out, 0));
+  return MSQ_OK;
+}
+
+// The entry's first graph, then its tail graph if some matrix is still
+// running (the host reads the per-matrix running flags in between).
+int launch_graph(msq_runtime* rt, GraphEntry& g) {
+  MSQ_TRY(launch_exec(rt, g.exec));
+  if (!g.tail_exec) return MSQ_OK;
+  const uint32_t count = static_cast<uint32_t>(g.layout.order.size());
+  g.flags.resize(count);
+  HIP_TRY(hipMemcpyAsync(g.flags.data(), g.active, count * sizeof(int), hipMemcpyDeviceToHost,
+                         rt->stream));
+  HIP_TRY(hipStreamSynchronize(rt->stream));
+  for (uint32_t i = 0; i < count; ++i)
+    if (g.flags[i]) return launch_exec(rt, g.tail_exec);
   return MSQ_OK;
 }
 
@@ -1562,7 +1587,7 @@ int launch_graph(msq_runtime* rt, GraphEntry& g) {
 // matrix that stops converged while unresolved reports MSQ_SOLVE_UNRESOLVED.
 int finish_solve(msq_runtime* rt, const Layout& L, const msq::SolveState* st,
                  const msq::JacobiStatus* jst, float* V, float* sroot, msq_solve_report* reports,
-                 const SolveMeta& meta) {
+                 const SolveMeta& meta, uint32_t* max_steps = nullptr) {
   const uint32_t count = static_cast<uint32_t>(L.order.size());
   std::vector<msq::SolveState> hs(count);
   std::vector<msq::JacobiStatus> hj(count);
@@ -1571,6 +1596,10 @@ int finish_solve(msq_runtime* rt, const Layout& L, const msq::SolveState* st,
   HIP_TRY(hipMemcpyAsync(hj.data(), jst, count * sizeof(msq::JacobiStatus), hipMemcpyDeviceToHost,
                          rt->stream));
   HIP_TRY(hipStreamSynchronize(rt->stream));
+  if (max_steps) {
+    *max_steps = 0;
+    for (uint32_t i = 0; i < count; ++i) *max_steps = std::max(*max_steps, hs[i].steps);
+  }
   if (reports) {
     for (uint32_t i = 0; i < count; ++i) {
       msq_solve_report& r = reports[i];
@@ -2566,7 +2595,42 @@ int msq_solve_batched_ex(msq_runtime* rt, uint32_t count, const uint32_t* orders
   // Device work: Jacobi, reconstruction, frozen factor, residual and
   // corrections (no host synchronisation below, except for orders above 256
   // and the second Jacobi pass of unresolved matrices).
-  auto enqueue = [&]() -> int {
+  // 5. Residual of X0, then corrections until each matrix stops.
+  auto residual_step = [&](uint32_t step) -> int {
+    MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, nullptr, R32, Rb, norms, active,
+                             count));
+    HIP_TRY(msq::launch_convergence(st, norms, active, stagnation_any ? save : nullptr, count,
+                                    step, rt->stream));
+    if (stagnation_any)
+      HIP_TRY(msq::launch_copy_tiles_f64(dsegs_stag, static_cast<uint32_t>(stag_members.size()),
+                                         L.max_ld, roots_out, xbest, save, rt->stream));
+    return MSQ_OK;
+  };
+  // Correction steps first..last (each ends with the residual of the new
+  // iterate).
+  auto correction_steps = [&](uint32_t first, uint32_t last) -> int {
+    for (uint32_t step = first; step <= last; ++step) {
+      for (Group& g : gl) {
+        const CorrTier* t = nullptr;
+        for (const CorrTier& c : tiers)
+          if (c.tier == g.corr) t = &c;
+        MSQ_TRY(correction_products(rt, g.corr, L, g.members, g.dsegs, g.plan, t->ov, t->ovt, R32,
+                                    Rb, sroot, active, cb, roots_out, &g.upper));
+      }
+      // Native: U = Q_t R, C = U Q, Y = C / (s_i + s_j), V2 = Q Y, E = V2 Q_t,
+      // each rounded to the format; X = sym(X + E).
+      for (NativeGroup& g : nl) {
+        const uint32_t gm = static_cast<uint32_t>(g.members.size());
+        MSQ_TRY(native_correction(rt, L, native_set(g), R32, nq, nqT, nqt, nqtT, qroot, cb.t1,
+                                  cb.t2, active));
+        HIP_TRY(msq::launch_add_symmetrize_f64(g.dsegs, gm, L.max_ld, roots_out, cb.t2, active,
+                                               rt->stream));
+      }
+      MSQ_TRY(residual_step(step));
+    }
+    return MSQ_OK;
+  };
+  auto enqueue = [&](uint32_t last) -> int {
     // 1. Jacobi (symmetricJacobi).
     MSQ_TRY(enqueue_jacobi(rt, jplan, djobs, matrices, V, diag, sroot, jst));
     if (jplan.has_large)
@@ -2690,37 +2754,8 @@ int msq_solve_batched_ex(msq_runtime* rt, uint32_t count, const uint32_t* orders
       MSQ_TRY(encode_operand(rt, t.tier, L, t.members, t.dsegs, V, t.ev, t.ov));
       MSQ_TRY(encode_operand(rt, t.tier, L, t.members, t.dsegs, Vt, t.evt, t.ovt));
     }
-    // 5. Residual of X0, then corrections until each matrix stops.
-    auto residual_step = [&](uint32_t step) -> int {
-      MSQ_TRY(residual_product(rt, plan_res, matrices, roots_out, nullptr, R32, Rb, norms, active,
-                               count));
-      HIP_TRY(msq::launch_convergence(st, norms, active, stagnation_any ? save : nullptr, count,
-                                      step, rt->stream));
-      if (stagnation_any)
-        HIP_TRY(msq::launch_copy_tiles_f64(dsegs_stag, static_cast<uint32_t>(stag_members.size()),
-                                           L.max_ld, roots_out, xbest, save, rt->stream));
-      return MSQ_OK;
-    };
     MSQ_TRY(residual_step(0));
-    for (uint32_t step = 1; step <= max_corrections; ++step) {
-      for (Group& g : gl) {
-        const CorrTier* t = nullptr;
-        for (const CorrTier& c : tiers)
-          if (c.tier == g.corr) t = &c;
-        MSQ_TRY(correction_products(rt, g.corr, L, g.members, g.dsegs, g.plan, t->ov, t->ovt, R32,
-                                    Rb, sroot, active, cb, roots_out, &g.upper));
-      }
-      // Native: U = Q_t R, C = U Q, Y = C / (s_i + s_j), V2 = Q Y, E = V2 Q_t,
-      // each rounded to the format; X = sym(X + E).
-      for (NativeGroup& g : nl) {
-        const uint32_t gm = static_cast<uint32_t>(g.members.size());
-        MSQ_TRY(native_correction(rt, L, native_set(g), R32, nq, nqT, nqt, nqtT, qroot, cb.t1,
-                                  cb.t2, active));
-        HIP_TRY(msq::launch_add_symmetrize_f64(g.dsegs, gm, L.max_ld, roots_out, cb.t2, active,
-                                               rt->stream));
-      }
-      MSQ_TRY(residual_step(step));
-    }
+    MSQ_TRY(correction_steps(1, last));
     // The stagnation policy returns the best iterate.
     if (stagnation_any)
       HIP_TRY(msq::launch_copy_tiles_f64(dsegs_stag, static_cast<uint32_t>(stag_members.size()),
@@ -2731,18 +2766,42 @@ int msq_solve_batched_ex(msq_runtime* rt, uint32_t count, const uint32_t* orders
     return MSQ_OK;
   };
   if (graph && !jplan.has_large && !refine_any && graph->calls >= 1) {
-    MSQ_TRY(capture_graph(rt, *graph, enqueue));
+    // The steps after the last one that the eager call needed go to a
+    // second graph (not under the stagnation policy, whose best iterate is
+    // copied after the last step).
+    const uint32_t split =
+        stagnation_any ? max_corrections : std::min(graph->split, max_corrections);
+    hipGraph_t hg = nullptr, tg = nullptr;
+    hipGraphExec_t he = nullptr, te = nullptr;
+    MSQ_TRY(capture_graph(rt, hg, he, [&] { return enqueue(split); }));
+    if (split < max_corrections) {
+      const int st2 = capture_graph(
+          rt, tg, te, [&] { return correction_steps(split + 1, max_corrections); });
+      if (st2 != MSQ_OK) {
+        (void)hipGraphExecDestroy(he);
+        (void)hipGraphDestroy(hg);
+        return st2;
+      }
+    }
+    graph->graph = hg;
+    graph->exec = he;
+    graph->tail_graph = tg;
+    graph->tail_exec = te;
     graph->layout = L;
     graph->st = st;
     graph->jst = jst;
     graph->V = V;
     graph->sroot = sroot;
+    graph->active = active;
     MSQ_TRY(launch_graph(rt, *graph));
   } else {
-    MSQ_TRY(enqueue());
+    MSQ_TRY(enqueue(max_corrections));
     if (graph) ++graph->calls;
   }
-  return finish_solve(rt, L, st, jst, V, sroot, reports, meta);
+  uint32_t steps = 0;
+  const int status = finish_solve(rt, L, st, jst, V, sroot, reports, meta, &steps);
+  if (status == MSQ_OK && graph && !graph->exec) graph->split = steps > 0 ? steps - 1 : 0;
+  return status;
 }
 
 int msq_last_factors(msq_runtime* rt, const float** vectors, const float** roots) {

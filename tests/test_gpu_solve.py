@@ -270,6 +270,45 @@ def test_graph_replays_match_eager(runtime):
         runtime.set_graphs(False)
 
 
+def test_graph_tail_steps_run_when_needed(runtime):
+    """A captured solve holds the correction steps up to the last one that
+    its eager call needed; later steps sit in a second graph that runs only
+    while some matrix is still running.  Replays on harder contents of the
+    same buffers (more corrections than the eager call needed) and on the
+    easy ones again must both give the eager results bit for bit."""
+    orders = [32, 48, 64, 96, 17]
+    # Diagonal matrices of powers of four: V = I and exact square roots, so
+    # the initial root already has a zero residual.
+    easy = [np.diag(4.0 ** (np.arange(n) % 5 - 2)) for n in orders]
+    hard = [G.single_matrix(0, n, 1e-5, index=k)[0] for k, n in enumerate(orders)]
+    o, a1 = capi.pack_batch(easy)
+    _, a2 = capi.pack_batch(hard)
+    _, _, _, total, _ = capi.batch_layout(o)
+    opts = MS.tier_options("bf16")
+    eager = []
+    for a in (a1, a2):
+        X = torch.zeros(total, dtype=torch.float64, device="cuda")
+        reps = runtime.solve_batched(o, dev(a), X, opts)
+        eager.append((host(X).copy(), [r["corrections"] for r in reps]))
+    say("eager corrections", eager[0][1], eager[1][1])
+    assert max(eager[1][1]) > max(eager[0][1])
+    A = dev(a1)
+    X = torch.zeros(total, dtype=torch.float64, device="cuda")
+    runtime.set_graphs(True)
+    try:
+        for call, a in enumerate([a1, a1, a2, a1, a2, a2]):
+            A.copy_(torch.from_numpy(a))
+            X.zero_()
+            reps = runtime.solve_batched(o, A, X, opts)
+            k = 0 if a is a1 else 1
+            got = host(X)
+            say("graph call", call, "matches eager", np.array_equal(got, eager[k][0]))
+            assert np.array_equal(got, eager[k][0]), call
+            assert [r["corrections"] for r in reps] == eager[k][1]
+    finally:
+        runtime.set_graphs(False)
+
+
 def test_autotuned_tiles_give_identical_roots(runtime):
     """The autotuner's table holds one of the kernel's tile edges per product
     type and order class; a tile edge only partitions the output, so solves
