@@ -289,8 +289,12 @@ int msq_runtime_set_rounding(msq_runtime *runtime, int mode, uint32_t seed);
  * the device work of each batch signature (orders, options, matrices and
  * roots pointers) in a HIP graph -- the first call runs eagerly, the second
  * is captured, later ones replay it (the plans and uploads stay in an arena
- * of the signature's own).  Batches with orders above 256 always run
- * eagerly.  At most 4 signatures are kept; enable = 0 releases them. */
+ * of the signature's own).  The capture takes two graphs: the work up to
+ * the last correction step that the eager call needed, and the steps after
+ * it, which a replay launches only if the host, after synchronising, finds
+ * a matrix still running (solves under the stagnation policy keep one
+ * graph).  Batches with orders above 256 always run eagerly.  At most 4
+ * signatures are kept; enable = 0 releases them. */
 int msq_runtime_set_graphs(msq_runtime *runtime, int enable);
 
 /* Eigensolver of the orders up to 256 (no counterpart in the reference, whose
